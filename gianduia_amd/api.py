@@ -1,0 +1,284 @@
+"""Python driver of libgnd_hip.so: device context, scenes, traversal and rendering.
+
+Device memory is owned by the library (gnd_hip_malloc); arrays cross the
+boundary as numpy buffers copied to and from HBM.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _abi as A
+
+
+class GndError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def _lib():
+    return A.load_library()
+
+
+def host_error() -> str:
+    return (_lib().gnd_host_last_error() or b"").decode()
+
+
+class DeviceBuffer:
+    def __init__(self, dev: "Device", nbytes: int):
+        self.dev = dev
+        self.nbytes = int(nbytes)
+        self.ptr = C.c_void_p()
+        dev._check(_lib().gnd_hip_malloc(dev.ctx, max(1, self.nbytes), C.byref(self.ptr)))
+
+    @property
+    def addr(self) -> int:
+        return int(self.ptr.value or 0)
+
+    def free(self) -> None:
+        if self.ptr.value:
+            _lib().gnd_hip_free(self.dev.ctx, self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            if self.dev.ctx is not None:
+                self.free()
+        except Exception:
+            pass
+
+
+class Device:
+    """One HIP context (gnd_hip_init) on one GPU."""
+
+    def __init__(self, device: int = 0):
+        self.ctx = None
+        ctx = C.c_void_p()
+        rc = _lib().gnd_hip_init(device, C.byref(ctx))
+        if rc != 0:
+            raise GndError(rc, (_lib().gnd_hip_last_error(None) or b"").decode())
+        self.ctx = ctx
+
+    def close(self) -> None:
+        if self.ctx is not None:
+            _lib().gnd_hip_shutdown(self.ctx)
+            self.ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc: int) -> None:
+        if rc != 0:
+            raise GndError(rc, (_lib().gnd_hip_last_error(self.ctx) or b"").decode())
+
+    # ---- info / tuning
+    def info(self) -> dict:
+        name = C.create_string_buffer(128)
+        cus = C.c_int()
+        hbm = C.c_size_t()
+        self._check(_lib().gnd_hip_device_info(self.ctx, name, 128, C.byref(cus), C.byref(hbm)))
+        return {"arch": name.value.decode(), "cus": cus.value, "hbm_bytes": hbm.value}
+
+    def tuning(self) -> dict:
+        t = A.gnd_hip_tuning()
+        self._check(_lib().gnd_hip_get_tuning(self.ctx, C.byref(t)))
+        return {"block_size": t.block_size, "stack_lds": t.stack_lds, "queue_capacity": t.queue_capacity,
+                "use_graph": t.use_graph}
+
+    def set_tuning(self, block_size=None, stack_lds=None, queue_capacity=None, use_graph=None) -> None:
+        cur = self.tuning()
+        t = A.gnd_hip_tuning(block_size or cur["block_size"], stack_lds or cur["stack_lds"],
+                             queue_capacity or cur["queue_capacity"],
+                             cur["use_graph"] if use_graph is None else int(use_graph))
+        self._check(_lib().gnd_hip_set_tuning(self.ctx, C.byref(t)))
+
+    def set_counters(self, on: bool) -> None:
+        self._check(_lib().gnd_hip_set_counters(self.ctx, int(bool(on))))
+
+    # ---- memory
+    def alloc(self, nbytes: int) -> DeviceBuffer:
+        return DeviceBuffer(self, nbytes)
+
+    def upload(self, arr: np.ndarray) -> DeviceBuffer:
+        arr = np.ascontiguousarray(arr)
+        b = DeviceBuffer(self, arr.nbytes)
+        if arr.nbytes:
+            self._check(_lib().gnd_hip_memcpy_h2d(self.ctx, b.ptr, arr.ctypes.data_as(C.c_void_p), arr.nbytes))
+        return b
+
+    def download(self, buf: DeviceBuffer, dtype, count: int) -> np.ndarray:
+        out = np.empty(count, dtype=dtype)
+        if out.nbytes:
+            self._check(_lib().gnd_hip_memcpy_d2h(self.ctx, out.ctypes.data_as(C.c_void_p), buf.ptr, out.nbytes))
+        return out
+
+    # ---- traversal
+    def _ray_soa(self, rays: dict, keep: list) -> A.gnd_ray_soa:
+        soa = A.gnd_ray_soa()
+        for k in ("ox", "oy", "oz", "dx", "dy", "dz", "tmin", "tmax"):
+            b = self.upload(np.asarray(rays[k], dtype=np.float32))
+            keep.append(b)
+            setattr(soa, k, b.addr)
+        return soa
+
+    def trace_closest(self, scene: "DeviceScene", rays: dict):
+        n = len(rays["ox"])
+        keep: list = []
+        soa = self._ray_soa(rays, keep)
+        hs = A.gnd_hit_soa()
+        bufs = {}
+        for k, dt in (("t", np.float32), ("inst", np.int32), ("prim", np.int32), ("u", np.float32),
+                      ("v", np.float32)):
+            bufs[k] = (self.alloc(4 * max(n, 1)), dt)
+            setattr(hs, k, bufs[k][0].addr)
+        st = A.gnd_hip_stats()
+        self._check(_lib().gnd_hip_trace_closest(self.ctx, scene.handle, C.byref(soa), n, C.byref(hs), C.byref(st)))
+        out = {k: self.download(b, dt, n) for k, (b, dt) in bufs.items()}
+        return out, st.as_dict()
+
+    def trace_any(self, scene: "DeviceScene", rays: dict):
+        n = len(rays["ox"])
+        keep: list = []
+        soa = self._ray_soa(rays, keep)
+        occ = self.alloc(4 * max(n, 1))
+        st = A.gnd_hip_stats()
+        self._check(_lib().gnd_hip_trace_any(self.ctx, scene.handle, C.byref(soa), n, occ.ptr, C.byref(st)))
+        return self.download(occ, np.int32, n), st.as_dict()
+
+    # ---- rendering
+    def render(self, scene: "DeviceScene", jobs: list["Job"], want_sq: bool = False):
+        """Renders the jobs in one call; returns (list of rgbw films as (H,W,4) arrays, stats)."""
+        cj = (A.gnd_hip_job * len(jobs))()
+        films = []
+        for i, j in enumerate(jobs):
+            w, h = j.camera.width, j.camera.height
+            f = self.alloc(w * h * 16)
+            sq = self.alloc(w * h * 12) if want_sq else None
+            films.append((f, sq, w, h))
+            cj[i].camera = j.camera
+            cj[i].spp = j.spp
+            cj[i].max_depth = j.max_depth
+            cj[i].integrator = j.integrator
+            cj[i].seed_offset = j.seed_offset
+            cj[i].film = f.addr
+            cj[i].film_sq = sq.addr if sq is not None else None
+        st = A.gnd_hip_stats()
+        self._check(_lib().gnd_hip_render(self.ctx, scene.handle, cj, len(jobs), C.byref(st)))
+        out = []
+        for f, sq, w, h in films:
+            rgbw = self.download(f, np.float32, w * h * 4).reshape(h, w, 4)
+            if want_sq:
+                out.append((rgbw, self.download(sq, np.float32, w * h * 3).reshape(h, w, 3)))
+            else:
+                out.append(rgbw)
+        return out, st.as_dict()
+
+
+@dataclass
+class Job:
+    camera: A.gnd_camera_desc
+    spp: int
+    max_depth: int
+    integrator: int = A.GND_INTEGRATOR_PATH
+    seed_offset: int = 0
+
+
+class HostScene:
+    """Host scene shell (XML / procedural / caller-built) with its BVH build."""
+
+    def __init__(self, handle):
+        self.handle = handle
+        self._arrays = None
+        self.build_ms = None
+
+    @classmethod
+    def generate(cls, name: str, params: str = "") -> "HostScene":
+        h = C.c_void_p()
+        rc = _lib().gnd_host_scene_generate(name.encode(), params.encode(), C.byref(h))
+        if rc != 0:
+            raise GndError(rc, host_error())
+        return cls(h)
+
+    @classmethod
+    def load_xml(cls, path: str) -> "HostScene":
+        h = C.c_void_p()
+        rc = _lib().gnd_host_scene_load_xml(path.encode(), C.byref(h))
+        if rc != 0:
+            raise GndError(rc, host_error())
+        return cls(h)
+
+    @classmethod
+    def from_desc(cls, desc: A.gnd_scene_desc) -> "HostScene":
+        h = C.c_void_p()
+        rc = _lib().gnd_host_scene_create(C.byref(desc), C.byref(h))
+        if rc != 0:
+            raise GndError(rc, host_error())
+        return cls(h)
+
+    def desc(self):
+        return _lib().gnd_host_scene_desc(self.handle)
+
+    def build(self, threads: int = 0):
+        if self._arrays is None:
+            out = C.POINTER(A.gnd_hip_scene_arrays)()
+            ms = C.c_double()
+            rc = _lib().gnd_host_scene_build(self.handle, threads, C.byref(out), C.byref(ms))
+            if rc != 0:
+                raise GndError(rc, host_error())
+            self._arrays = out
+            self.build_ms = ms.value
+        return self._arrays
+
+    def camera(self) -> A.gnd_camera_desc:
+        c = A.gnd_camera_desc()
+        C.memmove(C.byref(c), C.byref(self.desc().contents.camera), C.sizeof(c))
+        return c
+
+    def default_job(self, spp: int | None = None, seed_offset: int = 0) -> Job:
+        d = self.desc().contents
+        return Job(self.camera(), spp if spp is not None else d.spp, d.max_depth, d.integrator, seed_offset)
+
+    def free(self) -> None:
+        if self.handle:
+            _lib().gnd_host_scene_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class DeviceScene:
+    """A scene resident in HBM (gnd_hip_scene_upload)."""
+
+    def __init__(self, dev: Device, host: HostScene, threads: int = 0):
+        self.dev = dev
+        arrays = host.build(threads)
+        h = C.c_void_p()
+        dev._check(_lib().gnd_hip_scene_upload(dev.ctx, arrays, C.byref(h)))
+        self.handle = h
+        self.host = host
+
+    @property
+    def nbytes(self) -> int:
+        return int(_lib().gnd_hip_scene_bytes(self.handle))
+
+    def free(self) -> None:
+        if self.handle:
+            _lib().gnd_hip_scene_free(self.dev.ctx, self.handle)
+            self.handle = None
+
+
+def resolve(rgbw: np.ndarray) -> np.ndarray:
+    """Film::resolve: rgb = sum * (1/weight), 0 where the weight is 0."""
+    w = rgbw[..., 3:4]
+    inv = np.where(w > 0, np.float32(1.0) / np.where(w > 0, w, np.float32(1.0)), np.float32(0.0))
+    return (rgbw[..., :3] * inv).astype(np.float32)

@@ -1,0 +1,1543 @@
+// kernels.hip — CDNA4 kernels of the ray-tracing hot path and the C-ABI of
+// libgnd_hip.so (declared in include/gnd_hip.h).
+//
+// Wavefront form: camera rays, closest-hit traversal, shading + next-event
+// estimation, shadow traversal and the film gather run as separate kernels
+// over struct-of-arrays path state in HBM.  Live paths are compacted between
+// bounces with a wave-wide ballot + popcount and one atomic per wave.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "dev_common.h"
+#include "dev_shade.h"
+#include "dev_trace.h"
+
+using namespace gdev;
+
+// ============================================================================ device state
+
+struct PathSoA {
+    float *ox, *oy, *oz, *dx, *dy, *dz;
+    float *tpr, *tpg, *tpb, *Lr, *Lg, *Lb;
+    float *pdfPrev, *lx, *ly, *lz;
+    float *weight, *pfx, *pfy;
+    uint64_t* rng;
+    uint32_t *c0, *c1, *flags;  // c0: bounces | nullBounces<<16; c1: surface | volume bounces<<16
+    int32_t* job;
+    // event record between the NEE and scatter kernels (split mode)
+    int32_t *evKind, *evPrim, *evMedium;
+    float *evpx, *evpy, *evpz, *evnx, *evny, *evnz, *evtx, *evty, *evtz, *evu, *evv;
+};
+
+struct HitSoA {
+    float *t, *u, *v;
+    int32_t *inst, *prim;
+};
+
+struct ShadowSoA {
+    int32_t *path, *medium;
+    float *ox, *oy, *oz, *dx, *dy, *dz, *tmax;
+    float *pr, *pg, *pb;   // pending contribution
+    float *lpx, *lpy, *lpz;  // light point (transmittance segments)
+};
+
+struct DJob {
+    DCamera cam;
+    float* film;
+    float* filmSq;
+    uint64_t seedOffset;
+    int32_t maxDepth, integrator, W, H;
+};
+
+struct DChunk {     // one pass of one job inside a wave
+    uint32_t slotBase, npix;
+    int32_t job, pass;
+};
+
+struct DSplatTask { // one job inside a wave: its chunks in pass order
+    uint32_t pixBase;   // prefix over tasks of W*H
+    int32_t job, chunkBegin, chunkEnd;
+};
+
+// flags layout
+constexpr uint32_t F_SPECULAR = 1u;
+GDEV_INLINE int flagChannel(uint32_t f) { return (int)((f >> 1) & 3u) - 1; }  // -1 none, 0..2
+GDEV_INLINE int flagMedium(uint32_t f) { return (int)((f >> 8) & 0xFFFFu) - 1; }
+GDEV_INLINE uint32_t setMedium(uint32_t f, int m) { return (f & ~(0xFFFFu << 8)) | ((uint32_t)(m + 1) << 8); }
+
+struct Counters {
+    unsigned long long nodes, packs, insts, rays, shadowRays, nan, stackOverflow, pad;
+};
+
+// wave-aggregated append: ballot, popcount, one atomic per wave
+GDEV_INLINE uint32_t waveAppend(uint32_t* counter, bool pred) {
+    const uint64_t mask = __ballot(pred);
+    const uint32_t lane = __lane_id();
+    const uint32_t prefix = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    uint32_t base = 0;
+    if (mask != 0) {
+        const int leader = __ffsll((unsigned long long)mask) - 1;
+        if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+        base = __shfl(base, leader);
+    }
+    return base + prefix;
+}
+
+// XCD-aware block order: the dispatcher deals workgroups round-robin over the
+// 8 XCDs; remap so each XCD walks a contiguous range (coherent rays share L2).
+GDEV_INLINE uint32_t xcdBlock() {
+    const uint32_t g = gridDim.x, b = blockIdx.x;
+    if (g % 8u != 0u) return b;
+    return (b % 8u) * (g / 8u) + b / 8u;
+}
+
+template <int LDSD>
+GDEV_INLINE TravStack<LDSD> makeStack(int* lds) {
+    TravStack<LDSD> st;
+    st.lds = lds + threadIdx.x;
+    st.stride = (int)blockDim.x;
+    st.sp = 0;
+    st.overflow = false;
+    return st;
+}
+
+GDEV_INLINE void flushCounters(Counters* c, const uint32_t* cnt, bool overflow) {
+    unsigned long long a = cnt[0], b = cnt[1], d = cnt[2];
+    // wave reduction then one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_down(a, off);
+        b += __shfl_down(b, off);
+        d += __shfl_down(d, off);
+    }
+    if (__lane_id() == 0) {
+        atomicAdd(&c->nodes, a);
+        atomicAdd(&c->packs, b);
+        atomicAdd(&c->insts, d);
+    }
+    if (overflow) atomicAdd(&c->stackOverflow, 1ull);
+}
+
+// ============================================================================ verify-mode traversal
+
+template <int LDSD, bool ANY, bool COUNT>
+__global__ void kTraceBatch(DScene S, gnd_ray_soa R, int64_t n, gnd_hit_soa H, int32_t* occ, Counters* ctr) {
+    extern __shared__ int sStack[];
+    const int64_t i = (int64_t)xcdBlock() * blockDim.x + threadIdx.x;
+    uint32_t cnt[3] = {0, 0, 0};
+    TravStack<LDSD> st = makeStack<LDSD>(sStack);
+    if (i < n) {
+        HitRec h;
+        bool r = traceRay<ANY, COUNT, LDSD>(S, V3{R.ox[i], R.oy[i], R.oz[i]}, V3{R.dx[i], R.dy[i], R.dz[i]},
+                                           R.tmin[i], R.tmax[i], h, st, cnt);
+        if (ANY) {
+            occ[i] = r ? 1 : 0;
+        } else {
+            H.t[i] = r ? h.t : finf();
+            H.inst[i] = r ? h.inst : -1;
+            H.prim[i] = r ? h.prim : -1;
+            H.u[i] = r ? h.u : 0.0f;
+            H.v[i] = r ? h.v : 0.0f;
+        }
+    }
+    if (COUNT) flushCounters(ctr, cnt, st.overflow);
+    else if (st.overflow) atomicAdd(&ctr->stackOverflow, 1ull);
+}
+
+// ============================================================================ wavefront path tracer
+
+struct WaveArgs {
+    DScene S;
+    const DJob* jobs;
+    const DChunk* chunks;
+    int32_t nChunks;
+    uint32_t nSlots;
+    PathSoA P;
+    HitSoA Hq;            // hits indexed by queue slot
+    ShadowSoA Sh;
+    int32_t* qIn;
+    int32_t* qOut;
+    uint32_t* counts;     // [0] in, [1] out, [2] shadow
+    Counters* ctr;
+    int32_t pathOnlyFast; // shadows may use any-hit
+    int32_t split;        // NEE and scatter in separate kernels
+};
+
+// SamplerIntegrator::render per-sample set-up (integrator.h:88-125): seeding,
+// film jitter, lens sample, chromatic-aberration channel, time, camera ray.
+__global__ void kCamera(WaveArgs A) {
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= A.nSlots) return;
+    // chunk lookup (chunks are sorted by slotBase)
+    int lo = 0, hi = A.nChunks - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (A.chunks[mid].slotBase <= slot) lo = mid;
+        else hi = mid - 1;
+    }
+    const DChunk ch = A.chunks[lo];
+    const DJob& J = A.jobs[ch.job];
+    const uint32_t pix = slot - ch.slotBase;
+    const int W = J.W, H = J.H;
+    const int x = (int)(pix % (uint32_t)W), y = (int)(pix / (uint32_t)W);
+    const uint64_t pixelIdx = (uint64_t)(y * W + x);
+    const uint64_t g = pixelIdx + (uint64_t)ch.pass * (uint64_t)(W * H) + J.seedOffset;
+    Rng rng;
+    rng.s = pcgSeed(splitMix64(g));
+    const float jx = rng.next1D(), jy = rng.next1D();
+    const float pfx = x + jx, pfy = y + jy;
+    const float fx = pfx / W, fy = 1.0f - pfy / H;
+    const float lx = rng.next1D(), ly = rng.next1D();
+    const float chR = rng.next1D();
+    int chan;
+    float lambda;
+    if (!J.cam.hasCA) { chan = -1; lambda = 0.0f; }
+    else if (chR < 0.333333f) { chan = 0; lambda = 1.0f; }
+    else if (chR < 0.666666f) { chan = 1; lambda = 0.0f; }
+    else { chan = 2; lambda = -1.0f; }
+    (void)rng.next1D();  // time
+    V3 ro, rd;
+    const float w = shootRay(J.cam, fx, fy, lx, ly, lambda, ro, rd);
+    PathSoA& P = A.P;
+    P.ox[slot] = ro.x; P.oy[slot] = ro.y; P.oz[slot] = ro.z;
+    P.dx[slot] = rd.x; P.dy[slot] = rd.y; P.dz[slot] = rd.z;
+    P.tpr[slot] = 1.0f; P.tpg[slot] = 1.0f; P.tpb[slot] = 1.0f;
+    P.Lr[slot] = 0.0f; P.Lg[slot] = 0.0f; P.Lb[slot] = 0.0f;
+    P.pdfPrev[slot] = 1.0f;
+    P.lx[slot] = ro.x; P.ly[slot] = ro.y; P.lz[slot] = ro.z;
+    P.weight[slot] = w;
+    P.pfx[slot] = pfx;
+    P.pfy[slot] = pfy;
+    P.rng[slot] = rng.s;
+    P.c0[slot] = 0u;
+    P.c1[slot] = 0u;
+    P.flags[slot] = F_SPECULAR | ((uint32_t)(chan + 1) << 1);  // medium -1 -> field 0
+    P.job[slot] = ch.job;
+    const bool live = J.maxDepth > 0;
+    const uint32_t q = waveAppend(&A.counts[0], live);
+    if (live) A.qIn[q] = (int32_t)slot;
+}
+
+// closest hit for the live path rays (Scene::rayIntersect at the top of Li)
+template <int LDSD, bool COUNT>
+__global__ void kTracePaths(WaveArgs A) {
+    extern __shared__ int sStack[];
+    const uint32_t n = A.counts[0];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        A.counts[1] = 0u;
+        A.counts[2] = 0u;
+    }
+    uint32_t cnt[3] = {0, 0, 0};
+    TravStack<LDSD> st = makeStack<LDSD>(sStack);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = xcdBlock() * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int32_t p = A.qIn[i];
+        HitRec h;
+        const PathSoA& P = A.P;
+        bool r = traceRay<false, COUNT, LDSD>(A.S, V3{P.ox[p], P.oy[p], P.oz[p]}, V3{P.dx[p], P.dy[p], P.dz[p]},
+                                              kEps, finf(), h, st, cnt);
+        A.Hq.t[i] = r ? h.t : finf();
+        A.Hq.inst[i] = r ? h.inst : -1;
+        A.Hq.prim[i] = r ? h.prim : -1;
+        A.Hq.u[i] = h.u;
+        A.Hq.v[i] = h.v;
+    }
+    if (COUNT) flushCounters(A.ctr, cnt, st.overflow);
+    else if (st.overflow) atomicAdd(&A.ctr->stackOverflow, 1ull);
+}
+
+struct PathLocal {
+    V3 o, d, tp, L, last;
+    float pdfPrev;
+    Rng rng;
+    uint32_t c0, c1, flags;
+    int job;
+};
+
+GDEV_INLINE void loadPath(const PathSoA& P, int p, PathLocal& s) {
+    s.o = V3{P.ox[p], P.oy[p], P.oz[p]};
+    s.d = V3{P.dx[p], P.dy[p], P.dz[p]};
+    s.tp = V3{P.tpr[p], P.tpg[p], P.tpb[p]};
+    s.L = V3{P.Lr[p], P.Lg[p], P.Lb[p]};
+    s.last = V3{P.lx[p], P.ly[p], P.lz[p]};
+    s.pdfPrev = P.pdfPrev[p];
+    s.rng.s = P.rng[p];
+    s.c0 = P.c0[p];
+    s.c1 = P.c1[p];
+    s.flags = P.flags[p];
+    s.job = P.job[p];
+}
+GDEV_INLINE void storePath(const PathSoA& P, int p, const PathLocal& s) {
+    P.ox[p] = s.o.x; P.oy[p] = s.o.y; P.oz[p] = s.o.z;
+    P.dx[p] = s.d.x; P.dy[p] = s.d.y; P.dz[p] = s.d.z;
+    P.tpr[p] = s.tp.x; P.tpg[p] = s.tp.y; P.tpb[p] = s.tp.z;
+    P.Lr[p] = s.L.x; P.Lg[p] = s.L.y; P.Lb[p] = s.L.z;
+    P.lx[p] = s.last.x; P.ly[p] = s.last.y; P.lz[p] = s.last.z;
+    P.pdfPrev[p] = s.pdfPrev;
+    P.rng[p] = s.rng.s;
+    P.c0[p] = s.c0;
+    P.c1[p] = s.c1;
+    P.flags[p] = s.flags;
+}
+
+// Event kinds between NEE and scatter
+constexpr int EV_NONE = 0, EV_MEDIUM = 1, EV_SURFACE = 2;
+
+struct Event {
+    int kind;
+    V3 p, n, dpdu;
+    float u, v;
+    int prim;
+    int curMedium;
+};
+
+GDEV_INLINE int nextMedium(V3 wo, V3 wi, int cur, V3 n, int inside, int outside) {  // volPath.cpp:293-298
+    if (dot(wi, n) > 0.0f) return dot(wo, n) > 0.0f ? cur : outside;
+    return dot(wo, n) > 0.0f ? inside : cur;
+}
+
+// Scatter half of a bounce: phase or BSDF sampling, Russian roulette, next ray
+// (path.cpp:159-177, volPath.cpp:88-111 and 222-248).  Returns true if alive.
+GDEV_INLINE bool scatterStep(const DScene& S, const DJob& J, PathLocal& s, const Event& ev) {
+    const bool vol = J.integrator == GND_INTEGRATOR_VOLPATH;
+    uint32_t bounces = s.c0 & 0xFFFFu, nullB = s.c0 >> 16;
+    uint32_t surfB = s.c1 & 0xFFFFu, volB = s.c1 >> 16;
+    if (ev.kind == EV_MEDIUM) {
+        const DMedium& m = S.media[ev.curMedium];
+        V3 wo = -s.d, wi;
+        float ux = s.rng.next1D(), uy = s.rng.next1D();
+        s.pdfPrev = hgSample(hgClamp(m.g), wo, wi, ux, uy);
+        s.flags &= ~F_SPECULAR;
+        s.last = ev.p;
+        s.o = ev.p;
+        s.d = wi;  // medium unchanged
+        if (volB > 6) {
+            float q = stdMax(0.05f, stdMin(luminance(s.tp), 0.99f));
+            if (s.rng.next1D() > q) return false;
+            s.tp = cdiv(s.tp, q);
+        }
+        bounces++;
+        volB++;
+        s.c0 = bounces | (nullB << 16);
+        s.c1 = surfB | (volB << 16);
+        return (int)bounces < J.maxDepth;
+    }
+    // surface
+    Bsdf b;
+    b.fr.x = ev.dpdu;
+    b.fr.y = normalize(cross(ev.n, ev.dpdu));
+    b.fr.z = ev.n;
+    makeLobe(S.mats + S.prims[ev.prim].material, ev.u, ev.v, b.lobe);
+    V3 wiW = V3{0, 0, 0};
+    int st = 0;
+    float sx = s.rng.next1D(), sy = s.rng.next1D();
+    float uc = s.rng.next1D();
+    V3 fc = b.sample(-s.d, wiW, sx, sy, uc, s.pdfPrev, st);
+    if (isBlack(fc) || s.pdfPrev <= 1e-6f) return false;
+    if (st & BSDF_SPECULAR) s.flags |= F_SPECULAR;
+    else s.flags &= ~F_SPECULAR;
+    s.tp = s.tp * fc;
+    if (!vol) {
+        if (bounces > 2) {
+            float q = stdMax(0.05f, stdMin(luminance(s.tp), 0.99f));
+            if (s.rng.next1D() > q) return false;
+            s.tp = cdiv(s.tp, q);
+        }
+        s.last = ev.p;
+        s.o = ev.p;
+        s.d = wiW;
+        bounces++;
+        s.c0 = bounces | (nullB << 16);
+        return (int)bounces < J.maxDepth;
+    }
+    V3 inc = -s.d;
+    s.last = ev.p;
+    int inside, outside;
+    mediumInterface(S, ev.prim, inside, outside);
+    int nm = nextMedium(inc, wiW, ev.curMedium, ev.n, inside, outside);
+    s.o = ev.p;
+    s.d = wiW;
+    s.flags = setMedium(s.flags, nm);
+    if (surfB > 2) {
+        float q = stdMax(0.05f, stdMin(luminance(s.tp), 0.99f));
+        if (s.rng.next1D() > q) return false;
+        s.tp = cdiv(s.tp, q);
+    }
+    bounces++;
+    surfB++;
+    s.c0 = bounces | (nullB << 16);
+    s.c1 = surfB | (volB << 16);
+    return (int)bounces < J.maxDepth;
+}
+
+// First half of a bounce: medium sampling (volpath), emission with MIS, null
+// BSDF pass-through and the NEE light sample with its shadow ray.
+// Returns: 0 = dead, 1 = continues straight (already a new ray), 2 = needs scatter.
+GDEV_INLINE int neeStep(const WaveArgs& A, const DJob& J, PathLocal& s, const HitRec& h, Event& ev, bool& shadow,
+                        V3& so, V3& sd, float& stmax, int& smedium, V3& pending, V3& lightP) {
+    const DScene& S = A.S;
+    const bool vol = J.integrator == GND_INTEGRATOR_VOLPATH;
+    const bool hit = h.inst >= 0;
+    const float invNE = 1.0f / (float)(uint32_t)S.nEmitters;
+    shadow = false;
+    uint32_t bounces = s.c0 & 0xFFFFu, nullB = s.c0 >> 16;
+    const int medium = vol ? flagMedium(s.flags) : -1;
+    if (medium >= 0) {
+        MediumEvent me;
+        const float tm = hit ? h.t : finf();
+        s.tp = s.tp * mediumSample(S.media[medium], s.o, s.d, kEps, tm, s.rng, me);
+        if (me.valid) {
+            s.tp = s.tp * me.sigma_s;
+            if (isBlack(s.tp)) return 0;
+            const float lsp = 1.0f / (float)(uint32_t)S.nEmitters;
+            const int ep = pickEmitter(S, s.rng.next1D());
+            const float ux = s.rng.next1D(), uy = s.rng.next1D();
+            LightSample ls;
+            areaSample(S, ep, me.p, ux, uy, ls);
+            const V3 Li = cdiv(ls.Li, lsp);
+            if (ls.ok && ls.pdf > 1e-6f && !isBlack(Li)) {
+                const V3 wi = normalize(ls.p - me.p);
+                const float pp = hgPhase(dot(-s.d, wi), hgClamp(S.media[medium].g));
+                pending = pp > 1e-6f ? s.tp * pp * Li * powerHeuristic(ls.pdf * lsp, pp) : sameV(0.0f);
+                shadow = true;
+                so = me.p; sd = ls.dir; stmax = ls.tmax; smedium = medium; lightP = ls.p;
+            }
+            ev.kind = EV_MEDIUM;
+            ev.p = me.p;
+            ev.curMedium = medium;
+            return 2;
+        }
+    }
+    if (!hit) return 0;
+    Isect it;
+    fillInteraction(S, s.o, s.d, h, it);
+    const DPrim& pr = S.prims[it.prim];
+    Bsdf b;
+    const bool hasBsdf = makeBsdf(S, it, b);
+    if (pr.emitter >= 0) {
+        const bool skip = (bounces == 0 && !hasBsdf);
+        if (!skip) {
+            V3 Le = areaEval(pr, it.n, -s.d);
+            if (!isBlack(Le)) {
+                float w = 1.0f;
+                if (!(s.flags & F_SPECULAR)) {
+                    float lp = areaPdf(S, it.prim, s.last, it.p, it.n);
+                    w = powerHeuristic(s.pdfPrev, lp * invNE);
+                }
+                s.L = s.L + s.tp * Le * w;
+            }
+        }
+    }
+    int inside = -1, outside = -1;
+    if (vol) mediumInterface(S, it.prim, inside, outside);
+    if (!hasBsdf) {
+        if (nullB++ > 100) return 0;
+        s.c0 = bounces | (nullB << 16);
+        s.o = it.p;  // direction unchanged
+        if (vol) s.flags = setMedium(s.flags, dot(s.d, it.n) > 0.0f ? outside : inside);
+        return 1;
+    }
+    const float lsp = 1.0f / (float)(uint32_t)S.nEmitters;
+    const int ep = pickEmitter(S, s.rng.next1D());
+    const float ux = s.rng.next1D(), uy = s.rng.next1D();
+    LightSample ls;
+    areaSample(S, ep, it.p, ux, uy, ls);
+    const V3 Li = cdiv(ls.Li, lsp);
+    if (ls.ok && ls.pdf > 1e-6f && !isBlack(Li)) {
+        const V3 wi = normalize(ls.p - it.p);
+        const float bp = b.pdf(-s.d, wi);
+        const V3 f = b.f(-s.d, wi);
+        if (!isBlack(f) && bp > 1e-6f)
+            pending = s.tp * f * Li * __builtin_fabsf(dot(it.n, wi)) * powerHeuristic(ls.pdf * lsp, bp);
+        else pending = sameV(0.0f);
+        shadow = true;
+        so = it.p; sd = ls.dir; stmax = ls.tmax; lightP = ls.p;
+        smedium = vol ? nextMedium(-s.d, ls.dir, medium, it.n, inside, outside) : -1;
+    }
+    ev.kind = EV_SURFACE;
+    ev.p = it.p;
+    ev.n = it.n;
+    ev.dpdu = it.dpdu;
+    ev.u = it.u;
+    ev.v = it.v;
+    ev.prim = it.prim;
+    ev.curMedium = medium;
+    return 2;
+}
+
+// SPLIT = false: full bounce (NEE + scatter) in one kernel; shadow rays resolve
+// afterwards (valid when shadow rays consume no random numbers).
+// SPLIT = true: NEE only; the event is stored for kScatter.
+template <bool SPLIT>
+__global__ void kShade(WaveArgs A) {
+    const uint32_t n = A.counts[0];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t start = blockIdx.x * blockDim.x;
+    for (uint32_t base = start; base < n; base += stride) {
+        const uint32_t i = base + threadIdx.x;
+        const bool active = i < n;
+        bool alive = false, shadow = false;
+        int p = -1;
+        PathLocal s;
+        V3 so, sd, pending = sameV(0.0f), lightP;
+        float stmax = 0.0f;
+        int smedium = -1;
+        if (active) {
+            p = A.qIn[i];
+            loadPath(A.P, p, s);
+            const DJob& J = A.jobs[s.job];
+            HitRec h;
+            h.t = A.Hq.t[i];
+            h.inst = A.Hq.inst[i];
+            h.prim = A.Hq.prim[i];
+            h.u = A.Hq.u[i];
+            h.v = A.Hq.v[i];
+            Event ev;
+            ev.kind = EV_NONE;
+            int r = neeStep(A, J, s, h, ev, shadow, so, sd, stmax, smedium, pending, lightP);
+            if (!A.pathOnlyFast && shadow) {
+                // the general shadow kernel always needs the ray; the any-hit
+                // path skips zero contributions
+            } else if (shadow && isBlack(pending)) {
+                shadow = false;
+            }
+            if (r == 1) alive = true;
+            else if (r == 2) {
+                if (SPLIT) {
+                    alive = false;
+                    A.P.evKind[p] = ev.kind;
+                    A.P.evPrim[p] = ev.prim;
+                    A.P.evMedium[p] = ev.curMedium;
+                    A.P.evpx[p] = ev.p.x; A.P.evpy[p] = ev.p.y; A.P.evpz[p] = ev.p.z;
+                    A.P.evnx[p] = ev.n.x; A.P.evny[p] = ev.n.y; A.P.evnz[p] = ev.n.z;
+                    A.P.evtx[p] = ev.dpdu.x; A.P.evty[p] = ev.dpdu.y; A.P.evtz[p] = ev.dpdu.z;
+                    A.P.evu[p] = ev.u; A.P.evv[p] = ev.v;
+                } else {
+                    alive = scatterStep(A.S, J, s, ev);
+                }
+            } else if (SPLIT) {
+                A.P.evKind[p] = EV_NONE;
+            }
+            if (SPLIT && r == 1) A.P.evKind[p] = EV_NONE;
+            storePath(A.P, p, s);
+        }
+        const uint32_t qs = waveAppend(&A.counts[1], alive);
+        if (alive) A.qOut[qs] = p;
+        const uint32_t ss = waveAppend(&A.counts[2], shadow);
+        if (shadow) {
+            const ShadowSoA& Sh = A.Sh;
+            Sh.path[ss] = p;
+            Sh.medium[ss] = smedium;
+            Sh.ox[ss] = so.x; Sh.oy[ss] = so.y; Sh.oz[ss] = so.z;
+            Sh.dx[ss] = sd.x; Sh.dy[ss] = sd.y; Sh.dz[ss] = sd.z;
+            Sh.tmax[ss] = stmax;
+            Sh.pr[ss] = pending.x; Sh.pg[ss] = pending.y; Sh.pb[ss] = pending.z;
+            Sh.lpx[ss] = lightP.x; Sh.lpy[ss] = lightP.y; Sh.lpz[ss] = lightP.z;
+        }
+    }
+}
+
+// Split mode second half: the paths that recorded an event scatter now (after
+// their shadow rays consumed transmittance samples).
+__global__ void kScatter(WaveArgs A, const int32_t* qEv, uint32_t nEv) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < nEv; base += stride) {
+        const uint32_t i = base + threadIdx.x;
+        bool alive = false;
+        int p = -1;
+        if (i < nEv) {
+            p = qEv[i];
+            const int kind = A.P.evKind[p];
+            if (kind != EV_NONE) {
+                PathLocal s;
+                loadPath(A.P, p, s);
+                Event ev;
+                ev.kind = kind;
+                ev.prim = A.P.evPrim[p];
+                ev.curMedium = A.P.evMedium[p];
+                ev.p = V3{A.P.evpx[p], A.P.evpy[p], A.P.evpz[p]};
+                ev.n = V3{A.P.evnx[p], A.P.evny[p], A.P.evnz[p]};
+                ev.dpdu = V3{A.P.evtx[p], A.P.evty[p], A.P.evtz[p]};
+                ev.u = A.P.evu[p];
+                ev.v = A.P.evv[p];
+                alive = scatterStep(A.S, A.jobs[s.job], s, ev);
+                storePath(A.P, p, s);
+            }
+        }
+        const uint32_t qs = waveAppend(&A.counts[1], alive);
+        if (alive) A.qOut[qs] = p;
+    }
+}
+
+// Shadow rays.  FAST: any-hit test (no null-BSDF surfaces, no transmittance).
+// General: closest-hit loop through null-BSDF surfaces with per-segment medium
+// transmittance (path.cpp:122-140 / volPath.cpp:261-291).
+template <int LDSD, bool FAST, bool COUNT>
+__global__ void kShadow(WaveArgs A) {
+    extern __shared__ int sStack[];
+    const uint32_t n = A.counts[2];
+    uint32_t cnt[3] = {0, 0, 0};
+    TravStack<LDSD> st = makeStack<LDSD>(sStack);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const DScene& S = A.S;
+    for (uint32_t i = xcdBlock() * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const ShadowSoA& Sh = A.Sh;
+        const int p = Sh.path[i];
+        V3 o = V3{Sh.ox[i], Sh.oy[i], Sh.oz[i]}, d = V3{Sh.dx[i], Sh.dy[i], Sh.dz[i]};
+        float tmax = Sh.tmax[i];
+        const V3 pending = V3{Sh.pr[i], Sh.pg[i], Sh.pb[i]};
+        HitRec h;
+        if (FAST) {
+            bool occ = traceRay<true, COUNT, LDSD>(S, o, d, kEps, tmax, h, st, cnt);
+            if (!occ) {
+                A.P.Lr[p] = A.P.Lr[p] + pending.x;
+                A.P.Lg[p] = A.P.Lg[p] + pending.y;
+                A.P.Lb[p] = A.P.Lb[p] + pending.z;
+            }
+            continue;
+        }
+        const int job = A.P.job[p];
+        const bool vol = A.jobs[job].integrator == GND_INTEGRATOR_VOLPATH;
+        int medium = Sh.medium[i];
+        const V3 lightP = V3{Sh.lpx[i], Sh.lpy[i], Sh.lpz[i]};
+        V3 Tr = sameV(1.0f);
+        Rng rng;
+        rng.s = A.P.rng[p];
+        bool visible = false;
+        for (int iter = 0; iter < 1024; ++iter) {
+            const bool hit = traceRay<false, COUNT, LDSD>(S, o, d, kEps, tmax, h, st, cnt);
+            if (vol && medium >= 0) Tr = Tr * mediumTr(S.media[medium], o, d, kEps, hit ? h.t : tmax, rng);
+            if (vol && isBlack(Tr)) break;
+            if (!hit) { visible = true; break; }
+            Isect it;
+            fillInteraction(S, o, d, h, it);
+            const DMaterial& m = S.mats[S.prims[it.prim].material];
+            if (m.type != GND_MAT_INDEX) break;  // occluded
+            if (vol) {
+                int inside, outside;
+                mediumInterface(S, it.prim, inside, outside);
+                medium = dot(d, it.n) > 0.0f ? outside : inside;
+                tmax = length(it.p - lightP) - kEps;
+            } else {
+                tmax = h.t - length(it.p - o);  // the hit distance minus the walked length
+            }
+            o = it.p;
+        }
+        A.P.rng[p] = rng.s;
+        if (visible) {
+            V3 c = vol ? pending * Tr : pending;
+            A.P.Lr[p] = A.P.Lr[p] + c.x;
+            A.P.Lg[p] = A.P.Lg[p] + c.y;
+            A.P.Lb[p] = A.P.Lb[p] + c.z;
+        }
+    }
+    if (COUNT) flushCounters(A.ctr, cnt, st.overflow);
+    else if (st.overflow) atomicAdd(&A.ctr->stackOverflow, 1ull);
+}
+
+// Film gather (Film::addSample, film.h:26-69): one thread per (job, pixel)
+// sums the filtered contributions of the neighbouring samples of each pass in
+// pass order, so results do not depend on scheduling.
+__global__ void kSplat(WaveArgs A, const DSplatTask* tasks, int nTasks, uint32_t totalPix) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= totalPix) return;
+    int lo = 0, hi = nTasks - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (tasks[mid].pixBase <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    const DSplatTask tk = tasks[lo];
+    const DJob& J = A.jobs[tk.job];
+    const uint32_t pix = t - tk.pixBase;
+    const int W = J.W, H = J.H;
+    const int x = (int)(pix % (uint32_t)W), y = (int)(pix / (uint32_t)W);
+    const DCamera& c = J.cam;
+    const float rx = c.filter == GND_FILTER_BOX ? 0.5f : c.radiusX;
+    const float ry = c.filter == GND_FILTER_BOX ? 0.5f : c.radiusY;
+    const int reach = (int)ceilf(stdMax(rx, ry)) + 1;
+    float4 acc = reinterpret_cast<float4*>(J.film)[pix];
+    float3 accSq = J.filmSq ? reinterpret_cast<float3*>(J.filmSq)[pix] : float3{0, 0, 0};
+    unsigned long long nan = 0;
+    for (int ci = tk.chunkBegin; ci < tk.chunkEnd; ++ci) {
+        const DChunk ch = A.chunks[ci];
+        float sr = 0.0f, sg = 0.0f, sb = 0.0f, sw = 0.0f;
+        for (int dy = -reach; dy <= reach; ++dy) {
+            const int qy = y + dy;
+            if (qy < 0 || qy >= H) continue;
+            for (int dx = -reach; dx <= reach; ++dx) {
+                const int qx = x + dx;
+                if (qx < 0 || qx >= W) continue;
+                const uint32_t q = ch.slotBase + (uint32_t)(qy * W + qx);
+                const float px = A.P.pfx[q], py = A.P.pfy[q];
+                const int x0 = max(0, (int)ceilf(px - rx - 0.5f));
+                const int x1 = min(W - 1, (int)floorf(px + rx - 0.5f));
+                const int y0 = max(0, (int)ceilf(py - ry - 0.5f));
+                const int y1 = min(H - 1, (int)floorf(py + ry - 0.5f));
+                if (x < x0 || x > x1 || y < y0 || y > y1) continue;
+                const float w = filterEval(c, px - (x + 0.5f), py - (y + 0.5f));
+                if (!(w > 0.0f)) continue;
+                // final sample value (integrator.h:128-141)
+                V3 L = V3{A.P.Lr[q], A.P.Lg[q], A.P.Lb[q]} * A.P.weight[q];
+                const int chan = flagChannel(A.P.flags[q]);
+                if (chan == 0) L = V3{L.x * 3.0f, 0.0f, 0.0f};
+                else if (chan == 1) L = V3{0.0f, L.y * 3.0f, 0.0f};
+                else if (chan == 2) L = V3{0.0f, 0.0f, L.z * 3.0f};
+                if (hasNaN(L)) {
+                    L = sameV(0.0f);
+                    if (dx == 0 && dy == 0) nan++;
+                }
+                sr += L.x * w;
+                sg += L.y * w;
+                sb += L.z * w;
+                sw += w;
+            }
+        }
+        acc.x += sr;
+        acc.y += sg;
+        acc.z += sb;
+        acc.w += sw;
+        if (J.filmSq && sw > 0.0f) {
+            float vr = sr / sw, vg = sg / sw, vb = sb / sw;
+            accSq.x += vr * vr;
+            accSq.y += vg * vg;
+            accSq.z += vb * vb;
+        }
+    }
+    reinterpret_cast<float4*>(J.film)[pix] = acc;
+    if (J.filmSq) reinterpret_cast<float3*>(J.filmSq)[pix] = accSq;
+    if (nan) atomicAdd(&A.ctr->nan, nan);
+}
+
+__global__ void kFill(float* p, size_t n, float v) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void kFillU32(uint32_t* p, size_t n, uint32_t v) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ============================================================================ host side
+
+struct gnd_hip_scene {
+    DScene view{};
+    std::vector<void*> allocations;
+    size_t bytes = 0;
+    int hasIndex = 0, hasGrid = 0, nMedia = 0;
+    uint32_t nPrims = 0;
+};
+
+struct gnd_hip_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::mutex mu;
+    gnd_hip_tuning tuning{128, 16, 1 << 22, 0};
+    bool counters = false;
+    int numCUs = 0;
+    size_t hbm = 0;
+    std::string name;
+    Counters* dCounters = nullptr;
+    // wave buffers (grown on demand)
+    size_t capacity = 0;
+    void* pathBuf = nullptr;
+    PathSoA P{};
+    HitSoA Hq{};
+    ShadowSoA Sh{};
+    int32_t* q[2] = {nullptr, nullptr};
+    int32_t* qEv = nullptr;
+    uint32_t* counts = nullptr;
+    DJob* dJobs = nullptr;
+    size_t jobCap = 0;
+    DChunk* dChunks = nullptr;
+    size_t chunkCap = 0;
+    DSplatTask* dTasks = nullptr;
+    size_t taskCap = 0;
+    volatile int cancel = 0;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+namespace {
+
+thread_local std::string g_lastError;
+
+struct HipError : std::runtime_error {
+    int code;
+    HipError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define GND_CHECK(expr)                                                                                     \
+    do {                                                                                                    \
+        hipError_t e_ = (expr);                                                                             \
+        if (e_ != hipSuccess)                                                                               \
+            throw HipError(e_ == hipErrorOutOfMemory ? GND_E_NOMEM : GND_E_DEVICE,                         \
+                           std::string(#expr) + ": " + hipGetErrorString(e_));                              \
+    } while (0)
+
+template <class F>
+int guarded(gnd_hip_ctx* ctx, F&& f) {
+    try {
+        f();
+        return GND_OK;
+    } catch (const HipError& e) {
+        if (ctx) ctx->err = e.what();
+        g_lastError = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        if (ctx) ctx->err = e.what();
+        g_lastError = e.what();
+        return GND_E_INVALID;
+    }
+}
+
+template <class T>
+T* devAlloc(gnd_hip_scene* s, size_t count, const T* src) {
+    if (count == 0) count = 1;
+    void* p = nullptr;
+    GND_CHECK(hipMalloc(&p, count * sizeof(T)));
+    s->allocations.push_back(p);
+    s->bytes += count * sizeof(T);
+    if (src) GND_CHECK(hipMemcpy(p, src, count * sizeof(T), hipMemcpyHostToDevice));
+    return static_cast<T*>(p);
+}
+
+DNode toDNode(const gnd_bvh4_node& n, uint32_t innerBase, uint32_t leafBase) {
+    DNode d;
+    float* lo[3] = {&d.xlo.x, &d.ylo.x, &d.zlo.x};
+    float* hi[3] = {&d.xhi.x, &d.yhi.x, &d.zhi.x};
+    const float* slo[3] = {n.min_x, n.min_y, n.min_z};
+    const float* shi[3] = {n.max_x, n.max_y, n.max_z};
+    for (int a = 0; a < 3; ++a)
+        for (int i = 0; i < 4; ++i) {
+            lo[a][i] = slo[a][i];
+            hi[a][i] = shi[a][i];
+        }
+    uint32_t* ch = &d.child.x;
+    uint32_t* me = &d.meta.x;
+    for (int i = 0; i < 4; ++i) {
+        uint32_t type = n.child_type[i];
+        if (type == 1) ch[i] = n.offset[i] + innerBase;
+        else if (type == 2) ch[i] = n.offset[i] + leafBase;
+        else ch[i] = 0;
+        if (n.count[i] > 0x3FFFFFFFu) throw std::runtime_error("BVH leaf count too large");
+        me[i] = n.count[i] | (type << 30);
+    }
+    return d;
+}
+
+DCamera makeCamera(const gnd_camera_desc& c) {
+    DCamera d{};
+    std::memcpy(d.toWorld.m, c.to_world, sizeof(d.toWorld.m));
+    d.type = c.type;
+    d.width = c.width;
+    d.height = c.height;
+    d.blades = c.blades;
+    // PerspectiveCamera / ThinLensCamera ctor: m_scale = tan(Radians(fov / 2))
+    d.scale = std::tan((kPi / 180.f) * (c.fov * 0.5f));
+    d.aspect = c.width / (float)c.height;
+    d.lensRadius = c.lens_radius;
+    d.focalDistance = c.focal_distance;
+    d.k1 = c.k1;
+    d.caAxial = c.ca_axial;
+    d.caLateral = c.ca_lateral;
+    d.vignetting = c.vignetting;
+    d.filter = c.filter;
+    d.radiusX = c.filter_radius[0];
+    d.radiusY = c.filter_radius[1];
+    d.alpha = c.filter_alpha;
+    d.expX = std::exp(-c.filter_alpha * c.filter_radius[0] * c.filter_radius[0]);
+    d.expY = std::exp(-c.filter_alpha * c.filter_radius[1] * c.filter_radius[1]);
+    d.hasCA = (c.type == GND_CAMERA_THINLENS && (c.ca_lateral != 0.0f || c.ca_axial != 0.0f)) ? 1 : 0;
+    return d;
+}
+
+template <class T>
+void ensureDev(T*& p, size_t& cap, size_t need) {
+    if (need <= cap) return;
+    if (p) GND_CHECK(hipFree(p));
+    size_t n = std::max(need, cap * 2);
+    GND_CHECK(hipMalloc((void**)&p, n * sizeof(T)));
+    cap = n;
+}
+
+void ensureWaveBuffers(gnd_hip_ctx* ctx, size_t cap) {
+    if (cap <= ctx->capacity) return;
+    if (ctx->pathBuf) GND_CHECK(hipFree(ctx->pathBuf));
+    ctx->pathBuf = nullptr;
+    ctx->capacity = 0;
+    // 63 four-byte fields and one eight-byte field per slot, each array 256-B aligned
+    const size_t per = ((cap * 4 + 255) / 256) * 256;
+    const size_t bytes = 63 * per + ((cap * 8 + 255) / 256) * 256;
+    GND_CHECK(hipMalloc(&ctx->pathBuf, bytes));
+    char* b = static_cast<char*>(ctx->pathBuf);
+    size_t off = 0;
+    auto take = [&](size_t elemBytes) {
+        void* p = b + off;
+        off += ((cap * elemBytes + 255) / 256) * 256;
+        return p;
+    };
+    PathSoA& P = ctx->P;
+    P.rng = (uint64_t*)take(8);
+    float** f[] = {&P.ox, &P.oy, &P.oz, &P.dx, &P.dy, &P.dz, &P.tpr, &P.tpg, &P.tpb, &P.Lr, &P.Lg, &P.Lb,
+                   &P.pdfPrev, &P.lx, &P.ly, &P.lz, &P.weight, &P.pfx, &P.pfy, &P.evpx, &P.evpy, &P.evpz,
+                   &P.evnx, &P.evny, &P.evnz, &P.evtx, &P.evty, &P.evtz, &P.evu, &P.evv};
+    for (auto pp : f) *pp = (float*)take(4);
+    P.c0 = (uint32_t*)take(4);
+    P.c1 = (uint32_t*)take(4);
+    P.flags = (uint32_t*)take(4);
+    P.job = (int32_t*)take(4);
+    P.evKind = (int32_t*)take(4);
+    P.evPrim = (int32_t*)take(4);
+    P.evMedium = (int32_t*)take(4);
+    ctx->Hq.t = (float*)take(4);
+    ctx->Hq.u = (float*)take(4);
+    ctx->Hq.v = (float*)take(4);
+    ctx->Hq.inst = (int32_t*)take(4);
+    ctx->Hq.prim = (int32_t*)take(4);
+    ShadowSoA& S = ctx->Sh;
+    S.path = (int32_t*)take(4);
+    S.medium = (int32_t*)take(4);
+    float** g[] = {&S.ox, &S.oy, &S.oz, &S.dx, &S.dy, &S.dz, &S.tmax, &S.pr, &S.pg, &S.pb, &S.lpx, &S.lpy, &S.lpz};
+    for (auto pp : g) *pp = (float*)take(4);
+    ctx->q[0] = (int32_t*)take(4);
+    ctx->q[1] = (int32_t*)take(4);
+    ctx->qEv = (int32_t*)take(4);
+    if (off > bytes) throw std::runtime_error("wave buffer layout overflow");
+    ctx->capacity = cap;
+}
+
+using TraceKernel = void (*)(DScene, gnd_ray_soa, int64_t, gnd_hit_soa, int32_t*, Counters*);
+
+template <bool ANY, bool COUNT>
+TraceKernel pickTrace(int lds) {
+    switch (lds) {
+        case 8: return kTraceBatch<8, ANY, COUNT>;
+        case 32: return kTraceBatch<32, ANY, COUNT>;
+        default: return kTraceBatch<16, ANY, COUNT>;
+    }
+}
+int normLds(int lds) { return lds <= 8 ? 8 : (lds >= 32 ? 32 : 16); }
+
+using WaveKernel = void (*)(WaveArgs);
+template <bool COUNT>
+WaveKernel pickTracePaths(int lds) {
+    switch (lds) {
+        case 8: return kTracePaths<8, COUNT>;
+        case 32: return kTracePaths<32, COUNT>;
+        default: return kTracePaths<16, COUNT>;
+    }
+}
+template <bool FAST, bool COUNT>
+WaveKernel pickShadow(int lds) {
+    switch (lds) {
+        case 8: return kShadow<8, FAST, COUNT>;
+        case 32: return kShadow<32, FAST, COUNT>;
+        default: return kShadow<16, FAST, COUNT>;
+    }
+}
+
+float elapsedMs(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.0f;
+    GND_CHECK(hipEventElapsedTime(&ms, a, b));
+    return ms;
+}
+
+}  // namespace
+
+// ============================================================================ C-ABI
+
+extern "C" {
+
+const char* gnd_hip_last_error(const gnd_hip_ctx* ctx) { return ctx ? ctx->err.c_str() : g_lastError.c_str(); }
+
+int gnd_hip_init(int device, gnd_hip_ctx** out) {
+    if (!out) return GND_E_INVALID;
+    gnd_hip_ctx* ctx = new gnd_hip_ctx();
+    int rc = guarded(ctx, [&] {
+        int n = 0;
+        GND_CHECK(hipGetDeviceCount(&n));
+        if (device < 0 || device >= n) throw std::runtime_error("no such HIP device");
+        ctx->device = device;
+        GND_CHECK(hipSetDevice(device));
+        hipDeviceProp_t prop;
+        GND_CHECK(hipGetDeviceProperties(&prop, device));
+        ctx->numCUs = prop.multiProcessorCount;
+        ctx->hbm = prop.totalGlobalMem;
+        ctx->name = prop.gcnArchName;
+        GND_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        GND_CHECK(hipMalloc(&ctx->dCounters, sizeof(Counters)));
+        GND_CHECK(hipMemset(ctx->dCounters, 0, sizeof(Counters)));
+        GND_CHECK(hipMalloc(&ctx->counts, 16 * sizeof(uint32_t)));
+        GND_CHECK(hipEventCreate(&ctx->ev[0]));
+        GND_CHECK(hipEventCreate(&ctx->ev[1]));
+    });
+    if (rc != GND_OK) {
+        g_lastError = ctx->err;
+        delete ctx;
+        *out = nullptr;
+        return rc;
+    }
+    *out = ctx;
+    return GND_OK;
+}
+
+void gnd_hip_shutdown(gnd_hip_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->pathBuf) (void)hipFree(ctx->pathBuf);
+    if (ctx->dCounters) (void)hipFree(ctx->dCounters);
+    if (ctx->counts) (void)hipFree(ctx->counts);
+    if (ctx->dJobs) (void)hipFree(ctx->dJobs);
+    if (ctx->dChunks) (void)hipFree(ctx->dChunks);
+    if (ctx->dTasks) (void)hipFree(ctx->dTasks);
+    for (auto e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int gnd_hip_device_info(gnd_hip_ctx* ctx, char* name, int name_len, int* num_cus, size_t* hbm_bytes) {
+    if (!ctx) return GND_E_INVALID;
+    if (name && name_len > 0) {
+        std::strncpy(name, ctx->name.c_str(), name_len - 1);
+        name[name_len - 1] = 0;
+    }
+    if (num_cus) *num_cus = ctx->numCUs;
+    if (hbm_bytes) *hbm_bytes = ctx->hbm;
+    return GND_OK;
+}
+
+int gnd_hip_set_tuning(gnd_hip_ctx* ctx, const gnd_hip_tuning* t) {
+    if (!ctx || !t) return GND_E_INVALID;
+    if (t->block_size != 64 && t->block_size != 128 && t->block_size != 256) {
+        ctx->err = "block_size must be 64, 128 or 256";
+        return GND_E_INVALID;
+    }
+    if (t->queue_capacity < 1024) {
+        ctx->err = "queue_capacity too small";
+        return GND_E_INVALID;
+    }
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->tuning = *t;
+    ctx->tuning.stack_lds = normLds(t->stack_lds);
+    return GND_OK;
+}
+
+int gnd_hip_get_tuning(gnd_hip_ctx* ctx, gnd_hip_tuning* t) {
+    if (!ctx || !t) return GND_E_INVALID;
+    *t = ctx->tuning;
+    return GND_OK;
+}
+
+int gnd_hip_set_counters(gnd_hip_ctx* ctx, int enabled) {
+    if (!ctx) return GND_E_INVALID;
+    ctx->counters = enabled != 0;
+    return GND_OK;
+}
+
+int gnd_hip_malloc(gnd_hip_ctx* ctx, size_t bytes, void** dptr) {
+    return guarded(ctx, [&] {
+        GND_CHECK(hipSetDevice(ctx->device));
+        GND_CHECK(hipMalloc(dptr, bytes ? bytes : 1));
+    });
+}
+int gnd_hip_free(gnd_hip_ctx* ctx, void* dptr) {
+    return guarded(ctx, [&] { GND_CHECK(hipFree(dptr)); });
+}
+int gnd_hip_memcpy_h2d(gnd_hip_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    return guarded(ctx, [&] {
+        GND_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+        GND_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+int gnd_hip_memcpy_d2h(gnd_hip_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    return guarded(ctx, [&] {
+        GND_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        GND_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+int gnd_hip_memset(gnd_hip_ctx* ctx, void* dst, int value, size_t bytes) {
+    return guarded(ctx, [&] {
+        GND_CHECK(hipMemsetAsync(dst, value, bytes, ctx->stream));
+        GND_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+int gnd_hip_synchronize(gnd_hip_ctx* ctx) {
+    return guarded(ctx, [&] { GND_CHECK(hipStreamSynchronize(ctx->stream)); });
+}
+
+int gnd_hip_scene_upload(gnd_hip_ctx* ctx, const gnd_hip_scene_arrays* a, gnd_hip_scene** out) {
+    if (!ctx || !a || !out || !a->desc) return GND_E_INVALID;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    gnd_hip_scene* s = new gnd_hip_scene();
+    int rc = guarded(ctx, [&] {
+        GND_CHECK(hipSetDevice(ctx->device));
+        const gnd_scene_desc& d = *a->desc;
+        // TLAS
+        std::vector<DNode> tl(a->n_tlas_nodes);
+        for (uint32_t i = 0; i < a->n_tlas_nodes; ++i) tl[i] = toDNode(a->tlas_nodes[i], 0, 0);
+        // BLAS: children made absolute per mesh
+        std::vector<DNode> bl(a->n_blas_nodes);
+        for (uint32_t m = 0; m < a->n_meshes; ++m) {
+            const gnd_mesh_info& mi = a->meshes[m];
+            if (mi.node_base + mi.n_nodes > a->n_blas_nodes || mi.pack_base + mi.n_packs > a->n_packs)
+                throw std::runtime_error("mesh slice out of range");
+            for (uint32_t i = 0; i < mi.n_nodes; ++i)
+                bl[mi.node_base + i] = toDNode(a->blas_nodes[mi.node_base + i], mi.node_base, mi.pack_base);
+        }
+        if (d.nprims == 0) throw std::runtime_error("scene has no primitives");
+        std::vector<DInst> inst(d.nprims);
+        std::vector<DPrim> prims(d.nprims);
+        std::vector<int32_t> emitters;
+        for (uint32_t i = 0; i < d.nprims; ++i) {
+            const gnd_primitive_desc& p = d.prims[i];
+            DInst& I = inst[i];
+            std::memcpy(I.worldToObject.m, p.to_world_inv, sizeof(I.worldToObject.m));
+            I.radius = p.radius;
+            I.pad0 = I.pad1 = 0;
+            DPrim& P = prims[i];
+            std::memset(&P, 0, sizeof(P));
+            std::memcpy(P.toWorld.m, p.to_world, sizeof(P.toWorld.m));
+            std::memcpy(P.toWorldInv.m, p.to_world_inv, sizeof(P.toWorldInv.m));
+            P.shape = p.shape_type;
+            P.mesh = p.mesh;
+            P.material = p.material;
+            if (p.material < 0 || (uint32_t)p.material >= d.nmaterials) throw std::runtime_error("bad material index");
+            P.radius = p.radius;
+            P.radiance[0] = p.radiance[0]; P.radiance[1] = p.radiance[1]; P.radiance[2] = p.radiance[2];
+            P.emitter = -1;
+            if (p.has_emitter) {
+                P.emitter = (int32_t)emitters.size();
+                emitters.push_back((int32_t)i);
+            }
+            if (p.shape_type == GND_SHAPE_MESH) {
+                if (p.mesh < 0 || (uint32_t)p.mesh >= a->n_meshes) throw std::runtime_error("bad mesh index");
+                const gnd_mesh_info& mi = a->meshes[p.mesh];
+                if (mi.n_nodes == 0) throw std::runtime_error("mesh without geometry");
+                I.root = (int32_t)mi.node_base;
+                P.triBase = mi.tri_base;
+                P.nTris = mi.n_tris;
+                P.cdfBase = mi.cdf_base;
+                P.hasUV = mi.has_uv;
+                P.totalArea = mi.total_area;
+            } else {
+                I.root = -1;
+            }
+        }
+        if (emitters.empty()) throw std::runtime_error("scene has no emitter");
+        std::vector<DMaterial> mats(d.nmaterials);
+        for (uint32_t i = 0; i < d.nmaterials; ++i) {
+            const gnd_material_desc& m = d.materials[i];
+            DMaterial& M = mats[i];
+            std::memset(&M, 0, sizeof(M));
+            M.type = m.type;
+            M.checker = m.albedo_is_checker;
+            M.mediumInside = m.medium_inside;
+            M.mediumOutside = m.medium_outside;
+            for (int c = 0; c < 3; ++c) {
+                M.albedo[c] = m.albedo[c];
+                M.R[c] = m.R[c];
+                M.T[c] = m.T[c];
+                M.etaRGB[c] = m.eta_rgb[c];
+                M.k[c] = m.k_rgb[c];
+                M.Kd[c] = m.Kd[c];
+                M.Ks[c] = m.Ks[c];
+            }
+            M.eta = m.eta;
+            M.roughness = m.roughness;
+            // roughnessToAlpha then the distribution's clamp (microfacet.h:26-39)
+            float r = std::max(m.roughness, 1e-3f);
+            M.alpha = std::max(1e-4f, r * r);
+            if ((m.medium_inside >= (int)d.nmedia) || (m.medium_outside >= (int)d.nmedia))
+                throw std::runtime_error("bad medium index");
+        }
+        std::vector<DMedium> media(d.nmedia);
+        for (uint32_t i = 0; i < d.nmedia; ++i) {
+            const gnd_medium_desc& m = d.media[i];
+            DMedium& M = media[i];
+            std::memset(&M, 0, sizeof(M));
+            M.type = m.type;
+            for (int c = 0; c < 3; ++c) {
+                M.sigma_a[c] = m.sigma_a[c];
+                M.sigma_s[c] = m.sigma_s[c];
+                M.sigma_t[c] = m.sigma_a[c] + m.sigma_s[c];
+            }
+            M.g = m.g;
+            M.maxSigmaT = std::max({M.sigma_t[0], M.sigma_t[1], M.sigma_t[2]});
+            std::memcpy(M.toLocal.m, m.to_local, sizeof(M.toLocal.m));
+            if (m.type == GND_MEDIUM_GRID) {
+                if (!m.density || m.nx <= 0 || m.ny <= 0 || m.nz <= 0) throw std::runtime_error("bad grid medium");
+                s->hasGrid = 1;
+                M.nx = m.nx; M.ny = m.ny; M.nz = m.nz;
+                M.densityScale = m.density_scale;
+                // macro majorant grid (heterogeneous.cpp:63-103) over the dense [0, n) box
+                float ex = (float)(m.nx - 1) + 1.0f, ey = (float)(m.ny - 1) + 1.0f, ez = (float)(m.nz - 1) + 1.0f;
+                M.gridMaxX = ex; M.gridMaxY = ey; M.gridMaxZ = ez;
+                int target = m.macro_res > 0 ? m.macro_res : 32;
+                float maxE = std::max({ex, ey, ez});
+                M.resX = std::max(1, (int)std::round(target * (ex / maxE)));
+                M.resY = std::max(1, (int)std::round(target * (ey / maxE)));
+                M.resZ = std::max(1, (int)std::round(target * (ez / maxE)));
+                M.cellX = ex / M.resX;
+                M.cellY = ey / M.resY;
+                M.cellZ = ez / M.resZ;
+                std::vector<float> maj((size_t)M.resX * M.resY * M.resZ, 0.0f);
+                for (int z = 0; z < m.nz; ++z)
+                    for (int y = 0; y < m.ny; ++y)
+                        for (int x = 0; x < m.nx; ++x) {
+                            float dens = m.density[((size_t)z * m.ny + y) * m.nx + x] * m.density_scale;
+                            if (dens <= 0.0f || !std::isfinite(dens)) continue;
+                            int cx = std::clamp((int)((x - 0.0f) / M.cellX), 0, M.resX - 1);
+                            int cy = std::clamp((int)((y - 0.0f) / M.cellY), 0, M.resY - 1);
+                            int cz = std::clamp((int)((z - 0.0f) / M.cellZ), 0, M.resZ - 1);
+                            size_t fi = (size_t)cz * (M.resX * M.resY) + cy * M.resX + cx;
+                            float lm = dens * M.maxSigmaT;
+                            if (lm > maj[fi]) maj[fi] = lm;
+                        }
+                M.density = devAlloc<float>(s, (size_t)m.nx * m.ny * m.nz, m.density);
+                M.majorants = devAlloc<float>(s, maj.size(), maj.data());
+            }
+        }
+        for (uint32_t i = 0; i < d.nmaterials; ++i)
+            if (d.materials[i].type == GND_MAT_INDEX) s->hasIndex = 1;
+        DScene& v = s->view;
+        v.tlas = devAlloc<DNode>(s, tl.size(), tl.data());
+        v.tlasPrims = devAlloc<int32_t>(s, a->n_tlas_prims, a->tlas_prims);
+        v.inst = devAlloc<DInst>(s, inst.size(), inst.data());
+        v.blas = devAlloc<DNode>(s, bl.size(), bl.data());
+        v.packs = reinterpret_cast<const DPack*>(devAlloc<gnd_tri_pack4>(s, a->n_packs, a->packs));
+        v.prims = devAlloc<DPrim>(s, prims.size(), prims.data());
+        v.shading = devAlloc<float>(s, (size_t)a->n_tris * 24, a->shading);
+        v.cdf = devAlloc<float>(s, a->n_cdf, a->cdf);
+        v.mats = devAlloc<DMaterial>(s, mats.size(), mats.data());
+        v.media = devAlloc<DMedium>(s, media.size(), media.data());
+        v.emitters = devAlloc<int32_t>(s, emitters.size(), emitters.data());
+        v.nTlas = (int32_t)tl.size();
+        v.nEmitters = (int32_t)emitters.size();
+        v.nPrims = (int32_t)d.nprims;
+        v.hasIndex = s->hasIndex;
+        v.hasGrid = s->hasGrid;
+        s->nPrims = d.nprims;
+        s->nMedia = (int)d.nmedia;
+    });
+    if (rc != GND_OK) {
+        for (void* p : s->allocations) (void)hipFree(p);
+        delete s;
+        *out = nullptr;
+        return rc;
+    }
+    *out = s;
+    return GND_OK;
+}
+
+void gnd_hip_scene_free(gnd_hip_ctx* ctx, gnd_hip_scene* s) {
+    if (!s) return;
+    if (ctx) {
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+    }
+    for (void* p : s->allocations) (void)hipFree(p);
+    delete s;
+}
+
+size_t gnd_hip_scene_bytes(const gnd_hip_scene* s) { return s ? s->bytes : 0; }
+
+static int traceBatch(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_ray_soa* rays, int64_t n,
+                      const gnd_hit_soa* hits, int32_t* occ, bool any, gnd_hip_stats* stats) {
+    if (!ctx || !scene || !rays || n < 0 || (!any && !hits) || (any && !occ)) return GND_E_INVALID;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    return guarded(ctx, [&] {
+        GND_CHECK(hipSetDevice(ctx->device));
+        if (n == 0) return;
+        GND_CHECK(hipMemsetAsync(ctx->dCounters, 0, sizeof(Counters), ctx->stream));
+        const int B = ctx->tuning.block_size, lds = normLds(ctx->tuning.stack_lds);
+        int64_t blocks = (n + B - 1) / B;
+        blocks = (blocks + 7) / 8 * 8;
+        if (blocks > INT32_MAX) throw std::runtime_error("ray batch too large");
+        gnd_hit_soa h{};
+        if (hits) h = *hits;
+        TraceKernel k = any ? (ctx->counters ? pickTrace<true, true>(lds) : pickTrace<true, false>(lds))
+                            : (ctx->counters ? pickTrace<false, true>(lds) : pickTrace<false, false>(lds));
+        GND_CHECK(hipEventRecord(ctx->ev[0], ctx->stream));
+        hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(B), (size_t)B * lds * sizeof(int), ctx->stream,
+                           scene->view, *rays, n, h, occ, ctx->dCounters);
+        GND_CHECK(hipGetLastError());
+        GND_CHECK(hipEventRecord(ctx->ev[1], ctx->stream));
+        Counters c;
+        GND_CHECK(hipMemcpyAsync(&c, ctx->dCounters, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
+        GND_CHECK(hipStreamSynchronize(ctx->stream));
+        if (c.stackOverflow) throw HipError(GND_E_STACK, "traversal stack overflow");
+        if (stats) {
+            std::memset(stats, 0, sizeof(*stats));
+            stats->ms_total = stats->ms_trace = elapsedMs(ctx->ev[0], ctx->ev[1]);
+            stats->rays_closest = any ? 0 : (uint64_t)n;
+            stats->rays_shadow = any ? (uint64_t)n : 0;
+            stats->node_visits = c.nodes;
+            stats->pack_tests = c.packs;
+            stats->inst_tests = c.insts;
+            stats->trace_launches = 1;
+        }
+    });
+}
+
+int gnd_hip_trace_closest(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_ray_soa* rays, int64_t n,
+                          const gnd_hit_soa* hits, gnd_hip_stats* stats) {
+    return traceBatch(ctx, scene, rays, n, hits, nullptr, false, stats);
+}
+
+int gnd_hip_trace_any(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_ray_soa* rays, int64_t n,
+                      int32_t* occluded, gnd_hip_stats* stats) {
+    return traceBatch(ctx, scene, rays, n, nullptr, occluded, true, stats);
+}
+
+int gnd_hip_cancel(gnd_hip_ctx* ctx) {
+    if (!ctx) return GND_E_INVALID;
+    ctx->cancel = 1;
+    return GND_OK;
+}
+
+// ---------------------------------------------------------------- render
+
+struct WavePlan {
+    std::vector<DChunk> chunks;
+    std::vector<DSplatTask> tasks;
+    uint32_t slots = 0, pix = 0;
+};
+
+// Packs passes of the jobs into waves of at most `cap` slots.  A job's passes
+// enter waves in pass order; every pass is whole (the film gather needs the
+// neighbouring samples of the same pass).
+static std::vector<WavePlan> planWaves(const std::vector<gnd_hip_job>& jobs, size_t cap) {
+    std::vector<WavePlan> waves;
+    std::vector<uint32_t> next(jobs.size(), 0);
+    for (;;) {
+        WavePlan w;
+        bool any = false;
+        for (size_t j = 0; j < jobs.size(); ++j) {
+            const uint32_t npix = (uint32_t)jobs[j].camera.width * (uint32_t)jobs[j].camera.height;
+            if (next[j] >= jobs[j].spp) continue;
+            int first = (int)w.chunks.size();
+            while (next[j] < jobs[j].spp && (w.slots + npix <= cap || (w.slots == 0))) {
+                w.chunks.push_back({w.slots, npix, (int32_t)j, (int32_t)next[j]});
+                w.slots += npix;
+                next[j]++;
+                any = true;
+                if (w.slots >= cap) break;
+            }
+            if ((int)w.chunks.size() > first) {
+                w.tasks.push_back({w.pix, (int32_t)j, first, (int32_t)w.chunks.size()});
+                w.pix += npix;
+            }
+            if (w.slots >= cap) break;
+        }
+        if (!any) break;
+        waves.push_back(std::move(w));
+    }
+    return waves;
+}
+
+int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jobsIn, int njobs,
+                   gnd_hip_stats* stats) {
+    if (!ctx || !scene || !jobsIn || njobs <= 0) return GND_E_INVALID;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->cancel = 0;
+    return guarded(ctx, [&] {
+        GND_CHECK(hipSetDevice(ctx->device));
+        std::vector<gnd_hip_job> jobs(jobsIn, jobsIn + njobs);
+        size_t maxPix = 0;
+        for (auto& j : jobs) {
+            if (j.camera.width <= 0 || j.camera.height <= 0 || !j.film)
+                throw std::runtime_error("job needs a positive resolution and a film");
+            maxPix = std::max(maxPix, (size_t)j.camera.width * j.camera.height);
+        }
+        const size_t cap = std::max<size_t>((size_t)ctx->tuning.queue_capacity, maxPix);
+        if (cap > 0x7FFFFFFFull) throw std::runtime_error("wave capacity too large");
+        ensureWaveBuffers(ctx, cap);
+        std::vector<DJob> dj(jobs.size());
+        for (size_t j = 0; j < jobs.size(); ++j) {
+            DJob& J = dj[j];
+            J.cam = makeCamera(jobs[j].camera);
+            J.film = jobs[j].film;
+            J.filmSq = jobs[j].film_sq;
+            J.seedOffset = jobs[j].seed_offset;
+            J.maxDepth = jobs[j].max_depth;
+            J.integrator = jobs[j].integrator;
+            J.W = jobs[j].camera.width;
+            J.H = jobs[j].camera.height;
+            const size_t np = (size_t)J.W * J.H;
+            hipLaunchKernelGGL(kFill, dim3((unsigned)((np * 4 + 255) / 256)), dim3(256), 0, ctx->stream, J.film,
+                               np * 4, 0.0f);
+            if (J.filmSq)
+                hipLaunchKernelGGL(kFill, dim3((unsigned)((np * 3 + 255) / 256)), dim3(256), 0, ctx->stream,
+                                   J.filmSq, np * 3, 0.0f);
+        }
+        ensureDev(ctx->dJobs, ctx->jobCap, dj.size());
+        GND_CHECK(hipMemcpyAsync(ctx->dJobs, dj.data(), dj.size() * sizeof(DJob), hipMemcpyHostToDevice,
+                                 ctx->stream));
+        GND_CHECK(hipMemsetAsync(ctx->dCounters, 0, sizeof(Counters), ctx->stream));
+        const std::vector<WavePlan> waves = planWaves(jobs, cap);
+        bool anyVolGrid = false, anyVol = false;
+        for (auto& j : jobs) anyVol = anyVol || j.integrator == GND_INTEGRATOR_VOLPATH;
+        anyVolGrid = anyVol && scene->hasGrid;
+        const bool split = anyVolGrid;
+        // any-hit shadows are exact when no surface is a null BSDF and no
+        // shadow segment can cross a medium (path integrator ignores media)
+        const bool fastShadow = !scene->hasIndex && (!anyVol || scene->nMedia == 0);
+        const int B = ctx->tuning.block_size, lds = normLds(ctx->tuning.stack_lds);
+        const size_t shm = (size_t)B * lds * sizeof(int);
+        const bool cnt = ctx->counters;
+        WaveKernel kTrace = cnt ? pickTracePaths<true>(lds) : pickTracePaths<false>(lds);
+        WaveKernel kSh = fastShadow ? (cnt ? pickShadow<true, true>(lds) : pickShadow<true, false>(lds))
+                                    : (cnt ? pickShadow<false, true>(lds) : pickShadow<false, false>(lds));
+        const unsigned gridCap = (unsigned)std::max(8, ctx->numCUs * 8);
+        auto grid = [&](uint32_t n) {
+            unsigned b = (unsigned)((n + B - 1) / B);
+            b = std::min(b, gridCap);
+            b = std::max(8u, (b + 7) / 8 * 8);
+            return b;
+        };
+        std::vector<hipEvent_t> evs;
+        auto mark = [&]() {
+            hipEvent_t e;
+            GND_CHECK(hipEventCreate(&e));
+            GND_CHECK(hipEventRecord(e, ctx->stream));
+            evs.push_back(e);
+            return evs.size() - 1;
+        };
+        struct Span { size_t a, b; int kind; };  // 0 trace, 1 shadow, 2 other
+        std::vector<Span> spans;
+        uint64_t rays = 0, shadowRays = 0, launches = 0, samples = 0;
+        for (auto& j : jobs) samples += (uint64_t)j.camera.width * j.camera.height * j.spp;
+        const size_t t0 = mark();
+        for (const WavePlan& w : waves) {
+            if (ctx->cancel) throw HipError(GND_E_CANCELLED, "render cancelled");
+            ensureDev(ctx->dChunks, ctx->chunkCap, w.chunks.size());
+            ensureDev(ctx->dTasks, ctx->taskCap, w.tasks.size());
+            GND_CHECK(hipMemcpyAsync(ctx->dChunks, w.chunks.data(), w.chunks.size() * sizeof(DChunk),
+                                     hipMemcpyHostToDevice, ctx->stream));
+            GND_CHECK(hipMemcpyAsync(ctx->dTasks, w.tasks.data(), w.tasks.size() * sizeof(DSplatTask),
+                                     hipMemcpyHostToDevice, ctx->stream));
+            hipLaunchKernelGGL(kFillU32, dim3(1), dim3(16), 0, ctx->stream, ctx->counts, (size_t)16, 0u);
+            WaveArgs A{};
+            A.S = scene->view;
+            A.jobs = ctx->dJobs;
+            A.chunks = ctx->dChunks;
+            A.nChunks = (int32_t)w.chunks.size();
+            A.nSlots = w.slots;
+            A.P = ctx->P;
+            A.Hq = ctx->Hq;
+            A.Sh = ctx->Sh;
+            A.counts = ctx->counts;
+            A.ctr = ctx->dCounters;
+            A.pathOnlyFast = fastShadow ? 1 : 0;
+            A.split = split ? 1 : 0;
+            int cur = 0;
+            A.qIn = ctx->q[0];
+            A.qOut = ctx->q[1];
+            size_t e0 = mark();
+            hipLaunchKernelGGL(kCamera, dim3((w.slots + 255) / 256), dim3(256), 0, ctx->stream, A);
+            GND_CHECK(hipGetLastError());
+            spans.push_back({e0, mark(), 2});
+            uint32_t live = w.slots;
+            uint32_t hcounts[4];
+            while (live > 0) {
+                A.qIn = ctx->q[cur];
+                A.qOut = ctx->q[1 - cur];
+                size_t a = mark();
+                hipLaunchKernelGGL(kTrace, dim3(grid(live)), dim3(B), shm, ctx->stream, A);
+                GND_CHECK(hipGetLastError());
+                size_t b = mark();
+                spans.push_back({a, b, 0});
+                rays += live;
+                launches++;
+                if (split) {
+                    // NEE, then shadows (consuming transmittance samples), then scatter
+                    hipLaunchKernelGGL(kShade<true>, dim3(grid(live)), dim3(B), 0, ctx->stream, A);
+                    GND_CHECK(hipGetLastError());
+                } else {
+                    hipLaunchKernelGGL(kShade<false>, dim3(grid(live)), dim3(B), 0, ctx->stream, A);
+                    GND_CHECK(hipGetLastError());
+                }
+                size_t c = mark();
+                spans.push_back({b, c, 2});
+                GND_CHECK(hipMemcpyAsync(hcounts, ctx->counts, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                         ctx->stream));
+                GND_CHECK(hipStreamSynchronize(ctx->stream));
+                const uint32_t nShadow = hcounts[2];
+                if (nShadow > 0) {
+                    size_t d0 = mark();
+                    hipLaunchKernelGGL(kSh, dim3(grid(nShadow)), dim3(B), shm, ctx->stream, A);
+                    GND_CHECK(hipGetLastError());
+                    spans.push_back({d0, mark(), 1});
+                    shadowRays += nShadow;
+                }
+                if (split) {
+                    // the paths of this bounce with an event scatter now; the
+                    // in-queue holds them (kShade<true> appended nothing)
+                    size_t s0 = mark();
+                    hipLaunchKernelGGL(kScatter, dim3(grid(live)), dim3(B), 0, ctx->stream, A,
+                                       (const int32_t*)A.qIn, live);
+                    GND_CHECK(hipGetLastError());
+                    spans.push_back({s0, mark(), 2});
+                    GND_CHECK(hipMemcpyAsync(hcounts, ctx->counts, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                             ctx->stream));
+                    GND_CHECK(hipStreamSynchronize(ctx->stream));
+                }
+                live = hcounts[1];
+                // next bounce reads the out-queue; its count moves to slot 0
+                GND_CHECK(hipMemcpyAsync(ctx->counts, ctx->counts + 1, sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                                         ctx->stream));
+                cur = 1 - cur;
+            }
+            size_t s0 = mark();
+            hipLaunchKernelGGL(kSplat, dim3((w.pix + 255) / 256), dim3(256), 0, ctx->stream, A,
+                               (const DSplatTask*)ctx->dTasks, (int)w.tasks.size(), w.pix);
+            GND_CHECK(hipGetLastError());
+            spans.push_back({s0, mark(), 2});
+        }
+        const size_t t1 = mark();
+        Counters c;
+        GND_CHECK(hipMemcpyAsync(&c, ctx->dCounters, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
+        GND_CHECK(hipStreamSynchronize(ctx->stream));
+        if (stats) {
+            std::memset(stats, 0, sizeof(*stats));
+            stats->ms_total = elapsedMs(evs[t0], evs[t1]);
+            for (auto& sp : spans) {
+                double ms = elapsedMs(evs[sp.a], evs[sp.b]);
+                if (sp.kind == 0) stats->ms_trace += ms;
+                else if (sp.kind == 1) stats->ms_shadow += ms;
+                else stats->ms_shade += ms;
+            }
+            stats->samples = samples;
+            stats->rays_closest = rays;
+            stats->rays_shadow = shadowRays;
+            stats->node_visits = c.nodes;
+            stats->pack_tests = c.packs;
+            stats->inst_tests = c.insts;
+            stats->trace_launches = launches;
+            stats->nan_samples = c.nan;
+        }
+        for (auto e : evs) (void)hipEventDestroy(e);
+        if (c.stackOverflow) throw HipError(GND_E_STACK, "traversal stack overflow");
+    });
+}
+
+int gnd_hip_autotune(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* job, gnd_hip_tuning* out) {
+    (void)scene;
+    (void)job;
+    if (!ctx || !out) return GND_E_INVALID;
+    *out = ctx->tuning;
+    return GND_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,77 @@
+// scene_host.h — owned scene storage of the host shell and its flattening
+// into the arrays the device library uploads.
+#pragma once
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../../include/gnd_hip.h"
+#include "bvh_build.h"
+#include "hmath.h"
+
+namespace gnd {
+
+struct MeshData {
+    std::vector<float> positions;    // 9 per triangle
+    std::vector<float> normals;      // 9 per triangle or empty
+    std::vector<float> uvs;          // 6 per triangle or empty
+    std::vector<uint32_t> vertexIds; // 3 per triangle or empty
+    uint32_t ntris() const { return (uint32_t)(positions.size() / 9); }
+};
+
+struct MediumData {
+    gnd_medium_desc desc{};
+    std::vector<float> density;
+};
+
+class HostScene {
+public:
+    std::vector<MeshData> meshes;
+    std::vector<gnd_primitive_desc> prims;
+    std::vector<gnd_material_desc> materials;
+    std::vector<MediumData> media;
+    gnd_camera_desc camera{};
+    int integrator = GND_INTEGRATOR_PATH;
+    int maxDepth = 1000;
+    uint32_t spp = 1;
+
+    // helpers for generators and the XML loader
+    int addMesh(MeshData m);
+    int addMaterial(const gnd_material_desc& m);
+    int addPrim(int mesh, const Xform& toWorld, int material, const float* radiance = nullptr);
+    int addSphere(float radius, const Xform& toWorld, int material, const float* radiance = nullptr);
+    void setCamera(int type, int w, int h, float fov, const Xform& toWorld);
+
+    void copyFrom(const gnd_scene_desc& d);
+    const gnd_scene_desc& desc();  // rebuilt on each call
+    const gnd_hip_scene_arrays& build(int threads, double* ms);
+
+private:
+    gnd_scene_desc desc_{};
+    std::vector<gnd_mesh_desc> meshDescs_;
+    std::vector<gnd_medium_desc> mediumDescs_;
+    bool built_ = false;
+    // flattened storage
+    std::vector<gnd_bvh4_node> tlas_;
+    std::vector<int32_t> tlasPrims_;
+    std::vector<gnd_mesh_info> meshInfo_;
+    std::vector<gnd_bvh4_node> blas_;
+    std::vector<gnd_tri_pack4> packs_;
+    std::vector<float> shading_;
+    std::vector<float> cdf_;
+    gnd_hip_scene_arrays arrays_{};
+};
+
+gnd_material_desc materialMatte(float r, float g, float b);
+gnd_material_desc materialConductor(const float* R, const float* eta, const float* k, float roughness);
+gnd_material_desc materialGlass(float eta, float roughness, int inside = -1, int outside = -1);
+gnd_material_desc materialPlastic(const float* Kd, const float* Ks, float eta, float roughness);
+gnd_material_desc materialIndex(int inside, int outside);
+
+// procedural scenes (generators.cpp)
+std::unique_ptr<HostScene> generateScene(const std::string& name, const std::string& params);
+// reference XML schema loader (xml_loader.cpp)
+std::unique_ptr<HostScene> loadXmlScene(const std::string& path);
+
+}  // namespace gnd

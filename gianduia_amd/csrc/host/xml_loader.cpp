@@ -1,0 +1,493 @@
+// xml_loader.cpp — the reference's scene XML schema (src/core/parser.cpp:41-235)
+// read with a small in-tree XML reader (pugixml is not part of this build).
+//
+// Supported: <scene>, <camera type="perspective|thinlens"> with <filter
+// type="box|gaussian">, <sampler type="independent">, <integrator
+// type="path|volpath|path_hip|volpath_hip">, <primitive> with <shape
+// type="mesh"> (OBJ file) or <shape type="sphere">, <material
+// type="matte|conductor|glass|plastic|index"> with nested <medium
+// name="inside|outside" type="homogeneous">, <emitter type="area">, <ref id>
+// sharing of shapes, and <transform> with translate/scale/rotate/lookat steps
+// composed as t = step * t (parser.cpp:112-165).
+#include <cctype>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <stdexcept>
+
+#include "scene_host.h"
+
+namespace gnd {
+
+namespace {
+
+struct XNode {
+    std::string tag;
+    std::map<std::string, std::string> attr;
+    std::vector<XNode> kids;
+    std::string attrOr(const char* k, const std::string& d = "") const {
+        auto it = attr.find(k);
+        return it == attr.end() ? d : it->second;
+    }
+};
+
+class XmlReader {
+public:
+    explicit XmlReader(const std::string& text) : s_(text) {}
+    XNode parseDocument() {
+        skipMisc();
+        XNode root = parseElement();
+        return root;
+    }
+
+private:
+    const std::string& s_;
+    size_t i_ = 0;
+
+    [[noreturn]] void fail(const std::string& m) {
+        throw std::runtime_error("XML parsing failed: " + m + " at offset " + std::to_string(i_));
+    }
+    void skipWs() {
+        while (i_ < s_.size() && std::isspace((unsigned char)s_[i_])) ++i_;
+    }
+    bool startsWith(const char* p) const { return s_.compare(i_, std::strlen(p), p) == 0; }
+    void skipMisc() {
+        for (;;) {
+            skipWs();
+            if (startsWith("<?")) {
+                size_t e = s_.find("?>", i_);
+                if (e == std::string::npos) fail("unterminated declaration");
+                i_ = e + 2;
+            } else if (startsWith("<!--")) {
+                size_t e = s_.find("-->", i_);
+                if (e == std::string::npos) fail("unterminated comment");
+                i_ = e + 3;
+            } else if (startsWith("<!")) {
+                size_t e = s_.find('>', i_);
+                if (e == std::string::npos) fail("unterminated directive");
+                i_ = e + 1;
+            } else {
+                return;
+            }
+        }
+    }
+    std::string name() {
+        size_t b = i_;
+        while (i_ < s_.size() && (std::isalnum((unsigned char)s_[i_]) || s_[i_] == '_' || s_[i_] == '-' ||
+                                  s_[i_] == ':' || s_[i_] == '.'))
+            ++i_;
+        if (b == i_) fail("expected a name");
+        return s_.substr(b, i_ - b);
+    }
+    static std::string unescape(const std::string& v) {
+        std::string o;
+        for (size_t k = 0; k < v.size(); ++k) {
+            if (v[k] == '&') {
+                size_t e = v.find(';', k);
+                std::string ent = v.substr(k + 1, e - k - 1);
+                if (ent == "lt") o += '<';
+                else if (ent == "gt") o += '>';
+                else if (ent == "amp") o += '&';
+                else if (ent == "quot") o += '"';
+                else if (ent == "apos") o += '\'';
+                k = e;
+            } else o += v[k];
+        }
+        return o;
+    }
+    XNode parseElement() {
+        if (i_ >= s_.size() || s_[i_] != '<') fail("expected '<'");
+        ++i_;
+        XNode n;
+        n.tag = name();
+        for (;;) {
+            skipWs();
+            if (startsWith("/>")) { i_ += 2; return n; }
+            if (i_ < s_.size() && s_[i_] == '>') { ++i_; break; }
+            std::string k = name();
+            skipWs();
+            if (i_ >= s_.size() || s_[i_] != '=') fail("expected '='");
+            ++i_;
+            skipWs();
+            char q = s_[i_];
+            if (q != '"' && q != '\'') fail("expected a quoted value");
+            size_t e = s_.find(q, i_ + 1);
+            if (e == std::string::npos) fail("unterminated attribute");
+            n.attr[k] = unescape(s_.substr(i_ + 1, e - i_ - 1));
+            i_ = e + 1;
+        }
+        for (;;) {
+            skipMisc();
+            if (startsWith("</")) {
+                i_ += 2;
+                std::string c = name();
+                if (c != n.tag) fail("mismatched </" + c + ">");
+                skipWs();
+                if (i_ >= s_.size() || s_[i_] != '>') fail("expected '>'");
+                ++i_;
+                return n;
+            }
+            if (i_ < s_.size() && s_[i_] == '<') {
+                n.kids.push_back(parseElement());
+            } else {
+                size_t e = s_.find('<', i_);  // text content is ignored by the schema
+                if (e == std::string::npos) fail("unterminated element");
+                i_ = e;
+            }
+        }
+    }
+};
+
+// parseVector3 (parser.cpp:15-26): "x, y, z" or "x y z"
+Vec3 parseVec3(const std::string& v) {
+    std::stringstream ss(v);
+    float x, y, z;
+    char c1, c2;
+    if ((ss >> x >> c1 >> y >> c2 >> z) && c1 == ',' && c2 == ',') return Vec3(x, y, z);
+    std::stringstream s2(v);
+    if (!(s2 >> x >> y >> z)) throw std::runtime_error("Could not parse vector from string: " + v);
+    return Vec3(x, y, z);
+}
+float asFloat(const std::string& v, float d = 0.0f) { return v.empty() ? d : std::stof(v); }
+
+struct Props {
+    std::map<std::string, float> f;
+    std::map<std::string, int> i;
+    std::map<std::string, bool> b;
+    std::map<std::string, std::string> s;
+    std::map<std::string, Vec3> v;
+    std::map<std::string, Xform> t;
+    float getF(const char* k, float d) const { auto it = f.find(k); return it == f.end() ? d : it->second; }
+    int getI(const char* k, int d) const { auto it = i.find(k); return it == i.end() ? d : it->second; }
+    bool getB(const char* k, bool d) const { auto it = b.find(k); return it == b.end() ? d : it->second; }
+    bool hasV(const char* k) const { return v.count(k) > 0; }
+    Vec3 getV(const char* k, Vec3 d) const { auto it = v.find(k); return it == v.end() ? d : it->second; }
+    std::string getS(const char* k, const std::string& d) const { auto it = s.find(k); return it == s.end() ? d : it->second; }
+};
+
+Props readProps(const XNode& n, std::vector<const XNode*>& children) {
+    Props p;
+    for (const XNode& c : n.kids) {
+        const std::string nm = c.attrOr("name");
+        if (c.tag == "boolean") p.b[nm] = c.attrOr("value") == "true" || c.attrOr("value") == "1";
+        else if (c.tag == "integer") p.i[nm] = std::stoi(c.attrOr("value", "0"));
+        else if (c.tag == "float") p.f[nm] = asFloat(c.attrOr("value"));
+        else if (c.tag == "string") p.s[nm] = c.attrOr("value");
+        else if (c.tag == "point" || c.tag == "vector" || c.tag == "vector3" || c.tag == "color") {
+            if (c.attr.count("value")) p.v[nm] = parseVec3(c.attrOr("value"));
+            else if (c.tag == "color")
+                p.v[nm] = Vec3(asFloat(c.attrOr("r")), asFloat(c.attrOr("g")), asFloat(c.attrOr("b")));
+            else p.v[nm] = Vec3(asFloat(c.attrOr("x")), asFloat(c.attrOr("y")), asFloat(c.attrOr("z")));
+        } else if (c.tag == "vector2") {
+            std::string val = c.attrOr("value");
+            p.v[nm] = Vec3(asFloat(c.attrOr("x")), asFloat(c.attrOr("y")), 0.0f);
+            if (!val.empty()) {
+                std::stringstream ss(val);
+                float x, y;
+                char cm;
+                if (!((ss >> x >> cm >> y) && cm == ',')) {
+                    std::stringstream s2(val);
+                    s2 >> x >> y;
+                }
+                p.v[nm] = Vec3(x, y, 0.0f);
+            }
+        } else if (c.tag == "transform") {
+            Xform t;
+            for (const XNode& st : c.kids) {
+                Xform step;
+                if (st.tag == "translate")
+                    step = xformTranslate(asFloat(st.attrOr("x")), asFloat(st.attrOr("y")), asFloat(st.attrOr("z")));
+                else if (st.tag == "scale") {
+                    float v = asFloat(st.attrOr("value"));
+                    if (v != 0.0f) step = xformScale(v, v, v);
+                    else step = xformScale(asFloat(st.attrOr("x"), 1.f), asFloat(st.attrOr("y"), 1.f),
+                                           asFloat(st.attrOr("z"), 1.f));
+                } else if (st.tag == "rotate") {
+                    float a = asFloat(st.attrOr("angle"));
+                    Vec3 ax = st.attr.count("axis") ? parseVec3(st.attrOr("axis"))
+                                                    : Vec3(asFloat(st.attrOr("x")), asFloat(st.attrOr("y")),
+                                                           asFloat(st.attrOr("z")));
+                    step = xformRotate(a, ax.x, ax.y, ax.z);
+                } else if (st.tag == "lookat") {
+                    step = xformLookAt(parseVec3(st.attrOr("origin")), parseVec3(st.attrOr("target")),
+                                       parseVec3(st.attrOr("up")));
+                }
+                t = t.then(step);
+            }
+            p.t[nm] = t;
+        } else {
+            children.push_back(&c);
+        }
+    }
+    return p;
+}
+
+// Minimal OBJ reader (v / vn / vt / f with polygon fans), de-indexed like
+// Mesh::loadOBJ (mesh.cpp:148-229)
+MeshData loadObj(const std::string& path) {
+    std::ifstream in(path);
+    if (!in) throw std::runtime_error("Failed to load OBJ: " + path);
+    std::vector<Vec3> v, vn;
+    std::vector<std::pair<float, float>> vt;
+    struct Idx { int v, t, n; };
+    std::vector<Idx> corners;
+    std::string line;
+    auto fix = [](int k, size_t n) { return k < 0 ? (int)n + k : k - 1; };
+    while (std::getline(in, line)) {
+        std::stringstream ss(line);
+        std::string tag;
+        ss >> tag;
+        if (tag == "v") { float x, y, z; ss >> x >> y >> z; v.emplace_back(x, y, z); }
+        else if (tag == "vn") { float x, y, z; ss >> x >> y >> z; vn.emplace_back(x, y, z); }
+        else if (tag == "vt") { float x, y; ss >> x >> y; vt.emplace_back(x, y); }
+        else if (tag == "f") {
+            std::vector<Idx> poly;
+            std::string tok;
+            while (ss >> tok) {
+                Idx id{-1, -1, -1};
+                int slot = 0;
+                std::string cur;
+                for (size_t k = 0; k <= tok.size(); ++k) {
+                    if (k == tok.size() || tok[k] == '/') {
+                        if (!cur.empty()) {
+                            int val = std::stoi(cur);
+                            if (slot == 0) id.v = fix(val, v.size());
+                            else if (slot == 1) id.t = fix(val, vt.size());
+                            else id.n = fix(val, vn.size());
+                        }
+                        cur.clear();
+                        ++slot;
+                    } else cur += tok[k];
+                }
+                poly.push_back(id);
+            }
+            for (size_t k = 1; k + 1 < poly.size(); ++k) {
+                corners.push_back(poly[0]);
+                corners.push_back(poly[k]);
+                corners.push_back(poly[k + 1]);
+            }
+        }
+    }
+    if (corners.empty()) throw std::runtime_error("Mesh: no geometry loaded!");
+    MeshData m;
+    bool hasN = !vn.empty(), hasT = !vt.empty();
+    for (const Idx& c : corners) {
+        const Vec3& p = v.at(c.v);
+        m.positions.insert(m.positions.end(), {p.x, p.y, p.z});
+        m.vertexIds.push_back((uint32_t)c.v);
+        if (hasN) {
+            Vec3 n = c.n >= 0 ? vn.at(c.n) : Vec3(0, 0, 0);
+            m.normals.insert(m.normals.end(), {n.x, n.y, n.z});
+        }
+        if (hasT) {
+            auto t = c.t >= 0 ? vt.at(c.t) : std::make_pair(0.0f, 0.0f);
+            m.uvs.insert(m.uvs.end(), {t.first, t.second});
+        }
+    }
+    return m;
+}
+
+struct Loader {
+    HostScene& sc;
+    std::string baseDir;
+    std::map<std::string, int> meshIds;  // <shape id=...> sharing
+    std::map<std::string, float> sphereRadius;
+
+    int readMedium(const XNode& n) {
+        std::vector<const XNode*> kids;
+        Props p = readProps(n, kids);
+        MediumData md;
+        std::string type = n.attrOr("type");
+        if (type != "homogeneous") throw std::runtime_error("unsupported medium type: " + type);
+        md.desc.type = GND_MEDIUM_HOMOGENEOUS;
+        Vec3 sa = p.getV("sigma_a", Vec3(0.1f, 0.1f, 0.1f)), ss = p.getV("sigma_s", Vec3(0.1f, 0.1f, 0.1f));
+        for (int c = 0; c < 3; ++c) { md.desc.sigma_a[c] = sa[c]; md.desc.sigma_s[c] = ss[c]; }
+        md.desc.g = p.getF("g", 0.0f);
+        Mat4 I = Mat4::identity();
+        std::memcpy(md.desc.to_local, I.m, sizeof(I.m));
+        std::memcpy(md.desc.to_local_inv, I.m, sizeof(I.m));
+        sc.media.push_back(std::move(md));
+        return (int)sc.media.size() - 1;
+    }
+
+    int readMaterial(const XNode& n) {
+        std::vector<const XNode*> kids;
+        Props p = readProps(n, kids);
+        std::string type = n.attrOr("type");
+        int inside = -1, outside = -1;
+        for (const XNode* k : kids) {
+            if (k->tag == "medium") {
+                int id = readMedium(*k);
+                if (k->attrOr("name") == "outside") outside = id;
+                else inside = id;
+            }
+        }
+        gnd_material_desc m{};
+        if (type == "matte") {
+            m = materialMatte(0, 0, 0);
+            if (p.hasV("albedo")) { Vec3 a = p.getV("albedo", Vec3()); m.albedo[0] = a.x; m.albedo[1] = a.y; m.albedo[2] = a.z; }
+            else m.albedo_is_checker = 1;
+        } else if (type == "conductor") {
+            float R[3], E[3], K[3];
+            Vec3 r = p.getV("R", Vec3(1, 1, 1)), e = p.getV("eta", Vec3(1, 1, 1)), k = p.getV("k", Vec3(1, 1, 1));
+            for (int c = 0; c < 3; ++c) { R[c] = r[c]; E[c] = e[c]; K[c] = k[c]; }
+            m = materialConductor(R, E, K, p.getF("roughness", 0.0f));
+        } else if (type == "glass") {
+            m = materialGlass(p.getF("eta", 1.5f), p.getF("roughness", 0.0f), inside, outside);
+            Vec3 r = p.getV("R", Vec3(1, 1, 1)), t = p.getV("T", Vec3(1, 1, 1));
+            for (int c = 0; c < 3; ++c) { m.R[c] = r[c]; m.T[c] = t[c]; }
+        } else if (type == "plastic") {
+            float Kd[3], Ks[3];
+            Vec3 kd = p.getV("Kd", Vec3(0.25f, 0.25f, 0.25f)), ks = p.getV("Ks", Vec3(0.25f, 0.25f, 0.25f));
+            for (int c = 0; c < 3; ++c) { Kd[c] = kd[c]; Ks[c] = ks[c]; }
+            m = materialPlastic(Kd, Ks, p.getF("eta", 1.5f), p.getF("roughness", 0.1f));
+        } else if (type == "index") {
+            m = materialIndex(inside, outside);
+        } else {
+            throw std::runtime_error("unsupported material type: " + type);
+        }
+        return sc.addMaterial(m);
+    }
+
+    void readPrimitive(const XNode& n) {
+        std::vector<const XNode*> kids;
+        Props p = readProps(n, kids);
+        Xform x = p.t.count("toWorld") ? p.t.at("toWorld") : Xform();
+        if (p.t.count("toWorldEnd")) throw std::runtime_error("animated transforms (toWorldEnd) are not supported");
+        int mesh = -1, material = -1;
+        bool sphere = false, emitter = false;
+        float radius = 1.0f, Le[3] = {0, 0, 0};
+        for (const XNode* k : kids) {
+            if (k->tag == "shape" || k->tag == "ref") {
+                std::string id = k->attrOr("id");
+                if (k->tag == "ref") {
+                    if (meshIds.count(id)) mesh = meshIds[id];
+                    else if (sphereRadius.count(id)) { sphere = true; radius = sphereRadius[id]; }
+                    else throw std::runtime_error("Parsing error: no object referencing " + id + " found.");
+                    continue;
+                }
+                std::vector<const XNode*> sk;
+                Props sp = readProps(*k, sk);
+                std::string type = k->attrOr("type");
+                if (type == "mesh") {
+                    std::string f = sp.getS("filename", "");
+                    if (f.empty()) throw std::runtime_error("Mesh: filename missing!");
+                    std::string path = (!f.empty() && f[0] == '/') ? f : baseDir + f;
+                    mesh = sc.addMesh(loadObj(path));
+                    if (!id.empty()) meshIds[id] = mesh;
+                } else if (type == "sphere") {
+                    sphere = true;
+                    radius = sp.getF("radius", 1.0f);
+                    if (!id.empty()) sphereRadius[id] = radius;
+                } else {
+                    throw std::runtime_error("unsupported shape type: " + type);
+                }
+            } else if (k->tag == "material") {
+                material = readMaterial(*k);
+            } else if (k->tag == "emitter") {
+                std::vector<const XNode*> ek;
+                Props ep = readProps(*k, ek);
+                if (k->attrOr("type") != "area") throw std::runtime_error("only area emitters are supported");
+                emitter = true;
+                Vec3 r = ep.getV("radiance", Vec3(0, 0, 0));
+                Le[0] = r.x; Le[1] = r.y; Le[2] = r.z;
+            }
+        }
+        if (mesh < 0 && !sphere) throw std::runtime_error("Primitive: defined without a Shape!");
+        if (material < 0) {
+            gnd_material_desc m = materialMatte(0, 0, 0);
+            m.albedo_is_checker = 1;  // missing texture matte fallback (primitive.cpp:39-49)
+            material = sc.addMaterial(m);
+        }
+        if (sphere) sc.addSphere(radius, x, material, emitter ? Le : nullptr);
+        else sc.addPrim(mesh, x, material, emitter ? Le : nullptr);
+    }
+
+    void readCamera(const XNode& n) {
+        std::vector<const XNode*> kids;
+        Props p = readProps(n, kids);
+        std::string type = n.attrOr("type");
+        Xform x = p.t.count("toWorld") ? p.t.at("toWorld") : Xform();
+        sc.setCamera(type == "thinlens" ? GND_CAMERA_THINLENS : GND_CAMERA_PERSPECTIVE, p.getI("width", 1280),
+                     p.getI("height", 720), p.getF("fov", 45.0f), x);
+        gnd_camera_desc& c = sc.camera;
+        c.lens_radius = p.getF("lens_radius", 0.0f);
+        c.focal_distance = p.getF("focal_distance", 10.0f);
+        c.k1 = p.getF("k1", 0.0f);
+        c.ca_axial = p.getF("ca_axial", 0.0f);
+        c.ca_lateral = p.getF("ca_lateral", 0.0f);
+        c.vignetting = p.getB("vignetting", false) ? 1 : 0;
+        c.blades = p.getI("blades", 0);
+        if (p.getB("rolling_shutter", false)) throw std::runtime_error("rolling shutter is not supported");
+        for (const XNode* k : kids) {
+            if (k->tag == "filter") {
+                std::vector<const XNode*> fk;
+                Props fp = readProps(*k, fk);
+                if (k->attrOr("type") == "box") c.filter = GND_FILTER_BOX;
+                else {
+                    c.filter = GND_FILTER_GAUSSIAN;
+                    Vec3 r = fp.getV("radius", Vec3(2.0f, 2.0f, 0.0f));
+                    c.filter_radius[0] = r.x;
+                    c.filter_radius[1] = r.y;
+                    c.filter_alpha = fp.getF("alpha", 2.0f);
+                }
+            }
+        }
+    }
+
+    void readScene(const XNode& root) {
+        if (root.tag != "scene") throw std::runtime_error("root element must be <scene>");
+        bool haveCam = false, haveInt = false, haveSampler = false;
+        std::vector<const XNode*> kids;
+        readProps(root, kids);
+        for (const XNode* k : kids) {
+            if (k->tag == "camera") {
+                if (haveCam) throw std::runtime_error("Scene: cannot add multiple cameras!");
+                haveCam = true;
+                readCamera(*k);
+            } else if (k->tag == "sampler") {
+                if (haveSampler) throw std::runtime_error("Scene: cannot add multiple samplers!");
+                haveSampler = true;
+                std::vector<const XNode*> sk;
+                sc.spp = (uint32_t)readProps(*k, sk).getI("spp", 1);
+            } else if (k->tag == "integrator") {
+                if (haveInt) throw std::runtime_error("Scene: cannot add multiple integrators!");
+                haveInt = true;
+                std::vector<const XNode*> ik;
+                Props ip = readProps(*k, ik);
+                std::string t = k->attrOr("type");
+                if (t == "path" || t == "path_hip") sc.integrator = GND_INTEGRATOR_PATH;
+                else if (t == "volpath" || t == "volpath_hip") sc.integrator = GND_INTEGRATOR_VOLPATH;
+                else throw std::runtime_error("unsupported integrator: " + t);
+                sc.maxDepth = ip.getI("maxDepth", 1000);
+            } else if (k->tag == "primitive") {
+                readPrimitive(*k);
+            } else {
+                throw std::runtime_error("Scene: cannot add specified child!");
+            }
+        }
+        if (!haveCam) throw std::runtime_error("Scene: no camera specified!");
+        if (!haveInt) throw std::runtime_error("Scene: no integrator specified!");
+        if (!haveSampler) throw std::runtime_error("Scene: no sampler specified!");
+    }
+};
+
+}  // namespace
+
+std::unique_ptr<HostScene> loadXmlScene(const std::string& path) {
+    std::ifstream in(path);
+    if (!in) throw std::runtime_error("XML parsing failed: cannot open " + path);
+    std::stringstream ss;
+    ss << in.rdbuf();
+    std::string text = ss.str();
+    XmlReader r(text);
+    XNode root = r.parseDocument();
+    auto sc = std::make_unique<HostScene>();
+    Loader L{*sc};
+    size_t slash = path.find_last_of('/');
+    L.baseDir = slash == std::string::npos ? "" : path.substr(0, slash + 1);
+    L.readScene(root);
+    return sc;
+}
+
+}  // namespace gnd

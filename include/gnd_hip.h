@@ -1,0 +1,184 @@
+/*
+ * gnd_hip.h — C-ABI of the MI355X ray-tracing library (libgnd_hip.so).
+ *
+ * The reference renderer (daverlez/gianduia) has no process or device
+ * boundary: SamplerIntegrator::render (include/gianduia/core/integrator.h:49-155)
+ * calls Scene::rayIntersect (src/scene/scene.cpp:64-71) and the virtual
+ * Li() of the path / volpath integrators (src/integrators/path.cpp:31-178,
+ * src/integrators/volPath.cpp:37-249) on CPU threads.  This header is the one
+ * boundary the GPU build adds: the host shell keeps the scene graph, the XML
+ * loader and the BVH4 build, flattens them into the arrays below and drives the
+ * device through these entry points.  Plain pointers and sizes only; every
+ * function returns 0 on success or a negative GND_E_* status, with text from
+ * gnd_hip_last_error().  No C++ exception crosses this boundary.
+ *
+ * Thread-safety: calls on one context are serialized internally; separate
+ * contexts are independent.
+ */
+#ifndef GND_HIP_H
+#define GND_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "gnd_scene.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    GND_OK = 0,
+    GND_E_INVALID = -1,   /* bad argument / malformed scene (reference: std::runtime_error) */
+    GND_E_DEVICE = -2,    /* HIP runtime error                                  */
+    GND_E_NOMEM = -3,     /* device allocation failed / over the HBM budget     */
+    GND_E_CANCELLED = -4, /* render stopped by gnd_hip_cancel at a pass boundary */
+    GND_E_STACK = -5      /* traversal stack overflow detected on the device   */
+};
+
+typedef struct gnd_hip_ctx gnd_hip_ctx;
+typedef struct gnd_hip_scene gnd_hip_scene;
+
+/* BVHNode4, reference layout (include/gianduia/core/bvhBuilder.h:21-28): 144 B.
+ * Offsets are relative to the owning mesh (BLAS) or to tlas_prims (TLAS). */
+typedef struct gnd_bvh4_node {
+    float min_x[4], min_y[4], min_z[4];
+    float max_x[4], max_y[4], max_z[4];
+    uint32_t offset[4];
+    uint32_t count[4];
+    uint8_t child_type[4]; /* 0 empty, 1 inner, 2 leaf */
+    uint8_t pad[12];
+} gnd_bvh4_node;
+
+/* Mesh::TrianglePack4 (include/gianduia/shapes/mesh.h:37-43): 160 B. */
+typedef struct gnd_tri_pack4 {
+    float v0x[4], v0y[4], v0z[4];
+    float v1x[4], v1y[4], v1z[4];
+    float v2x[4], v2y[4], v2z[4];
+    uint32_t prim[4]; /* triangle index in the mesh, 0xFFFFFFFF for padding */
+} gnd_tri_pack4;
+
+typedef struct gnd_mesh_info {
+    uint32_t node_base, n_nodes; /* slice of blas_nodes                        */
+    uint32_t pack_base, n_packs; /* slice of packs                             */
+    uint32_t tri_base, n_tris;   /* slice of shading (and cdf at tri_base+mesh) */
+    uint32_t cdf_base;           /* slice of cdf, n_tris + 1 entries            */
+    uint32_t has_uv;
+    float total_area;            /* Mesh::m_totalArea (mesh.cpp:121-145)        */
+    float bounds_min[3], bounds_max[3];
+} gnd_mesh_info;
+
+/* Host-built, flattened scene.  All arrays are borrowed for the duration of
+ * gnd_hip_scene_upload and copied to HBM. */
+typedef struct gnd_hip_scene_arrays {
+    const gnd_scene_desc* desc;      /* primitives, materials, media, camera   */
+    uint32_t n_tlas_nodes;
+    const gnd_bvh4_node* tlas_nodes; /* BVH::build + buildWide (src/scene/bvh.cpp:10-36) */
+    uint32_t n_tlas_prims;
+    const int32_t* tlas_prims;       /* TLAS leaf slot -> primitive (scene order)  */
+    uint32_t n_meshes;
+    const gnd_mesh_info* meshes;
+    uint32_t n_blas_nodes;
+    const gnd_bvh4_node* blas_nodes; /* all meshes, each relative to its mesh  */
+    uint32_t n_packs;
+    const gnd_tri_pack4* packs;
+    uint32_t n_tris;
+    const float* shading;            /* 24 floats per triangle: p0 p1 p2 n0 n1 n2 uv0 uv1 uv2 */
+    uint32_t n_cdf;
+    const float* cdf;                /* per-mesh triangle-area CDFs            */
+} gnd_hip_scene_arrays;
+
+/* One render job: a camera over a resident scene, rendered progressively
+ * pass by pass like SamplerIntegrator::render.  film points to device memory
+ * of width*height*4 floats (weighted r,g,b sums and weight sum, the Film
+ * accumulator of include/gianduia/core/film.h:110-117); the caller resolves
+ * rgb = sum / weight.  film_sq (optional) receives per-pixel sums of squared
+ * per-pass values for variance estimates. */
+typedef struct gnd_hip_job {
+    gnd_camera_desc camera;
+    uint32_t spp;
+    int32_t max_depth;
+    int32_t integrator;     /* GND_INTEGRATOR_PATH / _VOLPATH                 */
+    uint64_t seed_offset;   /* added to pixelIdx + pass*W*H before SplitMix64 */
+    float* film;            /* device pointer                                 */
+    float* film_sq;         /* device pointer or NULL                         */
+} gnd_hip_job;
+
+typedef struct gnd_hip_stats {
+    double ms_total;        /* wall time of the call on the device stream     */
+    double ms_trace;        /* sum of closest-hit traversal kernel times      */
+    double ms_shadow;       /* sum of shadow traversal kernel times           */
+    double ms_shade;        /* sum of shading / camera / film kernel times    */
+    uint64_t samples;       /* camera samples (W*H*spp summed over jobs)      */
+    uint64_t rays_closest;  /* closest-hit queries                            */
+    uint64_t rays_shadow;   /* shadow / transmittance queries                 */
+    uint64_t node_visits;   /* BVH4 node pops (filled when counters are on)   */
+    uint64_t pack_tests;    /* TrianglePack4 tests (counters on)              */
+    uint64_t inst_tests;    /* instance (primitive) tests (counters on)       */
+    uint64_t trace_launches;
+    uint64_t nan_samples;   /* samples zeroed by the NaN/Inf guard            */
+} gnd_hip_stats;
+
+/* Tunables picked by gnd_hip_autotune or set by the caller. */
+typedef struct gnd_hip_tuning {
+    int32_t block_size;       /* threads per block of the traversal kernels (64..256) */
+    int32_t stack_lds;        /* per-lane traversal stack entries held in LDS (8..32) */
+    int64_t queue_capacity;   /* paths in flight per wave                            */
+    int32_t use_graph;        /* capture the bounce loop in a hipGraph                */
+} gnd_hip_tuning;
+
+/* ---- context ------------------------------------------------------------ */
+int gnd_hip_init(int device, gnd_hip_ctx** out);
+void gnd_hip_shutdown(gnd_hip_ctx* ctx);
+const char* gnd_hip_last_error(const gnd_hip_ctx* ctx);
+int gnd_hip_device_info(gnd_hip_ctx* ctx, char* name, int name_len, int* num_cus, size_t* hbm_bytes);
+int gnd_hip_set_tuning(gnd_hip_ctx* ctx, const gnd_hip_tuning* t);
+int gnd_hip_get_tuning(gnd_hip_ctx* ctx, gnd_hip_tuning* t);
+/* Traversal counters (node pops, pack tests, instance tests) on/off. */
+int gnd_hip_set_counters(gnd_hip_ctx* ctx, int enabled);
+
+/* ---- device memory helpers (for bindings without their own allocator) ---- */
+int gnd_hip_malloc(gnd_hip_ctx* ctx, size_t bytes, void** dptr);
+int gnd_hip_free(gnd_hip_ctx* ctx, void* dptr);
+int gnd_hip_memcpy_h2d(gnd_hip_ctx* ctx, void* dst, const void* src, size_t bytes);
+int gnd_hip_memcpy_d2h(gnd_hip_ctx* ctx, void* dst, const void* src, size_t bytes);
+int gnd_hip_memset(gnd_hip_ctx* ctx, void* dst, int value, size_t bytes);
+int gnd_hip_synchronize(gnd_hip_ctx* ctx);
+
+/* ---- scene ---------------------------------------------------------------- */
+/* Scene::activate's device half (src/scene/scene.cpp:46-62): uploads the
+ * host-built TLAS/BLAS/packs and material/light/medium tables to HBM. */
+int gnd_hip_scene_upload(gnd_hip_ctx* ctx, const gnd_hip_scene_arrays* arrays, gnd_hip_scene** out);
+void gnd_hip_scene_free(gnd_hip_ctx* ctx, gnd_hip_scene* scene);
+size_t gnd_hip_scene_bytes(const gnd_hip_scene* scene);
+
+/* ---- traversal (verify-mode entry points, bit-exact with the reference) ---- */
+/* Scene::rayIntersect(ray, isect) closest hit (src/scene/scene.cpp:64-66 ->
+ * src/scene/bvh.cpp:38-66).  rays/hits hold DEVICE pointers; n rays.  On a
+ * miss hit.t = +inf and hit.inst = hit.prim = -1. */
+int gnd_hip_trace_closest(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_ray_soa* rays, int64_t n,
+                          const gnd_hit_soa* hits, gnd_hip_stats* stats);
+/* Scene::rayIntersect(ray) any hit (src/scene/scene.cpp:68-71):
+ * occluded[i] = 1 if any surface lies in (tmin, tmax), else 0. */
+int gnd_hip_trace_any(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_ray_soa* rays, int64_t n,
+                      int32_t* occluded, gnd_hip_stats* stats);
+
+/* ---- rendering ------------------------------------------------------------ */
+/* SamplerIntegrator::render for one or more jobs over one resident scene.
+ * Jobs are rendered in one wavefront stream: paths of every job share the
+ * same bounce launches.  Films are cleared first (Film::clear). */
+int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jobs, int njobs,
+                   gnd_hip_stats* stats);
+/* Integrator::cancel (include/gianduia/core/integrator.h:22): honoured at the
+ * next wave boundary of a running gnd_hip_render on this context. */
+int gnd_hip_cancel(gnd_hip_ctx* ctx);
+
+/* Picks block size, LDS stack depth and queue capacity by timing the
+ * traversal and bounce kernels on the given scene and job. */
+int gnd_hip_autotune(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* job, gnd_hip_tuning* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

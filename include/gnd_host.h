@@ -1,0 +1,59 @@
+/*
+ * gnd_host.h — C API of the host scene shell in libgnd_hip.so.
+ *
+ * The host shell keeps what the reference keeps on the CPU: the scene graph
+ * parsed from the reference's XML schema (src/core/parser.cpp:41-235), the
+ * meshes with their BLAS build (src/shapes/mesh.cpp:17-146) and the TLAS build
+ * (src/scene/bvh.cpp:10-36).  It flattens a scene into gnd_hip_scene_arrays
+ * for gnd_hip_scene_upload, and offers the procedural scenes of the benchmark
+ * configurations.  A render through gnd_host_render behaves like
+ * SamplerIntegrator::render (include/gianduia/core/integrator.h:49-155) with
+ * the film resolved on the host.
+ */
+#ifndef GND_HOST_H
+#define GND_HOST_H
+
+#include "gnd_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gnd_host_scene gnd_host_scene;
+
+const char* gnd_host_last_error(void);
+
+/* Copies a caller-built description (all arrays are copied). */
+int gnd_host_scene_create(const gnd_scene_desc* desc, gnd_host_scene** out);
+/* Reference XML schema: scene/camera/sampler/integrator/primitive/shape(mesh
+ * with an OBJ file, sphere)/material/emitter(area)/medium/filter. */
+int gnd_host_scene_load_xml(const char* path, gnd_host_scene** out);
+/* Procedural scenes: "cbox" (512x512 Cornell box), "scaled" (1M-triangle
+ * displaced icosphere + 64 instances, 1920x1080 thin lens), "volumetric"
+ * (Cornell box + glass sphere with a homogeneous medium + 256^3 density grid),
+ * "procedural" (the scaled generator at a target triangle count).  params is
+ * "key=value;key=value" (seed, tris, width, height, spp, depth, grid). */
+int gnd_host_scene_generate(const char* name, const char* params, gnd_host_scene** out);
+void gnd_host_scene_free(gnd_host_scene* s);
+
+/* Description of the scene (valid until the scene is freed). */
+const gnd_scene_desc* gnd_host_scene_desc(gnd_host_scene* s);
+/* Builds (once) and returns the flattened arrays (valid until freed).
+ * threads <= 0 uses every hardware thread.  build_ms receives the time. */
+int gnd_host_scene_build(gnd_host_scene* s, int threads, const gnd_hip_scene_arrays** out, double* build_ms);
+
+/* Camera overrides used by batch jobs. */
+int gnd_host_camera_lookat(gnd_camera_desc* cam, const float* origin, const float* target, const float* up);
+
+/* Integrator::render for the scene's own camera/sampler/integrator on a
+ * context: uploads (once per context), renders every pass and writes the
+ * resolved RGB image (width*height*3 floats, Film::resolve) to rgb.  spp <= 0
+ * keeps the scene's sample count. */
+int gnd_host_render(gnd_hip_ctx* ctx, gnd_host_scene* s, int spp, uint64_t seed_offset, float* rgb,
+                    gnd_hip_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,156 @@
+"""Scene descriptions built in the tests (random triangle soups, instancing)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from gianduia_amd import _abi as A
+
+IDENT = np.eye(4, dtype=np.float32)
+
+
+def col_major(m: np.ndarray):
+    """4x4 row-major numpy matrix -> column-major float[16] (glm memory order)."""
+    return (C.c_float * 16)(*np.asarray(m, dtype=np.float32).T.reshape(-1).tolist())
+
+
+def trs(scale=(1, 1, 1), rot_deg=0.0, axis=(0, 1, 0), trans=(0, 0, 0)) -> tuple[np.ndarray, np.ndarray]:
+    """Returns (M, Minv) = T*R*S and S^-1*R^T*T^-1 in float32."""
+    ax = np.asarray(axis, np.float64)
+    ax = ax / np.linalg.norm(ax)
+    a = np.radians(rot_deg)
+    c, s = np.cos(a), np.sin(a)
+    x, y, z = ax
+    R = np.array([[c + x * x * (1 - c), x * y * (1 - c) - z * s, x * z * (1 - c) + y * s, 0],
+                  [y * x * (1 - c) + z * s, c + y * y * (1 - c), y * z * (1 - c) - x * s, 0],
+                  [z * x * (1 - c) - y * s, z * y * (1 - c) + x * s, c + z * z * (1 - c), 0],
+                  [0, 0, 0, 1]])
+    S = np.diag([scale[0], scale[1], scale[2], 1.0])
+    T = np.eye(4)
+    T[:3, 3] = trans
+    M = T @ R @ S
+    return M.astype(np.float32), np.linalg.inv(M).astype(np.float32)
+
+
+class SceneBuilder:
+    def __init__(self):
+        self.meshes = []      # (positions (n,9) float32, normals or None)
+        self.prims = []
+        self.materials = []
+        self.keep = []
+
+    def mesh(self, tris: np.ndarray, normals: np.ndarray | None = None) -> int:
+        self.meshes.append((np.ascontiguousarray(tris, np.float32).reshape(-1, 9),
+                            None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1, 9)))
+        return len(self.meshes) - 1
+
+    def matte(self, rgb=(0.5, 0.5, 0.5)) -> int:
+        m = A.gnd_material_desc()
+        m.type = A.GND_MAT_MATTE
+        m.albedo[:] = rgb
+        m.medium_inside = m.medium_outside = -1
+        self.materials.append(m)
+        return len(self.materials) - 1
+
+    def prim(self, mesh: int, M=None, Minv=None, material: int = 0, radiance=None, sphere_radius=None) -> int:
+        p = A.gnd_primitive_desc()
+        if sphere_radius is not None:
+            p.shape_type = A.GND_SHAPE_SPHERE
+            p.mesh = -1
+            p.radius = sphere_radius
+        else:
+            p.shape_type = A.GND_SHAPE_MESH
+            p.mesh = mesh
+            p.radius = 1.0
+        p.material = material
+        if radiance is not None:
+            p.has_emitter = 1
+            p.radiance[:] = radiance
+        M = IDENT if M is None else M
+        Minv = IDENT if Minv is None else Minv
+        p.to_world = col_major(M)
+        p.to_world_inv = col_major(Minv)
+        self.prims.append(p)
+        return len(self.prims) - 1
+
+    def desc(self, width=64, height=64) -> A.gnd_scene_desc:
+        md = (A.gnd_mesh_desc * max(1, len(self.meshes)))()
+        for i, (pos, nrm) in enumerate(self.meshes):
+            self.keep += [pos, nrm]
+            md[i].ntris = pos.shape[0]
+            md[i].positions = pos.ctypes.data_as(C.POINTER(C.c_float))
+            md[i].normals = nrm.ctypes.data_as(C.POINTER(C.c_float)) if nrm is not None else None
+            md[i].uvs = None
+            md[i].vertex_ids = None
+        pd = (A.gnd_primitive_desc * len(self.prims))(*self.prims)
+        mt = (A.gnd_material_desc * max(1, len(self.materials)))(*self.materials)
+        self.keep += [md, pd, mt]
+        d = A.gnd_scene_desc()
+        d.nmeshes = len(self.meshes)
+        d.meshes = md
+        d.nprims = len(self.prims)
+        d.prims = pd
+        d.nmaterials = len(self.materials)
+        d.materials = mt
+        d.nmedia = 0
+        d.media = None
+        cam = A.gnd_camera_desc()
+        cam.type = A.GND_CAMERA_PERSPECTIVE
+        cam.width, cam.height, cam.fov = width, height, 45.0
+        cam.to_world = col_major(IDENT)
+        cam.to_world_inv = col_major(IDENT)
+        cam.filter = A.GND_FILTER_BOX
+        d.camera = cam
+        d.integrator = A.GND_INTEGRATOR_PATH
+        d.max_depth = 4
+        d.spp = 1
+        self.keep.append(d)
+        return d
+
+
+def random_soup(rng: np.random.Generator, n: int, spread=1.0, size=0.2) -> np.ndarray:
+    c = rng.uniform(-spread, spread, size=(n, 1, 3))
+    v = c + rng.uniform(-size, size, size=(n, 3, 3))
+    return v.reshape(n, 9).astype(np.float32)
+
+
+def soup_scene(seed=0, n=2000, instances=1, sphere=False, light=True) -> SceneBuilder:
+    rng = np.random.default_rng(seed)
+    b = SceneBuilder()
+    mat = b.matte()
+    m = b.mesh(random_soup(rng, n))
+    for k in range(instances):
+        if k == 0:
+            b.prim(m, material=mat)
+        else:
+            M, Mi = trs(scale=rng.uniform(0.3, 1.0, 3), rot_deg=rng.uniform(0, 360), axis=rng.normal(size=3),
+                        trans=rng.uniform(-2, 2, 3))
+            b.prim(m, M, Mi, mat)
+    if sphere:
+        M, Mi = trs(trans=(0.3, -0.2, 0.1))
+        b.prim(-1, M, Mi, mat, sphere_radius=0.5)
+    if light:
+        lm = b.mesh(np.array([[-0.5, 3, -0.5, 0.5, 3, -0.5, 0.5, 3, 0.5]], np.float32))
+        b.prim(lm, material=mat, radiance=(5, 5, 5))
+    return b
+
+
+def random_rays(rng: np.random.Generator, n: int, spread=2.5, edge_cases=True) -> dict:
+    o = rng.uniform(-spread, spread, size=(n, 3)).astype(np.float32)
+    tgt = rng.uniform(-1, 1, size=(n, 3)).astype(np.float32)
+    d = tgt - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    d = d.astype(np.float32)
+    if edge_cases and n >= 64:
+        # axis-aligned directions (inf inverse components) and zero components
+        d[0:8] = np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1],
+                           [0.6, 0.8, 0], [0, 0.6, -0.8]], np.float32)
+        o[0:8] = 0.0
+    tmin = np.full(n, 1e-4, np.float32)
+    tmax = np.full(n, np.inf, np.float32)
+    if edge_cases and n >= 64:
+        tmax[8:16] = rng.uniform(0.1, 3.0, 8).astype(np.float32)   # short rays
+        tmax[16:20] = 0.0                                          # empty interval
+    return {"ox": o[:, 0].copy(), "oy": o[:, 1].copy(), "oz": o[:, 2].copy(), "dx": d[:, 0].copy(),
+            "dy": d[:, 1].copy(), "dz": d[:, 2].copy(), "tmin": tmin, "tmax": tmax}
