@@ -111,7 +111,9 @@ class gnd_hip_stats(C.Structure):
     _fields_ = [("ms_total", C.c_double), ("ms_trace", C.c_double), ("ms_shadow", C.c_double),
                 ("ms_shade", C.c_double), ("samples", C.c_uint64), ("rays_closest", C.c_uint64),
                 ("rays_shadow", C.c_uint64), ("node_visits", C.c_uint64), ("pack_tests", C.c_uint64),
-                ("inst_tests", C.c_uint64), ("trace_launches", C.c_uint64), ("nan_samples", C.c_uint64)]
+                ("inst_tests", C.c_uint64), ("trace_launches", C.c_uint64), ("nan_samples", C.c_uint64),
+                ("shadow_node_visits", C.c_uint64), ("shadow_pack_tests", C.c_uint64),
+                ("shadow_inst_tests", C.c_uint64), ("max_ray_nodes", C.c_uint64), ("bad_rays", C.c_uint64)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -147,6 +149,7 @@ HIP_API = {
     "gnd_hip_render": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(gnd_hip_job), C.c_int,
                                  C.POINTER(gnd_hip_stats)]),
     "gnd_hip_cancel": (C.c_int, [C.c_void_p]),
+    "gnd_hip_copy_probe": (C.c_int, [C.c_void_p, C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
     "gnd_hip_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(gnd_hip_job), C.POINTER(gnd_hip_tuning)]),
 }
 

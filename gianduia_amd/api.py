@@ -101,6 +101,12 @@ class Device:
     def set_counters(self, on: bool) -> None:
         self._check(_lib().gnd_hip_set_counters(self.ctx, int(bool(on))))
 
+    def copy_probe(self, nbytes: int, repeats: int = 10) -> float:
+        """Average ms of a plain device-to-device streaming copy of nbytes."""
+        ms = C.c_double()
+        self._check(_lib().gnd_hip_copy_probe(self.ctx, nbytes, repeats, C.byref(ms)))
+        return ms.value
+
     # ---- memory
     def alloc(self, nbytes: int) -> DeviceBuffer:
         return DeviceBuffer(self, nbytes)
@@ -178,6 +184,40 @@ class Device:
             else:
                 out.append(rgbw)
         return out, st.as_dict()
+
+
+class RenderSession:
+    """Jobs with device-resident films, rendered repeatedly without host copies."""
+
+    def __init__(self, dev: Device, scene: "DeviceScene", jobs: list["Job"], want_sq: bool = False):
+        self.dev, self.scene, self.jobs = dev, scene, jobs
+        self.cj = (A.gnd_hip_job * len(jobs))()
+        self.films = []
+        for i, j in enumerate(jobs):
+            w, h = j.camera.width, j.camera.height
+            f = dev.alloc(w * h * 16)
+            sq = dev.alloc(w * h * 12) if want_sq else None
+            self.films.append((f, sq, w, h))
+            self.cj[i].camera = j.camera
+            self.cj[i].spp = j.spp
+            self.cj[i].max_depth = j.max_depth
+            self.cj[i].integrator = j.integrator
+            self.cj[i].seed_offset = j.seed_offset
+            self.cj[i].film = f.addr
+            self.cj[i].film_sq = sq.addr if sq is not None else None
+
+    def run(self) -> dict:
+        st = A.gnd_hip_stats()
+        self.dev._check(_lib().gnd_hip_render(self.dev.ctx, self.scene.handle, self.cj, len(self.jobs), C.byref(st)))
+        return st.as_dict()
+
+    def film(self, i: int) -> np.ndarray:
+        f, _, w, h = self.films[i]
+        return self.dev.download(f, np.float32, w * h * 4).reshape(h, w, 4)
+
+    def film_sq(self, i: int) -> np.ndarray:
+        _, sq, w, h = self.films[i]
+        return self.dev.download(sq, np.float32, w * h * 3).reshape(h, w, 3)
 
 
 @dataclass

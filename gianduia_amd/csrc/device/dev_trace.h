@@ -72,71 +72,48 @@ GDEV_INLINE DNode loadNode(const DNode* nodes, int idx) {
     return n;
 }
 
-// Mesh::rayIntersect over one BLAS (object space).  lt is the object ray's
-// mutable tMax; returns true when a hit was committed (or any hit for ANY).
-template <bool ANY, bool COUNT, int LDSD>
-GDEV_INLINE bool traceMesh(const DScene& S, int root, V3 o, V3 d, float& lt, float& hu, float& hv, int& hp,
-                           TravStack<LDSD>& st, uint32_t* cnt) {
-    const V3 inv = V3{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
-    bool hitAny = false;
-    const int base = st.sp;
-    st.push(root);
-    while (st.sp > base) {
-        const int ni = st.pop();
-        const DNode n = loadNode(S.blas, ni);
-        if (COUNT) cnt[0]++;
-        const uint32_t mask = slab4(n, o, inv, kEps, lt);
-        if (mask == 0) continue;
-        for (int i = 0; i < 4; ++i) {
-            if (!((mask >> i) & 1u)) continue;
-            const uint32_t meta = compu(n.meta, i);
-            const uint32_t c = compu(n.child, i);
-            if ((meta >> 30) == 1u) {
-                st.push((int)c);
-            } else if ((meta >> 30) == 2u) {
-                const uint32_t np = meta & 0x3FFFFFFFu;
-                for (uint32_t p = c; p < c + np; ++p) {
-                    if (COUNT) cnt[1]++;
-                    const float4* pk = reinterpret_cast<const float4*>(S.packs + p);
-                    const float4 v0x = pk[0], v0y = pk[1], v0z = pk[2];
-                    const float4 v1x = pk[3], v1y = pk[4], v1z = pk[5];
-                    const float4 v2x = pk[6], v2y = pk[7], v2z = pk[8];
-                    const uint4 prim = reinterpret_cast<const uint4*>(pk)[9];
+// Mesh::rayIntersect's 4-lane Moller-Trumbore (mesh.cpp:262-319) on one pack:
+// lanes evaluated in order, committed with a strict t < lt.  Returns true on a
+// committed hit (for ANY: on the first valid lane).
+template <bool ANY>
+GDEV_INLINE bool testPack(const DPack* packs, uint32_t p, V3 o, V3 d, float& lt, float& hu, float& hv, int& hp) {
+    const float4* pk = reinterpret_cast<const float4*>(packs + p);
+    const float4 v0x = pk[0], v0y = pk[1], v0z = pk[2];
+    const float4 v1x = pk[3], v1y = pk[4], v1z = pk[5];
+    const float4 v2x = pk[6], v2y = pk[7], v2z = pk[8];
+    const uint4 prim = reinterpret_cast<const uint4*>(pk)[9];
+    bool hit = false;
 #pragma unroll
-                    for (int l = 0; l < 4; ++l) {
-                        const float ax = comp(v0x, l), ay = comp(v0y, l), az = comp(v0z, l);
-                        const float e1x = comp(v1x, l) - ax, e1y = comp(v1y, l) - ay, e1z = comp(v1z, l) - az;
-                        const float e2x = comp(v2x, l) - ax, e2y = comp(v2y, l) - ay, e2z = comp(v2z, l) - az;
-                        const float px = d.y * e2z - d.z * e2y;
-                        const float py = d.z * e2x - d.x * e2z;
-                        const float pz = d.x * e2y - d.y * e2x;
-                        const float det = (e1x * px + e1y * py) + e1z * pz;
-                        bool valid = __builtin_fabsf(det) > 1e-8f;
-                        const float invDet = 1.0f / det;
-                        const float tx = o.x - ax, ty = o.y - ay, tz = o.z - az;
-                        const float u = ((tx * px + ty * py) + tz * pz) * invDet;
-                        valid = valid && (u >= 0.0f) && (u <= 1.0f);
-                        const float qx = ty * e1z - tz * e1y;
-                        const float qy = tz * e1x - tx * e1z;
-                        const float qz = tx * e1y - ty * e1x;
-                        const float v = ((d.x * qx + d.y * qy) + d.z * qz) * invDet;
-                        valid = valid && (v >= 0.0f) && ((u + v) <= 1.0f);
-                        const float t = ((e2x * qx + e2y * qy) + e2z * qz) * invDet;
-                        valid = valid && (t > kEps) && (t < lt);
-                        if (valid) {
-                            if (ANY) return true;
-                            lt = t;
-                            hu = u;
-                            hv = v;
-                            hp = (int)compu(prim, l);
-                            hitAny = true;
-                        }
-                    }
-                }
-            }
+    for (int l = 0; l < 4; ++l) {
+        const float ax = comp(v0x, l), ay = comp(v0y, l), az = comp(v0z, l);
+        const float e1x = comp(v1x, l) - ax, e1y = comp(v1y, l) - ay, e1z = comp(v1z, l) - az;
+        const float e2x = comp(v2x, l) - ax, e2y = comp(v2y, l) - ay, e2z = comp(v2z, l) - az;
+        const float px = d.y * e2z - d.z * e2y;
+        const float py = d.z * e2x - d.x * e2z;
+        const float pz = d.x * e2y - d.y * e2x;
+        const float det = (e1x * px + e1y * py) + e1z * pz;
+        bool valid = __builtin_fabsf(det) > 1e-8f;
+        const float invDet = 1.0f / det;
+        const float tx = o.x - ax, ty = o.y - ay, tz = o.z - az;
+        const float u = ((tx * px + ty * py) + tz * pz) * invDet;
+        valid = valid && (u >= 0.0f) && (u <= 1.0f);
+        const float qx = ty * e1z - tz * e1y;
+        const float qy = tz * e1x - tx * e1z;
+        const float qz = tx * e1y - ty * e1x;
+        const float v = ((d.x * qx + d.y * qy) + d.z * qz) * invDet;
+        valid = valid && (v >= 0.0f) && ((u + v) <= 1.0f);
+        const float t = ((e2x * qx + e2y * qy) + e2z * qz) * invDet;
+        valid = valid && (t > kEps) && (t < lt);
+        if (valid) {
+            if (ANY) return true;
+            lt = t;
+            hu = u;
+            hv = v;
+            hp = (int)compu(prim, l);
+            hit = true;
         }
     }
-    return hitAny;
+    return hit;
 }
 
 // Sphere::rayIntersect with SolveQuadratic (object space, tMin = 1e-4)
@@ -154,63 +131,188 @@ GDEV_INLINE bool traceSphere(float radius, V3 o, V3 d, float& lt) {
     return false;
 }
 
+// Slab test with IEEE min/max.  Equal to slab4 when no product is NaN: node
+// bounds of non-empty children are finite, so that holds whenever the inverse
+// direction is finite (empty children are ignored by their type).
+GDEV_INLINE uint32_t slab4Fast(const DNode& n, V3 o, V3 inv, float tmin, float tmax) {
+    uint32_t mask = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        float t1x = (comp(n.xlo, i) - o.x) * inv.x;
+        float t2x = (comp(n.xhi, i) - o.x) * inv.x;
+        float t1y = (comp(n.ylo, i) - o.y) * inv.y;
+        float t2y = (comp(n.yhi, i) - o.y) * inv.y;
+        float t1z = (comp(n.zlo, i) - o.z) * inv.z;
+        float t2z = (comp(n.zhi, i) - o.z) * inv.z;
+        float tn = __builtin_fmaxf(__builtin_fmaxf(__builtin_fminf(t1x, t2x), __builtin_fminf(t1y, t2y)),
+                                   __builtin_fmaxf(__builtin_fminf(t1z, t2z), tmin));
+        float tf = __builtin_fminf(__builtin_fminf(__builtin_fmaxf(t1x, t2x), __builtin_fmaxf(t1y, t2y)),
+                                   __builtin_fminf(__builtin_fmaxf(t1z, t2z), tmax));
+        mask |= (tn <= tf) ? (1u << i) : 0u;
+    }
+    return mask;
+}
+
+GDEV_INLINE bool finite3(V3 v) {
+    return __builtin_isfinite(v.x) && __builtin_isfinite(v.y) && __builtin_isfinite(v.z);
+}
+
+// Up to four index ranges consumed in order (the leaf children of one node).
+struct RangeQueue {
+    uint32_t b0, e0, b1, e1, b2, e2, b3, e3;
+    int n;
+    GDEV_INLINE void clear() { n = 0; }
+    GDEV_INLINE bool empty() const { return n == 0; }
+    GDEV_INLINE void add(uint32_t b, uint32_t c) {
+        if (c == 0) return;
+        const uint32_t e = b + c;
+        if (n == 0) { b0 = b; e0 = e; }
+        else if (n == 1) { b1 = b; e1 = e; }
+        else if (n == 2) { b2 = b; e2 = e; }
+        else { b3 = b; e3 = e; }
+        ++n;
+    }
+    GDEV_INLINE uint32_t take() {
+        const uint32_t v = b0++;
+        if (b0 == e0) {
+            b0 = b1; e0 = e1; b1 = b2; e1 = e2; b2 = b3; e2 = e3;
+            --n;
+        }
+        return v;
+    }
+};
+
+enum : int { ACT_NONE = 0, ACT_PACK = 1, ACT_NODE = 2, ACT_INST = 3 };
+
 // Scene::rayIntersect: closest hit (ANY = false) or any hit (ANY = true).
+//
+// Each lane runs the reference traversal as a state machine whose steps are a
+// BVH4 node pop, a TrianglePack4 test, or an instance start / finish.  Per
+// lane the steps happen exactly in the reference order (a node's leaf
+// children are tested, in child order, before the next pop; the instances of
+// a TLAS leaf are traversed in order before the next TLAS pop), so results and
+// visit counts are the reference's.  Across the wave, every loop iteration
+// runs one kind of step, chosen by ballot as the kind most lanes are waiting
+// for: lanes with pending triangle packs wait while others pop nodes, and the
+// expensive pack test then runs with most of the wave active.
 template <bool ANY, bool COUNT, int LDSD>
 GDEV_INLINE bool traceRay(const DScene& S, V3 o, V3 d, float tmin, float tmax, HitRec& hit, TravStack<LDSD>& st,
-                          uint32_t* cnt) {
+                          uint32_t* cnt, bool active = true) {
     hit.inst = -1;
     hit.t = finf();
-    if (S.nTlas == 0) return false;
-    const V3 inv = V3{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
-    float tcur = tmax;          // the world ray's mutable tMax
-    float best = kFltMax;       // isect.t starts at FLT_MAX (bvh.cpp:39)
-    bool hitAny = false;
+    hit.prim = -1;
+    hit.u = hit.v = 0.0f;
+    const V3 winv = V3{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+    float tcur = tmax;        // the world ray's mutable tMax
+    float best = kFltMax;     // isect.t starts at FLT_MAX (bvh.cpp:39)
+    bool found = false;
+    bool done = !active || S.nTlas == 0;
+    int level = 0, blasBase = 0, curInst = -1;
+    V3 lo = o, ld = d, linv = winv;
+    float lt = tcur, hu = 0.0f, hv = 0.0f;
+    int hp = 0;
+    bool instHit = false;
+    RangeQueue tq, bq;
+    tq.clear();
+    bq.clear();
     st.sp = 0;
-    st.push(0);
-    while (st.sp > 0) {
-        const int ni = st.pop();
-        const DNode n = loadNode(S.tlas, ni);
-        if (COUNT) cnt[0]++;
-        const uint32_t mask = slab4(n, o, inv, tmin, tcur);
-        if (mask == 0) continue;
-        for (int i = 0; i < 4; ++i) {
-            if (!((mask >> i) & 1u)) continue;
-            const uint32_t meta = compu(n.meta, i);
-            const uint32_t c = compu(n.child, i);
-            if ((meta >> 30) == 1u) {
-                st.push((int)c);
-            } else if ((meta >> 30) == 2u) {
-                const uint32_t np = meta & 0x3FFFFFFFu;
-                for (uint32_t p = c; p < c + np; ++p) {
-                    const int pi = S.tlasPrims[p];
-                    const DInst& I = S.inst[pi];
-                    if (COUNT) cnt[2]++;
-                    const V3 lo = xfPoint(I.worldToObject, o);
-                    const V3 ld = xfVector(I.worldToObject, d);
-                    float lt = tcur;
-                    float hu = 0.0f, hv = 0.0f;
-                    int hp = 0;
-                    bool h;
-                    if (I.root >= 0) h = traceMesh<ANY, COUNT, LDSD>(S, I.root, lo, ld, lt, hu, hv, hp, st, cnt);
-                    else h = traceSphere(I.radius, lo, ld, lt);
-                    if (h) {
-                        if (ANY) return true;
-                        if (lt < best) {
-                            tcur = lt;
-                            best = lt;
-                            hit.t = lt;
-                            hit.u = hu;
-                            hit.v = hv;
-                            hit.inst = pi;
-                            hit.prim = hp;
-                            hitAny = true;
-                        }
+    if (!done) st.push(0);
+    for (;;) {
+        int act = ACT_NONE;
+        if (!done) {
+            if (!bq.empty()) act = ACT_PACK;
+            else if (level == 1) act = (st.sp > blasBase) ? ACT_NODE : ACT_INST;
+            else if (!tq.empty()) act = ACT_INST;
+            else if (st.sp > 0) act = ACT_NODE;
+            else done = true;
+        }
+        const uint64_t mP = __ballot(act == ACT_PACK);
+        const uint64_t mN = __ballot(act == ACT_NODE);
+        const uint64_t mI = __ballot(act == ACT_INST);
+        if ((mP | mN | mI) == 0) break;
+        const int nP = __popcll(mP), nN = __popcll(mN), nI = __popcll(mI);
+        int pick;
+        if (nI > 0 && nI >= nN && nI >= nP) pick = ACT_INST;
+        else if (nP > 0 && nP >= nN) pick = ACT_PACK;
+        else if (nN > 0) pick = ACT_NODE;
+        else pick = nP > 0 ? ACT_PACK : ACT_INST;
+        if (act != pick) continue;
+        if (pick == ACT_PACK) {
+            if (COUNT) cnt[1]++;
+            const uint32_t p = bq.take();
+            if (testPack<ANY>(S.packs, p, lo, ld, lt, hu, hv, hp)) {
+                if (ANY) return true;
+                instHit = true;
+            }
+        } else if (pick == ACT_NODE) {
+            if (COUNT) cnt[0]++;
+            const int ni = st.pop();
+            const bool blas = level == 1;
+            const DNode n = loadNode(blas ? S.blas : S.tlas, ni);
+            const V3 ro = blas ? lo : o;
+            const V3 ri = blas ? linv : winv;
+            const float rmin = blas ? kEps : tmin;
+            const float rmax = blas ? lt : tcur;
+            const bool fast = __all(finite3(ri));
+            const uint32_t mask = fast ? slab4Fast(n, ro, ri, rmin, rmax) : slab4(n, ro, ri, rmin, rmax);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                if (!((mask >> i) & 1u)) continue;
+                const uint32_t meta = compu(n.meta, i);
+                const uint32_t c = compu(n.child, i);
+                const uint32_t type = meta >> 30;
+                if (type == 1u) st.push((int)c);
+                else if (type == 2u) {
+                    if (blas) bq.add(c, meta & 0x3FFFFFFFu);
+                    else tq.add(c, meta & 0x3FFFFFFFu);
+                }
+            }
+        } else {  // ACT_INST
+            if (level == 1) {
+                // Primitive::rayIntersect returned: BVH::rayIntersect keeps the hit
+                if (instHit && lt < best) {
+                    tcur = lt;
+                    best = lt;
+                    hit.t = lt;
+                    hit.u = hu;
+                    hit.v = hv;
+                    hit.inst = curInst;
+                    hit.prim = hp;
+                    found = true;
+                }
+                level = 0;
+            } else {
+                const uint32_t slot = tq.take();
+                const int pi = S.tlasPrims[slot];
+                const DInst& I = S.inst[pi];
+                if (COUNT) cnt[2]++;
+                lo = xfPoint(I.worldToObject, o);
+                ld = xfVector(I.worldToObject, d);
+                lt = tcur;
+                curInst = pi;
+                instHit = false;
+                if (I.root >= 0) {
+                    linv = V3{1.0f / ld.x, 1.0f / ld.y, 1.0f / ld.z};
+                    level = 1;
+                    blasBase = st.sp;
+                    st.push(I.root);
+                } else if (traceSphere(I.radius, lo, ld, lt)) {
+                    if (ANY) return true;
+                    if (lt < best) {
+                        tcur = lt;
+                        best = lt;
+                        hit.t = lt;
+                        hit.u = 0.0f;
+                        hit.v = 0.0f;
+                        hit.inst = pi;
+                        hit.prim = 0;
+                        found = true;
                     }
                 }
             }
         }
     }
-    return hitAny;
+    return found;
 }
 
 }  // namespace gdev

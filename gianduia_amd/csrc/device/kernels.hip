@@ -75,7 +75,10 @@ GDEV_INLINE int flagMedium(uint32_t f) { return (int)((f >> 8) & 0xFFFFu) - 1; }
 GDEV_INLINE uint32_t setMedium(uint32_t f, int m) { return (f & ~(0xFFFFu << 8)) | ((uint32_t)(m + 1) << 8); }
 
 struct Counters {
-    unsigned long long nodes, packs, insts, rays, shadowRays, nan, stackOverflow, pad;
+    unsigned long long nodes, packs, insts;      // closest-hit traversals
+    unsigned long long snodes, spacks, sinsts;   // shadow / transmittance traversals
+    unsigned long long nan, stackOverflow;
+    unsigned long long maxRayNodes, badRays;     // worst closest-hit ray, rays with non-finite directions
 };
 
 // wave-aggregated append: ballot, popcount, one atomic per wave
@@ -110,7 +113,10 @@ GDEV_INLINE TravStack<LDSD> makeStack(int* lds) {
     return st;
 }
 
-GDEV_INLINE void flushCounters(Counters* c, const uint32_t* cnt, bool overflow) {
+GDEV_INLINE void flushCounters(Counters* c, const uint32_t* cnt, bool overflow, bool shadow = false,
+                               uint32_t maxRay = 0, uint32_t bad = 0) {
+    if (maxRay) atomicMax(&c->maxRayNodes, (unsigned long long)maxRay);
+    if (bad) atomicAdd(&c->badRays, (unsigned long long)bad);
     unsigned long long a = cnt[0], b = cnt[1], d = cnt[2];
     // wave reduction then one atomic per wave
     for (int off = 32; off > 0; off >>= 1) {
@@ -119,9 +125,9 @@ GDEV_INLINE void flushCounters(Counters* c, const uint32_t* cnt, bool overflow) 
         d += __shfl_down(d, off);
     }
     if (__lane_id() == 0) {
-        atomicAdd(&c->nodes, a);
-        atomicAdd(&c->packs, b);
-        atomicAdd(&c->insts, d);
+        atomicAdd(shadow ? &c->snodes : &c->nodes, a);
+        atomicAdd(shadow ? &c->spacks : &c->packs, b);
+        atomicAdd(shadow ? &c->sinsts : &c->insts, d);
     }
     if (overflow) atomicAdd(&c->stackOverflow, 1ull);
 }
@@ -236,21 +242,27 @@ __global__ void kTracePaths(WaveArgs A) {
         A.counts[2] = 0u;
     }
     uint32_t cnt[3] = {0, 0, 0};
+    uint32_t maxRay = 0, bad = 0;
     TravStack<LDSD> st = makeStack<LDSD>(sStack);
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t i = xcdBlock() * blockDim.x + threadIdx.x; i < n; i += stride) {
         const int32_t p = A.qIn[i];
         HitRec h;
         const PathSoA& P = A.P;
-        bool r = traceRay<false, COUNT, LDSD>(A.S, V3{P.ox[p], P.oy[p], P.oz[p]}, V3{P.dx[p], P.dy[p], P.dz[p]},
-                                              kEps, finf(), h, st, cnt);
+        const uint32_t before = cnt[0];
+        const V3 dir = V3{P.dx[p], P.dy[p], P.dz[p]};
+        bool r = traceRay<false, COUNT, LDSD>(A.S, V3{P.ox[p], P.oy[p], P.oz[p]}, dir, kEps, finf(), h, st, cnt);
+        if (COUNT) {
+            maxRay = max(maxRay, cnt[0] - before);
+            if (!__builtin_isfinite(dir.x) || !__builtin_isfinite(dir.y) || !__builtin_isfinite(dir.z)) bad++;
+        }
         A.Hq.t[i] = r ? h.t : finf();
         A.Hq.inst[i] = r ? h.inst : -1;
         A.Hq.prim[i] = r ? h.prim : -1;
         A.Hq.u[i] = h.u;
         A.Hq.v[i] = h.v;
     }
-    if (COUNT) flushCounters(A.ctr, cnt, st.overflow);
+    if (COUNT) flushCounters(A.ctr, cnt, st.overflow, false, maxRay, bad);
     else if (st.overflow) atomicAdd(&A.ctr->stackOverflow, 1ull);
 }
 
@@ -639,7 +651,7 @@ __global__ void kShadow(WaveArgs A) {
             A.P.Lb[p] = A.P.Lb[p] + c.z;
         }
     }
-    if (COUNT) flushCounters(A.ctr, cnt, st.overflow);
+    if (COUNT) flushCounters(A.ctr, cnt, st.overflow, true);
     else if (st.overflow) atomicAdd(&A.ctr->stackOverflow, 1ull);
 }
 
@@ -717,6 +729,13 @@ __global__ void kSplat(WaveArgs A, const DSplatTask* tasks, int nTasks, uint32_t
     if (nan) atomicAdd(&A.ctr->nan, nan);
 }
 
+// plain streaming copy (bandwidth probe and HBM-counter calibration)
+__global__ void kCopyProbe(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
 __global__ void kFill(float* p, size_t n, float v) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -741,7 +760,9 @@ struct gnd_hip_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     std::mutex mu;
-    gnd_hip_tuning tuning{128, 16, 1 << 22, 0};
+    // 40M paths in flight (~10 GB of path state): a whole 1080p x 16 spp frame
+    // per wave, so every bounce is one large launch
+    gnd_hip_tuning tuning{128, 16, 40000000, 0};
     bool counters = false;
     int numCUs = 0;
     size_t hbm = 0;
@@ -1399,7 +1420,9 @@ int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jo
         WaveKernel kTrace = cnt ? pickTracePaths<true>(lds) : pickTracePaths<false>(lds);
         WaveKernel kSh = fastShadow ? (cnt ? pickShadow<true, true>(lds) : pickShadow<true, false>(lds))
                                     : (cnt ? pickShadow<false, true>(lds) : pickShadow<false, false>(lds));
-        const unsigned gridCap = (unsigned)std::max(8, ctx->numCUs * 8);
+        // enough blocks for one item per thread up to 64 waves per CU (the
+        // traversal is latency bound: occupancy hides node-fetch latency)
+        const unsigned gridCap = (unsigned)std::max(8, ctx->numCUs * 64 * 64 / B);
         auto grid = [&](uint32_t n) {
             unsigned b = (unsigned)((n + B - 1) / B);
             b = std::min(b, gridCap);
@@ -1524,12 +1547,43 @@ int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jo
             stats->node_visits = c.nodes;
             stats->pack_tests = c.packs;
             stats->inst_tests = c.insts;
+            stats->shadow_node_visits = c.snodes;
+            stats->shadow_pack_tests = c.spacks;
+            stats->shadow_inst_tests = c.sinsts;
+            stats->max_ray_nodes = c.maxRayNodes;
+            stats->bad_rays = c.badRays;
             stats->trace_launches = launches;
             stats->nan_samples = c.nan;
         }
         for (auto e : evs) (void)hipEventDestroy(e);
         if (c.stackOverflow) throw HipError(GND_E_STACK, "traversal stack overflow");
     });
+}
+
+int gnd_hip_copy_probe(gnd_hip_ctx* ctx, size_t bytes, int repeats, double* ms_per_copy) {
+    if (!ctx || bytes < 16 || repeats <= 0) return GND_E_INVALID;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    void *a = nullptr, *b = nullptr;
+    int rc = guarded(ctx, [&] {
+        GND_CHECK(hipSetDevice(ctx->device));
+        const size_t n = bytes / 16;
+        GND_CHECK(hipMalloc(&a, n * 16));
+        GND_CHECK(hipMalloc(&b, n * 16));
+        hipLaunchKernelGGL(kFill, dim3((unsigned)((n * 4 + 255) / 256)), dim3(256), 0, ctx->stream, (float*)a,
+                           n * 4, 1.0f);
+        const unsigned grid = (unsigned)std::max(1, ctx->numCUs * 16);
+        hipLaunchKernelGGL(kCopyProbe, dim3(grid), dim3(256), 0, ctx->stream, (const float4*)a, (float4*)b, n);
+        GND_CHECK(hipEventRecord(ctx->ev[0], ctx->stream));
+        for (int r = 0; r < repeats; ++r)
+            hipLaunchKernelGGL(kCopyProbe, dim3(grid), dim3(256), 0, ctx->stream, (const float4*)a, (float4*)b, n);
+        GND_CHECK(hipGetLastError());
+        GND_CHECK(hipEventRecord(ctx->ev[1], ctx->stream));
+        GND_CHECK(hipEventSynchronize(ctx->ev[1]));
+        if (ms_per_copy) *ms_per_copy = elapsedMs(ctx->ev[0], ctx->ev[1]) / repeats;
+    });
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return rc;
 }
 
 int gnd_hip_autotune(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* job, gnd_hip_tuning* out) {

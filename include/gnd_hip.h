@@ -117,6 +117,11 @@ typedef struct gnd_hip_stats {
     uint64_t inst_tests;    /* instance (primitive) tests (counters on)       */
     uint64_t trace_launches;
     uint64_t nan_samples;   /* samples zeroed by the NaN/Inf guard            */
+    uint64_t shadow_node_visits;  /* the same counters for shadow rays      */
+    uint64_t shadow_pack_tests;
+    uint64_t shadow_inst_tests;
+    uint64_t max_ray_nodes; /* node pops of the worst closest-hit ray (counters on) */
+    uint64_t bad_rays;      /* rays with a non-finite direction (counters on)       */
 } gnd_hip_stats;
 
 /* Tunables picked by gnd_hip_autotune or set by the caller. */
@@ -172,6 +177,11 @@ int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jo
 /* Integrator::cancel (include/gianduia/core/integrator.h:22): honoured at the
  * next wave boundary of a running gnd_hip_render on this context. */
 int gnd_hip_cancel(gnd_hip_ctx* ctx);
+
+/* Bandwidth probe: `repeats` launches of a plain streaming copy of `bytes`
+ * between two device buffers; ms_per_copy receives the average kernel time.
+ * Used to calibrate HBM counters (a known-size copy reads and writes bytes). */
+int gnd_hip_copy_probe(gnd_hip_ctx* ctx, size_t bytes, int repeats, double* ms_per_copy);
 
 /* Picks block size, LDS stack depth and queue capacity by timing the
  * traversal and bounce kernels on the given scene and job. */
