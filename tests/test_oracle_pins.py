@@ -1,0 +1,353 @@
+"""Pins for the oracle functions the reference's own tests do not cover:
+brute force for traversal, closed forms and distribution checks for warps,
+BSDFs, the phase function and media.  Each check is chosen so that a dropped
+term, a wrong sign or a transposed operand fails it."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+from scipy import stats as sst
+
+from gianduia_amd import HostScene, _abi as A
+import oracle as O
+from oracle import RefScene, fp, vec
+from scenes import SceneBuilder, random_rays, random_soup, trs
+
+
+# ---------------------------------------------------------------- traversal vs brute force
+def brute_force(tris, rays):
+    """Closest Moller-Trumbore hit over all triangles in float32, evaluated in the
+    reference's operation order (mesh.cpp:262-291)."""
+    f = np.float32
+    v0, v1, v2 = tris[:, 0:3], tris[:, 3:6], tris[:, 6:9]
+    e1, e2 = (v1 - v0).astype(f), (v2 - v0).astype(f)
+    out_t = np.full(len(rays["ox"]), np.inf, f)
+    out_p = np.full(len(rays["ox"]), -1, np.int64)
+    for i in range(len(rays["ox"])):
+        o = np.array([rays["ox"][i], rays["oy"][i], rays["oz"][i]], f)
+        d = np.array([rays["dx"][i], rays["dy"][i], rays["dz"][i]], f)
+        px = d[1] * e2[:, 2] - d[2] * e2[:, 1]
+        py = d[2] * e2[:, 0] - d[0] * e2[:, 2]
+        pz = d[0] * e2[:, 1] - d[1] * e2[:, 0]
+        det = (e1[:, 0] * px + e1[:, 1] * py) + e1[:, 2] * pz
+        with np.errstate(all="ignore"):
+            inv = f(1.0) / det
+            tv = (o - v0).astype(f)
+            u = ((tv[:, 0] * px + tv[:, 1] * py) + tv[:, 2] * pz) * inv
+            qx = tv[:, 1] * e1[:, 2] - tv[:, 2] * e1[:, 1]
+            qy = tv[:, 2] * e1[:, 0] - tv[:, 0] * e1[:, 2]
+            qz = tv[:, 0] * e1[:, 1] - tv[:, 1] * e1[:, 0]
+            v = ((d[0] * qx + d[1] * qy) + d[2] * qz) * inv
+            t = ((e2[:, 0] * qx + e2[:, 1] * qy) + e2[:, 2] * qz) * inv
+            ok = (np.abs(det) > f(1e-8)) & (u >= 0) & (u <= 1) & (v >= 0) & ((u + v) <= 1) & \
+                 (t > f(1e-4)) & (t < rays["tmax"][i])
+        if ok.any():
+            k = np.nonzero(ok)[0]
+            j = k[np.argmin(t[k])]
+            out_t[i], out_p[i] = t[j], j
+    return out_t, out_p
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_closest_hit_equals_brute_force(seed):
+    rng = np.random.default_rng(seed)
+    tris = random_soup(rng, 400, spread=1.0, size=0.3)
+    b = SceneBuilder()
+    mt = b.matte()
+    b.prim(b.mesh(tris), material=mt)
+    b.prim(b.mesh(np.array([[-1, 9, -1, 1, 9, -1, 1, 9, 1]], np.float32)), material=mt, radiance=(1, 1, 1))
+    ref = RefScene(b.desc())
+    rays = random_rays(rng, 300, edge_cases=False)
+    r = ref.trace(rays)
+    bt, bp = brute_force(tris, rays)
+    hit = bp >= 0
+    # the light quad at y=9 is never reached by these rays
+    assert np.array_equal(r["inst"] >= 0, hit)
+    assert np.array_equal(r["t"][hit].view(np.uint32), bt[hit].view(np.uint32))
+    assert np.array_equal(r["prim"][hit], bp[hit])
+
+
+def test_any_hit_equals_closest_hit_existence():
+    b = SceneBuilder()
+    mt = b.matte()
+    rng = np.random.default_rng(4)
+    m = b.mesh(random_soup(rng, 800))
+    b.prim(m, material=mt)
+    M, Mi = trs(scale=(0.5, 0.7, 0.9), rot_deg=33, axis=(1, 2, 3), trans=(0.5, -0.3, 1.0))
+    b.prim(m, M, Mi, mt)
+    b.prim(-1, material=mt, sphere_radius=0.3, radiance=(1, 1, 1))
+    ref = RefScene(b.desc())
+    rays = random_rays(rng, 2000)
+    rays["tmax"] = np.where(np.isinf(rays["tmax"]), np.float32(1.5), rays["tmax"]).astype(np.float32)
+    r = ref.trace(rays)
+    occ = ref.trace(rays, any_hit=True)
+    assert np.array_equal(occ == 1, r["inst"] >= 0)
+
+
+def test_instanced_hit_equals_world_space_copy():
+    """A transformed instance hits what a pre-transformed copy of its triangles hits."""
+    rng = np.random.default_rng(6)
+    tris = random_soup(rng, 300)
+    M, Mi = trs(scale=(1, 1, 1), rot_deg=90, axis=(0, 0, 1), trans=(0, 0, 0))
+    b1 = SceneBuilder()
+    mt = b1.matte()
+    b1.prim(b1.mesh(tris), M, Mi, mt)
+    b1.prim(-1, material=mt, sphere_radius=0.01, radiance=(1, 1, 1))
+    # rotation by 90 deg about z maps (x, y, z) -> (-y, x, z) exactly
+    w = tris.reshape(-1, 3).copy()
+    w = np.stack([-w[:, 1], w[:, 0], w[:, 2]], 1).reshape(-1, 9).astype(np.float32)
+    b2 = SceneBuilder()
+    mt2 = b2.matte()
+    b2.prim(b2.mesh(w), material=mt2)
+    b2.prim(-1, material=mt2, sphere_radius=0.01, radiance=(1, 1, 1))
+    rays = random_rays(rng, 500, edge_cases=False)
+    r1 = RefScene(b1.desc()).trace(rays)
+    r2 = RefScene(b2.desc()).trace(rays)
+    both = (r1["inst"] == 0) & (r2["inst"] == 0)
+    assert both.sum() > 50
+    assert np.array_equal(r1["prim"][both], r2["prim"][both])
+    assert np.allclose(r1["t"][both], r2["t"][both], rtol=1e-5)
+
+
+# ---------------------------------------------------------------- sampler
+def test_sampler_seeding_is_splitmix_then_pcg():
+    # IndependentSampler::seed(idx) = PCG32.seed(splitMix64(idx), 1)
+    for idx in (0, 1, 12345, 2 ** 40 + 7):
+        s = O.lib().gnd_ref_splitmix64(idx)
+        out = (C.c_uint32 * 4)()
+        O.lib().gnd_ref_pcg32(s, 1, 4, out)
+        fl = np.zeros(4, np.float32)
+        O.lib().gnd_ref_sampler_floats(idx, 4, fp(fl))
+        expect = np.array([np.float32(np.float32(v) * np.float32(2.0 ** -32)) for v in out], np.float32)
+        assert np.array_equal(fl, expect)
+
+
+# ---------------------------------------------------------------- warps (warp_test.cpp chi^2 style)
+def _chi2_2d(points, pdf_fn, bins=20, lo=-1.0, hi=1.0, n=100000):
+    h, xe, ye = np.histogram2d(points[:, 0], points[:, 1], bins=bins, range=[[lo, hi], [lo, hi]])
+    exp = np.zeros_like(h)
+    for i in range(bins):
+        for j in range(bins):
+            xs = np.linspace(xe[i], xe[i + 1], 6)[:-1] + (xe[1] - xe[0]) / 10
+            ys = np.linspace(ye[j], ye[j + 1], 6)[:-1] + (ye[1] - ye[0]) / 10
+            X, Y = np.meshgrid(xs, ys)
+            exp[i, j] = pdf_fn(X, Y).mean() * (xe[1] - xe[0]) * (ye[1] - ye[0]) * n
+    m = exp > 5
+    return float((((h - exp) ** 2)[m] / exp[m]).sum()), int(m.sum())
+
+
+def test_uniform_disk_chi2():
+    rng = np.random.default_rng(42)
+    n = 100000
+    u = rng.random((n, 2)).astype(np.float32)
+    pts = np.zeros((n, 2))
+    out = np.zeros(3, np.float32)
+    for i in range(n):
+        O.lib().gnd_ref_warp(0, fp(u[i]), fp(out))
+        pts[i] = out[:2]
+    chi2, dof = _chi2_2d(pts, lambda x, y: (x * x + y * y <= 1) / math.pi, n=n)
+    assert chi2 < sst.chi2.ppf(0.999, dof - 1)
+
+
+def test_cosine_hemisphere_and_sphere():
+    rng = np.random.default_rng(7)
+    n = 50000
+    u = rng.random((n, 2)).astype(np.float32)
+    z = np.zeros(n)
+    zs = np.zeros(n)
+    out = np.zeros(3, np.float32)
+    for i in range(n):
+        O.lib().gnd_ref_warp(1, fp(u[i]), fp(out))
+        assert abs(np.dot(out, out) - 1) < 1e-5
+        z[i] = out[2]
+        O.lib().gnd_ref_warp(2, fp(u[i]), fp(out))
+        zs[i] = out[2]
+    # cosine-weighted: cos(theta)^2 ~ U[0,1]; uniform sphere: z ~ U[-1,1]
+    assert sst.kstest(z * z, "uniform").pvalue > 1e-3
+    assert sst.kstest((zs + 1) / 2, "uniform").pvalue > 1e-3
+    assert O.lib().gnd_ref_cosine_hemisphere_pdf(fp(vec(0, 0, 1))) == np.float32(0.31830988618379067154)
+    assert O.lib().gnd_ref_cosine_hemisphere_pdf(fp(vec(0, 0, -1))) == 0.0
+
+
+# ---------------------------------------------------------------- BSDF energy / pdf integrals
+def _mat(t, **kw):
+    m = A.gnd_material_desc()
+    m.type = t
+    for k, v in kw.items():
+        if isinstance(v, tuple):
+            getattr(m, k)[:] = v
+        else:
+            setattr(m, k, v)
+    m.medium_inside = m.medium_outside = -1
+    return m
+
+
+def _mc_weight(m, wo, n=20000, seed=1):
+    rng = np.random.default_rng(seed)
+    w = np.zeros(3, np.float32)
+    wi = np.zeros(3, np.float32)
+    pdf = np.zeros(1, np.float32)
+    acc = np.zeros(3)
+    for _ in range(n):
+        u = rng.random(3).astype(np.float32)
+        O.lib().gnd_ref_bsdf_sample_weight(C.byref(m), fp(wo), fp(u), 0.0, fp(w), fp(wi), fp(pdf))
+        acc += w
+    return acc / n
+
+
+def test_white_furnace_lambert_and_smooth_conductor():
+    wo = np.array([0.3, -0.2, 0.9327379], np.float32)
+    assert np.allclose(_mc_weight(_mat(A.GND_MAT_MATTE, albedo=(1.0, 1.0, 1.0)), wo), 1.0, atol=2e-3)
+    # conductor with k = 0 and eta -> large: F ~ 1; mirror weight is R * F exactly
+    m = _mat(A.GND_MAT_CONDUCTOR, R=(0.5, 0.5, 0.5), eta_rgb=(1, 1, 1), k_rgb=(1, 1, 1), roughness=0.0)
+    w = _mc_weight(m, wo, n=10)
+    out = np.zeros(3, np.float32)
+    O.lib().gnd_ref_fr_conductor(wo[2], fp(vec(1, 1, 1)), fp(vec(1, 1, 1)), fp(vec(1, 1, 1)), fp(out))
+    assert np.allclose(w, 0.5 * out, rtol=1e-6)
+
+
+@pytest.mark.parametrize("t,kw", [
+    (A.GND_MAT_CONDUCTOR, dict(R=(1, 1, 1), eta_rgb=(0.2, 0.2, 0.2), k_rgb=(3, 3, 3), roughness=0.4)),
+    (A.GND_MAT_PLASTIC, dict(Kd=(0.5, 0.5, 0.5), Ks=(0.5, 0.5, 0.5), eta=1.5, roughness=0.3)),
+    (A.GND_MAT_GLASS, dict(R=(1, 1, 1), T=(1, 1, 1), eta=1.5, roughness=0.3)),
+])
+def test_energy_bounded_and_pdf_consistent(t, kw):
+    m = _mat(t, **kw)
+    wo = np.array([0.4, 0.1, 0.9110434], np.float32)
+    w = _mc_weight(m, wo, n=8000)
+    assert np.all(w <= 1.02), f"energy gain {w}"
+    # sample/pdf consistency on sampled directions
+    rng = np.random.default_rng(3)
+    f = np.zeros(3, np.float32)
+    wi = np.zeros(3, np.float32)
+    pdf = np.zeros(1, np.float32)
+    for _ in range(200):
+        u = rng.random(2).astype(np.float32)
+        st = O.lib().gnd_ref_lobe_sample(C.byref(m), fp(wo), fp(u), np.float32(rng.random()), fp(f), fp(wi), fp(pdf))
+        if pdf[0] > 0:
+            assert abs(O.lib().gnd_ref_lobe_pdf(C.byref(m), fp(wo), fp(wi)) - pdf[0]) <= 1e-5 * max(1, pdf[0])
+
+
+def test_glass_pdf_integrates_to_one():
+    """Rough dielectric: pdf over the sphere of wi integrates to <= 1 (reflection + transmission)."""
+    m = _mat(A.GND_MAT_GLASS, R=(1, 1, 1), T=(1, 1, 1), eta=1.5, roughness=0.5)
+    wo = np.array([0.3, 0.0, 0.9539392], np.float32)
+    rng = np.random.default_rng(8)
+    n = 40000
+    acc = 0.0
+    out = np.zeros(3, np.float32)
+    for _ in range(n):
+        u = rng.random(2).astype(np.float32)
+        O.lib().gnd_ref_warp(2, fp(u), fp(out))
+        acc += O.lib().gnd_ref_lobe_pdf(C.byref(m), fp(wo), fp(out))
+    integral = acc / n * 4 * math.pi
+    assert 0.9 < integral < 1.03
+
+
+# ---------------------------------------------------------------- phase function
+def test_hg_pdf_normalized_and_sampling_mirrors_g():
+    g = 0.3
+    wo = np.array([0.0, 0.0, 1.0], np.float32)
+    rng = np.random.default_rng(11)
+    out = np.zeros(3, np.float32)
+    acc = 0.0
+    n = 40000
+    for _ in range(n):
+        u = rng.random(2).astype(np.float32)
+        O.lib().gnd_ref_warp(2, fp(u), fp(out))
+        acc += O.lib().gnd_ref_hg_pdf(g, fp(wo), fp(out))
+    assert abs(acc / n * 4 * math.pi - 1.0) < 0.02
+    # phase.h:49-73 draws mu = wo.wi from HG(g) written for wi towards the
+    # light while p() uses wo.wi: the sampled mean cosine is +g, while p(wo, .)
+    # has mean cosine -g (the documented quirk, kept on purpose)
+    wi = np.zeros(3, np.float32)
+    mus = []
+    for _ in range(20000):
+        u = rng.random(2).astype(np.float32)
+        O.lib().gnd_ref_hg_sample(g, fp(wo), fp(u), fp(wi))
+        mus.append(float(np.dot(wo, wi)))
+    assert abs(np.mean(mus) - g) < 0.02
+
+
+# ---------------------------------------------------------------- media
+def _media_scene(sig_t=2.0, density=None, n=8):
+    b = SceneBuilder()
+    mt = b.matte()
+    b.prim(-1, material=mt, sphere_radius=0.01, radiance=(1, 1, 1))
+    d = b.desc()
+    med = (A.gnd_medium_desc * 2)()
+    med[0].type = A.GND_MEDIUM_HOMOGENEOUS
+    med[0].sigma_a[:] = (0.4 * sig_t / 2, 0.4 * sig_t / 2, 0.4 * sig_t / 2)
+    med[0].sigma_s[:] = (1.6 * sig_t / 2, 1.6 * sig_t / 2, 1.6 * sig_t / 2)
+    eye = np.eye(4, dtype=np.float32)
+    med[0].to_local[:] = eye.reshape(-1).tolist()
+    grid = np.ascontiguousarray(np.full((n, n, n), 0.5, np.float32) if density is None else density)
+    b.keep.append(grid)
+    med[1].type = A.GND_MEDIUM_GRID
+    med[1].sigma_a[:] = (0.4, 0.4, 0.4)
+    med[1].sigma_s[:] = (1.6, 1.6, 1.6)
+    med[1].nx = med[1].ny = med[1].nz = n
+    med[1].density = grid.ctypes.data_as(C.POINTER(C.c_float))
+    med[1].density_scale = 1.0
+    med[1].macro_res = 4
+    # world [0,1]^3 -> index [0,n]^3
+    s = np.diag([n, n, n, 1]).astype(np.float32)
+    med[1].to_local[:] = s.T.reshape(-1).tolist()
+    b.keep.append(med)
+    d.nmedia = 2
+    d.media = med
+    return RefScene(d)
+
+
+def test_homogeneous_transmittance_closed_form():
+    s = _media_scene(sig_t=2.0)
+    out = np.zeros(7, np.float32)
+    draws = O.lib().gnd_ref_medium(s.h, 0, 0, fp(vec(0, 0, 0)), fp(vec(1, 0, 0)), 0.7, 1, fp(out))
+    assert draws == 0
+    assert np.allclose(out[:3], math.exp(-2.0 * 0.7), rtol=1e-6)
+    O.lib().gnd_ref_medium(s.h, 0, 0, fp(vec(0, 0, 0)), fp(vec(1, 0, 0)), np.inf, 1, fp(out))
+    assert np.all(out[:3] == 0)  # Tr(+inf) = 0 (homogeneous.cpp:23)
+
+
+def test_homogeneous_sampling_is_unbiased():
+    # E[weight * 1{no event}] = Tr(tmax) and P(event) = 1 - Tr for a grey medium
+    s = _media_scene(sig_t=2.0)
+    out = np.zeros(7, np.float32)
+    n, events, wsum = 20000, 0, 0.0
+    for seed in range(n):
+        O.lib().gnd_ref_medium(s.h, 0, 1, fp(vec(0, 0, 0)), fp(vec(1, 0, 0)), 0.5, seed, fp(out))
+        if out[3] > 0:
+            events += 1
+        else:
+            wsum += out[0]
+    tr = math.exp(-1.0)
+    assert abs(events / n - (1 - tr)) < 0.015
+    assert abs(wsum / n - tr) < 0.015
+
+
+def test_grid_ratio_tracking_matches_analytic():
+    # constant density 0.5 * sigma_t 2 -> Tr = exp(-1 * L) inside [0,1]^3
+    s = _media_scene()
+    out = np.zeros(7, np.float32)
+    L = 0.8
+    vals = []
+    for seed in range(4000):
+        O.lib().gnd_ref_medium(s.h, 1, 0, fp(vec(0.1, 0.5, 0.5)), fp(vec(1, 0, 0)), L, seed, fp(out))
+        vals.append(out[0])
+    assert abs(np.mean(vals) - math.exp(-1.0 * L)) < 0.02
+
+
+def test_grid_trilinear_sample():
+    n = 8
+    rng = np.random.default_rng(0)
+    dens = rng.random((n, n, n)).astype(np.float32)
+    s = _media_scene(density=dens, n=n)
+    g = O.lib().gnd_ref_grid_sample
+    # at integer coordinates the sample is the voxel value (x fastest in memory)
+    for (x, y, z) in [(0, 0, 0), (3, 4, 5), (7, 7, 7), (2, 6, 1)]:
+        assert g(s.h, 1, x, y, z) == dens[z, y, x]
+    # midway between two voxels along x: the average
+    assert abs(g(s.h, 1, 2.5, 3, 4) - 0.5 * (dens[4, 3, 2] + dens[4, 3, 3])) < 1e-6
+    # outside the grid: background 0 blends in
+    assert abs(g(s.h, 1, 7.5, 3, 4) - 0.5 * dens[4, 3, 7]) < 1e-6
