@@ -317,6 +317,17 @@ class DeviceScene:
             self.handle = None
 
 
+def camera_lookat(cam: A.gnd_camera_desc, origin, target, up=(0.0, 1.0, 0.0)) -> A.gnd_camera_desc:
+    """A copy of `cam` posed with the reference's lookAt (transform.h:141-144)."""
+    c = A.gnd_camera_desc()
+    C.memmove(C.byref(c), C.byref(cam), C.sizeof(c))
+    f3 = C.c_float * 3
+    rc = _lib().gnd_host_camera_lookat(C.byref(c), f3(*origin), f3(*target), f3(*up))
+    if rc != 0:
+        raise GndError(rc, host_error())
+    return c
+
+
 def resolve(rgbw: np.ndarray) -> np.ndarray:
     """Film::resolve: rgb = sum * (1/weight), 0 where the weight is 0."""
     w = rgbw[..., 3:4]

@@ -39,6 +39,13 @@ void pushTri(MeshData& m, Vec3 a, Vec3 b, Vec3 c, Vec3 n) {
     }
 }
 
+// filter=gaussian keeps the film default (Gaussian, radius 2, alpha 2);
+// anything else, or no key, selects the box filter the workloads use.
+void applyFilter(HostScene& s, const Params& p) {
+    auto it = p.find("filter");
+    if (it == p.end() || it->second != "gaussian") s.camera.filter = GND_FILTER_BOX;
+}
+
 // Quad a-b-c-d (counter-clockwise seen from the side the normal points to)
 void pushQuad(MeshData& m, Vec3 a, Vec3 b, Vec3 c, Vec3 d, Vec3 n) {
     pushTri(m, a, b, c, n);
@@ -216,7 +223,7 @@ std::unique_ptr<HostScene> cornellBox(const Params& p) {
                black, Le);
     int W = (int)num(p, "width", 512), H = (int)num(p, "height", 512);
     s->setCamera(GND_CAMERA_PERSPECTIVE, W, H, 40.0f, xformLookAt(Vec3(0, 0, 3.9f), Vec3(0, 0, 0), Vec3(0, 1, 0)));
-    s->camera.filter = GND_FILTER_BOX;
+    applyFilter(*s, p);
     s->spp = (uint32_t)num(p, "spp", 64);
     s->maxDepth = (int)num(p, "depth", 8);
     s->integrator = GND_INTEGRATOR_PATH;
@@ -274,7 +281,7 @@ std::unique_ptr<HostScene> scaledScene(const Params& p) {
     s->setCamera(GND_CAMERA_THINLENS, W, H, 40.0f, xformLookAt(Vec3(0, 1.2f, 5.5f), Vec3(0, 0, 0), Vec3(0, 1, 0)));
     s->camera.lens_radius = 0.05f;
     s->camera.focal_distance = 5.6f;
-    s->camera.filter = GND_FILTER_BOX;
+    applyFilter(*s, p);
     s->spp = (uint32_t)num(p, "spp", 16);
     s->maxDepth = (int)num(p, "depth", 16);
     s->integrator = GND_INTEGRATOR_PATH;
@@ -303,32 +310,36 @@ std::unique_ptr<HostScene> volumetricScene(const Params& p) {
     int glass = s->addMaterial(materialGlass(1.5f, 0.0f, 0, -1));
     s->addSphere(0.3f, xformTranslate(0.35f, -0.1f, 0.35f), glass);
     // density grid inside the box [-0.85,-0.05] x [-1,-0.2] x [-0.2,0.6]
-    MediumData grid;
-    grid.desc.type = GND_MEDIUM_GRID;
-    for (int c = 0; c < 3; ++c) { grid.desc.sigma_s[c] = 6.4f; grid.desc.sigma_a[c] = 1.6f; }
-    grid.desc.g = 0.0f;
-    grid.desc.nx = grid.desc.ny = grid.desc.nz = n;
-    grid.desc.density_scale = 1.0f;
-    grid.desc.macro_res = 32;
-    grid.density.resize((size_t)n * n * n);
-    ValueNoise noise(seed + 7);
-    for (int z = 0; z < n; ++z)
-        for (int y = 0; y < n; ++y)
-            for (int x = 0; x < n; ++x) {
-                Vec3 u((x + 0.5f) / n, (y + 0.5f) / n, (z + 0.5f) / n);
-                float v = 0.5f + 0.5f * noise.fbm(u, 5, 4.0f);
-                v = (v - 0.35f) * 2.0f;
-                grid.density[((size_t)z * n + y) * n + x] = std::min(1.0f, std::max(0.0f, v));
-            }
-    const float bx0 = -0.85f, by0 = -1.0f, bz0 = -0.2f, ext = 0.8f;
-    // world -> index: (p - b0) * n / ext
-    Xform toLocal = xformTranslate(-bx0, -by0, -bz0).then(xformScale(n / ext, n / ext, n / ext));
-    std::memcpy(grid.desc.to_local, toLocal.m.m, sizeof(toLocal.m.m));
-    std::memcpy(grid.desc.to_local_inv, toLocal.inv.m, sizeof(toLocal.inv.m));
-    s->media.push_back(std::move(grid));
-    int index = s->addMaterial(materialIndex(1, -1));
-    Xform boxX = xformScale(ext, ext, ext).then(xformTranslate(bx0 + ext * 0.5f, by0 + ext * 0.5f, bz0 + ext * 0.5f));
-    s->addPrim(s->addMesh(cubeMesh()), boxX, index);
+    // (grid=0 leaves it out: homogeneous medium only)
+    if (n > 0) {
+        MediumData grid;
+        grid.desc.type = GND_MEDIUM_GRID;
+        for (int c = 0; c < 3; ++c) { grid.desc.sigma_s[c] = 6.4f; grid.desc.sigma_a[c] = 1.6f; }
+        grid.desc.g = 0.0f;
+        grid.desc.nx = grid.desc.ny = grid.desc.nz = n;
+        grid.desc.density_scale = 1.0f;
+        grid.desc.macro_res = 32;
+        grid.density.resize((size_t)n * n * n);
+        ValueNoise noise(seed + 7);
+        for (int z = 0; z < n; ++z)
+            for (int y = 0; y < n; ++y)
+                for (int x = 0; x < n; ++x) {
+                    Vec3 u((x + 0.5f) / n, (y + 0.5f) / n, (z + 0.5f) / n);
+                    float v = 0.5f + 0.5f * noise.fbm(u, 5, 4.0f);
+                    v = (v - 0.35f) * 2.0f;
+                    grid.density[((size_t)z * n + y) * n + x] = std::min(1.0f, std::max(0.0f, v));
+                }
+        const float bx0 = -0.85f, by0 = -1.0f, bz0 = -0.2f, ext = 0.8f;
+        // world -> index: (p - b0) * n / ext
+        Xform toLocal = xformTranslate(-bx0, -by0, -bz0).then(xformScale(n / ext, n / ext, n / ext));
+        std::memcpy(grid.desc.to_local, toLocal.m.m, sizeof(toLocal.m.m));
+        std::memcpy(grid.desc.to_local_inv, toLocal.inv.m, sizeof(toLocal.inv.m));
+        s->media.push_back(std::move(grid));
+        int index = s->addMaterial(materialIndex(1, -1));
+        Xform boxX = xformScale(ext, ext, ext)
+                         .then(xformTranslate(bx0 + ext * 0.5f, by0 + ext * 0.5f, bz0 + ext * 0.5f));
+        s->addPrim(s->addMesh(cubeMesh()), boxX, index);
+    }
     s->spp = (uint32_t)num(p, "spp", 32);
     s->maxDepth = (int)num(p, "depth", 32);
     s->integrator = GND_INTEGRATOR_VOLPATH;
