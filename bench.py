@@ -37,6 +37,22 @@ def algorithmic_bytes(rays, nodes, packs, insts):
     return (RAY_BYTES + HIT_BYTES) * rays + NODE_BYTES * nodes + PACK_BYTES * packs + INST_BYTES * insts
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "kTracePaths_scaled1080p_hbm_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per closest-hit launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes (scaled by the copy-kernel calibration), or None."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(hs, target_s=12.0):
     """Times the CPU restatement (oracle) of the reference integrator on the host
     cores: a centred band of rows of pass 0 grown until it takes target_s, or
@@ -155,8 +171,10 @@ def main():
         trace_ms = sum(s["ms_trace"] for s in per_step) / a.steps
         launches = per_step[0]["trace_launches"]
         achieved = byts / (trace_ms / 1000.0) / 1e9
+        traffic = measured_traffic() if not a.params else None
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT) if traffic else None,
                 "kernel": "kTracePaths (closest-hit BVH4 traversal)",
                 "bytes_per_launch": byts / launches, "avg_launch_ms": trace_ms / launches,
                 "max_nodes_one_ray": cst["max_ray_nodes"], "nonfinite_rays": cst["bad_rays"],

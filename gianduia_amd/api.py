@@ -98,8 +98,30 @@ class Device:
                              cur["use_graph"] if use_graph is None else int(use_graph))
         self._check(_lib().gnd_hip_set_tuning(self.ctx, C.byref(t)))
 
+    def autotune(self, scene: "DeviceScene", job: "Job") -> dict:
+        """gnd_hip_autotune: times one pass of `job` per block size x LDS stack
+        depth and keeps the fastest; returns the chosen tuning."""
+        w, h = job.camera.width, job.camera.height
+        film = self.alloc(w * h * 16)
+        cj = A.gnd_hip_job()
+        cj.camera = job.camera
+        cj.spp = job.spp
+        cj.max_depth = job.max_depth
+        cj.integrator = job.integrator
+        cj.seed_offset = job.seed_offset
+        cj.film = film.addr
+        cj.film_sq = None
+        t = A.gnd_hip_tuning()
+        self._check(_lib().gnd_hip_autotune(self.ctx, scene.handle, C.byref(cj), C.byref(t)))
+        return self.tuning()
+
     def set_counters(self, on: bool) -> None:
         self._check(_lib().gnd_hip_set_counters(self.ctx, int(bool(on))))
+
+    def cancel(self) -> None:
+        """Integrator::cancel: stops a render running on this device from
+        another thread at its next bounce; that render raises GND_E_CANCELLED."""
+        self._check(_lib().gnd_hip_cancel(self.ctx))
 
     def copy_probe(self, nbytes: int, repeats: int = 10) -> float:
         """Average ms of a plain device-to-device streaming copy of nbytes."""

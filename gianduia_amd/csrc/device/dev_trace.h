@@ -13,6 +13,9 @@
 // The per-lane stack lives in LDS (short stack, `LDSD` entries per lane,
 // stride = block size so lanes hit distinct banks) and spills to private
 // scratch beyond that.  TLAS and BLAS entries share one stack.
+//
+// traceStream runs the traversal over a stream of rays so that persistent
+// kernels can hand a lane a new ray as soon as its ray is done.
 #pragma once
 
 #include "dev_common.h"
@@ -153,6 +156,8 @@ GDEV_INLINE uint32_t slab4Fast(const DNode& n, V3 o, V3 inv, float tmin, float t
     return mask;
 }
 
+GDEV_INLINE V3 selV3(bool c, V3 a, V3 b) { return V3{c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z}; }
+
 GDEV_INLINE bool finite3(V3 v) {
     return __builtin_isfinite(v.x) && __builtin_isfinite(v.y) && __builtin_isfinite(v.z);
 }
@@ -184,7 +189,22 @@ struct RangeQueue {
 
 enum : int { ACT_NONE = 0, ACT_PACK = 1, ACT_NODE = 2, ACT_INST = 3 };
 
-// Scene::rayIntersect: closest hit (ANY = false) or any hit (ANY = true).
+// The step kind run by the wave this iteration: the kind most lanes wait for
+// (instance steps first on ties, then pack tests), so the expensive steps run
+// with most of the wave active.  ACT_NONE when no lane has work.
+GDEV_INLINE int pickAction(int act) {
+    const uint64_t mP = __ballot(act == ACT_PACK);
+    const uint64_t mN = __ballot(act == ACT_NODE);
+    const uint64_t mI = __ballot(act == ACT_INST);
+    const int nP = __popcll(mP), nN = __popcll(mN), nI = __popcll(mI);
+    if (nI > 0 && nI >= nN && nI >= nP) return ACT_INST;
+    if (nP > 0 && nP >= nN) return ACT_PACK;
+    if (nN > 0) return ACT_NODE;
+    return ACT_NONE;
+}
+
+// Scene::rayIntersect over a stream of rays: closest hit (ANY = false) or any
+// hit (ANY = true).  All lanes of the wave call this together.
 //
 // Each lane runs the reference traversal as a state machine whose steps are a
 // BVH4 node pop, a TrianglePack4 test, or an instance start / finish.  Per
@@ -192,65 +212,99 @@ enum : int { ACT_NONE = 0, ACT_PACK = 1, ACT_NODE = 2, ACT_INST = 3 };
 // children are tested, in child order, before the next pop; the instances of
 // a TLAS leaf are traversed in order before the next TLAS pop), so results and
 // visit counts are the reference's.  Across the wave, every loop iteration
-// runs one kind of step, chosen by ballot as the kind most lanes are waiting
-// for: lanes with pending triangle packs wait while others pop nodes, and the
-// expensive pack test then runs with most of the wave active.
-template <bool ANY, bool COUNT, int LDSD>
-GDEV_INLINE bool traceRay(const DScene& S, V3 o, V3 d, float tmin, float tmax, HitRec& hit, TravStack<LDSD>& st,
-                          uint32_t* cnt, bool active = true) {
-    hit.inst = -1;
+// runs one kind of step (pickAction).
+//
+// Rays come from `fetch(idle, o, d, tmin, tmax, drained)`, called by the whole
+// wave while `drained` is false: it returns true on the idle lanes it hands a
+// ray and sets `drained` (wave-uniformly) once no ray is left.  A lane whose
+// ray is done reports it through `finish(found, hit, nodePops)` and becomes
+// idle, so a wave does not run at the pace of its slowest ray.
+template <bool ANY, bool COUNT, int LDSD, class Fetch, class Finish>
+GDEV_INLINE void traceStream(const DScene& S, TravStack<LDSD>& st, uint32_t* cnt, Fetch&& fetch,
+                             Finish&& finish) {
+    V3 o = V3{0.0f, 0.0f, 0.0f}, d = o, winv = o;  // world ray
+    float tmin = 0.0f, tcur = 0.0f;  // tcur: the world ray's mutable tMax
+    float best = kFltMax;            // isect.t, starts at FLT_MAX (bvh.cpp:39)
+    V3 lo = o, ld = o, linv = o;     // object-space ray of the current instance
+    float lt = 0.0f, hu = 0.0f, hv = 0.0f;
+    int hp = 0, curInst = -1, level = 0, blasBase = 0;
+    bool instHit = false, found = false, busy = false, done = true, drained = false;
+    uint32_t nodes = 0;
+    HitRec hit;
     hit.t = finf();
-    hit.prim = -1;
     hit.u = hit.v = 0.0f;
-    const V3 winv = V3{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
-    float tcur = tmax;        // the world ray's mutable tMax
-    float best = kFltMax;     // isect.t starts at FLT_MAX (bvh.cpp:39)
-    bool found = false;
-    bool done = !active || S.nTlas == 0;
-    int level = 0, blasBase = 0, curInst = -1;
-    V3 lo = o, ld = d, linv = winv;
-    float lt = tcur, hu = 0.0f, hv = 0.0f;
-    int hp = 0;
-    bool instHit = false;
+    hit.inst = hit.prim = -1;
     RangeQueue tq, bq;
     tq.clear();
     bq.clear();
-    st.sp = 0;
-    if (!done) st.push(0);
     for (;;) {
         int act = ACT_NONE;
-        if (!done) {
-            if (!bq.empty()) act = ACT_PACK;
-            else if (level == 1) act = (st.sp > blasBase) ? ACT_NODE : ACT_INST;
-            else if (!tq.empty()) act = ACT_INST;
-            else if (st.sp > 0) act = ACT_NODE;
-            else done = true;
+        if (busy) {
+            if (!done) {
+                if (!bq.empty()) act = ACT_PACK;
+                else if (level == 1) act = st.sp > blasBase ? ACT_NODE : ACT_INST;
+                else if (!tq.empty()) act = ACT_INST;
+                else if (st.sp > 0) act = ACT_NODE;
+                else done = true;
+            }
+            if (done) {
+                finish(found, hit, nodes);
+                busy = false;
+            }
         }
-        const uint64_t mP = __ballot(act == ACT_PACK);
-        const uint64_t mN = __ballot(act == ACT_NODE);
-        const uint64_t mI = __ballot(act == ACT_INST);
-        if ((mP | mN | mI) == 0) break;
-        const int nP = __popcll(mP), nN = __popcll(mN), nI = __popcll(mI);
-        int pick;
-        if (nI > 0 && nI >= nN && nI >= nP) pick = ACT_INST;
-        else if (nP > 0 && nP >= nN) pick = ACT_PACK;
-        else if (nN > 0) pick = ACT_NODE;
-        else pick = nP > 0 ? ACT_PACK : ACT_INST;
-        if (act != pick) continue;
-        if (pick == ACT_PACK) {
+        if (!drained) {
+            V3 ro, rd;
+            float rtmin, rtmax;
+            if (fetch(!busy, ro, rd, rtmin, rtmax, drained)) {
+                busy = true;
+                o = ro;
+                d = rd;
+                winv = V3{1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z};
+                tmin = rtmin;
+                tcur = rtmax;
+                best = kFltMax;
+                found = false;
+                done = S.nTlas == 0;
+                level = 0;
+                blasBase = 0;
+                instHit = false;
+                nodes = 0;
+                hit.t = finf();
+                hit.u = hit.v = 0.0f;
+                hit.inst = hit.prim = -1;
+                tq.clear();
+                bq.clear();
+                st.sp = 0;
+                if (!done) {
+                    st.push(0);
+                    act = ACT_NODE;
+                }
+            }
+        }
+        if (drained && __ballot(busy) == 0) break;
+        const int pick = pickAction(act);
+        if (pick == ACT_NONE || act != pick) continue;
+        if (act == ACT_PACK) {
             if (COUNT) cnt[1]++;
             const uint32_t p = bq.take();
             if (testPack<ANY>(S.packs, p, lo, ld, lt, hu, hv, hp)) {
-                if (ANY) return true;
-                instHit = true;
+                if (ANY) {
+                    found = true;
+                    done = true;
+                } else {
+                    instHit = true;
+                }
             }
-        } else if (pick == ACT_NODE) {
-            if (COUNT) cnt[0]++;
+        } else if (act == ACT_NODE) {
+            if (COUNT) {
+                cnt[0]++;
+                nodes++;
+            }
             const int ni = st.pop();
             const bool blas = level == 1;
             const DNode n = loadNode(blas ? S.blas : S.tlas, ni);
-            const V3 ro = blas ? lo : o;
-            const V3 ri = blas ? linv : winv;
+            const V3 ro = selV3(blas, lo, o);
+            const V3 ri = selV3(blas, linv, winv);
             const float rmin = blas ? kEps : tmin;
             const float rmax = blas ? lt : tcur;
             const bool fast = __all(finite3(ri));
@@ -267,52 +321,78 @@ GDEV_INLINE bool traceRay(const DScene& S, V3 o, V3 d, float tmin, float tmax, H
                     else tq.add(c, meta & 0x3FFFFFFFu);
                 }
             }
-        } else {  // ACT_INST
-            if (level == 1) {
-                // Primitive::rayIntersect returned: BVH::rayIntersect keeps the hit
-                if (instHit && lt < best) {
+        } else if (level == 1) {
+            // Primitive::rayIntersect returned: BVH::rayIntersect keeps the hit
+            if (instHit && lt < best) {
+                tcur = lt;
+                best = lt;
+                hit.t = lt;
+                hit.u = hu;
+                hit.v = hv;
+                hit.inst = curInst;
+                hit.prim = hp;
+                found = true;
+            }
+            level = 0;
+        } else {
+            const uint32_t slot = tq.take();
+            const int pi = S.tlasPrims[slot];
+            const DInst& I = S.inst[pi];
+            if (COUNT) cnt[2]++;
+            lo = xfPoint(I.worldToObject, o);
+            ld = xfVector(I.worldToObject, d);
+            lt = tcur;
+            curInst = pi;
+            instHit = false;
+            if (I.root >= 0) {
+                linv = V3{1.0f / ld.x, 1.0f / ld.y, 1.0f / ld.z};
+                level = 1;
+                blasBase = st.sp;
+                st.push(I.root);
+            } else if (traceSphere(I.radius, lo, ld, lt)) {
+                if (ANY) {
+                    found = true;
+                    done = true;
+                } else if (lt < best) {
                     tcur = lt;
                     best = lt;
                     hit.t = lt;
-                    hit.u = hu;
-                    hit.v = hv;
-                    hit.inst = curInst;
-                    hit.prim = hp;
+                    hit.u = 0.0f;
+                    hit.v = 0.0f;
+                    hit.inst = pi;
+                    hit.prim = 0;
                     found = true;
-                }
-                level = 0;
-            } else {
-                const uint32_t slot = tq.take();
-                const int pi = S.tlasPrims[slot];
-                const DInst& I = S.inst[pi];
-                if (COUNT) cnt[2]++;
-                lo = xfPoint(I.worldToObject, o);
-                ld = xfVector(I.worldToObject, d);
-                lt = tcur;
-                curInst = pi;
-                instHit = false;
-                if (I.root >= 0) {
-                    linv = V3{1.0f / ld.x, 1.0f / ld.y, 1.0f / ld.z};
-                    level = 1;
-                    blasBase = st.sp;
-                    st.push(I.root);
-                } else if (traceSphere(I.radius, lo, ld, lt)) {
-                    if (ANY) return true;
-                    if (lt < best) {
-                        tcur = lt;
-                        best = lt;
-                        hit.t = lt;
-                        hit.u = 0.0f;
-                        hit.v = 0.0f;
-                        hit.inst = pi;
-                        hit.prim = 0;
-                        found = true;
-                    }
                 }
             }
         }
     }
-    return found;
+}
+
+// One ray per lane (all lanes of the wave call together; inactive lanes pass
+// active = false).  Returns whether a hit was found; `hit` holds the closest.
+template <bool ANY, bool COUNT, int LDSD>
+GDEV_INLINE bool traceRay(const DScene& S, V3 o, V3 d, float tmin, float tmax, HitRec& hit, TravStack<LDSD>& st,
+                          uint32_t* cnt, bool active = true) {
+    bool result = false;
+    hit.t = finf();
+    hit.u = hit.v = 0.0f;
+    hit.inst = hit.prim = -1;
+    traceStream<ANY, COUNT, LDSD>(
+        S, st, cnt,
+        [&](bool idle, V3& ro, V3& rd, float& rtmin, float& rtmax, bool& drained) {
+            drained = true;
+            if (!idle || !active) return false;
+            ro = o;
+            rd = d;
+            rtmin = tmin;
+            rtmax = tmax;
+            return true;
+        },
+        [&](bool found, const HitRec& h, uint32_t) {
+            result = found;
+            hit = h;
+        });
+    return result;
 }
 
 }  // namespace gdev

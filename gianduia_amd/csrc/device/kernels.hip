@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -134,8 +135,13 @@ GDEV_INLINE void flushCounters(Counters* c, const uint32_t* cnt, bool overflow, 
 
 // ============================================================================ verify-mode traversal
 
+// Every wave kernel runs with at most 256 threads per block (the tuning
+// accepts 64, 128 or 256); the bound lets the compiler use more than 128
+// VGPRs instead of spilling to scratch.
+#define GND_WAVE_KERNEL __global__ __launch_bounds__(256)
+
 template <int LDSD, bool ANY, bool COUNT>
-__global__ void kTraceBatch(DScene S, gnd_ray_soa R, int64_t n, gnd_hit_soa H, int32_t* occ, Counters* ctr) {
+GND_WAVE_KERNEL void kTraceBatch(DScene S, gnd_ray_soa R, int64_t n, gnd_hit_soa H, int32_t* occ, Counters* ctr) {
     extern __shared__ int sStack[];
     const int64_t i = (int64_t)xcdBlock() * blockDim.x + threadIdx.x;
     uint32_t cnt[3] = {0, 0, 0};
@@ -232,36 +238,67 @@ __global__ void kCamera(WaveArgs A) {
     if (live) A.qIn[q] = (int32_t)slot;
 }
 
+// Persistent traversal kernels hand out rays with one atomic per wave on a
+// counter zeroed before the launch.  A lane whose ray is done takes a new one
+// as soon as kRefill lanes of its wave are idle (or the whole wave is), so a
+// wave does not run at the pace of its slowest ray.
+constexpr int kRefill = 16;
+constexpr uint32_t kFetchTrace = 3, kFetchShadow = 4;  // counter slots in WaveArgs::counts
+
+// Idle lanes of the wave take the next rays of a queue of n; returns the
+// queue index for this lane, or -1.  Sets `drained` (wave-uniform) once the
+// queue is exhausted.
+GDEV_INLINE int64_t refill(uint32_t* counter, uint32_t n, bool idle, uint64_t waveMask, bool& drained) {
+    const uint64_t m = __ballot(idle);
+    const int k = __popcll(m);
+    if (k == 0 || (k < kRefill && m != waveMask)) return -1;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if ((int)__lane_id() == leader) base = atomicAdd(counter, (uint32_t)k);
+    base = __shfl(base, leader);
+    if ((uint64_t)base + (uint64_t)k >= n) drained = true;
+    if (!idle) return -1;
+    const uint32_t i = base + (uint32_t)__popcll(m & ((1ull << __lane_id()) - 1ull));
+    return i < n ? (int64_t)i : -1;
+}
+
 // closest hit for the live path rays (Scene::rayIntersect at the top of Li)
 template <int LDSD, bool COUNT>
-__global__ void kTracePaths(WaveArgs A) {
+GND_WAVE_KERNEL void kTracePaths(WaveArgs A) {
     extern __shared__ int sStack[];
     const uint32_t n = A.counts[0];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         A.counts[1] = 0u;
         A.counts[2] = 0u;
     }
+    const uint64_t waveMask = __ballot(1);
     uint32_t cnt[3] = {0, 0, 0};
     uint32_t maxRay = 0, bad = 0;
     TravStack<LDSD> st = makeStack<LDSD>(sStack);
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = xcdBlock() * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const int32_t p = A.qIn[i];
-        HitRec h;
-        const PathSoA& P = A.P;
-        const uint32_t before = cnt[0];
-        const V3 dir = V3{P.dx[p], P.dy[p], P.dz[p]};
-        bool r = traceRay<false, COUNT, LDSD>(A.S, V3{P.ox[p], P.oy[p], P.oz[p]}, dir, kEps, finf(), h, st, cnt);
-        if (COUNT) {
-            maxRay = max(maxRay, cnt[0] - before);
-            if (!__builtin_isfinite(dir.x) || !__builtin_isfinite(dir.y) || !__builtin_isfinite(dir.z)) bad++;
-        }
-        A.Hq.t[i] = r ? h.t : finf();
-        A.Hq.inst[i] = r ? h.inst : -1;
-        A.Hq.prim[i] = r ? h.prim : -1;
-        A.Hq.u[i] = h.u;
-        A.Hq.v[i] = h.v;
-    }
+    const PathSoA& P = A.P;
+    int64_t cur = -1;  // queue index of this lane's ray
+    traceStream<false, COUNT, LDSD>(
+        A.S, st, cnt,
+        [&](bool idle, V3& o, V3& d, float& tmin, float& tmax, bool& drained) {
+            const int64_t i = refill(&A.counts[kFetchTrace], n, idle, waveMask, drained);
+            if (i < 0) return false;
+            cur = i;
+            const int32_t p = A.qIn[i];
+            o = V3{P.ox[p], P.oy[p], P.oz[p]};
+            d = V3{P.dx[p], P.dy[p], P.dz[p]};
+            if (COUNT && !finite3(d)) bad++;
+            tmin = kEps;
+            tmax = finf();
+            return true;
+        },
+        [&](bool found, const HitRec& h, uint32_t nodes) {
+            A.Hq.t[cur] = found ? h.t : finf();
+            A.Hq.inst[cur] = found ? h.inst : -1;
+            A.Hq.prim[cur] = found ? h.prim : -1;
+            A.Hq.u[cur] = h.u;
+            A.Hq.v[cur] = h.v;
+            if (COUNT) maxRay = max(maxRay, nodes);
+        });
     if (COUNT) flushCounters(A.ctr, cnt, st.overflow, false, maxRay, bad);
     else if (st.overflow) atomicAdd(&A.ctr->stackOverflow, 1ull);
 }
@@ -489,7 +526,7 @@ GDEV_INLINE int neeStep(const WaveArgs& A, const DJob& J, PathLocal& s, const Hi
 // afterwards (valid when shadow rays consume no random numbers).
 // SPLIT = true: NEE only; the event is stored for kScatter.
 template <bool SPLIT>
-__global__ void kShade(WaveArgs A) {
+GND_WAVE_KERNEL void kShade(WaveArgs A) {
     const uint32_t n = A.counts[0];
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t start = blockIdx.x * blockDim.x;
@@ -559,7 +596,7 @@ __global__ void kShade(WaveArgs A) {
 
 // Split mode second half: the paths that recorded an event scatter now (after
 // their shadow rays consumed transmittance samples).
-__global__ void kScatter(WaveArgs A, const int32_t* qEv, uint32_t nEv) {
+GND_WAVE_KERNEL void kScatter(WaveArgs A, const int32_t* qEv, uint32_t nEv) {
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < nEv; base += stride) {
         const uint32_t i = base + threadIdx.x;
@@ -593,29 +630,47 @@ __global__ void kScatter(WaveArgs A, const int32_t* qEv, uint32_t nEv) {
 // General: closest-hit loop through null-BSDF surfaces with per-segment medium
 // transmittance (path.cpp:122-140 / volPath.cpp:261-291).
 template <int LDSD, bool FAST, bool COUNT>
-__global__ void kShadow(WaveArgs A) {
+GND_WAVE_KERNEL void kShadow(WaveArgs A) {
     extern __shared__ int sStack[];
     const uint32_t n = A.counts[2];
     uint32_t cnt[3] = {0, 0, 0};
     TravStack<LDSD> st = makeStack<LDSD>(sStack);
-    const uint32_t stride = gridDim.x * blockDim.x;
     const DScene& S = A.S;
+    const ShadowSoA& Sh = A.Sh;
+    if constexpr (FAST) {
+        // any-hit rays, persistent with refill like kTracePaths
+        const uint64_t waveMask = __ballot(1);
+        int64_t cur = -1;
+        traceStream<true, COUNT, LDSD>(
+            S, st, cnt,
+            [&](bool idle, V3& o, V3& d, float& tmin, float& tmax, bool& drained) {
+                const int64_t i = refill(&A.counts[kFetchShadow], n, idle, waveMask, drained);
+                if (i < 0) return false;
+                cur = i;
+                o = V3{Sh.ox[i], Sh.oy[i], Sh.oz[i]};
+                d = V3{Sh.dx[i], Sh.dy[i], Sh.dz[i]};
+                tmin = kEps;
+                tmax = Sh.tmax[i];
+                return true;
+            },
+            [&](bool occluded, const HitRec&, uint32_t) {
+                if (occluded) return;
+                const int p = Sh.path[cur];
+                A.P.Lr[p] = A.P.Lr[p] + Sh.pr[cur];
+                A.P.Lg[p] = A.P.Lg[p] + Sh.pg[cur];
+                A.P.Lb[p] = A.P.Lb[p] + Sh.pb[cur];
+            });
+        if (COUNT) flushCounters(A.ctr, cnt, st.overflow, true);
+        else if (st.overflow) atomicAdd(&A.ctr->stackOverflow, 1ull);
+        return;
+    }
+    const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t i = xcdBlock() * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const ShadowSoA& Sh = A.Sh;
         const int p = Sh.path[i];
         V3 o = V3{Sh.ox[i], Sh.oy[i], Sh.oz[i]}, d = V3{Sh.dx[i], Sh.dy[i], Sh.dz[i]};
         float tmax = Sh.tmax[i];
         const V3 pending = V3{Sh.pr[i], Sh.pg[i], Sh.pb[i]};
         HitRec h;
-        if (FAST) {
-            bool occ = traceRay<true, COUNT, LDSD>(S, o, d, kEps, tmax, h, st, cnt);
-            if (!occ) {
-                A.P.Lr[p] = A.P.Lr[p] + pending.x;
-                A.P.Lg[p] = A.P.Lg[p] + pending.y;
-                A.P.Lb[p] = A.P.Lb[p] + pending.z;
-            }
-            continue;
-        }
         const int job = A.P.job[p];
         const bool vol = A.jobs[job].integrator == GND_INTEGRATOR_VOLPATH;
         int medium = Sh.medium[i];
@@ -783,7 +838,7 @@ struct gnd_hip_ctx {
     size_t chunkCap = 0;
     DSplatTask* dTasks = nullptr;
     size_t taskCap = 0;
-    volatile int cancel = 0;
+    std::atomic<int> cancel{0};  // set from any thread by gnd_hip_cancel
     hipEvent_t ev[2] = {nullptr, nullptr};
 };
 
@@ -942,17 +997,20 @@ template <bool ANY, bool COUNT>
 TraceKernel pickTrace(int lds) {
     switch (lds) {
         case 8: return kTraceBatch<8, ANY, COUNT>;
+        case 24: return kTraceBatch<24, ANY, COUNT>;
         case 32: return kTraceBatch<32, ANY, COUNT>;
         default: return kTraceBatch<16, ANY, COUNT>;
     }
 }
-int normLds(int lds) { return lds <= 8 ? 8 : (lds >= 32 ? 32 : 16); }
+// LDS stack depths compiled in: 8, 16, 24, 32 entries per lane
+int normLds(int lds) { return lds <= 8 ? 8 : (lds <= 16 ? 16 : (lds <= 24 ? 24 : 32)); }
 
 using WaveKernel = void (*)(WaveArgs);
 template <bool COUNT>
 WaveKernel pickTracePaths(int lds) {
     switch (lds) {
         case 8: return kTracePaths<8, COUNT>;
+        case 24: return kTracePaths<24, COUNT>;
         case 32: return kTracePaths<32, COUNT>;
         default: return kTracePaths<16, COUNT>;
     }
@@ -961,6 +1019,7 @@ template <bool FAST, bool COUNT>
 WaveKernel pickShadow(int lds) {
     switch (lds) {
         case 8: return kShadow<8, FAST, COUNT>;
+        case 24: return kShadow<24, FAST, COUNT>;
         case 32: return kShadow<32, FAST, COUNT>;
         default: return kShadow<16, FAST, COUNT>;
     }
@@ -1104,6 +1163,9 @@ int gnd_hip_scene_upload(gnd_hip_ctx* ctx, const gnd_hip_scene_arrays* a, gnd_hi
     int rc = guarded(ctx, [&] {
         GND_CHECK(hipSetDevice(ctx->device));
         const gnd_scene_desc& d = *a->desc;
+        // stack entries are non-negative int node indices
+        if (a->n_blas_nodes > 0x7FFFFFFFu || a->n_tlas_nodes > 0x7FFFFFFFu)
+            throw std::runtime_error("scene too large for the traversal stack encoding");
         // TLAS
         std::vector<DNode> tl(a->n_tlas_nodes);
         for (uint32_t i = 0; i < a->n_tlas_nodes; ++i) tl[i] = toDNode(a->tlas_nodes[i], 0, 0);
@@ -1476,6 +1538,9 @@ int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jo
             while (live > 0) {
                 A.qIn = ctx->q[cur];
                 A.qOut = ctx->q[1 - cur];
+                // zero the ray-fetch counters of the two persistent traversals
+                hipLaunchKernelGGL(kFillU32, dim3(1), dim3(64), 0, ctx->stream, ctx->counts + kFetchTrace,
+                                   (size_t)2, 0u);
                 size_t a = mark();
                 hipLaunchKernelGGL(kTrace, dim3(grid(live)), dim3(B), shm, ctx->stream, A);
                 GND_CHECK(hipGetLastError());
@@ -1517,6 +1582,7 @@ int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jo
                     GND_CHECK(hipStreamSynchronize(ctx->stream));
                 }
                 live = hcounts[1];
+                if (ctx->cancel.load()) throw HipError(GND_E_CANCELLED, "render cancelled");
                 // next bounce reads the out-queue; its count moves to slot 0
                 GND_CHECK(hipMemcpyAsync(ctx->counts, ctx->counts + 1, sizeof(uint32_t), hipMemcpyDeviceToDevice,
                                          ctx->stream));
@@ -1587,11 +1653,38 @@ int gnd_hip_copy_probe(gnd_hip_ctx* ctx, size_t bytes, int repeats, double* ms_p
 }
 
 int gnd_hip_autotune(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* job, gnd_hip_tuning* out) {
-    (void)scene;
-    (void)job;
-    if (!ctx || !out) return GND_E_INVALID;
+    if (!ctx || !scene || !job || !out || job->spp == 0) return GND_E_INVALID;
+    // probe: one pass of the job (its film is overwritten), timed on the
+    // device for every block size x LDS stack depth; the fastest is kept
+    gnd_hip_job probe = *job;
+    probe.spp = 1;
+    probe.film_sq = nullptr;
+    const gnd_hip_tuning start = ctx->tuning;
+    gnd_hip_tuning best = start;
+    double bestMs = 1e300;
+    gnd_hip_stats st;
+    int rc = gnd_hip_render(ctx, scene, &probe, 1, &st);  // warm-up: buffers, code objects
+    for (int b : {64, 128, 256}) {
+        for (int l : {8, 16, 24, 32}) {
+            if (rc != GND_OK) break;
+            gnd_hip_tuning t = start;
+            t.block_size = b;
+            t.stack_lds = l;
+            rc = gnd_hip_set_tuning(ctx, &t);
+            if (rc == GND_OK) rc = gnd_hip_render(ctx, scene, &probe, 1, &st);
+            if (rc == GND_OK && st.ms_total < bestMs) {
+                bestMs = st.ms_total;
+                best = t;
+            }
+        }
+    }
+    if (rc != GND_OK) {
+        (void)gnd_hip_set_tuning(ctx, &start);
+        return rc;
+    }
+    rc = gnd_hip_set_tuning(ctx, &best);
     *out = ctx->tuning;
-    return GND_OK;
+    return rc;
 }
 
 }  // extern "C"

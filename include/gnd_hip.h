@@ -127,7 +127,7 @@ typedef struct gnd_hip_stats {
 /* Tunables picked by gnd_hip_autotune or set by the caller. */
 typedef struct gnd_hip_tuning {
     int32_t block_size;       /* threads per block of the traversal kernels (64..256) */
-    int32_t stack_lds;        /* per-lane traversal stack entries held in LDS (8..32) */
+    int32_t stack_lds;        /* per-lane traversal stack entries held in LDS (8/16/24/32) */
     int64_t queue_capacity;   /* paths in flight per wave                            */
     int32_t use_graph;        /* capture the bounce loop in a hipGraph                */
 } gnd_hip_tuning;
@@ -183,8 +183,10 @@ int gnd_hip_cancel(gnd_hip_ctx* ctx);
  * Used to calibrate HBM counters (a known-size copy reads and writes bytes). */
 int gnd_hip_copy_probe(gnd_hip_ctx* ctx, size_t bytes, int repeats, double* ms_per_copy);
 
-/* Picks block size, LDS stack depth and queue capacity by timing the
- * traversal and bounce kernels on the given scene and job. */
+/* Picks the block size (64/128/256) and LDS stack depth (8/16/24/32) by
+ * timing one pass of `job` on `scene` with each pair; keeps the fastest as
+ * the context's tuning and returns it.  job->film is overwritten.  No
+ * counterpart in the reference (its CPU render has no launch shape). */
 int gnd_hip_autotune(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* job, gnd_hip_tuning* out);
 
 #ifdef __cplusplus
