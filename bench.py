@@ -178,6 +178,10 @@ def main():
                 "kernel": "kTracePaths (closest-hit BVH4 traversal)",
                 "bytes_per_launch": byts / launches, "avg_launch_ms": trace_ms / launches,
                 "max_nodes_one_ray": cst["max_ray_nodes"], "nonfinite_rays": cst["bad_rays"],
+                # lane steps / lane slots of the traversal loop (node, pack and
+                # instance steps; an instance costs a start and a finish step)
+                "simd_utilization": round((cst["node_visits"] + cst["pack_tests"] + 2 * cst["inst_tests"])
+                                          / max(1, cst["lane_iterations"]), 4),
                 "visits_per_ray": {"nodes": cst["node_visits"] / cst["rays_closest"],
                                    "packs": cst["pack_tests"] / cst["rays_closest"],
                                    "insts": cst["inst_tests"] / cst["rays_closest"]}}

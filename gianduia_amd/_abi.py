@@ -113,7 +113,8 @@ class gnd_hip_stats(C.Structure):
                 ("rays_shadow", C.c_uint64), ("node_visits", C.c_uint64), ("pack_tests", C.c_uint64),
                 ("inst_tests", C.c_uint64), ("trace_launches", C.c_uint64), ("nan_samples", C.c_uint64),
                 ("shadow_node_visits", C.c_uint64), ("shadow_pack_tests", C.c_uint64),
-                ("shadow_inst_tests", C.c_uint64), ("max_ray_nodes", C.c_uint64), ("bad_rays", C.c_uint64)]
+                ("shadow_inst_tests", C.c_uint64), ("max_ray_nodes", C.c_uint64), ("bad_rays", C.c_uint64),
+                ("lane_iterations", C.c_uint64), ("shadow_lane_iterations", C.c_uint64)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}

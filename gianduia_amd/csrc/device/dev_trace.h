@@ -218,7 +218,9 @@ GDEV_INLINE int pickAction(int act) {
 // wave while `drained` is false: it returns true on the idle lanes it hands a
 // ray and sets `drained` (wave-uniformly) once no ray is left.  A lane whose
 // ray is done reports it through `finish(found, hit, nodePops)` and becomes
-// idle, so a wave does not run at the pace of its slowest ray.
+// idle, so a wave does not run at the pace of its slowest ray.  With COUNT,
+// cnt[0..3] accumulate node pops, pack tests, instance tests and loop
+// iterations.
 template <bool ANY, bool COUNT, int LDSD, class Fetch, class Finish>
 GDEV_INLINE void traceStream(const DScene& S, TravStack<LDSD>& st, uint32_t* cnt, Fetch&& fetch,
                              Finish&& finish) {
@@ -238,6 +240,7 @@ GDEV_INLINE void traceStream(const DScene& S, TravStack<LDSD>& st, uint32_t* cnt
     tq.clear();
     bq.clear();
     for (;;) {
+        if (COUNT) cnt[3]++;
         int act = ACT_NONE;
         if (busy) {
             if (!done) {

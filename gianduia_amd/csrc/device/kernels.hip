@@ -80,6 +80,7 @@ struct Counters {
     unsigned long long snodes, spacks, sinsts;   // shadow / transmittance traversals
     unsigned long long nan, stackOverflow;
     unsigned long long maxRayNodes, badRays;     // worst closest-hit ray, rays with non-finite directions
+    unsigned long long iters, siters;            // lane slots of traversal loop iterations (64 per wave step)
 };
 
 // wave-aggregated append: ballot, popcount, one atomic per wave
@@ -118,17 +119,19 @@ GDEV_INLINE void flushCounters(Counters* c, const uint32_t* cnt, bool overflow, 
                                uint32_t maxRay = 0, uint32_t bad = 0) {
     if (maxRay) atomicMax(&c->maxRayNodes, (unsigned long long)maxRay);
     if (bad) atomicAdd(&c->badRays, (unsigned long long)bad);
-    unsigned long long a = cnt[0], b = cnt[1], d = cnt[2];
+    unsigned long long a = cnt[0], b = cnt[1], d = cnt[2], it = cnt[3];
     // wave reduction then one atomic per wave
     for (int off = 32; off > 0; off >>= 1) {
         a += __shfl_down(a, off);
         b += __shfl_down(b, off);
         d += __shfl_down(d, off);
+        it += __shfl_down(it, off);
     }
     if (__lane_id() == 0) {
         atomicAdd(shadow ? &c->snodes : &c->nodes, a);
         atomicAdd(shadow ? &c->spacks : &c->packs, b);
         atomicAdd(shadow ? &c->sinsts : &c->insts, d);
+        atomicAdd(shadow ? &c->siters : &c->iters, it);
     }
     if (overflow) atomicAdd(&c->stackOverflow, 1ull);
 }
@@ -139,12 +142,18 @@ GDEV_INLINE void flushCounters(Counters* c, const uint32_t* cnt, bool overflow, 
 // accepts 64, 128 or 256); the bound lets the compiler use more than 128
 // VGPRs instead of spilling to scratch.
 #define GND_WAVE_KERNEL __global__ __launch_bounds__(256)
+// Traversal kernels are latency bound: their register budget is capped so
+// that GND_TRACE_WAVES waves fit on each SIMD.
+#ifndef GND_TRACE_WAVES
+#define GND_TRACE_WAVES 4
+#endif
+#define GND_TRACE_KERNEL __global__ __launch_bounds__(256, GND_TRACE_WAVES)
 
 template <int LDSD, bool ANY, bool COUNT>
 GND_WAVE_KERNEL void kTraceBatch(DScene S, gnd_ray_soa R, int64_t n, gnd_hit_soa H, int32_t* occ, Counters* ctr) {
     extern __shared__ int sStack[];
     const int64_t i = (int64_t)xcdBlock() * blockDim.x + threadIdx.x;
-    uint32_t cnt[3] = {0, 0, 0};
+    uint32_t cnt[4] = {0, 0, 0, 0};
     TravStack<LDSD> st = makeStack<LDSD>(sStack);
     if (i < n) {
         HitRec h;
@@ -242,7 +251,10 @@ __global__ void kCamera(WaveArgs A) {
 // counter zeroed before the launch.  A lane whose ray is done takes a new one
 // as soon as kRefill lanes of its wave are idle (or the whole wave is), so a
 // wave does not run at the pace of its slowest ray.
-constexpr int kRefill = 16;
+#ifndef GND_REFILL_LANES
+#define GND_REFILL_LANES 24
+#endif
+constexpr int kRefill = GND_REFILL_LANES;
 constexpr uint32_t kFetchTrace = 3, kFetchShadow = 4;  // counter slots in WaveArgs::counts
 
 // Idle lanes of the wave take the next rays of a queue of n; returns the
@@ -264,7 +276,7 @@ GDEV_INLINE int64_t refill(uint32_t* counter, uint32_t n, bool idle, uint64_t wa
 
 // closest hit for the live path rays (Scene::rayIntersect at the top of Li)
 template <int LDSD, bool COUNT>
-GND_WAVE_KERNEL void kTracePaths(WaveArgs A) {
+GND_TRACE_KERNEL void kTracePaths(WaveArgs A) {
     extern __shared__ int sStack[];
     const uint32_t n = A.counts[0];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -272,7 +284,7 @@ GND_WAVE_KERNEL void kTracePaths(WaveArgs A) {
         A.counts[2] = 0u;
     }
     const uint64_t waveMask = __ballot(1);
-    uint32_t cnt[3] = {0, 0, 0};
+    uint32_t cnt[4] = {0, 0, 0, 0};
     uint32_t maxRay = 0, bad = 0;
     TravStack<LDSD> st = makeStack<LDSD>(sStack);
     const PathSoA& P = A.P;
@@ -626,44 +638,51 @@ GND_WAVE_KERNEL void kScatter(WaveArgs A, const int32_t* qEv, uint32_t nEv) {
     }
 }
 
-// Shadow rays.  FAST: any-hit test (no null-BSDF surfaces, no transmittance).
-// General: closest-hit loop through null-BSDF surfaces with per-segment medium
-// transmittance (path.cpp:122-140 / volPath.cpp:261-291).
-template <int LDSD, bool FAST, bool COUNT>
-GND_WAVE_KERNEL void kShadow(WaveArgs A) {
+// Shadow rays when no surface is a null BSDF and no segment can cross a
+// medium: one any-hit test each, persistent with refill like kTracePaths.
+template <int LDSD, bool COUNT>
+__global__ __launch_bounds__(256, GND_TRACE_WAVES + 1) void kShadowAny(WaveArgs A) {
     extern __shared__ int sStack[];
     const uint32_t n = A.counts[2];
-    uint32_t cnt[3] = {0, 0, 0};
+    uint32_t cnt[4] = {0, 0, 0, 0};
     TravStack<LDSD> st = makeStack<LDSD>(sStack);
     const DScene& S = A.S;
     const ShadowSoA& Sh = A.Sh;
-    if constexpr (FAST) {
-        // any-hit rays, persistent with refill like kTracePaths
-        const uint64_t waveMask = __ballot(1);
-        int64_t cur = -1;
-        traceStream<true, COUNT, LDSD>(
-            S, st, cnt,
-            [&](bool idle, V3& o, V3& d, float& tmin, float& tmax, bool& drained) {
-                const int64_t i = refill(&A.counts[kFetchShadow], n, idle, waveMask, drained);
-                if (i < 0) return false;
-                cur = i;
-                o = V3{Sh.ox[i], Sh.oy[i], Sh.oz[i]};
-                d = V3{Sh.dx[i], Sh.dy[i], Sh.dz[i]};
-                tmin = kEps;
-                tmax = Sh.tmax[i];
-                return true;
-            },
-            [&](bool occluded, const HitRec&, uint32_t) {
-                if (occluded) return;
-                const int p = Sh.path[cur];
-                A.P.Lr[p] = A.P.Lr[p] + Sh.pr[cur];
-                A.P.Lg[p] = A.P.Lg[p] + Sh.pg[cur];
-                A.P.Lb[p] = A.P.Lb[p] + Sh.pb[cur];
-            });
-        if (COUNT) flushCounters(A.ctr, cnt, st.overflow, true);
-        else if (st.overflow) atomicAdd(&A.ctr->stackOverflow, 1ull);
-        return;
-    }
+    const uint64_t waveMask = __ballot(1);
+    int64_t cur = -1;
+    traceStream<true, COUNT, LDSD>(
+        S, st, cnt,
+        [&](bool idle, V3& o, V3& d, float& tmin, float& tmax, bool& drained) {
+            const int64_t i = refill(&A.counts[kFetchShadow], n, idle, waveMask, drained);
+            if (i < 0) return false;
+            cur = i;
+            o = V3{Sh.ox[i], Sh.oy[i], Sh.oz[i]};
+            d = V3{Sh.dx[i], Sh.dy[i], Sh.dz[i]};
+            tmin = kEps;
+            tmax = Sh.tmax[i];
+            return true;
+        },
+        [&](bool occluded, const HitRec&, uint32_t) {
+            if (occluded) return;
+            const int p = Sh.path[cur];
+            A.P.Lr[p] = A.P.Lr[p] + Sh.pr[cur];
+            A.P.Lg[p] = A.P.Lg[p] + Sh.pg[cur];
+            A.P.Lb[p] = A.P.Lb[p] + Sh.pb[cur];
+        });
+    if (COUNT) flushCounters(A.ctr, cnt, st.overflow, true);
+    else if (st.overflow) atomicAdd(&A.ctr->stackOverflow, 1ull);
+}
+
+// General shadow rays: closest-hit loop through null-BSDF surfaces with
+// per-segment medium transmittance (path.cpp:122-140 / volPath.cpp:261-291).
+template <int LDSD, bool COUNT>
+GND_WAVE_KERNEL void kShadow(WaveArgs A) {
+    extern __shared__ int sStack[];
+    const uint32_t n = A.counts[2];
+    uint32_t cnt[4] = {0, 0, 0, 0};
+    TravStack<LDSD> st = makeStack<LDSD>(sStack);
+    const DScene& S = A.S;
+    const ShadowSoA& Sh = A.Sh;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t i = xcdBlock() * blockDim.x + threadIdx.x; i < n; i += stride) {
         const int p = Sh.path[i];
@@ -1018,10 +1037,10 @@ WaveKernel pickTracePaths(int lds) {
 template <bool FAST, bool COUNT>
 WaveKernel pickShadow(int lds) {
     switch (lds) {
-        case 8: return kShadow<8, FAST, COUNT>;
-        case 24: return kShadow<24, FAST, COUNT>;
-        case 32: return kShadow<32, FAST, COUNT>;
-        default: return kShadow<16, FAST, COUNT>;
+        case 8: return FAST ? kShadowAny<8, COUNT> : kShadow<8, COUNT>;
+        case 24: return FAST ? kShadowAny<24, COUNT> : kShadow<24, COUNT>;
+        case 32: return FAST ? kShadowAny<32, COUNT> : kShadow<32, COUNT>;
+        default: return FAST ? kShadowAny<16, COUNT> : kShadow<16, COUNT>;
     }
 }
 
@@ -1484,7 +1503,10 @@ int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jo
                                     : (cnt ? pickShadow<false, true>(lds) : pickShadow<false, false>(lds));
         // enough blocks for one item per thread up to 64 waves per CU (the
         // traversal is latency bound: occupancy hides node-fetch latency)
-        const unsigned gridCap = (unsigned)std::max(8, ctx->numCUs * 64 * 64 / B);
+#ifndef GND_GRID_WAVES_PER_CU
+#define GND_GRID_WAVES_PER_CU 64
+#endif
+        const unsigned gridCap = (unsigned)std::max(8, ctx->numCUs * GND_GRID_WAVES_PER_CU * 64 / B);
         auto grid = [&](uint32_t n) {
             unsigned b = (unsigned)((n + B - 1) / B);
             b = std::min(b, gridCap);
@@ -1618,6 +1640,8 @@ int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jo
             stats->shadow_inst_tests = c.sinsts;
             stats->max_ray_nodes = c.maxRayNodes;
             stats->bad_rays = c.badRays;
+            stats->lane_iterations = c.iters;
+            stats->shadow_lane_iterations = c.siters;
             stats->trace_launches = launches;
             stats->nan_samples = c.nan;
         }

@@ -122,6 +122,8 @@ typedef struct gnd_hip_stats {
     uint64_t shadow_inst_tests;
     uint64_t max_ray_nodes; /* node pops of the worst closest-hit ray (counters on) */
     uint64_t bad_rays;      /* rays with a non-finite direction (counters on)       */
+    uint64_t lane_iterations;        /* traversal loop iterations x 64 lanes, closest hit (counters on) */
+    uint64_t shadow_lane_iterations; /* the same for shadow rays (counters on)                        */
 } gnd_hip_stats;
 
 /* Tunables picked by gnd_hip_autotune or set by the caller. */
