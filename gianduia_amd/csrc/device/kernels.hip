@@ -34,6 +34,7 @@ struct PathSoA {
     uint64_t* rng;
     uint32_t *c0, *c1, *flags;  // c0: bounces | nullBounces<<16; c1: surface | volume bounces<<16
     int32_t* job;
+    int32_t* chunk;  // engine chunk of the slot (continuous batching)
     // event record between the NEE and scatter kernels (split mode)
     int32_t *evKind, *evPrim, *evMedium;
     float *evpx, *evpy, *evpz, *evnx, *evny, *evnz, *evtx, *evty, *evtz, *evu, *evv;
@@ -59,9 +60,11 @@ struct DJob {
     int32_t maxDepth, integrator, W, H;
 };
 
-struct DChunk {     // one pass of one job inside a wave
+struct DChunk {     // one pass of one job: npix path slots from slotBase
     uint32_t slotBase, npix;
     int32_t job, pass;
+    uint32_t first;  // index of its first path in the camera launch that starts it
+    int32_t id;      // engine chunk id (continuous batching), -1 otherwise
 };
 
 struct DSplatTask { // one job inside a wave: its chunks in pass order
@@ -81,7 +84,22 @@ struct Counters {
     unsigned long long nan, stackOverflow;
     unsigned long long maxRayNodes, badRays;     // worst closest-hit ray, rays with non-finite directions
     unsigned long long iters, siters;            // lane slots of traversal loop iterations (64 per wave step)
+    unsigned long long rays, srays, launches;    // closest-hit and shadow queries, bounces with work
 };
+
+// Paths that end leave their engine chunk (continuous batching): one atomic
+// per distinct chunk among the wave's ending lanes.  All lanes call this.
+GDEV_INLINE void retirePaths(uint32_t* chunkLive, bool ending, int chunk) {
+    if (!chunkLive) return;
+    uint64_t m = __ballot(ending);
+    while (m) {
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        const int c = __shfl(chunk, leader);
+        const uint64_t same = __ballot(ending && chunk == c);
+        if ((int)__lane_id() == leader) atomicSub(&chunkLive[c], (uint32_t)__popcll(same));
+        m &= ~same;
+    }
+}
 
 // wave-aggregated append: ballot, popcount, one atomic per wave
 GDEV_INLINE uint32_t waveAppend(uint32_t* counter, bool pred) {
@@ -186,7 +204,8 @@ struct WaveArgs {
     ShadowSoA Sh;
     int32_t* qIn;
     int32_t* qOut;
-    uint32_t* counts;     // [0] in, [1] out, [2] shadow
+    uint32_t* counts;     // [0] in, [1] out, [2] shadow, [3] [4] ray fetch
+    uint32_t* chunkLive;  // live paths per engine chunk, or null
     Counters* ctr;
     int32_t pathOnlyFast; // shadows may use any-hit
     int32_t split;        // NEE and scatter in separate kernels
@@ -194,19 +213,21 @@ struct WaveArgs {
 
 // SamplerIntegrator::render per-sample set-up (integrator.h:88-125): seeding,
 // film jitter, lens sample, chromatic-aberration channel, time, camera ray.
+// One thread per new path: t in [0, nSlots) over the chunks of this launch
+// (sorted by `first`); the path goes to slot slotBase + (t - first).
 __global__ void kCamera(WaveArgs A) {
-    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-    if (slot >= A.nSlots) return;
-    // chunk lookup (chunks are sorted by slotBase)
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= A.nSlots) return;
     int lo = 0, hi = A.nChunks - 1;
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
-        if (A.chunks[mid].slotBase <= slot) lo = mid;
+        if (A.chunks[mid].first <= t) lo = mid;
         else hi = mid - 1;
     }
     const DChunk ch = A.chunks[lo];
     const DJob& J = A.jobs[ch.job];
-    const uint32_t pix = slot - ch.slotBase;
+    const uint32_t pix = t - ch.first;
+    const uint32_t slot = ch.slotBase + pix;
     const int W = J.W, H = J.H;
     const int x = (int)(pix % (uint32_t)W), y = (int)(pix / (uint32_t)W);
     const uint64_t pixelIdx = (uint64_t)(y * W + x);
@@ -242,6 +263,7 @@ __global__ void kCamera(WaveArgs A) {
     P.c1[slot] = 0u;
     P.flags[slot] = F_SPECULAR | ((uint32_t)(chan + 1) << 1);  // medium -1 -> field 0
     P.job[slot] = ch.job;
+    P.chunk[slot] = ch.id;
     const bool live = J.maxDepth > 0;
     const uint32_t q = waveAppend(&A.counts[0], live);
     if (live) A.qIn[q] = (int32_t)slot;
@@ -279,10 +301,6 @@ template <int LDSD, bool COUNT>
 GND_TRACE_KERNEL void kTracePaths(WaveArgs A) {
     extern __shared__ int sStack[];
     const uint32_t n = A.counts[0];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        A.counts[1] = 0u;
-        A.counts[2] = 0u;
-    }
     const uint64_t waveMask = __ballot(1);
     uint32_t cnt[4] = {0, 0, 0, 0};
     uint32_t maxRay = 0, bad = 0;
@@ -545,7 +563,7 @@ GND_WAVE_KERNEL void kShade(WaveArgs A) {
     for (uint32_t base = start; base < n; base += stride) {
         const uint32_t i = base + threadIdx.x;
         const bool active = i < n;
-        bool alive = false, shadow = false;
+        bool alive = false, shadow = false, ending = false;
         int p = -1;
         PathLocal s;
         V3 so, sd, pending = sameV(0.0f), lightP;
@@ -589,7 +607,9 @@ GND_WAVE_KERNEL void kShade(WaveArgs A) {
             }
             if (SPLIT && r == 1) A.P.evKind[p] = EV_NONE;
             storePath(A.P, p, s);
+            ending = !alive && !(SPLIT && r == 2);
         }
+        retirePaths(A.chunkLive, ending, ending && A.chunkLive ? A.P.chunk[p] : -1);
         const uint32_t qs = waveAppend(&A.counts[1], alive);
         if (alive) A.qOut[qs] = p;
         const uint32_t ss = waveAppend(&A.counts[2], shadow);
@@ -608,11 +628,13 @@ GND_WAVE_KERNEL void kShade(WaveArgs A) {
 
 // Split mode second half: the paths that recorded an event scatter now (after
 // their shadow rays consumed transmittance samples).
-GND_WAVE_KERNEL void kScatter(WaveArgs A, const int32_t* qEv, uint32_t nEv) {
+GND_WAVE_KERNEL void kScatter(WaveArgs A) {
+    const uint32_t nEv = A.counts[0];
+    const int32_t* qEv = A.qIn;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < nEv; base += stride) {
         const uint32_t i = base + threadIdx.x;
-        bool alive = false;
+        bool alive = false, ending = false;
         int p = -1;
         if (i < nEv) {
             p = qEv[i];
@@ -631,8 +653,10 @@ GND_WAVE_KERNEL void kScatter(WaveArgs A, const int32_t* qEv, uint32_t nEv) {
                 ev.v = A.P.evv[p];
                 alive = scatterStep(A.S, A.jobs[s.job], s, ev);
                 storePath(A.P, p, s);
+                ending = !alive;
             }
         }
+        retirePaths(A.chunkLive, ending, ending && A.chunkLive ? A.P.chunk[p] : -1);
         const uint32_t qs = waveAppend(&A.counts[1], alive);
         if (alive) A.qOut[qs] = p;
     }
@@ -819,6 +843,22 @@ __global__ void kFillU32(uint32_t* p, size_t n, uint32_t v) {
     if (i < n) p[i] = v;
 }
 
+// End of a bounce: the out-queue becomes the next in-queue; out, shadow and
+// ray-fetch counters restart at zero (a kernel, so the step can be captured
+// in a graph).
+__global__ void kRotate(uint32_t* counts, Counters* c) {
+    if (threadIdx.x == 0) {
+        c->rays += counts[0];
+        c->srays += counts[2];
+        if (counts[0]) c->launches += 1ull;
+        counts[0] = counts[1];
+        counts[1] = 0u;
+        counts[2] = 0u;
+        counts[3] = 0u;
+        counts[4] = 0u;
+    }
+}
+
 // ============================================================================ host side
 
 struct gnd_hip_scene {
@@ -990,6 +1030,7 @@ void ensureWaveBuffers(gnd_hip_ctx* ctx, size_t cap) {
     P.c1 = (uint32_t*)take(4);
     P.flags = (uint32_t*)take(4);
     P.job = (int32_t*)take(4);
+    P.chunk = (int32_t*)take(4);
     P.evKind = (int32_t*)take(4);
     P.evPrim = (int32_t*)take(4);
     P.evMedium = (int32_t*)take(4);
@@ -1414,6 +1455,7 @@ struct WavePlan {
     std::vector<DChunk> chunks;
     std::vector<DSplatTask> tasks;
     uint32_t slots = 0, pix = 0;
+    int32_t maxDepth = 0;
 };
 
 // Packs passes of the jobs into waves of at most `cap` slots.  A job's passes
@@ -1430,7 +1472,7 @@ static std::vector<WavePlan> planWaves(const std::vector<gnd_hip_job>& jobs, siz
             if (next[j] >= jobs[j].spp) continue;
             int first = (int)w.chunks.size();
             while (next[j] < jobs[j].spp && (w.slots + npix <= cap || (w.slots == 0))) {
-                w.chunks.push_back({w.slots, npix, (int32_t)j, (int32_t)next[j]});
+                w.chunks.push_back({w.slots, npix, (int32_t)j, (int32_t)next[j], w.slots, -1});
                 w.slots += npix;
                 next[j]++;
                 any = true;
@@ -1439,6 +1481,7 @@ static std::vector<WavePlan> planWaves(const std::vector<gnd_hip_job>& jobs, siz
             if ((int)w.chunks.size() > first) {
                 w.tasks.push_back({w.pix, (int32_t)j, first, (int32_t)w.chunks.size()});
                 w.pix += npix;
+                w.maxDepth = std::max(w.maxDepth, jobs[j].max_depth);
             }
             if (w.slots >= cap) break;
         }
@@ -1446,6 +1489,151 @@ static std::vector<WavePlan> planWaves(const std::vector<gnd_hip_job>& jobs, siz
         waves.push_back(std::move(w));
     }
     return waves;
+}
+
+// Kernels and launch shape of a bounce for one scene and job mix.
+struct BouncePlan {
+    WaveKernel trace = nullptr, shadow = nullptr;
+    bool split = false, fastShadow = false;
+    int B = 128;
+    size_t shm = 0;
+    unsigned gridCap = 8;
+    // blocks for n queue entries: one entry per thread up to the grid cap
+    // (the traversal and shading kernels loop over their queue)
+    unsigned grid(uint32_t n) const {
+        unsigned b = (unsigned)((n + (uint32_t)B - 1) / (uint32_t)B);
+        b = std::min(b, gridCap);
+        return std::max(8u, (b + 7) / 8 * 8);
+    }
+};
+
+static BouncePlan makeBouncePlan(const gnd_hip_ctx* ctx, const gnd_hip_scene* scene, bool anyVol) {
+    BouncePlan K;
+    // the split NEE / scatter kernels keep transmittance sampling in the
+    // reference order when shadow rays cross a density grid
+    K.split = anyVol && scene->hasGrid;
+    // any-hit shadows are exact when no surface is a null BSDF and no
+    // shadow segment can cross a medium (the path integrator ignores media)
+    K.fastShadow = !scene->hasIndex && (!anyVol || scene->nMedia == 0);
+    K.B = ctx->tuning.block_size;
+    const int lds = normLds(ctx->tuning.stack_lds);
+    K.shm = (size_t)K.B * lds * sizeof(int);
+    const bool cnt = ctx->counters;
+    K.trace = cnt ? pickTracePaths<true>(lds) : pickTracePaths<false>(lds);
+    K.shadow = K.fastShadow ? (cnt ? pickShadow<true, true>(lds) : pickShadow<true, false>(lds))
+                            : (cnt ? pickShadow<false, true>(lds) : pickShadow<false, false>(lds));
+#ifndef GND_GRID_WAVES_PER_CU
+#define GND_GRID_WAVES_PER_CU 64
+#endif
+    // up to 64 waves per CU in flight: the traversal is latency bound
+    K.gridCap = (unsigned)std::max(8, ctx->numCUs * GND_GRID_WAVES_PER_CU * 64 / K.B);
+    return K;
+}
+
+// Device-timed spans of the render stream (kind 0 closest hit, 1 shadow,
+// 2 camera / shading / film).
+struct Timeline {
+    hipStream_t stream;
+    std::vector<hipEvent_t> evs;
+    struct Span { size_t a, b; int kind; };
+    std::vector<Span> spans;
+    explicit Timeline(hipStream_t s) : stream(s) {}
+    ~Timeline() {
+        for (auto e : evs) (void)hipEventDestroy(e);
+    }
+    size_t mark() {
+        hipEvent_t e;
+        GND_CHECK(hipEventCreate(&e));
+        GND_CHECK(hipEventRecord(e, stream));
+        evs.push_back(e);
+        return evs.size() - 1;
+    }
+    void lap(size_t& since, int kind) {
+        const size_t now = mark();
+        spans.push_back({since, now, kind});
+        since = now;
+    }
+    void sum(double& trace, double& shadow, double& other) const {
+        for (const Span& sp : spans) {
+            const double ms = elapsedMs(evs[sp.a], evs[sp.b]);
+            (sp.kind == 0 ? trace : sp.kind == 1 ? shadow : other) += ms;
+        }
+    }
+};
+
+// One bounce over the in-queue A.qIn (count in counts[0], at most `live`):
+// closest hit, shading and NEE, shadow rays, the split scatter, then the queue
+// rotation.  No host synchronisation, so the chain can be captured.
+static void launchBounce(gnd_hip_ctx* ctx, const WaveArgs& A, const BouncePlan& K, uint32_t live, Timeline* tl) {
+    const unsigned g = K.grid(live);
+    size_t t = tl ? tl->mark() : 0;
+    hipLaunchKernelGGL(K.trace, dim3(g), dim3(K.B), K.shm, ctx->stream, A);
+    GND_CHECK(hipGetLastError());
+    if (tl) tl->lap(t, 0);
+    if (K.split) hipLaunchKernelGGL(kShade<true>, dim3(g), dim3(K.B), 0, ctx->stream, A);
+    else hipLaunchKernelGGL(kShade<false>, dim3(g), dim3(K.B), 0, ctx->stream, A);
+    GND_CHECK(hipGetLastError());
+    if (tl) tl->lap(t, 2);
+    // NEE emits at most one shadow ray per path
+    hipLaunchKernelGGL(K.shadow, dim3(g), dim3(K.B), K.shm, ctx->stream, A);
+    GND_CHECK(hipGetLastError());
+    if (tl) tl->lap(t, 1);
+    if (K.split) {
+        // the paths of this bounce with an event scatter after their shadow
+        // rays consumed the transmittance samples
+        hipLaunchKernelGGL(kScatter, dim3(g), dim3(K.B), 0, ctx->stream, A);
+        GND_CHECK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(kRotate, dim3(1), dim3(64), 0, ctx->stream, A.counts, A.ctr);
+    GND_CHECK(hipGetLastError());
+    if (tl) tl->lap(t, 2);
+}
+
+static uint32_t readLive(gnd_hip_ctx* ctx) {
+    uint32_t live = 0;
+    GND_CHECK(hipMemcpyAsync(&live, ctx->counts, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    GND_CHECK(hipStreamSynchronize(ctx->stream));
+    return live;
+}
+
+static DJob makeDJob(const gnd_hip_job& j) {
+    DJob J;
+    J.cam = makeCamera(j.camera);
+    J.film = j.film;
+    J.filmSq = j.film_sq;
+    J.seedOffset = j.seed_offset;
+    J.maxDepth = j.max_depth;
+    J.integrator = j.integrator;
+    J.W = j.camera.width;
+    J.H = j.camera.height;
+    return J;
+}
+
+static void clearFilm(gnd_hip_ctx* ctx, const DJob& J) {
+    const size_t np = (size_t)J.W * J.H;
+    hipLaunchKernelGGL(kFill, dim3((unsigned)((np * 4 + 255) / 256)), dim3(256), 0, ctx->stream, J.film, np * 4,
+                       0.0f);
+    if (J.filmSq)
+        hipLaunchKernelGGL(kFill, dim3((unsigned)((np * 3 + 255) / 256)), dim3(256), 0, ctx->stream, J.filmSq,
+                           np * 3, 0.0f);
+}
+
+static void fillStats(gnd_hip_stats* stats, const Counters& c, uint64_t samples) {
+    stats->samples = samples;
+    stats->rays_closest = c.rays;
+    stats->rays_shadow = c.srays;
+    stats->trace_launches = c.launches;
+    stats->node_visits = c.nodes;
+    stats->pack_tests = c.packs;
+    stats->inst_tests = c.insts;
+    stats->shadow_node_visits = c.snodes;
+    stats->shadow_pack_tests = c.spacks;
+    stats->shadow_inst_tests = c.sinsts;
+    stats->max_ray_nodes = c.maxRayNodes;
+    stats->bad_rays = c.badRays;
+    stats->lane_iterations = c.iters;
+    stats->shadow_lane_iterations = c.siters;
+    stats->nan_samples = c.nan;
 }
 
 int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jobsIn, int njobs,
@@ -1457,75 +1645,33 @@ int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jo
         GND_CHECK(hipSetDevice(ctx->device));
         std::vector<gnd_hip_job> jobs(jobsIn, jobsIn + njobs);
         size_t maxPix = 0;
+        bool anyVol = false;
+        uint64_t samples = 0;
         for (auto& j : jobs) {
             if (j.camera.width <= 0 || j.camera.height <= 0 || !j.film)
                 throw std::runtime_error("job needs a positive resolution and a film");
             maxPix = std::max(maxPix, (size_t)j.camera.width * j.camera.height);
+            anyVol = anyVol || j.integrator == GND_INTEGRATOR_VOLPATH;
+            samples += (uint64_t)j.camera.width * j.camera.height * j.spp;
         }
         const size_t cap = std::max<size_t>((size_t)ctx->tuning.queue_capacity, maxPix);
         if (cap > 0x7FFFFFFFull) throw std::runtime_error("wave capacity too large");
         ensureWaveBuffers(ctx, cap);
         std::vector<DJob> dj(jobs.size());
         for (size_t j = 0; j < jobs.size(); ++j) {
-            DJob& J = dj[j];
-            J.cam = makeCamera(jobs[j].camera);
-            J.film = jobs[j].film;
-            J.filmSq = jobs[j].film_sq;
-            J.seedOffset = jobs[j].seed_offset;
-            J.maxDepth = jobs[j].max_depth;
-            J.integrator = jobs[j].integrator;
-            J.W = jobs[j].camera.width;
-            J.H = jobs[j].camera.height;
-            const size_t np = (size_t)J.W * J.H;
-            hipLaunchKernelGGL(kFill, dim3((unsigned)((np * 4 + 255) / 256)), dim3(256), 0, ctx->stream, J.film,
-                               np * 4, 0.0f);
-            if (J.filmSq)
-                hipLaunchKernelGGL(kFill, dim3((unsigned)((np * 3 + 255) / 256)), dim3(256), 0, ctx->stream,
-                                   J.filmSq, np * 3, 0.0f);
+            dj[j] = makeDJob(jobs[j]);
+            clearFilm(ctx, dj[j]);
         }
         ensureDev(ctx->dJobs, ctx->jobCap, dj.size());
         GND_CHECK(hipMemcpyAsync(ctx->dJobs, dj.data(), dj.size() * sizeof(DJob), hipMemcpyHostToDevice,
                                  ctx->stream));
         GND_CHECK(hipMemsetAsync(ctx->dCounters, 0, sizeof(Counters), ctx->stream));
         const std::vector<WavePlan> waves = planWaves(jobs, cap);
-        bool anyVolGrid = false, anyVol = false;
-        for (auto& j : jobs) anyVol = anyVol || j.integrator == GND_INTEGRATOR_VOLPATH;
-        anyVolGrid = anyVol && scene->hasGrid;
-        const bool split = anyVolGrid;
-        // any-hit shadows are exact when no surface is a null BSDF and no
-        // shadow segment can cross a medium (path integrator ignores media)
-        const bool fastShadow = !scene->hasIndex && (!anyVol || scene->nMedia == 0);
-        const int B = ctx->tuning.block_size, lds = normLds(ctx->tuning.stack_lds);
-        const size_t shm = (size_t)B * lds * sizeof(int);
-        const bool cnt = ctx->counters;
-        WaveKernel kTrace = cnt ? pickTracePaths<true>(lds) : pickTracePaths<false>(lds);
-        WaveKernel kSh = fastShadow ? (cnt ? pickShadow<true, true>(lds) : pickShadow<true, false>(lds))
-                                    : (cnt ? pickShadow<false, true>(lds) : pickShadow<false, false>(lds));
-        // enough blocks for one item per thread up to 64 waves per CU (the
-        // traversal is latency bound: occupancy hides node-fetch latency)
-#ifndef GND_GRID_WAVES_PER_CU
-#define GND_GRID_WAVES_PER_CU 64
-#endif
-        const unsigned gridCap = (unsigned)std::max(8, ctx->numCUs * GND_GRID_WAVES_PER_CU * 64 / B);
-        auto grid = [&](uint32_t n) {
-            unsigned b = (unsigned)((n + B - 1) / B);
-            b = std::min(b, gridCap);
-            b = std::max(8u, (b + 7) / 8 * 8);
-            return b;
-        };
-        std::vector<hipEvent_t> evs;
-        auto mark = [&]() {
-            hipEvent_t e;
-            GND_CHECK(hipEventCreate(&e));
-            GND_CHECK(hipEventRecord(e, ctx->stream));
-            evs.push_back(e);
-            return evs.size() - 1;
-        };
-        struct Span { size_t a, b; int kind; };  // 0 trace, 1 shadow, 2 other
-        std::vector<Span> spans;
-        uint64_t rays = 0, shadowRays = 0, launches = 0, samples = 0;
-        for (auto& j : jobs) samples += (uint64_t)j.camera.width * j.camera.height * j.spp;
-        const size_t t0 = mark();
+        const BouncePlan K = makeBouncePlan(ctx, scene, anyVol);
+        const bool graph = ctx->tuning.use_graph != 0;
+        Timeline tl(ctx->stream);
+        Timeline* tlp = graph ? nullptr : &tl;
+        const size_t t0 = tl.mark();
         for (const WavePlan& w : waves) {
             if (ctx->cancel) throw HipError(GND_E_CANCELLED, "render cancelled");
             ensureDev(ctx->dChunks, ctx->chunkCap, w.chunks.size());
@@ -1534,7 +1680,7 @@ int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jo
                                      hipMemcpyHostToDevice, ctx->stream));
             GND_CHECK(hipMemcpyAsync(ctx->dTasks, w.tasks.data(), w.tasks.size() * sizeof(DSplatTask),
                                      hipMemcpyHostToDevice, ctx->stream));
-            hipLaunchKernelGGL(kFillU32, dim3(1), dim3(16), 0, ctx->stream, ctx->counts, (size_t)16, 0u);
+            hipLaunchKernelGGL(kFillU32, dim3(1), dim3(64), 0, ctx->stream, ctx->counts, (size_t)16, 0u);
             WaveArgs A{};
             A.S = scene->view;
             A.jobs = ctx->dJobs;
@@ -1545,107 +1691,69 @@ int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jo
             A.Hq = ctx->Hq;
             A.Sh = ctx->Sh;
             A.counts = ctx->counts;
+            A.chunkLive = nullptr;
             A.ctr = ctx->dCounters;
-            A.pathOnlyFast = fastShadow ? 1 : 0;
-            A.split = split ? 1 : 0;
-            int cur = 0;
+            A.pathOnlyFast = K.fastShadow ? 1 : 0;
+            A.split = K.split ? 1 : 0;
             A.qIn = ctx->q[0];
             A.qOut = ctx->q[1];
-            size_t e0 = mark();
+            size_t tc = tl.mark();
             hipLaunchKernelGGL(kCamera, dim3((w.slots + 255) / 256), dim3(256), 0, ctx->stream, A);
             GND_CHECK(hipGetLastError());
-            spans.push_back({e0, mark(), 2});
+            if (tlp) tl.lap(tc, 2);
+            int cur = 0;
             uint32_t live = w.slots;
-            uint32_t hcounts[4];
+            if (graph) {
+                // a fixed chain of bounces captured once and launched as one
+                // graph (kernels exit at once on an empty queue); paths still
+                // alive after it continue below
+                const int depth = std::max(1, w.maxDepth + 1);
+                hipGraph_t gr = nullptr;
+                hipGraphExec_t ge = nullptr;
+                GND_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+                try {
+                    for (int d = 0; d < depth; ++d) {
+                        A.qIn = ctx->q[cur];
+                        A.qOut = ctx->q[1 - cur];
+                        launchBounce(ctx, A, K, live, nullptr);
+                        cur = 1 - cur;
+                    }
+                } catch (...) {
+                    if (hipStreamEndCapture(ctx->stream, &gr) == hipSuccess && gr) (void)hipGraphDestroy(gr);
+                    throw;
+                }
+                GND_CHECK(hipStreamEndCapture(ctx->stream, &gr));
+                const hipError_t ie = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
+                if (ie == hipSuccess) GND_CHECK(hipGraphLaunch(ge, ctx->stream));
+                live = ie == hipSuccess ? readLive(ctx) : 0;
+                if (ge) (void)hipGraphExecDestroy(ge);
+                (void)hipGraphDestroy(gr);
+                GND_CHECK(ie);
+            }
             while (live > 0) {
                 A.qIn = ctx->q[cur];
                 A.qOut = ctx->q[1 - cur];
-                // zero the ray-fetch counters of the two persistent traversals
-                hipLaunchKernelGGL(kFillU32, dim3(1), dim3(64), 0, ctx->stream, ctx->counts + kFetchTrace,
-                                   (size_t)2, 0u);
-                size_t a = mark();
-                hipLaunchKernelGGL(kTrace, dim3(grid(live)), dim3(B), shm, ctx->stream, A);
-                GND_CHECK(hipGetLastError());
-                size_t b = mark();
-                spans.push_back({a, b, 0});
-                rays += live;
-                launches++;
-                if (split) {
-                    // NEE, then shadows (consuming transmittance samples), then scatter
-                    hipLaunchKernelGGL(kShade<true>, dim3(grid(live)), dim3(B), 0, ctx->stream, A);
-                    GND_CHECK(hipGetLastError());
-                } else {
-                    hipLaunchKernelGGL(kShade<false>, dim3(grid(live)), dim3(B), 0, ctx->stream, A);
-                    GND_CHECK(hipGetLastError());
-                }
-                size_t c = mark();
-                spans.push_back({b, c, 2});
-                GND_CHECK(hipMemcpyAsync(hcounts, ctx->counts, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                                         ctx->stream));
-                GND_CHECK(hipStreamSynchronize(ctx->stream));
-                const uint32_t nShadow = hcounts[2];
-                if (nShadow > 0) {
-                    size_t d0 = mark();
-                    hipLaunchKernelGGL(kSh, dim3(grid(nShadow)), dim3(B), shm, ctx->stream, A);
-                    GND_CHECK(hipGetLastError());
-                    spans.push_back({d0, mark(), 1});
-                    shadowRays += nShadow;
-                }
-                if (split) {
-                    // the paths of this bounce with an event scatter now; the
-                    // in-queue holds them (kShade<true> appended nothing)
-                    size_t s0 = mark();
-                    hipLaunchKernelGGL(kScatter, dim3(grid(live)), dim3(B), 0, ctx->stream, A,
-                                       (const int32_t*)A.qIn, live);
-                    GND_CHECK(hipGetLastError());
-                    spans.push_back({s0, mark(), 2});
-                    GND_CHECK(hipMemcpyAsync(hcounts, ctx->counts, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                                             ctx->stream));
-                    GND_CHECK(hipStreamSynchronize(ctx->stream));
-                }
-                live = hcounts[1];
-                if (ctx->cancel.load()) throw HipError(GND_E_CANCELLED, "render cancelled");
-                // next bounce reads the out-queue; its count moves to slot 0
-                GND_CHECK(hipMemcpyAsync(ctx->counts, ctx->counts + 1, sizeof(uint32_t), hipMemcpyDeviceToDevice,
-                                         ctx->stream));
+                launchBounce(ctx, A, K, live, tlp);
                 cur = 1 - cur;
+                live = readLive(ctx);
+                if (ctx->cancel.load()) throw HipError(GND_E_CANCELLED, "render cancelled");
             }
-            size_t s0 = mark();
+            size_t ts = tl.mark();
             hipLaunchKernelGGL(kSplat, dim3((w.pix + 255) / 256), dim3(256), 0, ctx->stream, A,
                                (const DSplatTask*)ctx->dTasks, (int)w.tasks.size(), w.pix);
             GND_CHECK(hipGetLastError());
-            spans.push_back({s0, mark(), 2});
+            if (tlp) tl.lap(ts, 2);
         }
-        const size_t t1 = mark();
+        const size_t t1 = tl.mark();
         Counters c;
         GND_CHECK(hipMemcpyAsync(&c, ctx->dCounters, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
         GND_CHECK(hipStreamSynchronize(ctx->stream));
         if (stats) {
             std::memset(stats, 0, sizeof(*stats));
-            stats->ms_total = elapsedMs(evs[t0], evs[t1]);
-            for (auto& sp : spans) {
-                double ms = elapsedMs(evs[sp.a], evs[sp.b]);
-                if (sp.kind == 0) stats->ms_trace += ms;
-                else if (sp.kind == 1) stats->ms_shadow += ms;
-                else stats->ms_shade += ms;
-            }
-            stats->samples = samples;
-            stats->rays_closest = rays;
-            stats->rays_shadow = shadowRays;
-            stats->node_visits = c.nodes;
-            stats->pack_tests = c.packs;
-            stats->inst_tests = c.insts;
-            stats->shadow_node_visits = c.snodes;
-            stats->shadow_pack_tests = c.spacks;
-            stats->shadow_inst_tests = c.sinsts;
-            stats->max_ray_nodes = c.maxRayNodes;
-            stats->bad_rays = c.badRays;
-            stats->lane_iterations = c.iters;
-            stats->shadow_lane_iterations = c.siters;
-            stats->trace_launches = launches;
-            stats->nan_samples = c.nan;
+            stats->ms_total = elapsedMs(tl.evs[t0], tl.evs[t1]);
+            tl.sum(stats->ms_trace, stats->ms_shadow, stats->ms_shade);
+            fillStats(stats, c, samples);
         }
-        for (auto e : evs) (void)hipEventDestroy(e);
         if (c.stackOverflow) throw HipError(GND_E_STACK, "traversal stack overflow");
     });
 }

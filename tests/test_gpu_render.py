@@ -187,6 +187,28 @@ def test_tunings_give_identical_films(device):
         assert np.array_equal(films[0], f)
 
 
+@pytest.mark.parametrize("name,params,spp", [("cbox", "width=64;height=48", 3),
+                                             ("volumetric", "grid=16;width=40;height=32", 2),
+                                             ("scaled", "level=3;instances=4;width=48;height=32", 2)])
+def test_graph_bounce_chain_identical(device, name, params, spp):
+    """The bounce chain captured in a hipGraph (no host sync between bounces)
+    renders the same films as the synchronised loop, replay after replay."""
+    hs = HostScene.generate(name, params)
+    ds = DeviceScene(device, hs)
+    jobs = [hs.default_job(spp=spp), hs.default_job(spp=1, seed_offset=5)]
+    ref, st0 = device.render(ds, jobs)
+    old = device.tuning()
+    try:
+        device.set_tuning(use_graph=True)
+        for _ in range(3):
+            got, st = device.render(ds, jobs)
+            for a, b in zip(ref, got):
+                assert np.array_equal(a, b)
+            assert st["rays_closest"] == st0["rays_closest"] and st["rays_shadow"] == st0["rays_shadow"]
+    finally:
+        device.set_tuning(**old)
+
+
 def test_autotune_picks_valid_tuning(device):
     hs = HostScene.generate("cbox", "width=128;height=96")
     ds = DeviceScene(device, hs)
