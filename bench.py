@@ -99,6 +99,68 @@ def cpu_baseline(hs, target_s=12.0):
                       f"integrator, C++ restatement of the reference CPU integrator, {threads} threads)"}
 
 
+def batch_jobs(hs, n=64, seed=0):
+    """The batching workload: n jobs over the shared scene, each with a random
+    pose (thin lens looking at the scene centre from a random direction),
+    resolution log-uniform in 64^2..1024^2 and spp log-uniform in 1..256."""
+    import numpy as np
+    from gianduia_amd import Job, camera_lookat
+    rng = np.random.default_rng(seed)
+    base = hs.camera()
+    d = hs.desc().contents
+    jobs = []
+    for _ in range(n):
+        side = int(round(float(np.exp(rng.uniform(np.log(64), np.log(1024))))))
+        spp = int(round(float(np.exp(rng.uniform(np.log(1), np.log(256))))))
+        theta = rng.uniform(0.15, 0.45) * np.pi      # above the floor
+        phi = rng.uniform(0, 2 * np.pi)
+        r = rng.uniform(4.5, 6.5)
+        eye = (r * np.sin(theta) * np.cos(phi), r * np.cos(theta), r * np.sin(theta) * np.sin(phi))
+        cam = camera_lookat(base, eye, (0.0, 0.0, 0.0))
+        cam.width = cam.height = side
+        jobs.append(Job(cam, spp, d.max_depth, d.integrator, 0))
+    return jobs
+
+
+def batch_workload(dev, ds, hs):
+    """64-job mix rendered four ways, films resident in HBM: `plain` = one job
+    at a time with one pass per wave (the reference's pass loop,
+    integrator.h:79-150), `solo` = one job at a time with whole-frame waves,
+    `batched` = all jobs in one call sharing every bounce launch, and
+    `batched_graph` = the same with the bounce chain captured in a hipGraph.
+    Films are identical in all four."""
+    from gianduia_amd.api import RenderSession
+    jobs = batch_jobs(hs)
+    samples = sum(j.camera.width * j.camera.height * j.spp for j in jobs)
+    old = dev.tuning()
+    solo = [RenderSession(dev, ds, [j]) for j in jobs]
+    together = RenderSession(dev, ds, jobs)
+    res = {"jobs": len(jobs), "samples": samples}
+    try:
+        together.run()  # warm-up: wave buffers at full size
+        for mode in ("plain", "solo", "batched", "batched_graph"):
+            t0 = time.perf_counter()
+            if mode.startswith("batched"):
+                dev.set_tuning(use_graph=mode == "batched_graph")
+                together.run()
+            else:
+                for j, sess in zip(jobs, solo):
+                    if mode == "plain":
+                        dev.set_tuning(queue_capacity=max(1024, j.camera.width * j.camera.height))
+                    sess.run()
+            secs = time.perf_counter() - t0
+            dev.set_tuning(**old)
+            res[f"{mode}_s"] = round(secs, 3)
+            res[f"{mode}_msamples_per_s"] = round(samples / secs / 1e6, 2)
+            log(f"batch workload {mode}: {secs:.2f} s")
+    finally:
+        dev.set_tuning(**old)
+    best = min(res["batched_s"], res["batched_graph_s"])
+    res["batched_vs_plain"] = round(res["plain_s"] / best, 2)
+    res["batched_vs_solo"] = round(res["solo_s"] / best, 2)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +168,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-counters", action="store_true", help="skip the untimed counting pass")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-job batching workload")
     ap.add_argument("--params", default="", help="scene generator overrides, key=value;...")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration) and exit")
@@ -187,6 +250,11 @@ def main():
                                    "insts": cst["inst_tests"] / cst["rays_closest"]}}
         log(f"counting pass: {json.dumps(roof)}")
 
+    batch = None
+    if not a.no_batch:
+        batch = batch_workload(dev, ds, hs)
+        log(f"batch workload: {json.dumps(batch)}")
+
     cpu = None
     if not a.no_cpu_baseline:
         try:
@@ -210,6 +278,7 @@ def main():
         "shade_ms_per_step": round(sum(s["ms_shade"] for s in per_step) / a.steps, 3),
         "roofline": roof,
         "cpu_baseline": cpu,
+        "batch": batch,
     }
     if cpu:
         out["vs_cpu_baseline"] = round(msps / cpu["value"], 1)

@@ -125,6 +125,18 @@ class gnd_hip_tuning(C.Structure):
                 ("use_graph", C.c_int32)]
 
 
+class gnd_hip_server_config(C.Structure):
+    _fields_ = [("hbm_budget", C.c_size_t), ("path_slots", C.c_uint32), ("max_jobs", C.c_uint32)]
+
+
+class gnd_hip_job_status(C.Structure):
+    _fields_ = [("state", C.c_int32), ("passes_done", C.c_uint32), ("queue_ms", C.c_double),
+                ("run_ms", C.c_double), ("latency_ms", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 # exported symbols with their signatures: name -> (restype, argtypes)
 HIP_API = {
     "gnd_hip_init": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
@@ -152,6 +164,14 @@ HIP_API = {
     "gnd_hip_cancel": (C.c_int, [C.c_void_p]),
     "gnd_hip_copy_probe": (C.c_int, [C.c_void_p, C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
     "gnd_hip_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(gnd_hip_job), C.POINTER(gnd_hip_tuning)]),
+    "gnd_hip_server_create": (C.c_int, [C.c_void_p, C.POINTER(gnd_hip_server_config), C.POINTER(C.c_void_p)]),
+    "gnd_hip_server_destroy": (None, [C.c_void_p]),
+    "gnd_hip_server_add_scene": (C.c_int, [C.c_void_p, C.POINTER(gnd_hip_scene_arrays), C.POINTER(C.c_int32)]),
+    "gnd_hip_server_submit": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(gnd_hip_job), C.POINTER(C.c_uint64)]),
+    "gnd_hip_server_poll": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(gnd_hip_job_status)]),
+    "gnd_hip_server_wait": (C.c_int, [C.c_void_p, C.c_uint64, C.c_double, C.POINTER(gnd_hip_job_status)]),
+    "gnd_hip_server_stats": (C.c_int, [C.c_void_p, C.POINTER(gnd_hip_stats), C.POINTER(C.c_int32),
+                                       C.POINTER(C.c_size_t)]),
 }
 
 HOST_API = {

@@ -75,7 +75,10 @@ class Device:
 
     def _check(self, rc: int) -> None:
         if rc != 0:
-            raise GndError(rc, (_lib().gnd_hip_last_error(self.ctx) or b"").decode())
+            raise GndError(rc, self.last_error())
+
+    def last_error(self) -> str:
+        return (_lib().gnd_hip_last_error(self.ctx) or b"").decode()
 
     # ---- info / tuning
     def info(self) -> dict:

@@ -11,11 +11,16 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
+#include <deque>
+#include <map>
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "dev_common.h"
@@ -869,6 +874,18 @@ struct gnd_hip_scene {
     uint32_t nPrims = 0;
 };
 
+// Path state, hit and shadow queues and queue counters for `capacity` paths
+// in flight: 248 bytes per path slot in 61 SoA arrays, each 256-B aligned.
+struct WaveBuffers {
+    size_t capacity = 0;
+    void* pathBuf = nullptr;
+    PathSoA P{};
+    HitSoA Hq{};
+    ShadowSoA Sh{};
+    int32_t* q[2] = {nullptr, nullptr};
+    uint32_t* counts = nullptr;  // 16 queue / fetch counters
+};
+
 struct gnd_hip_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -882,15 +899,7 @@ struct gnd_hip_ctx {
     size_t hbm = 0;
     std::string name;
     Counters* dCounters = nullptr;
-    // wave buffers (grown on demand)
-    size_t capacity = 0;
-    void* pathBuf = nullptr;
-    PathSoA P{};
-    HitSoA Hq{};
-    ShadowSoA Sh{};
-    int32_t* q[2] = {nullptr, nullptr};
-    int32_t* qEv = nullptr;
-    uint32_t* counts = nullptr;
+    WaveBuffers wb;  // path state of gnd_hip_render (grown on demand)
     DJob* dJobs = nullptr;
     size_t jobCap = 0;
     DChunk* dChunks = nullptr;
@@ -1004,23 +1013,34 @@ void ensureDev(T*& p, size_t& cap, size_t need) {
     cap = n;
 }
 
-void ensureWaveBuffers(gnd_hip_ctx* ctx, size_t cap) {
-    if (cap <= ctx->capacity) return;
-    if (ctx->pathBuf) GND_CHECK(hipFree(ctx->pathBuf));
-    ctx->pathBuf = nullptr;
-    ctx->capacity = 0;
-    // 63 four-byte fields and one eight-byte field per slot, each array 256-B aligned
+size_t waveBufferBytes(size_t cap) {
     const size_t per = ((cap * 4 + 255) / 256) * 256;
-    const size_t bytes = 63 * per + ((cap * 8 + 255) / 256) * 256;
-    GND_CHECK(hipMalloc(&ctx->pathBuf, bytes));
-    char* b = static_cast<char*>(ctx->pathBuf);
+    return 60 * per + ((cap * 8 + 255) / 256) * 256;
+}
+
+void freeWaveBuffers(WaveBuffers& wb) {
+    if (wb.pathBuf) (void)hipFree(wb.pathBuf);
+    if (wb.counts) (void)hipFree(wb.counts);
+    wb = WaveBuffers{};
+}
+
+void ensureWaveBuffers(WaveBuffers& wb, size_t cap) {
+    if (!wb.counts) GND_CHECK(hipMalloc(&wb.counts, 16 * sizeof(uint32_t)));
+    if (cap <= wb.capacity) return;
+    if (wb.pathBuf) GND_CHECK(hipFree(wb.pathBuf));
+    wb.pathBuf = nullptr;
+    wb.capacity = 0;
+    // 60 four-byte fields and one eight-byte field per slot
+    const size_t bytes = waveBufferBytes(cap);
+    GND_CHECK(hipMalloc(&wb.pathBuf, bytes));
+    char* b = static_cast<char*>(wb.pathBuf);
     size_t off = 0;
     auto take = [&](size_t elemBytes) {
         void* p = b + off;
         off += ((cap * elemBytes + 255) / 256) * 256;
         return p;
     };
-    PathSoA& P = ctx->P;
+    PathSoA& P = wb.P;
     P.rng = (uint64_t*)take(8);
     float** f[] = {&P.ox, &P.oy, &P.oz, &P.dx, &P.dy, &P.dz, &P.tpr, &P.tpg, &P.tpb, &P.Lr, &P.Lg, &P.Lb,
                    &P.pdfPrev, &P.lx, &P.ly, &P.lz, &P.weight, &P.pfx, &P.pfy, &P.evpx, &P.evpy, &P.evpz,
@@ -1034,21 +1054,20 @@ void ensureWaveBuffers(gnd_hip_ctx* ctx, size_t cap) {
     P.evKind = (int32_t*)take(4);
     P.evPrim = (int32_t*)take(4);
     P.evMedium = (int32_t*)take(4);
-    ctx->Hq.t = (float*)take(4);
-    ctx->Hq.u = (float*)take(4);
-    ctx->Hq.v = (float*)take(4);
-    ctx->Hq.inst = (int32_t*)take(4);
-    ctx->Hq.prim = (int32_t*)take(4);
-    ShadowSoA& S = ctx->Sh;
+    wb.Hq.t = (float*)take(4);
+    wb.Hq.u = (float*)take(4);
+    wb.Hq.v = (float*)take(4);
+    wb.Hq.inst = (int32_t*)take(4);
+    wb.Hq.prim = (int32_t*)take(4);
+    ShadowSoA& S = wb.Sh;
     S.path = (int32_t*)take(4);
     S.medium = (int32_t*)take(4);
     float** g[] = {&S.ox, &S.oy, &S.oz, &S.dx, &S.dy, &S.dz, &S.tmax, &S.pr, &S.pg, &S.pb, &S.lpx, &S.lpy, &S.lpz};
     for (auto pp : g) *pp = (float*)take(4);
-    ctx->q[0] = (int32_t*)take(4);
-    ctx->q[1] = (int32_t*)take(4);
-    ctx->qEv = (int32_t*)take(4);
+    wb.q[0] = (int32_t*)take(4);
+    wb.q[1] = (int32_t*)take(4);
     if (off > bytes) throw std::runtime_error("wave buffer layout overflow");
-    ctx->capacity = cap;
+    wb.capacity = cap;
 }
 
 using TraceKernel = void (*)(DScene, gnd_ray_soa, int64_t, gnd_hit_soa, int32_t*, Counters*);
@@ -1116,7 +1135,6 @@ int gnd_hip_init(int device, gnd_hip_ctx** out) {
         GND_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         GND_CHECK(hipMalloc(&ctx->dCounters, sizeof(Counters)));
         GND_CHECK(hipMemset(ctx->dCounters, 0, sizeof(Counters)));
-        GND_CHECK(hipMalloc(&ctx->counts, 16 * sizeof(uint32_t)));
         GND_CHECK(hipEventCreate(&ctx->ev[0]));
         GND_CHECK(hipEventCreate(&ctx->ev[1]));
     });
@@ -1134,9 +1152,8 @@ void gnd_hip_shutdown(gnd_hip_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    if (ctx->pathBuf) (void)hipFree(ctx->pathBuf);
+    freeWaveBuffers(ctx->wb);
     if (ctx->dCounters) (void)hipFree(ctx->dCounters);
-    if (ctx->counts) (void)hipFree(ctx->counts);
     if (ctx->dJobs) (void)hipFree(ctx->dJobs);
     if (ctx->dChunks) (void)hipFree(ctx->dChunks);
     if (ctx->dTasks) (void)hipFree(ctx->dTasks);
@@ -1589,9 +1606,9 @@ static void launchBounce(gnd_hip_ctx* ctx, const WaveArgs& A, const BouncePlan& 
     if (tl) tl->lap(t, 2);
 }
 
-static uint32_t readLive(gnd_hip_ctx* ctx) {
+static uint32_t readLive(gnd_hip_ctx* ctx, const uint32_t* counts) {
     uint32_t live = 0;
-    GND_CHECK(hipMemcpyAsync(&live, ctx->counts, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    GND_CHECK(hipMemcpyAsync(&live, counts, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
     GND_CHECK(hipStreamSynchronize(ctx->stream));
     return live;
 }
@@ -1656,7 +1673,7 @@ int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jo
         }
         const size_t cap = std::max<size_t>((size_t)ctx->tuning.queue_capacity, maxPix);
         if (cap > 0x7FFFFFFFull) throw std::runtime_error("wave capacity too large");
-        ensureWaveBuffers(ctx, cap);
+        ensureWaveBuffers(ctx->wb, cap);
         std::vector<DJob> dj(jobs.size());
         for (size_t j = 0; j < jobs.size(); ++j) {
             dj[j] = makeDJob(jobs[j]);
@@ -1680,23 +1697,23 @@ int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jo
                                      hipMemcpyHostToDevice, ctx->stream));
             GND_CHECK(hipMemcpyAsync(ctx->dTasks, w.tasks.data(), w.tasks.size() * sizeof(DSplatTask),
                                      hipMemcpyHostToDevice, ctx->stream));
-            hipLaunchKernelGGL(kFillU32, dim3(1), dim3(64), 0, ctx->stream, ctx->counts, (size_t)16, 0u);
+            hipLaunchKernelGGL(kFillU32, dim3(1), dim3(64), 0, ctx->stream, ctx->wb.counts, (size_t)16, 0u);
             WaveArgs A{};
             A.S = scene->view;
             A.jobs = ctx->dJobs;
             A.chunks = ctx->dChunks;
             A.nChunks = (int32_t)w.chunks.size();
             A.nSlots = w.slots;
-            A.P = ctx->P;
-            A.Hq = ctx->Hq;
-            A.Sh = ctx->Sh;
-            A.counts = ctx->counts;
+            A.P = ctx->wb.P;
+            A.Hq = ctx->wb.Hq;
+            A.Sh = ctx->wb.Sh;
+            A.counts = ctx->wb.counts;
             A.chunkLive = nullptr;
             A.ctr = ctx->dCounters;
             A.pathOnlyFast = K.fastShadow ? 1 : 0;
             A.split = K.split ? 1 : 0;
-            A.qIn = ctx->q[0];
-            A.qOut = ctx->q[1];
+            A.qIn = ctx->wb.q[0];
+            A.qOut = ctx->wb.q[1];
             size_t tc = tl.mark();
             hipLaunchKernelGGL(kCamera, dim3((w.slots + 255) / 256), dim3(256), 0, ctx->stream, A);
             GND_CHECK(hipGetLastError());
@@ -1713,8 +1730,8 @@ int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jo
                 GND_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
                 try {
                     for (int d = 0; d < depth; ++d) {
-                        A.qIn = ctx->q[cur];
-                        A.qOut = ctx->q[1 - cur];
+                        A.qIn = ctx->wb.q[cur];
+                        A.qOut = ctx->wb.q[1 - cur];
                         launchBounce(ctx, A, K, live, nullptr);
                         cur = 1 - cur;
                     }
@@ -1725,17 +1742,17 @@ int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jo
                 GND_CHECK(hipStreamEndCapture(ctx->stream, &gr));
                 const hipError_t ie = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
                 if (ie == hipSuccess) GND_CHECK(hipGraphLaunch(ge, ctx->stream));
-                live = ie == hipSuccess ? readLive(ctx) : 0;
+                live = ie == hipSuccess ? readLive(ctx, ctx->wb.counts) : 0;
                 if (ge) (void)hipGraphExecDestroy(ge);
                 (void)hipGraphDestroy(gr);
                 GND_CHECK(ie);
             }
             while (live > 0) {
-                A.qIn = ctx->q[cur];
-                A.qOut = ctx->q[1 - cur];
+                A.qIn = ctx->wb.q[cur];
+                A.qOut = ctx->wb.q[1 - cur];
                 launchBounce(ctx, A, K, live, tlp);
                 cur = 1 - cur;
-                live = readLive(ctx);
+                live = readLive(ctx, ctx->wb.counts);
                 if (ctx->cancel.load()) throw HipError(GND_E_CANCELLED, "render cancelled");
             }
             size_t ts = tl.mark();
@@ -1817,6 +1834,526 @@ int gnd_hip_autotune(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* 
     rc = gnd_hip_set_tuning(ctx, &best);
     *out = ctx->tuning;
     return rc;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- serving engine
+//
+// Continuous batching: jobs queue in submission order; between bounces the
+// engine admits passes of queued jobs into free path slots (a pass takes W*H
+// contiguous slots), runs one bounce over every live path of the active scene,
+// retires passes whose paths have all ended (per-chunk live counters), and
+// gathers finished passes into their job's film in pass order, so a film is
+// bit-identical whatever else shares the GPU.  Scenes stay resident in HBM
+// until the budget needs their space (least recently used first).
+
+struct SrvScene {
+    const gnd_hip_scene_arrays* arrays = nullptr;  // borrowed until the server is destroyed
+    gnd_hip_scene* dev = nullptr;
+    size_t bytes = 0;
+    uint64_t lastUse = 0;
+    int running = 0;  // jobs of this scene admitted and not finished
+};
+
+struct SrvJob {
+    int scene = -1;
+    gnd_hip_job job{};
+    int slot = -1;            // entry in the device job table
+    uint32_t nextPass = 0;    // next pass to admit
+    uint32_t nextGather = 0;  // next pass to gather into the film
+    int32_t state = 0;        // 0 queued, 1 running, 2 done, < 0 failed (GND_E_*)
+    std::chrono::steady_clock::time_point tSubmit, tAdmit, tDone;
+    std::map<uint32_t, int> finished;  // pass -> chunk, all paths ended, awaiting the gather
+};
+
+struct SrvChunk {
+    uint64_t ticket = 0;
+    uint32_t pass = 0, slotBase = 0, npix = 0;
+    bool running = false;
+};
+
+struct gnd_hip_server {
+    gnd_hip_ctx* ctx = nullptr;
+    gnd_hip_server_config cfg{};
+    size_t pathBytes = 0;
+    WaveBuffers wb;
+    std::mutex mu;
+    std::condition_variable cv, done;
+    std::thread worker;
+    bool stop = false;
+    std::string err;
+    std::vector<SrvScene> scenes;
+    std::deque<uint64_t> queue;
+    std::unordered_map<uint64_t, SrvJob> jobs;
+    uint64_t nextTicket = 1, useClock = 0;
+    int active = -1;                  // scene of the running batch
+    // device tables
+    DJob* dJobs = nullptr;
+    std::vector<int> freeJobSlots;
+    uint32_t* dLive = nullptr;        // live paths per chunk
+    std::vector<SrvChunk> chunks;
+    std::vector<int> freeChunks;
+    std::vector<uint32_t> hLive;
+    DChunk* dCam = nullptr;
+    size_t camCap = 0;
+    DChunk* dGather = nullptr;
+    size_t gatherCap = 0;
+    DSplatTask* dTasks = nullptr;
+    size_t taskCap = 0;
+    Counters* dCounters = nullptr;
+    std::map<uint32_t, uint32_t> freeSlots;  // begin -> end of free path-slot ranges
+    uint32_t live = 0;
+    int cur = 0;
+    std::vector<int> running;  // chunks with paths in flight
+};
+
+namespace {
+
+constexpr int kSrvMaxChunks = 1 << 16;
+
+bool takeSlots(gnd_hip_server* s, uint32_t n, uint32_t& base) {
+    for (auto it = s->freeSlots.begin(); it != s->freeSlots.end(); ++it) {
+        const uint32_t b = it->first, e = it->second;
+        if (e - b < n) continue;
+        s->freeSlots.erase(it);
+        if (e - b > n) s->freeSlots[b + n] = e;
+        base = b;
+        return true;
+    }
+    return false;
+}
+
+void giveSlots(gnd_hip_server* s, uint32_t b, uint32_t n) {
+    uint32_t e = b + n;
+    auto next = s->freeSlots.lower_bound(b);
+    if (next != s->freeSlots.end() && next->first == e) {
+        e = next->second;
+        next = s->freeSlots.erase(next);
+    }
+    if (next != s->freeSlots.begin()) {
+        auto prev = std::prev(next);
+        if (prev->second == b) {
+            prev->second = e;
+            return;
+        }
+    }
+    s->freeSlots[b] = e;
+}
+
+size_t sceneEstimate(const gnd_hip_scene_arrays* a) {
+    size_t b = (size_t)a->n_tlas_nodes * sizeof(DNode) + (size_t)a->n_blas_nodes * sizeof(DNode) +
+               (size_t)a->n_packs * sizeof(DPack) + (size_t)a->n_tris * 24 * sizeof(float) +
+               (size_t)a->n_cdf * sizeof(float) + (size_t)a->n_tlas_prims * 4;
+    const gnd_scene_desc& d = *a->desc;
+    b += (size_t)d.nprims * (sizeof(DInst) + sizeof(DPrim)) + (size_t)d.nmaterials * sizeof(DMaterial);
+    for (uint32_t m = 0; m < d.nmedia; ++m)
+        if (d.media[m].type == GND_MEDIUM_GRID) b += (size_t)d.media[m].nx * d.media[m].ny * d.media[m].nz * 4 * 2;
+    return b;
+}
+
+size_t residentBytes(const gnd_hip_server* s) {
+    size_t b = s->pathBytes;
+    for (const SrvScene& sc : s->scenes)
+        if (sc.dev) b += sc.bytes;
+    return b;
+}
+
+// Makes scene `i` resident, evicting idle scenes (least recently used first)
+// while the budget would be exceeded.
+void makeResident(gnd_hip_server* s, int i) {
+    SrvScene& sc = s->scenes[i];
+    sc.lastUse = ++s->useClock;
+    if (sc.dev) return;
+    const size_t need = sceneEstimate(sc.arrays);
+    for (;;) {
+        if (residentBytes(s) + need <= s->cfg.hbm_budget) break;
+        int victim = -1;
+        for (int k = 0; k < (int)s->scenes.size(); ++k) {
+            const SrvScene& o = s->scenes[k];
+            if (k == i || !o.dev || o.running) continue;
+            if (victim < 0 || o.lastUse < s->scenes[victim].lastUse) victim = k;
+        }
+        if (victim < 0) throw HipError(GND_E_NOMEM, "scene does not fit the server's HBM budget");
+        gnd_hip_scene_free(s->ctx, s->scenes[victim].dev);
+        s->scenes[victim].dev = nullptr;
+    }
+    gnd_hip_scene* dev = nullptr;
+    s->ctx->mu.unlock();  // gnd_hip_scene_upload takes the context lock
+    const int rc = gnd_hip_scene_upload(s->ctx, sc.arrays, &dev);
+    s->ctx->mu.lock();
+    if (rc != GND_OK) throw HipError(rc, std::string("scene upload: ") + s->ctx->err);
+    sc.dev = dev;
+    sc.bytes = gnd_hip_scene_bytes(dev);
+}
+
+void failJob(gnd_hip_server* s, uint64_t t, int code) {
+    SrvJob& j = s->jobs[t];
+    j.tDone = std::chrono::steady_clock::now();
+    if (j.state == 0) j.tAdmit = j.tDone;
+    j.state = code;
+    if (j.slot >= 0) s->freeJobSlots.push_back(j.slot);
+    j.slot = -1;
+}
+
+// Admission: passes of the queue head while it belongs to the active scene and
+// path slots, chunk ids and job-table entries are free.  Returns new chunks.
+std::vector<int> admit(gnd_hip_server* s) {
+    std::vector<int> fresh;
+    while (!s->queue.empty()) {
+        const uint64_t t = s->queue.front();
+        SrvJob& j = s->jobs[t];
+        if (j.scene != s->active) break;
+        if (j.state == 0) {
+            if (s->freeJobSlots.empty()) break;
+            j.slot = s->freeJobSlots.back();
+            s->freeJobSlots.pop_back();
+            const DJob dj = makeDJob(j.job);
+            GND_CHECK(hipMemcpyAsync(s->dJobs + j.slot, &dj, sizeof(DJob), hipMemcpyHostToDevice,
+                                     s->ctx->stream));
+            clearFilm(s->ctx, dj);
+            j.state = 1;
+            j.tAdmit = std::chrono::steady_clock::now();
+            s->scenes[j.scene].running++;
+        }
+        const uint32_t npix = (uint32_t)j.job.camera.width * (uint32_t)j.job.camera.height;
+        while (j.nextPass < j.job.spp) {
+            uint32_t base = 0;
+            if (s->freeChunks.empty() || !takeSlots(s, npix, base)) return fresh;
+            const int c = s->freeChunks.back();
+            s->freeChunks.pop_back();
+            s->chunks[c] = SrvChunk{t, j.nextPass, base, npix, true};
+            s->hLive[c] = j.job.max_depth > 0 ? npix : 0u;
+            fresh.push_back(c);
+            j.nextPass++;
+        }
+        s->queue.pop_front();
+    }
+    return fresh;
+}
+
+WaveArgs serverArgs(gnd_hip_server* s, const BouncePlan& K) {
+    WaveArgs A{};
+    A.S = s->scenes[s->active].dev->view;
+    A.jobs = s->dJobs;
+    A.P = s->wb.P;
+    A.Hq = s->wb.Hq;
+    A.Sh = s->wb.Sh;
+    A.counts = s->wb.counts;
+    A.chunkLive = s->dLive;
+    A.ctr = s->dCounters;
+    A.pathOnlyFast = K.fastShadow ? 1 : 0;
+    A.split = K.split ? 1 : 0;
+    return A;
+}
+
+// One engine iteration; called with the server lock held.
+void serverStep(gnd_hip_server* s) {
+    gnd_hip_ctx* ctx = s->ctx;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    GND_CHECK(hipSetDevice(ctx->device));
+    if (s->running.empty() && s->live == 0) {
+        if (s->queue.empty()) return;
+        s->active = s->jobs[s->queue.front()].scene;
+        makeResident(s, s->active);
+    }
+    s->scenes[s->active].lastUse = ++s->useClock;
+    // the active scene decides the bounce kernels (every job of the batch renders it)
+    bool anyVol = false;
+    for (int c : s->running)
+        if (s->jobs[s->chunks[c].ticket].job.integrator == GND_INTEGRATOR_VOLPATH) anyVol = true;
+    for (uint64_t t : s->queue)
+        if (s->jobs[t].scene == s->active && s->jobs[t].job.integrator == GND_INTEGRATOR_VOLPATH) anyVol = true;
+    const BouncePlan K = makeBouncePlan(ctx, s->scenes[s->active].dev, anyVol);
+    WaveArgs A = serverArgs(s, K);
+    // 1. admission: camera rays of the new passes join the in-queue
+    const std::vector<int> fresh = admit(s);
+    if (!fresh.empty()) {
+        std::vector<DChunk> dc;
+        uint32_t n = 0;
+        for (int c : fresh) {
+            const SrvChunk& ch = s->chunks[c];
+            dc.push_back({ch.slotBase, ch.npix, s->jobs[ch.ticket].slot, (int32_t)ch.pass, n, c});
+            n += ch.npix;
+            GND_CHECK(hipMemcpyAsync(s->dLive + c, &s->hLive[c], sizeof(uint32_t), hipMemcpyHostToDevice,
+                                     ctx->stream));
+            s->running.push_back(c);
+        }
+        ensureDev(s->dCam, s->camCap, dc.size());
+        GND_CHECK(hipMemcpyAsync(s->dCam, dc.data(), dc.size() * sizeof(DChunk), hipMemcpyHostToDevice,
+                                 ctx->stream));
+        A.chunks = s->dCam;
+        A.nChunks = (int32_t)dc.size();
+        A.nSlots = n;
+        A.qIn = s->wb.q[s->cur];
+        A.qOut = s->wb.q[1 - s->cur];
+        hipLaunchKernelGGL(kCamera, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, A);
+        GND_CHECK(hipGetLastError());
+        s->live += n;
+    }
+    // 2. one bounce for every live path
+    if (s->live > 0) {
+        A.qIn = s->wb.q[s->cur];
+        A.qOut = s->wb.q[1 - s->cur];
+        launchBounce(ctx, A, K, s->live, nullptr);
+        s->cur = 1 - s->cur;
+    }
+    // 3. retirement: passes whose paths have all ended
+    int used = 0;  // chunk ids are handed out lowest first
+    for (int c : s->running) used = std::max(used, c + 1);
+    GND_CHECK(hipMemcpyAsync(&s->live, s->wb.counts, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    if (used)
+        GND_CHECK(hipMemcpyAsync(s->hLive.data(), s->dLive, (size_t)used * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                 ctx->stream));
+    GND_CHECK(hipStreamSynchronize(ctx->stream));
+    std::vector<int> still;
+    for (int c : s->running) {
+        SrvChunk& ch = s->chunks[c];
+        if (s->hLive[c] != 0) {
+            still.push_back(c);
+            continue;
+        }
+        ch.running = false;
+        s->jobs[ch.ticket].finished[ch.pass] = c;
+    }
+    s->running.swap(still);
+    // 4. gather finished passes into the films in pass order
+    std::vector<DChunk> gc;
+    std::vector<DSplatTask> tasks;
+    std::vector<uint64_t> completed;
+    std::vector<int> released;
+    uint32_t pix = 0;
+    for (auto& kv : s->jobs) {
+        SrvJob& j = kv.second;
+        if (j.state != 1) continue;
+        const int first = (int)gc.size();
+        for (auto it = j.finished.find(j.nextGather); it != j.finished.end(); it = j.finished.find(j.nextGather)) {
+            const SrvChunk& ch = s->chunks[it->second];
+            gc.push_back({ch.slotBase, ch.npix, j.slot, (int32_t)ch.pass, 0u, it->second});
+            released.push_back(it->second);
+            j.finished.erase(it);
+            j.nextGather++;
+        }
+        if ((int)gc.size() > first) {
+            tasks.push_back({pix, j.slot, first, (int32_t)gc.size()});
+            pix += (uint32_t)j.job.camera.width * (uint32_t)j.job.camera.height;
+        }
+        if (j.nextGather == j.job.spp) completed.push_back(kv.first);
+    }
+    if (!tasks.empty()) {
+        ensureDev(s->dGather, s->gatherCap, gc.size());
+        ensureDev(s->dTasks, s->taskCap, tasks.size());
+        GND_CHECK(hipMemcpyAsync(s->dGather, gc.data(), gc.size() * sizeof(DChunk), hipMemcpyHostToDevice,
+                                 ctx->stream));
+        GND_CHECK(hipMemcpyAsync(s->dTasks, tasks.data(), tasks.size() * sizeof(DSplatTask),
+                                 hipMemcpyHostToDevice, ctx->stream));
+        A.chunks = s->dGather;
+        A.nChunks = (int32_t)gc.size();
+        hipLaunchKernelGGL(kSplat, dim3((pix + 255) / 256), dim3(256), 0, ctx->stream, A,
+                           (const DSplatTask*)s->dTasks, (int)tasks.size(), pix);
+        GND_CHECK(hipGetLastError());
+        GND_CHECK(hipStreamSynchronize(ctx->stream));
+    }
+    for (int c : released) {
+        giveSlots(s, s->chunks[c].slotBase, s->chunks[c].npix);
+        s->chunks[c] = SrvChunk{};
+        s->freeChunks.push_back(c);
+    }
+    const auto now = std::chrono::steady_clock::now();
+    for (uint64_t t : completed) {
+        SrvJob& j = s->jobs[t];
+        j.state = 2;
+        j.tDone = now;
+        s->freeJobSlots.push_back(j.slot);
+        j.slot = -1;
+        s->scenes[j.scene].running--;
+    }
+    if (!completed.empty()) s->done.notify_all();
+}
+
+bool serverHasWork(const gnd_hip_server* s) { return !s->queue.empty() || !s->running.empty() || s->live > 0; }
+
+void serverLoop(gnd_hip_server* s) {
+    std::unique_lock<std::mutex> lk(s->mu);
+    while (!s->stop) {
+        if (!serverHasWork(s)) {
+            s->cv.wait(lk, [&] { return s->stop || serverHasWork(s); });
+            continue;
+        }
+        try {
+            serverStep(s);
+        } catch (const std::exception& e) {
+            // a device error fails every job in flight; the queue stays
+            s->err = e.what();
+            {
+                std::lock_guard<std::mutex> g(s->ctx->mu);
+                s->ctx->err = e.what();
+            }
+            const int code = dynamic_cast<const HipError*>(&e) ? static_cast<const HipError&>(e).code : GND_E_DEVICE;
+            for (auto& kv : s->jobs)
+                if (kv.second.state == 1) failJob(s, kv.first, code);
+            for (uint64_t t : s->queue)
+                if (s->jobs[t].scene == s->active) failJob(s, t, code);
+            std::deque<uint64_t> keep;
+            for (uint64_t t : s->queue)
+                if (s->jobs[t].state == 0) keep.push_back(t);
+            s->queue.swap(keep);
+            for (int c = 0; c < (int)s->chunks.size(); ++c)
+                if (s->chunks[c].ticket) {
+                    giveSlots(s, s->chunks[c].slotBase, s->chunks[c].npix);
+                    s->chunks[c] = SrvChunk{};
+                    s->freeChunks.push_back(c);
+                }
+            for (SrvScene& sc : s->scenes) sc.running = 0;
+            s->running.clear();
+            s->live = 0;
+            hipLaunchKernelGGL(kFillU32, dim3(1), dim3(64), 0, s->ctx->stream, s->wb.counts, (size_t)16, 0u);
+            s->done.notify_all();
+        }
+    }
+}
+
+void statusOf(const SrvJob& j, gnd_hip_job_status* st) {
+    using ms = std::chrono::duration<double, std::milli>;
+    st->state = j.state;
+    st->queue_ms = j.state >= 1 || j.state < 0 ? ms(j.tAdmit - j.tSubmit).count() : 0.0;
+    st->run_ms = j.state == 2 ? ms(j.tDone - j.tAdmit).count() : 0.0;
+    st->latency_ms = j.state == 2 || j.state < 0 ? ms(j.tDone - j.tSubmit).count() : 0.0;
+    st->passes_done = j.nextGather;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gnd_hip_server_create(gnd_hip_ctx* ctx, const gnd_hip_server_config* cfg, gnd_hip_server** out) {
+    if (!ctx || !cfg || !out || cfg->path_slots < 1024 || cfg->max_jobs < 1) return GND_E_INVALID;
+    gnd_hip_server* s = new gnd_hip_server();
+    s->ctx = ctx;
+    s->cfg = *cfg;
+    if (s->cfg.hbm_budget == 0) s->cfg.hbm_budget = ctx->hbm / 10 * 9;
+    s->pathBytes = waveBufferBytes(cfg->path_slots);
+    int rc = guarded(ctx, [&] {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        GND_CHECK(hipSetDevice(ctx->device));
+        if (s->pathBytes > s->cfg.hbm_budget) throw HipError(GND_E_NOMEM, "path slots exceed the HBM budget");
+        ensureWaveBuffers(s->wb, cfg->path_slots);
+        GND_CHECK(hipMalloc(&s->dJobs, (size_t)cfg->max_jobs * sizeof(DJob)));
+        GND_CHECK(hipMalloc(&s->dLive, (size_t)kSrvMaxChunks * sizeof(uint32_t)));
+        GND_CHECK(hipMalloc(&s->dCounters, sizeof(Counters)));
+        GND_CHECK(hipMemset(s->dCounters, 0, sizeof(Counters)));
+        hipLaunchKernelGGL(kFillU32, dim3(1), dim3(64), 0, ctx->stream, s->wb.counts, (size_t)16, 0u);
+        GND_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+    if (rc != GND_OK) {
+        freeWaveBuffers(s->wb);
+        if (s->dJobs) (void)hipFree(s->dJobs);
+        if (s->dLive) (void)hipFree(s->dLive);
+        if (s->dCounters) (void)hipFree(s->dCounters);
+        delete s;
+        return rc;
+    }
+    for (int i = (int)cfg->max_jobs - 1; i >= 0; --i) s->freeJobSlots.push_back(i);
+    s->chunks.resize(kSrvMaxChunks);
+    s->hLive.assign(kSrvMaxChunks, 0u);
+    for (int c = kSrvMaxChunks - 1; c >= 0; --c) s->freeChunks.push_back(c);
+    s->freeSlots[0] = cfg->path_slots;
+    s->worker = std::thread(serverLoop, s);
+    *out = s;
+    return GND_OK;
+}
+
+void gnd_hip_server_destroy(gnd_hip_server* s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    gnd_hip_ctx* ctx = s->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (SrvScene& sc : s->scenes)
+        if (sc.dev) gnd_hip_scene_free(ctx, sc.dev);
+    freeWaveBuffers(s->wb);
+    for (void* p : {(void*)s->dJobs, (void*)s->dLive, (void*)s->dCam, (void*)s->dGather, (void*)s->dTasks,
+                    (void*)s->dCounters})
+        if (p) (void)hipFree(p);
+    delete s;
+}
+
+int gnd_hip_server_add_scene(gnd_hip_server* s, const gnd_hip_scene_arrays* arrays, int32_t* scene_id) {
+    if (!s || !arrays || !arrays->desc || !scene_id) return GND_E_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    SrvScene sc;
+    sc.arrays = arrays;
+    s->scenes.push_back(sc);
+    *scene_id = (int32_t)s->scenes.size() - 1;
+    return GND_OK;
+}
+
+int gnd_hip_server_submit(gnd_hip_server* s, int32_t scene_id, const gnd_hip_job* job, uint64_t* ticket) {
+    if (!s || !job || !ticket || job->spp == 0 || !job->film || job->camera.width <= 0 || job->camera.height <= 0)
+        return GND_E_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (scene_id < 0 || scene_id >= (int32_t)s->scenes.size()) return GND_E_INVALID;
+    if ((uint64_t)job->camera.width * (uint64_t)job->camera.height > s->cfg.path_slots) {
+        s->ctx->err = "one pass of the job needs more path slots than the server has";
+        return GND_E_NOMEM;
+    }
+    const uint64_t t = s->nextTicket++;
+    SrvJob& j = s->jobs[t];
+    j.scene = scene_id;
+    j.job = *job;
+    j.tSubmit = std::chrono::steady_clock::now();
+    s->queue.push_back(t);
+    *ticket = t;
+    s->cv.notify_all();
+    return GND_OK;
+}
+
+int gnd_hip_server_poll(gnd_hip_server* s, uint64_t ticket, gnd_hip_job_status* st) {
+    if (!s || !st) return GND_E_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->jobs.find(ticket);
+    if (it == s->jobs.end()) return GND_E_INVALID;
+    statusOf(it->second, st);
+    return GND_OK;
+}
+
+int gnd_hip_server_wait(gnd_hip_server* s, uint64_t ticket, double timeout_s, gnd_hip_job_status* st) {
+    if (!s || !st) return GND_E_INVALID;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->jobs.find(ticket);
+    if (it == s->jobs.end()) return GND_E_INVALID;
+    const auto until = std::chrono::steady_clock::now() + std::chrono::duration<double>(timeout_s);
+    s->done.wait_until(lk, until, [&] { return it->second.state == 2 || it->second.state < 0; });
+    statusOf(it->second, st);
+    return GND_OK;
+}
+
+int gnd_hip_server_stats(gnd_hip_server* s, gnd_hip_stats* stats, int32_t* resident_scenes, size_t* resident_bytes) {
+    if (!s) return GND_E_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    return guarded(s->ctx, [&] {
+        std::lock_guard<std::mutex> g(s->ctx->mu);
+        if (stats) {
+            Counters c;
+            GND_CHECK(hipMemcpyAsync(&c, s->dCounters, sizeof(c), hipMemcpyDeviceToHost, s->ctx->stream));
+            GND_CHECK(hipStreamSynchronize(s->ctx->stream));
+            std::memset(stats, 0, sizeof(*stats));
+            uint64_t samples = 0;
+            for (auto& kv : s->jobs)
+                if (kv.second.state == 2)
+                    samples += (uint64_t)kv.second.job.camera.width * kv.second.job.camera.height * kv.second.job.spp;
+            fillStats(stats, c, samples);
+        }
+        int n = 0;
+        for (const SrvScene& sc : s->scenes) n += sc.dev ? 1 : 0;
+        if (resident_scenes) *resident_scenes = n;
+        if (resident_bytes) *resident_bytes = residentBytes(s);
+    });
 }
 
 }  // extern "C"

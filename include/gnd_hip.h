@@ -191,6 +191,48 @@ int gnd_hip_copy_probe(gnd_hip_ctx* ctx, size_t bytes, int repeats, double* ms_p
  * counterpart in the reference (its CPU render has no launch shape). */
 int gnd_hip_autotune(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* job, gnd_hip_tuning* out);
 
+/* ---- serving ---------------------------------------------------------------
+ * No counterpart in the reference (it renders one scene per process,
+ * include/gianduia/core/integrator.h:49-155).  A server owns a pool of path
+ * slots and renders queued jobs by continuous batching: between bounces it
+ * admits passes of queued jobs (in submission order) into free slots, and
+ * gathers each job's finished passes into its film in pass order, so a film
+ * is bit-identical to gnd_hip_render of the same job alone.  Scenes stay
+ * resident in HBM until the budget needs their space (least recently used
+ * scene without running jobs first).  A worker thread drives the engine;
+ * every call is thread-safe. */
+typedef struct gnd_hip_server gnd_hip_server;
+
+typedef struct gnd_hip_server_config {
+    size_t hbm_budget;     /* bytes for path slots + resident scenes; 0 = 90% of HBM */
+    uint32_t path_slots;   /* paths in flight (248 B each); a pass needs W*H of them */
+    uint32_t max_jobs;     /* jobs admitted at once                               */
+} gnd_hip_server_config;
+
+typedef struct gnd_hip_job_status {
+    int32_t state;         /* 0 queued, 1 running, 2 done, < 0 failed (GND_E_*)   */
+    uint32_t passes_done;  /* passes gathered into the film                      */
+    double queue_ms;       /* submission -> first admission                      */
+    double run_ms;         /* first admission -> film complete                   */
+    double latency_ms;     /* submission -> film complete (or failure)           */
+} gnd_hip_job_status;
+
+int gnd_hip_server_create(gnd_hip_ctx* ctx, const gnd_hip_server_config* cfg, gnd_hip_server** out);
+/* Stops the worker (jobs in flight are abandoned) and frees the server. */
+void gnd_hip_server_destroy(gnd_hip_server* srv);
+/* Adds a host-built scene to the pool; `arrays` stays borrowed until the
+ * server is destroyed (scenes are uploaded on first use and re-uploaded after
+ * eviction). */
+int gnd_hip_server_add_scene(gnd_hip_server* srv, const gnd_hip_scene_arrays* arrays, int32_t* scene_id);
+/* Queues a job (job->film: device memory the caller keeps until it is done). */
+int gnd_hip_server_submit(gnd_hip_server* srv, int32_t scene_id, const gnd_hip_job* job, uint64_t* ticket);
+int gnd_hip_server_poll(gnd_hip_server* srv, uint64_t ticket, gnd_hip_job_status* status);
+/* Waits until the job is done or failed, or timeout_s passes. */
+int gnd_hip_server_wait(gnd_hip_server* srv, uint64_t ticket, double timeout_s, gnd_hip_job_status* status);
+/* Totals since creation (samples of finished jobs, rays, counters). */
+int gnd_hip_server_stats(gnd_hip_server* srv, gnd_hip_stats* stats, int32_t* resident_scenes,
+                         size_t* resident_bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,93 @@
+"""Continuous-batching server (gnd_hip_server_*): jobs admitted between
+bounces into free path slots give films bit-identical to rendering each job
+alone with gnd_hip_render; scenes are evicted and re-uploaded under a small
+HBM budget; oversize jobs are refused."""
+import numpy as np
+import pytest
+
+from gianduia_amd import DeviceScene, GndError, HostScene
+from gianduia_amd import _abi as A
+from gianduia_amd.serve import Server, poisson_load, random_job
+
+pytestmark = pytest.mark.gpu
+
+
+def _jobs(hs, n, seed):
+    rng = np.random.default_rng(seed)
+    return [random_job(rng, hs, res=(32, 96), spp=(1, 6)) for _ in range(n)]
+
+
+def test_server_films_equal_solo_render(device):
+    a = HostScene.generate("cbox", "width=48;height=40")
+    b = HostScene.generate("volumetric", "grid=16;width=40;height=32")
+    c = HostScene.generate("scaled", "level=3;instances=6;width=40;height=32")
+    srv = Server(device, path_slots=20000, max_jobs=8)   # small pool: passes wait for slots
+    try:
+        ids = [srv.add_scene(h) for h in (a, b, c)]
+        work = []
+        for sid, hs, seed in ((ids[0], a, 1), (ids[1], b, 2), (ids[2], c, 3), (ids[0], a, 4)):
+            for j in _jobs(hs, 4, seed):
+                work.append((sid, hs, j))
+        tickets = [(hs, j, srv.submit(sid, j)) for sid, hs, j in work]
+        for hs, j, tk in tickets:
+            st = srv.wait(tk, 120)
+            assert st["state"] == 2 and st["passes_done"] == j.spp
+        for hs, j, tk in tickets:
+            ds = DeviceScene(device, hs)
+            solo, _ = device.render(ds, [j])
+            assert np.array_equal(srv.film(tk), solo[0]), "served film differs from the solo render"
+            ds.free()
+        st = srv.stats()
+        assert st["samples"] == sum(j.camera.width * j.camera.height * j.spp for _, j, _ in tickets)
+    finally:
+        srv.close()
+
+
+def test_server_evicts_under_budget(device):
+    scenes = [HostScene.generate("scaled", f"level=3;instances=4;seed={i};width=32;height=32") for i in range(3)]
+    probe = DeviceScene(device, scenes[0])
+    one = probe.nbytes
+    probe.free()
+    slots = 16384  # a pass of the largest job (96 x 96) fits
+    # path state (248 B per slot) plus room for one scene but not two
+    srv = Server(device, path_slots=slots, max_jobs=4, hbm_budget=248 * slots + int(1.5 * one))
+    try:
+        ids = [srv.add_scene(h) for h in scenes]
+        tickets = []
+        for rnd in range(2):
+            for sid, hs in zip(ids, scenes):
+                j = _jobs(hs, 1, 10 * rnd + sid)[0]
+                tickets.append((hs, j, srv.submit(sid, j)))
+                srv.wait(tickets[-1][2], 60)
+                assert srv.stats()["resident_scenes"] == 1
+        for hs, j, tk in tickets:
+            ds = DeviceScene(device, hs)
+            assert np.array_equal(srv.film(tk), device.render(ds, [j])[0][0])
+            ds.free()
+    finally:
+        srv.close()
+
+
+def test_server_rejects_oversize_and_bad_scene(device):
+    hs = HostScene.generate("cbox", "width=64;height=64")
+    srv = Server(device, path_slots=2048, max_jobs=2)
+    try:
+        sid = srv.add_scene(hs)
+        with pytest.raises(GndError) as e:
+            srv.submit(sid, hs.default_job(spp=1))   # 4096 paths per pass > 2048 slots
+        assert e.value.code == A.GND_E_NOMEM
+        with pytest.raises(GndError):
+            srv.submit(sid + 5, hs.default_job(spp=1))
+    finally:
+        srv.close()
+
+
+def test_poisson_load_completes(device):
+    pool = [HostScene.generate("procedural", f"tris={t};seed={i}") for i, t in enumerate((2e4, 2e5))]
+    srv = Server(device, path_slots=2_000_000, max_jobs=64)
+    try:
+        ids = [srv.add_scene(h) for h in pool]
+        r = poisson_load(srv, ids, qps=50, n_jobs=20, seed=3, res=(64, 256), spp=(1, 8))
+        assert r["jobs"] == 20 and r["p99_ms"] >= r["p50_ms"] > 0
+    finally:
+        srv.close()
