@@ -161,6 +161,48 @@ def batch_workload(dev, ds, hs):
     return res
 
 
+def serve_workload(qps_levels=(5, 10, 20, 50, 100, 200), jobs_per_level=40):
+    """Serving workload: a pool of 8 procedural scenes (1e4..5e6 triangles,
+    log-uniform, seeded), Poisson arrivals at each rate, jobs with random pose,
+    width log-uniform in 128..1920 (height 9/16..1 of it), spp log-uniform in
+    4..64, rendered by the continuous-batching server.  Latency is submission
+    to film complete."""
+    from gianduia_amd import Device
+    from gianduia_amd.serve import Server, poisson_load, scene_pool
+    dev = Device(0)
+    t0 = time.time()
+    pool = scene_pool()
+    build_ms = []
+    for hs in pool:
+        hs.build()
+        build_ms.append(round(hs.build_ms, 1))
+    log(f"serve: scene pool built in {time.time() - t0:.1f} s ({build_ms} ms BVH builds)")
+    srv = Server(dev, path_slots=16_000_000, max_jobs=256)
+    ids = [srv.add_scene(h) for h in pool]
+    tris = []
+    for hs in pool:
+        d = hs.desc().contents
+        tris.append(sum(d.meshes[d.prims[i].mesh].ntris for i in range(d.nprims) if d.prims[i].shape_type == 0))
+    # first use uploads each scene: one small job per scene, timed apart
+    t1 = time.perf_counter()
+    poisson_load(srv, ids, qps=1000.0, n_jobs=len(ids) * 2, seed=99, res=(64, 65), spp=(1, 2))
+    upload_s = time.perf_counter() - t1
+    levels = []
+    for k, qps in enumerate(qps_levels):
+        r = poisson_load(srv, ids, qps=qps, n_jobs=jobs_per_level, seed=k)
+        log(f"serve: {json.dumps(r)}")
+        levels.append(r)
+    st = srv.stats()
+    srv.close()
+    dev.close()
+    return {"metric": "p50/p99 job latency @QPS", "unit": "ms", "mode": "serve", "n_gpus": 1,
+            "data": "synthetic (procedural scenes and jobs, seeded)",
+            "config": {"scenes": len(pool), "instanced_triangles": tris, "path_slots": 16_000_000,
+                       "jobs_per_level": jobs_per_level, "width": [128, 1920], "spp": [4, 64]},
+            "scene_upload_s": round(upload_s, 2), "levels": levels,
+            "rays_closest": st["rays_closest"], "rays_shadow": st["rays_shadow"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -169,6 +211,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-counters", action="store_true", help="skip the untimed counting pass")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-job batching workload")
+    ap.add_argument("--serve", action="store_true",
+                    help="only run the serving workload (Poisson arrivals over a scene pool) and print its JSON")
     ap.add_argument("--params", default="", help="scene generator overrides, key=value;...")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration) and exit")
@@ -180,6 +224,9 @@ def main():
         print(json.dumps({"copy_probe_mib": a.copy_probe, "ms_per_copy": ms,
                           "gbs_read_plus_write": 2 * (a.copy_probe << 20) / (ms / 1e3) / 1e9}))
         dev.close()
+        return
+    if a.serve:
+        print(json.dumps(serve_workload()), flush=True)
         return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",

@@ -355,7 +355,7 @@ std::unique_ptr<HostScene> proceduralScene(const Params& p) {
     while (level < 9 && 20.0 * std::pow(4.0, level + 1) <= tris * 0.8) ++level;
     double rest = std::max(0.0, tris - 20.0 * std::pow(4.0, level));
     q["level"] = std::to_string(level);
-    q["instances"] = std::to_string((int)std::min(256.0, rest / 10000.0));
+    q["instances"] = std::to_string((int)std::min(1024.0, std::round(rest / 10000.0)));
     return scaledScene(q);
 }
 

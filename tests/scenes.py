@@ -105,7 +105,10 @@ class SceneBuilder:
         d.integrator = A.GND_INTEGRATOR_PATH
         d.max_depth = 4
         d.spp = 1
-        self.keep.append(d)
+        # the desc holds raw pointers into arrays this builder owns: keep the
+        # builder alive as long as the desc (`soup_scene(...).desc()` would
+        # otherwise free them before the library copies the scene)
+        d._builder = self
         return d
 
 

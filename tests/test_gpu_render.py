@@ -94,6 +94,31 @@ def test_single_sample_replay(device, name, params, min_frac):
     assert frac >= min_frac
 
 
+@pytest.mark.parametrize("name,params,n,ref_spp", [("cbox", "width=48;height=48", 16, 2048),
+                                                   ("volumetric", "grid=32;width=40;height=32", 16, 2048)])
+def test_equal_image_error(device, name, params, n, ref_spp):
+    """relMSE(GPU@n vs CPU@ref_spp) within 2% of relMSE(CPU@n vs CPU@ref_spp);
+    the high-spp CPU reference uses independent random streams (seed offset)."""
+    hs = HostScene.generate(name, params)
+    ds = DeviceScene(device, hs)
+    ref = RefScene(hs.desc())
+    cam = hs.camera()
+    W, H = cam.width, cam.height
+    print(f"equal image error {name}: CPU reference {ref_spp} spp", flush=True)
+    hi, _ = ref.render(threads=16, spp=ref_spp, seed_offset=1 << 40, width=W, height=H)
+    cpu, _ = ref.render(threads=16, spp=n, width=W, height=H)
+    gpu, _ = device.render(ds, [hs.default_job(spp=n)])
+    r = resolve(hi)
+
+    def relmse(x):
+        img = resolve(x)
+        return float(np.mean((img - r) ** 2 / (r * r + 1e-2)))
+
+    e_gpu, e_cpu = relmse(gpu[0]), relmse(cpu)
+    print(f"{name}: relMSE GPU {e_gpu:.6f} CPU {e_cpu:.6f}", flush=True)
+    assert e_gpu <= 1.02 * e_cpu
+
+
 # ---------------------------------------------------------------- materials and thin lens (path)
 def test_scaled_materials_match_oracle(device):
     print("render scaled level=4 instances=12 96x54 32spp", flush=True)
