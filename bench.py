@@ -9,9 +9,14 @@ depth 16, box filter, path integrator, FP32 — rendered through the C-ABI
 (gnd_hip_render) with the scene and film resident in HBM.
 
 Prints one JSON line: Msamples/s (whole job), Mrays/s, the roofline of the
-dominant kernel (closest-hit BVH4 traversal) and the CPU baseline (the C++
+dominant kernel (closest-hit BVH4 traversal), the CPU baseline (the C++
 restatement of the reference CPU integrator on the host cores, on a bounded
-band of rows of the same frame).
+sample of the same frame) and the 64-job batching workload.
+
+--scene cbox|volumetric measures the Cornell-box (512x512, 64 spp, depth 8)
+and volumetric (1280x720, 32 spp, depth 32) workloads instead; --serve runs
+the serving workload (Poisson arrivals over a pool of 8 scenes) and prints
+its own JSON line.
 """
 from __future__ import annotations
 
@@ -37,6 +42,13 @@ def algorithmic_bytes(rays, nodes, packs, insts):
     return (RAY_BYTES + HIT_BYTES) * rays + NODE_BYTES * nodes + PACK_BYTES * packs + INST_BYTES * insts
 
 
+WORKLOADS = {
+    "scaled": "scaled-geometry path tracing: 1,310,720-tri displaced icosphere + 64 instanced 10k-tri spheres, "
+              "GGX conductor/glass/plastic, quad light, thin lens 0.05",
+    "cbox": "Cornell box path tracing: 36 triangles, diffuse, quad light, pinhole fov 40",
+    "volumetric": "volumetric path tracing: Cornell box + glass sphere with a homogeneous medium (sigma_t 2, "
+                  "albedo 0.8, g 0.3) + 256^3 fBm density grid (sigma_t max 8) in an index box",
+}
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "kTracePaths_scaled1080p_hbm_traffic.md")
 
 
@@ -94,8 +106,9 @@ def cpu_baseline(hs, target_s=12.0):
         samples += n2
         passes += more
         desc = f"passes 0-{passes - 1} (full frames)"
+    integ = "volpath" if hs.desc().contents.integrator == 1 else "path"
     return {"value": samples / secs / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
-            "sample": f"{desc} of the same {W}x{H} frame ({samples} camera samples, {secs:.1f} s, path "
+            "sample": f"{desc} of the same {W}x{H} frame ({samples} camera samples, {secs:.1f} s, {integ} "
                       f"integrator, C++ restatement of the reference CPU integrator, {threads} threads)"}
 
 
@@ -213,6 +226,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-job batching workload")
     ap.add_argument("--serve", action="store_true",
                     help="only run the serving workload (Poisson arrivals over a scene pool) and print its JSON")
+    ap.add_argument("--scene", default="scaled", choices=sorted(WORKLOADS),
+                    help="workload scene: scaled (default, 1080p 1M tris), cbox (512^2 Cornell box), "
+                         "volumetric (720p, homogeneous + 256^3 grid media)")
     ap.add_argument("--params", default="", help="scene generator overrides, key=value;...")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration) and exit")
@@ -234,13 +250,14 @@ def main():
         sys.exit(2)
 
     from gianduia_amd import Device, DeviceScene, HostScene
+    from gianduia_amd import _abi as A
     from gianduia_amd.api import RenderSession
 
     dev = Device(0)
     info = dev.info()
     log(f"device {info}")
     t0 = time.time()
-    hs = HostScene.generate("scaled", a.params)
+    hs = HostScene.generate(a.scene, a.params)
     gen_s = time.time() - t0
     hs.build()
     ds = DeviceScene(dev, hs)
@@ -281,7 +298,7 @@ def main():
         trace_ms = sum(s["ms_trace"] for s in per_step) / a.steps
         launches = per_step[0]["trace_launches"]
         achieved = byts / (trace_ms / 1000.0) / 1e9
-        traffic = measured_traffic() if not a.params else None
+        traffic = measured_traffic() if not a.params and a.scene == "scaled" else None
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT) if traffic else None,
@@ -298,7 +315,7 @@ def main():
         log(f"counting pass: {json.dumps(roof)}")
 
     batch = None
-    if not a.no_batch:
+    if not a.no_batch and a.scene == "scaled":
         batch = batch_workload(dev, ds, hs)
         log(f"batch workload: {json.dumps(batch)}")
 
@@ -315,10 +332,10 @@ def main():
         "metric": METRIC, "value": round(msps, 3), "unit": "Msamples/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic (procedural scene built in code, seeded)",
-        "config": {"workload": "scaled-geometry path tracing: 1,310,720-tri displaced icosphere + 64 instanced "
-                               "10k-tri spheres, GGX conductor/glass/plastic, quad light, thin lens 0.05",
+        "config": {"workload": WORKLOADS[a.scene],
                    "width": cam.width, "height": cam.height, "spp": job.spp, "max_depth": job.max_depth,
-                   "instanced_triangles": inst_tris, "integrator": "path"},
+                   "instanced_triangles": inst_tris,
+                   "integrator": "volpath" if job.integrator == A.GND_INTEGRATOR_VOLPATH else "path"},
         "mrays_per_s": round(mrays, 2),
         "trace_ms_per_step": round(sum(s["ms_trace"] for s in per_step) / a.steps, 3),
         "shadow_ms_per_step": round(sum(s["ms_shadow"] for s in per_step) / a.steps, 3),
