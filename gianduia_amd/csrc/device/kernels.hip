@@ -70,6 +70,7 @@ struct DChunk {     // one pass of one job: npix path slots from slotBase
     int32_t job, pass;
     uint32_t first;  // index of its first path in the camera launch that starts it
     int32_t id;      // engine chunk id (continuous batching), -1 otherwise
+    uint32_t qBase;  // in-queue position of its first live path, after WaveArgs::camQueueBase
 };
 
 struct DSplatTask { // one job inside a wave: its chunks in pass order
@@ -104,6 +105,39 @@ GDEV_INLINE void retirePaths(uint32_t* chunkLive, bool ending, int chunk) {
         if ((int)__lane_id() == leader) atomicSub(&chunkLive[c], (uint32_t)__popcll(same));
         m &= ~same;
     }
+}
+
+// Block-aggregated append to two queues: one atomic per block and queue
+// (a counter shared by the whole launch serialises its atomics, so fewer is
+// faster).  Every thread of the block calls this; it returns this lane's
+// slots in the two queues.  c1 may be null.
+constexpr int kShadeBlock = 512;
+GDEV_INLINE void blockAppend2(uint32_t* c0, bool p0, uint32_t& pos0, uint32_t* c1, bool p1, uint32_t& pos1) {
+    __shared__ uint32_t sN[2][kShadeBlock / 64];
+    __shared__ uint32_t sBase[2];
+    const int w = (int)(threadIdx.x >> 6), nw = (int)(blockDim.x >> 6);
+    const uint64_t m0 = __ballot(p0), m1 = __ballot(c1 && p1);
+    if (__lane_id() == 0) {
+        sN[0][w] = (uint32_t)__popcll(m0);
+        sN[1][w] = (uint32_t)__popcll(m1);
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        uint32_t total = 0;
+        for (int i = 0; i < nw; ++i) total += sN[threadIdx.x][i];
+        uint32_t* c = threadIdx.x == 0 ? c0 : c1;
+        sBase[threadIdx.x] = (total && c) ? atomicAdd(c, total) : 0u;
+    }
+    __syncthreads();
+    uint32_t pre0 = 0, pre1 = 0;
+    for (int i = 0; i < w; ++i) {
+        pre0 += sN[0][i];
+        pre1 += sN[1][i];
+    }
+    const uint64_t lt = (1ull << __lane_id()) - 1ull;
+    pos0 = sBase[0] + pre0 + (uint32_t)__popcll(m0 & lt);
+    pos1 = sBase[1] + pre1 + (uint32_t)__popcll(m1 & lt);
+    __syncthreads();  // sN / sBase are reused by the next call
 }
 
 // wave-aggregated append: ballot, popcount, one atomic per wave
@@ -211,6 +245,7 @@ struct WaveArgs {
     int32_t* qOut;
     uint32_t* counts;     // [0] in, [1] out, [2] shadow, [3] [4] ray fetch
     uint32_t* chunkLive;  // live paths per engine chunk, or null
+    uint32_t camQueueBase;  // in-queue entries before the paths kCamera starts
     Counters* ctr;
     int32_t pathOnlyFast; // shadows may use any-hit
     int32_t split;        // NEE and scatter in separate kernels
@@ -269,9 +304,8 @@ __global__ void kCamera(WaveArgs A) {
     P.flags[slot] = F_SPECULAR | ((uint32_t)(chan + 1) << 1);  // medium -1 -> field 0
     P.job[slot] = ch.job;
     P.chunk[slot] = ch.id;
-    const bool live = J.maxDepth > 0;
-    const uint32_t q = waveAppend(&A.counts[0], live);
-    if (live) A.qIn[q] = (int32_t)slot;
+    // the host laid out the in-queue: no atomics, and queue order = slot order
+    if (J.maxDepth > 0) A.qIn[A.camQueueBase + ch.qBase + pix] = (int32_t)slot;
 }
 
 // Persistent traversal kernels hand out rays with one atomic per wave on a
@@ -282,24 +316,60 @@ __global__ void kCamera(WaveArgs A) {
 #define GND_REFILL_LANES 24
 #endif
 constexpr int kRefill = GND_REFILL_LANES;
-constexpr uint32_t kFetchTrace = 3, kFetchShadow = 4;  // counter slots in WaveArgs::counts
+// Queue counters: [0] in-queue size, [1] out-queue, [2] shadow queue, then
+// the ray-feed part counters of the two persistent traversals.
+#ifndef GND_FEED_PARTS
+#define GND_FEED_PARTS 1
+#endif
+constexpr int kParts = GND_FEED_PARTS;
+constexpr uint32_t kFeedTrace = 8, kFeedShadow = kFeedTrace + kParts, kNumCounts = 64;
 
-// Idle lanes of the wave take the next rays of a queue of n; returns the
-// queue index for this lane, or -1.  Sets `drained` (wave-uniform) once the
-// queue is exhausted.
-GDEV_INLINE int64_t refill(uint32_t* counter, uint32_t n, bool idle, uint64_t waveMask, bool& drained) {
-    const uint64_t m = __ballot(idle);
-    const int k = __popcll(m);
-    if (k == 0 || (k < kRefill && m != waveMask)) return -1;
-    const int leader = __ffsll((unsigned long long)m) - 1;
-    uint32_t base = 0;
-    if ((int)__lane_id() == leader) base = atomicAdd(counter, (uint32_t)k);
-    base = __shfl(base, leader);
-    if ((uint64_t)base + (uint64_t)k >= n) drained = true;
-    if (!idle) return -1;
-    const uint32_t i = base + (uint32_t)__popcll(m & ((1ull << __lane_id()) - 1ull));
-    return i < n ? (int64_t)i : -1;
-}
+// The persistent traversals' ray feed: waves take rays in queue order with
+// one atomic per refill.  The queue can be cut into kParts contiguous parts
+// with a counter each (a wave starts on part blockIdx.x % kParts and moves on
+// when its part is used up), which spreads the atomics over more addresses;
+// measured on the 1080p scene, one part is fastest (1: 71 ms, 4: 74 ms,
+// 16: 76 ms of traversal per frame): with a single counter every wave works
+// on the same region of the queue, so the rays in flight are coherent and
+// share BVH nodes in L2.
+struct RayFeed {
+    uint32_t* ctr;
+    uint32_t n;
+    int part, tried;  // wave-uniform
+    GDEV_INLINE void init(uint32_t* counters, uint32_t count) {
+        ctr = counters;
+        n = count;
+        part = (int)(blockIdx.x % (unsigned)kParts);
+        tried = 0;
+    }
+    GDEV_INLINE uint32_t edge(int p) const { return (uint32_t)(((uint64_t)n * (uint64_t)p) / kParts); }
+    // Idle lanes take the next rays (once kRefill lanes, or the whole wave,
+    // are idle); returns this lane's queue index or -1.  Sets `drained`
+    // (wave-uniformly) once every part is used up.
+    GDEV_INLINE int64_t take(bool idle, uint64_t waveMask, bool& drained) {
+        const uint64_t m = __ballot(idle);
+        const int k = __popcll(m);
+        if (k == 0 || (k < kRefill && m != waveMask)) return -1;
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        while (tried < kParts) {
+            const uint32_t lo = edge(part), len = edge(part + 1) - lo;
+            uint32_t base = 0;
+            if ((int)__lane_id() == leader) base = atomicAdd(&ctr[part], (uint32_t)k);
+            base = __shfl(base, leader);
+            if (base >= len || base + (uint32_t)k >= len) {
+                part = (part + 1) % kParts;
+                tried++;
+            }
+            if (base < len) {
+                if (!idle) return -1;
+                const uint32_t r = base + (uint32_t)__popcll(m & ((1ull << __lane_id()) - 1ull));
+                return r < len ? (int64_t)(lo + r) : -1;
+            }
+        }
+        drained = true;
+        return -1;
+    }
+};
 
 // closest hit for the live path rays (Scene::rayIntersect at the top of Li)
 template <int LDSD, bool COUNT>
@@ -311,11 +381,13 @@ GND_TRACE_KERNEL void kTracePaths(WaveArgs A) {
     uint32_t maxRay = 0, bad = 0;
     TravStack<LDSD> st = makeStack<LDSD>(sStack);
     const PathSoA& P = A.P;
+    RayFeed feed;
+    feed.init(A.counts + kFeedTrace, n);
     int64_t cur = -1;  // queue index of this lane's ray
     traceStream<false, COUNT, LDSD>(
         A.S, st, cnt,
         [&](bool idle, V3& o, V3& d, float& tmin, float& tmax, bool& drained) {
-            const int64_t i = refill(&A.counts[kFetchTrace], n, idle, waveMask, drained);
+            const int64_t i = feed.take(idle, waveMask, drained);
             if (i < 0) return false;
             cur = i;
             const int32_t p = A.qIn[i];
@@ -561,7 +633,7 @@ GDEV_INLINE int neeStep(const WaveArgs& A, const DJob& J, PathLocal& s, const Hi
 // afterwards (valid when shadow rays consume no random numbers).
 // SPLIT = true: NEE only; the event is stored for kScatter.
 template <bool SPLIT>
-GND_WAVE_KERNEL void kShade(WaveArgs A) {
+__global__ __launch_bounds__(kShadeBlock) void kShade(WaveArgs A) {
     const uint32_t n = A.counts[0];
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t start = blockIdx.x * blockDim.x;
@@ -615,9 +687,9 @@ GND_WAVE_KERNEL void kShade(WaveArgs A) {
             ending = !alive && !(SPLIT && r == 2);
         }
         retirePaths(A.chunkLive, ending, ending && A.chunkLive ? A.P.chunk[p] : -1);
-        const uint32_t qs = waveAppend(&A.counts[1], alive);
+        uint32_t qs, ss;
+        blockAppend2(&A.counts[1], alive, qs, &A.counts[2], shadow, ss);
         if (alive) A.qOut[qs] = p;
-        const uint32_t ss = waveAppend(&A.counts[2], shadow);
         if (shadow) {
             const ShadowSoA& Sh = A.Sh;
             Sh.path[ss] = p;
@@ -633,7 +705,7 @@ GND_WAVE_KERNEL void kShade(WaveArgs A) {
 
 // Split mode second half: the paths that recorded an event scatter now (after
 // their shadow rays consumed transmittance samples).
-GND_WAVE_KERNEL void kScatter(WaveArgs A) {
+__global__ __launch_bounds__(kShadeBlock) void kScatter(WaveArgs A) {
     const uint32_t nEv = A.counts[0];
     const int32_t* qEv = A.qIn;
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -662,7 +734,8 @@ GND_WAVE_KERNEL void kScatter(WaveArgs A) {
             }
         }
         retirePaths(A.chunkLive, ending, ending && A.chunkLive ? A.P.chunk[p] : -1);
-        const uint32_t qs = waveAppend(&A.counts[1], alive);
+        uint32_t qs, unused;
+        blockAppend2(&A.counts[1], alive, qs, nullptr, false, unused);
         if (alive) A.qOut[qs] = p;
     }
 }
@@ -678,11 +751,13 @@ __global__ __launch_bounds__(256, GND_TRACE_WAVES + 1) void kShadowAny(WaveArgs 
     const DScene& S = A.S;
     const ShadowSoA& Sh = A.Sh;
     const uint64_t waveMask = __ballot(1);
+    RayFeed feed;
+    feed.init(A.counts + kFeedShadow, n);
     int64_t cur = -1;
     traceStream<true, COUNT, LDSD>(
         S, st, cnt,
         [&](bool idle, V3& o, V3& d, float& tmin, float& tmax, bool& drained) {
-            const int64_t i = refill(&A.counts[kFetchShadow], n, idle, waveMask, drained);
+            const int64_t i = feed.take(idle, waveMask, drained);
             if (i < 0) return false;
             cur = i;
             o = V3{Sh.ox[i], Sh.oy[i], Sh.oz[i]};
@@ -852,15 +927,16 @@ __global__ void kFillU32(uint32_t* p, size_t n, uint32_t v) {
 // ray-fetch counters restart at zero (a kernel, so the step can be captured
 // in a graph).
 __global__ void kRotate(uint32_t* counts, Counters* c) {
-    if (threadIdx.x == 0) {
+    const uint32_t t = threadIdx.x;
+    if (t == 0) {
         c->rays += counts[0];
         c->srays += counts[2];
         if (counts[0]) c->launches += 1ull;
         counts[0] = counts[1];
         counts[1] = 0u;
         counts[2] = 0u;
-        counts[3] = 0u;
-        counts[4] = 0u;
+    } else if (t >= kFeedTrace && t < kNumCounts) {
+        counts[t] = 0u;
     }
 }
 
@@ -883,7 +959,7 @@ struct WaveBuffers {
     HitSoA Hq{};
     ShadowSoA Sh{};
     int32_t* q[2] = {nullptr, nullptr};
-    uint32_t* counts = nullptr;  // 16 queue / fetch counters
+    uint32_t* counts = nullptr;  // kNumCounts queue and ray-feed counters
 };
 
 struct gnd_hip_ctx {
@@ -1025,7 +1101,7 @@ void freeWaveBuffers(WaveBuffers& wb) {
 }
 
 void ensureWaveBuffers(WaveBuffers& wb, size_t cap) {
-    if (!wb.counts) GND_CHECK(hipMalloc(&wb.counts, 16 * sizeof(uint32_t)));
+    if (!wb.counts) GND_CHECK(hipMalloc(&wb.counts, kNumCounts * sizeof(uint32_t)));
     if (cap <= wb.capacity) return;
     if (wb.pathBuf) GND_CHECK(hipFree(wb.pathBuf));
     wb.pathBuf = nullptr;
@@ -1472,6 +1548,7 @@ struct WavePlan {
     std::vector<DChunk> chunks;
     std::vector<DSplatTask> tasks;
     uint32_t slots = 0, pix = 0;
+    uint32_t live = 0;  // paths with max depth > 0 (the first in-queue)
     int32_t maxDepth = 0;
 };
 
@@ -1489,7 +1566,7 @@ static std::vector<WavePlan> planWaves(const std::vector<gnd_hip_job>& jobs, siz
             if (next[j] >= jobs[j].spp) continue;
             int first = (int)w.chunks.size();
             while (next[j] < jobs[j].spp && (w.slots + npix <= cap || (w.slots == 0))) {
-                w.chunks.push_back({w.slots, npix, (int32_t)j, (int32_t)next[j], w.slots, -1});
+                w.chunks.push_back({w.slots, npix, (int32_t)j, (int32_t)next[j], w.slots, -1, 0u});
                 w.slots += npix;
                 next[j]++;
                 any = true;
@@ -1503,6 +1580,11 @@ static std::vector<WavePlan> planWaves(const std::vector<gnd_hip_job>& jobs, siz
             if (w.slots >= cap) break;
         }
         if (!any) break;
+        // in-queue layout of the camera paths: live chunks back to back
+        for (DChunk& c : w.chunks) {
+            c.qBase = w.live;
+            if (jobs[c.job].max_depth > 0) w.live += c.npix;
+        }
         waves.push_back(std::move(w));
     }
     return waves;
@@ -1520,6 +1602,13 @@ struct BouncePlan {
     unsigned grid(uint32_t n) const {
         unsigned b = (unsigned)((n + (uint32_t)B - 1) / (uint32_t)B);
         b = std::min(b, gridCap);
+        return std::max(8u, (b + 7) / 8 * 8);
+    }
+    // shading kernels: kShadeBlock threads per block, the same thread cap
+    unsigned shadeGrid(uint32_t n) const {
+        const unsigned cap = std::max(8u, gridCap * (unsigned)B / (unsigned)kShadeBlock);
+        unsigned b = (unsigned)((n + (uint32_t)kShadeBlock - 1) / (uint32_t)kShadeBlock);
+        b = std::min(b, cap);
         return std::max(8u, (b + 7) / 8 * 8);
     }
 };
@@ -1587,8 +1676,9 @@ static void launchBounce(gnd_hip_ctx* ctx, const WaveArgs& A, const BouncePlan& 
     hipLaunchKernelGGL(K.trace, dim3(g), dim3(K.B), K.shm, ctx->stream, A);
     GND_CHECK(hipGetLastError());
     if (tl) tl->lap(t, 0);
-    if (K.split) hipLaunchKernelGGL(kShade<true>, dim3(g), dim3(K.B), 0, ctx->stream, A);
-    else hipLaunchKernelGGL(kShade<false>, dim3(g), dim3(K.B), 0, ctx->stream, A);
+    const unsigned gs = K.shadeGrid(live);
+    if (K.split) hipLaunchKernelGGL(kShade<true>, dim3(gs), dim3(kShadeBlock), 0, ctx->stream, A);
+    else hipLaunchKernelGGL(kShade<false>, dim3(gs), dim3(kShadeBlock), 0, ctx->stream, A);
     GND_CHECK(hipGetLastError());
     if (tl) tl->lap(t, 2);
     // NEE emits at most one shadow ray per path
@@ -1598,7 +1688,7 @@ static void launchBounce(gnd_hip_ctx* ctx, const WaveArgs& A, const BouncePlan& 
     if (K.split) {
         // the paths of this bounce with an event scatter after their shadow
         // rays consumed the transmittance samples
-        hipLaunchKernelGGL(kScatter, dim3(g), dim3(K.B), 0, ctx->stream, A);
+        hipLaunchKernelGGL(kScatter, dim3(gs), dim3(kShadeBlock), 0, ctx->stream, A);
         GND_CHECK(hipGetLastError());
     }
     hipLaunchKernelGGL(kRotate, dim3(1), dim3(64), 0, ctx->stream, A.counts, A.ctr);
@@ -1697,7 +1787,7 @@ int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jo
                                      hipMemcpyHostToDevice, ctx->stream));
             GND_CHECK(hipMemcpyAsync(ctx->dTasks, w.tasks.data(), w.tasks.size() * sizeof(DSplatTask),
                                      hipMemcpyHostToDevice, ctx->stream));
-            hipLaunchKernelGGL(kFillU32, dim3(1), dim3(64), 0, ctx->stream, ctx->wb.counts, (size_t)16, 0u);
+            hipLaunchKernelGGL(kFillU32, dim3(1), dim3(64), 0, ctx->stream, ctx->wb.counts, (size_t)kNumCounts, 0u);
             WaveArgs A{};
             A.S = scene->view;
             A.jobs = ctx->dJobs;
@@ -1715,11 +1805,13 @@ int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jo
             A.qIn = ctx->wb.q[0];
             A.qOut = ctx->wb.q[1];
             size_t tc = tl.mark();
+            A.camQueueBase = 0;
             hipLaunchKernelGGL(kCamera, dim3((w.slots + 255) / 256), dim3(256), 0, ctx->stream, A);
             GND_CHECK(hipGetLastError());
+            hipLaunchKernelGGL(kFillU32, dim3(1), dim3(64), 0, ctx->stream, ctx->wb.counts, (size_t)1, w.live);
             if (tlp) tl.lap(tc, 2);
             int cur = 0;
-            uint32_t live = w.slots;
+            uint32_t live = w.live;
             if (graph) {
                 // a fixed chain of bounces captured once and launched as one
                 // graph (kernels exit at once on an empty queue); paths still
@@ -2070,11 +2162,12 @@ void serverStep(gnd_hip_server* s) {
     const std::vector<int> fresh = admit(s);
     if (!fresh.empty()) {
         std::vector<DChunk> dc;
-        uint32_t n = 0;
+        uint32_t n = 0, nLive = 0;
         for (int c : fresh) {
             const SrvChunk& ch = s->chunks[c];
-            dc.push_back({ch.slotBase, ch.npix, s->jobs[ch.ticket].slot, (int32_t)ch.pass, n, c});
+            dc.push_back({ch.slotBase, ch.npix, s->jobs[ch.ticket].slot, (int32_t)ch.pass, n, c, nLive});
             n += ch.npix;
+            if (s->jobs[ch.ticket].job.max_depth > 0) nLive += ch.npix;
             GND_CHECK(hipMemcpyAsync(s->dLive + c, &s->hLive[c], sizeof(uint32_t), hipMemcpyHostToDevice,
                                      ctx->stream));
             s->running.push_back(c);
@@ -2087,9 +2180,11 @@ void serverStep(gnd_hip_server* s) {
         A.nSlots = n;
         A.qIn = s->wb.q[s->cur];
         A.qOut = s->wb.q[1 - s->cur];
+        A.camQueueBase = s->live;  // after the survivors of the last bounce
         hipLaunchKernelGGL(kCamera, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, A);
         GND_CHECK(hipGetLastError());
-        s->live += n;
+        s->live += nLive;
+        hipLaunchKernelGGL(kFillU32, dim3(1), dim3(64), 0, ctx->stream, s->wb.counts, (size_t)1, s->live);
     }
     // 2. one bounce for every live path
     if (s->live > 0) {
@@ -2129,7 +2224,7 @@ void serverStep(gnd_hip_server* s) {
         const int first = (int)gc.size();
         for (auto it = j.finished.find(j.nextGather); it != j.finished.end(); it = j.finished.find(j.nextGather)) {
             const SrvChunk& ch = s->chunks[it->second];
-            gc.push_back({ch.slotBase, ch.npix, j.slot, (int32_t)ch.pass, 0u, it->second});
+            gc.push_back({ch.slotBase, ch.npix, j.slot, (int32_t)ch.pass, 0u, it->second, 0u});
             released.push_back(it->second);
             j.finished.erase(it);
             j.nextGather++;
@@ -2207,7 +2302,7 @@ void serverLoop(gnd_hip_server* s) {
             for (SrvScene& sc : s->scenes) sc.running = 0;
             s->running.clear();
             s->live = 0;
-            hipLaunchKernelGGL(kFillU32, dim3(1), dim3(64), 0, s->ctx->stream, s->wb.counts, (size_t)16, 0u);
+            hipLaunchKernelGGL(kFillU32, dim3(1), dim3(64), 0, s->ctx->stream, s->wb.counts, (size_t)kNumCounts, 0u);
             s->done.notify_all();
         }
     }
@@ -2242,7 +2337,7 @@ int gnd_hip_server_create(gnd_hip_ctx* ctx, const gnd_hip_server_config* cfg, gn
         GND_CHECK(hipMalloc(&s->dLive, (size_t)kSrvMaxChunks * sizeof(uint32_t)));
         GND_CHECK(hipMalloc(&s->dCounters, sizeof(Counters)));
         GND_CHECK(hipMemset(s->dCounters, 0, sizeof(Counters)));
-        hipLaunchKernelGGL(kFillU32, dim3(1), dim3(64), 0, ctx->stream, s->wb.counts, (size_t)16, 0u);
+        hipLaunchKernelGGL(kFillU32, dim3(1), dim3(64), 0, ctx->stream, s->wb.counts, (size_t)kNumCounts, 0u);
         GND_CHECK(hipStreamSynchronize(ctx->stream));
     });
     if (rc != GND_OK) {
