@@ -31,8 +31,13 @@ using namespace gdev;
 
 // ============================================================================ device state
 
-struct PathSoA {
+// The ray of each queued path, in queue order (the closest-hit kernel reads
+// rays with coalesced loads and without a path-index indirection).
+struct RayQ {
     float *ox, *oy, *oz, *dx, *dy, *dz;
+};
+
+struct PathSoA {
     float *tpr, *tpg, *tpb, *Lr, *Lg, *Lb;
     float *pdfPrev, *lx, *ly, *lz;
     float *weight, *pfx, *pfy;
@@ -243,6 +248,7 @@ struct WaveArgs {
     ShadowSoA Sh;
     int32_t* qIn;
     int32_t* qOut;
+    RayQ rIn, rOut;       // rays of the in- and out-queue entries
     uint32_t* counts;     // [0] in, [1] out, [2] shadow, [3] [4] ray fetch
     uint32_t* chunkLive;  // live paths per engine chunk, or null
     uint32_t camQueueBase;  // in-queue entries before the paths kCamera starts
@@ -250,6 +256,15 @@ struct WaveArgs {
     int32_t pathOnlyFast; // shadows may use any-hit
     int32_t split;        // NEE and scatter in separate kernels
 };
+
+GDEV_INLINE void storeRay(const RayQ& R, uint32_t i, V3 o, V3 d) {
+    R.ox[i] = o.x; R.oy[i] = o.y; R.oz[i] = o.z;
+    R.dx[i] = d.x; R.dy[i] = d.y; R.dz[i] = d.z;
+}
+GDEV_INLINE void loadRay(const RayQ& R, uint32_t i, V3& o, V3& d) {
+    o = V3{R.ox[i], R.oy[i], R.oz[i]};
+    d = V3{R.dx[i], R.dy[i], R.dz[i]};
+}
 
 // SamplerIntegrator::render per-sample set-up (integrator.h:88-125): seeding,
 // film jitter, lens sample, chromatic-aberration channel, time, camera ray.
@@ -289,8 +304,6 @@ __global__ void kCamera(WaveArgs A) {
     V3 ro, rd;
     const float w = shootRay(J.cam, fx, fy, lx, ly, lambda, ro, rd);
     PathSoA& P = A.P;
-    P.ox[slot] = ro.x; P.oy[slot] = ro.y; P.oz[slot] = ro.z;
-    P.dx[slot] = rd.x; P.dy[slot] = rd.y; P.dz[slot] = rd.z;
     P.tpr[slot] = 1.0f; P.tpg[slot] = 1.0f; P.tpb[slot] = 1.0f;
     P.Lr[slot] = 0.0f; P.Lg[slot] = 0.0f; P.Lb[slot] = 0.0f;
     P.pdfPrev[slot] = 1.0f;
@@ -305,7 +318,11 @@ __global__ void kCamera(WaveArgs A) {
     P.job[slot] = ch.job;
     P.chunk[slot] = ch.id;
     // the host laid out the in-queue: no atomics, and queue order = slot order
-    if (J.maxDepth > 0) A.qIn[A.camQueueBase + ch.qBase + pix] = (int32_t)slot;
+    if (J.maxDepth > 0) {
+        const uint32_t q = A.camQueueBase + ch.qBase + pix;
+        A.qIn[q] = (int32_t)slot;
+        storeRay(A.rIn, q, ro, rd);
+    }
 }
 
 // Persistent traversal kernels hand out rays with one atomic per wave on a
@@ -380,7 +397,6 @@ GND_TRACE_KERNEL void kTracePaths(WaveArgs A) {
     uint32_t cnt[4] = {0, 0, 0, 0};
     uint32_t maxRay = 0, bad = 0;
     TravStack<LDSD> st = makeStack<LDSD>(sStack);
-    const PathSoA& P = A.P;
     RayFeed feed;
     feed.init(A.counts + kFeedTrace, n);
     int64_t cur = -1;  // queue index of this lane's ray
@@ -390,9 +406,7 @@ GND_TRACE_KERNEL void kTracePaths(WaveArgs A) {
             const int64_t i = feed.take(idle, waveMask, drained);
             if (i < 0) return false;
             cur = i;
-            const int32_t p = A.qIn[i];
-            o = V3{P.ox[p], P.oy[p], P.oz[p]};
-            d = V3{P.dx[p], P.dy[p], P.dz[p]};
+            loadRay(A.rIn, (uint32_t)i, o, d);
             if (COUNT && !finite3(d)) bad++;
             tmin = kEps;
             tmax = finf();
@@ -419,8 +433,6 @@ struct PathLocal {
 };
 
 GDEV_INLINE void loadPath(const PathSoA& P, int p, PathLocal& s) {
-    s.o = V3{P.ox[p], P.oy[p], P.oz[p]};
-    s.d = V3{P.dx[p], P.dy[p], P.dz[p]};
     s.tp = V3{P.tpr[p], P.tpg[p], P.tpb[p]};
     s.L = V3{P.Lr[p], P.Lg[p], P.Lb[p]};
     s.last = V3{P.lx[p], P.ly[p], P.lz[p]};
@@ -432,8 +444,6 @@ GDEV_INLINE void loadPath(const PathSoA& P, int p, PathLocal& s) {
     s.job = P.job[p];
 }
 GDEV_INLINE void storePath(const PathSoA& P, int p, const PathLocal& s) {
-    P.ox[p] = s.o.x; P.oy[p] = s.o.y; P.oz[p] = s.o.z;
-    P.dx[p] = s.d.x; P.dy[p] = s.d.y; P.dz[p] = s.d.z;
     P.tpr[p] = s.tp.x; P.tpg[p] = s.tp.y; P.tpb[p] = s.tp.z;
     P.Lr[p] = s.L.x; P.Lg[p] = s.L.y; P.Lb[p] = s.L.z;
     P.lx[p] = s.last.x; P.ly[p] = s.last.y; P.lz[p] = s.last.z;
@@ -649,6 +659,7 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(WaveArgs A) {
         if (active) {
             p = A.qIn[i];
             loadPath(A.P, p, s);
+            loadRay(A.rIn, i, s.o, s.d);
             const DJob& J = A.jobs[s.job];
             HitRec h;
             h.t = A.Hq.t[i];
@@ -689,7 +700,10 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(WaveArgs A) {
         retirePaths(A.chunkLive, ending, ending && A.chunkLive ? A.P.chunk[p] : -1);
         uint32_t qs, ss;
         blockAppend2(&A.counts[1], alive, qs, &A.counts[2], shadow, ss);
-        if (alive) A.qOut[qs] = p;
+        if (alive) {
+            A.qOut[qs] = p;
+            storeRay(A.rOut, qs, s.o, s.d);
+        }
         if (shadow) {
             const ShadowSoA& Sh = A.Sh;
             Sh.path[ss] = p;
@@ -713,12 +727,13 @@ __global__ __launch_bounds__(kShadeBlock) void kScatter(WaveArgs A) {
         const uint32_t i = base + threadIdx.x;
         bool alive = false, ending = false;
         int p = -1;
+        PathLocal s;
         if (i < nEv) {
             p = qEv[i];
             const int kind = A.P.evKind[p];
             if (kind != EV_NONE) {
-                PathLocal s;
                 loadPath(A.P, p, s);
+                loadRay(A.rIn, i, s.o, s.d);
                 Event ev;
                 ev.kind = kind;
                 ev.prim = A.P.evPrim[p];
@@ -736,7 +751,10 @@ __global__ __launch_bounds__(kShadeBlock) void kScatter(WaveArgs A) {
         retirePaths(A.chunkLive, ending, ending && A.chunkLive ? A.P.chunk[p] : -1);
         uint32_t qs, unused;
         blockAppend2(&A.counts[1], alive, qs, nullptr, false, unused);
-        if (alive) A.qOut[qs] = p;
+        if (alive) {
+            A.qOut[qs] = p;
+            storeRay(A.rOut, qs, s.o, s.d);
+        }
     }
 }
 
@@ -951,7 +969,7 @@ struct gnd_hip_scene {
 };
 
 // Path state, hit and shadow queues and queue counters for `capacity` paths
-// in flight: 248 bytes per path slot in 61 SoA arrays, each 256-B aligned.
+// in flight: 272 bytes per path slot in 67 SoA arrays, each 256-B aligned.
 struct WaveBuffers {
     size_t capacity = 0;
     void* pathBuf = nullptr;
@@ -959,6 +977,7 @@ struct WaveBuffers {
     HitSoA Hq{};
     ShadowSoA Sh{};
     int32_t* q[2] = {nullptr, nullptr};
+    RayQ rq[2] = {};
     uint32_t* counts = nullptr;  // kNumCounts queue and ray-feed counters
 };
 
@@ -1091,7 +1110,7 @@ void ensureDev(T*& p, size_t& cap, size_t need) {
 
 size_t waveBufferBytes(size_t cap) {
     const size_t per = ((cap * 4 + 255) / 256) * 256;
-    return 60 * per + ((cap * 8 + 255) / 256) * 256;
+    return 66 * per + ((cap * 8 + 255) / 256) * 256;
 }
 
 void freeWaveBuffers(WaveBuffers& wb) {
@@ -1106,7 +1125,7 @@ void ensureWaveBuffers(WaveBuffers& wb, size_t cap) {
     if (wb.pathBuf) GND_CHECK(hipFree(wb.pathBuf));
     wb.pathBuf = nullptr;
     wb.capacity = 0;
-    // 60 four-byte fields and one eight-byte field per slot
+    // 66 four-byte fields and one eight-byte field per slot
     const size_t bytes = waveBufferBytes(cap);
     GND_CHECK(hipMalloc(&wb.pathBuf, bytes));
     char* b = static_cast<char*>(wb.pathBuf);
@@ -1118,7 +1137,7 @@ void ensureWaveBuffers(WaveBuffers& wb, size_t cap) {
     };
     PathSoA& P = wb.P;
     P.rng = (uint64_t*)take(8);
-    float** f[] = {&P.ox, &P.oy, &P.oz, &P.dx, &P.dy, &P.dz, &P.tpr, &P.tpg, &P.tpb, &P.Lr, &P.Lg, &P.Lb,
+    float** f[] = {&P.tpr, &P.tpg, &P.tpb, &P.Lr, &P.Lg, &P.Lb,
                    &P.pdfPrev, &P.lx, &P.ly, &P.lz, &P.weight, &P.pfx, &P.pfy, &P.evpx, &P.evpy, &P.evpz,
                    &P.evnx, &P.evny, &P.evnz, &P.evtx, &P.evty, &P.evtz, &P.evu, &P.evv};
     for (auto pp : f) *pp = (float*)take(4);
@@ -1142,6 +1161,8 @@ void ensureWaveBuffers(WaveBuffers& wb, size_t cap) {
     for (auto pp : g) *pp = (float*)take(4);
     wb.q[0] = (int32_t*)take(4);
     wb.q[1] = (int32_t*)take(4);
+    for (RayQ& r : wb.rq)
+        for (float** pp : {&r.ox, &r.oy, &r.oz, &r.dx, &r.dy, &r.dz}) *pp = (float*)take(4);
     if (off > bytes) throw std::runtime_error("wave buffer layout overflow");
     wb.capacity = cap;
 }
@@ -1804,6 +1825,8 @@ int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jo
             A.split = K.split ? 1 : 0;
             A.qIn = ctx->wb.q[0];
             A.qOut = ctx->wb.q[1];
+            A.rIn = ctx->wb.rq[0];
+            A.rOut = ctx->wb.rq[1];
             size_t tc = tl.mark();
             A.camQueueBase = 0;
             hipLaunchKernelGGL(kCamera, dim3((w.slots + 255) / 256), dim3(256), 0, ctx->stream, A);
@@ -1824,6 +1847,8 @@ int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jo
                     for (int d = 0; d < depth; ++d) {
                         A.qIn = ctx->wb.q[cur];
                         A.qOut = ctx->wb.q[1 - cur];
+                        A.rIn = ctx->wb.rq[cur];
+                        A.rOut = ctx->wb.rq[1 - cur];
                         launchBounce(ctx, A, K, live, nullptr);
                         cur = 1 - cur;
                     }
@@ -1842,6 +1867,8 @@ int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jo
             while (live > 0) {
                 A.qIn = ctx->wb.q[cur];
                 A.qOut = ctx->wb.q[1 - cur];
+                A.rIn = ctx->wb.rq[cur];
+                A.rOut = ctx->wb.rq[1 - cur];
                 launchBounce(ctx, A, K, live, tlp);
                 cur = 1 - cur;
                 live = readLive(ctx, ctx->wb.counts);
@@ -2180,6 +2207,8 @@ void serverStep(gnd_hip_server* s) {
         A.nSlots = n;
         A.qIn = s->wb.q[s->cur];
         A.qOut = s->wb.q[1 - s->cur];
+        A.rIn = s->wb.rq[s->cur];
+        A.rOut = s->wb.rq[1 - s->cur];
         A.camQueueBase = s->live;  // after the survivors of the last bounce
         hipLaunchKernelGGL(kCamera, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, A);
         GND_CHECK(hipGetLastError());
@@ -2190,6 +2219,8 @@ void serverStep(gnd_hip_server* s) {
     if (s->live > 0) {
         A.qIn = s->wb.q[s->cur];
         A.qOut = s->wb.q[1 - s->cur];
+        A.rIn = s->wb.rq[s->cur];
+        A.rOut = s->wb.rq[1 - s->cur];
         launchBounce(ctx, A, K, s->live, nullptr);
         s->cur = 1 - s->cur;
     }
