@@ -24,13 +24,21 @@ namespace gdev {
 
 constexpr int kStackCap = 128;
 
+using LdsPtr = __attribute__((address_space(3))) int*;
+
+template <int LDSD>
+constexpr int spillSize() { return kStackCap - LDSD > 0 ? kStackCap - LDSD : 1; }
+
+// Per-lane stack: `LDSD` entries in LDS (this lane's column, stride = block
+// size), deeper entries in a private array owned by the kernel.  Only
+// scalars and pointers live here, so the state stays in registers.
 template <int LDSD>
 struct TravStack {
-    int* lds;
+    LdsPtr lds;
+    int* spill;
     int stride;
     int sp;
     bool overflow;
-    int spill[kStackCap - LDSD > 0 ? kStackCap - LDSD : 1];
     GDEV_INLINE void push(int v) {
         if (sp < LDSD) lds[sp * stride] = v;
         else if (sp < kStackCap) spill[sp - LDSD] = v;
@@ -168,21 +176,33 @@ struct RangeQueue {
     int n;
     GDEV_INLINE void clear() { n = 0; }
     GDEV_INLINE bool empty() const { return n == 0; }
+    // add() and take() are written as per-field selects: stores chosen by
+    // `n`, or a run of plain field copies, are turned into an indexed array or
+    // a memmove by the optimiser, which then keeps the queue in scratch.
     GDEV_INLINE void add(uint32_t b, uint32_t c) {
         if (c == 0) return;
         const uint32_t e = b + c;
-        if (n == 0) { b0 = b; e0 = e; }
-        else if (n == 1) { b1 = b; e1 = e; }
-        else if (n == 2) { b2 = b; e2 = e; }
-        else { b3 = b; e3 = e; }
+        const bool s0 = n == 0, s1 = n == 1, s2 = n == 2, s3 = n >= 3;
+        b0 = s0 ? b : b0;
+        e0 = s0 ? e : e0;
+        b1 = s1 ? b : b1;
+        e1 = s1 ? e : e1;
+        b2 = s2 ? b : b2;
+        e2 = s2 ? e : e2;
+        b3 = s3 ? b : b3;
+        e3 = s3 ? e : e3;
         ++n;
     }
     GDEV_INLINE uint32_t take() {
-        const uint32_t v = b0++;
-        if (b0 == e0) {
-            b0 = b1; e0 = e1; b1 = b2; e1 = e2; b2 = b3; e2 = e3;
-            --n;
-        }
+        const uint32_t v = b0;
+        const bool next = v + 1u == e0;
+        b0 = next ? b1 : v + 1u;
+        e0 = next ? e1 : e0;
+        b1 = next ? b2 : b1;
+        e1 = next ? e2 : e1;
+        b2 = next ? b3 : b2;
+        e2 = next ? e3 : e2;
+        n -= next ? 1 : 0;
         return v;
     }
 };

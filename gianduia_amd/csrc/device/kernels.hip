@@ -168,9 +168,10 @@ GDEV_INLINE uint32_t xcdBlock() {
 }
 
 template <int LDSD>
-GDEV_INLINE TravStack<LDSD> makeStack(int* lds) {
+GDEV_INLINE TravStack<LDSD> makeStack(int* lds, int* spill) {
     TravStack<LDSD> st;
-    st.lds = lds + threadIdx.x;
+    st.lds = (LdsPtr)(lds + threadIdx.x);
+    st.spill = spill;
     st.stride = (int)blockDim.x;
     st.sp = 0;
     st.overflow = false;
@@ -216,7 +217,8 @@ GND_WAVE_KERNEL void kTraceBatch(DScene S, gnd_ray_soa R, int64_t n, gnd_hit_soa
     extern __shared__ int sStack[];
     const int64_t i = (int64_t)xcdBlock() * blockDim.x + threadIdx.x;
     uint32_t cnt[4] = {0, 0, 0, 0};
-    TravStack<LDSD> st = makeStack<LDSD>(sStack);
+    int spill[spillSize<LDSD>()];
+    TravStack<LDSD> st = makeStack<LDSD>(sStack, spill);
     if (i < n) {
         HitRec h;
         bool r = traceRay<ANY, COUNT, LDSD>(S, V3{R.ox[i], R.oy[i], R.oz[i]}, V3{R.dx[i], R.dy[i], R.dz[i]},
@@ -396,7 +398,8 @@ GND_TRACE_KERNEL void kTracePaths(WaveArgs A) {
     const uint64_t waveMask = __ballot(1);
     uint32_t cnt[4] = {0, 0, 0, 0};
     uint32_t maxRay = 0, bad = 0;
-    TravStack<LDSD> st = makeStack<LDSD>(sStack);
+    int spill[spillSize<LDSD>()];
+    TravStack<LDSD> st = makeStack<LDSD>(sStack, spill);
     RayFeed feed;
     feed.init(A.counts + kFeedTrace, n);
     int64_t cur = -1;  // queue index of this lane's ray
@@ -765,7 +768,8 @@ __global__ __launch_bounds__(256, GND_TRACE_WAVES + 1) void kShadowAny(WaveArgs 
     extern __shared__ int sStack[];
     const uint32_t n = A.counts[2];
     uint32_t cnt[4] = {0, 0, 0, 0};
-    TravStack<LDSD> st = makeStack<LDSD>(sStack);
+    int spill[spillSize<LDSD>()];
+    TravStack<LDSD> st = makeStack<LDSD>(sStack, spill);
     const DScene& S = A.S;
     const ShadowSoA& Sh = A.Sh;
     const uint64_t waveMask = __ballot(1);
@@ -802,7 +806,8 @@ GND_WAVE_KERNEL void kShadow(WaveArgs A) {
     extern __shared__ int sStack[];
     const uint32_t n = A.counts[2];
     uint32_t cnt[4] = {0, 0, 0, 0};
-    TravStack<LDSD> st = makeStack<LDSD>(sStack);
+    int spill[spillSize<LDSD>()];
+    TravStack<LDSD> st = makeStack<LDSD>(sStack, spill);
     const DScene& S = A.S;
     const ShadowSoA& Sh = A.Sh;
     const uint32_t stride = gridDim.x * blockDim.x;
