@@ -206,11 +206,16 @@ GDEV_INLINE void flushCounters(Counters* c, const uint32_t* cnt, bool overflow, 
 // VGPRs instead of spilling to scratch.
 #define GND_WAVE_KERNEL __global__ __launch_bounds__(256)
 // Traversal kernels are latency bound: their register budget is capped so
-// that GND_TRACE_WAVES waves fit on each SIMD.
+// that GND_TRACE_WAVES (closest hit) and GND_SHADOW_WAVES (any hit) waves fit
+// on each SIMD.  Measured on the 1080p scene: 4 and 4 (no spills); 5 waves
+// spill and lose more than the extra occupancy gains.
 #ifndef GND_TRACE_WAVES
 #define GND_TRACE_WAVES 4
 #endif
 #define GND_TRACE_KERNEL __global__ __launch_bounds__(256, GND_TRACE_WAVES)
+#ifndef GND_SHADOW_WAVES
+#define GND_SHADOW_WAVES 4
+#endif
 
 template <int LDSD, bool ANY, bool COUNT>
 GND_WAVE_KERNEL void kTraceBatch(DScene S, gnd_ray_soa R, int64_t n, gnd_hit_soa H, int32_t* occ, Counters* ctr) {
@@ -764,7 +769,7 @@ __global__ __launch_bounds__(kShadeBlock) void kScatter(WaveArgs A) {
 // Shadow rays when no surface is a null BSDF and no segment can cross a
 // medium: one any-hit test each, persistent with refill like kTracePaths.
 template <int LDSD, bool COUNT>
-__global__ __launch_bounds__(256, GND_TRACE_WAVES + 1) void kShadowAny(WaveArgs A) {
+__global__ __launch_bounds__(256, GND_SHADOW_WAVES) void kShadowAny(WaveArgs A) {
     extern __shared__ int sStack[];
     const uint32_t n = A.counts[2];
     uint32_t cnt[4] = {0, 0, 0, 0};
