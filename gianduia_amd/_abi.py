@@ -185,7 +185,22 @@ HOST_API = {
                                        C.POINTER(C.c_double)]),
     "gnd_host_camera_lookat": (C.c_int, [C.POINTER(gnd_camera_desc), f32p, f32p, f32p]),
     "gnd_host_render": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_uint64, f32p, C.POINTER(gnd_hip_stats)]),
+    "gnd_host_integrator_create": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    "gnd_host_integrator_from_scene": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
+    "gnd_host_integrator_set_callback": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "gnd_host_integrator_render": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "gnd_host_integrator_cancel": (C.c_int, [C.c_void_p]),
+    "gnd_host_integrator_film": (C.c_int, [C.c_void_p, f32p, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                           C.POINTER(C.c_int)]),
+    "gnd_host_integrator_save": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "gnd_host_integrator_free": (None, [C.c_void_p]),
+    "gnd_host_integrator_names": (C.c_int, [C.c_char_p, C.c_int]),
+    "gnd_host_sampler_floats": (C.c_int, [C.c_uint64, C.c_int, f32p]),
+    "gnd_host_image_save": (C.c_int, [C.c_char_p, f32p, C.c_int, C.c_int]),
 }
+
+# gnd_host_progress_fn
+PROGRESS_FN = C.CFUNCTYPE(None, C.c_int, f32p, C.c_int, C.c_int, C.c_void_p)
 
 _lib = None
 

@@ -342,6 +342,93 @@ class DeviceScene:
             self.handle = None
 
 
+class Integrator:
+    """Integrator plugin (gnd_host_integrator_*, include/gianduia/core/integrator.h:18-40):
+    render(scene) into the integrator's film in batches of passes, with a
+    progress callback per batch and cancel() from any thread."""
+
+    def __init__(self, dev: Device, name: str | None = None, scene: "HostScene | None" = None,
+                 max_depth: int = -1, passes_per_callback: int = 1):
+        self.dev = dev
+        h = C.c_void_p()
+        if name is None:
+            if scene is None:
+                raise ValueError("an integrator name or a scene is needed")
+            rc = _lib().gnd_host_integrator_from_scene(dev.ctx, scene.handle, passes_per_callback, C.byref(h))
+        else:
+            rc = _lib().gnd_host_integrator_create(dev.ctx, name.encode(), max_depth, passes_per_callback, C.byref(h))
+        if rc != 0:
+            raise GndError(rc, host_error())
+        self.handle = h
+        self._cb = None
+        self.callback_error = None
+
+    def set_callback(self, fn) -> None:
+        """fn(sample_index, rgb) with rgb an (H, W, 3) float32 copy of the resolved film."""
+        if fn is None:
+            self._cb = None
+            _lib().gnd_host_integrator_set_callback(self.handle, None, None)
+            return
+
+        def tramp(idx, rgb, w, h, _user):
+            try:
+                img = np.ctypeslib.as_array(rgb, shape=(h * w * 3,)).reshape(h, w, 3).copy()
+                fn(idx, img)
+            except Exception as e:  # cannot unwind through C
+                self.callback_error = e
+
+        self._cb = A.PROGRESS_FN(tramp)
+        _lib().gnd_host_integrator_set_callback(self.handle, C.cast(self._cb, C.c_void_p), None)
+
+    def render(self, scene: "HostScene") -> None:
+        rc = _lib().gnd_host_integrator_render(self.handle, scene.handle)
+        if rc != 0:
+            raise GndError(rc, host_error())
+        if self.callback_error is not None:
+            raise self.callback_error
+
+    def cancel(self) -> None:
+        _lib().gnd_host_integrator_cancel(self.handle)
+
+    def film(self):
+        """(resolved (H, W, 3) float32 image, passes rendered)."""
+        w, h, n = C.c_int(), C.c_int(), C.c_int()
+        _lib().gnd_host_integrator_film(self.handle, None, C.byref(w), C.byref(h), C.byref(n))
+        out = np.zeros((h.value, w.value, 3), np.float32)
+        _lib().gnd_host_integrator_film(self.handle, out.ctypes.data_as(A.f32p), None, None, None)
+        return out, n.value
+
+    def save(self, path: str) -> None:
+        rc = _lib().gnd_host_integrator_save(self.handle, path.encode())
+        if rc != 0:
+            raise GndError(rc, host_error())
+
+    def close(self) -> None:
+        if self.handle:
+            _lib().gnd_host_integrator_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def integrator_names() -> list[str]:
+    n = _lib().gnd_host_integrator_names(None, 0)
+    buf = C.create_string_buffer(n + 1)
+    _lib().gnd_host_integrator_names(buf, n + 1)
+    return buf.value.decode().split(",") if n else []
+
+
+def sampler_floats(sample_index: int, n: int) -> np.ndarray:
+    """IndependentSampler: the first n floats of a sample index."""
+    out = np.zeros(n, np.float32)
+    _lib().gnd_host_sampler_floats(sample_index, n, out.ctypes.data_as(A.f32p))
+    return out
+
+
 def camera_lookat(cam: A.gnd_camera_desc, origin, target, up=(0.0, 1.0, 0.0)) -> A.gnd_camera_desc:
     """A copy of `cam` posed with the reference's lookAt (transform.h:141-144)."""
     c = A.gnd_camera_desc()

@@ -34,7 +34,7 @@ DEVICE = [f"--offload-arch={ARCH}", "-fno-gpu-flush-denormals-to-zero",
           "-Wno-unused-command-line-argument"]
 
 HOST_SOURCES = ["host/hmath.cpp", "host/bvh_build.cpp", "host/scene_host.cpp", "host/generators.cpp",
-                "host/xml_loader.cpp", "host/host_capi.cpp"]
+                "host/xml_loader.cpp", "host/integrator.cpp", "host/host_capi.cpp"]
 DEVICE_SOURCES = ["device/kernels.hip"]
 
 

@@ -1,5 +1,6 @@
 // host_capi.cpp — extern "C" surface of the host scene shell (include/gnd_host.h).
 #include <cmath>
+#include <cstring>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -7,6 +8,7 @@
 #include <vector>
 
 #include "../../../include/gnd_host.h"
+#include "integrator.h"
 #include "scene_host.h"
 
 struct gnd_host_scene {
@@ -16,8 +18,25 @@ struct gnd_host_scene {
     std::mutex mu;
 };
 
+struct gnd_host_integrator {
+    std::unique_ptr<gnd::Integrator> impl;
+    gnd_hip_ctx* ctx = nullptr;
+};
+
 namespace {
 thread_local std::string g_hostErr;
+
+// The scene's device copy on `ctx` (built and uploaded on first use).
+gnd_hip_scene* deviceScene(gnd_hip_ctx* ctx, gnd_host_scene* s) {
+    std::lock_guard<std::mutex> g(s->mu);
+    for (auto& u : s->uploads)
+        if (u.first == ctx) return u.second;
+    const gnd_hip_scene_arrays& arr = s->scene->build(0, nullptr);
+    gnd_hip_scene* dev = nullptr;
+    if (gnd_hip_scene_upload(ctx, &arr, &dev) != GND_OK) throw std::runtime_error(gnd_hip_last_error(ctx));
+    s->uploads.push_back({ctx, dev});
+    return dev;
+}
 
 template <class F>
 int hostGuard(F&& f) {
@@ -97,29 +116,15 @@ int gnd_host_render(gnd_hip_ctx* ctx, gnd_host_scene* s, int spp, uint64_t seed_
                     gnd_hip_stats* stats) {
     if (!ctx || !s || !rgb) return GND_E_INVALID;
     gnd_hip_scene* dev = nullptr;
-    {
-        std::lock_guard<std::mutex> g(s->mu);
-        for (auto& u : s->uploads)
-            if (u.first == ctx) dev = u.second;
-        if (!dev) {
-            const gnd_hip_scene_arrays* arr = nullptr;
-            int rc = hostGuard([&] {
-                arr = &s->scene->build(0, nullptr);
-                return GND_OK;
-            });
-            if (rc != GND_OK) return rc;
-            rc = gnd_hip_scene_upload(ctx, arr, &dev);
-            if (rc != GND_OK) {
-                g_hostErr = gnd_hip_last_error(ctx);
-                return rc;
-            }
-            s->uploads.push_back({ctx, dev});
-        }
-    }
+    int rc = hostGuard([&] {
+        dev = deviceScene(ctx, s);
+        return GND_OK;
+    });
+    if (rc != GND_OK) return rc;
     const gnd_scene_desc& d = s->scene->desc();
     const size_t np = (size_t)d.camera.width * d.camera.height;
     void* film = nullptr;
-    int rc = gnd_hip_malloc(ctx, np * 4 * sizeof(float), &film);
+    rc = gnd_hip_malloc(ctx, np * 4 * sizeof(float), &film);
     if (rc != GND_OK) return rc;
     gnd_hip_job job{};
     job.camera = d.camera;
@@ -147,6 +152,125 @@ int gnd_host_render(gnd_hip_ctx* ctx, gnd_host_scene* s, int spp, uint64_t seed_
             rgb[3 * i + 0] = rgb[3 * i + 1] = rgb[3 * i + 2] = 0.0f;
         }
     }
+    return GND_OK;
+}
+
+// ---- Integrator plugin interface (include/gianduia/core/integrator.h:18-40)
+
+int gnd_host_integrator_create(gnd_hip_ctx* ctx, const char* name, int max_depth, int passes_per_callback,
+                               gnd_host_integrator** out) {
+    if (!ctx || !name || !out) return GND_E_INVALID;
+    return hostGuard([&] {
+        gnd::IntegratorParams p;
+        p.ctx = ctx;
+        if (max_depth >= 0) p.maxDepth = max_depth;
+        p.passesPerCallback = passes_per_callback > 0 ? passes_per_callback : 1;
+        auto* h = new gnd_host_integrator();
+        h->impl = gnd::IntegratorFactory::create(name, p);
+        h->ctx = ctx;
+        *out = h;
+        return GND_OK;
+    });
+}
+
+int gnd_host_integrator_from_scene(gnd_hip_ctx* ctx, gnd_host_scene* s, int passes_per_callback,
+                                   gnd_host_integrator** out) {
+    if (!ctx || !s || !out) return GND_E_INVALID;
+    const gnd_scene_desc& d = s->scene->desc();
+    return gnd_host_integrator_create(ctx, d.integrator == GND_INTEGRATOR_VOLPATH ? "volpath" : "path", d.max_depth,
+                                      passes_per_callback, out);
+}
+
+int gnd_host_integrator_set_callback(gnd_host_integrator* it, gnd_host_progress_fn fn, void* user) {
+    if (!it) return GND_E_INVALID;
+    if (!fn) {
+        it->impl->setCallback(nullptr);
+        return GND_OK;
+    }
+    it->impl->setCallback([fn, user](int sampleIndex, const gnd::Film& film) {
+        fn(sampleIndex, film.rgb().data(), film.width(), film.height(), user);
+    });
+    return GND_OK;
+}
+
+int gnd_host_integrator_render(gnd_host_integrator* it, gnd_host_scene* s) {
+    if (!it || !s) return GND_E_INVALID;
+    return hostGuard([&] {
+        gnd::RenderScene rs;
+        rs.device = deviceScene(it->ctx, s);
+        rs.desc = &s->scene->desc();
+        it->impl->render(rs);
+        return GND_OK;
+    });
+}
+
+int gnd_host_integrator_cancel(gnd_host_integrator* it) {
+    if (!it) return GND_E_INVALID;
+    it->impl->cancel();
+    return GND_OK;
+}
+
+int gnd_host_integrator_film(gnd_host_integrator* it, float* rgb, int* width, int* height, int* passes) {
+    if (!it) return GND_E_INVALID;
+    const gnd::Film& f = it->impl->film();
+    if (width) *width = f.width();
+    if (height) *height = f.height();
+    if (passes) *passes = it->impl->passesDone();
+    if (rgb) std::memcpy(rgb, f.rgb().data(), f.rgb().size() * sizeof(float));
+    return GND_OK;
+}
+
+int gnd_host_integrator_save(gnd_host_integrator* it, const char* path) {
+    if (!it || !path) return GND_E_INVALID;
+    return hostGuard([&] {
+        const std::string p(path);
+        const auto dot = p.find_last_of('.');
+        const std::string ext = dot == std::string::npos ? "" : p.substr(dot + 1);
+        if (ext == "pfm") it->impl->film().savePFM(p);
+        else if (ext == "png") it->impl->film().savePNG(p);
+        else throw std::runtime_error("unsupported image format: " + p);
+        return GND_OK;
+    });
+}
+
+void gnd_host_integrator_free(gnd_host_integrator* it) { delete it; }
+
+int gnd_host_integrator_names(char* buf, int len) {
+    return hostGuard([&] {
+        std::string all;
+        for (const std::string& n : gnd::IntegratorFactory::names()) all += (all.empty() ? "" : ",") + n;
+        if (buf && len > 0) {
+            std::strncpy(buf, all.c_str(), (size_t)len - 1);
+            buf[len - 1] = '\0';
+        }
+        return (int)all.size();
+    });
+}
+
+int gnd_host_image_save(const char* path, const float* rgb, int width, int height) {
+    if (!path || !rgb || width <= 0 || height <= 0) return GND_E_INVALID;
+    return hostGuard([&] {
+        gnd::Film f(width, height);
+        std::vector<float> rgbw((size_t)width * height * 4);
+        for (size_t i = 0; i < (size_t)width * height; ++i) {
+            for (int c = 0; c < 3; ++c) rgbw[4 * i + c] = rgb[3 * i + c];
+            rgbw[4 * i + 3] = 1.0f;
+        }
+        f.accumulate(rgbw.data());
+        f.resolve();
+        const std::string p(path);
+        if (p.size() > 4 && p.compare(p.size() - 4, 4, ".pfm") == 0) f.savePFM(p);
+        else if (p.size() > 4 && p.compare(p.size() - 4, 4, ".png") == 0) f.savePNG(p);
+        else throw std::runtime_error("unsupported image format: " + p);
+        return GND_OK;
+    });
+}
+
+int gnd_host_sampler_floats(uint64_t sample_index, int n, float* out) {
+    if (n < 0 || (n > 0 && !out)) return GND_E_INVALID;
+    gnd::IndependentSampler smp(1);
+    smp.seed(sample_index);
+    for (int i = 0; i < n; ++i) out[i] = smp.next1D();
     return GND_OK;
 }
 

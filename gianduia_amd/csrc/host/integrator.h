@@ -1,0 +1,144 @@
+// integrator.h — the reference's Integrator / Sampler / Film plugin
+// interfaces on the host, with the GPU integrators behind them.
+//
+//   Integrator      include/gianduia/core/integrator.h:18-40 (render, cancel,
+//                   preprocess, setCallback with (sampleIndex, film))
+//   Sampler         include/gianduia/core/sampler.h:8-27 (clone, seed,
+//                   next1D, next2D, getSampleCount)
+//   IndependentSampler  src/samplers/independent.cpp:7-47
+//   Film            include/gianduia/core/film.h:26-69 (accumulate, resolve)
+//   registration    include/gianduia/core/factory.h:26-36 (by XML name;
+//                   a duplicate name overrides with a warning)
+//
+// The GPU integrators render whole passes on the device through the C ABI
+// (gnd_hip_render) and accumulate them into a host Film between callbacks.
+#pragma once
+
+#include <atomic>
+#include <cstdint>
+#include <functional>
+#include <map>
+#include <memory>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../../include/gnd_hip.h"
+
+namespace gnd {
+
+class Sampler {
+public:
+    virtual ~Sampler() = default;
+    virtual std::unique_ptr<Sampler> clone() const = 0;
+    virtual void seed(uint64_t sampleIndex) = 0;
+    virtual float next1D() = 0;
+    virtual std::pair<float, float> next2D() = 0;
+    virtual size_t getSampleCount() const = 0;
+};
+
+// PCG32 seeded with splitMix64(index) and stream 1; floats are u32 * 2^-32
+// (so 1.0f is possible, pcg32.h:29-31).  The device sampler draws the same
+// sequence per (pixel, pass).
+class IndependentSampler : public Sampler {
+public:
+    explicit IndependentSampler(size_t spp) : spp_(spp) {}
+    std::unique_ptr<Sampler> clone() const override;
+    void seed(uint64_t sampleIndex) override;
+    float next1D() override;
+    std::pair<float, float> next2D() override;
+    size_t getSampleCount() const override { return spp_; }
+
+private:
+    uint32_t nextU32();
+    size_t spp_;
+    uint64_t state_ = 0, inc_ = 3;
+};
+
+// Host film: weighted RGB sums and weights per pixel (what the device
+// gathers per pass), resolved to RGB = sum / weight.
+class Film {
+public:
+    Film() = default;
+    Film(int w, int h) { reset(w, h); }
+    void reset(int w, int h);
+    void clear();
+    // adds device-format accumulators (w*h*4 floats: r, g, b, weight)
+    void accumulate(const float* rgbw);
+    void resolve();
+    int width() const { return w_; }
+    int height() const { return h_; }
+    const std::vector<float>& rgb() const { return rgb_; }  // resolved, w*h*3
+    void savePFM(const std::string& path) const;
+    void savePNG(const std::string& path) const;  // 8-bit sRGB, values clamped to [0, 1]
+
+private:
+    int w_ = 0, h_ = 0;
+    std::vector<double> acc_;  // r, g, b, weight per pixel
+    std::vector<float> rgb_;
+};
+
+// What an integrator renders: the scene description and its device copy.
+struct RenderScene {
+    const gnd_scene_desc* desc = nullptr;
+    gnd_hip_scene* device = nullptr;
+};
+
+class Integrator {
+public:
+    using RenderCallback = std::function<void(int sampleIndex, const Film& film)>;
+    virtual ~Integrator() = default;
+    virtual void render(const RenderScene& scene) = 0;
+    virtual void cancel() { stop_ = true; }
+    virtual void preprocess(const RenderScene&) {}
+    void setCallback(RenderCallback cb) { callback_ = std::move(cb); }
+    Film& film() { return film_; }
+    int passesDone() const { return passes_; }
+
+protected:
+    void notifyUpdate(int sampleIndex) {
+        if (callback_) callback_(sampleIndex, film_);
+    }
+    std::atomic<bool> stop_{false};
+    RenderCallback callback_;
+    Film film_;
+    int passes_ = 0;
+};
+
+// path / volpath on the GPU: passes are rendered in batches of
+// `passesPerCallback` (one gnd_hip_render each, seed offset = first pass *
+// W*H so pass s draws the reference's stream pixelIdx + s*W*H), accumulated
+// into the film, resolved, and reported to the callback.  cancel() stops at
+// the next batch and also interrupts the batch in flight.
+class HipIntegrator : public Integrator {
+public:
+    HipIntegrator(gnd_hip_ctx* ctx, int integrator, int maxDepth, int passesPerCallback)
+        : ctx_(ctx), integrator_(integrator), maxDepth_(maxDepth), batch_(passesPerCallback) {}
+    void render(const RenderScene& scene) override;
+    void cancel() override;
+
+private:
+    gnd_hip_ctx* ctx_;
+    int integrator_, maxDepth_, batch_;
+};
+
+struct IntegratorParams {
+    gnd_hip_ctx* ctx = nullptr;
+    int maxDepth = 1000;  // path.cpp:9, volPath.cpp:12
+    int passesPerCallback = 1;
+};
+
+class IntegratorFactory {
+public:
+    using Creator = std::function<std::unique_ptr<Integrator>(const IntegratorParams&)>;
+    static void registerClass(const std::string& name, Creator c);
+    static std::unique_ptr<Integrator> create(const std::string& name, const IntegratorParams& p);
+    static std::vector<std::string> names();
+};
+
+// Registers "path", "volpath", "path_hip" and "volpath_hip" (idempotent;
+// called by the factory before its first use, so static-library linking
+// cannot drop the registrations).
+void registerBuiltinIntegrators();
+
+}  // namespace gnd

@@ -52,6 +52,42 @@ int gnd_host_camera_lookat(gnd_camera_desc* cam, const float* origin, const floa
 int gnd_host_render(gnd_hip_ctx* ctx, gnd_host_scene* s, int spp, uint64_t seed_offset, float* rgb,
                     gnd_hip_stats* stats);
 
+/* ---- Integrator / Sampler plugin interface ----------------------------------
+ * Integrator (include/gianduia/core/integrator.h:18-40): render(scene),
+ * cancel(), setCallback(sampleIndex, film).  Integrators are created by their
+ * XML name through the factory (include/gianduia/core/factory.h:26-36):
+ * "path" and "volpath" (the GPU integrators under the reference's names),
+ * "path_hip", "volpath_hip".  render() renders the scene's camera at its
+ * sampler's spp into the integrator's film, in batches of passes_per_callback
+ * passes; the callback receives the index of the last pass done and the
+ * resolved film (width*height*3 floats, Film::resolve).  cancel() stops at the
+ * next batch (from any thread, also from inside the callback). */
+typedef struct gnd_host_integrator gnd_host_integrator;
+typedef void (*gnd_host_progress_fn)(int sample_index, const float* rgb, int width, int height, void* user);
+
+/* max_depth < 0 keeps the integrator default (1000, path.cpp:9). */
+int gnd_host_integrator_create(gnd_hip_ctx* ctx, const char* name, int max_depth, int passes_per_callback,
+                               gnd_host_integrator** out);
+/* The scene's own <integrator type=... maxDepth=...>. */
+int gnd_host_integrator_from_scene(gnd_hip_ctx* ctx, gnd_host_scene* s, int passes_per_callback,
+                                   gnd_host_integrator** out);
+int gnd_host_integrator_set_callback(gnd_host_integrator* it, gnd_host_progress_fn fn, void* user);
+int gnd_host_integrator_render(gnd_host_integrator* it, gnd_host_scene* s);
+int gnd_host_integrator_cancel(gnd_host_integrator* it);
+/* Resolved film (rgb may be NULL to query the size) and the passes rendered. */
+int gnd_host_integrator_film(gnd_host_integrator* it, float* rgb, int* width, int* height, int* passes);
+/* Film::savePNG / an HDR float image (integrator.h:152-153): ".png" (8-bit
+ * sRGB) or ".pfm" (32-bit float RGB). */
+int gnd_host_integrator_save(gnd_host_integrator* it, const char* path);
+void gnd_host_integrator_free(gnd_host_integrator* it);
+/* Registered integrator names, comma separated; returns the full length. */
+int gnd_host_integrator_names(char* buf, int len);
+/* Writes a width*height*3 float RGB image as ".png" or ".pfm" (the film writers). */
+int gnd_host_image_save(const char* path, const float* rgb, int width, int height);
+/* IndependentSampler (src/samplers/independent.cpp:7-47): the first n floats
+ * of sample index `sample_index`. */
+int gnd_host_sampler_floats(uint64_t sample_index, int n, float* out);
+
 #ifdef __cplusplus
 }
 #endif

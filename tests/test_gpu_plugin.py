@@ -1,0 +1,59 @@
+"""The Integrator plugin on the GPU (gnd_host_integrator_*): render(scene)
+equals gnd_hip_render of the same job, the callback fires once per batch of
+passes with the last pass index (integrator.h:147), cancel() from the
+callback stops at the next batch, the scene's own integrator is used, and the
+film is saved."""
+import numpy as np
+import pytest
+
+from gianduia_amd import DeviceScene, GndError, HostScene, Integrator, resolve
+
+pytestmark = pytest.mark.gpu
+
+
+def test_integrator_equals_render(device):
+    hs = HostScene.generate("cbox", "width=64;height=48;spp=4")
+    it = Integrator(device, "path", max_depth=8, passes_per_callback=4)
+    it.render(hs)
+    img, passes = it.film()
+    assert passes == 4
+    ds = DeviceScene(device, hs)
+    films, _ = device.render(ds, [hs.default_job(spp=4)])
+    assert np.array_equal(img, resolve(films[0]))
+
+
+def test_callbacks_per_batch_and_cancel(device):
+    hs = HostScene.generate("cbox", "width=32;height=32;spp=5")
+    it = Integrator(device, "path_hip", max_depth=8, passes_per_callback=2)
+    seen = []
+    it.set_callback(lambda idx, rgb: seen.append((idx, rgb.shape, float(rgb.mean()))))
+    it.render(hs)
+    assert [s[0] for s in seen] == [1, 3, 4]
+    assert all(s[1] == (32, 32, 3) for s in seen)
+    assert it.film()[1] == 5
+    # cancel from inside the callback: the render stops after the first batch
+    it2 = Integrator(device, "path", max_depth=8, passes_per_callback=2)
+    it2.set_callback(lambda idx, rgb: it2.cancel())
+    it2.render(hs)
+    assert it2.film()[1] == 2
+
+
+def test_scene_integrator_and_save(device, tmp_path):
+    hs = HostScene.generate("volumetric", "grid=16;width=40;height=32;spp=2;depth=12")
+    it = Integrator(device, scene=hs)          # <integrator type="volpath">
+    it.render(hs)
+    img, passes = it.film()
+    ds = DeviceScene(device, hs)
+    films, _ = device.render(ds, [hs.default_job()])
+    assert passes == 2 and np.array_equal(img, resolve(films[0]))
+    it.save(str(tmp_path / "out.png"))
+    it.save(str(tmp_path / "out.pfm"))
+    assert (tmp_path / "out.png").read_bytes()[:4] == b"\x89PNG"
+    assert (tmp_path / "out.pfm").read_bytes()[:2] == b"PF"
+    with pytest.raises(GndError):
+        it.save(str(tmp_path / "out.bmp"))
+
+
+def test_unknown_integrator(device):
+    with pytest.raises(GndError):
+        Integrator(device, "direct")
