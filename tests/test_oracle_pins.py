@@ -351,3 +351,52 @@ def test_grid_trilinear_sample():
     assert abs(g(s.h, 1, 2.5, 3, 4) - 0.5 * (dens[4, 3, 2] + dens[4, 3, 3])) < 1e-6
     # outside the grid: background 0 blends in
     assert abs(g(s.h, 1, 7.5, 3, 4) - 0.5 * dens[4, 3, 7]) < 1e-6
+
+
+# ---------------------------------------------------------------- integrator: furnace
+def _inward_box():
+    """The 12 triangles of [-1,1]^3 with inward geometric and shading normals."""
+    tris, nrm = [], []
+    for axis in range(3):
+        for sgn in (-1.0, 1.0):
+            u, v = [a for a in range(3) if a != axis]
+            corners = []
+            for a, b in ((-1, -1), (1, -1), (1, 1), (-1, 1)):
+                p = [0.0, 0.0, 0.0]
+                p[axis], p[u], p[v] = sgn, a, b
+                corners.append(p)
+            c = np.array(corners, np.float32)
+            inward = np.zeros(3, np.float32)
+            inward[axis] = -sgn
+            for t in ((0, 1, 2), (0, 2, 3)):
+                tri = c[list(t)]
+                if np.dot(np.cross(tri[1] - tri[0], tri[2] - tri[0]), inward) < 0:
+                    tri = tri[[0, 2, 1]]
+                tris.append(tri.reshape(9))
+                nrm.append(np.tile(inward, 3))
+    return np.array(tris, np.float32), np.array(nrm, np.float32)
+
+
+@pytest.mark.parametrize("albedo,filt", [(0.5, A.GND_FILTER_BOX), (0.8, A.GND_FILTER_BOX),
+                                         (0.5, A.GND_FILTER_GAUSSIAN)])
+def test_path_integrator_furnace(albedo, filt):
+    """Camera inside a closed box whose diffuse walls (albedo a) all emit Le:
+    every pixel converges to Le / (1 - a).  Emission, NEE with the area-light
+    pdf, MIS weights, Russian roulette and the film filter all enter; a dropped
+    MIS term, a wrong pdf conversion or a biased roulette moves the mean."""
+    tris, nrm = _inward_box()
+    b = SceneBuilder()
+    mt = b.matte((albedo, albedo, albedo))
+    b.prim(b.mesh(tris, nrm), material=mt, radiance=(1.0, 1.0, 1.0))
+    d = b.desc(width=8, height=8)
+    d.max_depth = 1000                       # path.cpp:9 default
+    d.camera.filter = filt
+    d.camera.filter_radius[:] = (2.0, 2.0)
+    d.camera.filter_alpha = 2.0
+    ref = RefScene(d)
+    spp = 256
+    rgbw, _ = ref.render(threads=8, spp=spp, width=8, height=8)
+    img = rgbw[..., :3] / rgbw[..., 3:4]
+    expect = 1.0 / (1.0 - albedo)
+    assert abs(img.mean() - expect) < 0.02 * expect, img.mean()
+    assert np.all(np.abs(img - expect) < 0.25 * expect)
