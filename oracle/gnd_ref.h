@@ -21,6 +21,17 @@
  * lane semantics are the reference's own (x86-64 Highway target is SSE2+).
  *
  * Build flags: -O3 -ffp-contract=off -fno-fast-math (no FMA contraction).
+ *
+ * How the restatement is pinned (tests/test_oracle_known_answers.py,
+ * tests/test_oracle_pins.py): the reference's own test values (Fresnel, GGX,
+ * BSDF lobes, camera, bounds, frame, colour, sphere), published PCG32 and
+ * SplitMix64 outputs, brute-force traversal, warp chi^2 / KS tests, BSDF
+ * energy and pdf integrals, HG normalisation, media closed forms and the
+ * integrator furnace.  Parity unpinned (no reference test or closed form
+ * fixes them): the glm scalar evaluation orders above, the Highway lane
+ * semantics, OpenVDB BoxSampler / worldToIndex on the dense grid, thin-lens
+ * distortion, chromatic aberration, vignetting and polygonal aperture, the
+ * Gaussian filter's shape, and the exact area-light sample positions.
  */
 #pragma once
 
