@@ -117,6 +117,16 @@ GDEV_INLINE void retirePaths(uint32_t* chunkLive, bool ending, int chunk) {
 // faster).  Every thread of the block calls this; it returns this lane's
 // slots in the two queues.  c1 may be null.
 constexpr int kShadeBlock = 512;
+// waves per SIMD the shading kernel and the transmittance shadow kernel are
+// compiled for (register budget; on the volumetric scene 4 waves take the
+// transmittance shadows from 53 to 48 ms per frame, the shading kernel gains
+// nothing from 3)
+#ifndef GND_SHADE_WAVES
+#define GND_SHADE_WAVES 2
+#endif
+#ifndef GND_SHADOWTR_WAVES
+#define GND_SHADOWTR_WAVES 4
+#endif
 GDEV_INLINE void blockAppend2(uint32_t* c0, bool p0, uint32_t& pos0, uint32_t* c1, bool p1, uint32_t& pos1) {
     __shared__ uint32_t sN[2][kShadeBlock / 64];
     __shared__ uint32_t sBase[2];
@@ -651,7 +661,7 @@ GDEV_INLINE int neeStep(const WaveArgs& A, const DJob& J, PathLocal& s, const Hi
 // afterwards (valid when shadow rays consume no random numbers).
 // SPLIT = true: NEE only; the event is stored for kScatter.
 template <bool SPLIT>
-__global__ __launch_bounds__(kShadeBlock) void kShade(WaveArgs A) {
+__global__ __launch_bounds__(kShadeBlock, GND_SHADE_WAVES) void kShade(WaveArgs A) {
     const uint32_t n = A.counts[0];
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t start = blockIdx.x * blockDim.x;
@@ -807,7 +817,7 @@ __global__ __launch_bounds__(256, GND_SHADOW_WAVES) void kShadowAny(WaveArgs A) 
 // General shadow rays: closest-hit loop through null-BSDF surfaces with
 // per-segment medium transmittance (path.cpp:122-140 / volPath.cpp:261-291).
 template <int LDSD, bool COUNT>
-GND_WAVE_KERNEL void kShadow(WaveArgs A) {
+__global__ __launch_bounds__(256, GND_SHADOWTR_WAVES) void kShadow(WaveArgs A) {
     extern __shared__ int sStack[];
     const uint32_t n = A.counts[2];
     uint32_t cnt[4] = {0, 0, 0, 0};
