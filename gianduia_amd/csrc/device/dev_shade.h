@@ -684,10 +684,12 @@ GDEV_INLINE float gridSample(const DMedium& m, double x, double y, double z) {
     return r1 + (float)(ux * (r2 - r1));
 }
 
+// Macro-cell DDA state; the step along an axis is the sign of the index-space
+// direction there (+1 if positive, else -1).
 struct GridWalk {
     V3 io, id;
     float t, tMax;
-    int cx, cy, cz, sx, sy, sz;
+    int cx, cy, cz;
     float tdx, tdy, tdz, tmx, tmy, tmz;
 };
 
@@ -728,15 +730,12 @@ GDEV_INLINE bool gridSetup(const DMedium& m, V3 o, V3 d, float tmin, float tmax,
     g.cx = clampi((int)floorf((pe.x - 0.0f) / m.cellX), 0, m.resX - 1);
     g.cy = clampi((int)floorf((pe.y - 0.0f) / m.cellY), 0, m.resY - 1);
     g.cz = clampi((int)floorf((pe.z - 0.0f) / m.cellZ), 0, m.resZ - 1);
-    g.sx = g.id.x > 0.0f ? 1 : -1;
-    g.sy = g.id.y > 0.0f ? 1 : -1;
-    g.sz = g.id.z > 0.0f ? 1 : -1;
     g.tdx = g.id.x != 0.0f ? __builtin_fabsf(m.cellX / g.id.x) : kFltMax;
     g.tdy = g.id.y != 0.0f ? __builtin_fabsf(m.cellY / g.id.y) : kFltMax;
     g.tdz = g.id.z != 0.0f ? __builtin_fabsf(m.cellZ / g.id.z) : kFltMax;
-    float nbx = 0.0f + (g.cx + (g.sx > 0 ? 1 : 0)) * m.cellX;
-    float nby = 0.0f + (g.cy + (g.sy > 0 ? 1 : 0)) * m.cellY;
-    float nbz = 0.0f + (g.cz + (g.sz > 0 ? 1 : 0)) * m.cellZ;
+    float nbx = 0.0f + (g.cx + (g.id.x > 0.0f ? 1 : 0)) * m.cellX;
+    float nby = 0.0f + (g.cy + (g.id.y > 0.0f ? 1 : 0)) * m.cellY;
+    float nbz = 0.0f + (g.cz + (g.id.z > 0.0f ? 1 : 0)) * m.cellZ;
     g.tmx = g.id.x != 0.0f ? (nbx - g.io.x) / g.id.x : kFltMax;
     g.tmy = g.id.y != 0.0f ? (nby - g.io.y) / g.id.y : kFltMax;
     g.tmz = g.id.z != 0.0f ? (nbz - g.io.z) / g.id.z : kFltMax;
@@ -747,13 +746,99 @@ GDEV_INLINE bool gridSetup(const DMedium& m, V3 o, V3 d, float tmin, float tmax,
 }
 GDEV_INLINE void gridAdvance(GridWalk& g) {
     if (g.tmx < g.tmy) {
-        if (g.tmx < g.tmz) { g.tmx += g.tdx; g.cx += g.sx; }
-        else { g.tmz += g.tdz; g.cz += g.sz; }
+        if (g.tmx < g.tmz) { g.tmx += g.tdx; g.cx += g.id.x > 0.0f ? 1 : -1; }
+        else { g.tmz += g.tdz; g.cz += g.id.z > 0.0f ? 1 : -1; }
     } else {
-        if (g.tmy < g.tmz) { g.tmy += g.tdy; g.cy += g.sy; }
-        else { g.tmz += g.tdz; g.cz += g.sz; }
+        if (g.tmy < g.tmz) { g.tmy += g.tdy; g.cy += g.id.y > 0.0f ? 1 : -1; }
+        else { g.tmz += g.tdz; g.cz += g.id.z > 0.0f ? 1 : -1; }
     }
 }
+
+// The macro-cell walk of ratio tracking (Medium::Tr, heterogeneous.cpp:106-229,
+// SAMPLE = false) and delta tracking (Medium::sample, heterogeneous.cpp:251-376,
+// SAMPLE = true) in resumable form: step() enters one cell or makes one
+// tentative collision and returns true once the walk has ended.  The scalar
+// functions below loop on it; the wavefront kernels interleave the steps of
+// many walks so that a lane whose walk ends takes new work.
+//   ratio tracking: `tr` is the transmittance, or zero if `hit` (the walk
+//                   ended below the throughput floor or by roulette)
+//   delta tracking: `hit` = a real collision at `t`, density `dens`, channel `ch`
+template <bool SAMPLE>
+struct GridTrack {
+    GridWalk g;
+    float t, tExit, invMaj;
+    V3 tr;
+    float dens;
+    int ch;
+    bool inCell, hit;
+    GDEV_INLINE bool start(const DMedium& m, V3 o, V3 d, float tmin, float tmax) {
+        tr = sameV(1.0f);
+        inCell = false;
+        hit = false;
+        if (!gridSetup(m, o, d, tmin, tmax, g)) return false;
+        t = g.t;
+        return true;
+    }
+    // t = the cell's exit; the walk ends there or moves to the next cell
+    GDEV_INLINE bool leaveCell() {
+        inCell = false;
+        t = tExit;
+        if (t >= g.tMax) return true;
+        gridAdvance(g);
+        return false;
+    }
+    GDEV_INLINE bool step(const DMedium& m, Rng& rng) {
+        if (!inCell) {
+            if (!(t < g.tMax)) return true;
+            if (g.cx < 0 || g.cx >= m.resX || g.cy < 0 || g.cy >= m.resY || g.cz < 0 || g.cz >= m.resZ) return true;
+            const float maj = m.majorants[(size_t)g.cz * (m.resX * m.resY) + g.cy * m.resX + g.cx];
+            tExit = stdMin4(g.tmx, g.tmy, g.tmz, g.tMax);
+            if (!(maj > 0.0f)) return leaveCell();
+            invMaj = 1.0f / maj;
+            inCell = true;
+        }
+        if (!(t < tExit)) return leaveCell();
+        float stp = -logf(1.0f - rng.next1D()) * invMaj;
+        if (stp <= 0.0f) stp = 1e-5f;
+        if (t + stp >= tExit) return leaveCell();
+        t += stp;
+        double px = g.io.x + g.id.x * t, py = g.io.y + g.id.y * t, pz = g.io.z + g.id.z * t;
+        float dn = gridSample(m, px, py, pz) * m.densityScale;
+        if (dn < 1e-5f) dn = 0.0f;
+        if (!(dn > 0.0f)) return false;
+        if (SAMPLE) {
+            float r = rng.next1D() * 3.0f;
+            int c = (int)r < 2 ? (int)r : 2;
+            float prob = dn * m.sigma_t[c] * invMaj;
+            if (rng.next1D() < prob) {
+                hit = true;
+                dens = dn;
+                ch = c;
+                return true;
+            }
+            return false;
+        } else {
+            V3 lst = V3{m.sigma_t[0], m.sigma_t[1], m.sigma_t[2]} * dn;
+            tr = tr * (sameV(1.0f) - lst * invMaj);
+            float maxTr = stdMax(stdMax(tr.x, tr.y), tr.z);
+            if (maxTr < 1e-4f) {
+                hit = true;
+                return true;
+            }
+            if (maxTr < 0.1f) {
+                float q = stdMax(0.05f, 1.0f - maxTr);
+                if (rng.next1D() < q) {
+                    hit = true;
+                    return true;
+                }
+                tr = cdiv(tr, 1.0f - q);
+            }
+            return false;
+        }
+    }
+    // ratio tracking: the segment's transmittance
+    GDEV_INLINE V3 transmittance() const { return hit ? sameV(0.0f) : tr; }
+};
 
 // Medium::Tr (homogeneous.cpp:22-30; heterogeneous.cpp:106-229 ratio tracking)
 GDEV_INLINE V3 mediumTr(const DMedium& m, V3 o, V3 d, float tmin, float tmax, Rng& rng) {
@@ -761,43 +846,11 @@ GDEV_INLINE V3 mediumTr(const DMedium& m, V3 o, V3 d, float tmin, float tmax, Rn
         if (__builtin_isinf(tmax)) return sameV(0.0f);
         return V3{expf(-m.sigma_t[0] * tmax), expf(-m.sigma_t[1] * tmax), expf(-m.sigma_t[2] * tmax)};
     }
-    V3 tr = sameV(1.0f);
-    GridWalk g;
-    if (!gridSetup(m, o, d, tmin, tmax, g)) return tr;
-    float t = g.t;
-    const V3 st = V3{m.sigma_t[0], m.sigma_t[1], m.sigma_t[2]};
-    while (t < g.tMax) {
-        if (g.cx < 0 || g.cx >= m.resX || g.cy < 0 || g.cy >= m.resY || g.cz < 0 || g.cz >= m.resZ) break;
-        float maj = m.majorants[(size_t)g.cz * (m.resX * m.resY) + g.cy * m.resX + g.cx];
-        float tExit = stdMin4(g.tmx, g.tmy, g.tmz, g.tMax);
-        if (maj > 0.0f) {
-            float invMaj = 1.0f / maj;
-            while (t < tExit) {
-                float step = -logf(1.0f - rng.next1D()) * invMaj;
-                if (step <= 0.0f) step = 1e-5f;
-                if (t + step >= tExit) break;
-                t += step;
-                double px = g.io.x + g.id.x * t, py = g.io.y + g.id.y * t, pz = g.io.z + g.id.z * t;
-                float dens = gridSample(m, px, py, pz) * m.densityScale;
-                if (dens < 1e-5f) dens = 0.0f;
-                if (dens > 0.0f) {
-                    V3 lst = st * dens;
-                    tr = tr * (sameV(1.0f) - lst * invMaj);
-                    float maxTr = stdMax(stdMax(tr.x, tr.y), tr.z);
-                    if (maxTr < 1e-4f) return sameV(0.0f);
-                    if (maxTr < 0.1f) {
-                        float q = stdMax(0.05f, 1.0f - maxTr);
-                        if (rng.next1D() < q) return sameV(0.0f);
-                        tr = cdiv(tr, 1.0f - q);
-                    }
-                }
-            }
-        }
-        t = tExit;
-        if (t >= g.tMax) break;
-        gridAdvance(g);
+    GridTrack<false> k;
+    if (!k.start(m, o, d, tmin, tmax)) return sameV(1.0f);
+    while (!k.step(m, rng)) {
     }
-    return tr;
+    return k.transmittance();
 }
 
 struct MediumEvent {
@@ -805,6 +858,19 @@ struct MediumEvent {
     V3 p;
     V3 sigma_s;
 };
+
+// a delta-tracking collision as a medium event, and the throughput factor
+// 1 / sigma_t(channel) it returns (heterogeneous.cpp:251-376)
+GDEV_INLINE void gridEvent(const DMedium& m, V3 o, V3 d, const GridTrack<true>& k, MediumEvent& ev) {
+    ev.valid = true;
+    ev.p = o + d * k.t;
+    ev.sigma_s = V3{m.sigma_s[0], m.sigma_s[1], m.sigma_s[2]} * k.dens;
+}
+GDEV_INLINE V3 gridWeight(const DMedium& m, const GridTrack<true>& k) {
+    V3 lst = V3{m.sigma_t[0], m.sigma_t[1], m.sigma_t[2]} * k.dens;
+    float sc = k.ch == 0 ? lst.x : (k.ch == 1 ? lst.y : lst.z);
+    return cdiv(sameV(1.0f), sc);
+}
 
 // Medium::sample (homogeneous.cpp:32-63; heterogeneous.cpp:251-376 delta tracking)
 GDEV_INLINE V3 mediumSample(const DMedium& m, V3 o, V3 d, float tmin, float tmax, Rng& rng, MediumEvent& ev) {
@@ -827,43 +893,13 @@ GDEV_INLINE V3 mediumSample(const DMedium& m, V3 o, V3 d, float tmin, float tmax
         }
         return cdiv(tr, pdf);
     }
-    GridWalk g;
-    if (!gridSetup(m, o, d, tmin, tmax, g)) return sameV(1.0f);
-    float t = g.t;
-    while (t < g.tMax) {
-        if (g.cx < 0 || g.cx >= m.resX || g.cy < 0 || g.cy >= m.resY || g.cz < 0 || g.cz >= m.resZ) break;
-        float maj = m.majorants[(size_t)g.cz * (m.resX * m.resY) + g.cy * m.resX + g.cx];
-        float tExit = stdMin4(g.tmx, g.tmy, g.tmz, g.tMax);
-        if (maj > 0.0f) {
-            float invMaj = 1.0f / maj;
-            while (t < tExit) {
-                float step = -logf(1.0f - rng.next1D()) * invMaj;
-                if (step <= 0.0f) step = 1e-5f;
-                if (t + step >= tExit) break;
-                t += step;
-                double px = g.io.x + g.id.x * t, py = g.io.y + g.id.y * t, pz = g.io.z + g.id.z * t;
-                float dens = gridSample(m, px, py, pz) * m.densityScale;
-                if (dens < 1e-5f) dens = 0.0f;
-                if (dens > 0.0f) {
-                    float r = rng.next1D() * 3.0f;
-                    int ch = (int)r < 2 ? (int)r : 2;
-                    float prob = dens * m.sigma_t[ch] * invMaj;
-                    if (rng.next1D() < prob) {
-                        ev.valid = true;
-                        ev.p = o + d * t;
-                        ev.sigma_s = V3{m.sigma_s[0], m.sigma_s[1], m.sigma_s[2]} * dens;
-                        V3 lst = st * dens;
-                        float sc = ch == 0 ? lst.x : (ch == 1 ? lst.y : lst.z);
-                        return cdiv(sameV(1.0f), sc);
-                    }
-                }
-            }
-        }
-        t = tExit;
-        if (t >= g.tMax) break;
-        gridAdvance(g);
+    GridTrack<true> k;
+    if (!k.start(m, o, d, tmin, tmax)) return sameV(1.0f);
+    while (!k.step(m, rng)) {
     }
-    return sameV(1.0f);
+    if (!k.hit) return sameV(1.0f);
+    gridEvent(m, o, d, k, ev);
+    return gridWeight(m, k);
 }
 
 // ------------------------------------------------------------------ camera

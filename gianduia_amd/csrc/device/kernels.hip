@@ -112,46 +112,50 @@ GDEV_INLINE void retirePaths(uint32_t* chunkLive, bool ending, int chunk) {
     }
 }
 
-// Block-aggregated append to two queues: one atomic per block and queue
+// Block-aggregated append to three queues: one atomic per block and queue
 // (a counter shared by the whole launch serialises its atomics, so fewer is
 // faster).  Every thread of the block calls this; it returns this lane's
-// slots in the two queues.  c1 may be null.
+// slots in the queues.  c1 and c2 may be null.
 constexpr int kShadeBlock = 512;
 // waves per SIMD the shading kernel and the transmittance shadow kernel are
-// compiled for (register budget; on the volumetric scene 4 waves take the
-// transmittance shadows from 53 to 48 ms per frame, the shading kernel gains
-// nothing from 3)
+// compiled for (register budget; on the volumetric scene the shading kernel
+// gains nothing from 3, the persistent transmittance kernel runs 2 waves in
+// 45 ms per frame, 4 waves (with spills) in 52)
 #ifndef GND_SHADE_WAVES
 #define GND_SHADE_WAVES 2
 #endif
 #ifndef GND_SHADOWTR_WAVES
-#define GND_SHADOWTR_WAVES 4
+#define GND_SHADOWTR_WAVES 2
 #endif
-GDEV_INLINE void blockAppend2(uint32_t* c0, bool p0, uint32_t& pos0, uint32_t* c1, bool p1, uint32_t& pos1) {
-    __shared__ uint32_t sN[2][kShadeBlock / 64];
-    __shared__ uint32_t sBase[2];
+GDEV_INLINE void blockAppend3(uint32_t* c0, bool p0, uint32_t& pos0, uint32_t* c1, bool p1, uint32_t& pos1,
+                              uint32_t* c2, bool p2, uint32_t& pos2) {
+    __shared__ uint32_t sN[3][kShadeBlock / 64];
+    __shared__ uint32_t sBase[3];
     const int w = (int)(threadIdx.x >> 6), nw = (int)(blockDim.x >> 6);
-    const uint64_t m0 = __ballot(p0), m1 = __ballot(c1 && p1);
+    const uint64_t m0 = __ballot(p0), m1 = __ballot(c1 && p1), m2 = __ballot(c2 && p2);
     if (__lane_id() == 0) {
         sN[0][w] = (uint32_t)__popcll(m0);
         sN[1][w] = (uint32_t)__popcll(m1);
+        sN[2][w] = (uint32_t)__popcll(m2);
     }
     __syncthreads();
-    if (threadIdx.x < 2) {
+    if (threadIdx.x < 3) {
         uint32_t total = 0;
         for (int i = 0; i < nw; ++i) total += sN[threadIdx.x][i];
-        uint32_t* c = threadIdx.x == 0 ? c0 : c1;
+        uint32_t* c = threadIdx.x == 0 ? c0 : (threadIdx.x == 1 ? c1 : c2);
         sBase[threadIdx.x] = (total && c) ? atomicAdd(c, total) : 0u;
     }
     __syncthreads();
-    uint32_t pre0 = 0, pre1 = 0;
+    uint32_t pre0 = 0, pre1 = 0, pre2 = 0;
     for (int i = 0; i < w; ++i) {
         pre0 += sN[0][i];
         pre1 += sN[1][i];
+        pre2 += sN[2][i];
     }
     const uint64_t lt = (1ull << __lane_id()) - 1ull;
     pos0 = sBase[0] + pre0 + (uint32_t)__popcll(m0 & lt);
     pos1 = sBase[1] + pre1 + (uint32_t)__popcll(m1 & lt);
+    pos2 = sBase[2] + pre2 + (uint32_t)__popcll(m2 & lt);
     __syncthreads();  // sN / sBase are reused by the next call
 }
 
@@ -266,12 +270,16 @@ struct WaveArgs {
     int32_t* qIn;
     int32_t* qOut;
     RayQ rIn, rOut;       // rays of the in- and out-queue entries
-    uint32_t* counts;     // [0] in, [1] out, [2] shadow, [3] [4] ray fetch
+    int32_t* gIn;         // in-queue indices of the paths starting in a density
+    int32_t* gOut;        // grid (split mode), and the same list being built
+    uint32_t* counts;     // [0] in, [1] out, [2] shadow, [3] grid out, [4] grid in,
+                          // [8..] ray feeds
     uint32_t* chunkLive;  // live paths per engine chunk, or null
     uint32_t camQueueBase;  // in-queue entries before the paths kCamera starts
     Counters* ctr;
     int32_t pathOnlyFast; // shadows may use any-hit
-    int32_t split;        // NEE and scatter in separate kernels
+    int32_t split;        // NEE and scatter in separate kernels, density-grid
+                          // delta tracking in kMedium ahead of them
 };
 
 GDEV_INLINE void storeRay(const RayQ& R, uint32_t i, V3 o, V3 d) {
@@ -356,7 +364,8 @@ constexpr int kRefill = GND_REFILL_LANES;
 #define GND_FEED_PARTS 1
 #endif
 constexpr int kParts = GND_FEED_PARTS;
-constexpr uint32_t kFeedTrace = 8, kFeedShadow = kFeedTrace + kParts, kNumCounts = 64;
+constexpr uint32_t kFeedTrace = 8, kFeedShadow = kFeedTrace + kParts, kFeedMedium = kFeedShadow + kParts,
+                   kNumCounts = 64;
 
 // The persistent traversals' ray feed: waves take rays in queue order with
 // one atomic per refill.  The queue can be cut into kParts contiguous parts
@@ -565,8 +574,11 @@ GDEV_INLINE bool scatterStep(const DScene& S, const DJob& J, PathLocal& s, const
 // First half of a bounce: medium sampling (volpath), emission with MIS, null
 // BSDF pass-through and the NEE light sample with its shadow ray.
 // Returns: 0 = dead, 1 = continues straight (already a new ray), 2 = needs scatter.
+// pre: the medium was a density grid that kMedium already walked (throughput
+// and sampler state updated); preHit / preP = its collision, if any.
 GDEV_INLINE int neeStep(const WaveArgs& A, const DJob& J, PathLocal& s, const HitRec& h, Event& ev, bool& shadow,
-                        V3& so, V3& sd, float& stmax, int& smedium, V3& pending, V3& lightP) {
+                        V3& so, V3& sd, float& stmax, int& smedium, V3& pending, V3& lightP, bool pre, bool preHit,
+                        V3 preP) {
     const DScene& S = A.S;
     const bool vol = J.integrator == GND_INTEGRATOR_VOLPATH;
     const bool hit = h.inst >= 0;
@@ -576,10 +588,15 @@ GDEV_INLINE int neeStep(const WaveArgs& A, const DJob& J, PathLocal& s, const Hi
     const int medium = vol ? flagMedium(s.flags) : -1;
     if (medium >= 0) {
         MediumEvent me;
-        const float tm = hit ? h.t : finf();
-        s.tp = s.tp * mediumSample(S.media[medium], s.o, s.d, kEps, tm, s.rng, me);
+        if (pre) {
+            me.valid = preHit;
+            me.p = preP;
+        } else {
+            const float tm = hit ? h.t : finf();
+            s.tp = s.tp * mediumSample(S.media[medium], s.o, s.d, kEps, tm, s.rng, me);
+            if (me.valid) s.tp = s.tp * me.sigma_s;
+        }
         if (me.valid) {
-            s.tp = s.tp * me.sigma_s;
             if (isBlack(s.tp)) return 0;
             const float lsp = 1.0f / (float)(uint32_t)S.nEmitters;
             const int ep = pickEmitter(S, s.rng.next1D());
@@ -657,6 +674,78 @@ GDEV_INLINE int neeStep(const WaveArgs& A, const DJob& J, PathLocal& s, const Hi
     return 2;
 }
 
+// Paths whose ray starts in a density grid, when kMedium runs (split mode).
+GDEV_INLINE bool gridWalked(const WaveArgs& A, const DJob& J, uint32_t flags) {
+    const int medium = J.integrator == GND_INTEGRATOR_VOLPATH ? flagMedium(flags) : -1;
+    return A.split && medium >= 0 && A.S.media[medium].type == GND_MEDIUM_GRID;
+}
+
+// Delta tracking (Medium::sample, heterogeneous.cpp:251-376) for the paths of
+// the in-queue whose ray starts in a density grid (the list gIn, built by the
+// shading kernels of the bounce before), ahead of kShade: the first sampler
+// use of a volpath bounce (volPath.cpp:40-48).  Persistent, one walk step per
+// lane per iteration; a lane whose walk ends takes the next path, so a wave
+// does not run at the pace of its longest walk.  Writes the throughput (times
+// the 1/sigma_t weight and, at a collision, sigma_s) and the sampler state;
+// evKind / evp* = whether and where the walk collided.
+__global__ __launch_bounds__(256) void kMedium(WaveArgs A) {
+    const uint32_t n = A.counts[4];
+    const uint64_t waveMask = __ballot(1);
+    const DScene& S = A.S;
+    RayFeed feed;
+    feed.init(A.counts + kFeedMedium, n);
+    bool drained = false;
+    int64_t cur = -1;
+    int p = -1, medium = 0;
+    Rng rng;
+    GridTrack<true> k;
+    for (;;) {
+        if (!drained) {
+            const int64_t j = feed.take(cur < 0, waveMask, drained);
+            if (j >= 0) {
+                const int32_t i = A.gIn[j];
+                const int q = A.qIn[i];
+                medium = flagMedium(A.P.flags[q]);
+                V3 o, d;
+                loadRay(A.rIn, (uint32_t)i, o, d);
+                const float tm = A.Hq.inst[i] >= 0 ? A.Hq.t[i] : finf();
+                p = q;
+                rng.s = A.P.rng[q];
+                cur = i;
+                // a ray that misses the grid box leaves the path as it is
+                if (!k.start(S.media[medium], o, d, kEps, tm)) {
+                    A.P.evKind[q] = EV_NONE;
+                    cur = -1;
+                }
+            }
+        }
+        const bool busy = cur >= 0;
+        if (__ballot(busy) == 0) {
+            if (drained) break;
+            continue;
+        }
+        if (busy && k.step(S.media[medium], rng)) {
+            const DMedium& m = S.media[medium];
+            V3 tp = V3{A.P.tpr[p], A.P.tpg[p], A.P.tpb[p]};
+            if (k.hit) {
+                V3 o, d;
+                loadRay(A.rIn, (uint32_t)cur, o, d);
+                MediumEvent me;
+                gridEvent(m, o, d, k, me);
+                tp = tp * gridWeight(m, k);
+                tp = tp * me.sigma_s;
+                A.P.evpx[p] = me.p.x; A.P.evpy[p] = me.p.y; A.P.evpz[p] = me.p.z;
+            } else {
+                tp = tp * sameV(1.0f);
+            }
+            A.P.tpr[p] = tp.x; A.P.tpg[p] = tp.y; A.P.tpb[p] = tp.z;
+            A.P.rng[p] = rng.s;
+            A.P.evKind[p] = k.hit ? EV_MEDIUM : EV_NONE;
+            cur = -1;
+        }
+    }
+}
+
 // SPLIT = false: full bounce (NEE + scatter) in one kernel; shadow rays resolve
 // afterwards (valid when shadow rays consume no random numbers).
 // SPLIT = true: NEE only; the event is stored for kScatter.
@@ -668,7 +757,7 @@ __global__ __launch_bounds__(kShadeBlock, GND_SHADE_WAVES) void kShade(WaveArgs 
     for (uint32_t base = start; base < n; base += stride) {
         const uint32_t i = base + threadIdx.x;
         const bool active = i < n;
-        bool alive = false, shadow = false, ending = false;
+        bool alive = false, shadow = false, ending = false, toGrid = false;
         int p = -1;
         PathLocal s;
         V3 so, sd, pending = sameV(0.0f), lightP;
@@ -687,7 +776,14 @@ __global__ __launch_bounds__(kShadeBlock, GND_SHADE_WAVES) void kShade(WaveArgs 
             h.v = A.Hq.v[i];
             Event ev;
             ev.kind = EV_NONE;
-            int r = neeStep(A, J, s, h, ev, shadow, so, sd, stmax, smedium, pending, lightP);
+            bool pre = false, preHit = false;
+            V3 preP = sameV(0.0f);
+            if (SPLIT && gridWalked(A, J, s.flags)) {
+                pre = true;
+                preHit = A.P.evKind[p] == EV_MEDIUM;
+                if (preHit) preP = V3{A.P.evpx[p], A.P.evpy[p], A.P.evpz[p]};
+            }
+            int r = neeStep(A, J, s, h, ev, shadow, so, sd, stmax, smedium, pending, lightP, pre, preHit, preP);
             if (!A.pathOnlyFast && shadow) {
                 // the general shadow kernel always needs the ray; the any-hit
                 // path skips zero contributions
@@ -712,16 +808,18 @@ __global__ __launch_bounds__(kShadeBlock, GND_SHADE_WAVES) void kShade(WaveArgs 
                 A.P.evKind[p] = EV_NONE;
             }
             if (SPLIT && r == 1) A.P.evKind[p] = EV_NONE;
+            toGrid = SPLIT && alive && gridWalked(A, J, s.flags);
             storePath(A.P, p, s);
             ending = !alive && !(SPLIT && r == 2);
         }
         retirePaths(A.chunkLive, ending, ending && A.chunkLive ? A.P.chunk[p] : -1);
-        uint32_t qs, ss;
-        blockAppend2(&A.counts[1], alive, qs, &A.counts[2], shadow, ss);
+        uint32_t qs, ss, gs;
+        blockAppend3(&A.counts[1], alive, qs, &A.counts[2], shadow, ss, SPLIT ? &A.counts[3] : nullptr, toGrid, gs);
         if (alive) {
             A.qOut[qs] = p;
             storeRay(A.rOut, qs, s.o, s.d);
         }
+        if (toGrid) A.gOut[gs] = (int32_t)qs;
         if (shadow) {
             const ShadowSoA& Sh = A.Sh;
             Sh.path[ss] = p;
@@ -743,7 +841,7 @@ __global__ __launch_bounds__(kShadeBlock) void kScatter(WaveArgs A) {
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < nEv; base += stride) {
         const uint32_t i = base + threadIdx.x;
-        bool alive = false, ending = false;
+        bool alive = false, ending = false, toGrid = false;
         int p = -1;
         PathLocal s;
         if (i < nEv) {
@@ -762,17 +860,19 @@ __global__ __launch_bounds__(kShadeBlock) void kScatter(WaveArgs A) {
                 ev.u = A.P.evu[p];
                 ev.v = A.P.evv[p];
                 alive = scatterStep(A.S, A.jobs[s.job], s, ev);
+                toGrid = alive && gridWalked(A, A.jobs[s.job], s.flags);
                 storePath(A.P, p, s);
                 ending = !alive;
             }
         }
         retirePaths(A.chunkLive, ending, ending && A.chunkLive ? A.P.chunk[p] : -1);
-        uint32_t qs, unused;
-        blockAppend2(&A.counts[1], alive, qs, nullptr, false, unused);
+        uint32_t qs, unused, gs;
+        blockAppend3(&A.counts[1], alive, qs, nullptr, false, unused, &A.counts[3], toGrid, gs);
         if (alive) {
             A.qOut[qs] = p;
             storeRay(A.rOut, qs, s.o, s.d);
         }
+        if (toGrid) A.gOut[gs] = (int32_t)qs;
     }
 }
 
@@ -816,6 +916,11 @@ __global__ __launch_bounds__(256, GND_SHADOW_WAVES) void kShadowAny(WaveArgs A) 
 
 // General shadow rays: closest-hit loop through null-BSDF surfaces with
 // per-segment medium transmittance (path.cpp:122-140 / volPath.cpp:261-291).
+// Persistent: each lane runs one shadow ray as a small state machine (trace
+// the next segment, then walk a density grid one ratio-tracking step at a
+// time) and takes the next ray when it ends.  An iteration either traverses
+// or steps the walks, whichever more lanes wait for: a traversal costs many
+// walk steps, so doing both every iteration would leave the walks crawling.
 template <int LDSD, bool COUNT>
 __global__ __launch_bounds__(256, GND_SHADOWTR_WAVES) void kShadow(WaveArgs A) {
     extern __shared__ int sStack[];
@@ -825,46 +930,121 @@ __global__ __launch_bounds__(256, GND_SHADOWTR_WAVES) void kShadow(WaveArgs A) {
     TravStack<LDSD> st = makeStack<LDSD>(sStack, spill);
     const DScene& S = A.S;
     const ShadowSoA& Sh = A.Sh;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = xcdBlock() * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const int p = Sh.path[i];
-        V3 o = V3{Sh.ox[i], Sh.oy[i], Sh.oz[i]}, d = V3{Sh.dx[i], Sh.dy[i], Sh.dz[i]};
-        float tmax = Sh.tmax[i];
-        const V3 pending = V3{Sh.pr[i], Sh.pg[i], Sh.pb[i]};
-        HitRec h;
-        const int job = A.P.job[p];
-        const bool vol = A.jobs[job].integrator == GND_INTEGRATOR_VOLPATH;
-        int medium = Sh.medium[i];
-        const V3 lightP = V3{Sh.lpx[i], Sh.lpy[i], Sh.lpz[i]};
-        V3 Tr = sameV(1.0f);
-        Rng rng;
-        rng.s = A.P.rng[p];
-        bool visible = false;
-        for (int iter = 0; iter < 1024; ++iter) {
-            const bool hit = traceRay<false, COUNT, LDSD>(S, o, d, kEps, tmax, h, st, cnt);
-            if (vol && medium >= 0) Tr = Tr * mediumTr(S.media[medium], o, d, kEps, hit ? h.t : tmax, rng);
-            if (vol && isBlack(Tr)) break;
-            if (!hit) { visible = true; break; }
-            Isect it;
-            fillInteraction(S, o, d, h, it);
-            const DMaterial& m = S.mats[S.prims[it.prim].material];
-            if (m.type != GND_MAT_INDEX) break;  // occluded
-            if (vol) {
-                int inside, outside;
-                mediumInterface(S, it.prim, inside, outside);
-                medium = dot(d, it.n) > 0.0f ? outside : inside;
-                tmax = length(it.p - lightP) - kEps;
-            } else {
-                tmax = h.t - length(it.p - o);  // the hit distance minus the walked length
+    const uint64_t waveMask = __ballot(1);
+    RayFeed feed;
+    feed.init(A.counts + kFeedShadow, n);
+    bool drained = false;
+    constexpr int kIdle = 0, kTrace = 1, kWalk = 2;
+    constexpr int kVisible = 0, kOccluded = 1, kOnward = 2;
+    int64_t cur = -1;
+    int phase = kIdle, medium = -1, iter = 0, p = -1;
+    bool vol = false;
+    V3 o = sameV(0.0f), d = sameV(0.0f), Tr = sameV(1.0f);
+    float tmax = 0.0f;
+    Rng rng;
+    // what follows the segment in flight: visible, occluded, or onward from
+    // the null surface at `no` into `nmedium` up to `ntmax`
+    int next = kVisible, nmedium = -1;
+    V3 no = sameV(0.0f);
+    float ntmax = 0.0f;
+    GridTrack<false> k;
+    for (;;) {
+        if (!drained) {
+            const int64_t i = feed.take(phase == kIdle, waveMask, drained);
+            if (i >= 0) {
+                cur = i;
+                p = Sh.path[i];
+                o = V3{Sh.ox[i], Sh.oy[i], Sh.oz[i]};
+                d = V3{Sh.dx[i], Sh.dy[i], Sh.dz[i]};
+                tmax = Sh.tmax[i];
+                vol = A.jobs[A.P.job[p]].integrator == GND_INTEGRATOR_VOLPATH;
+                medium = Sh.medium[i];
+                Tr = sameV(1.0f);
+                rng.s = A.P.rng[p];
+                iter = 0;
+                phase = kTrace;
             }
-            o = it.p;
         }
-        A.P.rng[p] = rng.s;
-        if (visible) {
-            V3 c = vol ? pending * Tr : pending;
-            A.P.Lr[p] = A.P.Lr[p] + c.x;
-            A.P.Lg[p] = A.P.Lg[p] + c.y;
-            A.P.Lb[p] = A.P.Lb[p] + c.z;
+        const int nTrace = __popcll(__ballot(phase == kTrace)), nWalk = __popcll(__ballot(phase == kWalk));
+        if (nTrace + nWalk == 0) {
+            if (drained) break;
+            continue;
+        }
+        // one traversal or one walk step per iteration, for whichever phase
+        // more lanes wait in
+        const bool tracing = nTrace > nWalk;
+        bool segDone = false;
+        if (tracing && phase == kTrace) {
+            HitRec h;
+            const bool hit = traceRay<false, COUNT, LDSD>(S, o, d, kEps, tmax, h, st, cnt);
+            if (!hit) {
+                next = kVisible;
+            } else {
+                Isect it;
+                fillInteraction(S, o, d, h, it);
+                const DMaterial& m = S.mats[S.prims[it.prim].material];
+                if (m.type != GND_MAT_INDEX) {
+                    next = kOccluded;
+                } else {
+                    next = kOnward;
+                    if (vol) {
+                        int inside, outside;
+                        mediumInterface(S, it.prim, inside, outside);
+                        nmedium = dot(d, it.n) > 0.0f ? outside : inside;
+                        const V3 lightP = V3{Sh.lpx[cur], Sh.lpy[cur], Sh.lpz[cur]};
+                        ntmax = length(it.p - lightP) - kEps;
+                    } else {
+                        nmedium = medium;
+                        ntmax = h.t - length(it.p - o);  // the hit distance minus the walked length
+                    }
+                    no = it.p;
+                }
+            }
+            segDone = true;
+            if (vol && medium >= 0) {
+                const DMedium& md = S.media[medium];
+                const float segT = hit ? h.t : tmax;
+                if (md.type == GND_MEDIUM_GRID) {
+                    if (k.start(md, o, d, kEps, segT)) {
+                        phase = kWalk;
+                        segDone = false;
+                    } else {
+                        Tr = Tr * sameV(1.0f);
+                    }
+                } else {
+                    Tr = Tr * mediumTr(md, o, d, kEps, segT, rng);
+                }
+            }
+        } else if (!tracing && phase == kWalk) {
+            if (k.step(S.media[medium], rng)) {
+                Tr = Tr * k.transmittance();
+                segDone = true;
+            }
+        }
+        if (segDone) {
+            bool fin = true, visible = false;
+            if (vol && isBlack(Tr)) {
+            } else if (next == kVisible) {
+                visible = true;
+            } else if (next == kOnward && ++iter < 1024) {
+                fin = false;
+                o = no;
+                medium = nmedium;
+                tmax = ntmax;
+                phase = kTrace;
+            }
+            if (fin) {
+                A.P.rng[p] = rng.s;
+                if (visible) {
+                    const V3 pending = V3{Sh.pr[cur], Sh.pg[cur], Sh.pb[cur]};
+                    const V3 c = vol ? pending * Tr : pending;
+                    A.P.Lr[p] = A.P.Lr[p] + c.x;
+                    A.P.Lg[p] = A.P.Lg[p] + c.y;
+                    A.P.Lb[p] = A.P.Lb[p] + c.z;
+                }
+                cur = -1;
+                phase = kIdle;
+            }
         }
     }
     if (COUNT) flushCounters(A.ctr, cnt, st.overflow, true);
@@ -961,8 +1141,8 @@ __global__ void kFillU32(uint32_t* p, size_t n, uint32_t v) {
     if (i < n) p[i] = v;
 }
 
-// End of a bounce: the out-queue becomes the next in-queue; out, shadow and
-// ray-fetch counters restart at zero (a kernel, so the step can be captured
+// End of a bounce: the out-queue (and its density-grid list) becomes the next
+// in-queue; out, shadow and ray-fetch counters restart at zero (a kernel, so the step can be captured
 // in a graph).
 __global__ void kRotate(uint32_t* counts, Counters* c) {
     const uint32_t t = threadIdx.x;
@@ -973,6 +1153,8 @@ __global__ void kRotate(uint32_t* counts, Counters* c) {
         counts[0] = counts[1];
         counts[1] = 0u;
         counts[2] = 0u;
+        counts[4] = counts[3];
+        counts[3] = 0u;
     } else if (t >= kFeedTrace && t < kNumCounts) {
         counts[t] = 0u;
     }
@@ -989,7 +1171,7 @@ struct gnd_hip_scene {
 };
 
 // Path state, hit and shadow queues and queue counters for `capacity` paths
-// in flight: 272 bytes per path slot in 67 SoA arrays, each 256-B aligned.
+// in flight: 280 bytes per path slot in 69 SoA arrays, each 256-B aligned.
 struct WaveBuffers {
     size_t capacity = 0;
     void* pathBuf = nullptr;
@@ -998,6 +1180,7 @@ struct WaveBuffers {
     ShadowSoA Sh{};
     int32_t* q[2] = {nullptr, nullptr};
     RayQ rq[2] = {};
+    int32_t* gl[2] = {nullptr, nullptr};  // density-grid lists of the two queues
     uint32_t* counts = nullptr;  // kNumCounts queue and ray-feed counters
 };
 
@@ -1130,7 +1313,7 @@ void ensureDev(T*& p, size_t& cap, size_t need) {
 
 size_t waveBufferBytes(size_t cap) {
     const size_t per = ((cap * 4 + 255) / 256) * 256;
-    return 66 * per + ((cap * 8 + 255) / 256) * 256;
+    return 68 * per + ((cap * 8 + 255) / 256) * 256;
 }
 
 void freeWaveBuffers(WaveBuffers& wb) {
@@ -1145,7 +1328,7 @@ void ensureWaveBuffers(WaveBuffers& wb, size_t cap) {
     if (wb.pathBuf) GND_CHECK(hipFree(wb.pathBuf));
     wb.pathBuf = nullptr;
     wb.capacity = 0;
-    // 66 four-byte fields and one eight-byte field per slot
+    // 68 four-byte fields and one eight-byte field per slot
     const size_t bytes = waveBufferBytes(cap);
     GND_CHECK(hipMalloc(&wb.pathBuf, bytes));
     char* b = static_cast<char*>(wb.pathBuf);
@@ -1181,6 +1364,8 @@ void ensureWaveBuffers(WaveBuffers& wb, size_t cap) {
     for (auto pp : g) *pp = (float*)take(4);
     wb.q[0] = (int32_t*)take(4);
     wb.q[1] = (int32_t*)take(4);
+    wb.gl[0] = (int32_t*)take(4);
+    wb.gl[1] = (int32_t*)take(4);
     for (RayQ& r : wb.rq)
         for (float** pp : {&r.ox, &r.oy, &r.oz, &r.dx, &r.dy, &r.dz}) *pp = (float*)take(4);
     if (off > bytes) throw std::runtime_error("wave buffer layout overflow");
@@ -1645,13 +1830,14 @@ struct BouncePlan {
         b = std::min(b, gridCap);
         return std::max(8u, (b + 7) / 8 * 8);
     }
-    // shading kernels: kShadeBlock threads per block, the same thread cap
-    unsigned shadeGrid(uint32_t n) const {
-        const unsigned cap = std::max(8u, gridCap * (unsigned)B / (unsigned)kShadeBlock);
-        unsigned b = (unsigned)((n + (uint32_t)kShadeBlock - 1) / (uint32_t)kShadeBlock);
+    // kernels with `block` threads per block, the same thread cap
+    unsigned gridFor(uint32_t n, unsigned block) const {
+        const unsigned cap = std::max(8u, gridCap * (unsigned)B / block);
+        unsigned b = (unsigned)((n + block - 1) / block);
         b = std::min(b, cap);
         return std::max(8u, (b + 7) / 8 * 8);
     }
+    unsigned shadeGrid(uint32_t n) const { return gridFor(n, (unsigned)kShadeBlock); }
 };
 
 static BouncePlan makeBouncePlan(const gnd_hip_ctx* ctx, const gnd_hip_scene* scene, bool anyVol) {
@@ -1708,6 +1894,16 @@ struct Timeline {
     }
 };
 
+// in-queue `cur` of the two (with its rays and density-grid list), out-queue the other
+static void bindQueues(WaveArgs& A, const WaveBuffers& wb, int cur) {
+    A.qIn = wb.q[cur];
+    A.qOut = wb.q[1 - cur];
+    A.rIn = wb.rq[cur];
+    A.rOut = wb.rq[1 - cur];
+    A.gIn = wb.gl[cur];
+    A.gOut = wb.gl[1 - cur];
+}
+
 // One bounce over the in-queue A.qIn (count in counts[0], at most `live`):
 // closest hit, shading and NEE, shadow rays, the split scatter, then the queue
 // rotation.  No host synchronisation, so the chain can be captured.
@@ -1718,6 +1914,11 @@ static void launchBounce(gnd_hip_ctx* ctx, const WaveArgs& A, const BouncePlan& 
     GND_CHECK(hipGetLastError());
     if (tl) tl->lap(t, 0);
     const unsigned gs = K.shadeGrid(live);
+    if (K.split) {
+        // density-grid delta tracking for the paths that start in a grid
+        hipLaunchKernelGGL(kMedium, dim3(K.gridFor(live, 256)), dim3(256), 0, ctx->stream, A);
+        GND_CHECK(hipGetLastError());
+    }
     if (K.split) hipLaunchKernelGGL(kShade<true>, dim3(gs), dim3(kShadeBlock), 0, ctx->stream, A);
     else hipLaunchKernelGGL(kShade<false>, dim3(gs), dim3(kShadeBlock), 0, ctx->stream, A);
     GND_CHECK(hipGetLastError());
@@ -1843,10 +2044,7 @@ int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jo
             A.ctr = ctx->dCounters;
             A.pathOnlyFast = K.fastShadow ? 1 : 0;
             A.split = K.split ? 1 : 0;
-            A.qIn = ctx->wb.q[0];
-            A.qOut = ctx->wb.q[1];
-            A.rIn = ctx->wb.rq[0];
-            A.rOut = ctx->wb.rq[1];
+            bindQueues(A, ctx->wb, 0);
             size_t tc = tl.mark();
             A.camQueueBase = 0;
             hipLaunchKernelGGL(kCamera, dim3((w.slots + 255) / 256), dim3(256), 0, ctx->stream, A);
@@ -1865,10 +2063,7 @@ int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jo
                 GND_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
                 try {
                     for (int d = 0; d < depth; ++d) {
-                        A.qIn = ctx->wb.q[cur];
-                        A.qOut = ctx->wb.q[1 - cur];
-                        A.rIn = ctx->wb.rq[cur];
-                        A.rOut = ctx->wb.rq[1 - cur];
+                        bindQueues(A, ctx->wb, cur);
                         launchBounce(ctx, A, K, live, nullptr);
                         cur = 1 - cur;
                     }
@@ -1885,10 +2080,7 @@ int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jo
                 GND_CHECK(ie);
             }
             while (live > 0) {
-                A.qIn = ctx->wb.q[cur];
-                A.qOut = ctx->wb.q[1 - cur];
-                A.rIn = ctx->wb.rq[cur];
-                A.rOut = ctx->wb.rq[1 - cur];
+                bindQueues(A, ctx->wb, cur);
                 launchBounce(ctx, A, K, live, tlp);
                 cur = 1 - cur;
                 live = readLive(ctx, ctx->wb.counts);
@@ -2225,10 +2417,7 @@ void serverStep(gnd_hip_server* s) {
         A.chunks = s->dCam;
         A.nChunks = (int32_t)dc.size();
         A.nSlots = n;
-        A.qIn = s->wb.q[s->cur];
-        A.qOut = s->wb.q[1 - s->cur];
-        A.rIn = s->wb.rq[s->cur];
-        A.rOut = s->wb.rq[1 - s->cur];
+        bindQueues(A, s->wb, s->cur);
         A.camQueueBase = s->live;  // after the survivors of the last bounce
         hipLaunchKernelGGL(kCamera, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, A);
         GND_CHECK(hipGetLastError());
@@ -2237,10 +2426,7 @@ void serverStep(gnd_hip_server* s) {
     }
     // 2. one bounce for every live path
     if (s->live > 0) {
-        A.qIn = s->wb.q[s->cur];
-        A.qOut = s->wb.q[1 - s->cur];
-        A.rIn = s->wb.rq[s->cur];
-        A.rOut = s->wb.rq[1 - s->cur];
+        bindQueues(A, s->wb, s->cur);
         launchBounce(ctx, A, K, s->live, nullptr);
         s->cur = 1 - s->cur;
     }
