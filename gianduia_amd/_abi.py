@@ -122,7 +122,7 @@ class gnd_hip_stats(C.Structure):
 
 class gnd_hip_tuning(C.Structure):
     _fields_ = [("block_size", C.c_int32), ("stack_lds", C.c_int32), ("queue_capacity", C.c_int64),
-                ("use_graph", C.c_int32)]
+                ("use_graph", C.c_int32), ("refill_lanes", C.c_int32), ("shadow_refill_lanes", C.c_int32)]
 
 
 class gnd_hip_server_config(C.Structure):

@@ -92,18 +92,25 @@ class Device:
         t = A.gnd_hip_tuning()
         self._check(_lib().gnd_hip_get_tuning(self.ctx, C.byref(t)))
         return {"block_size": t.block_size, "stack_lds": t.stack_lds, "queue_capacity": t.queue_capacity,
-                "use_graph": t.use_graph}
+                "use_graph": t.use_graph, "refill_lanes": t.refill_lanes,
+                "shadow_refill_lanes": t.shadow_refill_lanes}
 
-    def set_tuning(self, block_size=None, stack_lds=None, queue_capacity=None, use_graph=None) -> None:
+    def set_tuning(self, block_size=None, stack_lds=None, queue_capacity=None, use_graph=None,
+                   refill_lanes=None, shadow_refill_lanes=None) -> None:
+        """Changes the given tunables; refill_lanes / shadow_refill_lanes 0
+        means the per-scene default."""
         cur = self.tuning()
         t = A.gnd_hip_tuning(block_size or cur["block_size"], stack_lds or cur["stack_lds"],
                              queue_capacity or cur["queue_capacity"],
-                             cur["use_graph"] if use_graph is None else int(use_graph))
+                             cur["use_graph"] if use_graph is None else int(use_graph),
+                             cur["refill_lanes"] if refill_lanes is None else int(refill_lanes),
+                             cur["shadow_refill_lanes"] if shadow_refill_lanes is None else int(shadow_refill_lanes))
         self._check(_lib().gnd_hip_set_tuning(self.ctx, C.byref(t)))
 
     def autotune(self, scene: "DeviceScene", job: "Job") -> dict:
         """gnd_hip_autotune: times one pass of `job` per block size x LDS stack
-        depth and keeps the fastest; returns the chosen tuning."""
+        depth, then per closest-hit and any-hit refill threshold, keeping the
+        fastest of each; returns the chosen tuning."""
         w, h = job.camera.width, job.camera.height
         film = self.alloc(w * h * 16)
         cj = A.gnd_hip_job()

@@ -280,6 +280,8 @@ struct WaveArgs {
     int32_t pathOnlyFast; // shadows may use any-hit
     int32_t split;        // NEE and scatter in separate kernels, density-grid
                           // delta tracking in kMedium ahead of them
+    int32_t refill;       // idle lanes before a wave refills: closest-hit
+    int32_t srefill;      // and any-hit traversal
 };
 
 GDEV_INLINE void storeRay(const RayQ& R, uint32_t i, V3 o, V3 d) {
@@ -352,12 +354,13 @@ __global__ void kCamera(WaveArgs A) {
 
 // Persistent traversal kernels hand out rays with one atomic per wave on a
 // counter zeroed before the launch.  A lane whose ray is done takes a new one
-// as soon as kRefill lanes of its wave are idle (or the whole wave is), so a
-// wave does not run at the pace of its slowest ray.
-#ifndef GND_REFILL_LANES
-#define GND_REFILL_LANES 24
-#endif
-constexpr int kRefill = GND_REFILL_LANES;
+// as soon as `refill` lanes of its wave are idle (or the whole wave is), so a
+// wave does not run at the pace of its slowest ray.  The best threshold
+// depends on the scene: a BVH that stays in L2 makes traversal steps cheap
+// next to a refill (fetching rays from HBM), so refills should gather more
+// lanes; a large BVH makes idle lanes the bigger loss.  The traversal
+// kernels take it from the tuning; the medium-walk kernels use kWalkRefill.
+constexpr int kWalkRefill = 24;
 // Queue counters: [0] in-queue size, [1] out-queue, [2] shadow queue, then
 // the ray-feed part counters of the two persistent traversals.
 #ifndef GND_FEED_PARTS
@@ -378,21 +381,22 @@ constexpr uint32_t kFeedTrace = 8, kFeedShadow = kFeedTrace + kParts, kFeedMediu
 struct RayFeed {
     uint32_t* ctr;
     uint32_t n;
-    int part, tried;  // wave-uniform
-    GDEV_INLINE void init(uint32_t* counters, uint32_t count) {
+    int part, tried, refill;  // wave-uniform
+    GDEV_INLINE void init(uint32_t* counters, uint32_t count, int refillLanes) {
         ctr = counters;
         n = count;
+        refill = refillLanes;
         part = (int)(blockIdx.x % (unsigned)kParts);
         tried = 0;
     }
     GDEV_INLINE uint32_t edge(int p) const { return (uint32_t)(((uint64_t)n * (uint64_t)p) / kParts); }
-    // Idle lanes take the next rays (once kRefill lanes, or the whole wave,
+    // Idle lanes take the next rays (once `refill` lanes, or the whole wave,
     // are idle); returns this lane's queue index or -1.  Sets `drained`
     // (wave-uniformly) once every part is used up.
     GDEV_INLINE int64_t take(bool idle, uint64_t waveMask, bool& drained) {
         const uint64_t m = __ballot(idle);
         const int k = __popcll(m);
-        if (k == 0 || (k < kRefill && m != waveMask)) return -1;
+        if (k == 0 || (k < refill && m != waveMask)) return -1;
         const int leader = __ffsll((unsigned long long)m) - 1;
         while (tried < kParts) {
             const uint32_t lo = edge(part), len = edge(part + 1) - lo;
@@ -425,7 +429,7 @@ GND_TRACE_KERNEL void kTracePaths(WaveArgs A) {
     int spill[spillSize<LDSD>()];
     TravStack<LDSD> st = makeStack<LDSD>(sStack, spill);
     RayFeed feed;
-    feed.init(A.counts + kFeedTrace, n);
+    feed.init(A.counts + kFeedTrace, n, A.refill);
     int64_t cur = -1;  // queue index of this lane's ray
     traceStream<false, COUNT, LDSD>(
         A.S, st, cnt,
@@ -693,7 +697,7 @@ __global__ __launch_bounds__(256) void kMedium(WaveArgs A) {
     const uint64_t waveMask = __ballot(1);
     const DScene& S = A.S;
     RayFeed feed;
-    feed.init(A.counts + kFeedMedium, n);
+    feed.init(A.counts + kFeedMedium, n, kWalkRefill);
     bool drained = false;
     int64_t cur = -1;
     int p = -1, medium = 0;
@@ -889,7 +893,7 @@ __global__ __launch_bounds__(256, GND_SHADOW_WAVES) void kShadowAny(WaveArgs A) 
     const ShadowSoA& Sh = A.Sh;
     const uint64_t waveMask = __ballot(1);
     RayFeed feed;
-    feed.init(A.counts + kFeedShadow, n);
+    feed.init(A.counts + kFeedShadow, n, A.srefill);
     int64_t cur = -1;
     traceStream<true, COUNT, LDSD>(
         S, st, cnt,
@@ -932,7 +936,7 @@ __global__ __launch_bounds__(256, GND_SHADOWTR_WAVES) void kShadow(WaveArgs A) {
     const ShadowSoA& Sh = A.Sh;
     const uint64_t waveMask = __ballot(1);
     RayFeed feed;
-    feed.init(A.counts + kFeedShadow, n);
+    feed.init(A.counts + kFeedShadow, n, kWalkRefill);
     bool drained = false;
     constexpr int kIdle = 0, kTrace = 1, kWalk = 2;
     constexpr int kVisible = 0, kOccluded = 1, kOnward = 2;
@@ -1168,6 +1172,7 @@ struct gnd_hip_scene {
     size_t bytes = 0;
     int hasIndex = 0, hasGrid = 0, nMedia = 0;
     uint32_t nPrims = 0;
+    size_t accelBytes = 0;  // TLAS, instances, BLAS and triangle packs
 };
 
 // Path state, hit and shadow queues and queue counters for `capacity` paths
@@ -1191,7 +1196,7 @@ struct gnd_hip_ctx {
     std::mutex mu;
     // 40M paths in flight (~10 GB of path state): a whole 1080p x 16 spp frame
     // per wave, so every bounce is one large launch
-    gnd_hip_tuning tuning{128, 16, 40000000, 0};
+    gnd_hip_tuning tuning{128, 16, 40000000, 0, 0, 0};
     bool counters = false;
     int numCUs = 0;
     size_t hbm = 0;
@@ -1486,6 +1491,10 @@ int gnd_hip_set_tuning(gnd_hip_ctx* ctx, const gnd_hip_tuning* t) {
         ctx->err = "queue_capacity too small";
         return GND_E_INVALID;
     }
+    if (t->refill_lanes < 0 || t->refill_lanes > 64 || t->shadow_refill_lanes < 0 || t->shadow_refill_lanes > 64) {
+        ctx->err = "refill lanes must be 0 (per scene) or 1..64";
+        return GND_E_INVALID;
+    }
     std::lock_guard<std::mutex> g(ctx->mu);
     ctx->tuning = *t;
     ctx->tuning.stack_lds = normLds(t->stack_lds);
@@ -1678,6 +1687,9 @@ int gnd_hip_scene_upload(gnd_hip_ctx* ctx, const gnd_hip_scene_arrays* a, gnd_hi
         v.inst = devAlloc<DInst>(s, inst.size(), inst.data());
         v.blas = devAlloc<DNode>(s, bl.size(), bl.data());
         v.packs = reinterpret_cast<const DPack*>(devAlloc<gnd_tri_pack4>(s, a->n_packs, a->packs));
+        s->accelBytes = tl.size() * sizeof(DNode) + (size_t)a->n_tlas_prims * sizeof(int32_t) +
+                        inst.size() * sizeof(DInst) + bl.size() * sizeof(DNode) +
+                        (size_t)a->n_packs * sizeof(gnd_tri_pack4);
         v.prims = devAlloc<DPrim>(s, prims.size(), prims.data());
         v.shading = devAlloc<float>(s, (size_t)a->n_tris * 24, a->shading);
         v.cdf = devAlloc<float>(s, a->n_cdf, a->cdf);
@@ -1821,6 +1833,7 @@ struct BouncePlan {
     WaveKernel trace = nullptr, shadow = nullptr;
     bool split = false, fastShadow = false;
     int B = 128;
+    int refill = 32, srefill = 40;
     size_t shm = 0;
     unsigned gridCap = 8;
     // blocks for n queue entries: one entry per thread up to the grid cap
@@ -1860,6 +1873,12 @@ static BouncePlan makeBouncePlan(const gnd_hip_ctx* ctx, const gnd_hip_scene* sc
 #endif
     // up to 64 waves per CU in flight: the traversal is latency bound
     K.gridCap = (unsigned)std::max(8, ctx->numCUs * GND_GRID_WAVES_PER_CU * 64 / K.B);
+    // refill thresholds: measured best on a 36-triangle Cornell box (56 / 64)
+    // and a 1M-triangle scene (32 / 40); a BVH within one XCD's 4 MiB L2 counts
+    // as small
+    const bool small = scene->accelBytes <= (size_t(4) << 20);
+    K.refill = ctx->tuning.refill_lanes > 0 ? ctx->tuning.refill_lanes : (small ? 56 : 32);
+    K.srefill = ctx->tuning.shadow_refill_lanes > 0 ? ctx->tuning.shadow_refill_lanes : (small ? 64 : 40);
     return K;
 }
 
@@ -2044,6 +2063,8 @@ int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jo
             A.ctr = ctx->dCounters;
             A.pathOnlyFast = K.fastShadow ? 1 : 0;
             A.split = K.split ? 1 : 0;
+            A.refill = K.refill;
+            A.srefill = K.srefill;
             bindQueues(A, ctx->wb, 0);
             size_t tc = tl.mark();
             A.camQueueBase = 0;
@@ -2135,7 +2156,9 @@ int gnd_hip_copy_probe(gnd_hip_ctx* ctx, size_t bytes, int repeats, double* ms_p
 int gnd_hip_autotune(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* job, gnd_hip_tuning* out) {
     if (!ctx || !scene || !job || !out || job->spp == 0) return GND_E_INVALID;
     // probe: one pass of the job (its film is overwritten), timed on the
-    // device for every block size x LDS stack depth; the fastest is kept
+    // device (best of two) for every candidate; coordinate search over block
+    // size x LDS stack depth, then the closest-hit and the any-hit refill
+    // thresholds, keeping the fastest setting of each in turn
     gnd_hip_job probe = *job;
     probe.spp = 1;
     probe.film_sq = nullptr;
@@ -2144,19 +2167,37 @@ int gnd_hip_autotune(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* 
     double bestMs = 1e300;
     gnd_hip_stats st;
     int rc = gnd_hip_render(ctx, scene, &probe, 1, &st);  // warm-up: buffers, code objects
-    for (int b : {64, 128, 256}) {
+    auto tryTuning = [&](const gnd_hip_tuning& t) {
+        if (rc != GND_OK) return;
+        rc = gnd_hip_set_tuning(ctx, &t);
+        double ms = 1e300;
+        for (int r = 0; r < 2 && rc == GND_OK; ++r) {
+            rc = gnd_hip_render(ctx, scene, &probe, 1, &st);
+            if (rc == GND_OK) ms = std::min(ms, st.ms_total);
+        }
+        if (rc == GND_OK && ms < bestMs) {
+            bestMs = ms;
+            best = t;
+        }
+    };
+    for (int b : {64, 128, 256})
         for (int l : {8, 16, 24, 32}) {
-            if (rc != GND_OK) break;
             gnd_hip_tuning t = start;
             t.block_size = b;
             t.stack_lds = l;
-            rc = gnd_hip_set_tuning(ctx, &t);
-            if (rc == GND_OK) rc = gnd_hip_render(ctx, scene, &probe, 1, &st);
-            if (rc == GND_OK && st.ms_total < bestMs) {
-                bestMs = st.ms_total;
-                best = t;
-            }
+            tryTuning(t);
         }
+    const gnd_hip_tuning shape = best;
+    for (int r : {24, 32, 40, 48, 56, 64}) {
+        gnd_hip_tuning t = shape;
+        t.refill_lanes = r;
+        tryTuning(t);
+    }
+    const gnd_hip_tuning closest = best;
+    for (int r : {24, 32, 40, 48, 56, 64}) {
+        gnd_hip_tuning t = closest;
+        t.shadow_refill_lanes = r;
+        tryTuning(t);
     }
     if (rc != GND_OK) {
         (void)gnd_hip_set_tuning(ctx, &start);
@@ -2375,6 +2416,8 @@ WaveArgs serverArgs(gnd_hip_server* s, const BouncePlan& K) {
     A.ctr = s->dCounters;
     A.pathOnlyFast = K.fastShadow ? 1 : 0;
     A.split = K.split ? 1 : 0;
+    A.refill = K.refill;
+    A.srefill = K.srefill;
     return A;
 }
 

@@ -132,6 +132,11 @@ typedef struct gnd_hip_tuning {
     int32_t stack_lds;        /* per-lane traversal stack entries held in LDS (8/16/24/32) */
     int64_t queue_capacity;   /* paths in flight per wave                            */
     int32_t use_graph;        /* capture the bounce loop in a hipGraph                */
+    int32_t refill_lanes;     /* idle lanes (1..64) before a closest-hit traversal wave
+                                 takes new rays; 0 = per scene (56 when the BVH fits in
+                                 4 MiB, else 32)                                        */
+    int32_t shadow_refill_lanes;  /* the same for any-hit shadow traversal; 0 = per
+                                 scene (64, else 40)                                    */
 } gnd_hip_tuning;
 
 /* ---- context ------------------------------------------------------------ */
@@ -185,10 +190,12 @@ int gnd_hip_cancel(gnd_hip_ctx* ctx);
  * Used to calibrate HBM counters (a known-size copy reads and writes bytes). */
 int gnd_hip_copy_probe(gnd_hip_ctx* ctx, size_t bytes, int repeats, double* ms_per_copy);
 
-/* Picks the block size (64/128/256) and LDS stack depth (8/16/24/32) by
- * timing one pass of `job` on `scene` with each pair; keeps the fastest as
- * the context's tuning and returns it.  job->film is overwritten.  No
- * counterpart in the reference (its CPU render has no launch shape). */
+/* Picks the block size (64/128/256) and LDS stack depth (8/16/24/32), then
+ * the closest-hit and the any-hit refill thresholds (24..64), by timing one
+ * pass of `job` on `scene` (best of two) per candidate, keeping the fastest of
+ * each in turn; sets the result as the context's tuning and returns it.
+ * job->film is overwritten.  No counterpart in the reference (its CPU render
+ * has no launch shape). */
 int gnd_hip_autotune(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* job, gnd_hip_tuning* out);
 
 /* ---- serving ---------------------------------------------------------------

@@ -196,15 +196,17 @@ def test_seed_offset_changes_samples(device):
 
 # ---------------------------------------------------------------- launch shape, autotune, cancel
 def test_tunings_give_identical_films(device):
-    """Block size, LDS stack depth and wave capacity change scheduling only."""
+    """Block size, LDS stack depth, wave capacity and refill thresholds change
+    scheduling only."""
     hs = HostScene.generate("scaled", "level=4;instances=12;width=64;height=40")
     ds = DeviceScene(device, hs)
     job = hs.default_job(spp=2)
     old = device.tuning()
     films = []
     try:
-        for b, l, q in [(64, 8, 1 << 20), (128, 16, 64 * 40), (256, 24, 1 << 20), (128, 32, 3000)]:
-            device.set_tuning(block_size=b, stack_lds=l, queue_capacity=q)
+        for b, l, q, r, sr in [(64, 8, 1 << 20, 0, 0), (128, 16, 64 * 40, 1, 64), (256, 24, 1 << 20, 64, 1),
+                               (128, 32, 3000, 40, 24)]:
+            device.set_tuning(block_size=b, stack_lds=l, queue_capacity=q, refill_lanes=r, shadow_refill_lanes=sr)
             films.append(device.render(ds, [job])[0][0])
     finally:
         device.set_tuning(**old)
@@ -241,6 +243,8 @@ def test_autotune_picks_valid_tuning(device):
     try:
         t = device.autotune(ds, hs.default_job(spp=4))
         assert t["block_size"] in (64, 128, 256) and t["stack_lds"] in (8, 16, 24, 32)
+        assert t["refill_lanes"] in (0, 24, 32, 40, 48, 56, 64)
+        assert t["shadow_refill_lanes"] in (0, 24, 32, 40, 48, 56, 64)
         a, _ = device.render(ds, [hs.default_job(spp=2)])
     finally:
         device.set_tuning(**old)
