@@ -87,12 +87,9 @@ GDEV_INLINE DNode loadNode(const DNode* nodes, int idx) {
 // lanes evaluated in order, committed with a strict t < lt.  Returns true on a
 // committed hit (for ANY: on the first valid lane).
 template <bool ANY>
-GDEV_INLINE bool testPack(const DPack* packs, uint32_t p, V3 o, V3 d, float& lt, float& hu, float& hv, int& hp) {
-    const float4* pk = reinterpret_cast<const float4*>(packs + p);
-    const float4 v0x = pk[0], v0y = pk[1], v0z = pk[2];
-    const float4 v1x = pk[3], v1y = pk[4], v1z = pk[5];
-    const float4 v2x = pk[6], v2y = pk[7], v2z = pk[8];
-    const uint4 prim = reinterpret_cast<const uint4*>(pk)[9];
+GDEV_INLINE bool testPackData(float4 v0x, float4 v0y, float4 v0z, float4 v1x, float4 v1y, float4 v1z, float4 v2x,
+                              float4 v2y, float4 v2z, uint4 prim, V3 o, V3 d, float& lt, float& hu, float& hv,
+                              int& hp) {
     bool hit = false;
 #pragma unroll
     for (int l = 0; l < 4; ++l) {
@@ -125,6 +122,13 @@ GDEV_INLINE bool testPack(const DPack* packs, uint32_t p, V3 o, V3 d, float& lt,
         }
     }
     return hit;
+}
+
+template <bool ANY>
+GDEV_INLINE bool testPack(const DPack* packs, uint32_t p, V3 o, V3 d, float& lt, float& hu, float& hv, int& hp) {
+    const float4* pk = reinterpret_cast<const float4*>(packs + p);
+    return testPackData<ANY>(pk[0], pk[1], pk[2], pk[3], pk[4], pk[5], pk[6], pk[7], pk[8],
+                             reinterpret_cast<const uint4*>(pk)[9], o, d, lt, hu, hv, hp);
 }
 
 // Sphere::rayIntersect with SolveQuadratic (object space, tMin = 1e-4)
