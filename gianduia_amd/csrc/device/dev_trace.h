@@ -213,14 +213,23 @@ struct RangeQueue {
 
 enum : int { ACT_NONE = 0, ACT_PACK = 1, ACT_NODE = 2, ACT_INST = 3 };
 
-// The step kind run by the wave this iteration: the kind most lanes wait for
-// (instance steps first on ties, then pack tests), so the expensive steps run
-// with most of the wave active.  ACT_NONE when no lane has work.
+// The step kind run by the wave this iteration: the kind most lanes wait for,
+// with lanes waiting for a pack test or an instance step counted twice (instance
+// steps first on ties, then pack tests), so the expensive steps run with most
+// of the wave active.  ACT_NONE when no lane has work.  Counting the rarer
+// kinds double runs them before too many lanes idle behind node pops: 1080p
+// scene traversal 42.6 -> 41.0 ms per frame; weights (2,5,5)/2 do no better.
+#ifndef GND_PACK_WEIGHT
+#define GND_PACK_WEIGHT 2
+#endif
+#ifndef GND_INST_WEIGHT
+#define GND_INST_WEIGHT 2
+#endif
 GDEV_INLINE int pickAction(int act) {
     const uint64_t mP = __ballot(act == ACT_PACK);
     const uint64_t mN = __ballot(act == ACT_NODE);
     const uint64_t mI = __ballot(act == ACT_INST);
-    const int nP = __popcll(mP), nN = __popcll(mN), nI = __popcll(mI);
+    const int nP = __popcll(mP) * GND_PACK_WEIGHT, nN = __popcll(mN), nI = __popcll(mI) * GND_INST_WEIGHT;
     if (nI > 0 && nI >= nN && nI >= nP) return ACT_INST;
     if (nP > 0 && nP >= nN) return ACT_PACK;
     if (nN > 0) return ACT_NODE;
