@@ -84,7 +84,7 @@ def cpu_baseline(hs, target_s=12.0):
     def run(p0, p1, r0, r1):
         stats = (C.c_double * 5)()
         rc = lib().gnd_ref_render(ref.h, threads, p0, p1, r0, r1, 0, rgbw.ctypes.data_as(C.POINTER(C.c_float)),
-                                  None, stats)
+                                  None, stats, None)
         if rc != 0:
             raise RuntimeError("oracle render failed")
         return stats[0], (r1 - r0) * W * (p1 - p0)

@@ -104,7 +104,7 @@ class gnd_hip_scene_arrays(C.Structure):
 class gnd_hip_job(C.Structure):
     _fields_ = [("camera", gnd_camera_desc), ("spp", C.c_uint32), ("max_depth", C.c_int32),
                 ("integrator", C.c_int32), ("seed_offset", C.c_uint64), ("film", C.c_void_p),
-                ("film_sq", C.c_void_p)]
+                ("film_sq", C.c_void_p), ("film_aov", C.c_void_p)]
 
 
 class gnd_hip_stats(C.Structure):
@@ -193,6 +193,8 @@ HOST_API = {
     "gnd_host_integrator_film": (C.c_int, [C.c_void_p, f32p, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                            C.POINTER(C.c_int)]),
     "gnd_host_integrator_save": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "gnd_host_integrator_set_aovs": (C.c_int, [C.c_void_p, C.c_int]),
+    "gnd_host_integrator_aovs": (C.c_int, [C.c_void_p, f32p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "gnd_host_integrator_free": (None, [C.c_void_p]),
     "gnd_host_integrator_names": (C.c_int, [C.c_char_p, C.c_int]),
     "gnd_host_sampler_floats": (C.c_int, [C.c_uint64, C.c_int, f32p]),

@@ -190,15 +190,19 @@ class Device:
         return self.download(occ, np.int32, n), st.as_dict()
 
     # ---- rendering
-    def render(self, scene: "DeviceScene", jobs: list["Job"], want_sq: bool = False):
-        """Renders the jobs in one call; returns (list of rgbw films as (H,W,4) arrays, stats)."""
+    def render(self, scene: "DeviceScene", jobs: list["Job"], want_sq: bool = False, want_aov: bool = False):
+        """Renders the jobs in one call; returns (films, stats).  Each film is
+        the (H,W,4) rgbw array, or a tuple of it with the (H,W,3) per-pass
+        squares (want_sq) and the (H,W,9) weighted AOV sums (want_aov)."""
         cj = (A.gnd_hip_job * len(jobs))()
         films = []
         for i, j in enumerate(jobs):
             w, h = j.camera.width, j.camera.height
             f = self.alloc(w * h * 16)
             sq = self.alloc(w * h * 12) if want_sq else None
-            films.append((f, sq, w, h))
+            av = self.alloc(w * h * 36) if want_aov else None
+            films.append((f, sq, av, w, h))
+            cj[i].film_aov = av.addr if av is not None else None
             cj[i].camera = j.camera
             cj[i].spp = j.spp
             cj[i].max_depth = j.max_depth
@@ -209,12 +213,13 @@ class Device:
         st = A.gnd_hip_stats()
         self._check(_lib().gnd_hip_render(self.ctx, scene.handle, cj, len(jobs), C.byref(st)))
         out = []
-        for f, sq, w, h in films:
-            rgbw = self.download(f, np.float32, w * h * 4).reshape(h, w, 4)
+        for f, sq, av, w, h in films:
+            item = [self.download(f, np.float32, w * h * 4).reshape(h, w, 4)]
             if want_sq:
-                out.append((rgbw, self.download(sq, np.float32, w * h * 3).reshape(h, w, 3)))
-            else:
-                out.append(rgbw)
+                item.append(self.download(sq, np.float32, w * h * 3).reshape(h, w, 3))
+            if want_aov:
+                item.append(self.download(av, np.float32, w * h * 9).reshape(h, w, 9))
+            out.append(item[0] if len(item) == 1 else tuple(item))
         return out, st.as_dict()
 
 
@@ -405,6 +410,21 @@ class Integrator:
         _lib().gnd_host_integrator_film(self.handle, out.ctypes.data_as(A.f32p), None, None, None)
         return out, n.value
 
+    def set_aovs(self, enabled: bool = True) -> None:
+        """Film AOVs (albedo, normal, depth, roughness, metallic) on the next render."""
+        _lib().gnd_host_integrator_set_aovs(self.handle, int(bool(enabled)))
+
+    def aovs(self) -> dict:
+        """The resolved AOV images of the last render (set_aovs first)."""
+        w, h = C.c_int(), C.c_int()
+        rc = _lib().gnd_host_integrator_aovs(self.handle, None, C.byref(w), C.byref(h))
+        if rc != 0:
+            raise GndError(rc, host_error())
+        out = np.zeros((h.value, w.value, 9), np.float32)
+        _lib().gnd_host_integrator_aovs(self.handle, out.ctypes.data_as(A.f32p), None, None)
+        return {"albedo": out[..., 0:3], "normal": out[..., 3:6], "depth": out[..., 6], "roughness": out[..., 7],
+                "metallic": out[..., 8]}
+
     def save(self, path: str) -> None:
         rc = _lib().gnd_host_integrator_save(self.handle, path.encode())
         if rc != 0:
@@ -445,6 +465,19 @@ def camera_lookat(cam: A.gnd_camera_desc, origin, target, up=(0.0, 1.0, 0.0)) ->
     if rc != 0:
         raise GndError(rc, host_error())
     return c
+
+
+def resolve_aovs(rgbw: np.ndarray, aov: np.ndarray) -> dict:
+    """Film::resolve for the AOV images (film.cpp:50-95): sums * (1/weight);
+    where the weight is 0: albedo and normal 0, depth -1, roughness and
+    metallic 0."""
+    w = rgbw[..., 3:4]
+    has = w > 0
+    inv = np.where(has, np.float32(1.0) / np.where(has, w, np.float32(1.0)), np.float32(0.0))
+    v = (aov * inv).astype(np.float32)
+    depth = np.where(has[..., 0], v[..., 6], np.float32(-1.0)).astype(np.float32)
+    return {"albedo": v[..., 0:3], "normal": v[..., 3:6], "depth": depth, "roughness": v[..., 7],
+            "metallic": v[..., 8]}
 
 
 def resolve(rgbw: np.ndarray) -> np.ndarray:

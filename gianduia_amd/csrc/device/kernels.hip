@@ -48,7 +48,15 @@ struct PathSoA {
     // event record between the NEE and scatter kernels (split mode)
     int32_t *evKind, *evPrim, *evMedium;
     float *evpx, *evpy, *evpz, *evnx, *evny, *evnz, *evtx, *evty, *evtz, *evu, *evv;
+    // AOV record per slot (allocated when a job asks for AOVs, else null):
+    // field k of slot p at aov[k * aovStride + p], fields as in kAovFloats
+    float* aov;
+    uint32_t aovStride;
 };
+
+// AOVRecord (include/gianduia/core/film.h:13-19): albedo rgb, shading normal
+// xyz mapped to [0,1], depth along the camera forward axis, roughness, metallic
+constexpr int kAovFloats = 9;
 
 struct HitSoA {
     float *t, *u, *v;
@@ -66,6 +74,7 @@ struct DJob {
     DCamera cam;
     float* film;
     float* filmSq;
+    float* filmAov;  // weighted AOV sums, kAovFloats per pixel, or null
     uint64_t seedOffset;
     int32_t maxDepth, integrator, W, H;
 };
@@ -85,6 +94,7 @@ struct DSplatTask { // one job inside a wave: its chunks in pass order
 
 // flags layout
 constexpr uint32_t F_SPECULAR = 1u;
+constexpr uint32_t F_GBUF = 1u << 24;  // the path still needs its AOV albedo and normal
 GDEV_INLINE int flagChannel(uint32_t f) { return (int)((f >> 1) & 3u) - 1; }  // -1 none, 0..2
 GDEV_INLINE int flagMedium(uint32_t f) { return (int)((f >> 8) & 0xFFFFu) - 1; }
 GDEV_INLINE uint32_t setMedium(uint32_t f, int m) { return (f & ~(0xFFFFu << 8)) | ((uint32_t)(m + 1) << 8); }
@@ -341,7 +351,13 @@ __global__ void kCamera(WaveArgs A) {
     P.rng[slot] = rng.s;
     P.c0[slot] = 0u;
     P.c1[slot] = 0u;
-    P.flags[slot] = F_SPECULAR | ((uint32_t)(chan + 1) << 1);  // medium -1 -> field 0
+    const bool aov = J.filmAov && P.aov;
+    P.flags[slot] = F_SPECULAR | ((uint32_t)(chan + 1) << 1) | (aov ? F_GBUF : 0u);  // medium -1 -> field 0
+    if (aov) {
+        const float init[kAovFloats] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, -1.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int k = 0; k < kAovFloats; ++k) P.aov[(size_t)k * P.aovStride + slot] = init[k];
+    }
     P.job[slot] = ch.job;
     P.chunk[slot] = ch.id;
     // the host laid out the in-queue: no atomics, and queue order = slot order
@@ -575,14 +591,61 @@ GDEV_INLINE bool scatterStep(const DScene& S, const DJob& J, PathLocal& s, const
     return (int)bounces < J.maxDepth;
 }
 
+// The integrators' G-buffer writes (path.cpp:35-61 and 99-106, volPath.cpp:
+// 117-146 and 184-191) into slot p's AOV record.  A miss: depth / roughness /
+// metallic reset at bounce 0; albedo (the environment, none here) and normal 0.
+GDEV_INLINE void aovPut(const PathSoA& P, int p, int k, float v) { P.aov[(size_t)k * P.aovStride + p] = v; }
+GDEV_INLINE void aovMiss(const PathSoA& P, int p, PathLocal& s, uint32_t bounces) {
+    if (bounces == 0) {
+        aovPut(P, p, 6, -1.0f);
+        aovPut(P, p, 7, 0.0f);
+        aovPut(P, p, 8, 0.0f);
+    }
+    if (s.flags & F_GBUF) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) aovPut(P, p, k, 0.0f);
+        s.flags &= ~F_GBUF;
+    }
+}
+// A surface with a BSDF: depth along Camera::getForward (camera.h:40-43),
+// roughness and metallic (Material::getRoughness / getMetallic: conductor and
+// glass roughness, conductor metallic 1) at bounce 0; albedo
+// (Material::getAlbedo: matte albedo texture, conductor R) and the shading
+// normal mapped to [0,1] at the first surface with a non-specular lobe, or at
+// the last bounce.
+GDEV_INLINE bool lobeNonSpecular(int kind) {
+    return kind == L_LAMBERT || kind == L_MF_REFL || kind == L_MF_FRESNEL || kind == L_ROUGH_PLASTIC;
+}
+GDEV_INLINE void aovSurface(const WaveArgs& A, const DJob& J, int p, PathLocal& s, const Isect& it, const Bsdf& b,
+                            uint32_t bounces) {
+    const DMaterial& m = A.S.mats[A.S.prims[it.prim].material];
+    if (bounces == 0) {
+        const V3 fwd = xfVector(J.cam.toWorld, V3{0.0f, 0.0f, -1.0f});
+        aovPut(A.P, p, 6, dot(it.p - s.last, fwd));
+        aovPut(A.P, p, 7, (m.type == GND_MAT_CONDUCTOR || m.type == GND_MAT_GLASS) ? m.roughness : 0.0f);
+        aovPut(A.P, p, 8, m.type == GND_MAT_CONDUCTOR ? 1.0f : 0.0f);
+    }
+    if ((s.flags & F_GBUF) && (lobeNonSpecular(b.lobe.kind) || (int)bounces == J.maxDepth - 1)) {
+        const V3 alb = m.type == GND_MAT_MATTE ? b.lobe.albedo : (m.type == GND_MAT_CONDUCTOR ? c3(m.R) : sameV(0.0f));
+        const V3 nn = V3{it.n.x / 2.0f, it.n.y / 2.0f, it.n.z / 2.0f} + sameV(0.5f);
+        aovPut(A.P, p, 0, alb.x);
+        aovPut(A.P, p, 1, alb.y);
+        aovPut(A.P, p, 2, alb.z);
+        aovPut(A.P, p, 3, nn.x);
+        aovPut(A.P, p, 4, nn.y);
+        aovPut(A.P, p, 5, nn.z);
+        s.flags &= ~F_GBUF;
+    }
+}
+
 // First half of a bounce: medium sampling (volpath), emission with MIS, null
 // BSDF pass-through and the NEE light sample with its shadow ray.
 // Returns: 0 = dead, 1 = continues straight (already a new ray), 2 = needs scatter.
 // pre: the medium was a density grid that kMedium already walked (throughput
 // and sampler state updated); preHit / preP = its collision, if any.
-GDEV_INLINE int neeStep(const WaveArgs& A, const DJob& J, PathLocal& s, const HitRec& h, Event& ev, bool& shadow,
-                        V3& so, V3& sd, float& stmax, int& smedium, V3& pending, V3& lightP, bool pre, bool preHit,
-                        V3 preP) {
+GDEV_INLINE int neeStep(const WaveArgs& A, const DJob& J, int p, PathLocal& s, const HitRec& h, Event& ev,
+                        bool& shadow, V3& so, V3& sd, float& stmax, int& smedium, V3& pending, V3& lightP, bool pre,
+                        bool preHit, V3 preP) {
     const DScene& S = A.S;
     const bool vol = J.integrator == GND_INTEGRATOR_VOLPATH;
     const bool hit = h.inst >= 0;
@@ -621,7 +684,11 @@ GDEV_INLINE int neeStep(const WaveArgs& A, const DJob& J, PathLocal& s, const Hi
             return 2;
         }
     }
-    if (!hit) return 0;
+    const bool aovOn = J.filmAov && A.P.aov;
+    if (!hit) {
+        if (aovOn) aovMiss(A.P, p, s, bounces);
+        return 0;
+    }
     Isect it;
     fillInteraction(S, s.o, s.d, h, it);
     const DPrim& pr = S.prims[it.prim];
@@ -650,6 +717,7 @@ GDEV_INLINE int neeStep(const WaveArgs& A, const DJob& J, PathLocal& s, const Hi
         if (vol) s.flags = setMedium(s.flags, dot(s.d, it.n) > 0.0f ? outside : inside);
         return 1;
     }
+    if (aovOn) aovSurface(A, J, p, s, it, b, bounces);
     const float lsp = 1.0f / (float)(uint32_t)S.nEmitters;
     const int ep = pickEmitter(S, s.rng.next1D());
     const float ux = s.rng.next1D(), uy = s.rng.next1D();
@@ -787,7 +855,7 @@ __global__ __launch_bounds__(kShadeBlock, GND_SHADE_WAVES) void kShade(WaveArgs 
                 preHit = A.P.evKind[p] == EV_MEDIUM;
                 if (preHit) preP = V3{A.P.evpx[p], A.P.evpy[p], A.P.evpz[p]};
             }
-            int r = neeStep(A, J, s, h, ev, shadow, so, sd, stmax, smedium, pending, lightP, pre, preHit, preP);
+            int r = neeStep(A, J, p, s, h, ev, shadow, so, sd, stmax, smedium, pending, lightP, pre, preHit, preP);
             if (!A.pathOnlyFast && shadow) {
                 // the general shadow kernel always needs the ray; the any-hit
                 // path skips zero contributions
@@ -1078,10 +1146,17 @@ __global__ void kSplat(WaveArgs A, const DSplatTask* tasks, int nTasks, uint32_t
     const int reach = (int)ceilf(stdMax(rx, ry)) + 1;
     float4 acc = reinterpret_cast<float4*>(J.film)[pix];
     float3 accSq = J.filmSq ? reinterpret_cast<float3*>(J.filmSq)[pix] : float3{0, 0, 0};
+    const bool aov = J.filmAov && A.P.aov;
+    float accA[kAovFloats];
+#pragma unroll
+    for (int k = 0; k < kAovFloats; ++k) accA[k] = aov ? J.filmAov[(size_t)pix * kAovFloats + k] : 0.0f;
     unsigned long long nan = 0;
     for (int ci = tk.chunkBegin; ci < tk.chunkEnd; ++ci) {
         const DChunk ch = A.chunks[ci];
         float sr = 0.0f, sg = 0.0f, sb = 0.0f, sw = 0.0f;
+        float sa[kAovFloats];
+#pragma unroll
+        for (int k = 0; k < kAovFloats; ++k) sa[k] = 0.0f;
         for (int dy = -reach; dy <= reach; ++dy) {
             const int qy = y + dy;
             if (qy < 0 || qy >= H) continue;
@@ -1111,12 +1186,20 @@ __global__ void kSplat(WaveArgs A, const DSplatTask* tasks, int nTasks, uint32_t
                 sg += L.y * w;
                 sb += L.z * w;
                 sw += w;
+                if (aov) {
+#pragma unroll
+                    for (int k = 0; k < kAovFloats; ++k) sa[k] += A.P.aov[(size_t)k * A.P.aovStride + q] * w;
+                }
             }
         }
         acc.x += sr;
         acc.y += sg;
         acc.z += sb;
         acc.w += sw;
+        if (aov) {
+#pragma unroll
+            for (int k = 0; k < kAovFloats; ++k) accA[k] += sa[k];
+        }
         if (J.filmSq && sw > 0.0f) {
             float vr = sr / sw, vg = sg / sw, vb = sb / sw;
             accSq.x += vr * vr;
@@ -1126,6 +1209,10 @@ __global__ void kSplat(WaveArgs A, const DSplatTask* tasks, int nTasks, uint32_t
     }
     reinterpret_cast<float4*>(J.film)[pix] = acc;
     if (J.filmSq) reinterpret_cast<float3*>(J.filmSq)[pix] = accSq;
+    if (aov) {
+#pragma unroll
+        for (int k = 0; k < kAovFloats; ++k) J.filmAov[(size_t)pix * kAovFloats + k] = accA[k];
+    }
     if (nan) atomicAdd(&A.ctr->nan, nan);
 }
 
@@ -1180,6 +1267,7 @@ struct gnd_hip_scene {
 struct WaveBuffers {
     size_t capacity = 0;
     void* pathBuf = nullptr;
+    void* aovBuf = nullptr;  // per-slot AOV records, allocated on first use
     PathSoA P{};
     HitSoA Hq{};
     ShadowSoA Sh{};
@@ -1323,6 +1411,7 @@ size_t waveBufferBytes(size_t cap) {
 
 void freeWaveBuffers(WaveBuffers& wb) {
     if (wb.pathBuf) (void)hipFree(wb.pathBuf);
+    if (wb.aovBuf) (void)hipFree(wb.aovBuf);
     if (wb.counts) (void)hipFree(wb.counts);
     wb = WaveBuffers{};
 }
@@ -1332,6 +1421,10 @@ void ensureWaveBuffers(WaveBuffers& wb, size_t cap) {
     if (cap <= wb.capacity) return;
     if (wb.pathBuf) GND_CHECK(hipFree(wb.pathBuf));
     wb.pathBuf = nullptr;
+    if (wb.aovBuf) GND_CHECK(hipFree(wb.aovBuf));
+    wb.aovBuf = nullptr;
+    wb.P.aov = nullptr;
+    wb.P.aovStride = 0;
     wb.capacity = 0;
     // 68 four-byte fields and one eight-byte field per slot
     const size_t bytes = waveBufferBytes(cap);
@@ -1375,6 +1468,14 @@ void ensureWaveBuffers(WaveBuffers& wb, size_t cap) {
         for (float** pp : {&r.ox, &r.oy, &r.oz, &r.dx, &r.dy, &r.dz}) *pp = (float*)take(4);
     if (off > bytes) throw std::runtime_error("wave buffer layout overflow");
     wb.capacity = cap;
+}
+
+// AOV records for every slot of the wave buffers (36 bytes per slot)
+void ensureAovBuffers(WaveBuffers& wb) {
+    if (wb.aovBuf || wb.capacity == 0) return;
+    GND_CHECK(hipMalloc(&wb.aovBuf, wb.capacity * kAovFloats * sizeof(float)));
+    wb.P.aov = static_cast<float*>(wb.aovBuf);
+    wb.P.aovStride = (uint32_t)wb.capacity;
 }
 
 using TraceKernel = void (*)(DScene, gnd_ray_soa, int64_t, gnd_hit_soa, int32_t*, Counters*);
@@ -1421,6 +1522,10 @@ float elapsedMs(hipEvent_t a, hipEvent_t b) {
 
 // ============================================================================ C-ABI
 
+// Defined with the host scene shell: frees the device copies that host scenes
+// made on `ctx` (called at shutdown while the context is still valid).
+void gndReleaseContextUploads(gnd_hip_ctx* ctx);
+
 extern "C" {
 
 const char* gnd_hip_last_error(const gnd_hip_ctx* ctx) { return ctx ? ctx->err.c_str() : g_lastError.c_str(); }
@@ -1457,6 +1562,7 @@ int gnd_hip_init(int device, gnd_hip_ctx** out) {
 
 void gnd_hip_shutdown(gnd_hip_ctx* ctx) {
     if (!ctx) return;
+    gndReleaseContextUploads(ctx);
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     freeWaveBuffers(ctx->wb);
@@ -1969,6 +2075,7 @@ static DJob makeDJob(const gnd_hip_job& j) {
     J.cam = makeCamera(j.camera);
     J.film = j.film;
     J.filmSq = j.film_sq;
+    J.filmAov = j.film_aov;
     J.seedOffset = j.seed_offset;
     J.maxDepth = j.max_depth;
     J.integrator = j.integrator;
@@ -1984,6 +2091,9 @@ static void clearFilm(gnd_hip_ctx* ctx, const DJob& J) {
     if (J.filmSq)
         hipLaunchKernelGGL(kFill, dim3((unsigned)((np * 3 + 255) / 256)), dim3(256), 0, ctx->stream, J.filmSq,
                            np * 3, 0.0f);
+    if (J.filmAov)
+        hipLaunchKernelGGL(kFill, dim3((unsigned)((np * kAovFloats + 255) / 256)), dim3(256), 0, ctx->stream,
+                           J.filmAov, np * kAovFloats, 0.0f);
 }
 
 static void fillStats(gnd_hip_stats* stats, const Counters& c, uint64_t samples) {
@@ -2025,6 +2135,8 @@ int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jo
         const size_t cap = std::max<size_t>((size_t)ctx->tuning.queue_capacity, maxPix);
         if (cap > 0x7FFFFFFFull) throw std::runtime_error("wave capacity too large");
         ensureWaveBuffers(ctx->wb, cap);
+        for (int j = 0; j < njobs; ++j)
+            if (jobs[j].film_aov) ensureAovBuffers(ctx->wb);
         std::vector<DJob> dj(jobs.size());
         for (size_t j = 0; j < jobs.size(); ++j) {
             dj[j] = makeDJob(jobs[j]);
@@ -2676,6 +2788,16 @@ int gnd_hip_server_submit(gnd_hip_server* s, int32_t scene_id, const gnd_hip_job
     if ((uint64_t)job->camera.width * (uint64_t)job->camera.height > s->cfg.path_slots) {
         s->ctx->err = "one pass of the job needs more path slots than the server has";
         return GND_E_NOMEM;
+    }
+    if (job->film_aov && !s->wb.aovBuf) {
+        // AOV records for the slot pool (the worker is not inside a step while
+        // the server lock is held)
+        std::lock_guard<std::mutex> g(s->ctx->mu);
+        const int rc = guarded(s->ctx, [&] {
+            GND_CHECK(hipSetDevice(s->ctx->device));
+            ensureAovBuffers(s->wb);
+        });
+        if (rc != GND_OK) return rc;
     }
     const uint64_t t = s->nextTicket++;
     SrvJob& j = s->jobs[t];

@@ -3,6 +3,7 @@
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -26,8 +27,16 @@ struct gnd_host_integrator {
 namespace {
 thread_local std::string g_hostErr;
 
+// Host scenes holding device copies: a context being shut down frees the
+// copies made on it first (gndReleaseContextUploads), so a host scene freed
+// later never touches a dead context.  Lock order: g_uploadMu, then a
+// scene's mu.
+std::mutex g_uploadMu;
+std::set<gnd_host_scene*> g_withUploads;
+
 // The scene's device copy on `ctx` (built and uploaded on first use).
 gnd_hip_scene* deviceScene(gnd_hip_ctx* ctx, gnd_host_scene* s) {
+    std::lock_guard<std::mutex> r(g_uploadMu);
     std::lock_guard<std::mutex> g(s->mu);
     for (auto& u : s->uploads)
         if (u.first == ctx) return u.second;
@@ -35,6 +44,7 @@ gnd_hip_scene* deviceScene(gnd_hip_ctx* ctx, gnd_host_scene* s) {
     gnd_hip_scene* dev = nullptr;
     if (gnd_hip_scene_upload(ctx, &arr, &dev) != GND_OK) throw std::runtime_error(gnd_hip_last_error(ctx));
     s->uploads.push_back({ctx, dev});
+    g_withUploads.insert(s);
     return dev;
 }
 
@@ -86,7 +96,13 @@ int gnd_host_scene_generate(const char* name, const char* params, gnd_host_scene
 
 void gnd_host_scene_free(gnd_host_scene* s) {
     if (!s) return;
-    for (auto& u : s->uploads) gnd_hip_scene_free(u.first, u.second);
+    {
+        std::lock_guard<std::mutex> r(g_uploadMu);
+        g_withUploads.erase(s);
+        std::lock_guard<std::mutex> g(s->mu);
+        for (auto& u : s->uploads) gnd_hip_scene_free(u.first, u.second);
+        s->uploads.clear();
+    }
     delete s;
 }
 
@@ -220,6 +236,25 @@ int gnd_host_integrator_film(gnd_host_integrator* it, float* rgb, int* width, in
     return GND_OK;
 }
 
+int gnd_host_integrator_set_aovs(gnd_host_integrator* it, int enabled) {
+    if (!it) return GND_E_INVALID;
+    it->impl->setAovs(enabled != 0);
+    return GND_OK;
+}
+
+int gnd_host_integrator_aovs(gnd_host_integrator* it, float* out, int* width, int* height) {
+    if (!it) return GND_E_INVALID;
+    const gnd::Film& f = it->impl->film();
+    if (width) *width = f.width();
+    if (height) *height = f.height();
+    if (!f.hasAovs()) {
+        g_hostErr = "the film has no AOVs (enable them before render)";
+        return GND_E_INVALID;
+    }
+    if (out) std::memcpy(out, f.aovs().data(), f.aovs().size() * sizeof(float));
+    return GND_OK;
+}
+
 int gnd_host_integrator_save(gnd_host_integrator* it, const char* path) {
     if (!it || !path) return GND_E_INVALID;
     return hostGuard([&] {
@@ -275,3 +310,22 @@ int gnd_host_sampler_floats(uint64_t sample_index, int n, float* out) {
 }
 
 }  // extern "C"
+
+// Called by gnd_hip_shutdown while `ctx` is still valid: frees the device
+// copies host scenes made on it and forgets them.
+void gndReleaseContextUploads(gnd_hip_ctx* ctx) {
+    std::lock_guard<std::mutex> r(g_uploadMu);
+    for (auto it = g_withUploads.begin(); it != g_withUploads.end();) {
+        gnd_host_scene* s = *it;
+        std::lock_guard<std::mutex> g(s->mu);
+        for (auto u = s->uploads.begin(); u != s->uploads.end();) {
+            if (u->first == ctx) {
+                gnd_hip_scene_free(ctx, u->second);
+                u = s->uploads.erase(u);
+            } else {
+                ++u;
+            }
+        }
+        it = s->uploads.empty() ? g_withUploads.erase(it) : std::next(it);
+    }
+}

@@ -52,29 +52,42 @@ std::pair<float, float> IndependentSampler::next2D() {
 }
 
 // ---------------------------------------------------------------- film
-void Film::reset(int w, int h) {
+void Film::reset(int w, int h, bool aovs) {
     w_ = w;
     h_ = h;
     acc_.assign((size_t)w * h * 4, 0.0);
     rgb_.assign((size_t)w * h * 3, 0.0f);
+    aovAcc_.assign(aovs ? (size_t)w * h * kAovs : 0, 0.0);
+    aov_.assign(aovs ? (size_t)w * h * kAovs : 0, 0.0f);
 }
 
 void Film::clear() {
     std::fill(acc_.begin(), acc_.end(), 0.0);
     std::fill(rgb_.begin(), rgb_.end(), 0.0f);
+    std::fill(aovAcc_.begin(), aovAcc_.end(), 0.0);
+    std::fill(aov_.begin(), aov_.end(), 0.0f);
 }
 
 void Film::accumulate(const float* rgbw) {
     for (size_t i = 0; i < acc_.size(); ++i) acc_[i] += rgbw[i];
 }
 
-// Film::resolve (src/core/film.cpp:50-94): rgb = sum * (1 / weight)
+void Film::accumulateAovs(const float* sums) {
+    for (size_t i = 0; i < aovAcc_.size(); ++i) aovAcc_[i] += sums[i];
+}
+
+// Film::resolve (src/core/film.cpp:50-94): value = sum * (1 / weight); where
+// the weight is 0, black and the AOV defaults (depth -1)
 void Film::resolve() {
     const size_t n = (size_t)w_ * h_;
     for (size_t i = 0; i < n; ++i) {
         const double wsum = acc_[4 * i + 3];
         const float inv = wsum > 0.0 ? 1.0f / (float)wsum : 0.0f;
         for (int c = 0; c < 3; ++c) rgb_[3 * i + c] = (float)acc_[4 * i + c] * inv;
+        if (!aovAcc_.empty()) {
+            for (int k = 0; k < kAovs; ++k) aov_[kAovs * i + k] = (float)aovAcc_[kAovs * i + k] * inv;
+            if (!(wsum > 0.0)) aov_[kAovs * i + 6] = -1.0f;
+        }
     }
 }
 
@@ -168,11 +181,16 @@ void HipIntegrator::render(const RenderScene& scene) {
     const gnd_scene_desc& d = *scene.desc;
     const int W = d.camera.width, H = d.camera.height;
     const uint32_t spp = d.spp;
-    film_.reset(W, H);
+    film_.reset(W, H, aovs_);
     float* dev = nullptr;
+    float* devAov = nullptr;
     check(gnd_hip_malloc(ctx_, (size_t)W * H * 4 * sizeof(float), (void**)&dev), ctx_);
-    std::vector<float> acc((size_t)W * H * 4);
+    std::vector<float> acc((size_t)W * H * 4), aovSums;
     try {
+        if (aovs_) {
+            check(gnd_hip_malloc(ctx_, (size_t)W * H * Film::kAovs * sizeof(float), (void**)&devAov), ctx_);
+            aovSums.resize((size_t)W * H * Film::kAovs);
+        }
         for (uint32_t s = 0; s < spp && !stop_; s += (uint32_t)batch_) {
             gnd_hip_job job{};
             job.camera = d.camera;
@@ -181,20 +199,27 @@ void HipIntegrator::render(const RenderScene& scene) {
             job.integrator = integrator_;
             job.seed_offset = (uint64_t)s * (uint64_t)W * (uint64_t)H;
             job.film = dev;
+            job.film_aov = devAov;
             const int rc = gnd_hip_render(ctx_, scene.device, &job, 1, nullptr);
             if (rc == GND_E_CANCELLED) break;
             check(rc, ctx_);
             check(gnd_hip_memcpy_d2h(ctx_, acc.data(), dev, acc.size() * sizeof(float)), ctx_);
             film_.accumulate(acc.data());
+            if (devAov) {
+                check(gnd_hip_memcpy_d2h(ctx_, aovSums.data(), devAov, aovSums.size() * sizeof(float)), ctx_);
+                film_.accumulateAovs(aovSums.data());
+            }
             film_.resolve();
             passes_ += (int)job.spp;
             notifyUpdate((int)(s + job.spp - 1));  // integrator.h:147
         }
     } catch (...) {
         gnd_hip_free(ctx_, dev);
+        if (devAov) gnd_hip_free(ctx_, devAov);
         throw;
     }
     gnd_hip_free(ctx_, dev);
+    if (devAov) gnd_hip_free(ctx_, devAov);
 }
 
 void HipIntegrator::cancel() {

@@ -56,19 +56,26 @@ private:
 };
 
 // Host film: weighted RGB sums and weights per pixel (what the device
-// gathers per pass), resolved to RGB = sum / weight.
+// gathers per pass), resolved to RGB = sum / weight; optionally the AOV sums
+// (AOVRecord, film.h:13-19), resolved the same way (film.cpp:50-95).
 class Film {
 public:
+    static constexpr int kAovs = 9;  // albedo rgb, normal xyz, depth, roughness, metallic
     Film() = default;
     Film(int w, int h) { reset(w, h); }
-    void reset(int w, int h);
+    void reset(int w, int h, bool aovs = false);
     void clear();
     // adds device-format accumulators (w*h*4 floats: r, g, b, weight)
     void accumulate(const float* rgbw);
+    // adds device-format AOV sums (w*h*9 floats)
+    void accumulateAovs(const float* sums);
     void resolve();
     int width() const { return w_; }
     int height() const { return h_; }
-    const std::vector<float>& rgb() const { return rgb_; }  // resolved, w*h*3
+    bool hasAovs() const { return !aovAcc_.empty(); }
+    const std::vector<float>& rgb() const { return rgb_; }   // resolved, w*h*3
+    const std::vector<float>& aovs() const { return aov_; }  // resolved, w*h*9 (where the
+                                                              // weight is 0: 0,0,0, 0,0,0, -1, 0, 0)
     void savePFM(const std::string& path) const;
     void savePNG(const std::string& path) const;  // 8-bit sRGB, values clamped to [0, 1]
 
@@ -76,6 +83,8 @@ private:
     int w_ = 0, h_ = 0;
     std::vector<double> acc_;  // r, g, b, weight per pixel
     std::vector<float> rgb_;
+    std::vector<double> aovAcc_;
+    std::vector<float> aov_;
 };
 
 // What an integrator renders: the scene description and its device copy.
@@ -92,6 +101,8 @@ public:
     virtual void cancel() { stop_ = true; }
     virtual void preprocess(const RenderScene&) {}
     void setCallback(RenderCallback cb) { callback_ = std::move(cb); }
+    // also fill the film's AOV images (Integrator::Li's AOVRecord argument)
+    void setAovs(bool on) { aovs_ = on; }
     Film& film() { return film_; }
     int passesDone() const { return passes_; }
 
@@ -103,6 +114,7 @@ protected:
     RenderCallback callback_;
     Film film_;
     int passes_ = 0;
+    bool aovs_ = false;
 };
 
 // path / volpath on the GPU: passes are rendered in batches of

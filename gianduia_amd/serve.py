@@ -25,6 +25,7 @@ class Ticket:
     job: Job
     film: DeviceBuffer
     t_submit: float
+    aov: DeviceBuffer | None = None
 
 
 class Server:
@@ -45,9 +46,10 @@ class Server:
         self.scenes.append(hs)
         return sid.value
 
-    def submit(self, scene_id: int, job: Job) -> Ticket:
+    def submit(self, scene_id: int, job: Job, want_aov: bool = False) -> Ticket:
         w, h = job.camera.width, job.camera.height
         film = self.dev.alloc(w * h * 16)
+        aov = self.dev.alloc(w * h * 36) if want_aov else None
         cj = A.gnd_hip_job()
         cj.camera = job.camera
         cj.spp = job.spp
@@ -56,9 +58,10 @@ class Server:
         cj.seed_offset = job.seed_offset
         cj.film = film.addr
         cj.film_sq = None
+        cj.film_aov = aov.addr if aov is not None else None
         t = C.c_uint64()
         self.dev._check(_lib().gnd_hip_server_submit(self.handle, scene_id, C.byref(cj), C.byref(t)))
-        return Ticket(t.value, job, film, time.perf_counter())
+        return Ticket(t.value, job, film, time.perf_counter(), aov)
 
     def poll(self, t: Ticket) -> dict:
         st = A.gnd_hip_job_status()
@@ -76,6 +79,11 @@ class Server:
     def film(self, t: Ticket) -> np.ndarray:
         w, h = t.job.camera.width, t.job.camera.height
         return self.dev.download(t.film, np.float32, w * h * 4).reshape(h, w, 4)
+
+    def aov(self, t: Ticket) -> np.ndarray:
+        """The job's (H, W, 9) weighted AOV sums (submitted with want_aov)."""
+        w, h = t.job.camera.width, t.job.camera.height
+        return self.dev.download(t.aov, np.float32, w * h * 9).reshape(h, w, 9)
 
     def stats(self) -> dict:
         st = A.gnd_hip_stats()

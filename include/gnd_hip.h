@@ -102,6 +102,11 @@ typedef struct gnd_hip_job {
     uint64_t seed_offset;   /* added to pixelIdx + pass*W*H before SplitMix64 */
     float* film;            /* device pointer                                 */
     float* film_sq;         /* device pointer or NULL                         */
+    float* film_aov;        /* device pointer or NULL: per pixel the filter-weighted
+                               sums of the AOVRecord (film.h:13-19) of each sample:
+                               albedo rgb, normal xyz (mapped to [0,1]), depth,
+                               roughness, metallic (9 floats); divide by the film's
+                               weight to resolve (film.cpp:50-95)                 */
 } gnd_hip_job;
 
 typedef struct gnd_hip_stats {

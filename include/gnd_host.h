@@ -76,6 +76,13 @@ int gnd_host_integrator_render(gnd_host_integrator* it, gnd_host_scene* s);
 int gnd_host_integrator_cancel(gnd_host_integrator* it);
 /* Resolved film (rgb may be NULL to query the size) and the passes rendered. */
 int gnd_host_integrator_film(gnd_host_integrator* it, float* rgb, int* width, int* height, int* passes);
+/* Film AOVs (include/gianduia/core/film.h:13-107): set_aovs before render()
+ * makes the film also hold the resolved AOV images; aovs() copies them as
+ * width*height*9 floats per pixel: albedo rgb, normal xyz (mapped to [0,1]),
+ * depth, roughness, metallic (where no sample landed: 0, 0, -1, 0, 0).
+ * GND_E_INVALID if the last render had no AOVs. */
+int gnd_host_integrator_set_aovs(gnd_host_integrator* it, int enabled);
+int gnd_host_integrator_aovs(gnd_host_integrator* it, float* out, int* width, int* height);
 /* Film::savePNG / an HDR float image (integrator.h:152-153): ".png" (8-bit
  * sRGB) or ".pfm" (32-bit float RGB). */
 int gnd_host_integrator_save(gnd_host_integrator* it, const char* path);

@@ -350,11 +350,20 @@ struct RenderOptions {
     int sppBegin = 0, sppEnd = -1;   // pass range (default all)
     int rowBegin = 0, rowEnd = -1;   // row range (default all)
     bool accumulateSquares = false;
+    bool aovs = false;               // also splat the per-sample AOV records
 };
+// AOVRecord (include/gianduia/core/film.h:13-19)
+struct AOV {
+    C3 albedo{0.0f};
+    V3 normal{0.0f, 0.0f, 0.0f};
+    float depth = -1.0f, roughness = 0.0f, metallic = 0.0f;
+};
+constexpr int kAovFloats = 9;  // albedo rgb, normal xyz, depth, roughness, metallic
 struct Film {
     int width = 0, height = 0;
     std::vector<float> rgbw;     // 4 floats per pixel: r, g, b, weight
     std::vector<float> sq;       // optional per-pixel sum of squares (r,g,b)
+    std::vector<float> aov;      // optional weighted AOV sums, kAovFloats per pixel
 };
 void render(const Scene& s, const RenderOptions& opt, Film& film, Counters* c);
 

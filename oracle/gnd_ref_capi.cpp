@@ -121,11 +121,13 @@ int gnd_ref_trace(void* h, const gnd_ray_soa* rays, int64_t n, const gnd_hit_soa
 }
 
 // Progressive render of passes [spp_begin, spp_end) into rgbw (4 floats per
-// pixel: weighted r,g,b sums and weight sum) and optional sq (per-pixel sums
-// of squared per-pass values).  stats: [0]=seconds, [1]=ray queries,
-// [2]=node visits, [3]=pack tests, [4]=instance tests.
+// pixel: weighted r,g,b sums and weight sum), optional sq (per-pixel sums of
+// squared per-pass values) and optional aov (9 floats per pixel: weighted
+// albedo rgb, normal xyz, depth, roughness, metallic sums).  stats:
+// [0]=seconds, [1]=ray queries, [2]=node visits, [3]=pack tests,
+// [4]=instance tests.
 int gnd_ref_render(void* h, int threads, int spp_begin, int spp_end, int row_begin, int row_end,
-                   uint64_t seed_offset, float* rgbw, float* sq, double* stats) {
+                   uint64_t seed_offset, float* rgbw, float* sq, double* stats, float* aov) {
     try {
         Scene* s = static_cast<Scene*>(h);
         RenderOptions o;
@@ -136,6 +138,7 @@ int gnd_ref_render(void* h, int threads, int spp_begin, int spp_end, int row_beg
         o.rowEnd = row_end;
         o.seedOffset = seed_offset;
         o.accumulateSquares = sq != nullptr;
+        o.aovs = aov != nullptr;
         Film f;
         Counters c;
         auto t0 = std::chrono::steady_clock::now();
@@ -143,6 +146,7 @@ int gnd_ref_render(void* h, int threads, int spp_begin, int spp_end, int row_beg
         auto t1 = std::chrono::steady_clock::now();
         std::memcpy(rgbw, f.rgbw.data(), f.rgbw.size() * sizeof(float));
         if (sq) std::memcpy(sq, f.sq.data(), f.sq.size() * sizeof(float));
+        if (aov) std::memcpy(aov, f.aov.data(), f.aov.size() * sizeof(float));
         if (stats) {
             stats[0] = std::chrono::duration<double>(t1 - t0).count();
             stats[1] = (double)c.rays;

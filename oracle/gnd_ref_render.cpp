@@ -879,18 +879,74 @@ static float shootRay(const Camera& c, const CamSample& cs, Ray& ray) {
     return weight;
 }
 
+// ------------------------------------------------------------ AOVs
+// Material::getAlbedo / getRoughness / getMetallic (material.h:24-34;
+// matte.cpp:55-57 albedo texture, conductor.cpp:122-132 R / roughness / 1,
+// glass.cpp:117-119 roughness; plastic and index keep the defaults)
+static C3 aovAlbedo(const gnd_material_desc& m, const BSDF& b) {
+    if (m.type == GND_MAT_MATTE) return b.lobe.R;  // the albedo texture at the hit
+    if (m.type == GND_MAT_CONDUCTOR) return C3(m.R[0], m.R[1], m.R[2]);
+    return C3(0.0f);
+}
+static float aovRoughness(const gnd_material_desc& m) {
+    return (m.type == GND_MAT_CONDUCTOR || m.type == GND_MAT_GLASS) ? m.roughness : 0.0f;
+}
+static float aovMetallic(const gnd_material_desc& m) { return m.type == GND_MAT_CONDUCTOR ? 1.0f : 0.0f; }
+// Camera::getForward (camera.h:40-43): the camera-space -z axis in world space
+static V3 cameraForward(const Camera& c) { return xfVector(c.toWorld, V3(0.0f, 0.0f, -1.0f)); }
+
+// The G-buffer writes of both integrators (path.cpp:35-61 and 99-106,
+// volPath.cpp:117-146 and 184-191).  A miss: depth / metallic / roughness
+// reset at bounce 0, albedo = environment (none here, so black), normal 0.
+static void aovMiss(AOV* a, int bounces, bool& needs) {
+    if (!a) return;
+    if (bounces == 0) {
+        a->depth = -1.0f;
+        a->metallic = 0.0f;
+        a->roughness = 0.0f;
+    }
+    if (needs) {
+        a->albedo = C3(0.0f);
+        a->normal = V3(0.0f, 0.0f, 0.0f);
+        needs = false;
+    }
+}
+// A surface with a BSDF: depth / metallic / roughness at bounce 0; albedo and
+// shading normal (mapped to [0,1]) at the first surface with a non-specular
+// lobe, or at the last bounce.
+static void aovSurface(AOV* a, const Scene& s, const Isect& it, const BSDF& b, V3 primaryO, int bounces,
+                       bool& needs) {
+    if (!a) return;
+    const gnd_material_desc& m = s.materials[s.prims[it.prim].material].d;
+    if (bounces == 0) {
+        a->depth = Dot(it.p - primaryO, cameraForward(s.camera));
+        a->metallic = aovMetallic(m);
+        a->roughness = aovRoughness(m);
+    }
+    if (needs && (b.nonSpecular() || bounces == s.maxDepth - 1)) {
+        a->albedo = aovAlbedo(m, b);
+        a->normal = it.n / 2.0f + V3(0.5f, 0.5f, 0.5f);
+        needs = false;
+    }
+}
+
 // ------------------------------------------------------------ integrators
 // PathIntegrator::Li (src/integrators/path.cpp:31-178)
-static C3 LiPath(const Scene& s, Ray r, Sampler& smp, Counters* cnt) {
+static C3 LiPath(const Scene& s, Ray r, Sampler& smp, Counters* cnt, AOV* aov) {
     C3 L(0.0f), tp(1.0f);
     int bounces = 0, nullBounces = 0;
     bool specular = true;
     float pdfPrev = 1.0f;
     V3 lastScatter = r.o;
+    const V3 primaryO = r.o;
+    bool needsG = aov != nullptr;
     float invNE = 1.0f / s.emitters.size();
     while (bounces < s.maxDepth) {
         Hit h;
-        if (!intersectClosest(s, r, h, cnt)) break;
+        if (!intersectClosest(s, r, h, cnt)) {
+            aovMiss(aov, bounces, needsG);
+            break;
+        }
         Isect it;
         fillInteraction(s, r, h, it);
         const Prim& pr = s.prims[it.prim];
@@ -918,6 +974,7 @@ static C3 LiPath(const Scene& s, Ray r, Sampler& smp, Counters* cnt) {
             r = nr;
             continue;
         }
+        aovSurface(aov, s, it, bsdf, primaryO, bounces, needsG);
         // next event estimation
         float lsp = 1.0f / s.emitters.size();
         int ep = pickEmitter(s, smp.next1D());
@@ -1009,12 +1066,14 @@ static int nextMedium(V3 wo, V3 wi, int cur, const Isect& it) {  // volPath.cpp:
 }
 
 // VolumetricPathIntegrator::Li (volPath.cpp:37-249)
-static C3 LiVolPath(const Scene& s, Ray r, Sampler& smp, Counters* cnt) {
+static C3 LiVolPath(const Scene& s, Ray r, Sampler& smp, Counters* cnt, AOV* aov) {
     C3 L(0.0f), tp(1.0f);
     int bounces = 0, surfB = 0, volB = 0, nullB = 0;
     float pdfPrev = 1.0f;
     bool specular = true;
     V3 lastScatter = r.o;
+    const V3 primaryO = r.o;
+    bool needsG = aov != nullptr;
     float invNE = 1.0f / s.emitters.size();
     while (bounces < s.maxDepth) {
         Hit h;
@@ -1064,7 +1123,10 @@ static C3 LiVolPath(const Scene& s, Ray r, Sampler& smp, Counters* cnt) {
             continue;
         }
         int curMedium = r.medium;
-        if (!hitSurface) break;
+        if (!hitSurface) {
+            aovMiss(aov, bounces, needsG);
+            break;
+        }
         Isect it;
         fillInteraction(s, r, h, it);
         const Prim& pr = s.prims[it.prim];
@@ -1093,6 +1155,7 @@ static C3 LiVolPath(const Scene& s, Ray r, Sampler& smp, Counters* cnt) {
             r = nr;
             continue;
         }
+        aovSurface(aov, s, it, bsdf, primaryO, bounces, needsG);
         float lsp = 1.0f / s.emitters.size();
         int ep = pickEmitter(s, smp.next1D());
         LightSample ls;
@@ -1154,6 +1217,7 @@ static float filterEval(const Camera& c, float ox, float oy) {
 struct SampleRec {
     float px, py;  // pFilm in pixel units
     C3 L;
+    AOV aov;
 };
 
 // SamplerIntegrator::render (include/gianduia/core/integrator.h:49-150).  Each
@@ -1167,6 +1231,7 @@ void render(const Scene& s, const RenderOptions& opt, Film& film, Counters* cnt)
     film.height = H;
     film.rgbw.assign((size_t)W * H * 4, 0.0f);
     if (opt.accumulateSquares) film.sq.assign((size_t)W * H * 3, 0.0f);
+    if (opt.aovs) film.aov.assign((size_t)W * H * kAovFloats, 0.0f);
     if (s.emitters.empty()) throw std::runtime_error("oracle render: scene has no emitter");
     int rb = opt.rowBegin, re = opt.rowEnd < 0 ? H : opt.rowEnd;
     int sb = opt.sppBegin, se = opt.sppEnd < 0 ? (int)s.spp : opt.sppEnd;
@@ -1204,7 +1269,10 @@ void render(const Scene& s, const RenderOptions& opt, Film& film, Counters* cnt)
                     cs.time = smp.next1D();
                     Ray ray;
                     float w = shootRay(cam, cs, ray);
-                    C3 raw = s.integrator == GND_INTEGRATOR_VOLPATH ? LiVolPath(s, ray, smp, c) : LiPath(s, ray, smp, c);
+                    AOV aov;
+                    AOV* ap = opt.aovs ? &aov : nullptr;
+                    C3 raw = s.integrator == GND_INTEGRATOR_VOLPATH ? LiVolPath(s, ray, smp, c, ap)
+                                                                    : LiPath(s, ray, smp, c, ap);
                     raw = raw * w;
                     C3 nc(0.0f);
                     if (ch == -1) nc = raw;
@@ -1212,7 +1280,7 @@ void render(const Scene& s, const RenderOptions& opt, Film& film, Counters* cnt)
                     else if (ch == 1) nc.g = raw.g * 3.0f;
                     else nc.b = raw.b * 3.0f;
                     if (nc.hasNaNs()) nc = C3(0.0f);
-                    samples[(size_t)y * W + x] = {pfx, pfy, nc};
+                    samples[(size_t)y * W + x] = {pfx, pfy, nc, aov};
                 }
             }
         };
@@ -1228,6 +1296,7 @@ void render(const Scene& s, const RenderOptions& opt, Film& film, Counters* cnt)
                 if (y >= re) break;
                 for (int x = 0; x < W; ++x) {
                     float sr = 0, sg = 0, sbb = 0, sw = 0;
+                    float sa[kAovFloats] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
                     for (int dy = -reach; dy <= reach; ++dy) {
                         int qy = y + dy;
                         if (qy < rb || qy >= re) continue;
@@ -1246,6 +1315,12 @@ void render(const Scene& s, const RenderOptions& opt, Film& film, Counters* cnt)
                                 sg += q.L.g * wgt;
                                 sbb += q.L.b * wgt;
                                 sw += wgt;
+                                if (opt.aovs) {
+                                    const float v[kAovFloats] = {q.aov.albedo.r, q.aov.albedo.g, q.aov.albedo.b,
+                                                                 q.aov.normal.x, q.aov.normal.y, q.aov.normal.z,
+                                                                 q.aov.depth,    q.aov.roughness, q.aov.metallic};
+                                    for (int k = 0; k < kAovFloats; ++k) sa[k] += v[k] * wgt;
+                                }
                             }
                         }
                     }
@@ -1254,6 +1329,10 @@ void render(const Scene& s, const RenderOptions& opt, Film& film, Counters* cnt)
                     acc[1] += sg;
                     acc[2] += sbb;
                     acc[3] += sw;
+                    if (opt.aovs) {
+                        float* av = &film.aov[((size_t)y * W + x) * kAovFloats];
+                        for (int k = 0; k < kAovFloats; ++k) av[k] += sa[k];
+                    }
                     if (opt.accumulateSquares && sw > 0.0f) {
                         float* q2 = &film.sq[((size_t)y * W + x) * 3];
                         float vr = sr / sw, vg = sg / sw, vb = sbb / sw;
@@ -1478,8 +1557,9 @@ float gnd_ref_grid_sample(void* h, int medium, double x, double y, double z) {
     return gridSample(s->media[medium], x, y, z);
 }
 
-// Single-path radiance for (pixel, pass): out rgb
-void gnd_ref_path_sample(void* h, int x, int y, int pass, uint64_t seed_offset, float* out) {
+// Single-path radiance for (pixel, pass): out rgb, then (if aov is not null)
+// the path's AOV record: albedo rgb, normal xyz, depth, roughness, metallic
+void gnd_ref_path_sample(void* h, int x, int y, int pass, uint64_t seed_offset, float* out, float* aov) {
     Scene* s = static_cast<Scene*>(h);
     int W = s->camera.d.width, H = s->camera.d.height;
     Sampler smp;
@@ -1493,10 +1573,18 @@ void gnd_ref_path_sample(void* h, int x, int y, int pass, uint64_t seed_offset, 
     cs.time = smp.next1D();
     Ray ray;
     float w = shootRay(s->camera, cs, ray);
-    C3 L = s->integrator == GND_INTEGRATOR_VOLPATH ? LiVolPath(*s, ray, smp, nullptr) : LiPath(*s, ray, smp, nullptr);
+    AOV a;
+    AOV* ap = aov ? &a : nullptr;
+    C3 L = s->integrator == GND_INTEGRATOR_VOLPATH ? LiVolPath(*s, ray, smp, nullptr, ap)
+                                                   : LiPath(*s, ray, smp, nullptr, ap);
     L = L * w;
     if (L.hasNaNs()) L = C3(0.0f);
     out[0] = L.r; out[1] = L.g; out[2] = L.b;
+    if (aov) {
+        const float v[kAovFloats] = {a.albedo.r, a.albedo.g, a.albedo.b, a.normal.x, a.normal.y,
+                                     a.normal.z, a.depth,    a.roughness, a.metallic};
+        for (int k = 0; k < kAovFloats; ++k) aov[k] = v[k];
+    }
 }
 
 }  // extern "C"

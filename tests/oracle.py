@@ -34,7 +34,7 @@ _API = {
     "gnd_ref_trace": (C.c_int, [C.c_void_p, C.POINTER(A.gnd_ray_soa), C.c_int64, C.POINTER(A.gnd_hit_soa), C.c_int,
                                 C.c_void_p]),
     "gnd_ref_render": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, f32p, f32p,
-                                 C.POINTER(C.c_double)]),
+                                 C.POINTER(C.c_double), f32p]),
     "gnd_ref_pcg32": (None, [C.c_uint64, C.c_uint64, C.c_int, C.POINTER(C.c_uint32)]),
     "gnd_ref_splitmix64": (C.c_uint64, [C.c_uint64]),
     "gnd_ref_sampler_floats": (None, [C.c_uint64, C.c_int, f32p]),
@@ -60,7 +60,7 @@ _API = {
     "gnd_ref_surface": (C.c_int, [C.c_void_p, f32p, f32p, f32p]),
     "gnd_ref_medium": (C.c_int, [C.c_void_p, C.c_int, C.c_int, f32p, f32p, C.c_float, C.c_uint64, f32p]),
     "gnd_ref_grid_sample": (C.c_float, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double]),
-    "gnd_ref_path_sample": (None, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_uint64, f32p]),
+    "gnd_ref_path_sample": (None, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_uint64, f32p, f32p]),
 }
 
 _lib = None
@@ -157,24 +157,31 @@ class RefScene:
         return (out, cnt.reshape(-1, 3)[:n]) if counts else out
 
     def render(self, threads=1, spp=None, spp_begin=0, row_begin=0, row_end=-1, seed_offset=0, squares=False,
-               width=None, height=None):
-        d = None
+               width=None, height=None, aovs=False):
+        """Returns (rgbw, stats), (rgbw, sq, stats) with squares, and the AOV
+        sums (H, W, 9) appended with aovs."""
         W, H = width, height
         rgbw = np.zeros((H, W, 4), np.float32)
         sq = np.zeros((H, W, 3), np.float32) if squares else None
+        aov = np.zeros((H, W, 9), np.float32) if aovs else None
         stats = (C.c_double * 5)()
         rc = lib().gnd_ref_render(self.h, threads, spp_begin, -1 if spp is None else spp, row_begin, row_end,
-                                  seed_offset, fp(rgbw), fp(sq) if squares else None, stats)
+                                  seed_offset, fp(rgbw), fp(sq) if squares else None, stats,
+                                  fp(aov) if aovs else None)
         if rc != 0:
             raise RuntimeError(lib().gnd_ref_last_error().decode())
-        return (rgbw, sq, list(stats)) if squares else (rgbw, list(stats))
+        out = (rgbw, sq, list(stats)) if squares else (rgbw, list(stats))
+        return out + (aov,) if aovs else out
 
     def camera_ray(self, pfilm, plens=(0.5, 0.5), lam=0.0):
         out = np.zeros(7, np.float32)
         lib().gnd_ref_camera_ray(self.h, fp(vec(*pfilm)), fp(vec(*plens)), lam, fp(out))
         return out[:3], out[3:6], float(out[6])
 
-    def path_sample(self, x, y, p, seed_offset=0):
+    def path_sample(self, x, y, p, seed_offset=0, aov=False):
+        """Radiance of one (pixel, pass) path; with aov also its AOV record
+        (albedo rgb, normal xyz, depth, roughness, metallic)."""
         out = np.zeros(3, np.float32)
-        lib().gnd_ref_path_sample(self.h, x, y, p, seed_offset, fp(out))
-        return out
+        rec = np.zeros(9, np.float32)
+        lib().gnd_ref_path_sample(self.h, x, y, p, seed_offset, fp(out), fp(rec) if aov else None)
+        return (out, rec) if aov else out

@@ -53,6 +53,28 @@ class SceneBuilder:
         self.materials.append(m)
         return len(self.materials) - 1
 
+    def conductor(self, R=(0.9, 0.8, 0.7), roughness=0.0, eta=(0.2, 0.92, 1.1), k=(3.9, 2.45, 2.14)) -> int:
+        m = A.gnd_material_desc()
+        m.type = A.GND_MAT_CONDUCTOR
+        m.R[:] = R
+        m.eta_rgb[:] = eta
+        m.k_rgb[:] = k
+        m.roughness = roughness
+        m.medium_inside = m.medium_outside = -1
+        self.materials.append(m)
+        return len(self.materials) - 1
+
+    def glass(self, eta=1.5, roughness=0.0) -> int:
+        m = A.gnd_material_desc()
+        m.type = A.GND_MAT_GLASS
+        m.R[:] = (1.0, 1.0, 1.0)
+        m.T[:] = (1.0, 1.0, 1.0)
+        m.eta = eta
+        m.roughness = roughness
+        m.medium_inside = m.medium_outside = -1
+        self.materials.append(m)
+        return len(self.materials) - 1
+
     def prim(self, mesh: int, M=None, Minv=None, material: int = 0, radiance=None, sphere_radius=None) -> int:
         p = A.gnd_primitive_desc()
         if sphere_radius is not None:
@@ -157,3 +179,32 @@ def random_rays(rng: np.random.Generator, n: int, spread=2.5, edge_cases=True) -
         tmax[16:20] = 0.0                                          # empty interval
     return {"ox": o[:, 0].copy(), "oy": o[:, 1].copy(), "oz": o[:, 2].copy(), "dx": d[:, 0].copy(),
             "dy": d[:, 1].copy(), "dz": d[:, 2].copy(), "tmin": tmin, "tmax": tmax}
+
+
+def quad(x0, x1, y0, y1, z, facing=+1) -> np.ndarray:
+    """Two triangles of the rectangle [x0,x1] x [y0,y1] at depth z, wound so
+    the geometric normal is +z (facing=+1) or -z (facing=-1)."""
+    a, b, c, d = (x0, y0, z), (x1, y0, z), (x1, y1, z), (x0, y1, z)
+    tris = [a + b + c, a + c + d] if facing > 0 else [a + c + b, a + d + c]
+    return np.array(tris, np.float32)
+
+
+def mirror_scene(front_material: str | None, roughness=0.0, integrator=A.GND_INTEGRATOR_PATH, max_depth=4):
+    """Camera at the origin looking down -z (16x16, fov 45, box filter, 1 spp).
+    A quad of `front_material` ('conductor', 'glass' or None) at z = -2 faces
+    the camera and fills the view; a matte wall (0.2, 0.4, 0.6) at z = +1
+    behind the camera faces -z; a small light sits far off to the side."""
+    b = SceneBuilder()
+    wall = b.matte((0.2, 0.4, 0.6))
+    b.prim(b.mesh(quad(-10, 10, -10, 10, 1.0, facing=-1)), material=wall)
+    if front_material == "conductor":
+        b.prim(b.mesh(quad(-10, 10, -10, 10, -2.0, facing=+1)), material=b.conductor(roughness=roughness))
+    elif front_material == "glass":
+        b.prim(b.mesh(quad(-10, 10, -10, 10, -2.0, facing=+1)), material=b.glass(roughness=roughness))
+    light = np.array([[30, 30, 30, 31, 30, 30, 31, 30, 31]], np.float32)
+    b.prim(b.mesh(light), material=wall, radiance=(5, 5, 5))
+    d = b.desc(width=16, height=16)
+    d.integrator = integrator
+    d.max_depth = max_depth
+    return d
+

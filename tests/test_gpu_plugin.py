@@ -6,7 +6,7 @@ film is saved."""
 import numpy as np
 import pytest
 
-from gianduia_amd import DeviceScene, GndError, HostScene, Integrator, resolve
+from gianduia_amd import DeviceScene, GndError, HostScene, Integrator, resolve, resolve_aovs
 
 pytestmark = pytest.mark.gpu
 
@@ -57,3 +57,41 @@ def test_scene_integrator_and_save(device, tmp_path):
 def test_unknown_integrator(device):
     with pytest.raises(GndError):
         Integrator(device, "direct")
+
+
+def test_integrator_aovs_equal_device_render(device):
+    """Film AOV images of the plugin (one batch) resolve the device's AOV sums
+    exactly like resolve_aovs; without set_aovs the film has none."""
+    hs = HostScene.generate("scaled", "level=3;instances=4;width=48;height=32;spp=3")
+    it = Integrator(device, "path", max_depth=6, passes_per_callback=3)
+    it.set_aovs(True)
+    it.render(hs)
+    got = it.aovs()
+    ds = DeviceScene(device, hs)
+    j = hs.default_job(spp=3)
+    j.max_depth = 6
+    out, _ = device.render(ds, [j], want_aov=True)
+
+    g, gaov = out[0]
+    want = resolve_aovs(g, gaov)
+    for k in want:
+        assert np.array_equal(got[k], want[k]), k
+    it2 = Integrator(device, "path", max_depth=6)
+    it2.render(hs)
+    with pytest.raises(GndError):
+        it2.aovs()
+
+
+def test_host_scene_outlives_its_context():
+    """A host scene whose device copy was made on a context that is shut down
+    first: the shutdown frees that copy, so freeing the host scene afterwards
+    does not touch the dead context (it used to synchronise its stream)."""
+    from gianduia_amd import Device
+    dev = Device(0)
+    hs = HostScene.generate("cbox", "width=16;height=16;spp=1")
+    it = Integrator(dev, "path", max_depth=2)
+    it.render(hs)  # uploads the scene on dev's context and caches the copy
+    it.close()
+    dev.close()
+    hs.free()
+

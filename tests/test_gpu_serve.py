@@ -43,6 +43,29 @@ def test_server_films_equal_solo_render(device):
         srv.close()
 
 
+def test_server_aovs_equal_solo_render(device):
+    """AOV films of served jobs equal the solo render's (continuous batching
+    keeps per-path records and gathers them in pass order like radiance)."""
+    a = HostScene.generate("cbox", "width=48;height=40")
+    b = HostScene.generate("volumetric", "grid=16;width=40;height=32")
+    srv = Server(device, path_slots=20000, max_jobs=8)
+    try:
+        ids = [srv.add_scene(h) for h in (a, b)]
+        work = [(ids[0], a, j) for j in _jobs(a, 3, 7)] + [(ids[1], b, j) for j in _jobs(b, 3, 8)]
+        tickets = [(hs, j, srv.submit(sid, j, want_aov=(k % 2 == 0))) for k, (sid, hs, j) in enumerate(work)]
+        for hs, j, tk in tickets:
+            assert srv.wait(tk, 120)["state"] == 2
+        for hs, j, tk in tickets:
+            ds = DeviceScene(device, hs)
+            out, _ = device.render(ds, [j], want_aov=True)
+            assert np.array_equal(srv.film(tk), out[0][0])
+            if tk.aov is not None:
+                assert np.array_equal(srv.aov(tk), out[0][1]), "served AOVs differ from the solo render"
+            ds.free()
+    finally:
+        srv.close()
+
+
 def test_server_evicts_under_budget(device):
     scenes = [HostScene.generate("scaled", f"level=3;instances=4;seed={i};width=32;height=32") for i in range(3)]
     probe = DeviceScene(device, scenes[0])

@@ -12,7 +12,7 @@ from scipy import stats as sst
 from gianduia_amd import HostScene, _abi as A
 import oracle as O
 from oracle import RefScene, fp, vec
-from scenes import SceneBuilder, random_rays, random_soup, trs
+from scenes import SceneBuilder, mirror_scene, random_rays, random_soup, trs
 
 
 # ---------------------------------------------------------------- traversal vs brute force
@@ -400,3 +400,77 @@ def test_path_integrator_furnace(albedo, filt):
     expect = 1.0 / (1.0 - albedo)
     assert abs(img.mean() - expect) < 0.02 * expect, img.mean()
     assert np.all(np.abs(img - expect) < 0.25 * expect)
+
+
+# ---------------------------------------------------------------- AOVs
+def _aov_image(d):
+    """1 spp, box filter: every pixel holds exactly one sample's AOV record."""
+    ref = RefScene(d)
+    rgbw, _, aov = ref.render(threads=4, spp=1, width=16, height=16, aovs=True)
+    assert np.all(rgbw[..., 3] == 1.0)
+    return aov
+
+
+@pytest.mark.parametrize("integrator", [A.GND_INTEGRATOR_PATH, A.GND_INTEGRATOR_VOLPATH])
+def test_aov_specular_chain_and_first_hit(integrator):
+    """AOVRecord semantics (path.cpp:35-106, volPath.cpp:117-191) on scenes
+    with closed-form answers.  Smooth mirror at z=-2 facing the camera, matte
+    wall (0.2,0.4,0.6) at z=+1 behind it: depth/roughness/metallic come from
+    the first hit (2, 0, 1), albedo and normal from the wall seen in the
+    mirror (the specular hit passes the G-buffer on).  A rough mirror keeps
+    them at the first hit (R, +z normal); with max depth 1 the last-bounce
+    rule takes the smooth mirror's R; rough glass has no albedo; nothing in
+    view gives the miss defaults."""
+    half = np.float32(0.5)
+    # smooth mirror: albedo / normal from the wall behind the camera
+    a = _aov_image(mirror_scene("conductor", 0.0, integrator))
+    assert np.all(a[..., 0:3] == np.array([0.2, 0.4, 0.6], np.float32))
+    assert np.all(a[..., 3:6] == np.array([half, half, 0.0], np.float32))
+    assert np.allclose(a[..., 6], 2.0, rtol=0, atol=2e-6)
+    assert np.all(a[..., 7] == 0.0) and np.all(a[..., 8] == 1.0)
+    # rough mirror: everything from the first hit
+    a = _aov_image(mirror_scene("conductor", 0.3, integrator))
+    assert np.all(a[..., 0:3] == np.array([0.9, 0.8, 0.7], np.float32))
+    assert np.all(a[..., 3:6] == np.array([half, half, 1.0], np.float32))
+    assert np.all(a[..., 7] == np.float32(0.3)) and np.all(a[..., 8] == 1.0)
+    # max depth 1: bounce 0 is the last bounce, so the smooth mirror's own R
+    a = _aov_image(mirror_scene("conductor", 0.0, integrator, max_depth=1))
+    assert np.all(a[..., 0:3] == np.array([0.9, 0.8, 0.7], np.float32))
+    assert np.all(a[..., 3:6] == np.array([half, half, 1.0], np.float32))
+    # rough glass: non-specular, no albedo, its roughness, not metallic
+    a = _aov_image(mirror_scene("glass", 0.2, integrator))
+    assert np.all(a[..., 0:3] == 0.0) and np.all(a[..., 3:6] == np.array([half, half, 1.0], np.float32))
+    assert np.all(a[..., 7] == np.float32(0.2)) and np.all(a[..., 8] == 0.0)
+    assert np.allclose(a[..., 6], 2.0, rtol=0, atol=2e-6)
+    # nothing in front of the camera: the miss defaults
+    a = _aov_image(mirror_scene(None, 0.0, integrator))
+    assert np.all(a[..., 0:6] == 0.0) and np.all(a[..., 6] == -1.0)
+    assert np.all(a[..., 7:9] == 0.0)
+
+
+def test_aov_depth_is_distance_along_forward():
+    """Depth = Dot(p - o, Camera::getForward()) with forward = toWorld(0,0,-1)
+    (camera.h:40-43).  The camera is turned 20 degrees about y, so the plane
+    z = -2 is tilted against the view axis: for each pixel the camera ray of
+    its pass-0 sample (jitter from the sampler stream, ray from the camera
+    probe) meets the plane at t = (-2 - o.z) / d.z, and depth must equal
+    t * Dot(d, forward) (float rounding aside).  A wrong forward sign or axis,
+    or a depth taken from the ray parameter, fails this."""
+    from scenes import col_major
+    d = mirror_scene("conductor", 0.3)
+    M, Mi = trs(rot_deg=20.0, axis=(0, 1, 0))
+    d.camera.to_world = col_major(M)
+    d.camera.to_world_inv = col_major(Mi)
+    a = _aov_image(d)
+    ref = RefScene(d)
+    fwd = M[:3, :3].astype(np.float64) @ np.array([0.0, 0.0, -1.0])
+    W = H = 16
+    for y in range(H):
+        for x in range(W):
+            u = np.zeros(2, np.float32)
+            O.lib().gnd_ref_sampler_floats(y * W + x, 2, fp(u))
+            o, dr, _ = ref.camera_ray(((x + u[0]) / W, 1.0 - (y + u[1]) / H))
+            o, dr = o.astype(np.float64), dr.astype(np.float64)
+            t = (-2.0 - o[2]) / dr[2]
+            assert abs(a[y, x, 6] - t * dr @ fwd) < 1e-5 * t, (x, y)
+    assert np.ptp(a[..., 6]) > 0.05  # tilted plane: depth varies across the image
