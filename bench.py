@@ -14,9 +14,9 @@ restatement of the reference CPU integrator on the host cores, on a bounded
 sample of the same frame) and the 64-job batching workload.
 
 --scene cbox|volumetric measures the Cornell-box (512x512, 64 spp, depth 8)
-and volumetric (1280x720, 32 spp, depth 32) workloads instead; --serve runs
-the serving workload (Poisson arrivals over a pool of 8 scenes) and prints
-its own JSON line.
+and volumetric (1280x720, 32 spp, depth 32) workloads instead; --aov also
+accumulates the film AOVs; --serve runs the serving workload (Poisson
+arrivals over a pool of 8 scenes) and prints its own JSON line.
 """
 from __future__ import annotations
 
@@ -230,6 +230,8 @@ def main():
                     help="workload scene: scaled (default, 1080p 1M tris), cbox (512^2 Cornell box), "
                          "volumetric (720p, homogeneous + 256^3 grid media)")
     ap.add_argument("--params", default="", help="scene generator overrides, key=value;...")
+    ap.add_argument("--aov", action="store_true",
+                    help="also accumulate the film AOVs (albedo, normal, depth, roughness, metallic)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration) and exit")
     a = ap.parse_args()
@@ -271,7 +273,7 @@ def main():
     log(f"scene: generated {gen_s:.1f}s, BVH build {hs.build_ms:.0f} ms, {inst_tris} instanced triangles, "
         f"{ds.nbytes / 2**20:.0f} MiB in HBM")
     job = hs.default_job()
-    sess = RenderSession(dev, ds, [job])
+    sess = RenderSession(dev, ds, [job], want_aov=a.aov)
     for i in range(a.warmup):
         st = sess.run()
         log(f"warmup {i}: {st['ms_total']:.1f} ms")
@@ -335,7 +337,8 @@ def main():
         "config": {"workload": WORKLOADS[a.scene],
                    "width": cam.width, "height": cam.height, "spp": job.spp, "max_depth": job.max_depth,
                    "instanced_triangles": inst_tris,
-                   "integrator": "volpath" if job.integrator == A.GND_INTEGRATOR_VOLPATH else "path"},
+                   "integrator": "volpath" if job.integrator == A.GND_INTEGRATOR_VOLPATH else "path",
+                   "aovs": bool(a.aov)},
         "mrays_per_s": round(mrays, 2),
         "trace_ms_per_step": round(sum(s["ms_trace"] for s in per_step) / a.steps, 3),
         "shadow_ms_per_step": round(sum(s["ms_shadow"] for s in per_step) / a.steps, 3),

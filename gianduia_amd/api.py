@@ -226,15 +226,20 @@ class Device:
 class RenderSession:
     """Jobs with device-resident films, rendered repeatedly without host copies."""
 
-    def __init__(self, dev: Device, scene: "DeviceScene", jobs: list["Job"], want_sq: bool = False):
+    def __init__(self, dev: Device, scene: "DeviceScene", jobs: list["Job"], want_sq: bool = False,
+                 want_aov: bool = False):
         self.dev, self.scene, self.jobs = dev, scene, jobs
         self.cj = (A.gnd_hip_job * len(jobs))()
         self.films = []
+        self.aovs = []
         for i, j in enumerate(jobs):
             w, h = j.camera.width, j.camera.height
             f = dev.alloc(w * h * 16)
             sq = dev.alloc(w * h * 12) if want_sq else None
+            av = dev.alloc(w * h * 36) if want_aov else None
             self.films.append((f, sq, w, h))
+            self.aovs.append(av)
+            self.cj[i].film_aov = av.addr if av is not None else None
             self.cj[i].camera = j.camera
             self.cj[i].spp = j.spp
             self.cj[i].max_depth = j.max_depth
@@ -255,6 +260,10 @@ class RenderSession:
     def film_sq(self, i: int) -> np.ndarray:
         _, sq, w, h = self.films[i]
         return self.dev.download(sq, np.float32, w * h * 3).reshape(h, w, 3)
+
+    def film_aov(self, i: int) -> np.ndarray:
+        _, _, w, h = self.films[i]
+        return self.dev.download(self.aovs[i], np.float32, w * h * 9).reshape(h, w, 9)
 
 
 @dataclass
