@@ -194,6 +194,7 @@ HOST_API = {
                                            C.POINTER(C.c_int)]),
     "gnd_host_integrator_save": (C.c_int, [C.c_void_p, C.c_char_p]),
     "gnd_host_integrator_set_aovs": (C.c_int, [C.c_void_p, C.c_int]),
+    "gnd_host_integrator_save_aov": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p]),
     "gnd_host_integrator_aovs": (C.c_int, [C.c_void_p, f32p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "gnd_host_integrator_free": (None, [C.c_void_p]),
     "gnd_host_integrator_names": (C.c_int, [C.c_char_p, C.c_int]),

@@ -434,6 +434,13 @@ class Integrator:
         return {"albedo": out[..., 0:3], "normal": out[..., 3:6], "depth": out[..., 6], "roughness": out[..., 7],
                 "metallic": out[..., 8]}
 
+    def save_aov(self, aov: str, path: str) -> None:
+        """Writes one AOV image ("albedo", "normal", "depth", "roughness",
+        "metallic") as .exr or .pfm."""
+        rc = _lib().gnd_host_integrator_save_aov(self.handle, aov.encode(), path.encode())
+        if rc != 0:
+            raise GndError(rc, host_error())
+
     def save(self, path: str) -> None:
         rc = _lib().gnd_host_integrator_save(self.handle, path.encode())
         if rc != 0:

@@ -263,7 +263,16 @@ int gnd_host_integrator_save(gnd_host_integrator* it, const char* path) {
         const std::string ext = dot == std::string::npos ? "" : p.substr(dot + 1);
         if (ext == "pfm") it->impl->film().savePFM(p);
         else if (ext == "png") it->impl->film().savePNG(p);
+        else if (ext == "exr") it->impl->film().saveEXR(p);
         else throw std::runtime_error("unsupported image format: " + p);
+        return GND_OK;
+    });
+}
+
+int gnd_host_integrator_save_aov(gnd_host_integrator* it, const char* aov, const char* path) {
+    if (!it || !aov || !path) return GND_E_INVALID;
+    return hostGuard([&] {
+        it->impl->film().saveAov(aov, path);
         return GND_OK;
     });
 }
@@ -296,6 +305,7 @@ int gnd_host_image_save(const char* path, const float* rgb, int width, int heigh
         const std::string p(path);
         if (p.size() > 4 && p.compare(p.size() - 4, 4, ".pfm") == 0) f.savePFM(p);
         else if (p.size() > 4 && p.compare(p.size() - 4, 4, ".png") == 0) f.savePNG(p);
+        else if (p.size() > 4 && p.compare(p.size() - 4, 4, ".exr") == 0) f.saveEXR(p);
         else throw std::runtime_error("unsupported image format: " + p);
         return GND_OK;
     });

@@ -91,12 +91,116 @@ void Film::resolve() {
     }
 }
 
-void Film::savePFM(const std::string& path) const {
+void writePFM(const std::string& path, const float* px, int w, int h, int channels) {
     std::ofstream f(path, std::ios::binary);
     if (!f) throw std::runtime_error("cannot write " + path);
-    f << "PF\n" << w_ << " " << h_ << "\n-1.0\n";  // little-endian, rows bottom to top
-    for (int y = h_ - 1; y >= 0; --y)
-        f.write(reinterpret_cast<const char*>(&rgb_[(size_t)y * w_ * 3]), (std::streamsize)w_ * 3 * sizeof(float));
+    f << (channels == 3 ? "PF" : "Pf") << "\n" << w << " " << h << "\n-1.0\n";  // little-endian, bottom row first
+    for (int y = h - 1; y >= 0; --y)
+        f.write(reinterpret_cast<const char*>(px + (size_t)y * w * channels),
+                (std::streamsize)w * channels * sizeof(float));
+    if (!f) throw std::runtime_error("write failed: " + path);
+}
+
+void Film::savePFM(const std::string& path) const { writePFM(path, rgb_.data(), w_, h_, 3); }
+
+namespace {
+void putBytes(std::vector<unsigned char>& v, const void* p, size_t n) {
+    const unsigned char* b = static_cast<const unsigned char*>(p);
+    v.insert(v.end(), b, b + n);
+}
+template <class T>
+void putLE(std::vector<unsigned char>& v, T x) {  // the host is little-endian, as EXR is
+    putBytes(v, &x, sizeof(T));
+}
+void putStr(std::vector<unsigned char>& v, const char* s) { putBytes(v, s, std::strlen(s) + 1); }
+void attribute(std::vector<unsigned char>& v, const char* name, const char* type,
+               const std::vector<unsigned char>& value) {
+    putStr(v, name);
+    putStr(v, type);
+    putLE<int32_t>(v, (int32_t)value.size());
+    v.insert(v.end(), value.begin(), value.end());
+}
+}  // namespace
+
+// OpenEXR single-part scanline file: magic 20000630, version 2, the required
+// header attributes, a table of scanline-block offsets, then one block per
+// row (y, byte count, each channel's row in channel-name order).  Written
+// uncompressed; OpenEXR readers take any compression the header names.
+void writeEXR(const std::string& path, const float* px, int w, int h, int channels) {
+    if (channels != 1 && channels != 3) throw std::runtime_error("EXR: 1 or 3 channels");
+    std::vector<unsigned char> out;
+    putLE<int32_t>(out, 20000630);
+    putLE<int32_t>(out, 2);
+    // channels, sorted by name (B, G, R), 32-bit float, sampling 1
+    static const char* rgbNames[3] = {"B", "G", "R"};
+    static const int rgbIndex[3] = {2, 1, 0};  // position of B, G, R in the input
+    std::vector<unsigned char> ch;
+    for (int c = 0; c < channels; ++c) {
+        putStr(ch, channels == 3 ? rgbNames[c] : "Y");
+        putLE<int32_t>(ch, 2);            // pixel type FLOAT
+        putLE<uint8_t>(ch, 0);            // pLinear
+        putLE<uint8_t>(ch, 0);
+        putLE<uint8_t>(ch, 0);
+        putLE<uint8_t>(ch, 0);            // reserved
+        putLE<int32_t>(ch, 1);            // xSampling
+        putLE<int32_t>(ch, 1);            // ySampling
+    }
+    ch.push_back(0);
+    attribute(out, "channels", "chlist", ch);
+    attribute(out, "compression", "compression", {0});  // NO_COMPRESSION
+    std::vector<unsigned char> box;
+    for (int32_t v : {0, 0, w - 1, h - 1}) putLE<int32_t>(box, v);
+    attribute(out, "dataWindow", "box2i", box);
+    attribute(out, "displayWindow", "box2i", box);
+    attribute(out, "lineOrder", "lineOrder", {0});  // INCREASING_Y
+    std::vector<unsigned char> f1, v2;
+    putLE<float>(f1, 1.0f);
+    attribute(out, "pixelAspectRatio", "float", f1);
+    putLE<float>(v2, 0.0f);
+    putLE<float>(v2, 0.0f);
+    attribute(out, "screenWindowCenter", "v2f", v2);
+    attribute(out, "screenWindowWidth", "float", f1);
+    out.push_back(0);  // end of header
+    const size_t rowBytes = (size_t)w * channels * sizeof(float);
+    const size_t table = out.size();
+    const size_t first = table + (size_t)h * sizeof(uint64_t);
+    for (int y = 0; y < h; ++y) putLE<uint64_t>(out, (uint64_t)(first + (size_t)y * (8 + rowBytes)));
+    std::vector<float> row((size_t)w);
+    for (int y = 0; y < h; ++y) {
+        putLE<int32_t>(out, y);
+        putLE<int32_t>(out, (int32_t)rowBytes);
+        for (int c = 0; c < channels; ++c) {
+            const int src = channels == 3 ? rgbIndex[c] : 0;
+            for (int x = 0; x < w; ++x) row[x] = px[((size_t)y * w + x) * channels + src];
+            putBytes(out, row.data(), row.size() * sizeof(float));
+        }
+    }
+    std::ofstream f(path, std::ios::binary);
+    if (!f) throw std::runtime_error("cannot write " + path);
+    f.write(reinterpret_cast<const char*>(out.data()), (std::streamsize)out.size());
+    if (!f) throw std::runtime_error("write failed: " + path);
+}
+
+void Film::saveEXR(const std::string& path) const { writeEXR(path, rgb_.data(), w_, h_, 3); }
+
+void Film::saveAov(const std::string& kind, const std::string& path) const {
+    if (aov_.empty()) throw std::runtime_error("the film has no AOVs");
+    int first, n;
+    if (kind == "albedo") { first = 0; n = 3; }
+    else if (kind == "normal") { first = 3; n = 3; }
+    else if (kind == "depth") { first = 6; n = 1; }
+    else if (kind == "roughness") { first = 7; n = 1; }
+    else if (kind == "metallic") { first = 8; n = 1; }
+    else throw std::runtime_error("unknown AOV: " + kind);
+    const size_t np = (size_t)w_ * h_;
+    std::vector<float> img(np * n);
+    for (size_t i = 0; i < np; ++i)
+        for (int c = 0; c < n; ++c) img[i * n + c] = aov_[i * kAovs + first + c];
+    const auto dot = path.find_last_of('.');
+    const std::string ext = dot == std::string::npos ? "" : path.substr(dot + 1);
+    if (ext == "exr") writeEXR(path, img.data(), w_, h_, n);
+    else if (ext == "pfm") writePFM(path, img.data(), w_, h_, n);
+    else throw std::runtime_error("unsupported image format: " + path);
 }
 
 namespace {

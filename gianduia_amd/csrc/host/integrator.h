@@ -78,6 +78,10 @@ public:
                                                               // weight is 0: 0,0,0, 0,0,0, -1, 0, 0)
     void savePFM(const std::string& path) const;
     void savePNG(const std::string& path) const;  // 8-bit sRGB, values clamped to [0, 1]
+    void saveEXR(const std::string& path) const;  // float R, G, B (Image2D::saveEXR, image2d.h:62-97)
+    // one resolved AOV image: "albedo" / "normal" (R, G, B) or "depth" /
+    // "roughness" / "metallic" (Y), as ".exr" or ".pfm"
+    void saveAov(const std::string& kind, const std::string& path) const;
 
 private:
     int w_ = 0, h_ = 0;
@@ -86,6 +90,12 @@ private:
     std::vector<double> aovAcc_;
     std::vector<float> aov_;
 };
+
+// Scanline OpenEXR file, uncompressed, 32-bit float channels (channels = 3:
+// interleaved R, G, B input; 1: Y), rows top to bottom.
+void writeEXR(const std::string& path, const float* px, int w, int h, int channels);
+// PFM (little-endian, rows bottom to top), 3 channels ("PF") or 1 ("Pf")
+void writePFM(const std::string& path, const float* px, int w, int h, int channels);
 
 // What an integrator renders: the scene description and its device copy.
 struct RenderScene {

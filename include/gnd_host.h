@@ -83,13 +83,18 @@ int gnd_host_integrator_film(gnd_host_integrator* it, float* rgb, int* width, in
  * GND_E_INVALID if the last render had no AOVs. */
 int gnd_host_integrator_set_aovs(gnd_host_integrator* it, int enabled);
 int gnd_host_integrator_aovs(gnd_host_integrator* it, float* out, int* width, int* height);
-/* Film::savePNG / an HDR float image (integrator.h:152-153): ".png" (8-bit
- * sRGB) or ".pfm" (32-bit float RGB). */
+/* Film::savePNG / Film::saveEXR (integrator.h:152-153, film.cpp:153-194):
+ * ".png" (8-bit sRGB), ".exr" (OpenEXR scanline, float R G B, uncompressed)
+ * or ".pfm" (32-bit float RGB). */
 int gnd_host_integrator_save(gnd_host_integrator* it, const char* path);
+/* One resolved AOV image (Image2D::saveEXR of the film's AOV images,
+ * image2d.h:62-97): aov = "albedo" | "normal" (R G B) or "depth" |
+ * "roughness" | "metallic" (Y), path ".exr" or ".pfm". */
+int gnd_host_integrator_save_aov(gnd_host_integrator* it, const char* aov, const char* path);
 void gnd_host_integrator_free(gnd_host_integrator* it);
 /* Registered integrator names, comma separated; returns the full length. */
 int gnd_host_integrator_names(char* buf, int len);
-/* Writes a width*height*3 float RGB image as ".png" or ".pfm" (the film writers). */
+/* Writes a width*height*3 float RGB image as ".png", ".exr" or ".pfm" (the film writers). */
 int gnd_host_image_save(const char* path, const float* rgb, int width, int height);
 /* IndependentSampler (src/samplers/independent.cpp:7-47): the first n floats
  * of sample index `sample_index`. */

@@ -95,3 +95,28 @@ def test_host_scene_outlives_its_context():
     dev.close()
     hs.free()
 
+
+def test_integrator_saves_exr_and_aov_images(device, tmp_path):
+    """Film::saveEXR and the AOV images (Image2D::saveEXR, "Y" for the scalar
+    ones) read back equal to the film's resolved images."""
+    from imgio import read_exr, read_pfm
+    hs = HostScene.generate("cbox", "width=24;height=20;spp=2")
+    it = Integrator(device, "path", max_depth=4, passes_per_callback=2)
+    it.set_aovs(True)
+    it.render(hs)
+    img, _ = it.film()
+    av = it.aovs()
+    it.save(str(tmp_path / "beauty.exr"))
+    _, ch = read_exr((tmp_path / "beauty.exr").read_bytes())
+    assert np.array_equal(np.stack([ch["R"], ch["G"], ch["B"]], -1), img)
+    for kind in ("albedo", "normal", "depth", "roughness", "metallic"):
+        it.save_aov(kind, str(tmp_path / f"{kind}.exr"))
+        it.save_aov(kind, str(tmp_path / f"{kind}.pfm"))
+        _, ch = read_exr((tmp_path / f"{kind}.exr").read_bytes())
+        want = av[kind] if av[kind].ndim == 3 else av[kind][..., None]
+        got = np.stack([ch["R"], ch["G"], ch["B"]], -1) if "R" in ch else ch["Y"][..., None]
+        assert np.array_equal(got, want), kind
+        assert np.array_equal(read_pfm((tmp_path / f"{kind}.pfm").read_bytes()), want), kind
+    with pytest.raises(GndError):
+        it.save_aov("emission", str(tmp_path / "x.exr"))
+

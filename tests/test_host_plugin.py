@@ -1,6 +1,6 @@
 """CPU tests of the host plugin layer: integrator registry, the host
 IndependentSampler (equal to the oracle's restatement, itself pinned by the
-published PCG32 / SplitMix64 outputs) and the PNG / PFM film writers."""
+published PCG32 / SplitMix64 outputs) and the PNG / PFM / EXR film writers."""
 import ctypes as C
 import struct
 import zlib
@@ -10,6 +10,7 @@ import pytest
 
 from gianduia_amd import _abi as A, integrator_names, sampler_floats
 import oracle as O
+from imgio import read_exr
 from oracle import fp
 
 
@@ -74,3 +75,17 @@ def test_png_decodes(tmp_path):
     assert np.all(rows[:, 0] == 0)
     pix = rows[:, 1:].reshape(7, 13, 3)
     assert np.abs(pix.astype(int) - _srgb8(img.astype(np.float64)).astype(int)).max() <= 1
+
+
+def test_exr_roundtrip(tmp_path):
+    """Film::saveEXR (R, G, B float channels, Image2D::saveEXR): every pixel
+    reads back exactly through an independent reader of the file layout."""
+    img = _image()
+    img[0, 0] = (np.inf, -1e-40, 1e-40)   # non-finite and denormal values survive
+    p = tmp_path / "f.exr"
+    assert A.load_library().gnd_host_image_save(str(p).encode(), img.ctypes.data_as(A.f32p), 13, 7) == 0
+    attrs, chans = read_exr(p.read_bytes())
+    assert sorted(chans) == ["B", "G", "R"]
+    back = np.stack([chans["R"], chans["G"], chans["B"]], axis=-1)
+    assert np.array_equal(back.view(np.uint32), img.view(np.uint32))
+
