@@ -340,6 +340,7 @@ def main():
                    "integrator": "volpath" if job.integrator == A.GND_INTEGRATOR_VOLPATH else "path",
                    "aovs": bool(a.aov)},
         "mrays_per_s": round(mrays, 2),
+        "scene_build_ms": round(hs.build_ms, 1),
         "trace_ms_per_step": round(sum(s["ms_trace"] for s in per_step) / a.steps, 3),
         "shadow_ms_per_step": round(sum(s["ms_shadow"] for s in per_step) / a.steps, 3),
         "shade_ms_per_step": round(sum(s["ms_shade"] for s in per_step) / a.steps, 3),

@@ -1,12 +1,20 @@
 // bvh_build.cpp — binned-SAH BVH build, 4-wide collapse and triangle packing
 // on the host (see bvh_build.h for the reference functions it reproduces).
 #include "bvh_build.h"
+#include "task_pool.h"
 
+#include <algorithm>
+#include <array>
 #include <atomic>
-#include <future>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <functional>
+#include <limits>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
+#include <thread>
 
 namespace gnd {
 
@@ -14,41 +22,38 @@ namespace {
 
 constexpr int kBins = 12;          // nBuckets, bvhBuilder.cpp:79
 constexpr int kMaxDepth = 63;      // leaf forced at depth 63, bvhBuilder.cpp:65
-constexpr int kParallelMin = 1 << 15;
+constexpr int kTaskMin = 1 << 12;  // subtrees at least this big become pool tasks
 
+// every field is written when the node is made (slots are left uninitialised)
 struct TreeNode {
     Box box;
-    TreeNode* kid[2] = {nullptr, nullptr};
-    int axis = 0;
-    int start = 0;
-    int n = 0;  // > 0 for leaves
-    int subtreeSize = 1;
+    int kid[2];
+    int axis;
+    int start;
+    int n;  // > 0 for leaves
+    int subtreeSize;
 };
 
+// Binned SAH (bvhBuilder.cpp:59-182).  Independent subtrees are built as
+// pool tasks.  Nodes live in one array sized for the worst case (2N - 1): the
+// subtree over n primitives owns 2n - 1 consecutive slots (its root the
+// first, then the left and the right subtree's slots), so threads never share
+// a counter and a subtree's nodes stay together; the flattened tree is a
+// depth-first walk, so the slot numbers do not matter.  Splitting the passes
+// of the top nodes over threads as well (bounds, binning, a rank-based
+// parallel form of the partition) measured slower than subtree tasks alone.
 class SahBuilder {
 public:
-    SahBuilder(std::vector<PrimRef>& refs, int threads) : refs_(refs), budget_(threads > 1 ? threads - 1 : 0) {}
-
-    TreeNode* run() { return build(0, (int)refs_.size(), 0); }
-
-    ~SahBuilder() {
-        for (auto& p : pools_) delete[] p.first;
-    }
+    SahBuilder(std::vector<PrimRef>& refs, TaskPool* pool)
+        : refs_(refs), pool_(pool), nodes_(new TreeNode[refs.empty() ? 1 : 2 * refs.size() - 1]) {}
+    int run() { return build(0, (int)refs_.size(), 0, 0); }
+    const TreeNode& node(int i) const { return nodes_[i]; }
+    TaskPool* pool() const { return pool_; }
 
 private:
     std::vector<PrimRef>& refs_;
-    std::atomic<int> budget_;
-    std::mutex poolMutex_;
-    std::vector<std::pair<TreeNode*, size_t>> pools_;
-
-    TreeNode* newNode() {
-        // small per-call allocations are fine at these node counts; keep
-        // ownership in pools_ so the tree is freed in one place
-        TreeNode* n = new TreeNode[1];
-        std::lock_guard<std::mutex> g(poolMutex_);
-        pools_.push_back({n, 1});
-        return n;
-    }
+    TaskPool* pool_;
+    std::unique_ptr<TreeNode[]> nodes_;
 
     // Bucket index of a centroid: int(12 * (c - lo) / extent), 12 -> 11
     static inline int binOf(const Vec3& c, int dim, float lo, float extent) {
@@ -56,24 +61,47 @@ private:
         return b == kBins ? kBins - 1 : b;
     }
 
-    TreeNode* makeLeaf(TreeNode* node, int start, int n, const Box& box) {
-        node->start = start;
-        node->n = n;
-        node->box = box;
-        return node;
+    // std::partition on bidirectional iterators (the reference's partition)
+    template <class G>
+    int partition(int start, int end, G&& goesLeft) {
+        int first = start, last = end;
+        for (;;) {
+            for (;;) {
+                if (first == last) return first;
+                if (goesLeft(refs_[first])) ++first;
+                else break;
+            }
+            --last;
+            for (;;) {
+                if (first == last) return first;
+                if (!goesLeft(refs_[last])) --last;
+                else break;
+            }
+            std::swap(refs_[first], refs_[last]);
+            ++first;
+        }
     }
 
-    TreeNode* build(int start, int end, int depth) {
-        TreeNode* node = newNode();
+    int makeLeaf(int id, int start, int n, const Box& box) {
+        TreeNode& t = nodes_[id];
+        t.box = box;
+        t.kid[0] = t.kid[1] = -1;
+        t.axis = 0;
+        t.start = start;
+        t.n = n;
+        t.subtreeSize = 1;
+        return id;
+    }
+
+    int build(int start, int end, int depth, int id) {
         const int n = end - start;
         Box box;
         for (int i = start; i < end; ++i) box.grow(refs_[i].box);
-        if (n == 1 || depth == kMaxDepth) return makeLeaf(node, start, n, box);
-
+        if (n == 1 || depth == kMaxDepth) return makeLeaf(id, start, n, box);
         Box cbox;
         for (int i = start; i < end; ++i) cbox.grow(refs_[i].centroid);
         const int dim = cbox.widestAxis();
-        if (cbox.hi[dim] == cbox.lo[dim]) return makeLeaf(node, start, n, box);
+        if (cbox.hi[dim] == cbox.lo[dim]) return makeLeaf(id, start, n, box);
 
         float extent = cbox.hi[dim] - cbox.lo[dim];
         if (extent == 0) extent = 1.0f;
@@ -82,7 +110,7 @@ private:
         int cnt[kBins] = {0};
         Box bb[kBins];
         for (int i = start; i < end; ++i) {
-            int b = binOf(refs_[i].centroid, dim, lo, extent);
+            const int b = binOf(refs_[i].centroid, dim, lo, extent);
             cnt[b]++;
             bb[b].grow(refs_[i].box);
         }
@@ -116,87 +144,80 @@ private:
             float cost = splitCost(i);
             if (cost < best) { best = cost; bestSplit = i; }
         }
-        if (!(n > 4 || best < (float)n)) return makeLeaf(node, start, n, box);
+        if (!(n > 4 || best < (float)n)) return makeLeaf(id, start, n, box);
 
-        // Hoare partition (std::partition on bidirectional iterators)
-        int first = start, last = end;
-        auto goesLeft = [&](const PrimRef& r) { return binOf(r.centroid, dim, lo, extent) <= bestSplit; };
-        for (;;) {
-            for (;;) {
-                if (first == last) goto partitioned;
-                if (goesLeft(refs_[first])) ++first;
-                else break;
-            }
-            --last;
-            for (;;) {
-                if (first == last) goto partitioned;
-                if (!goesLeft(refs_[last])) --last;
-                else break;
-            }
-            std::swap(refs_[first], refs_[last]);
-            ++first;
-        }
-    partitioned:
-        const int mid = first;
+        const int mid = partition(start, end, [&](const PrimRef& r) {
+            return binOf(r.centroid, dim, lo, extent) <= bestSplit;
+        });
 
-        TreeNode *k0 = nullptr, *k1 = nullptr;
-        bool spawned = false;
-        if (n > kParallelMin) {
-            int b = budget_.load();
-            while (b > 0 && !budget_.compare_exchange_weak(b, b - 1)) {
-            }
-            if (b > 0) {
-                spawned = true;
-                auto fut = std::async(std::launch::async, [&] { return build(start, mid, depth + 1); });
-                k1 = build(mid, end, depth + 1);
-                k0 = fut.get();
-                budget_.fetch_add(1);
-            }
+        int k0 = -1, k1 = -1;
+        const int id0 = id + 1, id1 = id + 2 * (mid - start);
+        if (pool_ && mid - start >= kTaskMin && end - mid >= kTaskMin) {
+            std::atomic<int> pending{1};
+            pool_->submit([&] {
+                k0 = build(start, mid, depth + 1, id0);
+                pending.fetch_sub(1, std::memory_order_release);
+            });
+            k1 = build(mid, end, depth + 1, id1);
+            pool_->waitFor(pending);
+        } else {
+            k0 = build(start, mid, depth + 1, id0);
+            k1 = build(mid, end, depth + 1, id1);
         }
-        if (!spawned) {
-            k0 = build(start, mid, depth + 1);
-            k1 = build(mid, end, depth + 1);
-        }
-        node->kid[0] = k0;
-        node->kid[1] = k1;
-        node->box = k0->box;
-        node->box.grow(k1->box);
-        node->axis = dim;
-        node->n = 0;
-        node->subtreeSize = 1 + k0->subtreeSize + k1->subtreeSize;
-        return node;
+        TreeNode& t = nodes_[id];
+        t.kid[0] = k0;
+        t.kid[1] = k1;
+        t.box = nodes_[k0].box;
+        t.box.grow(nodes_[k1].box);
+        t.axis = dim;
+        t.start = 0;
+        t.n = 0;
+        t.subtreeSize = 1 + nodes_[k0].subtreeSize + nodes_[k1].subtreeSize;
+        return id;
     }
 };
 
-// depth-first flatten, left child at parent + 1 (bvhBuilder.cpp:29-44)
-uint32_t flatten(const TreeNode* t, std::vector<BinaryNode>& out, uint32_t& next) {
-    uint32_t me = next++;
-    BinaryNode& bn = out[me];
-    bn.box = t->box;
-    if (t->n > 0) {
-        bn.offset = (uint32_t)t->start;
-        bn.count = (uint16_t)t->n;
+// depth-first flatten, left child at parent + 1 (bvhBuilder.cpp:29-44): the
+// node of a subtree goes to `pos`, its left subtree from pos + 1, its right
+// subtree after the left one; big subtrees are written as pool tasks
+void flatten(const SahBuilder& b, int id, BinaryNode* out, uint32_t pos) {
+    const TreeNode& t = b.node(id);
+    BinaryNode& bn = out[pos];
+    bn.box = t.box;
+    if (t.n > 0) {
+        bn.offset = (uint32_t)t.start;
+        bn.count = (uint16_t)t.n;
         bn.axis = 0;
-        return me;
+        return;
     }
+    const uint32_t right = pos + 1 + (uint32_t)b.node(t.kid[0]).subtreeSize;
     bn.count = 0;
-    bn.axis = (uint8_t)t->axis;
-    flatten(t->kid[0], out, next);
-    uint32_t r = flatten(t->kid[1], out, next);
-    out[me].offset = r;
-    return me;
+    bn.axis = (uint8_t)t.axis;
+    bn.offset = right;
+    TaskPool* pool = b.pool();
+    if (pool && b.node(t.kid[0]).subtreeSize >= kTaskMin && b.node(t.kid[1]).subtreeSize >= kTaskMin) {
+        std::atomic<int> pending{1};
+        pool->submit([&] {
+            flatten(b, t.kid[0], out, pos + 1);
+            pending.fetch_sub(1, std::memory_order_release);
+        });
+        flatten(b, t.kid[1], out, right);
+        pool->waitFor(pending);
+    } else {
+        flatten(b, t.kid[0], out, pos + 1);
+        flatten(b, t.kid[1], out, right);
+    }
 }
 
 }  // namespace
 
-BinaryBVH buildBinary(std::vector<PrimRef>& refs, int threads) {
+BinaryBVH buildBinary(std::vector<PrimRef>& refs, TaskPool* pool) {
     BinaryBVH out;
     if (refs.empty()) return out;
-    SahBuilder b(refs, threads);
-    TreeNode* root = b.run();
-    out.nodes.resize(root->subtreeSize);
-    uint32_t next = 0;
-    flatten(root, out.nodes, next);
+    SahBuilder b(refs, refs.size() >= (size_t)2 * kTaskMin ? pool : nullptr);
+    const int root = b.run();
+    out.nodes.resize(b.node(root).subtreeSize);
+    flatten(b, root, out.nodes.data(), 0);
     out.order.resize(refs.size());
     // leaves own contiguous, depth-first ordered ranges of the partitioned
     // array, so slot i holds refs[i]
@@ -205,42 +226,102 @@ BinaryBVH buildBinary(std::vector<PrimRef>& refs, int threads) {
 }
 
 namespace {
-uint32_t collapseNode(const std::vector<BinaryNode>& bin, std::vector<gnd_bvh4_node>& out, uint32_t root) {
-    uint32_t me = (uint32_t)out.size();
-    out.emplace_back();
-    uint32_t cand[4];
+// Greedy largest-area collapse (bvhBuilder.cpp:184-262).  A wide node's
+// children come from expanding its binary root: repeatedly replace the inner
+// candidate with the largest box area by its two children, up to 4.  In the
+// binary array (depth-first, left child at e + 1, right child at offset) the
+// subtree of node e spans [e, end), so every candidate carries its end.
+struct Cand {
+    uint32_t node, end;
+};
+int expand(const std::vector<BinaryNode>& bin, Cand root, Cand* cand) {
     int nc = 1;
     cand[0] = root;
     while (nc < 4) {
         int best = -1;
         float bestArea = -1.0f;
         for (int i = 0; i < nc; ++i) {
-            const BinaryNode& c = bin[cand[i]];
+            const BinaryNode& c = bin[cand[i].node];
             if (c.count == 0) {
                 float a = c.box.area();
                 if (a > bestArea) { bestArea = a; best = i; }
             }
         }
         if (best < 0) break;
-        uint32_t e = cand[best];
+        const Cand e = cand[best];
         for (int i = best; i + 1 < nc; ++i) cand[i] = cand[i + 1];
-        cand[nc - 1] = e + 1;
-        cand[nc] = bin[e].offset;
+        cand[nc - 1] = {e.node + 1, bin[e.node].offset};
+        cand[nc] = {bin[e.node].offset, e.end};
         ++nc;
     }
+    return nc;
+}
+
+constexpr uint32_t kCollapseTask = 1 << 14;  // binary subtrees at least this big become tasks
+
+// Wide nodes of the collapsed subtree of each wide root (indexed by binary
+// node), so the write pass can place subtrees at their pre-order offsets.
+uint32_t countWide(const std::vector<BinaryNode>& bin, Cand r, std::vector<uint32_t>& count, TaskPool* pool) {
+    Cand cand[4];
+    const int nc = expand(bin, r, cand);
+    uint32_t total = 1;
+    std::atomic<int> pending{0};
+    uint32_t sub[4] = {0, 0, 0, 0};
+    for (int i = 0; i < nc; ++i) {
+        if (bin[cand[i].node].count > 0) continue;
+        if (pool && cand[i].end - cand[i].node >= kCollapseTask) {
+            pending.fetch_add(1, std::memory_order_relaxed);
+            pool->submit([&, i] {
+                sub[i] = countWide(bin, cand[i], count, pool);
+                pending.fetch_sub(1, std::memory_order_release);
+            });
+        } else {
+            sub[i] = countWide(bin, cand[i], count, nullptr);
+        }
+    }
+    if (pool) pool->waitFor(pending);
+    for (int i = 0; i < nc; ++i) total += sub[i];
+    count[r.node] = total;
+    return total;
+}
+
+void writeWide(const std::vector<BinaryNode>& bin, Cand r, uint32_t me, const std::vector<uint32_t>& count,
+               gnd_bvh4_node* out, TaskPool* pool) {
+    Cand cand[4];
+    const int nc = expand(bin, r, cand);
     uint8_t type[4];
     uint32_t off[4], cnt[4];
+    uint32_t next = me + 1;
+    std::atomic<int> pending{0};
     for (int i = 0; i < nc; ++i) {
-        const BinaryNode& c = bin[cand[i]];
-        if (c.count > 0) { type[i] = 2; off[i] = c.offset; cnt[i] = c.count; }
-        else { type[i] = 1; off[i] = collapseNode(bin, out, cand[i]); cnt[i] = 0; }
+        const BinaryNode& c = bin[cand[i].node];
+        if (c.count > 0) {
+            type[i] = 2;
+            off[i] = c.offset;
+            cnt[i] = c.count;
+            continue;
+        }
+        type[i] = 1;
+        off[i] = next;
+        cnt[i] = 0;
+        const uint32_t at = next;
+        next += count[cand[i].node];
+        if (pool && cand[i].end - cand[i].node >= kCollapseTask) {
+            pending.fetch_add(1, std::memory_order_relaxed);
+            pool->submit([&, i, at] {
+                writeWide(bin, cand[i], at, count, out, pool);
+                pending.fetch_sub(1, std::memory_order_release);
+            });
+        } else {
+            writeWide(bin, cand[i], at, count, out, nullptr);
+        }
     }
     gnd_bvh4_node& n4 = out[me];
     std::memset(&n4, 0, sizeof(n4));
     const float inf = std::numeric_limits<float>::infinity();
     for (int i = 0; i < 4; ++i) {
         if (i < nc) {
-            const Box& b = bin[cand[i]].box;
+            const Box& b = bin[cand[i].node].box;
             n4.child_type[i] = type[i];
             n4.offset[i] = off[i];
             n4.count[i] = cnt[i];
@@ -252,52 +333,80 @@ uint32_t collapseNode(const std::vector<BinaryNode>& bin, std::vector<gnd_bvh4_n
             n4.max_x[i] = n4.max_y[i] = n4.max_z[i] = -inf;
         }
     }
-    return me;
+    if (pool) pool->waitFor(pending);
 }
 }  // namespace
 
-std::vector<gnd_bvh4_node> collapseWide(const std::vector<BinaryNode>& nodes) {
+std::vector<gnd_bvh4_node> collapseWide(const std::vector<BinaryNode>& nodes, TaskPool* pool) {
     std::vector<gnd_bvh4_node> out;
     if (nodes.empty()) return out;
-    out.reserve(nodes.size() / 2 + 1);
-    collapseNode(nodes, out, 0);
+    const Cand root{0, (uint32_t)nodes.size()};
+    std::vector<uint32_t> count(nodes.size(), 0);
+    const uint32_t n = countWide(nodes, root, count, pool);
+    out.resize(n);
+    writeWide(nodes, root, 0, count, out.data(), pool);
     return out;
 }
 
-void buildMesh(const float* P, uint32_t ntris, int threads, MeshBuild& out) {
+// runs fn(lo, hi) over [0, n) on the pool, or inline
+template <class F>
+void forRange(TaskPool* pool, int64_t n, F&& fn) {
+    if (pool && n >= 4096) pool->parallelFor(n, fn);
+    else fn(0, n);
+}
+
+void buildMesh(const float* P, uint32_t ntris, TaskPool* pool, MeshBuild& out) {
     std::vector<PrimRef> refs(ntris);
-    for (uint32_t i = 0; i < ntris; ++i) {
-        Box b;
-        for (int k = 0; k < 3; ++k) b.grow(Vec3(P[9 * i + 3 * k], P[9 * i + 3 * k + 1], P[9 * i + 3 * k + 2]));
-        refs[i] = {b, b.center(), (int)i};
-    }
-    BinaryBVH bvh = buildBinary(refs, threads);
-    if (!bvh.nodes.empty()) out.bounds = bvh.nodes[0].box;
-    out.packs.clear();
-    for (auto& node : bvh.nodes) {
-        if (node.count == 0) continue;
-        const int first = (int)node.offset, count = node.count;
-        const uint32_t packBase = (uint32_t)out.packs.size();
-        const int np = (count + 3) / 4;
-        for (int p = 0; p < np; ++p) {
-            gnd_tri_pack4 pk;
-            std::memset(&pk, 0, sizeof(pk));
-            for (int l = 0; l < 4; ++l) {
-                int k = p * 4 + l;
-                if (k >= count) { pk.prim[l] = 0xFFFFFFFFu; continue; }
-                uint32_t t = (uint32_t)bvh.order[first + k];
-                pk.prim[l] = t;
-                const float* v = P + 9 * t;
-                pk.v0x[l] = v[0]; pk.v0y[l] = v[1]; pk.v0z[l] = v[2];
-                pk.v1x[l] = v[3]; pk.v1y[l] = v[4]; pk.v1z[l] = v[5];
-                pk.v2x[l] = v[6]; pk.v2y[l] = v[7]; pk.v2z[l] = v[8];
-            }
-            out.packs.push_back(pk);
+    forRange(pool, ntris, [&](int64_t lo, int64_t hi) {
+        for (int64_t i = lo; i < hi; ++i) {
+            Box b;
+            for (int k = 0; k < 3; ++k) b.grow(Vec3(P[9 * i + 3 * k], P[9 * i + 3 * k + 1], P[9 * i + 3 * k + 2]));
+            refs[i] = {b, b.center(), (int)i};
         }
-        node.offset = packBase;
-        node.count = (uint16_t)np;
+    });
+    BinaryBVH bvh = buildBinary(refs, pool);
+    if (!bvh.nodes.empty()) out.bounds = bvh.nodes[0].box;
+    // packs of each leaf in node order: offsets by a prefix sum, then the
+    // (gather-bound) packing in parallel
+    std::vector<uint32_t> leaves, packBase;
+    uint32_t npacks = 0;
+    for (uint32_t i = 0; i < (uint32_t)bvh.nodes.size(); ++i) {
+        if (bvh.nodes[i].count == 0) continue;
+        leaves.push_back(i);
+        packBase.push_back(npacks);
+        npacks += (bvh.nodes[i].count + 3u) / 4u;
     }
-    out.nodes = collapseWide(bvh.nodes);
+    out.packs.clear();
+    out.packs.resize(npacks);  // every lane is written below
+    forRange(pool, (int64_t)leaves.size(), [&](int64_t lo, int64_t hi) {
+        for (int64_t li = lo; li < hi; ++li) {
+            BinaryNode& node = bvh.nodes[leaves[li]];
+            const int first = (int)node.offset, count = node.count;
+            const int np = (count + 3) / 4;
+            for (int p = 0; p < np; ++p) {
+                gnd_tri_pack4& pk = out.packs[packBase[li] + p];
+                for (int l = 0; l < 4; ++l) {
+                    int k = p * 4 + l;
+                    if (k >= count) {  // padding lane: zero vertices, no primitive
+                        pk.prim[l] = 0xFFFFFFFFu;
+                        pk.v0x[l] = pk.v0y[l] = pk.v0z[l] = 0.0f;
+                        pk.v1x[l] = pk.v1y[l] = pk.v1z[l] = 0.0f;
+                        pk.v2x[l] = pk.v2y[l] = pk.v2z[l] = 0.0f;
+                        continue;
+                    }
+                    uint32_t t = (uint32_t)bvh.order[first + k];
+                    pk.prim[l] = t;
+                    const float* v = P + 9 * t;
+                    pk.v0x[l] = v[0]; pk.v0y[l] = v[1]; pk.v0z[l] = v[2];
+                    pk.v1x[l] = v[3]; pk.v1y[l] = v[4]; pk.v1z[l] = v[5];
+                    pk.v2x[l] = v[6]; pk.v2y[l] = v[7]; pk.v2z[l] = v[8];
+                }
+            }
+            node.offset = packBase[li];
+            node.count = (uint16_t)np;
+        }
+    });
+    out.nodes = collapseWide(bvh.nodes, pool);
     out.cdf.assign(ntris + 1, 0.0f);
     float total = 0.0f;
     for (uint32_t i = 0; i < ntris; ++i) {

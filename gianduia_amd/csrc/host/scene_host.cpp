@@ -4,8 +4,12 @@
 
 #include <atomic>
 #include <chrono>
+#include <cstring>
+#include <memory>
 #include <stdexcept>
 #include <thread>
+
+#include "task_pool.h"
 
 namespace gnd {
 
@@ -178,37 +182,64 @@ const gnd_scene_desc& HostScene::desc() {
 
 // Corner normals as Mesh::loadOBJ leaves them (mesh.cpp:163-228): given
 // normals are normalized, missing ones are generated per shared vertex.
-static void cornerNormals(const MeshData& m, float* out /* 9 per tri, stride 24 */) {
+// runs fn(lo, hi) over [0, n) on the pool, or inline
+template <class F>
+static void forRange(TaskPool* pool, int64_t n, F&& fn) {
+    if (pool && n >= 4096) pool->parallelFor(n, fn);
+    else fn(0, n);
+}
+
+// The smooth normals are sums of the area-weighted face normals of each
+// vertex in face order (as a sequential scatter would add them); a
+// vertex -> incident-corner list kept in face order lets every vertex sum its
+// own faces in that order, in parallel, with the same bits.
+static void cornerNormals(const MeshData& m, float* out /* 9 per tri, stride 24 */, TaskPool* pool) {
     const uint32_t nt = m.ntris();
     if (!m.normals.empty()) {
-        for (uint32_t c = 0; c < 3 * nt; ++c) {
-            Vec3 n = normalize(Vec3(m.normals[3 * c], m.normals[3 * c + 1], m.normals[3 * c + 2]));
+        forRange(pool, 3 * (int64_t)nt, [&](int64_t lo, int64_t hi) {
+            for (int64_t c = lo; c < hi; ++c) {
+                Vec3 n = normalize(Vec3(m.normals[3 * c], m.normals[3 * c + 1], m.normals[3 * c + 2]));
+                float* o = out + 24 * (c / 3) + 9 + 3 * (c % 3);
+                o[0] = n.x; o[1] = n.y; o[2] = n.z;
+            }
+        });
+        return;
+    }
+    auto vid = [&](uint32_t c) { return m.vertexIds.empty() ? c : m.vertexIds[c]; };
+    uint32_t nv = 0;
+    for (uint32_t c = 0; c < 3 * nt; ++c) nv = std::max(nv, vid(c) + 1);
+    std::vector<Vec3> w(nt);
+    forRange(pool, nt, [&](int64_t lo, int64_t hi) {
+        for (int64_t f = lo; f < hi; ++f) {
+            const float* p = &m.positions[9 * f];
+            Vec3 p0(p[0], p[1], p[2]), p1(p[3], p[4], p[5]), p2(p[6], p[7], p[8]);
+            w[f] = cross(p1 - p0, p2 - p0);
+        }
+    });
+    std::vector<uint32_t> first(nv + 1, 0), faces(3 * (size_t)nt);
+    for (uint32_t c = 0; c < 3 * nt; ++c) first[vid(c) + 1]++;
+    for (uint32_t v = 0; v < nv; ++v) first[v + 1] += first[v];
+    {
+        std::vector<uint32_t> fill(first.begin(), first.end() - 1);
+        for (uint32_t c = 0; c < 3 * nt; ++c) faces[fill[vid(c)]++] = c / 3;
+    }
+    std::vector<Vec3> acc(nv);
+    forRange(pool, nv, [&](int64_t lo, int64_t hi) {
+        for (int64_t v = lo; v < hi; ++v) {
+            Vec3 a(0, 0, 0);
+            for (uint32_t i = first[v]; i < first[v + 1]; ++i) a = a + w[faces[i]];
+            float len = length(a);
+            if (len > 0.0f) a = a / len;
+            acc[v] = a;
+        }
+    });
+    forRange(pool, 3 * (int64_t)nt, [&](int64_t lo, int64_t hi) {
+        for (int64_t c = lo; c < hi; ++c) {
+            const Vec3& n = acc[vid((uint32_t)c)];
             float* o = out + 24 * (c / 3) + 9 + 3 * (c % 3);
             o[0] = n.x; o[1] = n.y; o[2] = n.z;
         }
-        return;
-    }
-    uint32_t nv = 0;
-    for (uint32_t c = 0; c < 3 * nt; ++c) nv = std::max(nv, (m.vertexIds.empty() ? c : m.vertexIds[c]) + 1);
-    std::vector<Vec3> acc(nv, Vec3(0, 0, 0));
-    for (uint32_t f = 0; f < nt; ++f) {
-        const float* p = &m.positions[9 * f];
-        Vec3 p0(p[0], p[1], p[2]), p1(p[3], p[4], p[5]), p2(p[6], p[7], p[8]);
-        Vec3 w = cross(p1 - p0, p2 - p0);
-        for (int k = 0; k < 3; ++k) {
-            uint32_t id = m.vertexIds.empty() ? 3 * f + k : m.vertexIds[3 * f + k];
-            acc[id] = acc[id] + w;
-        }
-    }
-    for (auto& a : acc) {
-        float len = length(a);
-        if (len > 0.0f) a = a / len;
-    }
-    for (uint32_t c = 0; c < 3 * nt; ++c) {
-        const Vec3& n = acc[m.vertexIds.empty() ? c : m.vertexIds[c]];
-        float* o = out + 24 * (c / 3) + 9 + 3 * (c % 3);
-        o[0] = n.x; o[1] = n.y; o[2] = n.z;
-    }
+    });
 }
 
 const gnd_hip_scene_arrays& HostScene::build(int threads, double* ms) {
@@ -226,55 +257,70 @@ const gnd_hip_scene_arrays& HostScene::build(int threads, double* ms) {
         if (!anyEmitter) throw std::runtime_error("scene has no emitter");
         const size_t nm = meshes.size();
         std::vector<MeshBuild> mb(nm);
-        // BLAS builds: big meshes use the thread budget internally, small ones run side by side
-        {
-            std::vector<std::thread> pool;
-            std::atomic<size_t> next{0};
-            int workers = std::min<int>(threads, (int)std::max<size_t>(nm, 1));
-            auto work = [&]() {
-                for (;;) {
-                    size_t i = next.fetch_add(1);
-                    if (i >= nm) break;
-                    int inner = meshes[i].ntris() > 100000 ? threads : 1;
-                    buildMesh(meshes[i].positions.data(), meshes[i].ntris(), inner, mb[i]);
-                }
-            };
-            for (int w = 1; w < workers; ++w) pool.emplace_back(work);
-            work();
-            for (auto& t : pool) t.join();
+        // one pool for the whole build: the BLAS builds are tasks, and big
+        // meshes split their own work into tasks of the same pool
+        std::unique_ptr<TaskPool> pool;
+        if (threads > 1) pool = std::make_unique<TaskPool>(threads - 1);
+        if (pool) {
+            std::atomic<int> pending{(int)nm};
+            for (size_t i = 0; i < nm; ++i)
+                pool->submit([&, i] {
+                    buildMesh(meshes[i].positions.data(), meshes[i].ntris(), pool.get(), mb[i]);
+                    pending.fetch_sub(1, std::memory_order_release);
+                });
+            pool->waitFor(pending);
+        } else {
+            for (size_t i = 0; i < nm; ++i) buildMesh(meshes[i].positions.data(), meshes[i].ntris(), nullptr, mb[i]);
         }
+        // the scene-wide arrays are sized once (no zero fill: every element is
+        // written below) and each mesh's part is copied in parallel
         meshInfo_.assign(nm, gnd_mesh_info{});
+        size_t nNodes = 0, nPacks = 0, nCdf = 0, nTris = 0;
+        for (size_t i = 0; i < nm; ++i) {
+            gnd_mesh_info& mi = meshInfo_[i];
+            mi.node_base = (uint32_t)nNodes;
+            mi.pack_base = (uint32_t)nPacks;
+            mi.cdf_base = (uint32_t)nCdf;
+            mi.tri_base = (uint32_t)nTris;
+            nNodes += mb[i].nodes.size();
+            nPacks += mb[i].packs.size();
+            nCdf += mb[i].cdf.size();
+            nTris += meshes[i].ntris();
+        }
         blas_.clear();
         packs_.clear();
         cdf_.clear();
-        uint32_t ntot = 0;
-        for (size_t i = 0; i < nm; ++i) ntot += meshes[i].ntris();
-        shading_.assign((size_t)ntot * 24, 0.0f);
-        uint32_t triBase = 0;
+        shading_.clear();
+        blas_.resize(nNodes);
+        packs_.resize(nPacks);
+        cdf_.resize(nCdf);
+        shading_.resize(nTris * 24);
         for (size_t i = 0; i < nm; ++i) {
             gnd_mesh_info& mi = meshInfo_[i];
-            mi.node_base = (uint32_t)blas_.size();
             mi.n_nodes = (uint32_t)mb[i].nodes.size();
-            mi.pack_base = (uint32_t)packs_.size();
             mi.n_packs = (uint32_t)mb[i].packs.size();
-            mi.tri_base = triBase;
             mi.n_tris = meshes[i].ntris();
-            mi.cdf_base = (uint32_t)cdf_.size();
             mi.has_uv = meshes[i].uvs.empty() ? 0u : 1u;
             mi.total_area = mb[i].totalArea;
             mi.bounds_min[0] = mb[i].bounds.lo.x; mi.bounds_min[1] = mb[i].bounds.lo.y; mi.bounds_min[2] = mb[i].bounds.lo.z;
             mi.bounds_max[0] = mb[i].bounds.hi.x; mi.bounds_max[1] = mb[i].bounds.hi.y; mi.bounds_max[2] = mb[i].bounds.hi.z;
-            blas_.insert(blas_.end(), mb[i].nodes.begin(), mb[i].nodes.end());
-            packs_.insert(packs_.end(), mb[i].packs.begin(), mb[i].packs.end());
-            cdf_.insert(cdf_.end(), mb[i].cdf.begin(), mb[i].cdf.end());
-            float* sh = &shading_[(size_t)triBase * 24];
+            forRange(pool.get(), (int64_t)mb[i].nodes.size(), [&](int64_t lo, int64_t hi) {
+                std::copy(mb[i].nodes.begin() + lo, mb[i].nodes.begin() + hi, blas_.begin() + mi.node_base + lo);
+            });
+            forRange(pool.get(), (int64_t)mb[i].packs.size(), [&](int64_t lo, int64_t hi) {
+                std::copy(mb[i].packs.begin() + lo, mb[i].packs.begin() + hi, packs_.begin() + mi.pack_base + lo);
+            });
+            std::copy(mb[i].cdf.begin(), mb[i].cdf.end(), cdf_.begin() + mi.cdf_base);
+            float* sh = &shading_[(size_t)mi.tri_base * 24];
             const MeshData& m = meshes[i];
-            for (uint32_t t = 0; t < m.ntris(); ++t) {
-                std::memcpy(sh + 24 * t, &m.positions[9 * t], 9 * sizeof(float));
-                if (!m.uvs.empty()) std::memcpy(sh + 24 * t + 18, &m.uvs[6 * t], 6 * sizeof(float));
-            }
-            cornerNormals(m, sh);
-            triBase += m.ntris();
+            forRange(pool.get(), m.ntris(), [&](int64_t lo, int64_t hi) {
+                for (int64_t t = lo; t < hi; ++t) {
+                    std::memcpy(sh + 24 * t, &m.positions[9 * t], 9 * sizeof(float));
+                    if (!m.uvs.empty()) std::memcpy(sh + 24 * t + 18, &m.uvs[6 * t], 6 * sizeof(float));
+                    else std::memset(sh + 24 * t + 18, 0, 6 * sizeof(float));
+                }
+            });
+            cornerNormals(m, sh, pool.get());
         }
         // TLAS over primitive world bounds (bvh.cpp:10-36, primitive.cpp:74-76)
         std::vector<PrimRef> refs(prims.size());
@@ -294,7 +340,7 @@ const gnd_hip_scene_arrays& HostScene::build(int threads, double* ms) {
             for (int c = 1; c < 8; ++c) wb.grow(xformPoint(M, sb.corner(c)));
             refs[i] = {wb, wb.lo * 0.5f + wb.hi * 0.5f, (int)i};
         }
-        BinaryBVH tl = buildBinary(refs, 1);
+        BinaryBVH tl = buildBinary(refs, nullptr);
         tlasPrims_.assign(tl.order.begin(), tl.order.end());
         tlas_ = collapseWide(tl.nodes);
         built_ = true;

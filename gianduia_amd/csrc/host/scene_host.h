@@ -12,6 +12,7 @@
 
 namespace gnd {
 
+
 struct MeshData {
     std::vector<float> positions;    // 9 per triangle
     std::vector<float> normals;      // 9 per triangle or empty
@@ -56,10 +57,10 @@ private:
     std::vector<gnd_bvh4_node> tlas_;
     std::vector<int32_t> tlasPrims_;
     std::vector<gnd_mesh_info> meshInfo_;
-    std::vector<gnd_bvh4_node> blas_;
-    std::vector<gnd_tri_pack4> packs_;
-    std::vector<float> shading_;
-    std::vector<float> cdf_;
+    BuildVector<gnd_bvh4_node> blas_;
+    BuildVector<gnd_tri_pack4> packs_;
+    BuildVector<float> shading_;
+    BuildVector<float> cdf_;
     gnd_hip_scene_arrays arrays_{};
 };
 

@@ -75,6 +75,53 @@ def test_procedural_100k_build_matches_oracle():
     _check_same(hs)
 
 
+def _all_bytes(a):
+    """Every array of a built scene as bytes (nodes, primitive order, mesh
+    table, packs, shading data, CDF)."""
+    def raw(ptr, n, size):
+        return C.string_at(ptr, n * size) if n else b""
+    return [raw(a.tlas_nodes, a.n_tlas_nodes, 144), raw(a.tlas_prims, a.n_tlas_prims, 4),
+            raw(a.meshes, a.n_meshes, C.sizeof(A.gnd_mesh_info)), raw(a.blas_nodes, a.n_blas_nodes, 144),
+            raw(a.packs, a.n_packs, 160), raw(a.shading, a.n_tris * 24, 4), raw(a.cdf, a.n_cdf, 4)]
+
+
+def test_threaded_build_equals_one_thread():
+    """The task-parallel build (subtree tasks, parallel packing, per-vertex
+    normal gathers) gives the same bytes as one thread, shading data included."""
+    params = "level=6;instances=16"
+    h1, h8 = HostScene.generate("scaled", params), HostScene.generate("scaled", params)  # alive while read
+    one = _all_bytes(h1.build(threads=1).contents)
+    many = _all_bytes(h8.build(threads=8).contents)
+    for k, (x, y) in enumerate(zip(one, many)):
+        assert x == y, f"array {k} differs between 1 and 8 threads"
+
+
+def test_smooth_normals_sum_faces_in_order():
+    """Mesh::activate's smooth normals: per vertex the area-weighted face
+    normals added in face order, then normalised.  The builder gathers each
+    vertex's faces from a face-ordered list; this checks it against the
+    sequential scatter (np.add.at adds in index order) in float32."""
+    hs = HostScene.generate("scaled", "level=4;instances=2")
+    a = hs.build(threads=8).contents
+    m = hs.desc().contents.meshes[0]
+    nt = m.ntris
+    assert m.vertex_ids and not m.normals
+    P = np.ctypeslib.as_array(m.positions, shape=(nt * 9,)).reshape(nt, 3, 3).astype(np.float32)
+    vid = np.ctypeslib.as_array(m.vertex_ids, shape=(nt * 3,)).astype(np.int64)
+    e1, e2 = P[:, 1] - P[:, 0], P[:, 2] - P[:, 0]
+    w = np.stack([e1[:, 1] * e2[:, 2] - e2[:, 1] * e1[:, 2], e1[:, 2] * e2[:, 0] - e2[:, 2] * e1[:, 0],
+                  e1[:, 0] * e2[:, 1] - e2[:, 0] * e1[:, 1]], -1).astype(np.float32)
+    acc = np.zeros((vid.max() + 1, 3), np.float32)
+    np.add.at(acc, vid, np.repeat(w, 3, axis=0))
+    d = (acc[:, 0] * acc[:, 0] + acc[:, 1] * acc[:, 1]) + acc[:, 2] * acc[:, 2]
+    ln = np.sqrt(d.astype(np.float32)).astype(np.float32)
+    nrm = np.where(ln[:, None] > 0, acc / np.where(ln > 0, ln, 1)[:, None], acc).astype(np.float32)
+    base = a.meshes[0].tri_base
+    sh = np.ctypeslib.as_array(a.shading, shape=(a.n_tris * 24,)).reshape(-1, 24)[base:base + nt]
+    got = sh[:, 9:18].reshape(nt * 3, 3)
+    assert np.array_equal(got.view(np.uint32), nrm[vid].view(np.uint32))
+
+
 def test_bvh_invariants():
     hs = HostScene.from_desc(soup_scene(seed=9, n=3000).desc())
     a = _arrays(hs)
