@@ -212,7 +212,7 @@ def serve_workload(qps_levels=(5, 10, 20, 50, 100, 200), jobs_per_level=40):
             "data": "synthetic (procedural scenes and jobs, seeded)",
             "config": {"scenes": len(pool), "instanced_triangles": tris, "path_slots": 16_000_000,
                        "jobs_per_level": jobs_per_level, "width": [128, 1920], "spp": [4, 64]},
-            "scene_upload_s": round(upload_s, 2), "levels": levels,
+            "scene_build_ms": build_ms, "scene_upload_s": round(upload_s, 2), "levels": levels,
             "rays_closest": st["rays_closest"], "rays_shadow": st["rays_shadow"]}
 
 
