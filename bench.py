@@ -338,7 +338,7 @@ def main():
                    "width": cam.width, "height": cam.height, "spp": job.spp, "max_depth": job.max_depth,
                    "instanced_triangles": inst_tris,
                    "integrator": "volpath" if job.integrator == A.GND_INTEGRATOR_VOLPATH else "path",
-                   "aovs": bool(a.aov)},
+                   "aovs": bool(a.aov), **({"params": a.params} if a.params else {})},
         "mrays_per_s": round(mrays, 2),
         "scene_build_ms": round(hs.build_ms, 1),
         "trace_ms_per_step": round(sum(s["ms_trace"] for s in per_step) / a.steps, 3),

@@ -54,13 +54,23 @@ class gnd_camera_desc(C.Structure):
                 ("filter", C.c_int32), ("filter_radius", C.c_float * 2), ("filter_alpha", C.c_float)]
 
 
+GND_LIGHT_POINT, GND_LIGHT_SPOT, GND_LIGHT_DIRECTIONAL, GND_LIGHT_ENVIRONMENT = range(4)
+
+
+class gnd_light_desc(C.Structure):
+    _fields_ = [("type", C.c_int32), ("emitter_index", C.c_int32), ("position", C.c_float * 3),
+                ("direction", C.c_float * 3), ("power", C.c_float * 3), ("irradiance", C.c_float * 3),
+                ("spot_size", C.c_float), ("blend", C.c_float), ("strength", C.c_float),
+                ("env_width", C.c_int32), ("env_height", C.c_int32), ("env_rgb", f32p)]
+
+
 class gnd_scene_desc(C.Structure):
     _fields_ = [("nmeshes", C.c_uint32), ("meshes", C.POINTER(gnd_mesh_desc)),
                 ("nprims", C.c_uint32), ("prims", C.POINTER(gnd_primitive_desc)),
                 ("nmaterials", C.c_uint32), ("materials", C.POINTER(gnd_material_desc)),
                 ("nmedia", C.c_uint32), ("media", C.POINTER(gnd_medium_desc)),
                 ("camera", gnd_camera_desc), ("integrator", C.c_int32), ("max_depth", C.c_int32),
-                ("spp", C.c_uint32)]
+                ("spp", C.c_uint32), ("nlights", C.c_uint32), ("lights", C.POINTER(gnd_light_desc))]
 
 
 class gnd_ray_soa(C.Structure):
@@ -200,6 +210,7 @@ HOST_API = {
     "gnd_host_integrator_names": (C.c_int, [C.c_char_p, C.c_int]),
     "gnd_host_sampler_floats": (C.c_int, [C.c_uint64, C.c_int, f32p]),
     "gnd_host_image_save": (C.c_int, [C.c_char_p, f32p, C.c_int, C.c_int]),
+    "gnd_host_image_load": (C.c_int, [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), f32p, C.c_size_t]),
 }
 
 # gnd_host_progress_fn

@@ -472,6 +472,20 @@ def sampler_floats(sample_index: int, n: int) -> np.ndarray:
     return out
 
 
+def load_exr(path: str) -> np.ndarray:
+    """An OpenEXR image as (H, W, 3) float32 R, G, B, read like Film's EXR
+    constructor (film.cpp:32-48): a channel the file lacks reads as 0."""
+    w, h = C.c_int(), C.c_int()
+    rc = _lib().gnd_host_image_load(str(path).encode(), C.byref(w), C.byref(h), None, 0)
+    if rc != 0:
+        raise GndError(rc, host_error())
+    out = np.zeros((h.value, w.value, 3), np.float32)
+    rc = _lib().gnd_host_image_load(str(path).encode(), C.byref(w), C.byref(h), out.ctypes.data_as(A.f32p), out.size)
+    if rc != 0:
+        raise GndError(rc, host_error())
+    return out
+
+
 def camera_lookat(cam: A.gnd_camera_desc, origin, target, up=(0.0, 1.0, 0.0)) -> A.gnd_camera_desc:
     """A copy of `cam` posed with the reference's lookAt (transform.h:141-144)."""
     c = A.gnd_camera_desc()

@@ -34,7 +34,7 @@ DEVICE = [f"--offload-arch={ARCH}", "-fno-gpu-flush-denormals-to-zero",
           "-Wno-unused-command-line-argument"]
 
 HOST_SOURCES = ["host/hmath.cpp", "host/bvh_build.cpp", "host/scene_host.cpp", "host/generators.cpp",
-                "host/xml_loader.cpp", "host/integrator.cpp", "host/host_capi.cpp"]
+                "host/xml_loader.cpp", "host/integrator.cpp", "host/exr_read.cpp", "host/host_capi.cpp"]
 DEVICE_SOURCES = ["device/kernels.hip"]
 
 
@@ -65,7 +65,7 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
         list(ex.map(_run, tasks))
     objs = [t[-1] for t in tasks]
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-pthread", *objs, "-o", tmp])
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-pthread", *objs, "-lz", "-o", tmp])
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")

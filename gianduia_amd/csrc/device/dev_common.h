@@ -21,6 +21,7 @@ constexpr float kInv2Pi = 0.15915494309189533577f;
 constexpr float kInv4Pi = 0.07957747154594766788f;
 constexpr float kFltMax = 3.402823466e+38f;
 constexpr double kPiD = 3.14159265358979323846;
+constexpr double kInvPiD = 0.31830988618379067154;  // M_1_PI
 
 GDEV_INLINE float finf() { return __builtin_inff(); }
 
@@ -144,6 +145,16 @@ struct alignas(16) DMedium {
     const float* majorants;
 };
 
+// Point / spot / directional / environment emitter
+struct DLight {
+    int32_t type;
+    float pos[3];
+    float dir[3];      // spot: normalised cone axis; directional: towards the light
+    float power[3];
+    float irr[3];
+    float cosTotal, cosFalloff;
+};
+
 struct DScene {
     const DNode* tlas;
     const int32_t* tlasPrims;
@@ -155,8 +166,18 @@ struct DScene {
     const float* cdf;
     const DMaterial* mats;
     const DMedium* media;
-    const int32_t* emitters;  // primitive index of each emitter
+    const int32_t* emitters;  // emitter list: primitive index (area) or -(light + 1)
     int32_t nTlas, nEmitters, nPrims, hasIndex, hasGrid, pad;
+    const DLight* lights;
+    // environment map (envLight >= 0): RGB image, row-major, and its
+    // Distribution2D: per-row conditional func / cdf / integral, marginal cdf
+    const float* envRgb;
+    const float* envCondFunc;  // envH * envW
+    const float* envCondCdf;   // envH * (envW + 1)
+    const float* envCondInt;   // envH
+    const float* envMargCdf;   // envH + 1
+    float envMargInt, envStrength;
+    int32_t nLights, envLight, envW, envH;
 };
 
 struct HitRec {

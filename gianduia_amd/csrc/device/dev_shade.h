@@ -635,6 +635,160 @@ GDEV_INLINE int pickEmitter(const DScene& S, float rnd) {  // scene.h:28-34
     return S.emitters[idx];
 }
 
+// ------------------------------------------------------------------ non-area lights
+// SphericalCoordinates / SphericalDirection (constants.h:80-100)
+GDEV_INLINE void sphericalCoordinates(V3 w, float& theta, float& phi) {
+    theta = acosf(clampf(w.z, -1.0f, 1.0f));
+    phi = atan2f(w.y, w.x);
+    if (phi < 0.0f) phi += 2.0f * kPi;
+}
+// Film::getPixelBilinear (film.cpp:125-151): u wraps, v clamps
+GDEV_INLINE V3 envBilinear(const DScene& S, float u, float v) {
+    const int W = S.envW, H = S.envH;
+    float x = u * W - 0.5f;
+    float y = v * H - 0.5f;
+    int x0 = (int)floorf(x);
+    int y0 = (int)floorf(y);
+    float dx = x - x0;
+    float dy = y - y0;
+    int px0 = (x0 % W + W) % W;
+    int px1 = (px0 + 1) % W;
+    int py0 = y0 < 0 ? 0 : (y0 > H - 1 ? H - 1 : y0);
+    int py1 = y0 + 1 < 0 ? 0 : (y0 + 1 > H - 1 ? H - 1 : y0 + 1);
+    auto px = [&](int xx, int yy) {
+        const float* c = S.envRgb + 3 * ((size_t)yy * W + xx);
+        return V3{c[0], c[1], c[2]};
+    };
+    V3 c0 = px(px0, py0) * (1.0f - dx) + px(px1, py0) * dx;
+    V3 c1 = px(px0, py1) * (1.0f - dx) + px(px1, py1) * dx;
+    return c0 * (1.0f - dy) + c1 * dy;
+}
+// EnvironmentMap::eval (environment.cpp:31-40); u, v in double as phi * M_1_PI
+GDEV_INLINE V3 envEval(const DScene& S, V3 w) {
+    float theta, phi;
+    sphericalCoordinates(w, theta, phi);
+    float u = (float)((double)phi * kInvPiD * 0.5);
+    float v = (float)((double)theta * kInvPiD);
+    return envBilinear(S, u, v) * S.envStrength;
+}
+// Distribution2D::Pdf (distribution.cpp:75-79)
+GDEV_INLINE float envMapPdf(const DScene& S, float u, float v) {
+    int iu = (int)(u * S.envW), iv = (int)(v * S.envH);
+    iu = iu < 0 ? 0 : (iu > S.envW - 1 ? S.envW - 1 : iu);
+    iv = iv < 0 ? 0 : (iv > S.envH - 1 ? S.envH - 1 : iv);
+    return S.envCondFunc[(size_t)iv * S.envW + iu] / S.envMargInt;
+}
+// EnvironmentMap::pdf (environment.cpp:75-91)
+GDEV_INLINE float envPdf(const DScene& S, V3 refP, V3 infoP) {
+    V3 wi = normalize(infoP - refP);
+    float theta, phi;
+    sphericalCoordinates(wi, theta, phi);
+    float sinTheta = sinf(theta);
+    if (sinTheta <= 0.0f) return 0.0f;
+    float u = (float)((double)phi * kInvPiD * 0.5);
+    float v = (float)((double)theta * kInvPiD);
+    float mapPdf = envMapPdf(S, u, v);
+    return (float)((double)mapPdf / (2.0 * kPiD * kPiD * (double)sinTheta));
+}
+// Distribution1D::SampleContinuous (distribution.cpp:20-34) over func / cdf
+// of n entries: std::upper_bound, clamp, linear remap within the bucket
+GDEV_INLINE float sample1D(const float* func, const float* cdf, int n, float funcInt, float u, float& pdf, int& off) {
+    int lo = 0, cnt = n + 1;  // first cdf entry greater than u
+    while (cnt > 0) {
+        int step = cnt / 2, mid = lo + step;
+        if (!(u < cdf[mid])) { lo = mid + 1; cnt -= step + 1; }
+        else cnt = step;
+    }
+    int offset = lo - 1;
+    offset = offset < 0 ? 0 : (offset > n - 1 ? n - 1 : offset);
+    off = offset;
+    float du = u - cdf[offset];
+    if ((cdf[offset + 1] - cdf[offset]) > 0.0f) du /= (cdf[offset + 1] - cdf[offset]);
+    pdf = (funcInt > 0.0f) ? func[offset] / funcInt : 0.0f;
+    return (offset + du) / n;
+}
+// Emitter::sample of the non-area emitters (pointlight.cpp:18-33,
+// spotlight.cpp:28-60, directional.cpp:17-33, environment.cpp:42-73); the
+// pdf is always written, so ls.ok
+GDEV_INLINE void lightSample(const DScene& S, int k, V3 refP, float ux, float uy, LightSample& ls) {
+    const DLight& L = S.lights[k];
+    ls.ok = true;
+    ls.Li = sameV(0.0f);
+    if (L.type == GND_LIGHT_POINT || L.type == GND_LIGHT_SPOT) {
+        const V3 lp = V3{L.pos[0], L.pos[1], L.pos[2]};
+        const V3 I = V3{L.power[0], L.power[1], L.power[2]} * kInv4Pi;
+        ls.p = lp;
+        ls.pdf = 1.0f;
+        if (L.type == GND_LIGHT_POINT) {
+            float squaredDist = lengthSq(lp - refP);
+            ls.dir = normalize(ls.p - refP);
+            ls.tmax = length(ls.p - refP) - kEps;
+            ls.Li = cdiv(I, squaredDist);
+        } else {
+            V3 wi = normalize(lp - refP);
+            float dist = length(lp - refP);
+            float squaredDist = dist * dist;
+            ls.dir = wi;
+            ls.tmax = dist - kEps;
+            float cosTheta = dot(-wi, V3{L.dir[0], L.dir[1], L.dir[2]});
+            if (cosTheta <= L.cosTotal) return;
+            float falloff = 1.0f;
+            if (cosTheta < L.cosFalloff) {
+                float t = (cosTheta - L.cosTotal) / (L.cosFalloff - L.cosTotal);
+                falloff = (t * t) * (3.0f - 2.0f * t);
+            }
+            ls.Li = cdiv(I * falloff, squaredDist);
+        }
+        return;
+    }
+    if (L.type == GND_LIGHT_DIRECTIONAL) {
+        const V3 wi = V3{L.dir[0], L.dir[1], L.dir[2]};
+        ls.p = refP + wi * 1e5f;
+        ls.pdf = 1.0f;
+        ls.dir = wi;
+        ls.tmax = kFltMax;
+        ls.Li = V3{L.irr[0], L.irr[1], L.irr[2]};
+        return;
+    }
+    // environment: Distribution2D::SampleContinuous (distribution.cpp:65-73)
+    float pdf1, pdf0;
+    int v, unused;
+    const float d1 = sample1D(S.envCondInt, S.envMargCdf, S.envH, S.envMargInt, uy, pdf1, v);
+    const float d0 = sample1D(S.envCondFunc + (size_t)v * S.envW, S.envCondCdf + (size_t)v * (S.envW + 1), S.envW,
+                              S.envCondInt[v], ux, pdf0, unused);
+    const float mapPdf = pdf0 * pdf1;
+    if (mapPdf == 0.0f) {
+        ls.pdf = 0.0f;
+        return;
+    }
+    float theta = (float)((double)d1 * kPiD);
+    float phi = (float)((double)(d0 * 2.0f) * kPiD);
+    float cosT = cosf(theta), sinT = sinf(theta), cosP = cosf(phi), sinP = sinf(phi);
+    V3 wi = V3{sinT * cosP, sinT * sinP, cosT};
+    ls.p = refP + wi * 1e5f;
+    ls.dir = wi;
+    ls.tmax = finf();
+    float sinTheta = sinf(theta);
+    if (sinTheta <= 0.0f) {
+        ls.pdf = 0.0f;
+        return;
+    }
+    ls.pdf = (float)((double)mapPdf / (2.0 * kPiD * kPiD * (double)sinTheta));
+    ls.Li = cdiv(envEval(S, wi), ls.pdf);
+}
+// the NEE sample of emitter-list entry e (area emitter primitive, or
+// -(light + 1)); delta = Emitter::isDelta
+GDEV_INLINE void emitterSample(const DScene& S, int e, V3 refP, float ux, float uy, LightSample& ls, bool& delta) {
+    if (e >= 0) {
+        delta = false;
+        areaSample(S, e, refP, ux, uy, ls);
+        return;
+    }
+    const int k = -e - 1;
+    delta = S.lights[k].type != GND_LIGHT_ENVIRONMENT;
+    lightSample(S, k, refP, ux, uy, ls);
+}
+
 // ------------------------------------------------------------------ media
 GDEV_INLINE float hgPhase(float cosT, float g) {  // phase.h:86-89
     float denom = 1.0f + g * g + 2.0f * g * cosT;

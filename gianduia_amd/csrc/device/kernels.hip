@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <condition_variable>
 #include <cstdio>
@@ -593,17 +594,21 @@ GDEV_INLINE bool scatterStep(const DScene& S, const DJob& J, PathLocal& s, const
 
 // The integrators' G-buffer writes (path.cpp:35-61 and 99-106, volPath.cpp:
 // 117-146 and 184-191) into slot p's AOV record.  A miss: depth / roughness /
-// metallic reset at bounce 0; albedo (the environment, none here) and normal 0.
+// metallic reset at bounce 0; albedo = the environment along the ray (black
+// without one), normal 0.
 GDEV_INLINE void aovPut(const PathSoA& P, int p, int k, float v) { P.aov[(size_t)k * P.aovStride + p] = v; }
-GDEV_INLINE void aovMiss(const PathSoA& P, int p, PathLocal& s, uint32_t bounces) {
+GDEV_INLINE void aovMiss(const PathSoA& P, int p, PathLocal& s, uint32_t bounces, V3 envAlbedo) {
     if (bounces == 0) {
         aovPut(P, p, 6, -1.0f);
         aovPut(P, p, 7, 0.0f);
         aovPut(P, p, 8, 0.0f);
     }
     if (s.flags & F_GBUF) {
+        aovPut(P, p, 0, envAlbedo.x);
+        aovPut(P, p, 1, envAlbedo.y);
+        aovPut(P, p, 2, envAlbedo.z);
 #pragma unroll
-        for (int k = 0; k < 6; ++k) aovPut(P, p, k, 0.0f);
+        for (int k = 3; k < 6; ++k) aovPut(P, p, k, 0.0f);
         s.flags &= ~F_GBUF;
     }
 }
@@ -669,12 +674,13 @@ GDEV_INLINE int neeStep(const WaveArgs& A, const DJob& J, int p, PathLocal& s, c
             const int ep = pickEmitter(S, s.rng.next1D());
             const float ux = s.rng.next1D(), uy = s.rng.next1D();
             LightSample ls;
-            areaSample(S, ep, me.p, ux, uy, ls);
+            bool delta;
+            emitterSample(S, ep, me.p, ux, uy, ls, delta);
             const V3 Li = cdiv(ls.Li, lsp);
             if (ls.ok && ls.pdf > 1e-6f && !isBlack(Li)) {
                 const V3 wi = normalize(ls.p - me.p);
                 const float pp = hgPhase(dot(-s.d, wi), hgClamp(S.media[medium].g));
-                pending = pp > 1e-6f ? s.tp * pp * Li * powerHeuristic(ls.pdf * lsp, pp) : sameV(0.0f);
+                pending = pp > 1e-6f ? s.tp * pp * Li * powerHeuristic(ls.pdf * lsp, delta ? 0.0f : pp) : sameV(0.0f);
                 shadow = true;
                 so = me.p; sd = ls.dir; stmax = ls.tmax; smedium = medium; lightP = ls.p;
             }
@@ -686,7 +692,23 @@ GDEV_INLINE int neeStep(const WaveArgs& A, const DJob& J, int p, PathLocal& s, c
     }
     const bool aovOn = J.filmAov && A.P.aov;
     if (!hit) {
-        if (aovOn) aovMiss(A.P, p, s, bounces);
+        // the environment along the escaping ray, MIS-weighted against NEE
+        // unless the last bounce was specular (path.cpp:36-55, volPath.cpp:
+        // 117-139); its pdf is taken from the origin, the reference point of
+        // the default SurfaceInteraction there
+        V3 env = sameV(0.0f);
+        if (S.envLight >= 0) {
+            env = envEval(S, s.d);
+            if (!isBlack(env)) {
+                float w = 1.0f;
+                if (!(s.flags & F_SPECULAR)) {
+                    const float pe = envPdf(S, sameV(0.0f), s.o + s.d * 1e5f) * invNE;
+                    w = powerHeuristic(s.pdfPrev, pe);
+                }
+                s.L = s.L + s.tp * env * w;
+            }
+        }
+        if (aovOn) aovMiss(A.P, p, s, bounces, env);
         return 0;
     }
     Isect it;
@@ -722,14 +744,15 @@ GDEV_INLINE int neeStep(const WaveArgs& A, const DJob& J, int p, PathLocal& s, c
     const int ep = pickEmitter(S, s.rng.next1D());
     const float ux = s.rng.next1D(), uy = s.rng.next1D();
     LightSample ls;
-    areaSample(S, ep, it.p, ux, uy, ls);
+    bool delta;
+    emitterSample(S, ep, it.p, ux, uy, ls, delta);
     const V3 Li = cdiv(ls.Li, lsp);
     if (ls.ok && ls.pdf > 1e-6f && !isBlack(Li)) {
         const V3 wi = normalize(ls.p - it.p);
         const float bp = b.pdf(-s.d, wi);
         const V3 f = b.f(-s.d, wi);
         if (!isBlack(f) && bp > 1e-6f)
-            pending = s.tp * f * Li * __builtin_fabsf(dot(it.n, wi)) * powerHeuristic(ls.pdf * lsp, bp);
+            pending = s.tp * f * Li * __builtin_fabsf(dot(it.n, wi)) * powerHeuristic(ls.pdf * lsp, delta ? 0.0f : bp);
         else pending = sameV(0.0f);
         shadow = true;
         so = it.p; sd = ls.dir; stmax = ls.tmax; lightP = ls.p;
@@ -1369,6 +1392,75 @@ DNode toDNode(const gnd_bvh4_node& n, uint32_t innerBase, uint32_t leafBase) {
     return d;
 }
 
+// Light constructors (pointlight.cpp:9-12, spotlight.cpp:9-22, directional.cpp:8-11)
+DLight makeLight(const gnd_light_desc& l) {
+    DLight L;
+    std::memset(&L, 0, sizeof(L));
+    L.type = l.type;
+    for (int c = 0; c < 3; ++c) {
+        L.pos[c] = l.position[c];
+        L.power[c] = l.power[c];
+        L.irr[c] = l.irradiance[c];
+    }
+    const float dx = l.direction[0], dy = l.direction[1], dz = l.direction[2];
+    if (l.type == GND_LIGHT_SPOT || l.type == GND_LIGHT_DIRECTIONAL) {
+        // Normalize(v) = v * (1 / sqrt(dot)); the directional light points back
+        const float inv = 1.0f / std::sqrt((dx * dx + dy * dy) + dz * dz);
+        const float sgn = l.type == GND_LIGHT_DIRECTIONAL ? -1.0f : 1.0f;
+        L.dir[0] = sgn * (dx * inv);
+        L.dir[1] = sgn * (dy * inv);
+        L.dir[2] = sgn * (dz * inv);
+    }
+    if (l.type == GND_LIGHT_SPOT) {
+        float halfAngle = l.spot_size * 0.5f;
+        float innerAngle = halfAngle * (1.0f - l.blend);
+        L.cosTotal = std::cos((kPi / 180.f) * halfAngle);
+        L.cosFalloff = std::cos((kPi / 180.f) * innerAngle);
+    }
+    if (l.type == GND_LIGHT_ENVIRONMENT && (!l.env_rgb || l.env_width <= 0 || l.env_height <= 0))
+        throw std::runtime_error("environment light without an image");
+    return L;
+}
+
+// EnvironmentMap::buildPDFs (environment.cpp:106-120) with Distribution1D /
+// Distribution2D (distribution.cpp:6-18, 53-64): per row the luminance times
+// sin(theta of the row centre), a piecewise-constant CDF per row and over the
+// rows' integrals.
+struct EnvTables {
+    std::vector<float> rgb, condFunc, condCdf, condInt, margCdf;
+    float margInt = 0.0f;
+};
+static void dist1D(const float* f, int n, float* cdf, float& funcInt) {
+    cdf[0] = 0.0f;
+    for (int i = 1; i < n + 1; ++i) cdf[i] = cdf[i - 1] + f[i - 1] / n;
+    funcInt = cdf[n];
+    if (funcInt == 0.0f) {
+        for (int i = 1; i < n + 1; ++i) cdf[i] = float(i) / float(n);
+    } else {
+        for (int i = 1; i < n + 1; ++i) cdf[i] /= funcInt;
+    }
+}
+EnvTables envTables(const gnd_light_desc& l) {
+    EnvTables t;
+    const int W = l.env_width, H = l.env_height;
+    t.rgb.assign(l.env_rgb, l.env_rgb + (size_t)W * H * 3);
+    t.condFunc.resize((size_t)W * H);
+    for (int i = 0; i < H; i++) {
+        float sinTheta = std::sin(M_PI * (i + 0.5f) / H);  // double sine, stored as float
+        for (int j = 0; j < W; j++) {
+            const float* c = &t.rgb[3 * ((size_t)i * W + j)];
+            const float lum = c[0] * 0.212671f + c[1] * 0.715160f + c[2] * 0.072169f;  // Color3f::luminance
+            t.condFunc[(size_t)i * W + j] = lum * sinTheta;
+        }
+    }
+    t.condCdf.resize((size_t)H * (W + 1));
+    t.condInt.resize(H);
+    for (int v = 0; v < H; ++v) dist1D(&t.condFunc[(size_t)v * W], W, &t.condCdf[(size_t)v * (W + 1)], t.condInt[v]);
+    t.margCdf.resize(H + 1);
+    dist1D(t.condInt.data(), H, t.margCdf.data(), t.margInt);
+    return t;
+}
+
 DCamera makeCamera(const gnd_camera_desc& c) {
     DCamera d{};
     std::memcpy(d.toWorld.m, c.to_world, sizeof(d.toWorld.m));
@@ -1692,11 +1784,7 @@ int gnd_hip_scene_upload(gnd_hip_ctx* ctx, const gnd_hip_scene_arrays* a, gnd_hi
             if (p.material < 0 || (uint32_t)p.material >= d.nmaterials) throw std::runtime_error("bad material index");
             P.radius = p.radius;
             P.radiance[0] = p.radiance[0]; P.radiance[1] = p.radiance[1]; P.radiance[2] = p.radiance[2];
-            P.emitter = -1;
-            if (p.has_emitter) {
-                P.emitter = (int32_t)emitters.size();
-                emitters.push_back((int32_t)i);
-            }
+            P.emitter = p.has_emitter ? 0 : -1;  // slot assigned below
             if (p.shape_type == GND_SHAPE_MESH) {
                 if (p.mesh < 0 || (uint32_t)p.mesh >= a->n_meshes) throw std::runtime_error("bad mesh index");
                 const gnd_mesh_info& mi = a->meshes[p.mesh];
@@ -1709,6 +1797,35 @@ int gnd_hip_scene_upload(gnd_hip_ctx* ctx, const gnd_hip_scene_arrays* a, gnd_hi
                 P.totalArea = mi.total_area;
             } else {
                 I.root = -1;
+            }
+        }
+        // emitter list in document order (Scene::addChild, scene.cpp:9-40):
+        // the lights at their emitter_index, the area emitters of the
+        // primitives in the other slots in primitive order
+        int nArea = 0;
+        for (uint32_t i = 0; i < d.nprims; ++i) nArea += prims[i].emitter >= 0 ? 1 : 0;
+        const int nEm = nArea + (int)d.nlights;
+        emitters.assign(nEm, INT32_MIN);
+        std::vector<DLight> lights(std::max<uint32_t>(1, d.nlights));
+        int envLight = -1;
+        for (uint32_t k = 0; k < d.nlights; ++k) {
+            const gnd_light_desc& ld = d.lights[k];
+            const int e = ld.emitter_index;
+            if (e < 0 || e >= nEm || emitters[e] != INT32_MIN) throw std::runtime_error("bad light emitter_index");
+            emitters[e] = -(int32_t)k - 1;
+            lights[k] = makeLight(ld);
+            if (ld.type == GND_LIGHT_ENVIRONMENT) {
+                if (envLight >= 0) throw std::runtime_error("more than one environment light");
+                envLight = (int)k;
+            }
+        }
+        {
+            int slot = 0;
+            for (uint32_t i = 0; i < d.nprims; ++i) {
+                if (prims[i].emitter < 0) continue;
+                while (emitters[slot] != INT32_MIN) ++slot;
+                prims[i].emitter = slot;
+                emitters[slot] = (int32_t)i;
             }
         }
         if (emitters.empty()) throw std::runtime_error("scene has no emitter");
@@ -1802,6 +1919,21 @@ int gnd_hip_scene_upload(gnd_hip_ctx* ctx, const gnd_hip_scene_arrays* a, gnd_hi
         v.mats = devAlloc<DMaterial>(s, mats.size(), mats.data());
         v.media = devAlloc<DMedium>(s, media.size(), media.data());
         v.emitters = devAlloc<int32_t>(s, emitters.size(), emitters.data());
+        v.lights = devAlloc<DLight>(s, lights.size(), lights.data());
+        v.nLights = (int32_t)d.nlights;
+        v.envLight = envLight;
+        if (envLight >= 0) {
+            const EnvTables t = envTables(d.lights[envLight]);
+            v.envRgb = devAlloc<float>(s, t.rgb.size(), t.rgb.data());
+            v.envCondFunc = devAlloc<float>(s, t.condFunc.size(), t.condFunc.data());
+            v.envCondCdf = devAlloc<float>(s, t.condCdf.size(), t.condCdf.data());
+            v.envCondInt = devAlloc<float>(s, t.condInt.size(), t.condInt.data());
+            v.envMargCdf = devAlloc<float>(s, t.margCdf.size(), t.margCdf.data());
+            v.envMargInt = t.margInt;
+            v.envStrength = d.lights[envLight].strength;
+            v.envW = d.lights[envLight].env_width;
+            v.envH = d.lights[envLight].env_height;
+        }
         v.nTlas = (int32_t)tl.size();
         v.nEmitters = (int32_t)emitters.size();
         v.nPrims = (int32_t)d.nprims;

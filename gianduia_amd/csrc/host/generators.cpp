@@ -359,15 +359,89 @@ std::unique_ptr<HostScene> proceduralScene(const Params& p) {
     return scaledScene(q);
 }
 
+// Procedural sky for the environment light: a width 2n x height n
+// latitude-longitude image (row 0 = +z, the reference's spherical
+// coordinates), y up: a horizon-to-zenith gradient above the horizon, a dim
+// ground below it and a small bright sun disk, so that the map's importance
+// sampling has a peak to find.
+std::vector<float> skyImage(int n) {
+    const int W = 2 * n, H = n;
+    std::vector<float> img((size_t)W * H * 3);
+    const Vec3 sun = normalize(Vec3(0.4f, 0.8f, 0.3f));
+    const float cosSun = std::cos(3.0f * 3.14159265f / 180.0f);
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const float th = 3.14159265f * (y + 0.5f) / H, ph = 2.0f * 3.14159265f * (x + 0.5f) / W;
+            const Vec3 d(std::sin(th) * std::cos(ph), std::sin(th) * std::sin(ph), std::cos(th));
+            Vec3 c;
+            if (d.y >= 0.0f) {
+                const float t = d.y;
+                c = Vec3(0.9f, 0.9f, 1.0f) * (1.0f - t) + Vec3(0.3f, 0.5f, 0.9f) * t;
+            } else {
+                c = Vec3(0.25f, 0.22f, 0.2f);
+            }
+            if (dot(d, sun) > cosSun) c = c + Vec3(50.0f, 45.0f, 40.0f);
+            float* o = &img[3 * ((size_t)y * W + x)];
+            o[0] = c.x; o[1] = c.y; o[2] = c.z;
+        }
+    return img;
+}
+
+// lights=<comma list> appends non-area emitters to any generated scene,
+// after its area emitters, in list order: point, spot, sun (directional),
+// env (the procedural sky, envres=n rows, default 64; envstrength).
+void addLights(HostScene& s, const Params& p) {
+    auto it = p.find("lights");
+    if (it == p.end() || it->second.empty()) return;
+    std::stringstream ss(it->second);
+    std::string k;
+    while (std::getline(ss, k, ',')) {
+        LightData l;
+        if (k == "point") {
+            l = lightDefaults(GND_LIGHT_POINT);
+            const float pos[3] = {0.3f, 0.7f, 0.6f}, pw[3] = {6.0f, 5.5f, 5.0f};
+            for (int c = 0; c < 3; ++c) { l.desc.position[c] = pos[c]; l.desc.power[c] = pw[c]; }
+        } else if (k == "spot") {
+            l = lightDefaults(GND_LIGHT_SPOT);
+            const float pos[3] = {-0.4f, 0.8f, 1.6f}, dir[3] = {0.3f, -0.7f, -1.0f}, pw[3] = {30.0f, 28.0f, 24.0f};
+            for (int c = 0; c < 3; ++c) {
+                l.desc.position[c] = pos[c];
+                l.desc.direction[c] = dir[c];
+                l.desc.power[c] = pw[c];
+            }
+            l.desc.spot_size = 50.0f;
+            l.desc.blend = 0.3f;
+        } else if (k == "sun") {
+            l = lightDefaults(GND_LIGHT_DIRECTIONAL);
+            const float dir[3] = {-0.3f, -1.0f, -0.4f}, irr[3] = {2.0f, 1.9f, 1.7f};
+            for (int c = 0; c < 3; ++c) { l.desc.direction[c] = dir[c]; l.desc.irradiance[c] = irr[c]; }
+        } else if (k == "env") {
+            l = lightDefaults(GND_LIGHT_ENVIRONMENT);
+            const int n = (int)num(p, "envres", 64);
+            if (n < 1 || n > 8192) throw std::runtime_error("envres out of range");
+            l.env = skyImage(n);
+            l.desc.env_width = 2 * n;
+            l.desc.env_height = n;
+            l.desc.strength = (float)num(p, "envstrength", 1.0);
+        } else {
+            throw std::runtime_error("unknown light kind: " + k);
+        }
+        s.addLight(std::move(l));
+    }
+}
+
 }  // namespace
 
 std::unique_ptr<HostScene> generateScene(const std::string& name, const std::string& params) {
     Params p = parseParams(params);
-    if (name == "cbox") return cornellBox(p);
-    if (name == "scaled") return scaledScene(p);
-    if (name == "volumetric") return volumetricScene(p);
-    if (name == "procedural") return proceduralScene(p);
-    throw std::runtime_error("unknown scene generator: " + name);
+    std::unique_ptr<HostScene> s;
+    if (name == "cbox") s = cornellBox(p);
+    else if (name == "scaled") s = scaledScene(p);
+    else if (name == "volumetric") s = volumetricScene(p);
+    else if (name == "procedural") s = proceduralScene(p);
+    else throw std::runtime_error("unknown scene generator: " + name);
+    addLights(*s, p);
+    return s;
 }
 
 }  // namespace gnd

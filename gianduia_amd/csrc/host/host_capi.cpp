@@ -311,6 +311,22 @@ int gnd_host_image_save(const char* path, const float* rgb, int width, int heigh
     });
 }
 
+int gnd_host_image_load(const char* path, int* width, int* height, float* rgb, size_t capacity) {
+    if (!path || !width || !height) return GND_E_INVALID;
+    return hostGuard([&] {
+        int w = 0, h = 0;
+        std::vector<float> px;
+        gnd::readEXR(path, w, h, px);
+        *width = w;
+        *height = h;
+        if (rgb) {
+            if (capacity < px.size()) throw std::runtime_error("image buffer too small");
+            std::memcpy(rgb, px.data(), px.size() * sizeof(float));
+        }
+        return GND_OK;
+    });
+}
+
 int gnd_host_sampler_floats(uint64_t sample_index, int n, float* out) {
     if (n < 0 || (n > 0 && !out)) return GND_E_INVALID;
     gnd::IndependentSampler smp(1);

@@ -96,6 +96,10 @@ private:
 void writeEXR(const std::string& path, const float* px, int w, int h, int channels);
 // PFM (little-endian, rows bottom to top), 3 channels ("PF") or 1 ("Pf")
 void writePFM(const std::string& path, const float* px, int w, int h, int channels);
+// Scanline OpenEXR file -> interleaved R, G, B floats over its data window,
+// rows top to bottom (channels it lacks read as 0); NONE / RLE / ZIPS / ZIP
+// compression, UINT / HALF / FLOAT channels.  Throws on anything else.
+void readEXR(const std::string& path, int& width, int& height, std::vector<float>& rgb);
 
 // What an integrator renders: the scene description and its device copy.
 struct RenderScene {

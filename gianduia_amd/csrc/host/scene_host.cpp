@@ -122,6 +122,44 @@ void HostScene::setCamera(int type, int w, int h, float fov, const Xform& toWorl
     camera.filter_alpha = 2.0f;
 }
 
+LightData lightDefaults(int type) {
+    LightData l;
+    gnd_light_desc& d = l.desc;
+    d.type = type;
+    d.emitter_index = -1;
+    d.direction[0] = 0.0f;
+    d.direction[1] = type == GND_LIGHT_SPOT ? 0.0f : -1.0f;
+    d.direction[2] = type == GND_LIGHT_SPOT ? -1.0f : 0.0f;
+    for (int c = 0; c < 3; ++c) {
+        d.power[c] = 10.0f;
+        d.irradiance[c] = 1.0f;
+    }
+    d.spot_size = 45.0f;
+    d.blend = 0.15f;
+    d.strength = 1.0f;
+    return l;
+}
+
+int HostScene::emitterCount() const {
+    int n = (int)lights.size();
+    for (auto& p : prims) n += p.has_emitter ? 1 : 0;
+    return n;
+}
+
+void HostScene::addLight(LightData l) {
+    if (l.desc.type == GND_LIGHT_ENVIRONMENT) {
+        for (auto& o : lights)
+            if (o.desc.type == GND_LIGHT_ENVIRONMENT)
+                throw std::runtime_error("Scene: cannot add multiple infinite area lights!");
+        if (l.desc.env_width <= 0 || l.desc.env_height <= 0 ||
+            l.env.size() != (size_t)l.desc.env_width * l.desc.env_height * 3)
+            throw std::runtime_error("environment light needs a width*height RGB image");
+    }
+    l.desc.emitter_index = emitterCount();
+    lights.push_back(std::move(l));
+    built_ = false;
+}
+
 void HostScene::copyFrom(const gnd_scene_desc& d) {
     meshes.clear();
     for (uint32_t i = 0; i < d.nmeshes; ++i) {
@@ -142,6 +180,14 @@ void HostScene::copyFrom(const gnd_scene_desc& d) {
         if (md.desc.type == GND_MEDIUM_GRID && md.desc.density)
             md.density.assign(md.desc.density, md.desc.density + (size_t)md.desc.nx * md.desc.ny * md.desc.nz);
         media.push_back(std::move(md));
+    }
+    lights.clear();
+    for (uint32_t i = 0; i < d.nlights; ++i) {
+        LightData l;
+        l.desc = d.lights[i];
+        if (l.desc.type == GND_LIGHT_ENVIRONMENT && l.desc.env_rgb)
+            l.env.assign(l.desc.env_rgb, l.desc.env_rgb + (size_t)l.desc.env_width * l.desc.env_height * 3);
+        lights.push_back(std::move(l));
     }
     camera = d.camera;
     integrator = d.integrator;
@@ -173,10 +219,17 @@ const gnd_scene_desc& HostScene::desc() {
     desc_.materials = materials.data();
     desc_.nmedia = (uint32_t)media.size();
     desc_.media = mediumDescs_.data();
+    lightDescs_.resize(lights.size());
+    for (size_t i = 0; i < lights.size(); ++i) {
+        lightDescs_[i] = lights[i].desc;
+        lightDescs_[i].env_rgb = lights[i].env.empty() ? nullptr : lights[i].env.data();
+    }
     desc_.camera = camera;
     desc_.integrator = integrator;
     desc_.max_depth = maxDepth;
     desc_.spp = spp;
+    desc_.nlights = (uint32_t)lights.size();
+    desc_.lights = lightDescs_.empty() ? nullptr : lightDescs_.data();
     return desc_;
 }
 
@@ -252,9 +305,15 @@ const gnd_hip_scene_arrays& HostScene::build(int threads, double* ms) {
             if (p.material < 0 || p.material >= (int)materials.size())
                 throw std::runtime_error("primitive references a missing material");
         }
-        bool anyEmitter = false;
-        for (auto& p : prims) anyEmitter = anyEmitter || p.has_emitter;
-        if (!anyEmitter) throw std::runtime_error("scene has no emitter");
+        if (emitterCount() == 0) throw std::runtime_error("scene has no emitter");
+        {
+            std::vector<char> taken((size_t)emitterCount(), 0);
+            for (auto& l : lights) {
+                const int e = l.desc.emitter_index;
+                if (e < 0 || e >= emitterCount() || taken[e]) throw std::runtime_error("bad light emitter_index");
+                taken[e] = 1;
+            }
+        }
         const size_t nm = meshes.size();
         std::vector<MeshBuild> mb(nm);
         // one pool for the whole build: the BLAS builds are tasks, and big

@@ -26,12 +26,22 @@ struct MediumData {
     std::vector<float> density;
 };
 
+struct LightData {
+    gnd_light_desc desc{};
+    std::vector<float> env;  // environment: width*height*3 RGB
+};
+
+// Light defaults of the reference constructors (pointlight.cpp:9-12,
+// spotlight.cpp:9-22, directional.cpp:8-11, environment.cpp:14-27).
+LightData lightDefaults(int type);
+
 class HostScene {
 public:
     std::vector<MeshData> meshes;
     std::vector<gnd_primitive_desc> prims;
     std::vector<gnd_material_desc> materials;
     std::vector<MediumData> media;
+    std::vector<LightData> lights;
     gnd_camera_desc camera{};
     int integrator = GND_INTEGRATOR_PATH;
     int maxDepth = 1000;
@@ -42,6 +52,9 @@ public:
     int addMaterial(const gnd_material_desc& m);
     int addPrim(int mesh, const Xform& toWorld, int material, const float* radiance = nullptr);
     int addSphere(float radius, const Xform& toWorld, int material, const float* radiance = nullptr);
+    // appends a light after every emitter declared so far (document order)
+    void addLight(LightData l);
+    int emitterCount() const;
     void setCamera(int type, int w, int h, float fov, const Xform& toWorld);
 
     void copyFrom(const gnd_scene_desc& d);
@@ -52,6 +65,7 @@ private:
     gnd_scene_desc desc_{};
     std::vector<gnd_mesh_desc> meshDescs_;
     std::vector<gnd_medium_desc> mediumDescs_;
+    std::vector<gnd_light_desc> lightDescs_;
     bool built_ = false;
     // flattened storage
     std::vector<gnd_bvh4_node> tlas_;

@@ -6,7 +6,9 @@
 // type="path|volpath|path_hip|volpath_hip">, <primitive> with <shape
 // type="mesh"> (OBJ file) or <shape type="sphere">, <material
 // type="matte|conductor|glass|plastic|index"> with nested <medium
-// name="inside|outside" type="homogeneous">, <emitter type="area">, <ref id>
+// name="inside|outside" type="homogeneous">, <emitter type="area"> inside a
+// primitive, scene-level <emitter type="point|spot|directional|environment">
+// (the environment image an OpenEXR file), <ref id>
 // sharing of shapes, and <transform> with translate/scale/rotate/lookat steps
 // composed as t = step * t (parser.cpp:112-165).
 #include <cctype>
@@ -15,6 +17,7 @@
 #include <sstream>
 #include <stdexcept>
 
+#include "integrator.h"
 #include "scene_host.h"
 
 namespace gnd {
@@ -403,6 +406,49 @@ struct Loader {
         else sc.addPrim(mesh, x, material, emitter ? Le : nullptr);
     }
 
+    // A scene-level emitter (Scene::addChild, scene.cpp:33-40): point,
+    // spot, directional or an environment map read from an EXR file.  Its
+    // position in the emitter list is its place in the document.
+    void readEmitter(const XNode& n) {
+        std::vector<const XNode*> kids;
+        Props p = readProps(n, kids);
+        const std::string type = n.attrOr("type");
+        int t;
+        if (type == "point") t = GND_LIGHT_POINT;
+        else if (type == "spot") t = GND_LIGHT_SPOT;
+        else if (type == "directional") t = GND_LIGHT_DIRECTIONAL;
+        else if (type == "environment") t = GND_LIGHT_ENVIRONMENT;
+        else if (type == "area") throw std::runtime_error("an area emitter belongs inside a <primitive>");
+        else throw std::runtime_error("unsupported emitter type: " + type);
+        LightData l = lightDefaults(t);
+        gnd_light_desc& d = l.desc;
+        auto put = [](float* dst, Vec3 v) { dst[0] = v.x; dst[1] = v.y; dst[2] = v.z; };
+        if (t == GND_LIGHT_POINT || t == GND_LIGHT_SPOT) {
+            if (!p.hasV("position")) throw std::runtime_error("Property 'position' is missing");
+            put(d.position, p.getV("position", Vec3()));
+            put(d.power, p.getV("power", Vec3(d.power[0], d.power[1], d.power[2])));
+        }
+        if (t == GND_LIGHT_SPOT || t == GND_LIGHT_DIRECTIONAL)
+            put(d.direction, p.getV("direction", Vec3(d.direction[0], d.direction[1], d.direction[2])));
+        if (t == GND_LIGHT_SPOT) {
+            d.spot_size = p.getF("spot_size", d.spot_size);
+            d.blend = p.getF("blend", d.blend);
+        }
+        if (t == GND_LIGHT_DIRECTIONAL) put(d.irradiance, p.getV("irradiance", Vec3(d.irradiance[0], d.irradiance[1], d.irradiance[2])));
+        if (t == GND_LIGHT_ENVIRONMENT) {
+            const std::string f = p.getS("filename", "");
+            const std::string path = (!f.empty() && f[0] == '/') ? f : baseDir + f;
+            std::ifstream probe(path, std::ios::binary);
+            if (f.empty() || !probe) throw std::runtime_error("EnvMap: Invalid environment map path.");
+            const size_t dot = path.find_last_of('.');
+            if (dot == std::string::npos || path.substr(dot) != ".exr")
+                throw std::runtime_error("EnvMap: Invalid environment map extension.");
+            readEXR(path, d.env_width, d.env_height, l.env);
+            d.strength = p.getF("strength", 1.0f);
+        }
+        sc.addLight(std::move(l));
+    }
+
     void readCamera(const XNode& n) {
         std::vector<const XNode*> kids;
         Props p = readProps(n, kids);
@@ -462,6 +508,8 @@ struct Loader {
                 sc.maxDepth = ip.getI("maxDepth", 1000);
             } else if (k->tag == "primitive") {
                 readPrimitive(*k);
+            } else if (k->tag == "emitter") {
+                readEmitter(*k);
             } else {
                 throw std::runtime_error("Scene: cannot add specified child!");
             }

@@ -96,6 +96,13 @@ void gnd_host_integrator_free(gnd_host_integrator* it);
 int gnd_host_integrator_names(char* buf, int len);
 /* Writes a width*height*3 float RGB image as ".png", ".exr" or ".pfm" (the film writers). */
 int gnd_host_image_save(const char* path, const float* rgb, int width, int height);
+/* Reads a scanline OpenEXR file the way Film(const std::string&) does
+ * (src/core/film.cpp:32-48 via Image2D::loadEXR, include/gianduia/core/
+ * image2d.h:100-134): R, G, B as floats over the data window, a missing
+ * channel read as 0.  NONE / RLE / ZIPS / ZIP compression, UINT / HALF /
+ * FLOAT channels.  Writes the size; with rgb non-NULL also the pixels
+ * (width*height*3 floats, capacity = floats available). */
+int gnd_host_image_load(const char* path, int* width, int* height, float* rgb, size_t capacity);
 /* IndependentSampler (src/samplers/independent.cpp:7-47): the first n floats
  * of sample index `sample_index`. */
 int gnd_host_sampler_floats(uint64_t sample_index, int n, float* out);

@@ -20,6 +20,8 @@
  *                 (src/materials/{homogeneous,heterogeneous}.cpp).
  *   camera      — perspective / thinlens + film filter
  *                 (src/cameras/{perspective,thinLens}.cpp, include/gianduia/core/filter.h).
+ *   light       — point / spot / directional / environment emitters
+ *                 (src/emitters/{pointlight,spotlight,directional,environment}.cpp).
  *
  * Matrices are column-major float[16] (m[c*4 + r]), the memory order of the
  * reference's glm::mat4.
@@ -47,6 +49,7 @@ enum { GND_MEDIUM_HOMOGENEOUS = 0, GND_MEDIUM_GRID = 1 };
 enum { GND_CAMERA_PERSPECTIVE = 0, GND_CAMERA_THINLENS = 1 };
 enum { GND_FILTER_GAUSSIAN = 0, GND_FILTER_BOX = 1 };
 enum { GND_INTEGRATOR_PATH = 0, GND_INTEGRATOR_VOLPATH = 1 };
+enum { GND_LIGHT_POINT = 0, GND_LIGHT_SPOT = 1, GND_LIGHT_DIRECTIONAL = 2, GND_LIGHT_ENVIRONMENT = 3 };
 
 typedef struct gnd_mesh_desc {
     uint32_t ntris;
@@ -115,6 +118,28 @@ typedef struct gnd_camera_desc {
     float filter_alpha;
 } gnd_camera_desc;
 
+/* An emitter that is not a shape (Scene::addChild's EEmitter case,
+ * src/scene/scene.cpp:33-40).  The scene's emitter list, from which NEE picks
+ * uniformly (scene.h:28-34), holds the area emitters of the primitives and
+ * these lights in document order: `emitter_index` is this light's position in
+ * it; the area emitters fill the other positions in primitive order.  At most
+ * one environment light. */
+typedef struct gnd_light_desc {
+    int32_t type;            /* GND_LIGHT_*                                            */
+    int32_t emitter_index;
+    float position[3];       /* point, spot                                            */
+    float direction[3];      /* spot: cone axis; directional: direction the light
+                                travels ("direction", default (0,-1,0))                 */
+    float power[3];          /* point, spot (default 10)                               */
+    float irradiance[3];     /* directional (default 1)                                */
+    float spot_size;         /* spot: full cone angle in degrees (default 45)          */
+    float blend;             /* spot: soft-edge fraction (default 0.15)                */
+    float strength;          /* environment (default 1)                                */
+    int32_t env_width, env_height;
+    const float* env_rgb;    /* environment: width*height RGB floats, row 0 = theta 0
+                                (+z), column 0 = phi 0 (the EXR's pixel layout)         */
+} gnd_light_desc;
+
 typedef struct gnd_scene_desc {
     uint32_t nmeshes;
     const gnd_mesh_desc* meshes;
@@ -128,6 +153,8 @@ typedef struct gnd_scene_desc {
     int32_t integrator;
     int32_t max_depth;
     uint32_t spp;
+    uint32_t nlights;
+    const gnd_light_desc* lights;
 } gnd_scene_desc;
 
 /* Struct-of-arrays ray batch (32 B per ray) and hit batch (20 B per ray).   */

@@ -310,12 +310,44 @@ struct Camera {
     float aspect = 1.0f;
 };
 
+// Distribution1D / Distribution2D (src/math/distribution.cpp:6-81)
+struct Distribution1D {
+    std::vector<float> func, cdf;
+    float funcInt = 0.0f;
+    void build(const float* f, int n);
+    int count() const { return (int)func.size(); }
+    float sampleContinuous(float u, float* pdf, int* off) const;
+};
+struct Distribution2D {
+    std::vector<Distribution1D> conditional;  // one per row (v), over the columns (u)
+    Distribution1D marginal;
+    void build(const float* data, int nu, int nv);
+    P2 sampleContinuous(P2 u, float* pdf) const;
+    float pdf(P2 p) const;
+};
+// Point / spot / directional / environment emitters
+// (src/emitters/{pointlight,spotlight,directional,environment}.cpp)
+struct Light {
+    gnd_light_desc d;
+    V3 position, direction;  // spot: normalised cone axis; directional: towards the light
+    C3 power, irradiance;
+    float cosTotalWidth = 0.0f, cosFalloffStart = 0.0f;
+    int width = 0, height = 0;  // environment image
+    std::vector<C3> rgb;
+    float strength = 1.0f;
+    Distribution2D dist;
+    bool isDelta() const { return d.type != GND_LIGHT_ENVIRONMENT; }
+};
+
 struct Scene {
     std::vector<Mesh> meshes;
     std::vector<Prim> prims;            // scene order
     std::vector<int> tlasPrims;         // TLAS leaf slot -> prim index (BVH::build reorder)
     std::vector<BVHNode4> tlas;
-    std::vector<int> emitters;          // prim index of each emitter (scene addChild order)
+    std::vector<int> emitters;          // scene addChild order: prim index (area emitter)
+                                        // or -(light index + 1)
+    std::vector<Light> lights;
+    int envLight = -1;                  // index into lights of the environment, if any
     std::vector<Material> materials;
     std::vector<Medium> media;
     Camera camera;
@@ -325,6 +357,7 @@ struct Scene {
 };
 
 void sceneBuild(Scene& s, const gnd_scene_desc& d);
+Light lightFromDesc(const gnd_light_desc& d);
 
 bool intersectClosest(const Scene& s, Ray& ray, Hit& hit, Counters* c);
 bool intersectAny(const Scene& s, Ray& ray, Counters* c);

@@ -2,6 +2,7 @@
 // and BVH4 traversal, restated from the reference.  Test infrastructure only.
 #include "gnd_ref.h"
 
+#include <climits>
 #include <memory>
 #include <stdexcept>
 
@@ -367,6 +368,91 @@ static void mediumFromDesc(Medium& md, const gnd_medium_desc& d) {
 }
 
 // Scene::activate + BVH::build (src/scene/scene.cpp:46-62, src/scene/bvh.cpp:10-36)
+// Distribution1D (distribution.cpp:6-18)
+void Distribution1D::build(const float* f, int n) {
+    func.assign(f, f + n);
+    cdf.assign(n + 1, 0.0f);
+    cdf[0] = 0.0f;
+    for (int i = 1; i < n + 1; ++i) cdf[i] = cdf[i - 1] + func[i - 1] / n;
+    funcInt = cdf[n];
+    if (funcInt == 0.0f) {
+        for (int i = 1; i < n + 1; ++i) cdf[i] = float(i) / float(n);
+    } else {
+        for (int i = 1; i < n + 1; ++i) cdf[i] /= funcInt;
+    }
+}
+// Distribution1D::SampleContinuous (distribution.cpp:20-34)
+float Distribution1D::sampleContinuous(float u, float* pdf, int* off) const {
+    auto it = std::upper_bound(cdf.begin(), cdf.end(), u);
+    int offset = std::clamp<int>((int)std::distance(cdf.begin(), it) - 1, 0, (int)cdf.size() - 2);
+    if (off) *off = offset;
+    float du = u - cdf[offset];
+    if ((cdf[offset + 1] - cdf[offset]) > 0.0f) du /= (cdf[offset + 1] - cdf[offset]);
+    if (pdf) *pdf = (funcInt > 0.0f) ? func[offset] / funcInt : 0.0f;
+    return (offset + du) / count();
+}
+// Distribution2D (distribution.cpp:53-80)
+void Distribution2D::build(const float* data, int nu, int nv) {
+    conditional.assign(nv, Distribution1D());
+    std::vector<float> marginalFunc(nv);
+    for (int v = 0; v < nv; ++v) {
+        conditional[v].build(&data[(size_t)v * nu], nu);
+        marginalFunc[v] = conditional[v].funcInt;
+    }
+    marginal.build(marginalFunc.data(), nv);
+}
+P2 Distribution2D::sampleContinuous(P2 u, float* pdf) const {
+    float pdfs[2];
+    int v;
+    float d1 = marginal.sampleContinuous(u.y, &pdfs[1], &v);
+    float d0 = conditional[v].sampleContinuous(u.x, &pdfs[0], nullptr);
+    *pdf = pdfs[0] * pdfs[1];
+    return {d0, d1};
+}
+float Distribution2D::pdf(P2 p) const {
+    int iu = std::clamp<int>((int)(p.x * conditional[0].count()), 0, conditional[0].count() - 1);
+    int iv = std::clamp<int>((int)(p.y * marginal.count()), 0, marginal.count() - 1);
+    return conditional[iv].func[iu] / marginal.funcInt;
+}
+
+// Light constructors (pointlight.cpp:9-12, spotlight.cpp:9-22,
+// directional.cpp:8-11, environment.cpp:14-27 with buildPDFs :106-120)
+Light lightFromDesc(const gnd_light_desc& d) {
+    Light L;
+    L.d = d;
+    L.d.env_rgb = nullptr;
+    L.position = V3(d.position[0], d.position[1], d.position[2]);
+    L.power = C3(d.power[0], d.power[1], d.power[2]);
+    L.irradiance = C3(d.irradiance[0], d.irradiance[1], d.irradiance[2]);
+    const V3 dir(d.direction[0], d.direction[1], d.direction[2]);
+    if (d.type == GND_LIGHT_SPOT) {
+        L.direction = Normalize(dir);
+        float halfAngle = d.spot_size * 0.5f;
+        float innerAngle = halfAngle * (1.0f - d.blend);
+        L.cosTotalWidth = std::cos((kPi / 180.f) * halfAngle);
+        L.cosFalloffStart = std::cos((kPi / 180.f) * innerAngle);
+    } else if (d.type == GND_LIGHT_DIRECTIONAL) {
+        L.direction = -Normalize(dir);
+    } else if (d.type == GND_LIGHT_ENVIRONMENT) {
+        if (!d.env_rgb || d.env_width <= 0 || d.env_height <= 0)
+            throw std::runtime_error("environment light without an image");
+        L.width = d.env_width;
+        L.height = d.env_height;
+        L.strength = d.strength;
+        L.rgb.resize((size_t)L.width * L.height);
+        for (size_t i = 0; i < L.rgb.size(); ++i)
+            L.rgb[i] = C3(d.env_rgb[3 * i], d.env_rgb[3 * i + 1], d.env_rgb[3 * i + 2]);
+        const int rows = L.height, cols = L.width;
+        std::vector<float> img((size_t)rows * cols);
+        for (int i = 0; i < rows; i++) {
+            float sinTheta = std::sin(M_PI * (i + 0.5f) / rows);  // double sine, stored as float
+            for (int j = 0; j < cols; j++) img[(size_t)i * cols + j] = L.rgb[(size_t)i * cols + j].luminance() * sinTheta;
+        }
+        L.dist.build(img.data(), cols, rows);
+    }
+    return L;
+}
+
 void sceneBuild(Scene& s, const gnd_scene_desc& d) {
     s.meshes.resize(d.nmeshes);
     for (uint32_t i = 0; i < d.nmeshes; ++i) {
@@ -389,10 +475,7 @@ void sceneBuild(Scene& s, const gnd_scene_desc& d) {
         p.toWorld = toM4(pd.to_world);
         p.toWorldInv = toM4(pd.to_world_inv);
         p.radiance = C3(pd.radiance[0], pd.radiance[1], pd.radiance[2]);
-        if (pd.has_emitter) {
-            p.emitter = (int)s.emitters.size();
-            s.emitters.push_back((int)i);
-        }
+        if (pd.has_emitter) p.emitter = 0;  // slot assigned below
         Bounds sb;
         if (p.shapeType == GND_SHAPE_MESH) sb = s.meshes[p.mesh].bounds;
         else { sb.pMin = V3(-p.radius, -p.radius, -p.radius); sb.pMax = V3(p.radius, p.radius, p.radius); }
@@ -401,6 +484,35 @@ void sceneBuild(Scene& s, const gnd_scene_desc& d) {
         wb.pMin = wb.pMax = xfPoint(p.toWorld, sb.corner(0));
         for (int c = 1; c < 8; ++c) wb = Union(wb, xfPoint(p.toWorld, sb.corner(c)));
         p.worldBounds = wb;
+    }
+    // emitter list in document order: lights at their emitter_index, area
+    // emitters of the primitives in the other slots in primitive order
+    // (Scene::addChild, scene.cpp:9-40)
+    s.lights.clear();
+    s.envLight = -1;
+    int nArea = 0;
+    for (auto& p : s.prims) nArea += p.emitter >= 0 ? 1 : 0;
+    const int nEm = nArea + (int)d.nlights;
+    s.emitters.assign(nEm, INT32_MIN);
+    for (uint32_t k = 0; k < d.nlights; ++k) {
+        Light L = lightFromDesc(d.lights[k]);
+        const int e = d.lights[k].emitter_index;
+        if (e < 0 || e >= nEm || s.emitters[e] != INT32_MIN) throw std::runtime_error("bad light emitter_index");
+        if (L.d.type == GND_LIGHT_ENVIRONMENT) {
+            if (s.envLight >= 0) throw std::runtime_error("Scene: cannot add multiple infinite area lights!");
+            s.envLight = (int)k;
+        }
+        s.emitters[e] = -(int)k - 1;
+        s.lights.push_back(std::move(L));
+    }
+    {
+        int slot = 0;
+        for (uint32_t i = 0; i < d.nprims; ++i) {
+            if (s.prims[i].emitter < 0) continue;
+            while (s.emitters[slot] != INT32_MIN) ++slot;
+            s.prims[i].emitter = slot;
+            s.emitters[slot] = (int)i;
+        }
     }
     std::vector<BuildInfo> data;
     for (uint32_t i = 0; i < d.nprims; ++i) {

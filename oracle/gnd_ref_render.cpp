@@ -617,6 +617,153 @@ static int pickEmitter(const Scene& s, float rnd) {  // scene.h:28-34
     return s.emitters[idx];
 }
 
+// ------------------------------------------------------------ lights (src/emitters/*.cpp)
+// SphericalCoordinates / SphericalDirection (constants.h:80-100)
+static P2 sphericalCoordinates(V3 w) {
+    P2 r{std::acos(std::clamp(w.z, -1.0f, 1.0f)), std::atan2(w.y, w.x)};
+    if (r.y < 0.0f) r.y += 2.0f * kPi;
+    return r;
+}
+static V3 sphericalDirection(float theta, float phi) {
+    float cosTheta = std::cos(theta), sinTheta = std::sin(theta);
+    float cosPhi = std::cos(phi), sinPhi = std::sin(phi);
+    return V3(sinTheta * cosPhi, sinTheta * sinPhi, cosTheta);
+}
+// Film::getPixelBilinear (film.cpp:125-151): u wraps, v clamps
+static C3 envBilinear(const Light& L, float u, float v) {
+    const int W = L.width, H = L.height;
+    float x = u * W - 0.5f;
+    float y = v * H - 0.5f;
+    int x0 = (int)std::floor(x);
+    int y0 = (int)std::floor(y);
+    float dx = x - x0;
+    float dy = y - y0;
+    int px0 = (x0 % W + W) % W;
+    int px1 = (px0 + 1) % W;
+    int py0 = std::clamp(y0, 0, H - 1);
+    int py1 = std::clamp(y0 + 1, 0, H - 1);
+    C3 c00 = L.rgb[(size_t)py0 * W + px0], c10 = L.rgb[(size_t)py0 * W + px1];
+    C3 c01 = L.rgb[(size_t)py1 * W + px0], c11 = L.rgb[(size_t)py1 * W + px1];
+    C3 c0 = c00 * (1.0f - dx) + c10 * dx;
+    C3 c1 = c01 * (1.0f - dx) + c11 * dx;
+    return c0 * (1.0f - dy) + c1 * dy;
+}
+// EnvironmentMap::eval (environment.cpp:31-40)
+static C3 envEval(const Light& L, V3 w) {
+    P2 sph = sphericalCoordinates(w);
+    float u = sph.y * M_1_PI * 0.5f;
+    float v = sph.x * M_1_PI;
+    return envBilinear(L, u, v) * L.strength;  // float * Color3f, one product per channel
+}
+// EnvironmentMap::pdf (environment.cpp:75-91)
+static float envPdf(const Light& L, V3 refP, V3 infoP) {
+    V3 wi = Normalize(infoP - refP);
+    P2 sph = sphericalCoordinates(wi);
+    float sinTheta = std::sin(sph.x);
+    if (sinTheta <= 0.0f) return 0.0f;
+    float u = sph.y * M_1_PI * 0.5f;
+    float v = sph.x * M_1_PI;
+    float mapPdf = L.dist.pdf({u, v});
+    return mapPdf / (2.0f * M_PI * M_PI * sinTheta);
+}
+// Emitter::sample of the non-area emitters (pointlight.cpp:18-33,
+// spotlight.cpp:28-60, directional.cpp:17-33, environment.cpp:42-73)
+static C3 lightSample(const Light& L, V3 refP, P2 u, LightSample& ls) {
+    switch (L.d.type) {
+        case GND_LIGHT_POINT: {
+            C3 I = L.power * kInv4Pi;
+            float squaredDist = LengthSq(L.position - refP);
+            ls.p = L.position;
+            ls.pdf = 1.0f;
+            ls.shadow.o = refP;
+            ls.shadow.d = Normalize(ls.p - refP);
+            ls.shadow.tMin = kEpsilon;
+            ls.shadow.tMax = Length(ls.p - refP) - kEpsilon;
+            return I / squaredDist;
+        }
+        case GND_LIGHT_SPOT: {
+            V3 wi = Normalize(L.position - refP);
+            float dist = Length(L.position - refP);
+            float squaredDist = dist * dist;
+            ls.p = L.position;
+            ls.pdf = 1.0f;
+            ls.shadow.o = refP;
+            ls.shadow.d = wi;
+            ls.shadow.tMin = kEpsilon;
+            ls.shadow.tMax = dist - kEpsilon;
+            float cosTheta = Dot(-wi, L.direction);
+            if (cosTheta <= L.cosTotalWidth) return C3(0.0f);
+            float falloff = 1.0f;
+            if (cosTheta < L.cosFalloffStart) {
+                float t = (cosTheta - L.cosTotalWidth) / (L.cosFalloffStart - L.cosTotalWidth);
+                falloff = (t * t) * (3.0f - 2.0f * t);
+            }
+            C3 I = L.power * kInv4Pi;
+            return (I * falloff) / squaredDist;
+        }
+        case GND_LIGHT_DIRECTIONAL: {
+            V3 wi = L.direction;
+            ls.p = refP + wi * 1e5f;
+            ls.pdf = 1.0f;
+            ls.shadow.o = refP;
+            ls.shadow.d = wi;
+            ls.shadow.tMin = kEpsilon;
+            ls.shadow.tMax = std::numeric_limits<float>::max();
+            return L.irradiance;
+        }
+        default: {  // environment
+            float mapPdf;
+            P2 uv = L.dist.sampleContinuous(u, &mapPdf);
+            if (mapPdf == 0.0f) {
+                ls.pdf = 0.0f;
+                return C3(0.0f);
+            }
+            float theta = uv.y * M_PI;
+            float phi = uv.x * 2.0f * M_PI;
+            V3 wi = sphericalDirection(theta, phi);
+            ls.p = refP + wi * 1e5f;
+            ls.n = -wi;
+            ls.shadow = Ray();
+            ls.shadow.o = refP;
+            ls.shadow.d = wi;
+            float sinTheta = std::sin(theta);
+            if (sinTheta <= 0.0f) {
+                ls.pdf = 0.0f;
+                return C3(0.0f);
+            }
+            ls.pdf = mapPdf / (2.0f * M_PI * M_PI * sinTheta);
+            return envEval(L, wi) / ls.pdf;
+        }
+    }
+}
+// one NEE light sample from emitter-list entry `e` (area or other light);
+// `delta` is Emitter::isDelta
+static C3 emitterSample(const Scene& s, int e, V3 refP, P2 u, LightSample& ls, bool& pdfSet, bool& delta) {
+    if (e >= 0) {
+        delta = false;
+        return areaSample(s, s.prims[e], refP, u, ls, pdfSet);
+    }
+    const Light& L = s.lights[-e - 1];
+    delta = L.isDelta();
+    pdfSet = true;
+    return lightSample(L, refP, u, ls);
+}
+// the environment seen by a ray that left the scene, with the MIS weight
+// against NEE (path.cpp:36-55, volPath.cpp:117-139); pdf against the origin
+// as the reference's default SurfaceInteraction reference point
+static bool envMiss(const Scene& s, const Ray& r, bool specular, float pdfPrev, C3& Le, float& w) {
+    if (s.envLight < 0) return false;
+    const Light& L = s.lights[s.envLight];
+    Le = envEval(L, r.d);
+    if (Le.isBlack()) return false;
+    w = 1.0f;
+    if (!specular) {
+        float pe = envPdf(L, V3(0, 0, 0), r.o + r.d * 1e5f) * (1.0f / s.emitters.size());
+        w = powerHeuristic(pdfPrev, pe);
+    }
+    return true;
+}
+
 // ------------------------------------------------------------ media
 struct MediumEvent {
     bool valid = false;
@@ -897,8 +1044,9 @@ static V3 cameraForward(const Camera& c) { return xfVector(c.toWorld, V3(0.0f, 0
 
 // The G-buffer writes of both integrators (path.cpp:35-61 and 99-106,
 // volPath.cpp:117-146 and 184-191).  A miss: depth / metallic / roughness
-// reset at bounce 0, albedo = environment (none here, so black), normal 0.
-static void aovMiss(AOV* a, int bounces, bool& needs) {
+// reset at bounce 0, albedo = the environment along the ray (black without
+// one), normal 0.
+static void aovMiss(AOV* a, const Scene& s, V3 dir, int bounces, bool& needs) {
     if (!a) return;
     if (bounces == 0) {
         a->depth = -1.0f;
@@ -906,7 +1054,7 @@ static void aovMiss(AOV* a, int bounces, bool& needs) {
         a->roughness = 0.0f;
     }
     if (needs) {
-        a->albedo = C3(0.0f);
+        a->albedo = s.envLight >= 0 ? envEval(s.lights[s.envLight], dir) : C3(0.0f);
         a->normal = V3(0.0f, 0.0f, 0.0f);
         needs = false;
     }
@@ -944,7 +1092,10 @@ static C3 LiPath(const Scene& s, Ray r, Sampler& smp, Counters* cnt, AOV* aov) {
     while (bounces < s.maxDepth) {
         Hit h;
         if (!intersectClosest(s, r, h, cnt)) {
-            aovMiss(aov, bounces, needsG);
+            C3 Le;
+            float we;
+            if (envMiss(s, r, specular, pdfPrev, Le, we)) L = L + tp * Le * we;
+            aovMiss(aov, s, r.d, bounces, needsG);
             break;
         }
         Isect it;
@@ -980,7 +1131,8 @@ static C3 LiPath(const Scene& s, Ray r, Sampler& smp, Counters* cnt, AOV* aov) {
         int ep = pickEmitter(s, smp.next1D());
         LightSample ls;
         bool pdfSet;
-        C3 Li = areaSample(s, s.prims[ep], it.p, smp.next2D(), ls, pdfSet);
+        bool delta;
+        C3 Li = emitterSample(s, ep, it.p, smp.next2D(), ls, pdfSet, delta);
         Li = Li / lsp;
         if (pdfSet && ls.pdf > 1e-6f && !Li.isBlack()) {
             bool occluded = false;
@@ -1004,7 +1156,7 @@ static C3 LiPath(const Scene& s, Ray r, Sampler& smp, Counters* cnt, AOV* aov) {
                 float bp = bsdf.pdf(-r.d, wi);
                 C3 f = bsdf.f(-r.d, wi);
                 if (!f.isBlack() && bp > 1e-6f) {
-                    float wl = powerHeuristic(ls.pdf * lsp, bp);
+                    float wl = powerHeuristic(ls.pdf * lsp, delta ? 0.0f : bp);
                     L = L + tp * f * Li * std::abs(Dot(it.n, wi)) * wl;
                 }
             }
@@ -1088,7 +1240,8 @@ static C3 LiVolPath(const Scene& s, Ray r, Sampler& smp, Counters* cnt, AOV* aov
             int ep = pickEmitter(s, smp.next1D());
             LightSample ls;
             bool pdfSet;
-            C3 Li = areaSample(s, s.prims[ep], mi.p, smp.next2D(), ls, pdfSet);
+            bool delta;
+            C3 Li = emitterSample(s, ep, mi.p, smp.next2D(), ls, pdfSet, delta);
             Li = Li / lsp;
             if (pdfSet && ls.pdf > 1e-6f && !Li.isBlack()) {
                 Ray sr = ls.shadow;
@@ -1098,7 +1251,7 @@ static C3 LiVolPath(const Scene& s, Ray r, Sampler& smp, Counters* cnt, AOV* aov
                     V3 wi = Normalize(ls.p - mi.p);
                     float pp = hgPhase(Dot(mi.wo, wi), mi.g);
                     if (pp > 1e-6f) {
-                        float wl = powerHeuristic(ls.pdf * lsp, pp);
+                        float wl = powerHeuristic(ls.pdf * lsp, delta ? 0.0f : pp);
                         L = L + tp * pp * Li * wl * Tr;
                     }
                 }
@@ -1124,7 +1277,10 @@ static C3 LiVolPath(const Scene& s, Ray r, Sampler& smp, Counters* cnt, AOV* aov
         }
         int curMedium = r.medium;
         if (!hitSurface) {
-            aovMiss(aov, bounces, needsG);
+            C3 Le;
+            float we;
+            if (envMiss(s, r, specular, pdfPrev, Le, we)) L = L + tp * Le * we;
+            aovMiss(aov, s, r.d, bounces, needsG);
             break;
         }
         Isect it;
@@ -1160,7 +1316,8 @@ static C3 LiVolPath(const Scene& s, Ray r, Sampler& smp, Counters* cnt, AOV* aov
         int ep = pickEmitter(s, smp.next1D());
         LightSample ls;
         bool pdfSet;
-        C3 Li = areaSample(s, s.prims[ep], it.p, smp.next2D(), ls, pdfSet);
+        bool delta;
+        C3 Li = emitterSample(s, ep, it.p, smp.next2D(), ls, pdfSet, delta);
         Li = Li / lsp;
         if (pdfSet && ls.pdf > 1e-6f && !Li.isBlack()) {
             Ray sr = ls.shadow;
@@ -1171,7 +1328,7 @@ static C3 LiVolPath(const Scene& s, Ray r, Sampler& smp, Counters* cnt, AOV* aov
                 float bp = bsdf.pdf(-r.d, wi);
                 C3 f = bsdf.f(-r.d, wi);
                 if (!f.isBlack() && bp > 1e-6f) {
-                    float wl = powerHeuristic(ls.pdf * lsp, bp);
+                    float wl = powerHeuristic(ls.pdf * lsp, delta ? 0.0f : bp);
                     L = L + tp * f * Li * std::abs(Dot(it.n, wi)) * wl * Tr;
                 }
             }
@@ -1555,6 +1712,26 @@ int gnd_ref_medium(void* h, int medium, int which, const float* o, const float* 
 float gnd_ref_grid_sample(void* h, int medium, double x, double y, double z) {
     Scene* s = static_cast<Scene*>(h);
     return gridSample(s->media[medium], x, y, z);
+}
+
+// One NEE sample of light `k` from refP with the 2D sample u: out = sampled
+// point (3), pdf, returned radiance Li (3), shadow direction (3), tMax.
+void gnd_ref_light_sample(void* h, int k, const float* refP, const float* u, float* out) {
+    Scene* s = static_cast<Scene*>(h);
+    LightSample ls;
+    C3 Li = lightSample(s->lights[k], V3(refP[0], refP[1], refP[2]), P2{u[0], u[1]}, ls);
+    const float v[11] = {ls.p.x, ls.p.y, ls.p.z, ls.pdf, Li.r, Li.g, Li.b,
+                         ls.shadow.d.x, ls.shadow.d.y, ls.shadow.d.z, ls.shadow.tMax};
+    for (int i = 0; i < 11; ++i) out[i] = v[i];
+}
+// EnvironmentMap::pdf towards direction dir from the origin, and eval along it
+float gnd_ref_env_pdf(void* h, const float* dir, float* Le) {
+    Scene* s = static_cast<Scene*>(h);
+    const Light& L = s->lights[s->envLight];
+    V3 w(dir[0], dir[1], dir[2]);
+    C3 e = envEval(L, w);
+    if (Le) { Le[0] = e.r; Le[1] = e.g; Le[2] = e.b; }
+    return envPdf(L, V3(0, 0, 0), w * 1e5f);
 }
 
 // Single-path radiance for (pixel, pass): out rgb, then (if aov is not null)

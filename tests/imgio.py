@@ -56,3 +56,92 @@ def read_pfm(data: bytes) -> np.ndarray:
     assert kind in (b"PF", b"Pf") and scale < 0
     c = 3 if kind == b"PF" else 1
     return np.frombuffer(head[3], "<f4").reshape(h, w, c)[::-1]
+
+
+def _half_bits(a: np.ndarray) -> np.ndarray:
+    return np.asarray(a, np.float16).view("<u2")
+
+
+def _rle(raw: bytes) -> bytes:
+    """OpenEXR RLE: runs of 3+ equal bytes as (count - 1, byte), the rest as
+    (-literal count, bytes), counts up to 127 / 128."""
+    out, i, n = bytearray(), 0, len(raw)
+    while i < n:
+        j = i
+        while j + 1 < n and raw[j + 1] == raw[i] and j - i < 127:
+            j += 1
+        if j - i >= 2:
+            out += bytes([j - i]) + raw[i:i + 1]
+            i = j + 1
+            continue
+        k = i
+        while k < n and k - i < 127 and not (k + 2 < n and raw[k] == raw[k + 1] == raw[k + 2]):
+            k += 1
+        out += bytes([(-(k - i)) & 0xFF]) + raw[i:k]
+        i = k
+    return bytes(out)
+
+
+def _pre(raw: bytes) -> bytes:
+    """The byte reordering ZIP and RLE apply first: even-indexed bytes then
+    odd-indexed ones, then each byte replaced by its difference to the one
+    before, plus 128 (mod 256)."""
+    b = np.frombuffer(raw, np.uint8)
+    t = np.concatenate([b[0::2], b[1::2]]).astype(np.int32)
+    d = t.copy()
+    d[1:] = (t[1:] - t[:-1] + 128) & 0xFF
+    return d.astype(np.uint8).tobytes()
+
+
+def write_exr(path, channels: dict, ptype: int = 2, compression: int = 0, origin=(0, 0)):
+    """Scanline single-part OpenEXR writer from the published file layout:
+    channels {name: (H, W) array} stored sorted by name with pixel type
+    `ptype` (0 UINT, 1 HALF, 2 FLOAT) and `compression` 0 NONE, 1 RLE,
+    2 ZIPS, 3 ZIP; the data window starts at `origin` (x, y).  A block that
+    does not shrink is stored raw, as the format prescribes."""
+    import zlib
+    names = sorted(channels)
+    h, w = next(iter(channels.values())).shape
+    x0, y0 = origin
+
+    def attr(name, typ, payload):
+        return name.encode() + b"\0" + typ.encode() + b"\0" + struct.pack("<i", len(payload)) + payload
+
+    chl = b"".join(n.encode() + b"\0" + struct.pack("<iB3xii", ptype, 0, 1, 1) for n in names) + b"\0"
+    box = struct.pack("<iiii", x0, y0, x0 + w - 1, y0 + h - 1)
+    head = struct.pack("<ii", 20000630, 2)
+    head += attr("channels", "chlist", chl) + attr("compression", "compression", bytes([compression]))
+    head += attr("dataWindow", "box2i", box) + attr("displayWindow", "box2i", box)
+    head += attr("lineOrder", "lineOrder", b"\0") + attr("pixelAspectRatio", "float", struct.pack("<f", 1.0))
+    head += attr("screenWindowCenter", "v2f", struct.pack("<ff", 0, 0))
+    head += attr("screenWindowWidth", "float", struct.pack("<f", 1.0)) + b"\0"
+    lpb = 16 if compression == 3 else 1
+    blocks = []
+    for b0 in range(0, h, lpb):
+        raw = bytearray()
+        for y in range(b0, min(h, b0 + lpb)):
+            for n in names:
+                row = channels[n][y]
+                if ptype == 1:
+                    raw += _half_bits(row).tobytes()
+                elif ptype == 0:
+                    raw += np.asarray(row, "<u4").tobytes()
+                else:
+                    raw += np.asarray(row, "<f4").tobytes()
+        raw = bytes(raw)
+        if compression == 1:
+            packed = _rle(_pre(raw))
+        elif compression in (2, 3):
+            packed = zlib.compress(_pre(raw))
+        else:
+            packed = raw
+        if len(packed) >= len(raw):
+            packed = raw
+        blocks.append(struct.pack("<ii", y0 + b0, len(packed)) + packed)
+    table_end = len(head) + 8 * len(blocks)
+    offs, pos = [], table_end
+    for b in blocks:
+        offs.append(pos)
+        pos += len(b)
+    with open(path, "wb") as f:
+        f.write(head + struct.pack(f"<{len(offs)}Q", *offs) + b"".join(blocks))

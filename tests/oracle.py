@@ -61,6 +61,8 @@ _API = {
     "gnd_ref_medium": (C.c_int, [C.c_void_p, C.c_int, C.c_int, f32p, f32p, C.c_float, C.c_uint64, f32p]),
     "gnd_ref_grid_sample": (C.c_float, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double]),
     "gnd_ref_path_sample": (None, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_uint64, f32p, f32p]),
+    "gnd_ref_light_sample": (None, [C.c_void_p, C.c_int, f32p, f32p, f32p]),
+    "gnd_ref_env_pdf": (C.c_float, [C.c_void_p, f32p, f32p]),
 }
 
 _lib = None
@@ -177,6 +179,18 @@ class RefScene:
         out = np.zeros(7, np.float32)
         lib().gnd_ref_camera_ray(self.h, fp(vec(*pfilm)), fp(vec(*plens)), lam, fp(out))
         return out[:3], out[3:6], float(out[6])
+
+    def light_sample(self, k, ref_p, u):
+        """Light k's NEE sample from ref_p: dict(p, pdf, Li, dir, tmax)."""
+        out = np.zeros(11, np.float32)
+        lib().gnd_ref_light_sample(self.h, k, fp(vec(*ref_p)), fp(vec(*u)), fp(out))
+        return {"p": out[0:3], "pdf": float(out[3]), "Li": out[4:7], "dir": out[7:10], "tmax": float(out[10])}
+
+    def env_pdf(self, d):
+        """(solid-angle pdf of the environment light towards d, its radiance along d)."""
+        le = np.zeros(3, np.float32)
+        pdf = lib().gnd_ref_env_pdf(self.h, fp(vec(*d)), fp(le))
+        return pdf, le
 
     def path_sample(self, x, y, p, seed_offset=0, aov=False):
         """Radiance of one (pixel, pass) path; with aov also its AOV record

@@ -53,6 +53,28 @@ class SceneBuilder:
         self.materials.append(m)
         return len(self.materials) - 1
 
+    def light(self, kind, position=(0, 0, 0), direction=None, power=(10, 10, 10), irradiance=(1, 1, 1),
+              spot_size=45.0, blend=0.15, strength=1.0, env=None) -> int:
+        """Appends a light after every emitter added so far (document order)."""
+        if not hasattr(self, "lights"):
+            self.lights = []
+        l = A.gnd_light_desc()
+        l.type = {"point": A.GND_LIGHT_POINT, "spot": A.GND_LIGHT_SPOT, "directional": A.GND_LIGHT_DIRECTIONAL,
+                  "environment": A.GND_LIGHT_ENVIRONMENT}[kind]
+        l.emitter_index = sum(1 for p in self.prims if p.has_emitter) + len(self.lights)
+        l.position[:] = position
+        l.direction[:] = direction if direction is not None else ((0, 0, -1) if kind == "spot" else (0, -1, 0))
+        l.power[:] = power
+        l.irradiance[:] = irradiance
+        l.spot_size, l.blend, l.strength = spot_size, blend, strength
+        if env is not None:
+            env = np.ascontiguousarray(env, np.float32)
+            self.keep.append(env)
+            l.env_height, l.env_width = env.shape[0], env.shape[1]
+            l.env_rgb = env.ctypes.data_as(C.POINTER(C.c_float))
+        self.lights.append(l)
+        return len(self.lights) - 1
+
     def conductor(self, R=(0.9, 0.8, 0.7), roughness=0.0, eta=(0.2, 0.92, 1.1), k=(3.9, 2.45, 2.14)) -> int:
         m = A.gnd_material_desc()
         m.type = A.GND_MAT_CONDUCTOR
@@ -127,6 +149,12 @@ class SceneBuilder:
         d.integrator = A.GND_INTEGRATOR_PATH
         d.max_depth = 4
         d.spp = 1
+        lights = getattr(self, "lights", [])
+        if lights:
+            la = (A.gnd_light_desc * len(lights))(*lights)
+            self.keep.append(la)
+            d.nlights = len(lights)
+            d.lights = la
         # the desc holds raw pointers into arrays this builder owns: keep the
         # builder alive as long as the desc (`soup_scene(...).desc()` would
         # otherwise free them before the library copies the scene)

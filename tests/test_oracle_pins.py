@@ -12,7 +12,7 @@ from scipy import stats as sst
 from gianduia_amd import HostScene, _abi as A
 import oracle as O
 from oracle import RefScene, fp, vec
-from scenes import SceneBuilder, mirror_scene, random_rays, random_soup, trs
+from scenes import SceneBuilder, col_major, mirror_scene, quad, random_rays, random_soup, trs
 
 
 # ---------------------------------------------------------------- traversal vs brute force
@@ -474,3 +474,161 @@ def test_aov_depth_is_distance_along_forward():
             t = (-2.0 - o[2]) / dr[2]
             assert abs(a[y, x, 6] - t * dr @ fwd) < 1e-5 * t, (x, y)
     assert np.ptp(a[..., 6]) > 0.05  # tilted plane: depth varies across the image
+
+
+# ---------------------------------------------------------------- lights (src/emitters/*.cpp)
+def _plane_scene(albedo=0.5, look="down", W=16, H=12):
+    """A 100 x 100 matte plane at y = 0 facing +y; the camera 3 above it
+    looking straight down (or up at the sky from just above the plane)."""
+    b = SceneBuilder()
+    mt = b.matte((albedo, albedo, albedo))
+    tris = quad(-50, 50, -50, 50, 0.0, facing=+1)          # z = 0 plane facing +z ...
+    Mq, Mqi = trs(rot_deg=-90.0, axis=(1, 0, 0))            # ... turned to y = 0 facing +y
+    b.prim(b.mesh(tris), Mq, Mqi, material=mt)
+    rot = -90.0 if look == "down" else 90.0
+    M, Mi = trs(rot_deg=rot, axis=(1, 0, 0), trans=(0, 3.0 if look == "down" else 0.5, 0))
+    return b, (M, Mi), (W, H)
+
+
+def _desc(b, cam, wh, max_depth=1):
+    d = b.desc(width=wh[0], height=wh[1])
+    d.camera.to_world = col_major(cam[0])
+    d.camera.to_world_inv = col_major(cam[1])
+    d.max_depth = max_depth
+    return d
+
+
+def _first_hits(ref, W, H):
+    """Per pixel the pass-0 camera ray and its hit on the plane y = 0."""
+    pts = np.zeros((H, W, 3))
+    for y in range(H):
+        for x in range(W):
+            u = np.zeros(2, np.float32)
+            O.lib().gnd_ref_sampler_floats(y * W + x, 2, fp(u))
+            o, d, _ = ref.camera_ray(((x + u[0]) / W, 1.0 - (y + u[1]) / H))
+            o, d = o.astype(np.float64), d.astype(np.float64)
+            pts[y, x] = o + d * (-o[1] / d[1])
+    return pts
+
+
+@pytest.mark.parametrize("kind", ["point", "spot", "directional"])
+def test_delta_lights_on_a_plane_closed_form(kind):
+    """Depth 1 on a Lambertian plane: each pixel is NEE only, f * Li * cos with
+    MIS weight 1 (delta lights, path.cpp:150-152).  Point: I = P / 4pi over
+    d^2 (pointlight.cpp:18-33); spot: the same inside the cone with the
+    smoothstep falloff between the inner and outer angle (spotlight.cpp:28-60);
+    directional: the irradiance along -direction (directional.cpp:17-33)."""
+    b, cam, (W, H) = _plane_scene()
+    P = np.array([0.4, 2.0, -0.3])
+    if kind == "point":
+        b.light("point", position=P, power=(40, 30, 20))
+    elif kind == "spot":
+        b.light("spot", position=(0.1, 1.5, 0.2), direction=(0, -1, 0), power=(40, 30, 20), spot_size=60.0, blend=0.4)
+    else:
+        b.light("directional", direction=(0.3, -1.0, 0.2), irradiance=(2.0, 1.5, 1.0))
+    d = _desc(b, cam, (W, H))
+    ref = RefScene(d)
+    img, _ = ref.render(threads=4, spp=1, width=W, height=H)
+    assert np.all(img[..., 3] == 1.0)
+    pts = _first_hits(ref, W, H)
+    f = 0.5 / np.pi
+    if kind == "directional":
+        wi = -np.array([0.3, -1.0, 0.2]) / np.linalg.norm([0.3, -1.0, 0.2])
+        want = f * np.array([2.0, 1.5, 1.0]) * wi[1] * np.ones((H, W, 1))
+    else:
+        Lp = P if kind == "point" else np.array([0.1, 1.5, 0.2])
+        v = Lp - pts
+        d2 = (v * v).sum(-1)
+        wi = v / np.sqrt(d2)[..., None]
+        I = np.array([40, 30, 20]) / (4 * np.pi)
+        want = f * I * (wi[..., 1] / d2)[..., None]
+        if kind == "spot":
+            cos_t = wi[..., 1]                                   # Dot(-wi, (0,-1,0))
+            c_out, c_in = np.cos(np.radians(30.0)), np.cos(np.radians(30.0 * 0.6))
+            t = np.clip((cos_t - c_out) / (c_in - c_out), 0, 1)
+            fall = np.where(cos_t <= c_out, 0.0, np.where(cos_t < c_in, t * t * (3 - 2 * t), 1.0))
+            want = want * fall[..., None]
+            assert (fall == 0).any() and (fall == 1).any() and ((fall > 0) & (fall < 1)).any()
+    assert np.allclose(img[..., :3], want, rtol=2e-4, atol=1e-6)
+
+
+def _bright_square_env(W=16, H=8):
+    """The reference's test map (tests/emitters/environment_test.cpp:17-28):
+    10 inside u, v in (0.4, 0.6), else 0.1."""
+    env = np.full((H, W, 3), 0.1, np.float32)
+    for y in range(H):
+        for x in range(W):
+            u, v = x / W, y / H
+            if 0.4 < u < 0.6 and 0.4 < v < 0.6:
+                env[y, x] = 10.0
+    return env
+
+
+def test_environment_seen_directly_and_aov():
+    """Camera rays that leave the scene at bounce 0 see the map (eval with
+    weight 1, path.cpp:41-55): for a constant map every pixel is its value
+    times the strength, and the AOV albedo is the same (path.cpp:56-58)."""
+    b, cam, (W, H) = _plane_scene(look="up")
+    b.light("environment", env=np.full((4, 8, 3), (0.25, 0.5, 0.75), np.float32), strength=2.0)
+    d = _desc(b, cam, (W, H))
+    img, _, aov = RefScene(d).render(threads=4, spp=1, width=W, height=H, aovs=True)
+    want = np.array([0.5, 1.0, 1.5], np.float32)
+    assert np.allclose(img[..., :3], want, rtol=1e-6)
+    assert np.allclose(aov[..., 0:3], want, rtol=1e-6) and np.all(aov[..., 6] == -1.0)
+
+
+def test_environment_furnace():
+    """A Lambertian plane (albedo a) under a constant sky L converges to a*L:
+    NEE towards the map and BSDF rays that escape, combined by MIS."""
+    b, cam, (W, H) = _plane_scene(albedo=0.5, W=12, H=10)
+    b.light("environment", env=np.full((8, 16, 3), 0.8, np.float32))
+    d = _desc(b, cam, (W, H), max_depth=4)
+    img, _ = RefScene(d).render(threads=8, spp=256, width=W, height=H)
+    v = img[..., :3] / img[..., 3:4]
+    assert abs(v.mean() - 0.4) < 0.004, v.mean()
+
+
+def test_environment_sampling_matches_distribution():
+    """Importance sampling of the map (environment.cpp:42-73 with
+    Distribution2D, distribution.cpp:53-80): sampled directions are
+    distributed as f(u,v) = luminance * sin(theta of the row), piecewise
+    constant per pixel (a chi-square over half-pixel bins, the reference's
+    disabled statistical test), and pdf(w) equals that density divided by
+    2 pi^2 sin(theta) (environment.cpp:75-91)."""
+    env = _bright_square_env()
+    H, W = env.shape[:2]
+    b, cam, wh = _plane_scene()
+    b.light("environment", env=env)
+    ref = RefScene(_desc(b, cam, wh))
+    lum = env[..., 0] * 0.212671 + env[..., 1] * 0.715160 + env[..., 2] * 0.072169
+    f = lum * np.sin(np.pi * (np.arange(H) + 0.5) / H)[:, None]
+    prob_pix = f / f.sum()
+    rng = np.random.default_rng(7)
+    n = 40000
+    counts = np.zeros((2 * H, 2 * W))
+    for k in range(n):
+        u = rng.random(2).astype(np.float32)
+        sm = ref.light_sample(0, (0, 0, 0), u)
+        if sm["pdf"] <= 0:
+            continue
+        w = sm["dir"]
+        th = np.arccos(np.clip(w[2], -1, 1))
+        ph = np.arctan2(w[1], w[0]) % (2 * np.pi)
+        counts[min(int(th / np.pi * 2 * H), 2 * H - 1), min(int(ph / (2 * np.pi) * 2 * W), 2 * W - 1)] += 1
+    expect = np.repeat(np.repeat(prob_pix / 4, 2, 0), 2, 1) * counts.sum()
+    m = expect > 10
+    chi2 = float((((counts - expect) ** 2)[m] / expect[m]).sum())
+    dof = int(m.sum()) - 1
+    assert chi2 < sst.chi2.ppf(0.999, dof), (chi2, dof)
+    # pdf of directions = density over uv / (2 pi^2 sin theta)
+    for _ in range(200):
+        z = rng.uniform(-0.98, 0.98)
+        ph = rng.uniform(0, 2 * np.pi)
+        w = np.array([np.sqrt(1 - z * z) * np.cos(ph), np.sqrt(1 - z * z) * np.sin(ph), z], np.float32)
+        pdf, le = ref.env_pdf(w)
+        th = np.arccos(np.clip(w[2], -1, 1))
+        iv = min(int(th / np.pi * H), H - 1)
+        iu = min(int((np.arctan2(w[1], w[0]) % (2 * np.pi)) / (2 * np.pi) * W), W - 1)
+        want = prob_pix[iv, iu] * (H * W) / (2 * np.pi ** 2 * np.sin(th))
+        assert abs(pdf - want) <= 1e-4 * want, (pdf, want)
+
