@@ -31,6 +31,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak (8 TB/s)
 RAY_BYTES, HIT_BYTES, NODE_BYTES, PACK_BYTES, INST_BYTES = 32, 20, 144, 160, 64
+INTEGRATOR_NAMES = ("path", "volpath", "direct", "ao")
 METRIC = "Msamples/s & Mrays/s (1080p, 1M tris) at equal MSE; p50/p99 job latency @QPS"
 
 
@@ -106,7 +107,7 @@ def cpu_baseline(hs, target_s=12.0):
         samples += n2
         passes += more
         desc = f"passes 0-{passes - 1} (full frames)"
-    integ = "volpath" if hs.desc().contents.integrator == 1 else "path"
+    integ = INTEGRATOR_NAMES[hs.desc().contents.integrator]
     return {"value": samples / secs / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
             "sample": f"{desc} of the same {W}x{H} frame ({samples} camera samples, {secs:.1f} s, {integ} "
                       f"integrator, C++ restatement of the reference CPU integrator, {threads} threads)"}
@@ -337,7 +338,7 @@ def main():
         "config": {"workload": WORKLOADS[a.scene],
                    "width": cam.width, "height": cam.height, "spp": job.spp, "max_depth": job.max_depth,
                    "instanced_triangles": inst_tris,
-                   "integrator": "volpath" if job.integrator == A.GND_INTEGRATOR_VOLPATH else "path",
+                   "integrator": INTEGRATOR_NAMES[job.integrator],
                    "aovs": bool(a.aov), **({"params": a.params} if a.params else {})},
         "mrays_per_s": round(mrays, 2),
         "scene_build_ms": round(hs.build_ms, 1),

@@ -16,7 +16,7 @@ GND_MAT_MATTE, GND_MAT_CONDUCTOR, GND_MAT_GLASS, GND_MAT_PLASTIC, GND_MAT_INDEX 
 GND_MEDIUM_HOMOGENEOUS, GND_MEDIUM_GRID = 0, 1
 GND_CAMERA_PERSPECTIVE, GND_CAMERA_THINLENS = 0, 1
 GND_FILTER_GAUSSIAN, GND_FILTER_BOX = 0, 1
-GND_INTEGRATOR_PATH, GND_INTEGRATOR_VOLPATH = 0, 1
+GND_INTEGRATOR_PATH, GND_INTEGRATOR_VOLPATH, GND_INTEGRATOR_DIRECT, GND_INTEGRATOR_AO = 0, 1, 2, 3
 
 GND_OK, GND_E_INVALID, GND_E_DEVICE, GND_E_NOMEM, GND_E_CANCELLED, GND_E_STACK = 0, -1, -2, -3, -4, -5
 
@@ -70,7 +70,8 @@ class gnd_scene_desc(C.Structure):
                 ("nmaterials", C.c_uint32), ("materials", C.POINTER(gnd_material_desc)),
                 ("nmedia", C.c_uint32), ("media", C.POINTER(gnd_medium_desc)),
                 ("camera", gnd_camera_desc), ("integrator", C.c_int32), ("max_depth", C.c_int32),
-                ("spp", C.c_uint32), ("nlights", C.c_uint32), ("lights", C.POINTER(gnd_light_desc))]
+                ("spp", C.c_uint32), ("nlights", C.c_uint32), ("lights", C.POINTER(gnd_light_desc)),
+                ("ao_length", C.c_float)]
 
 
 class gnd_ray_soa(C.Structure):
@@ -114,7 +115,7 @@ class gnd_hip_scene_arrays(C.Structure):
 class gnd_hip_job(C.Structure):
     _fields_ = [("camera", gnd_camera_desc), ("spp", C.c_uint32), ("max_depth", C.c_int32),
                 ("integrator", C.c_int32), ("seed_offset", C.c_uint64), ("film", C.c_void_p),
-                ("film_sq", C.c_void_p), ("film_aov", C.c_void_p)]
+                ("film_sq", C.c_void_p), ("film_aov", C.c_void_p), ("ao_length", C.c_float)]
 
 
 class gnd_hip_stats(C.Structure):

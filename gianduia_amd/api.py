@@ -118,6 +118,7 @@ class Device:
         cj.spp = job.spp
         cj.max_depth = job.max_depth
         cj.integrator = job.integrator
+        cj.ao_length = job.ao_length
         cj.seed_offset = job.seed_offset
         cj.film = film.addr
         cj.film_sq = None
@@ -207,6 +208,7 @@ class Device:
             cj[i].spp = j.spp
             cj[i].max_depth = j.max_depth
             cj[i].integrator = j.integrator
+            cj[i].ao_length = j.ao_length
             cj[i].seed_offset = j.seed_offset
             cj[i].film = f.addr
             cj[i].film_sq = sq.addr if sq is not None else None
@@ -244,6 +246,7 @@ class RenderSession:
             self.cj[i].spp = j.spp
             self.cj[i].max_depth = j.max_depth
             self.cj[i].integrator = j.integrator
+            self.cj[i].ao_length = j.ao_length
             self.cj[i].seed_offset = j.seed_offset
             self.cj[i].film = f.addr
             self.cj[i].film_sq = sq.addr if sq is not None else None
@@ -273,6 +276,7 @@ class Job:
     max_depth: int
     integrator: int = A.GND_INTEGRATOR_PATH
     seed_offset: int = 0
+    ao_length: float = 5.0  # ao integrator: occlusion ray length (ao.cpp:9)
 
 
 class HostScene:
@@ -328,7 +332,8 @@ class HostScene:
 
     def default_job(self, spp: int | None = None, seed_offset: int = 0) -> Job:
         d = self.desc().contents
-        return Job(self.camera(), spp if spp is not None else d.spp, d.max_depth, d.integrator, seed_offset)
+        return Job(self.camera(), spp if spp is not None else d.spp, d.max_depth, d.integrator, seed_offset,
+                   d.ao_length if d.ao_length > 0 else 5.0)
 
     def free(self) -> None:
         if self.handle:

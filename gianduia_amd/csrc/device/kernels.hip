@@ -78,6 +78,7 @@ struct DJob {
     float* filmAov;  // weighted AOV sums, kAovFloats per pixel, or null
     uint64_t seedOffset;
     int32_t maxDepth, integrator, W, H;
+    float aoLength;  // ao: occlusion ray tMax
 };
 
 struct DChunk {     // one pass of one job: npix path slots from slotBase
@@ -769,6 +770,111 @@ GDEV_INLINE int neeStep(const WaveArgs& A, const DJob& J, int p, PathLocal& s, c
     return 2;
 }
 
+// DirectLight::Li (src/integrators/direct.cpp:10-76) as two bounces of the
+// wavefront loop.  Bounce 0, the camera hit: its emission, one light sample
+// (balance-heuristic MIS, weight 1 for delta lights) whose shadow ray is a
+// plain any-hit test, and one BSDF sample that becomes the path's next ray.
+// Bounce 1, that ray's hit: its emission, MIS-weighted against the light
+// pdf.  Between the two the sample's weight f*|cos|/pdf rides in the
+// throughput, its pdf in pdfPrev, and F_SPECULAR holds "sampled type ==
+// BSDF_SPECULAR" (direct.cpp:66: an equality test, false for every lobe
+// here since each also carries a reflection or transmission bit).
+// Returns 0 = done, 1 = continues with the BSDF ray.
+GDEV_INLINE int directStep(const WaveArgs& A, PathLocal& s, const HitRec& h, bool& shadow, V3& so, V3& sd,
+                           float& stmax, V3& pending, V3& lightP) {
+    const DScene& S = A.S;
+    shadow = false;
+    if (h.inst < 0) return 0;  // no environment term in this integrator
+    Isect it;
+    fillInteraction(S, s.o, s.d, h, it);
+    const DPrim& pr = S.prims[it.prim];
+    const float lsp = 1.0f / (float)(uint32_t)S.nEmitters;
+    if ((s.c0 & 0xFFFFu) > 0) {
+        if (pr.emitter >= 0) {
+            const V3 Lb = areaEval(pr, it.n, -s.d);
+            if (!isBlack(Lb)) {
+                float lpg = areaPdf(S, it.prim, s.last, it.p, it.n);
+                if (__builtin_isinf(lpg)) lpg = 0.0f;
+                const float pl = (s.flags & F_SPECULAR) ? 0.0f : lpg * lsp;
+                const float den = s.pdfPrev + pl;
+                if (den > 1e-6f) s.L = s.L + cdiv(s.tp * Lb * s.pdfPrev, den);
+            }
+        }
+        return 0;
+    }
+    if (pr.emitter >= 0) s.L = s.L + areaEval(pr, it.n, -s.d);
+    Bsdf b;
+    if (!makeBsdf(S, it, b)) return 0;
+    const int ep = pickEmitter(S, s.rng.next1D());
+    const float ux = s.rng.next1D(), uy = s.rng.next1D();
+    LightSample ls;
+    bool delta;
+    emitterSample(S, ep, it.p, ux, uy, ls, delta);
+    const V3 Li = cdiv(ls.Li, lsp);
+    if (ls.ok && ls.pdf > 1e-6f && !isBlack(Li)) {
+        // the reference tests the shadow ray first and then weighs; the
+        // weight draws no random numbers, so the ray is only cast when the
+        // contribution is non-zero
+        const V3 wi = normalize(ls.p - it.p);
+        const float bp = b.pdf(-s.d, wi);
+        const V3 f = b.f(-s.d, wi);
+        if (!isBlack(f) && bp > 1e-6f) {
+            const float pl = ls.pdf * lsp;
+            const float pb = delta ? 0.0f : bp;
+            const float den = pl + pb;
+            if (den > 1e-6f) {
+                pending = cdiv(f * Li * __builtin_fabsf(dot(it.n, wi)) * pl, den);
+                shadow = true;
+                so = it.p; sd = ls.dir; stmax = ls.tmax; lightP = ls.p;
+            }
+        }
+    }
+    V3 wiW = V3{0, 0, 0};
+    int st = 0;
+    const float sx = s.rng.next1D(), sy = s.rng.next1D();
+    const float uc = s.rng.next1D();
+    float bpdf = 0.0f;
+    const V3 fc = b.sample(-s.d, wiW, sx, sy, uc, bpdf, st);
+    if (isBlack(fc) || !(bpdf > 1e-6f)) return 0;
+    s.tp = fc;
+    s.pdfPrev = bpdf;
+    if (st == BSDF_SPECULAR) s.flags |= F_SPECULAR;
+    else s.flags &= ~F_SPECULAR;
+    s.last = it.p;
+    s.o = it.p;
+    s.d = wiW;
+    s.c0 = 1u;
+    return 1;
+}
+
+// AmbientOcclusion::Li (src/integrators/ao.cpp:14-40): a camera ray that
+// misses is white; a hit casts one cosine-weighted ray of length aoLength
+// about the shading normal (frame from a fixed helper axis), white when
+// nothing is hit.  The occlusion ray goes through the shadow queue with a
+// contribution of 1.
+GDEV_INLINE void aoStep(const WaveArgs& A, const DJob& J, PathLocal& s, const HitRec& h, bool& shadow, V3& so,
+                        V3& sd, float& stmax, V3& pending, V3& lightP) {
+    shadow = false;
+    if (h.inst < 0) {
+        s.L = sameV(1.0f);
+        return;
+    }
+    Isect it;
+    fillInteraction(A.S, s.o, s.d, h, it);
+    const float ux = s.rng.next1D(), uy = s.rng.next1D();
+    const V3 ld = squareToCosineHemisphere(ux, uy);
+    const V3 n = it.n;
+    const V3 t1 = __builtin_fabsf(n.x) > 0.1f ? normalize(cross(V3{0.0f, 1.0f, 0.0f}, n))
+                                              : normalize(cross(V3{1.0f, 0.0f, 0.0f}, n));
+    const V3 t2 = cross(n, t1);
+    sd = t1 * ld.x + t2 * ld.y + n * ld.z;
+    so = it.p;
+    stmax = J.aoLength;
+    lightP = it.p;
+    pending = sameV(1.0f);
+    shadow = true;
+}
+
 // Paths whose ray starts in a density grid, when kMedium runs (split mode).
 GDEV_INLINE bool gridWalked(const WaveArgs& A, const DJob& J, uint32_t flags) {
     const int medium = J.integrator == GND_INTEGRATOR_VOLPATH ? flagMedium(flags) : -1;
@@ -878,7 +984,15 @@ __global__ __launch_bounds__(kShadeBlock, GND_SHADE_WAVES) void kShade(WaveArgs 
                 preHit = A.P.evKind[p] == EV_MEDIUM;
                 if (preHit) preP = V3{A.P.evpx[p], A.P.evpy[p], A.P.evpz[p]};
             }
-            int r = neeStep(A, J, p, s, h, ev, shadow, so, sd, stmax, smedium, pending, lightP, pre, preHit, preP);
+            int r;
+            if (J.integrator == GND_INTEGRATOR_DIRECT) {
+                r = directStep(A, s, h, shadow, so, sd, stmax, pending, lightP);
+            } else if (J.integrator == GND_INTEGRATOR_AO) {
+                aoStep(A, J, s, h, shadow, so, sd, stmax, pending, lightP);
+                r = 0;
+            } else {
+                r = neeStep(A, J, p, s, h, ev, shadow, so, sd, stmax, smedium, pending, lightP, pre, preHit, preP);
+            }
             if (!A.pathOnlyFast && shadow) {
                 // the general shadow kernel always needs the ray; the any-hit
                 // path skips zero contributions
@@ -1033,7 +1147,7 @@ __global__ __launch_bounds__(256, GND_SHADOWTR_WAVES) void kShadow(WaveArgs A) {
     constexpr int kVisible = 0, kOccluded = 1, kOnward = 2;
     int64_t cur = -1;
     int phase = kIdle, medium = -1, iter = 0, p = -1;
-    bool vol = false;
+    bool vol = false, opaque = false;  // opaque: direct / ao, any surface occludes
     V3 o = sameV(0.0f), d = sameV(0.0f), Tr = sameV(1.0f);
     float tmax = 0.0f;
     Rng rng;
@@ -1052,7 +1166,11 @@ __global__ __launch_bounds__(256, GND_SHADOWTR_WAVES) void kShadow(WaveArgs A) {
                 o = V3{Sh.ox[i], Sh.oy[i], Sh.oz[i]};
                 d = V3{Sh.dx[i], Sh.dy[i], Sh.dz[i]};
                 tmax = Sh.tmax[i];
-                vol = A.jobs[A.P.job[p]].integrator == GND_INTEGRATOR_VOLPATH;
+                {
+                    const int32_t integ = A.jobs[A.P.job[p]].integrator;
+                    vol = integ == GND_INTEGRATOR_VOLPATH;
+                    opaque = integ == GND_INTEGRATOR_DIRECT || integ == GND_INTEGRATOR_AO;
+                }
                 medium = Sh.medium[i];
                 Tr = sameV(1.0f);
                 rng.s = A.P.rng[p];
@@ -1078,7 +1196,7 @@ __global__ __launch_bounds__(256, GND_SHADOWTR_WAVES) void kShadow(WaveArgs A) {
                 Isect it;
                 fillInteraction(S, o, d, h, it);
                 const DMaterial& m = S.mats[S.prims[it.prim].material];
-                if (m.type != GND_MAT_INDEX) {
+                if (m.type != GND_MAT_INDEX || opaque) {
                     next = kOccluded;
                 } else {
                     next = kOnward;
@@ -2028,6 +2146,15 @@ struct WavePlan {
     int32_t maxDepth = 0;
 };
 
+// Bounces a job's paths may take: max_depth for path / volpath; the camera
+// hit and one BSDF ray for direct; the camera hit for ao (their max_depth is
+// not used, as in the reference).
+static int jobDepth(const gnd_hip_job& j) {
+    if (j.integrator == GND_INTEGRATOR_DIRECT) return 2;
+    if (j.integrator == GND_INTEGRATOR_AO) return 1;
+    return j.max_depth;
+}
+
 // Packs passes of the jobs into waves of at most `cap` slots.  A job's passes
 // enter waves in pass order; every pass is whole (the film gather needs the
 // neighbouring samples of the same pass).
@@ -2051,7 +2178,7 @@ static std::vector<WavePlan> planWaves(const std::vector<gnd_hip_job>& jobs, siz
             if ((int)w.chunks.size() > first) {
                 w.tasks.push_back({w.pix, (int32_t)j, first, (int32_t)w.chunks.size()});
                 w.pix += npix;
-                w.maxDepth = std::max(w.maxDepth, jobs[j].max_depth);
+                w.maxDepth = std::max(w.maxDepth, jobDepth(jobs[j]));
             }
             if (w.slots >= cap) break;
         }
@@ -2059,7 +2186,7 @@ static std::vector<WavePlan> planWaves(const std::vector<gnd_hip_job>& jobs, siz
         // in-queue layout of the camera paths: live chunks back to back
         for (DChunk& c : w.chunks) {
             c.qBase = w.live;
-            if (jobs[c.job].max_depth > 0) w.live += c.npix;
+            if (jobDepth(jobs[c.job]) > 0) w.live += c.npix;
         }
         waves.push_back(std::move(w));
     }
@@ -2209,8 +2336,9 @@ static DJob makeDJob(const gnd_hip_job& j) {
     J.filmSq = j.film_sq;
     J.filmAov = j.film_aov;
     J.seedOffset = j.seed_offset;
-    J.maxDepth = j.max_depth;
+    J.maxDepth = jobDepth(j);
     J.integrator = j.integrator;
+    J.aoLength = j.ao_length > 0.0f ? j.ao_length : 5.0f;
     J.W = j.camera.width;
     J.H = j.camera.height;
     return J;
@@ -2260,6 +2388,8 @@ int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jo
         for (auto& j : jobs) {
             if (j.camera.width <= 0 || j.camera.height <= 0 || !j.film)
                 throw std::runtime_error("job needs a positive resolution and a film");
+            if (j.integrator < GND_INTEGRATOR_PATH || j.integrator > GND_INTEGRATOR_AO)
+                throw std::runtime_error("unknown integrator");
             maxPix = std::max(maxPix, (size_t)j.camera.width * j.camera.height);
             anyVol = anyVol || j.integrator == GND_INTEGRATOR_VOLPATH;
             samples += (uint64_t)j.camera.width * j.camera.height * j.spp;
@@ -2639,7 +2769,7 @@ std::vector<int> admit(gnd_hip_server* s) {
             const int c = s->freeChunks.back();
             s->freeChunks.pop_back();
             s->chunks[c] = SrvChunk{t, j.nextPass, base, npix, true};
-            s->hLive[c] = j.job.max_depth > 0 ? npix : 0u;
+            s->hLive[c] = jobDepth(j.job) > 0 ? npix : 0u;
             fresh.push_back(c);
             j.nextPass++;
         }
@@ -2693,7 +2823,7 @@ void serverStep(gnd_hip_server* s) {
             const SrvChunk& ch = s->chunks[c];
             dc.push_back({ch.slotBase, ch.npix, s->jobs[ch.ticket].slot, (int32_t)ch.pass, n, c, nLive});
             n += ch.npix;
-            if (s->jobs[ch.ticket].job.max_depth > 0) nLive += ch.npix;
+            if (jobDepth(s->jobs[ch.ticket].job) > 0) nLive += ch.npix;
             GND_CHECK(hipMemcpyAsync(s->dLive + c, &s->hLive[c], sizeof(uint32_t), hipMemcpyHostToDevice,
                                      ctx->stream));
             s->running.push_back(c);
@@ -2913,7 +3043,8 @@ int gnd_hip_server_add_scene(gnd_hip_server* s, const gnd_hip_scene_arrays* arra
 }
 
 int gnd_hip_server_submit(gnd_hip_server* s, int32_t scene_id, const gnd_hip_job* job, uint64_t* ticket) {
-    if (!s || !job || !ticket || job->spp == 0 || !job->film || job->camera.width <= 0 || job->camera.height <= 0)
+    if (!s || !job || !ticket || job->spp == 0 || !job->film || job->camera.width <= 0 || job->camera.height <= 0 ||
+        job->integrator < GND_INTEGRATOR_PATH || job->integrator > GND_INTEGRATOR_AO)
         return GND_E_INVALID;
     std::lock_guard<std::mutex> lk(s->mu);
     if (scene_id < 0 || scene_id >= (int32_t)s->scenes.size()) return GND_E_INVALID;

@@ -441,6 +441,17 @@ std::unique_ptr<HostScene> generateScene(const std::string& name, const std::str
     else if (name == "procedural") s = proceduralScene(p);
     else throw std::runtime_error("unknown scene generator: " + name);
     addLights(*s, p);
+    // integrator=path|volpath|direct|ao (aolength=) swaps the scene's integrator
+    auto it = p.find("integrator");
+    if (it != p.end()) {
+        const std::string& t = it->second;
+        if (t == "path") s->integrator = GND_INTEGRATOR_PATH;
+        else if (t == "volpath") s->integrator = GND_INTEGRATOR_VOLPATH;
+        else if (t == "direct") s->integrator = GND_INTEGRATOR_DIRECT;
+        else if (t == "ao") s->integrator = GND_INTEGRATOR_AO;
+        else throw std::runtime_error("unknown integrator: " + t);
+    }
+    s->aoLength = (float)num(p, "aolength", 5.0);
     return s;
 }
 

@@ -147,6 +147,7 @@ int gnd_host_render(gnd_hip_ctx* ctx, gnd_host_scene* s, int spp, uint64_t seed_
     job.spp = spp > 0 ? (uint32_t)spp : d.spp;
     job.max_depth = d.max_depth;
     job.integrator = d.integrator;
+    job.ao_length = d.ao_length;
     job.seed_offset = seed_offset;
     job.film = static_cast<float*>(film);
     rc = gnd_hip_render(ctx, dev, &job, 1, stats);
@@ -193,7 +194,9 @@ int gnd_host_integrator_from_scene(gnd_hip_ctx* ctx, gnd_host_scene* s, int pass
                                    gnd_host_integrator** out) {
     if (!ctx || !s || !out) return GND_E_INVALID;
     const gnd_scene_desc& d = s->scene->desc();
-    return gnd_host_integrator_create(ctx, d.integrator == GND_INTEGRATOR_VOLPATH ? "volpath" : "path", d.max_depth,
+    static const char* names[] = {"path", "volpath", "direct", "ao"};
+    if (d.integrator < GND_INTEGRATOR_PATH || d.integrator > GND_INTEGRATOR_AO) return GND_E_INVALID;
+    return gnd_host_integrator_create(ctx, names[d.integrator], d.max_depth,
                                       passes_per_callback, out);
 }
 

@@ -301,6 +301,7 @@ void HipIntegrator::render(const RenderScene& scene) {
             job.spp = std::min<uint32_t>((uint32_t)batch_, spp - s);
             job.max_depth = maxDepth_;
             job.integrator = integrator_;
+            job.ao_length = d.ao_length;
             job.seed_offset = (uint64_t)s * (uint64_t)W * (uint64_t)H;
             job.film = dev;
             job.film_aov = devAov;
@@ -382,6 +383,10 @@ void registerBuiltinIntegrators() {
         IntegratorFactory::registerClass("volpath", make(GND_INTEGRATOR_VOLPATH));
         IntegratorFactory::registerClass("path_hip", make(GND_INTEGRATOR_PATH));
         IntegratorFactory::registerClass("volpath_hip", make(GND_INTEGRATOR_VOLPATH));
+        IntegratorFactory::registerClass("direct", make(GND_INTEGRATOR_DIRECT));
+        IntegratorFactory::registerClass("direct_hip", make(GND_INTEGRATOR_DIRECT));
+        IntegratorFactory::registerClass("ao", make(GND_INTEGRATOR_AO));
+        IntegratorFactory::registerClass("ao_hip", make(GND_INTEGRATOR_AO));
     });
 }
 

@@ -162,7 +162,7 @@ public:
     static std::vector<std::string> names();
 };
 
-// Registers "path", "volpath", "path_hip" and "volpath_hip" (idempotent;
+// Registers "path", "volpath", "direct", "ao" and their "_hip" aliases (idempotent;
 // called by the factory before its first use, so static-library linking
 // cannot drop the registrations).
 void registerBuiltinIntegrators();

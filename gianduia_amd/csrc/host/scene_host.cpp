@@ -192,6 +192,7 @@ void HostScene::copyFrom(const gnd_scene_desc& d) {
     camera = d.camera;
     integrator = d.integrator;
     maxDepth = d.max_depth;
+    aoLength = d.ao_length > 0.0f ? d.ao_length : 5.0f;
     spp = d.spp;
     built_ = false;
 }
@@ -227,6 +228,7 @@ const gnd_scene_desc& HostScene::desc() {
     desc_.camera = camera;
     desc_.integrator = integrator;
     desc_.max_depth = maxDepth;
+    desc_.ao_length = aoLength;
     desc_.spp = spp;
     desc_.nlights = (uint32_t)lights.size();
     desc_.lights = lightDescs_.empty() ? nullptr : lightDescs_.data();

@@ -45,6 +45,7 @@ public:
     gnd_camera_desc camera{};
     int integrator = GND_INTEGRATOR_PATH;
     int maxDepth = 1000;
+    float aoLength = 5.0f;  // ao integrator "length" (ao.cpp:9)
     uint32_t spp = 1;
 
     // helpers for generators and the XML loader

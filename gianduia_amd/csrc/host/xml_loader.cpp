@@ -3,7 +3,7 @@
 //
 // Supported: <scene>, <camera type="perspective|thinlens"> with <filter
 // type="box|gaussian">, <sampler type="independent">, <integrator
-// type="path|volpath|path_hip|volpath_hip">, <primitive> with <shape
+// type="path|volpath|direct|ao" (or with a _hip suffix)>, <primitive> with <shape
 // type="mesh"> (OBJ file) or <shape type="sphere">, <material
 // type="matte|conductor|glass|plastic|index"> with nested <medium
 // name="inside|outside" type="homogeneous">, <emitter type="area"> inside a
@@ -504,7 +504,13 @@ struct Loader {
                 std::string t = k->attrOr("type");
                 if (t == "path" || t == "path_hip") sc.integrator = GND_INTEGRATOR_PATH;
                 else if (t == "volpath" || t == "volpath_hip") sc.integrator = GND_INTEGRATOR_VOLPATH;
+                else if (t == "direct" || t == "direct_hip") sc.integrator = GND_INTEGRATOR_DIRECT;
+                else if (t == "ao" || t == "ao_hip") sc.integrator = GND_INTEGRATOR_AO;
                 else throw std::runtime_error("unsupported integrator: " + t);
+                // ao.cpp:9; a length <= 0 (no occluder can be nearer than
+                // tMin) is kept as a tiny positive length with the same result
+                const float len = ip.getF("length", 5.0f);
+                sc.aoLength = len > 0.0f ? len : 1e-30f;
                 sc.maxDepth = ip.getI("maxDepth", 1000);
             } else if (k->tag == "primitive") {
                 readPrimitive(*k);

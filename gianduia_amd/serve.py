@@ -55,6 +55,7 @@ class Server:
         cj.spp = job.spp
         cj.max_depth = job.max_depth
         cj.integrator = job.integrator
+        cj.ao_length = job.ao_length
         cj.seed_offset = job.seed_offset
         cj.film = film.addr
         cj.film_sq = None

@@ -98,7 +98,7 @@ typedef struct gnd_hip_job {
     gnd_camera_desc camera;
     uint32_t spp;
     int32_t max_depth;
-    int32_t integrator;     /* GND_INTEGRATOR_PATH / _VOLPATH                 */
+    int32_t integrator;     /* GND_INTEGRATOR_PATH / _VOLPATH / _DIRECT / _AO  */
     uint64_t seed_offset;   /* added to pixelIdx + pass*W*H before SplitMix64 */
     float* film;            /* device pointer                                 */
     float* film_sq;         /* device pointer or NULL                         */
@@ -107,6 +107,7 @@ typedef struct gnd_hip_job {
                                albedo rgb, normal xyz (mapped to [0,1]), depth,
                                roughness, metallic (9 floats); divide by the film's
                                weight to resolve (film.cpp:50-95)                 */
+    float ao_length;        /* GND_INTEGRATOR_AO: occlusion ray tMax (> 0)     */
 } gnd_hip_job;
 
 typedef struct gnd_hip_stats {

@@ -48,7 +48,10 @@ enum {
 enum { GND_MEDIUM_HOMOGENEOUS = 0, GND_MEDIUM_GRID = 1 };
 enum { GND_CAMERA_PERSPECTIVE = 0, GND_CAMERA_THINLENS = 1 };
 enum { GND_FILTER_GAUSSIAN = 0, GND_FILTER_BOX = 1 };
-enum { GND_INTEGRATOR_PATH = 0, GND_INTEGRATOR_VOLPATH = 1 };
+/* path: src/integrators/path.cpp; volpath: volPath.cpp; direct: direct.cpp
+ * (emission + one light sample + one BSDF sample, any-hit shadows); ao:
+ * ao.cpp (one cosine-weighted occlusion ray of length ao_length)          */
+enum { GND_INTEGRATOR_PATH = 0, GND_INTEGRATOR_VOLPATH = 1, GND_INTEGRATOR_DIRECT = 2, GND_INTEGRATOR_AO = 3 };
 enum { GND_LIGHT_POINT = 0, GND_LIGHT_SPOT = 1, GND_LIGHT_DIRECTIONAL = 2, GND_LIGHT_ENVIRONMENT = 3 };
 
 typedef struct gnd_mesh_desc {
@@ -155,6 +158,8 @@ typedef struct gnd_scene_desc {
     uint32_t spp;
     uint32_t nlights;
     const gnd_light_desc* lights;
+    float ao_length;         /* ao integrator: occlusion ray length ("length",
+                                default 5, ao.cpp:9)                           */
 } gnd_scene_desc;
 
 /* Struct-of-arrays ray batch (32 B per ray) and hit batch (20 B per ray).   */

@@ -354,6 +354,7 @@ struct Scene {
     int integrator = GND_INTEGRATOR_PATH;
     int maxDepth = 1000;
     uint32_t spp = 1;
+    float aoLength = 5.0f;              // ao.cpp:9 ("length"; a desc value <= 0 selects it)
 };
 
 void sceneBuild(Scene& s, const gnd_scene_desc& d);

@@ -531,6 +531,7 @@ void sceneBuild(Scene& s, const gnd_scene_desc& d) {
     s.camera.aspect = d.camera.width / (float)d.camera.height;
     s.integrator = d.integrator;
     s.maxDepth = d.max_depth;
+    s.aoLength = d.ao_length > 0.0f ? d.ao_length : 5.0f;
     s.spp = d.spp;
 }
 

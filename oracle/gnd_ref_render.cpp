@@ -1218,6 +1218,89 @@ static int nextMedium(V3 wo, V3 wi, int cur, const Isect& it) {  // volPath.cpp:
 }
 
 // VolumetricPathIntegrator::Li (volPath.cpp:37-249)
+// DirectLight::Li (src/integrators/direct.cpp:10-76): emission of the
+// camera hit, one light sample (balance heuristic, p_bsdf = 0 for delta
+// lights; the shadow ray a plain any-hit test) and one BSDF sample whose hit
+// adds its emission; "sampledType == BSDF_SPECULAR" (direct.cpp:66) is an
+// equality test.  No environment term and no AOV writes.
+static C3 LiDirect(const Scene& s, Ray r, Sampler& smp, Counters* cnt) {
+    Hit h;
+    if (!intersectClosest(s, r, h, cnt)) return C3(0.0f);
+    C3 L(0.0f);
+    Isect it;
+    fillInteraction(s, r, h, it);
+    const Prim& pr = s.prims[it.prim];
+    if (pr.emitter >= 0) L = L + areaEval(pr, it.n, -r.d);
+    BSDF bsdf;
+    if (!makeBSDF(s, it, bsdf)) return L;
+    float lightSelectPdf = 1.0f / s.emitters.size();
+    int ep = pickEmitter(s, smp.next1D());
+    LightSample ls;
+    bool pdfSet, delta;
+    C3 Li = emitterSample(s, ep, it.p, smp.next2D(), ls, pdfSet, delta);
+    Li = Li / lightSelectPdf;
+    if (pdfSet && ls.pdf > 1e-6f && !Li.isBlack()) {
+        Ray sr = ls.shadow;
+        if (!intersectAny(s, sr, cnt)) {
+            V3 wi = Normalize(ls.p - it.p);
+            float bsdfPdf = bsdf.pdf(-r.d, wi);
+            C3 f = bsdf.f(-r.d, wi);
+            if (!f.isBlack() && bsdfPdf > 1e-6f) {
+                float pLight = ls.pdf * lightSelectPdf;
+                float pBsdf = delta ? 0.0f : bsdfPdf;
+                float misDenom = pLight + pBsdf;
+                if (misDenom > 1e-6f) L = L + f * Li * std::abs(Dot(it.n, wi)) * pLight / misDenom;
+            }
+        }
+    }
+    V3 wi;
+    float bsdfPdf = 0.0f;
+    int st = 0;
+    P2 u2 = smp.next2D();
+    float uc = smp.next1D();
+    C3 f = bsdf.sample(-r.d, &wi, u2, uc, &bsdfPdf, &st);
+    if (!f.isBlack() && bsdfPdf > 1e-6f) {
+        Ray sec;
+        sec.o = it.p;
+        sec.d = wi;
+        Hit h2;
+        if (intersectClosest(s, sec, h2, cnt)) {
+            Isect it2;
+            fillInteraction(s, sec, h2, it2);
+            const Prim& p2 = s.prims[it2.prim];
+            if (p2.emitter >= 0) {
+                C3 Lb = areaEval(p2, it2.n, -wi);
+                if (!Lb.isBlack()) {
+                    float lightPdfGeo = areaPdfAt(s, p2, it.p, it2.p, it2.n);
+                    if (std::isinf(lightPdfGeo)) lightPdfGeo = 0.0f;
+                    float pLight = st == BSDF_SPECULAR ? 0.0f : lightPdfGeo * lightSelectPdf;
+                    float misDenom = bsdfPdf + pLight;
+                    if (misDenom > 1e-6f) L = L + f * Lb * bsdfPdf / misDenom;
+                }
+            }
+        }
+    }
+    return L;
+}
+
+// AmbientOcclusion::Li (src/integrators/ao.cpp:14-40)
+static C3 LiAO(const Scene& s, Ray r, Sampler& smp, Counters* cnt) {
+    Hit h;
+    if (!intersectClosest(s, r, h, cnt)) return C3(1.0f);
+    Isect it;
+    fillInteraction(s, r, h, it);
+    V3 local = squareToCosineHemisphere(smp.next2D());
+    V3 n = it.n;
+    V3 t1 = std::abs(n.x) > 0.1f ? Normalize(Cross(V3(0.0f, 1.0f, 0.0f), n)) : Normalize(Cross(V3(1.0f, 0.0f, 0.0f), n));
+    V3 t2 = Cross(n, t1);
+    Ray sec;
+    sec.o = it.p;
+    sec.d = t1 * local.x + t2 * local.y + n * local.z;
+    sec.tMax = s.aoLength;
+    if (intersectAny(s, sec, cnt)) return C3(0.0f);
+    return C3(1.0f);
+}
+
 static C3 LiVolPath(const Scene& s, Ray r, Sampler& smp, Counters* cnt, AOV* aov) {
     C3 L(0.0f), tp(1.0f);
     int bounces = 0, surfB = 0, volB = 0, nullB = 0;
@@ -1359,6 +1442,16 @@ static C3 LiVolPath(const Scene& s, Ray r, Sampler& smp, Counters* cnt, AOV* aov
     return L;
 }
 
+// SamplerIntegrator::Li of the scene's integrator
+static C3 integrate(const Scene& s, const Ray& ray, Sampler& smp, Counters* c, AOV* ap) {
+    switch (s.integrator) {
+        case GND_INTEGRATOR_VOLPATH: return LiVolPath(s, ray, smp, c, ap);
+        case GND_INTEGRATOR_DIRECT: return LiDirect(s, ray, smp, c);
+        case GND_INTEGRATOR_AO: return LiAO(s, ray, smp, c);
+        default: return LiPath(s, ray, smp, c, ap);
+    }
+}
+
 // ------------------------------------------------------------ film + driver
 // Film::addSample filter weights (film.h:26-69, filter.h:20-54)
 static float filterEval(const Camera& c, float ox, float oy) {
@@ -1428,8 +1521,7 @@ void render(const Scene& s, const RenderOptions& opt, Film& film, Counters* cnt)
                     float w = shootRay(cam, cs, ray);
                     AOV aov;
                     AOV* ap = opt.aovs ? &aov : nullptr;
-                    C3 raw = s.integrator == GND_INTEGRATOR_VOLPATH ? LiVolPath(s, ray, smp, c, ap)
-                                                                    : LiPath(s, ray, smp, c, ap);
+                    C3 raw = integrate(s, ray, smp, c, ap);
                     raw = raw * w;
                     C3 nc(0.0f);
                     if (ch == -1) nc = raw;
@@ -1752,8 +1844,7 @@ void gnd_ref_path_sample(void* h, int x, int y, int pass, uint64_t seed_offset, 
     float w = shootRay(s->camera, cs, ray);
     AOV a;
     AOV* ap = aov ? &a : nullptr;
-    C3 L = s->integrator == GND_INTEGRATOR_VOLPATH ? LiVolPath(*s, ray, smp, nullptr, ap)
-                                                   : LiPath(*s, ray, smp, nullptr, ap);
+    C3 L = integrate(*s, ray, smp, nullptr, ap);
     L = L * w;
     if (L.hasNaNs()) L = C3(0.0f);
     out[0] = L.r; out[1] = L.g; out[2] = L.b;

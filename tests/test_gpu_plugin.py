@@ -56,7 +56,22 @@ def test_scene_integrator_and_save(device, tmp_path):
 
 def test_unknown_integrator(device):
     with pytest.raises(GndError):
-        Integrator(device, "direct")
+        Integrator(device, "photonmapper")
+
+
+@pytest.mark.parametrize("name,params", [("direct", "integrator=direct"), ("ao", "integrator=ao;aolength=0.7")])
+def test_direct_and_ao_plugins_equal_render(device, name, params):
+    """The direct and ao integrators by name and from the scene (its AO
+    length travels with the scene) render what gnd_hip_render renders."""
+    hs = HostScene.generate("cbox", f"width=40;height=32;spp=3;{params}")
+    ds = DeviceScene(device, hs)
+    films, _ = device.render(ds, [hs.default_job()])
+    # one batch of 3 passes, as the device render sums them (per-pass batches
+    # would sum on the host in double and round differently)
+    for it in (Integrator(device, name, passes_per_callback=3), Integrator(device, scene=hs, passes_per_callback=3)):
+        it.render(hs)
+        img, passes = it.film()
+        assert passes == 3 and np.array_equal(img, resolve(films[0]))
 
 
 def test_integrator_aovs_equal_device_render(device):

@@ -240,6 +240,53 @@ def test_environment_miss_albedo_aov(device):
     assert miss.any() and np.all(av["albedo"][miss].max(axis=-1) > 0)
 
 
+# ---------------------------------------------------------------- direct-lighting and AO integrators
+INTEGRATOR_SCENES = [
+    ("cbox", "width=64;height=64;integrator=direct"),
+    # every emitter kind; the environment only through the light sample
+    ("cbox", "width=64;height=64;integrator=direct;lights=point,spot,sun,env;envres=32"),
+    ("scaled", "level=4;instances=12;width=96;height=54;integrator=direct"),
+    # null-BSDF box: it occludes shadow rays (general shadow kernel, opaque rays)
+    ("volumetric", "grid=32;width=80;height=48;integrator=direct"),
+    ("cbox", "width=64;height=64;integrator=ao;aolength=0.5"),
+    ("scaled", "level=4;instances=12;width=96;height=54;integrator=ao;aolength=1.5"),
+    ("volumetric", "grid=32;width=80;height=48;integrator=ao"),
+]
+
+
+@pytest.mark.parametrize("name,params", INTEGRATOR_SCENES)
+def test_direct_ao_single_sample_replay(device, name, params):
+    """DirectLight (direct.cpp:10-76) and AmbientOcclusion (ao.cpp:14-40) on
+    the wavefront loop: at 1 spp nearly every pixel equals the oracle's."""
+    print(f"replay {name} {params}", flush=True)
+    g, _, r, _, st = _render_pair(device, name, params, 1, squares=False)
+    assert st["nan_samples"] == 0
+    frac = _replay_fraction(g, r)
+    print(f"{name} {params}: {frac:.4f} of paths reproduce the oracle", flush=True)
+    assert frac >= 0.99
+
+
+@pytest.mark.parametrize("name,params", INTEGRATOR_SCENES)
+def test_direct_ao_match_oracle(device, name, params):
+    print(f"render {name} {params} 16spp", flush=True)
+    g, gsq, r, rsq, st = _render_pair(device, name, params, 16)
+    assert st["nan_samples"] == 0
+    _check_stats(g, gsq, r, rsq, 16, f"{name} {params}")
+
+
+def test_direct_ao_leave_aov_defaults(device):
+    """The direct and AO integrators write no G-buffer, so their AOV films
+    resolve to the AOVRecord defaults (film.h:13-19)."""
+    for integ in ("direct", "ao"):
+        hs = HostScene.generate("cbox", f"width=32;height=24;integrator={integ}")
+        ds = DeviceScene(device, hs)
+        out, _ = device.render(ds, [hs.default_job(spp=2)], want_aov=True)
+        g, gaov = out[0]
+        av = resolve_aovs(g, gaov)
+        assert np.all(av["depth"] == -1.0) and not av["albedo"].any() and not av["normal"].any()
+        assert not av["roughness"].any() and not av["metallic"].any()
+
+
 # ---------------------------------------------------------------- media (volpath)
 @pytest.mark.parametrize("params", ["grid=0;width=80;height=48", "grid=32;width=80;height=48"])
 def test_volumetric_matches_oracle(device, params):
@@ -262,9 +309,11 @@ def test_batch_equals_solo(device):
     c3.width, c3.height = 17, 13
     jobs = [Job(cam, 3, 12, A.GND_INTEGRATOR_VOLPATH, 0),
             Job(c2, 5, 8, A.GND_INTEGRATOR_PATH, 7),
-            Job(c3, 2, 20, A.GND_INTEGRATOR_VOLPATH, 3)]
+            Job(c3, 2, 20, A.GND_INTEGRATOR_VOLPATH, 3),
+            Job(c2, 3, 0, A.GND_INTEGRATOR_DIRECT, 5),
+            Job(c3, 4, 0, A.GND_INTEGRATOR_AO, 1, 0.7)]
     together, st = device.render(ds, jobs)
-    assert st["samples"] == 48 * 32 * 3 + 40 * 24 * 5 + 17 * 13 * 2
+    assert st["samples"] == 48 * 32 * 3 + 40 * 24 * 5 + 17 * 13 * 2 + 40 * 24 * 3 + 17 * 13 * 4
     for i, j in enumerate(jobs):
         alone, _ = device.render(ds, [j])
         assert np.array_equal(alone[0], together[i]), f"job {i} differs when batched"

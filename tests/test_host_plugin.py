@@ -16,7 +16,7 @@ from oracle import fp
 
 def test_registry_names():
     names = integrator_names()
-    for n in ("path", "volpath", "path_hip", "volpath_hip"):
+    for n in ("path", "volpath", "path_hip", "volpath_hip", "direct", "direct_hip", "ao", "ao_hip"):
         assert n in names
 
 

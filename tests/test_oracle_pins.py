@@ -632,3 +632,79 @@ def test_environment_sampling_matches_distribution():
         want = prob_pix[iv, iu] * (H * W) / (2 * np.pi ** 2 * np.sin(th))
         assert abs(pdf - want) <= 1e-4 * want, (pdf, want)
 
+
+
+# ---------------------------------------------------------------- direct-lighting and AO integrators
+@pytest.mark.parametrize("albedo", [0.5, 0.8])
+def test_direct_integrator_furnace(albedo):
+    """DirectLight (direct.cpp:10-76) inside a closed box whose diffuse walls
+    (albedo a) all emit Le: the camera hit's emission plus one bounce of
+    direct light gives Le * (1 + a).  The light sample and the BSDF sample
+    each carry part of the bounce under balance-heuristic MIS; dropping
+    either, or weights that do not sum to one, moves the mean."""
+    tris, nrm = _inward_box()
+    b = SceneBuilder()
+    b.prim(b.mesh(tris, nrm), material=b.matte((albedo, albedo, albedo)), radiance=(1.0, 1.0, 1.0))
+    d = b.desc(width=8, height=8)
+    d.integrator = A.GND_INTEGRATOR_DIRECT
+    d.max_depth = 0                              # not used by the direct integrator
+    img, _ = RefScene(d).render(threads=8, spp=512, width=8, height=8)
+    v = img[..., :3] / img[..., 3:4]
+    expect = 1.0 + albedo
+    assert abs(v.mean() - expect) < 0.01 * expect, v.mean()
+    assert np.all(np.abs(v - expect) < 0.1 * expect)
+
+
+def test_direct_integrator_delta_light_closed_form():
+    """A point light over a Lambertian plane: the light sample has MIS weight
+    1 (p_bsdf = 0 for a delta light) and the BSDF ray escapes, so each pixel
+    is exactly f * I * cos / d^2 at 1 spp."""
+    b, cam, (W, H) = _plane_scene()
+    P = np.array([0.4, 2.0, -0.3])
+    b.light("point", position=P, power=(40, 30, 20))
+    d = _desc(b, cam, (W, H))
+    d.integrator = A.GND_INTEGRATOR_DIRECT
+    ref = RefScene(d)
+    img, _ = ref.render(threads=4, spp=1, width=W, height=H)
+    pts = _first_hits(ref, W, H)
+    v = P - pts
+    d2 = (v * v).sum(-1)
+    wi = v / np.sqrt(d2)[..., None]
+    want = 0.5 / np.pi * np.array([40, 30, 20]) / (4 * np.pi) * (wi[..., 1] / d2)[..., None]
+    assert np.allclose(img[..., :3], want, rtol=2e-4, atol=1e-6)
+
+
+@pytest.mark.parametrize("length,expect", [(0.5, 1.0), (2.0, 0.25), (4.0, 1.0 / 16)])
+def test_ao_between_parallel_planes(length, expect):
+    """AmbientOcclusion (ao.cpp:14-40) on a floor under a parallel ceiling at
+    height h = 1: a cosine-weighted ray of length L reaches the ceiling when
+    cos(theta) > h / L, which has probability 1 - (h/L)^2, so the mean is
+    (h/L)^2 for L > h and 1 for L <= h.  A sampling frame tilted off the
+    normal, or uniform instead of cosine sampling, changes it."""
+    b = SceneBuilder()
+    mt = b.matte((0.5, 0.5, 0.5))
+    Mf, Mfi = trs(rot_deg=-90.0, axis=(1, 0, 0))                  # floor y = 0 facing +y
+    b.prim(b.mesh(quad(-50, 50, -50, 50, 0.0, facing=+1)), Mf, Mfi, material=mt)
+    Mc, Mci = trs(rot_deg=-90.0, axis=(1, 0, 0), trans=(0, 1.0, 0))
+    b.prim(b.mesh(quad(-50, 50, -50, 50, 0.0, facing=-1)), Mc, Mci, material=mt)  # ceiling y = 1 facing -y
+    b.prim(b.mesh(np.array([[30, 30, 30, 31, 30, 30, 31, 30, 31]], np.float32)), material=mt, radiance=(1, 1, 1))
+    M, Mi = trs(rot_deg=-90.0, axis=(1, 0, 0), trans=(0, 0.5, 0))  # camera at y = 0.5 looking down
+    d = _desc(b, (M, Mi), (8, 8))
+    d.integrator = A.GND_INTEGRATOR_AO
+    d.ao_length = length
+    img, _ = RefScene(d).render(threads=8, spp=256, width=8, height=8)
+    v = img[..., :3] / img[..., 3:4]
+    assert np.allclose(v[..., 0], v[..., 1]) and np.allclose(v[..., 0], v[..., 2])
+    if expect == 1.0:
+        assert np.all(v == 1.0)
+    else:
+        assert abs(v.mean() - expect) < 0.015, v.mean()
+
+
+def test_ao_misses_are_white():
+    b, cam, (W, H) = _plane_scene(look="up")
+    b.light("point", position=(0, 5, 0))
+    d = _desc(b, cam, (W, H))
+    d.integrator = A.GND_INTEGRATOR_AO
+    img, _ = RefScene(d).render(threads=4, spp=2, width=W, height=H)
+    assert np.all(img[..., :3] == img[..., 3:4])
