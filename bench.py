@@ -15,8 +15,9 @@ sample of the same frame) and the 64-job batching workload.
 
 --scene cbox|volumetric measures the Cornell-box (512x512, 64 spp, depth 8)
 and volumetric (1280x720, 32 spp, depth 32) workloads instead; --aov also
-accumulates the film AOVs; --serve runs the serving workload (Poisson
-arrivals over a pool of 8 scenes) and prints its own JSON line.
+accumulates the film AOVs; --denoise times the film denoiser on the frame;
+--serve runs the serving workload (Poisson arrivals over a pool of 8 scenes)
+and prints its own JSON line.
 """
 from __future__ import annotations
 
@@ -33,6 +34,25 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak (8 TB/s)
 RAY_BYTES, HIT_BYTES, NODE_BYTES, PACK_BYTES, INST_BYTES = 32, 20, 144, 160, 64
 INTEGRATOR_NAMES = ("path", "volpath", "direct", "ao")
 METRIC = "Msamples/s & Mrays/s (1080p, 1M tris) at equal MSE; p50/p99 job latency @QPS"
+
+
+def denoise_workload(dev, sess, with_aovs: bool) -> dict:
+    """Device time of gnd_hip_denoise on the frame just rendered (resolved
+    radiance, plus the albedo / normal AOV guides when they were rendered):
+    best of 3 calls, and the algorithmic bytes of one a-trous pass (each
+    pixel's 48-byte record read once, its 16-byte result written once)."""
+    import numpy as np
+    from gianduia_amd import resolve, resolve_aovs
+    g = sess.film(0)
+    c = resolve(g)
+    al = nm = None
+    if with_aovs:
+        av = resolve_aovs(g, sess.film_aov(0))
+        al, nm = av["albedo"], av["normal"]
+    best = min(dev.denoise(c, al, nm, timed=True)[1] for _ in range(3))
+    H, W = c.shape[:2]
+    return {"ms": round(best, 3), "width": W, "height": H, "guides": bool(with_aovs), "passes": 5,
+            "pass_algorithmic_bytes": 64 * W * H}
 
 
 def log(*a):
@@ -233,6 +253,9 @@ def main():
     ap.add_argument("--params", default="", help="scene generator overrides, key=value;...")
     ap.add_argument("--aov", action="store_true",
                     help="also accumulate the film AOVs (albedo, normal, depth, roughness, metallic)")
+    ap.add_argument("--denoise", action="store_true",
+                    help="also time the film denoiser (gnd_hip_denoise) on the rendered frame, with the AOV "
+                         "guides when --aov is given")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration) and exit")
     a = ap.parse_args()
@@ -317,6 +340,11 @@ def main():
                                    "insts": cst["inst_tests"] / cst["rays_closest"]}}
         log(f"counting pass: {json.dumps(roof)}")
 
+    den = None
+    if a.denoise:
+        den = denoise_workload(dev, sess, a.aov)
+        log(f"denoise: {json.dumps(den)}")
+
     batch = None
     if not a.no_batch and a.scene == "scaled":
         batch = batch_workload(dev, ds, hs)
@@ -349,6 +377,8 @@ def main():
         "cpu_baseline": cpu,
         "batch": batch,
     }
+    if den:
+        out["denoise"] = den
     if cpu:
         out["vs_cpu_baseline"] = round(msps / cpu["value"], 1)
     print(json.dumps(out), flush=True)

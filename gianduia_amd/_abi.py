@@ -174,6 +174,8 @@ HIP_API = {
                                  C.POINTER(gnd_hip_stats)]),
     "gnd_hip_cancel": (C.c_int, [C.c_void_p]),
     "gnd_hip_copy_probe": (C.c_int, [C.c_void_p, C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
+    "gnd_hip_denoise": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
+                                  C.POINTER(C.c_double)]),
     "gnd_hip_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(gnd_hip_job), C.POINTER(gnd_hip_tuning)]),
     "gnd_hip_server_create": (C.c_int, [C.c_void_p, C.POINTER(gnd_hip_server_config), C.POINTER(C.c_void_p)]),
     "gnd_hip_server_destroy": (None, [C.c_void_p]),
@@ -211,6 +213,7 @@ HOST_API = {
     "gnd_host_integrator_names": (C.c_int, [C.c_char_p, C.c_int]),
     "gnd_host_sampler_floats": (C.c_int, [C.c_uint64, C.c_int, f32p]),
     "gnd_host_image_save": (C.c_int, [C.c_char_p, f32p, C.c_int, C.c_int]),
+    "gnd_host_integrator_denoise": (C.c_int, [C.c_void_p]),
     "gnd_host_image_load": (C.c_int, [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), f32p, C.c_size_t]),
 }
 

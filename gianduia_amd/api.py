@@ -134,6 +134,35 @@ class Device:
         another thread at its next bounce; that render raises GND_E_CANCELLED."""
         self._check(_lib().gnd_hip_cancel(self.ctx))
 
+    def denoise(self, color: np.ndarray, albedo: np.ndarray | None = None, normal: np.ndarray | None = None,
+                timed: bool = False):
+        """gnd_hip_denoise of (H, W, 3) float32 images (radiance; albedo and
+        the AOV normal n*0.5+0.5 as optional guides).  Returns the denoised
+        image, and the device milliseconds with timed=True."""
+        color = np.ascontiguousarray(color, np.float32)
+        H, W = color.shape[:2]
+        bufs = []
+        try:
+            for a in (color, albedo, normal):
+                if a is None:
+                    bufs.append(None)
+                    continue
+                a = np.ascontiguousarray(a, np.float32)
+                assert a.shape == color.shape
+                b = DeviceBuffer(self, a.nbytes)
+                self._check(_lib().gnd_hip_memcpy_h2d(self.ctx, b.ptr, a.ctypes.data_as(C.c_void_p), a.nbytes))
+                bufs.append(b)
+            ms = C.c_double()
+            ptr = [b.ptr if b is not None else None for b in bufs]
+            self._check(_lib().gnd_hip_denoise(self.ctx, ptr[0], ptr[1], ptr[2], W, H, ptr[0], C.byref(ms)))
+            out = np.empty_like(color)
+            self._check(_lib().gnd_hip_memcpy_d2h(self.ctx, out.ctypes.data_as(C.c_void_p), bufs[0].ptr, out.nbytes))
+        finally:
+            for b in bufs:
+                if b is not None:
+                    b.free()
+        return (out, ms.value) if timed else out
+
     def copy_probe(self, nbytes: int, repeats: int = 10) -> float:
         """Average ms of a plain device-to-device streaming copy of nbytes."""
         ms = C.c_double()
@@ -443,6 +472,13 @@ class Integrator:
         """Writes one AOV image ("albedo", "normal", "depth", "roughness",
         "metallic") as .exr or .pfm."""
         rc = _lib().gnd_host_integrator_save_aov(self.handle, aov.encode(), path.encode())
+        if rc != 0:
+            raise GndError(rc, host_error())
+
+    def denoise(self) -> None:
+        """Denoiser::execute on the film: the resolved radiance is replaced
+        by its denoised version (guided by the AOVs when they are on)."""
+        rc = _lib().gnd_host_integrator_denoise(self.handle)
         if rc != 0:
             raise GndError(rc, host_error())
 

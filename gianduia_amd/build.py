@@ -35,7 +35,7 @@ DEVICE = [f"--offload-arch={ARCH}", "-fno-gpu-flush-denormals-to-zero",
 
 HOST_SOURCES = ["host/hmath.cpp", "host/bvh_build.cpp", "host/scene_host.cpp", "host/generators.cpp",
                 "host/xml_loader.cpp", "host/integrator.cpp", "host/exr_read.cpp", "host/host_capi.cpp"]
-DEVICE_SOURCES = ["device/kernels.hip"]
+DEVICE_SOURCES = ["device/kernels.hip", "device/denoise.hip"]
 
 
 def _run(cmd: list[str]) -> None:

@@ -27,6 +27,7 @@
 #include "dev_common.h"
 #include "dev_shade.h"
 #include "dev_trace.h"
+#include "denoise.h"
 
 using namespace gdev;
 
@@ -1440,6 +1441,8 @@ struct gnd_hip_ctx {
     size_t taskCap = 0;
     std::atomic<int> cancel{0};  // set from any thread by gnd_hip_cancel
     hipEvent_t ev[2] = {nullptr, nullptr};
+    float4* dnScratch = nullptr;  // denoiser buffers (grown on demand)
+    size_t dnCap = 0;             // pixels dnScratch holds
 };
 
 namespace {
@@ -1780,6 +1783,7 @@ void gnd_hip_shutdown(gnd_hip_ctx* ctx) {
     if (ctx->dJobs) (void)hipFree(ctx->dJobs);
     if (ctx->dChunks) (void)hipFree(ctx->dChunks);
     if (ctx->dTasks) (void)hipFree(ctx->dTasks);
+    if (ctx->dnScratch) (void)hipFree(ctx->dnScratch);
     for (auto e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -2498,6 +2502,30 @@ int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jo
             fillStats(stats, c, samples);
         }
         if (c.stackOverflow) throw HipError(GND_E_STACK, "traversal stack overflow");
+    });
+}
+
+int gnd_hip_denoise(gnd_hip_ctx* ctx, const float* color, const float* albedo, const float* normal, int width,
+                    int height, float* output, double* ms) {
+    if (!ctx || !color || !output || width <= 0 || height <= 0) return GND_E_INVALID;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    return guarded(ctx, [&] {
+        GND_CHECK(hipSetDevice(ctx->device));
+        const size_t np = (size_t)width * (size_t)height;
+        if (np > ((size_t)1 << 28)) throw std::runtime_error("image too large to denoise");
+        if (np > ctx->dnCap) {
+            if (ctx->dnScratch) GND_CHECK(hipFree(ctx->dnScratch));
+            ctx->dnScratch = nullptr;
+            ctx->dnCap = 0;
+            GND_CHECK(hipMalloc((void**)&ctx->dnScratch, 4 * np * sizeof(float4)));
+            ctx->dnCap = np;
+        }
+        GND_CHECK(hipEventRecord(ctx->ev[0], ctx->stream));
+        GND_CHECK(gndDenoiseLaunch(ctx->stream, color, albedo, normal, width, height, output, ctx->dnScratch, nullptr,
+                                   nullptr));
+        GND_CHECK(hipEventRecord(ctx->ev[1], ctx->stream));
+        GND_CHECK(hipEventSynchronize(ctx->ev[1]));
+        if (ms) *ms = elapsedMs(ctx->ev[0], ctx->ev[1]);
     });
 }
 

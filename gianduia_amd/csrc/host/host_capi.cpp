@@ -280,6 +280,14 @@ int gnd_host_integrator_save_aov(gnd_host_integrator* it, const char* aov, const
     });
 }
 
+int gnd_host_integrator_denoise(gnd_host_integrator* it) {
+    if (!it) return GND_E_INVALID;
+    return hostGuard([&] {
+        gnd::Denoiser().execute(it->ctx, &it->impl->film());
+        return GND_OK;
+    });
+}
+
 void gnd_host_integrator_free(gnd_host_integrator* it) { delete it; }
 
 int gnd_host_integrator_names(char* buf, int len) {

@@ -327,6 +327,43 @@ void HipIntegrator::render(const RenderScene& scene) {
     if (devAov) gnd_hip_free(ctx_, devAov);
 }
 
+void Denoiser::execute(gnd_hip_ctx* ctx, Film* film) {
+    if (!film || film->width() <= 0 || film->rgb().empty()) return;
+    const int W = film->width(), H = film->height();
+    const size_t n = (size_t)W * H * 3;
+    std::vector<float> albedo, normal;
+    if (film->hasAovs()) {
+        albedo.resize(n);
+        normal.resize(n);
+        const std::vector<float>& a = film->aovs();
+        for (size_t i = 0; i < (size_t)W * H; ++i)
+            for (int c = 0; c < 3; ++c) {
+                albedo[3 * i + c] = a[i * Film::kAovs + c];
+                normal[3 * i + c] = a[i * Film::kAovs + 3 + c];
+            }
+    }
+    float* d[3] = {nullptr, nullptr, nullptr};  // color (also the output), albedo, normal
+    const int nbuf = albedo.empty() ? 1 : 3;
+    try {
+        for (int k = 0; k < nbuf; ++k) check(gnd_hip_malloc(ctx, n * sizeof(float), (void**)&d[k]), ctx);
+        check(gnd_hip_memcpy_h2d(ctx, d[0], film->rgb().data(), n * sizeof(float)), ctx);
+        if (nbuf == 3) {
+            check(gnd_hip_memcpy_h2d(ctx, d[1], albedo.data(), n * sizeof(float)), ctx);
+            check(gnd_hip_memcpy_h2d(ctx, d[2], normal.data(), n * sizeof(float)), ctx);
+        }
+        check(gnd_hip_denoise(ctx, d[0], d[1], d[2], W, H, d[0], nullptr), ctx);
+        std::vector<float> out(n);
+        check(gnd_hip_memcpy_d2h(ctx, out.data(), d[0], n * sizeof(float)), ctx);
+        film->replaceRgb(out.data());
+    } catch (...) {
+        for (float* p : d)
+            if (p) gnd_hip_free(ctx, p);
+        throw;
+    }
+    for (float* p : d)
+        if (p) gnd_hip_free(ctx, p);
+}
+
 void HipIntegrator::cancel() {
     stop_ = true;
     if (ctx_) gnd_hip_cancel(ctx_);

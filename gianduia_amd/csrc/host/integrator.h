@@ -82,6 +82,9 @@ public:
     // one resolved AOV image: "albedo" / "normal" (R, G, B) or "depth" /
     // "roughness" / "metallic" (Y), as ".exr" or ".pfm"
     void saveAov(const std::string& kind, const std::string& path) const;
+    // overwrites the resolved radiance (w*h*3) until the next resolve(), as
+    // Denoiser::execute writes into the film's radiance image
+    void replaceRgb(const float* rgb) { rgb_.assign(rgb, rgb + (size_t)w_ * h_ * 3); }
 
 private:
     int w_ = 0, h_ = 0;
@@ -100,6 +103,15 @@ void writePFM(const std::string& path, const float* px, int w, int h, int channe
 // rows top to bottom (channels it lacks read as 0); NONE / RLE / ZIPS / ZIP
 // compression, UINT / HALF / FLOAT channels.  Throws on anything else.
 void readEXR(const std::string& path, int& width, int& height, std::vector<float>& rgb);
+
+// Denoiser (include/gianduia/core/denoiser.h, src/core/denoiser.cpp:6-55):
+// execute(film) denoises the film's resolved radiance in place on the GPU
+// (gnd_hip_denoise), guided by the film's albedo and normal images when it
+// has AOVs.
+class Denoiser {
+public:
+    void execute(gnd_hip_ctx* ctx, Film* film);
+};
 
 // What an integrator renders: the scene description and its device copy.
 struct RenderScene {

@@ -196,6 +196,17 @@ int gnd_hip_cancel(gnd_hip_ctx* ctx);
  * Used to calibrate HBM counters (a known-size copy reads and writes bytes). */
 int gnd_hip_copy_probe(gnd_hip_ctx* ctx, size_t bytes, int repeats, double* ms_per_copy);
 
+/* Denoises a resolved film image in place of Denoiser::execute
+ * (src/core/denoiser.cpp:6-55: OIDN filter "RT", hdr, albedo and normal
+ * guides).  OIDN's trained network is not available here, so the filter is
+ * an edge-avoiding a-trous wavelet filter (five passes) on albedo-demodulated
+ * radiance guided by the albedo and normal images; see denoise.hip.  All
+ * pointers are device memory of width*height*3 floats: color (radiance),
+ * albedo (or NULL), normal (the AOV's n*0.5+0.5 with 0 = no surface, or
+ * NULL), output (may be color).  ms (optional): device time of the call. */
+int gnd_hip_denoise(gnd_hip_ctx* ctx, const float* color, const float* albedo, const float* normal, int width,
+                    int height, float* output, double* ms);
+
 /* Picks the block size (64/128/256) and LDS stack depth (8/16/24/32), then
  * the closest-hit and the any-hit refill thresholds (24..64), by timing one
  * pass of `job` on `scene` (best of two) per candidate, keeping the fastest of

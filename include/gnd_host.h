@@ -91,6 +91,11 @@ int gnd_host_integrator_save(gnd_host_integrator* it, const char* path);
  * image2d.h:62-97): aov = "albedo" | "normal" (R G B) or "depth" |
  * "roughness" | "metallic" (Y), path ".exr" or ".pfm". */
 int gnd_host_integrator_save_aov(gnd_host_integrator* it, const char* aov, const char* path);
+/* Denoiser::execute on the integrator's film (src/core/denoiser.cpp:6-55,
+ * src/app/application.cpp:299-300): the resolved radiance is replaced by its
+ * denoised version (gnd_hip_denoise), guided by the film's albedo and normal
+ * images when AOVs are on; save() afterwards writes the denoised image. */
+int gnd_host_integrator_denoise(gnd_host_integrator* it);
 void gnd_host_integrator_free(gnd_host_integrator* it);
 /* Registered integrator names, comma separated; returns the full length. */
 int gnd_host_integrator_names(char* buf, int len);

@@ -38,6 +38,7 @@ _API = {
     "gnd_ref_pcg32": (None, [C.c_uint64, C.c_uint64, C.c_int, C.POINTER(C.c_uint32)]),
     "gnd_ref_splitmix64": (C.c_uint64, [C.c_uint64]),
     "gnd_ref_sampler_floats": (None, [C.c_uint64, C.c_int, f32p]),
+    "gnd_ref_denoise": (None, [f32p, f32p, f32p, C.c_int, C.c_int, f32p]),
     "gnd_ref_fr_dielectric": (C.c_float, [C.c_float, C.c_float, C.c_float]),
     "gnd_ref_refract": (C.c_int, [f32p, f32p, C.c_float, f32p]),
     "gnd_ref_fr_conductor": (None, [C.c_float, f32p, f32p, f32p, f32p]),
@@ -92,6 +93,17 @@ def fp(a: np.ndarray):
 
 def vec(*v) -> np.ndarray:
     return np.array(v, dtype=np.float32)
+
+
+def denoise(color: np.ndarray, albedo: np.ndarray | None = None, normal: np.ndarray | None = None) -> np.ndarray:
+    """The film denoiser's a-trous filter on the CPU ((H, W, 3) float32)."""
+    color = np.ascontiguousarray(color, np.float32)
+    H, W = color.shape[:2]
+    a = None if albedo is None else np.ascontiguousarray(albedo, np.float32)
+    n = None if normal is None else np.ascontiguousarray(normal, np.float32)
+    out = np.empty_like(color)
+    lib().gnd_ref_denoise(fp(color), None if a is None else fp(a), None if n is None else fp(n), W, H, fp(out))
+    return out
 
 
 class RefScene:
