@@ -22,6 +22,9 @@ constexpr float kInv4Pi = 0.07957747154594766788f;
 constexpr float kFltMax = 3.402823466e+38f;
 constexpr double kPiD = 3.14159265358979323846;
 constexpr double kInvPiD = 0.31830988618379067154;  // M_1_PI
+// buckets of the guide tables that narrow the CDF searches of environment
+// sampling: entry j = the first CDF index whose value exceeds j / kEnvGuide
+constexpr int kEnvGuide = 64;
 
 GDEV_INLINE float finf() { return __builtin_inff(); }
 
@@ -176,6 +179,8 @@ struct DScene {
     const float* envCondCdf;   // envH * (envW + 1)
     const float* envCondInt;   // envH
     const float* envMargCdf;   // envH + 1
+    const int32_t* envCondGuide;  // envH * (kEnvGuide + 2): search ranges of the row CDFs
+    const int32_t* envMargGuide;  // kEnvGuide + 2: search ranges of the marginal CDF
     float envMargInt, envStrength;
     int32_t nLights, envLight, envW, envH;
 };

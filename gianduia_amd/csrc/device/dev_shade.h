@@ -691,9 +691,16 @@ GDEV_INLINE float envPdf(const DScene& S, V3 refP, V3 infoP) {
     return (float)((double)mapPdf / (2.0 * kPiD * kPiD * (double)sinTheta));
 }
 // Distribution1D::SampleContinuous (distribution.cpp:20-34) over func / cdf
-// of n entries: std::upper_bound, clamp, linear remap within the bucket
-GDEV_INLINE float sample1D(const float* func, const float* cdf, int n, float funcInt, float u, float& pdf, int& off) {
-    int lo = 0, cnt = n + 1;  // first cdf entry greater than u
+// of n entries: std::upper_bound, clamp, linear remap within the bucket.
+// The search runs over [guide[k], guide[k+1]] for u's bucket k = floor(u *
+// kEnvGuide) (exact: a power-of-two product): upper_bound is monotone in u and
+// guide[j] is its value at j / kEnvGuide, so the first entry greater than u
+// lies in that range, and the result is upper_bound's over the whole table.
+GDEV_INLINE float sample1D(const float* func, const float* cdf, const int32_t* guide, int n, float funcInt, float u,
+                           float& pdf, int& off) {
+    int k = (int)(u * (float)kEnvGuide);
+    k = k < 0 ? 0 : (k > kEnvGuide ? kEnvGuide : k);
+    int lo = guide[k], cnt = guide[k + 1] - lo;  // first cdf entry greater than u
     while (cnt > 0) {
         int step = cnt / 2, mid = lo + step;
         if (!(u < cdf[mid])) { lo = mid + 1; cnt -= step + 1; }
@@ -753,9 +760,9 @@ GDEV_INLINE void lightSample(const DScene& S, int k, V3 refP, float ux, float uy
     // environment: Distribution2D::SampleContinuous (distribution.cpp:65-73)
     float pdf1, pdf0;
     int v, unused;
-    const float d1 = sample1D(S.envCondInt, S.envMargCdf, S.envH, S.envMargInt, uy, pdf1, v);
-    const float d0 = sample1D(S.envCondFunc + (size_t)v * S.envW, S.envCondCdf + (size_t)v * (S.envW + 1), S.envW,
-                              S.envCondInt[v], ux, pdf0, unused);
+    const float d1 = sample1D(S.envCondInt, S.envMargCdf, S.envMargGuide, S.envH, S.envMargInt, uy, pdf1, v);
+    const float d0 = sample1D(S.envCondFunc + (size_t)v * S.envW, S.envCondCdf + (size_t)v * (S.envW + 1),
+                              S.envCondGuide + (size_t)v * (kEnvGuide + 2), S.envW, S.envCondInt[v], ux, pdf0, unused);
     const float mapPdf = pdf0 * pdf1;
     if (mapPdf == 0.0f) {
         ls.pdf = 0.0f;

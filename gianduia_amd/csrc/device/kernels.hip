@@ -1549,8 +1549,19 @@ DLight makeLight(const gnd_light_desc& l) {
 // rows' integrals.
 struct EnvTables {
     std::vector<float> rgb, condFunc, condCdf, condInt, margCdf;
+    std::vector<int32_t> condGuide, margGuide;
     float margInt = 0.0f;
 };
+// guide[j] = std::upper_bound(cdf, cdf + n + 1, j / kEnvGuide) - cdf for
+// j = 0..kEnvGuide, guide[kEnvGuide + 1] = n + 1 (u = 1 exactly, which the
+// sampler can draw)
+static void guide1D(const float* cdf, int n, int32_t* g) {
+    for (int j = 0; j <= kEnvGuide; ++j) {
+        const float u = (float)j / (float)kEnvGuide;
+        g[j] = (int32_t)(std::upper_bound(cdf, cdf + n + 1, u) - cdf);
+    }
+    g[kEnvGuide + 1] = n + 1;
+}
 static void dist1D(const float* f, int n, float* cdf, float& funcInt) {
     cdf[0] = 0.0f;
     for (int i = 1; i < n + 1; ++i) cdf[i] = cdf[i - 1] + f[i - 1] / n;
@@ -1579,6 +1590,11 @@ EnvTables envTables(const gnd_light_desc& l) {
     for (int v = 0; v < H; ++v) dist1D(&t.condFunc[(size_t)v * W], W, &t.condCdf[(size_t)v * (W + 1)], t.condInt[v]);
     t.margCdf.resize(H + 1);
     dist1D(t.condInt.data(), H, t.margCdf.data(), t.margInt);
+    t.condGuide.resize((size_t)H * (kEnvGuide + 2));
+    for (int v = 0; v < H; ++v)
+        guide1D(&t.condCdf[(size_t)v * (W + 1)], W, &t.condGuide[(size_t)v * (kEnvGuide + 2)]);
+    t.margGuide.resize(kEnvGuide + 2);
+    guide1D(t.margCdf.data(), H, t.margGuide.data());
     return t;
 }
 
@@ -2051,6 +2067,8 @@ int gnd_hip_scene_upload(gnd_hip_ctx* ctx, const gnd_hip_scene_arrays* a, gnd_hi
             v.envCondCdf = devAlloc<float>(s, t.condCdf.size(), t.condCdf.data());
             v.envCondInt = devAlloc<float>(s, t.condInt.size(), t.condInt.data());
             v.envMargCdf = devAlloc<float>(s, t.margCdf.size(), t.margCdf.data());
+            v.envCondGuide = devAlloc<int32_t>(s, t.condGuide.size(), t.condGuide.data());
+            v.envMargGuide = devAlloc<int32_t>(s, t.margGuide.size(), t.margGuide.data());
             v.envMargInt = t.margInt;
             v.envStrength = d.lights[envLight].strength;
             v.envW = d.lights[envLight].env_width;

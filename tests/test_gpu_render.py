@@ -458,3 +458,65 @@ def test_render_rejects_bad_jobs(device):
     j.camera.width = 0
     with pytest.raises(GndError):
         device.render(ds, [j])
+
+
+# ---------------------------------------------------------------- XML scene with every emitter kind
+XML_LIGHTS = """<scene>
+  <camera type="perspective">
+    <transform name="toWorld"><lookat origin="0, 1, 4" target="0, 0, 0" up="0, 1, 0"/></transform>
+    <float name="fov" value="50"/>
+    <integer name="width" value="64"/><integer name="height" value="48"/>
+    <filter type="box"/>
+  </camera>
+  <sampler type="independent"><integer name="spp" value="1"/></sampler>
+  <integrator type="{integrator}"><integer name="maxDepth" value="6"/></integrator>
+  <emitter type="point"><point name="position" value="0.5, 1.5, 0.5"/><color name="power" value="8, 7, 6"/></emitter>
+  <primitive>
+    <transform name="toWorld"><scale x="6" y="1" z="6"/><translate x="0" y="-0.5" z="0"/></transform>
+    <shape type="mesh"><string name="filename" value="plane.obj"/></shape>
+    <material type="matte"><color name="albedo" value="0.6 0.6 0.6"/></material>
+  </primitive>
+  <primitive>
+    <shape type="sphere"><float name="radius" value="0.5"/></shape>
+    <material type="conductor"><float name="roughness" value="0.2"/></material>
+  </primitive>
+  <primitive>
+    <transform name="toWorld"><translate x="-1.2" y="0" z="0.3"/></transform>
+    <shape type="sphere"><float name="radius" value="0.4"/></shape>
+    <material type="plastic"><float name="roughness" value="0.3"/></material>
+    <emitter type="area"><color name="radiance" value="2, 1, 0.5"/></emitter>
+  </primitive>
+  <emitter type="spot"><point name="position" value="-1, 2, 1"/><vector name="direction" value="0.5, -1, -0.5"/>
+    <float name="spot_size" value="40"/></emitter>
+  <emitter type="directional"><vector name="direction" value="-0.3, -1, -0.2"/><color name="irradiance" value="0.8, 0.8, 0.7"/></emitter>
+  <emitter type="environment"><string name="filename" value="sky.exr"/><float name="strength" value="0.7"/></emitter>
+</scene>
+"""
+
+
+@pytest.mark.parametrize("integrator", ["path", "direct"])
+def test_xml_scene_with_all_emitters_matches_oracle(device, tmp_path, integrator):
+    """A scene file with point, area, spot, directional and environment
+    emitters (the map a ZIP-compressed half-float OpenEXR file) loads through
+    the XML loader and renders on the GPU as the oracle renders it."""
+    import imgio
+    rng = np.random.default_rng(11)
+    H, W = 16, 32
+    sky = {c: (0.3 + rng.random((H, W))).astype(np.float32) for c in "RGB"}
+    sky["R"][4:6, 10:13] = 40.0                       # a hot spot for the importance sampling
+    imgio.write_exr(tmp_path / "sky.exr", sky, ptype=1, compression=3)
+    (tmp_path / "plane.obj").write_text("v -0.5 0 -0.5\nv 0.5 0 -0.5\nv 0.5 0 0.5\nv -0.5 0 0.5\n"
+                                        "vn 0 1 0\nf 1//1 3//1 2//1\nf 1//1 4//1 3//1\n")
+    (tmp_path / "scene.xml").write_text(XML_LIGHTS.replace("{integrator}", integrator))
+    hs = HostScene.load_xml(str(tmp_path / "scene.xml"))
+    assert hs.desc().contents.nlights == 4
+    ds = DeviceScene(device, hs)
+    out, st = device.render(ds, [hs.default_job(spp=1)])
+    r, _ = RefScene(hs.desc()).render(threads=16, spp=1, width=64, height=48)
+    frac = _replay_fraction(out[0], r)
+    print(f"xml {integrator}: {frac:.4f} of paths reproduce the oracle", flush=True)
+    assert st["nan_samples"] == 0 and frac >= 0.99
+    out16, _ = device.render(ds, [hs.default_job(spp=16)], want_sq=True)
+    g, gsq = out16[0]
+    r16, rsq, _ = RefScene(hs.desc()).render(threads=16, spp=16, squares=True, width=64, height=48)
+    _check_stats(g, gsq, r16, rsq, 16, f"xml {integrator}")
