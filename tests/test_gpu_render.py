@@ -349,10 +349,12 @@ def test_seed_offset_changes_samples(device):
 
 
 # ---------------------------------------------------------------- launch shape, autotune, cancel
-def test_tunings_give_identical_films(device):
+@pytest.mark.parametrize("params", ["level=4;instances=12;width=64;height=40",
+                                    "level=3;instances=6;width=48;height=32;lights=sun,env;integrator=direct"])
+def test_tunings_give_identical_films(device, params):
     """Block size, LDS stack depth, wave capacity and refill thresholds change
     scheduling only."""
-    hs = HostScene.generate("scaled", "level=4;instances=12;width=64;height=40")
+    hs = HostScene.generate("scaled", params)
     ds = DeviceScene(device, hs)
     job = hs.default_job(spp=2)
     old = device.tuning()
@@ -370,7 +372,9 @@ def test_tunings_give_identical_films(device):
 
 @pytest.mark.parametrize("name,params,spp", [("cbox", "width=64;height=48", 3),
                                              ("volumetric", "grid=16;width=40;height=32", 2),
-                                             ("scaled", "level=3;instances=4;width=48;height=32", 2)])
+                                             ("scaled", "level=3;instances=4;width=48;height=32", 2),
+                                             ("cbox", "width=48;height=40;integrator=direct;lights=point,env", 2),
+                                             ("volumetric", "grid=16;width=40;height=32;integrator=ao", 2)])
 def test_graph_bounce_chain_identical(device, name, params, spp):
     """The bounce chain captured in a hipGraph (no host sync between bounces)
     renders the same films as the synchronised loop, replay after replay."""

@@ -66,6 +66,29 @@ def test_server_aovs_equal_solo_render(device):
         srv.close()
 
 
+def test_server_direct_ao_and_lights_equal_solo_render(device):
+    """Served jobs of every integrator, over a scene with point, spot, sun
+    and sky lights, equal their solo renders bit for bit."""
+    hs = HostScene.generate("cbox", "width=48;height=40;lights=point,spot,sun,env;envres=16")
+    srv = Server(device, path_slots=12000, max_jobs=8)
+    try:
+        sid = srv.add_scene(hs)
+        jobs = _jobs(hs, 8, 9)
+        for k, j in enumerate(jobs):
+            j.integrator = (A.GND_INTEGRATOR_PATH, A.GND_INTEGRATOR_DIRECT, A.GND_INTEGRATOR_AO)[k % 3]
+            j.ao_length = 0.6
+        tickets = [(j, srv.submit(sid, j)) for j in jobs]
+        for j, tk in tickets:
+            assert srv.wait(tk, 120)["state"] == 2
+        ds = DeviceScene(device, hs)
+        for j, tk in tickets:
+            solo, _ = device.render(ds, [j])
+            assert np.array_equal(srv.film(tk), solo[0]), f"integrator {j.integrator}: served film differs"
+        ds.free()
+    finally:
+        srv.close()
+
+
 def test_server_evicts_under_budget(device):
     scenes = [HostScene.generate("scaled", f"level=3;instances=4;seed={i};width=32;height=32") for i in range(3)]
     probe = DeviceScene(device, scenes[0])
