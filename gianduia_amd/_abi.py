@@ -51,7 +51,9 @@ class gnd_camera_desc(C.Structure):
                 ("to_world", C.c_float * 16), ("to_world_inv", C.c_float * 16), ("lens_radius", C.c_float),
                 ("focal_distance", C.c_float), ("k1", C.c_float), ("ca_axial", C.c_float),
                 ("ca_lateral", C.c_float), ("vignetting", C.c_int32), ("blades", C.c_int32),
-                ("filter", C.c_int32), ("filter_radius", C.c_float * 2), ("filter_alpha", C.c_float)]
+                ("filter", C.c_int32), ("filter_radius", C.c_float * 2), ("filter_alpha", C.c_float),
+                ("has_motion", C.c_int32), ("to_world_end", C.c_float * 16), ("rolling_shutter", C.c_int32),
+                ("scan_duration", C.c_float), ("exposure_duration", C.c_float)]
 
 
 GND_LIGHT_POINT, GND_LIGHT_SPOT, GND_LIGHT_DIRECTIONAL, GND_LIGHT_ENVIRONMENT = range(4)

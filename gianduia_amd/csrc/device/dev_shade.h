@@ -1074,11 +1074,17 @@ struct DCamera {
     int filter;
     float radiusX, radiusY, alpha;
     float expX, expY;
-    int hasCA, pad;
+    int hasCA, animated;      // animated: the camera moves (motion blur)
+    int rolling;              // thin lens rolling shutter
+    float scanDuration, exposureDuration;
+    M4 toWorldEnd;
+    DMotion motion;
 };
 
-GDEV_INLINE float shootRay(const DCamera& c, float fx, float fy, float lx, float ly, float lambdaOffset, V3& ro,
-                           V3& rd) {
+// time: the sample's time draw (rolling shutter remaps it on the thin lens);
+// out: the ray's time, which only animated geometry reads
+GDEV_INLINE float shootRay(const DCamera& c, float fx, float fy, float lx, float ly, float lambdaOffset, float time,
+                           V3& ro, V3& rd, float& rayTime) {
     V3 dir, org = V3{0, 0, 0};
     float weight = 1.0f;
     if (c.type == GND_CAMERA_PERSPECTIVE) {
@@ -1112,8 +1118,25 @@ GDEV_INLINE float shootRay(const DCamera& c, float fx, float fy, float lx, float
             weight = c2 * c2;
         }
     }
-    ro = xfPoint(c.toWorld, org);
-    rd = xfVector(c.toWorld, dir);
+    // thinLens.cpp:81-87: rolling shutter; perspective.cpp:27-29 uses the draw
+    float t = time;
+    if (c.type != GND_CAMERA_PERSPECTIVE && c.rolling) {
+        const float scanline = 1.0f - fy;
+        t = scanline * c.scanDuration + (time * c.exposureDuration);
+    }
+    rayTime = t;
+    // AnimatedTransform::interpolate: the keyframes at or beyond the ends
+    if (!c.animated || t <= 0.0f) {
+        ro = xfPoint(c.toWorld, org);
+        rd = xfVector(c.toWorld, dir);
+    } else if (t >= 1.0f) {
+        ro = xfPoint(c.toWorldEnd, org);
+        rd = xfVector(c.toWorldEnd, dir);
+    } else {
+        const M4 M = motionMatrix(c.motion, (t - 0.0f) / (1.0f - 0.0f));
+        ro = xfPoint(M, org);
+        rd = xfVector(M, dir);
+    }
     return weight;
 }
 

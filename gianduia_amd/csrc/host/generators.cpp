@@ -452,6 +452,19 @@ std::unique_ptr<HostScene> generateScene(const std::string& name, const std::str
         else throw std::runtime_error("unknown integrator: " + t);
     }
     s->aoLength = (float)num(p, "aolength", 5.0);
+    // cammove=<distance>: the camera slides that far along its right axis and
+    // turns by camturn degrees about world y over the shutter interval
+    // (motion blur); rolling=1 turns on the thin lens's rolling shutter
+    const double move = num(p, "cammove", 0.0), turn = num(p, "camturn", 0.0);
+    if (move != 0.0 || turn != 0.0) {
+        Xform start;  // only the forward matrix matters here
+        std::memcpy(start.m.m, s->camera.to_world, sizeof(start.m.m));
+        const Xform end = start.then(xformTranslate((float)move, 0.0f, 0.0f))
+                              .then(xformRotate((float)turn, 0.0f, 1.0f, 0.0f));
+        s->camera.has_motion = 1;
+        std::memcpy(s->camera.to_world_end, end.m.m, sizeof(s->camera.to_world_end));
+    }
+    if (num(p, "rolling", 0.0) != 0.0) s->camera.rolling_shutter = 1;
     return s;
 }
 

@@ -120,6 +120,9 @@ void HostScene::setCamera(int type, int w, int h, float fov, const Xform& toWorl
     camera.filter = GND_FILTER_GAUSSIAN;  // film.cpp:29 default filter
     camera.filter_radius[0] = camera.filter_radius[1] = 2.0f;
     camera.filter_alpha = 2.0f;
+    std::memcpy(camera.to_world_end, toWorld.m.m, sizeof(camera.to_world_end));
+    camera.scan_duration = 0.9f;      // thinLens.cpp:30
+    camera.exposure_duration = 0.1f;  // thinLens.cpp:31
 }
 
 LightData lightDefaults(int type) {

@@ -464,7 +464,15 @@ struct Loader {
         c.ca_lateral = p.getF("ca_lateral", 0.0f);
         c.vignetting = p.getB("vignetting", false) ? 1 : 0;
         c.blades = p.getI("blades", 0);
-        if (p.getB("rolling_shutter", false)) throw std::runtime_error("rolling shutter is not supported");
+        if (p.t.count("toWorldEnd")) {  // camera.h:22-25
+            c.has_motion = 1;
+            std::memcpy(c.to_world_end, p.t.at("toWorldEnd").m.m, sizeof(c.to_world_end));
+        }
+        if (type == "thinlens") {  // thinLens.cpp:29-31
+            c.rolling_shutter = p.getB("rolling_shutter", false) ? 1 : 0;
+            c.scan_duration = p.getF("scan_duration", 0.9f);
+            c.exposure_duration = p.getF("exposure_duration", 0.1f);
+        }
         for (const XNode* k : kids) {
             if (k->tag == "filter") {
                 std::vector<const XNode*> fk;

@@ -119,6 +119,15 @@ typedef struct gnd_camera_desc {
     int32_t filter;
     float filter_radius[2];
     float filter_alpha;
+    /* motion blur (Camera ctor, camera.h:20-25, AnimatedTransform): with
+     * has_motion the camera moves from to_world at time 0 to to_world_end at
+     * time 1 (decomposed, mixed / slerped per ray); rolling shutter
+     * (thin lens only, thinLens.cpp:29-31, 81-87): time = (1 - film y) *
+     * scan_duration + sample time * exposure_duration                      */
+    int32_t has_motion;
+    float to_world_end[16];
+    int32_t rolling_shutter;
+    float scan_duration, exposure_duration;
 } gnd_camera_desc;
 
 /* An emitter that is not a shape (Scene::addChild's EEmitter case,

@@ -221,6 +221,7 @@ struct Ray {
     float tMin = kEpsilon;
     float tMax = kInf;
     int medium = -1;
+    float time = 0.0f;  // ray.h:19, read by animated transforms only
 };
 
 struct Mesh {
@@ -303,9 +304,21 @@ struct Prim {
     M4 toWorld, toWorldInv;
     Bounds worldBounds;
 };
+// AnimatedTransform (src/math/animated.cpp, include/gianduia/math/animated.h):
+// keyframes at times 0 and 1, decomposed into translation, rotation
+// quaternion (x, y, z, w) and scale when the matrices differ
+struct Motion {
+    bool animated = false;
+    M4 start, end;
+    float T[2][3], R[2][4], S[2][3];
+};
+Motion motionFromMatrices(const float* start, const float* end);
+M4 motionInterpolate(const Motion& mo, float time);
+
 struct Camera {
     gnd_camera_desc d;
     M4 toWorld, toWorldInv;
+    Motion motion;
     float scale = 1.0f;
     float aspect = 1.0f;
 };

@@ -526,6 +526,8 @@ void sceneBuild(Scene& s, const gnd_scene_desc& d) {
     s.camera.d = d.camera;
     s.camera.toWorld = toM4(d.camera.to_world);
     s.camera.toWorldInv = toM4(d.camera.to_world_inv);
+    s.camera.motion = motionFromMatrices(d.camera.to_world, d.camera.has_motion ? d.camera.to_world_end
+                                                                                : d.camera.to_world);
     // PerspectiveCamera / ThinLensCamera ctor: m_scale = tan(Radians(fov/2))
     s.camera.scale = std::tan((kPi / 180.f) * (d.camera.fov * 0.5f));
     s.camera.aspect = d.camera.width / (float)d.camera.height;

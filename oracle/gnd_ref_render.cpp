@@ -1020,9 +1020,18 @@ static float shootRay(const Camera& c, const CamSample& cs, Ray& ray) {
             weight = c2 * c2;
         }
     }
+    // rolling shutter (thinLens.cpp:81-87), then the camera transform at that
+    // time (AnimatedTransform::interpolate)
+    float finalTime = cs.time;
+    if (c.d.type != GND_CAMERA_PERSPECTIVE && c.d.rolling_shutter) {
+        float scanline = 1.0f - cs.pFilm.y;
+        finalTime = scanline * c.d.scan_duration + (cs.time * c.d.exposure_duration);
+    }
+    const M4 toWorld = motionInterpolate(c.motion, finalTime);
     ray = Ray();
-    ray.o = xfPoint(c.toWorld, org);
-    ray.d = xfVector(c.toWorld, dir);
+    ray.o = xfPoint(toWorld, org);
+    ray.d = xfVector(toWorld, dir);
+    ray.time = finalTime;
     return weight;
 }
 
@@ -1747,17 +1756,20 @@ int gnd_ref_bounds_intersect(const float* bmin, const float* bmax, const float* 
 
 // Camera ray for a given camera sample (pFilm in [0,1]^2 with y up, pLens,
 // lambdaOffset).  out: o[3], d[3], weight.
-void gnd_ref_camera_ray(void* h, const float* pfilm, const float* plens, float lambda_offset, float* out) {
+void gnd_ref_camera_ray(void* h, const float* pfilm, const float* plens, float lambda_offset, float time,
+                        float* out) {
     Scene* s = static_cast<Scene*>(h);
     CamSample cs;
     cs.pFilm = P2{pfilm[0], pfilm[1]};
     cs.pLens = P2{plens[0], plens[1]};
     cs.lambdaOffset = lambda_offset;
+    cs.time = time;
     Ray r;
     float w = shootRay(s->camera, cs, r);
     out[0] = r.o.x; out[1] = r.o.y; out[2] = r.o.z;
     out[3] = r.d.x; out[4] = r.d.y; out[5] = r.d.z;
     out[6] = w;
+    out[7] = r.time;
 }
 
 // Surface interaction of a closest hit: out p[3], n[3], dpdu[3], uv[2]

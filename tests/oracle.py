@@ -57,7 +57,9 @@ _API = {
     "gnd_ref_hg_sample": (C.c_float, [C.c_float, f32p, f32p, f32p]),
     "gnd_ref_luminance": (C.c_float, [C.c_float, C.c_float, C.c_float]),
     "gnd_ref_bounds_intersect": (C.c_int, [f32p, f32p, f32p, f32p, C.c_float, f32p]),
-    "gnd_ref_camera_ray": (None, [C.c_void_p, f32p, f32p, C.c_float, f32p]),
+    "gnd_ref_camera_ray": (None, [C.c_void_p, f32p, f32p, C.c_float, C.c_float, f32p]),
+    "gnd_ref_motion_matrix": (None, [f32p, f32p, C.c_float, f32p]),
+    "gnd_ref_decompose": (None, [f32p, f32p, f32p, f32p]),
     "gnd_ref_surface": (C.c_int, [C.c_void_p, f32p, f32p, f32p]),
     "gnd_ref_medium": (C.c_int, [C.c_void_p, C.c_int, C.c_int, f32p, f32p, C.c_float, C.c_uint64, f32p]),
     "gnd_ref_grid_sample": (C.c_float, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double]),
@@ -93,6 +95,22 @@ def fp(a: np.ndarray):
 
 def vec(*v) -> np.ndarray:
     return np.array(v, dtype=np.float32)
+
+
+def motion_matrix(start: np.ndarray, end: np.ndarray, time: float) -> np.ndarray:
+    """AnimatedTransform(start, 0, end, 1).interpolate(time) of column-major
+    4x4 float32 matrices (returned column-major, flat)."""
+    out = np.zeros(16, np.float32)
+    lib().gnd_ref_motion_matrix(fp(np.ascontiguousarray(start, np.float32).ravel()),
+                                fp(np.ascontiguousarray(end, np.float32).ravel()), time, fp(out))
+    return out
+
+
+def decompose(m: np.ndarray):
+    """glm::decompose of a column-major affine matrix: T, quaternion (x, y, z, w), S."""
+    T, Q, S = np.zeros(3, np.float32), np.zeros(4, np.float32), np.zeros(3, np.float32)
+    lib().gnd_ref_decompose(fp(np.ascontiguousarray(m, np.float32).ravel()), fp(T), fp(Q), fp(S))
+    return T, Q, S
 
 
 def denoise(color: np.ndarray, albedo: np.ndarray | None = None, normal: np.ndarray | None = None) -> np.ndarray:
@@ -187,9 +205,13 @@ class RefScene:
         out = (rgbw, sq, list(stats)) if squares else (rgbw, list(stats))
         return out + (aov,) if aovs else out
 
-    def camera_ray(self, pfilm, plens=(0.5, 0.5), lam=0.0):
-        out = np.zeros(7, np.float32)
-        lib().gnd_ref_camera_ray(self.h, fp(vec(*pfilm)), fp(vec(*plens)), lam, fp(out))
+    def camera_ray(self, pfilm, plens=(0.5, 0.5), lam=0.0, time=0.0, with_time=False):
+        """Camera ray (origin, direction, weight[, ray time]) for a film
+        position, lens sample, chromatic offset and time draw."""
+        out = np.zeros(8, np.float32)
+        lib().gnd_ref_camera_ray(self.h, fp(vec(*pfilm)), fp(vec(*plens)), lam, time, fp(out))
+        if with_time:
+            return out[:3], out[3:6], float(out[6]), float(out[7])
         return out[:3], out[3:6], float(out[6])
 
     def light_sample(self, k, ref_p, u):

@@ -464,6 +464,28 @@ def test_render_rejects_bad_jobs(device):
         device.render(ds, [j])
 
 
+# ---------------------------------------------------------------- camera motion blur, rolling shutter
+CAMERA_MOTION = [
+    ("cbox", "width=64;height=64;cammove=0.3;camturn=8"),                       # pinhole, slide + turn
+    ("scaled", "level=3;instances=6;width=64;height=40;cammove=0.2;camturn=-5;rolling=1"),  # thin lens
+    ("volumetric", "grid=16;width=48;height=32;camturn=12"),
+]
+
+
+@pytest.mark.parametrize("name,params", CAMERA_MOTION)
+def test_camera_motion_matches_oracle(device, name, params):
+    """A moving camera (AnimatedTransform: decomposed keyframes, mix / slerp
+    per sample time) and the thin lens's rolling shutter: single-sample
+    replay against the oracle and per-pixel statistics at 16 spp."""
+    print(f"camera motion {name} {params}", flush=True)
+    g, _, r, _, st = _render_pair(device, name, params, 1, squares=False)
+    frac = _replay_fraction(g, r)
+    print(f"{name} {params}: {frac:.4f} of paths reproduce the oracle", flush=True)
+    assert st["nan_samples"] == 0 and frac >= 0.98
+    g, gsq, r, rsq, _ = _render_pair(device, name, params, 16)
+    _check_stats(g, gsq, r, rsq, 16, f"motion {name}")
+
+
 # ---------------------------------------------------------------- XML scene with every emitter kind
 XML_LIGHTS = """<scene>
   <camera type="perspective">

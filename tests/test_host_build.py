@@ -244,3 +244,23 @@ def test_xml_loader(tmp_path):
     from gianduia_amd import GndError
     with pytest.raises(GndError):
         HostScene.load_xml(str(bad))
+
+
+def test_xml_camera_motion_and_rolling_shutter(tmp_path):
+    """<transform name="toWorldEnd"> on the camera (camera.h:20-25) and the
+    thin lens's rolling-shutter properties (thinLens.cpp:29-31) reach the
+    camera description."""
+    (tmp_path / "plane.obj").write_text(PLANE_OBJ)
+    xml = CBOX_XML.replace('<camera type="perspective">', '<camera type="thinlens">').replace(
+        '<filter type="box"/>',
+        '<filter type="box"/><transform name="toWorldEnd"><lookat origin="0.5, 0, 3.9" target="0, 0, 0" '
+        'up="0, 1, 0"/></transform><boolean name="rolling_shutter" value="true"/>'
+        '<float name="scan_duration" value="0.6"/>')
+    (tmp_path / "scene.xml").write_text(xml)
+    c = HostScene.load_xml(str(tmp_path / "scene.xml")).desc().contents.camera
+    assert c.has_motion == 1 and c.rolling_shutter == 1
+    assert abs(c.scan_duration - 0.6) < 1e-7 and abs(c.exposure_duration - 0.1) < 1e-7
+    end = np.array(c.to_world_end[:]).reshape(4, 4).T
+    assert np.allclose(end[:3, 3], (0.5, 0, 3.9), atol=1e-6)
+    start = HostScene.load_xml(str(tmp_path / "scene.xml")).desc().contents.camera.to_world[:]
+    assert not np.allclose(start, c.to_world_end[:])

@@ -744,3 +744,76 @@ def test_denoiser_filter_properties():
     # the red channel of the right (green) region keeps its albedo ratio: no red bleeds in
     ratio = d[:, W // 2:, 0] / d[:, W // 2:, 1]
     assert np.all(np.abs(ratio - 0.2 / 0.7) < 0.02)
+
+
+# ---------------------------------------------------------------- motion blur (AnimatedTransform)
+def _trs(t, axis, deg, s):
+    """Column-major T * R(axis, deg) * S and the quaternion (x, y, z, w) of R."""
+    a = np.asarray(axis, np.float64) / np.linalg.norm(axis)
+    th = np.radians(deg)
+    K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+    R = np.eye(3) + np.sin(th) * K + (1 - np.cos(th)) * K @ K
+    M = np.eye(4)
+    M[:3, :3] = R @ np.diag(s)
+    M[:3, 3] = t
+    q = np.array([*(a * np.sin(th / 2)), np.cos(th / 2)])
+    return M.T.astype(np.float32).ravel(), q
+
+
+@pytest.mark.parametrize("t,axis,deg,s", [((1, 2, 3), (0, 1, 0), 30, (1, 1, 1)),
+                                          ((-2, 0.5, 4), (1, 2, -1), 75, (2, 0.5, 3)),
+                                          ((0, 0, 0), (1, 0, 0), 170, (1, 2, 1)),     # trace < 0 branch
+                                          ((3, -1, 0), (0.3, -1, 0.2), 200, (0.7, 0.7, 0.7))])
+def test_decompose_recovers_trs(t, axis, deg, s):
+    """glm::decompose as AnimatedTransform calls it (animated.cpp:24-28):
+    translation, scale and rotation of a T * R * S matrix come back (the
+    quaternion up to sign)."""
+    M, q = _trs(t, axis, deg, s)
+    T, Q, S = O.decompose(M)
+    assert np.allclose(T, t, atol=1e-6) and np.allclose(S, s, rtol=1e-5)
+    assert np.allclose(Q, q, atol=1e-5) or np.allclose(Q, -q, atol=1e-5)
+
+
+def test_motion_interpolation_closed_form():
+    """AnimatedTransform::interpolate (animated.cpp:30-47): the keyframes at
+    or beyond the ends (bit for bit), and half way between a translation +
+    rotation of 80 degrees about y the mid translation and a 40-degree
+    rotation (slerp), with the scale mixed linearly."""
+    A, _ = _trs((0, 0, 0), (0, 1, 0), 0, (1, 1, 1))
+    B, _ = _trs((4, 2, 0), (0, 1, 0), 80, (2, 2, 2))
+    for t in (-0.5, 0.0):
+        assert np.array_equal(O.motion_matrix(A, B, t), A)
+    for t in (1.0, 1.5):
+        assert np.array_equal(O.motion_matrix(A, B, t), B)
+    mid, _ = _trs((2, 1, 0), (0, 1, 0), 40, (1.5, 1.5, 1.5))
+    assert np.allclose(O.motion_matrix(A, B, 0.5), mid, atol=1e-5)
+    q1, _ = _trs((0, 0, 0), (0, 1, 0), 20, (1.25, 1.25, 1.25))
+    got = O.motion_matrix(A, B, 0.25).reshape(4, 4).T
+    assert np.allclose(got[:3, :3], q1.reshape(4, 4).T[:3, :3], atol=1e-5) and np.allclose(got[:3, 3], (1, 0.5, 0))
+
+
+def test_camera_motion_and_rolling_shutter_rays():
+    """A moving camera (camera.h:20-25) shoots from the interpolated pose at
+    the sample's time; the thin lens's rolling shutter replaces that time by
+    (1 - film y) * scan + time * exposure (thinLens.cpp:81-87)."""
+    b = SceneBuilder()
+    b.prim(b.mesh(quad(-1, 1, -1, 1, -5.0)), material=b.matte(), radiance=(1, 1, 1))
+    d = b.desc(width=16, height=16)
+    start, _ = _trs((0, 0, 0), (0, 1, 0), 0, (1, 1, 1))
+    end, _ = _trs((2, 0, 0), (0, 1, 0), 0, (1, 1, 1))
+    d.camera.has_motion = 1
+    d.camera.to_world_end[:] = end
+    ref = RefScene(d)
+    for t in (0.0, 0.25, 0.7, 1.0):
+        o, dr, _, rt = ref.camera_ray((0.3, 0.6), time=t, with_time=True)
+        assert np.allclose(o, (2 * t, 0, 0), atol=1e-6) and rt == np.float32(t)
+    d.camera.type = A.GND_CAMERA_THINLENS
+    d.camera.rolling_shutter = 1
+    d.camera.scan_duration, d.camera.exposure_duration = 0.9, 0.1
+    ref = RefScene(d)
+    for fy, t in ((0.2, 0.5), (0.9, 0.1)):
+        _, _, _, rt = ref.camera_ray((0.5, fy), time=t, with_time=True)
+        want = np.float32(np.float32(1 - fy) * np.float32(0.9)) + np.float32(t) * np.float32(0.1)
+        assert abs(rt - want) < 1e-6
+        o, _, _ = ref.camera_ray((0.5, fy), time=t)
+        assert np.allclose(o, (2 * rt, 0, 0), atol=1e-5)
