@@ -14,6 +14,8 @@ namespace gdev {
 
 #define GDEV_INLINE __device__ __forceinline__
 
+#include "motion.h"
+
 constexpr float kEps = 0.0001f;                     // constants.h:13
 constexpr float kPi = 3.14159265358979323846f;
 constexpr float kInvPi = 0.31830988618379067154f;
@@ -78,74 +80,11 @@ GDEV_INLINE V3 xfVector(const M4& M, V3 v) {
               (m[1] * v.x + m[5] * v.y) + (m[9] * v.z + m[13] * 0.0f),
               (m[2] * v.x + m[6] * v.y) + (m[10] * v.z + m[14] * 0.0f)};
 }
-// AnimatedTransform (src/math/animated.cpp): the two keyframes of a moving
-// transform decomposed (glm::decompose, animated.cpp:24-28) into
-// translation, rotation quaternion (x, y, z, w) and scale
-struct DMotion {
-    float t0[3], t1[3], s0[3], s1[3], r0[4], r1[4];
-};
-// glm::mix(x, y, a) = x * (1 - a) + y * a
-GDEV_INLINE float gmix(float x, float y, float a) { return x * (1.0f - a) + y * a; }
-// glm mat4 * mat4: column i = ((A0 * B[i][0] + A1 * B[i][1]) + A2 * B[i][2]) + A3 * B[i][3]
-GDEV_INLINE M4 mul44(const M4& A, const M4& B) {
-    M4 R;
-    for (int i = 0; i < 4; ++i)
-        for (int r = 0; r < 4; ++r)
-            R.m[i * 4 + r] = ((A.m[0 * 4 + r] * B.m[i * 4 + 0] + A.m[1 * 4 + r] * B.m[i * 4 + 1]) +
-                              A.m[2 * 4 + r] * B.m[i * 4 + 2]) +
-                             A.m[3 * 4 + r] * B.m[i * 4 + 3];
-    return R;
-}
-// AnimatedTransform::interpolate (animated.cpp:30-47) at dt strictly inside
-// (0, 1): mix the translations and scales, glm::slerp the rotations, then
-// translate(I, T) * mat4_cast(R) * scale(I, S) in glm's evaluation order
-GDEV_INLINE M4 motionMatrix(const DMotion& mo, float dt) {
-    float T[3], S[3], q[4];
-    for (int k = 0; k < 3; ++k) {
-        T[k] = gmix(mo.t0[k], mo.t1[k], dt);
-        S[k] = gmix(mo.s0[k], mo.s1[k], dt);
-    }
-    // glm::slerp (gtc/quaternion.inl); quaternion dot = (w w' + x x') + (y y' + z z')
-    const float* x = mo.r0;
-    float z[4] = {mo.r1[0], mo.r1[1], mo.r1[2], mo.r1[3]};
-    float cosT = (x[3] * z[3] + x[0] * z[0]) + (x[1] * z[1] + x[2] * z[2]);
-    if (cosT < 0.0f) {
-        for (int k = 0; k < 4; ++k) z[k] = -z[k];
-        cosT = -cosT;
-    }
-    if (cosT > 1.0f - 1.1920929e-7f) {  // 1 - epsilon<float>: linear mix
-        for (int k = 0; k < 4; ++k) q[k] = gmix(x[k], z[k], dt);
-    } else {
-        const float angle = acosf(cosT);
-        const float a = sinf((1.0f - dt) * angle), b = sinf(dt * angle), sn = sinf(angle);
-        for (int k = 0; k < 4; ++k) q[k] = (a * x[k] + b * z[k]) / sn;
-    }
-    M4 Tm, Rm, Sm;
-    for (int k = 0; k < 16; ++k) Tm.m[k] = Sm.m[k] = (k % 5 == 0) ? 1.0f : 0.0f;
-    // glm::translate(I, v): column 3 = ((I0 v0 + I1 v1) + I2 v2) + I3
-    for (int r = 0; r < 4; ++r) {
-        const float id0 = r == 0 ? 1.0f : 0.0f, id1 = r == 1 ? 1.0f : 0.0f, id2 = r == 2 ? 1.0f : 0.0f,
-                    id3 = r == 3 ? 1.0f : 0.0f;
-        Tm.m[12 + r] = ((id0 * T[0] + id1 * T[1]) + id2 * T[2]) + id3;
-    }
-    // glm::scale(I, v): column k = I_k * v_k
-    for (int c = 0; c < 3; ++c)
-        for (int r = 0; r < 4; ++r) Sm.m[c * 4 + r] = ((c == r) ? 1.0f : 0.0f) * S[c];
-    // glm::mat3_cast into a mat4
-    const float qxx = q[0] * q[0], qyy = q[1] * q[1], qzz = q[2] * q[2];
-    const float qxz = q[0] * q[2], qxy = q[0] * q[1], qyz = q[1] * q[2];
-    const float qwx = q[3] * q[0], qwy = q[3] * q[1], qwz = q[3] * q[2];
-    for (int k = 0; k < 16; ++k) Rm.m[k] = (k % 5 == 0) ? 1.0f : 0.0f;
-    Rm.m[0] = 1.0f - 2.0f * (qyy + qzz);
-    Rm.m[1] = 2.0f * (qxy + qwz);
-    Rm.m[2] = 2.0f * (qxz - qwy);
-    Rm.m[4] = 2.0f * (qxy - qwz);
-    Rm.m[5] = 1.0f - 2.0f * (qxx + qzz);
-    Rm.m[6] = 2.0f * (qyz + qwx);
-    Rm.m[8] = 2.0f * (qxz + qwy);
-    Rm.m[9] = 2.0f * (qyz - qwx);
-    Rm.m[10] = 1.0f - 2.0f * (qxx + qyy);
-    return mul44(mul44(Tm, Rm), Sm);
+// AnimatedTransform at dt inside (0, 1) as an M4 (motion.h)
+GDEV_INLINE M4 motionMatrix(const GndMotion& mo, float dt) {
+    M4 M;
+    gndMotionMatrix(mo, dt, M.m);
+    return M;
 }
 // transpose(mat3(Minv)) * n (transform.h:58-61)
 GDEV_INLINE V3 xfNormal(const M4& Minv, V3 n) {

@@ -1078,7 +1078,7 @@ struct DCamera {
     int rolling;              // thin lens rolling shutter
     float scanDuration, exposureDuration;
     M4 toWorldEnd;
-    DMotion motion;
+    GndMotion motion;
 };
 
 // time: the sample's time draw (rolling shutter remaps it on the thin lens);

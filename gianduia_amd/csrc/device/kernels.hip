@@ -1600,75 +1600,6 @@ EnvTables envTables(const gnd_light_desc& l) {
     return t;
 }
 
-// glm::decompose (gtx/matrix_decompose.inl) of an affine column-major
-// matrix, as AnimatedTransform::decompose calls it (animated.cpp:24-28):
-// translation = column 3; scale and shear by Gram-Schmidt over columns 0..2
-// (lengths as sqrt((x x + y y) + z z), normalisation as division by the
-// length); a flip of handedness negates all three; the rotation quaternion
-// (x, y, z, w) from the orthonormal columns by the trace method.
-static void decomposeTRS(const float* m, float T[3], float Q[4], float S[3]) {
-    const float w = m[15];
-    float L[16];
-    for (int k = 0; k < 16; ++k) L[k] = m[k] / w;
-    T[0] = L[12];
-    T[1] = L[13];
-    T[2] = L[14];
-    float R[3][3];
-    for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) R[i][j] = L[i * 4 + j];
-    auto len = [](const float* v) { return std::sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]); };
-    auto dot3 = [](const float* a, const float* b) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; };
-    auto unit = [&](float* v) {
-        const float l = len(v);
-        for (int k = 0; k < 3; ++k) v[k] = v[k] * 1.0f / l;
-    };
-    auto combine = [](float* a, const float* b, float bs) {
-        for (int k = 0; k < 3; ++k) a[k] = a[k] * 1.0f + b[k] * bs;
-    };
-    float skew[3];
-    S[0] = len(R[0]);
-    unit(R[0]);
-    skew[2] = dot3(R[0], R[1]);
-    combine(R[1], R[0], -skew[2]);
-    S[1] = len(R[1]);
-    unit(R[1]);
-    skew[2] /= S[1];
-    skew[1] = dot3(R[0], R[2]);
-    combine(R[2], R[0], -skew[1]);
-    skew[0] = dot3(R[1], R[2]);
-    combine(R[2], R[1], -skew[0]);
-    S[2] = len(R[2]);
-    unit(R[2]);
-    const float c[3] = {R[1][1] * R[2][2] - R[2][1] * R[1][2], R[1][2] * R[2][0] - R[2][2] * R[1][0],
-                        R[1][0] * R[2][1] - R[2][0] * R[1][1]};
-    if (dot3(R[0], c) < 0.0f)
-        for (int i = 0; i < 3; ++i) {
-            S[i] *= -1.0f;
-            for (int k = 0; k < 3; ++k) R[i][k] *= -1.0f;
-        }
-    const float trace = R[0][0] + R[1][1] + R[2][2];
-    if (trace > 0.0f) {
-        float root = std::sqrt(trace + 1.0f);
-        Q[3] = 0.5f * root;
-        root = 0.5f / root;
-        Q[0] = root * (R[1][2] - R[2][1]);
-        Q[1] = root * (R[2][0] - R[0][2]);
-        Q[2] = root * (R[0][1] - R[1][0]);
-    } else {
-        static const int next[3] = {1, 2, 0};
-        int i = 0;
-        if (R[1][1] > R[0][0]) i = 1;
-        if (R[2][2] > R[i][i]) i = 2;
-        const int j = next[i], k = next[j];
-        float root = std::sqrt(R[i][i] - R[j][j] - R[k][k] + 1.0f);
-        Q[i] = 0.5f * root;
-        root = 0.5f / root;
-        Q[j] = root * (R[i][j] + R[j][i]);
-        Q[k] = root * (R[i][k] + R[k][i]);
-        Q[3] = root * (R[j][k] - R[k][j]);
-    }
-}
-
 DCamera makeCamera(const gnd_camera_desc& c) {
     DCamera d{};
     std::memcpy(d.toWorld.m, c.to_world, sizeof(d.toWorld.m));
@@ -1696,8 +1627,7 @@ DCamera makeCamera(const gnd_camera_desc& c) {
     std::memcpy(d.toWorldEnd.m, c.has_motion ? c.to_world_end : c.to_world, sizeof(d.toWorldEnd.m));
     d.animated = c.has_motion && std::memcmp(c.to_world, c.to_world_end, sizeof(c.to_world)) != 0 ? 1 : 0;
     if (d.animated) {
-        decomposeTRS(c.to_world, d.motion.t0, d.motion.r0, d.motion.s0);
-        decomposeTRS(c.to_world_end, d.motion.t1, d.motion.r1, d.motion.s1);
+        gndMotionFrom(c.to_world, c.to_world_end, d.motion);
     }
     d.rolling = c.type == GND_CAMERA_THINLENS && c.rolling_shutter ? 1 : 0;
     d.scanDuration = c.scan_duration;
