@@ -43,7 +43,8 @@ class gnd_medium_desc(C.Structure):
 class gnd_primitive_desc(C.Structure):
     _fields_ = [("shape_type", C.c_int32), ("mesh", C.c_int32), ("radius", C.c_float), ("material", C.c_int32),
                 ("has_emitter", C.c_int32), ("radiance", C.c_float * 3), ("to_world", C.c_float * 16),
-                ("to_world_inv", C.c_float * 16)]
+                ("to_world_inv", C.c_float * 16), ("has_motion", C.c_int32), ("to_world_end", C.c_float * 16),
+                ("to_world_end_inv", C.c_float * 16)]
 
 
 class gnd_camera_desc(C.Structure):

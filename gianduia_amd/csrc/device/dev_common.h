@@ -115,7 +115,8 @@ struct alignas(16) DInst {
     M4 worldToObject;
     int32_t root;   // absolute BLAS root node, -1 for an analytic sphere
     float radius;
-    int32_t pad0, pad1;
+    int32_t motion;  // index into DScene::motions, -1 when the primitive does not move
+    int32_t pad1;
 };
 
 // Shading view of a primitive
@@ -129,7 +130,14 @@ struct alignas(16) DPrim {
     uint32_t triBase, nTris, cdfBase, hasUV;
     float totalArea, radius;
     float radiance[3];
-    int32_t pad;
+    int32_t motion;  // index into DScene::motions, -1 when the primitive does not move
+};
+
+// A moving primitive (Primitive ctor, primitive.cpp:6-11): its keyframes and
+// their inverses as given, and the decomposition AnimatedTransform mixes
+struct alignas(16) DPrimMotion {
+    M4 start, startInv, end, endInv;
+    GndMotion mo;
 };
 
 struct alignas(16) DMaterial {
@@ -178,7 +186,8 @@ struct DScene {
     const DMaterial* mats;
     const DMedium* media;
     const int32_t* emitters;  // emitter list: primitive index (area) or -(light + 1)
-    int32_t nTlas, nEmitters, nPrims, hasIndex, hasGrid, pad;
+    int32_t nTlas, nEmitters, nPrims, hasIndex, hasGrid, hasMotion;
+    const DPrimMotion* motions;
     const DLight* lights;
     // environment map (envLight >= 0): RGB image, row-major, and its
     // Distribution2D: per-row conditional func / cdf / integral, marginal cdf
@@ -192,6 +201,22 @@ struct DScene {
     float envMargInt, envStrength;
     int32_t nLights, envLight, envW, envH;
 };
+
+// Primitive::getToWorld(time) / its inverse (primitive.cpp:48-51, 82-84;
+// AnimatedTransform::interpolate): the keyframes at or beyond the ends, else
+// the interpolated matrix and its glm::inverse
+GDEV_INLINE void motionAt(const DPrimMotion& m, float time, M4& M, M4& Minv) {
+    if (time <= 0.0f) {
+        M = m.start;
+        Minv = m.startInv;
+    } else if (time >= 1.0f) {
+        M = m.end;
+        Minv = m.endInv;
+    } else {
+        gndMotionMatrix(m.mo, (time - 0.0f) / (1.0f - 0.0f), M.m);
+        gndInverse44(M.m, Minv.m);
+    }
+}
 
 struct HitRec {
     float t, u, v;

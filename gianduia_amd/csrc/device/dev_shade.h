@@ -471,10 +471,20 @@ GDEV_INLINE V3 dpduFallback(V3 n) {  // mesh.cpp:370-373
     return divs(V3{0.0f, n.z, -n.y}, __builtin_sqrtf(n.y * n.y + n.z * n.z));
 }
 
+// The primitive's object-to-world matrix and inverse at a ray time
+// (Primitive::getToWorld, primitive.cpp:82-84)
+GDEV_INLINE void primXf(const DScene& S, const DPrim& pr, float time, M4& W, M4& Wi) {
+    W = pr.toWorld;
+    Wi = pr.toWorldInv;
+    if (pr.motion >= 0) motionAt(S.motions[pr.motion], time, W, Wi);
+}
+
 // Mesh::fillInteraction / Sphere::fillInteraction + Primitive::fillInteraction
-// (mesh.cpp:328-390, sphere.cpp:41-55, primitive.cpp:61-72)
-GDEV_INLINE void fillInteraction(const DScene& S, V3 ro, V3 rd, const HitRec& h, Isect& it) {
+// (mesh.cpp:328-390, sphere.cpp:41-55, primitive.cpp:61-72); time: the ray's
+GDEV_INLINE void fillInteraction(const DScene& S, V3 ro, V3 rd, const HitRec& h, Isect& it, float time) {
     const DPrim& pr = S.prims[h.inst];
+    M4 W, Wi;
+    primXf(S, pr, time, W, Wi);
     V3 P, N, dpdu;
     float uu = h.u, vv = h.v;
     if (pr.shape == GND_SHAPE_MESH) {
@@ -501,8 +511,8 @@ GDEV_INLINE void fillInteraction(const DScene& S, V3 ro, V3 rd, const HitRec& h,
             dpdu = normalize(dpduFallback(N));
         }
     } else {
-        V3 lo = xfPoint(S.inst[h.inst].worldToObject, ro);
-        V3 ld = xfVector(S.inst[h.inst].worldToObject, rd);
+        V3 lo = xfPoint(Wi, ro);
+        V3 ld = xfVector(Wi, rd);
         P = lo + ld * h.t;
         N = normalize(P - V3{0, 0, 0});
         float phi = atan2f(N.x, N.z);
@@ -515,9 +525,9 @@ GDEV_INLINE void fillInteraction(const DScene& S, V3 ro, V3 rd, const HitRec& h,
     it.u = uu;
     it.v = vv;
     it.prim = h.inst;
-    it.p = xfPoint(pr.toWorld, P);
-    it.n = normalize(xfNormal(pr.toWorldInv, N));
-    it.dpdu = normalize(xfVector(pr.toWorld, dpdu));
+    it.p = xfPoint(W, P);
+    it.n = normalize(xfNormal(Wi, N));
+    it.dpdu = normalize(xfVector(W, dpdu));
 }
 
 GDEV_INLINE bool makeBsdf(const DScene& S, const Isect& it, Bsdf& b) {
@@ -554,9 +564,12 @@ struct LightSample {
     bool ok;  // pdf written (else the reference leaves it unset and Li is black)
 };
 
-// AreaLight::sample with Mesh::sampleSurface / Sphere::sampleSurface
-GDEV_INLINE void areaSample(const DScene& S, int primIdx, V3 refP, float sx, float sy, LightSample& ls) {
+// AreaLight::sample with Mesh::sampleSurface / Sphere::sampleSurface; time:
+// the reference interaction's (area.cpp:54)
+GDEV_INLINE void areaSample(const DScene& S, int primIdx, V3 refP, float sx, float sy, LightSample& ls, float time) {
     const DPrim& pr = S.prims[primIdx];
+    M4 W, Wi;
+    primXf(S, pr, time, W, Wi);
     V3 lp = V3{0, 0, 0}, ln = V3{0, 0, 0};
     float areaPdf = 0.0f;
     if (pr.shape == GND_SHAPE_MESH) {
@@ -588,10 +601,10 @@ GDEV_INLINE void areaSample(const DScene& S, int primIdx, V3 refP, float sx, flo
         ln = normalize(lp - V3{0, 0, 0});
         areaPdf = kInv4Pi / (pr.radius * pr.radius);
     }
-    ls.p = xfPoint(pr.toWorld, lp);
-    ls.n = normalize(xfNormal(pr.toWorldInv, ln));
+    ls.p = xfPoint(W, lp);
+    ls.n = normalize(xfNormal(Wi, ln));
     Frame fr = frameFromNormal(ln);
-    V3 t1 = xfVector(pr.toWorld, fr.x), t2 = xfVector(pr.toWorld, fr.y);
+    V3 t1 = xfVector(W, fr.x), t2 = xfVector(W, fr.y);
     float areaScale = length(cross(t1, t2));
     if (areaScale > 0.0f) areaPdf /= areaScale;
     ls.dir = normalize(ls.p - refP);
@@ -606,14 +619,16 @@ GDEV_INLINE void areaSample(const DScene& S, int primIdx, V3 refP, float sx, flo
     ls.Li = cdiv(areaEval(pr, ls.n, normalize(refP - ls.p)), ls.pdf);
 }
 
-// AreaLight::pdf (area.cpp:90-119)
-GDEV_INLINE float areaPdf(const DScene& S, int primIdx, V3 refP, V3 p, V3 n) {
+// AreaLight::pdf (area.cpp:90-119); time: the reference interaction's (area.cpp:92)
+GDEV_INLINE float areaPdf(const DScene& S, int primIdx, V3 refP, V3 p, V3 n, float time) {
     const DPrim& pr = S.prims[primIdx];
-    V3 ln = xfNormal(pr.toWorld, n);
+    M4 W, Wi;
+    primXf(S, pr, time, W, Wi);
+    V3 ln = xfNormal(W, n);
     float ap = pr.shape == GND_SHAPE_MESH ? (pr.totalArea > 0.0f ? 1.0f / pr.totalArea : 0.0f)
                                           : kInv4Pi / (pr.radius * pr.radius);
     Frame fr = frameFromNormal(ln);
-    V3 t1 = xfVector(pr.toWorld, fr.x), t2 = xfVector(pr.toWorld, fr.y);
+    V3 t1 = xfVector(W, fr.x), t2 = xfVector(W, fr.y);
     float areaScale = length(cross(t1, t2));
     if (areaScale > 0.0f) ap /= areaScale;
     float dist = lengthSq(refP - p);
@@ -785,10 +800,11 @@ GDEV_INLINE void lightSample(const DScene& S, int k, V3 refP, float ux, float uy
 }
 // the NEE sample of emitter-list entry e (area emitter primitive, or
 // -(light + 1)); delta = Emitter::isDelta
-GDEV_INLINE void emitterSample(const DScene& S, int e, V3 refP, float ux, float uy, LightSample& ls, bool& delta) {
+GDEV_INLINE void emitterSample(const DScene& S, int e, V3 refP, float ux, float uy, LightSample& ls, bool& delta,
+                               float time) {
     if (e >= 0) {
         delta = false;
-        areaSample(S, e, refP, ux, uy, ls);
+        areaSample(S, e, refP, ux, uy, ls, time);
         return;
     }
     const int k = -e - 1;

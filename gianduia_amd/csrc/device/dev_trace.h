@@ -254,11 +254,13 @@ GDEV_INLINE int pickAction(int act) {
 // idle, so a wave does not run at the pace of its slowest ray.  With COUNT,
 // cnt[0..3] accumulate node pops, pack tests, instance tests and loop
 // iterations.
-template <bool ANY, bool COUNT, int LDSD, class Fetch, class Finish>
+// MOTION: the scene has moving primitives (instances take the ray's time)
+template <bool ANY, bool COUNT, int LDSD, bool MOTION = false, class Fetch, class Finish>
 GDEV_INLINE void traceStream(const DScene& S, TravStack<LDSD>& st, uint32_t* cnt, Fetch&& fetch,
                              Finish&& finish) {
     V3 o = V3{0.0f, 0.0f, 0.0f}, d = o, winv = o;  // world ray
     float tmin = 0.0f, tcur = 0.0f;  // tcur: the world ray's mutable tMax
+    float time = 0.0f;               // the ray's time (moving primitives)
     float best = kFltMax;            // isect.t, starts at FLT_MAX (bvh.cpp:39)
     V3 lo = o, ld = o, linv = o;     // object-space ray of the current instance
     float lt = 0.0f, hu = 0.0f, hv = 0.0f;
@@ -290,11 +292,12 @@ GDEV_INLINE void traceStream(const DScene& S, TravStack<LDSD>& st, uint32_t* cnt
         }
         if (!drained) {
             V3 ro, rd;
-            float rtmin, rtmax;
-            if (fetch(!busy, ro, rd, rtmin, rtmax, drained)) {
+            float rtmin, rtmax, rtime = 0.0f;
+            if (fetch(!busy, ro, rd, rtmin, rtmax, rtime, drained)) {
                 busy = true;
                 o = ro;
                 d = rd;
+                time = rtime;
                 winv = V3{1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z};
                 tmin = rtmin;
                 tcur = rtmax;
@@ -375,8 +378,16 @@ GDEV_INLINE void traceStream(const DScene& S, TravStack<LDSD>& st, uint32_t* cnt
             const int pi = S.tlasPrims[slot];
             const DInst& I = S.inst[pi];
             if (COUNT) cnt[2]++;
-            lo = xfPoint(I.worldToObject, o);
-            ld = xfVector(I.worldToObject, d);
+            if (!MOTION || I.motion < 0) {
+                lo = xfPoint(I.worldToObject, o);
+                ld = xfVector(I.worldToObject, d);
+            } else {
+                // Primitive::rayIntersect: interpolate(time).inverse() (primitive.cpp:48-50)
+                M4 W, Wi;
+                motionAt(S.motions[I.motion], time, W, Wi);
+                lo = xfPoint(Wi, o);
+                ld = xfVector(Wi, d);
+            }
             lt = tcur;
             curInst = pi;
             instHit = false;
@@ -406,22 +417,23 @@ GDEV_INLINE void traceStream(const DScene& S, TravStack<LDSD>& st, uint32_t* cnt
 
 // One ray per lane (all lanes of the wave call together; inactive lanes pass
 // active = false).  Returns whether a hit was found; `hit` holds the closest.
-template <bool ANY, bool COUNT, int LDSD>
-GDEV_INLINE bool traceRay(const DScene& S, V3 o, V3 d, float tmin, float tmax, HitRec& hit, TravStack<LDSD>& st,
-                          uint32_t* cnt, bool active = true) {
+template <bool ANY, bool COUNT, int LDSD, bool MOTION = false>
+GDEV_INLINE bool traceRay(const DScene& S, V3 o, V3 d, float tmin, float tmax, float time, HitRec& hit,
+                          TravStack<LDSD>& st, uint32_t* cnt, bool active = true) {
     bool result = false;
     hit.t = finf();
     hit.u = hit.v = 0.0f;
     hit.inst = hit.prim = -1;
-    traceStream<ANY, COUNT, LDSD>(
+    traceStream<ANY, COUNT, LDSD, MOTION>(
         S, st, cnt,
-        [&](bool idle, V3& ro, V3& rd, float& rtmin, float& rtmax, bool& drained) {
+        [&](bool idle, V3& ro, V3& rd, float& rtmin, float& rtmax, float& rtime, bool& drained) {
             drained = true;
             if (!idle || !active) return false;
             ro = o;
             rd = d;
             rtmin = tmin;
             rtmax = tmax;
+            rtime = time;
             return true;
         },
         [&](bool found, const HitRec& h, uint32_t) {

@@ -50,6 +50,7 @@ struct PathSoA {
     // event record between the NEE and scatter kernels (split mode)
     int32_t *evKind, *evPrim, *evMedium;
     float *evpx, *evpy, *evpz, *evnx, *evny, *evnz, *evtx, *evty, *evtz, *evu, *evv;
+    float* time;  // the camera sample's ray time (moving primitives)
     // AOV record per slot (allocated when a job asks for AOVs, else null):
     // field k of slot p at aov[k * aovStride + p], fields as in kAovFloats
     float* aov;
@@ -70,6 +71,7 @@ struct ShadowSoA {
     float *ox, *oy, *oz, *dx, *dy, *dz, *tmax;
     float *pr, *pg, *pb;   // pending contribution
     float *lpx, *lpy, *lpz;  // light point (transmittance segments)
+    float* time;             // the shadow ray's time (moving primitives)
 };
 
 struct DJob {
@@ -98,6 +100,7 @@ struct DSplatTask { // one job inside a wave: its chunks in pass order
 // flags layout
 constexpr uint32_t F_SPECULAR = 1u;
 constexpr uint32_t F_GBUF = 1u << 24;  // the path still needs its AOV albedo and normal
+constexpr uint32_t F_TIME0 = 1u << 25;  // the path's ray has time 0 (direct's BSDF ray, direct.cpp:60)
 GDEV_INLINE int flagChannel(uint32_t f) { return (int)((f >> 1) & 3u) - 1; }  // -1 none, 0..2
 GDEV_INLINE int flagMedium(uint32_t f) { return (int)((f >> 8) & 0xFFFFu) - 1; }
 GDEV_INLINE uint32_t setMedium(uint32_t f, int m) { return (f & ~(0xFFFFu << 8)) | ((uint32_t)(m + 1) << 8); }
@@ -254,7 +257,7 @@ GND_WAVE_KERNEL void kTraceBatch(DScene S, gnd_ray_soa R, int64_t n, gnd_hit_soa
     if (i < n) {
         HitRec h;
         bool r = traceRay<ANY, COUNT, LDSD>(S, V3{R.ox[i], R.oy[i], R.oz[i]}, V3{R.dx[i], R.dy[i], R.dz[i]},
-                                           R.tmin[i], R.tmax[i], h, st, cnt);
+                                           R.tmin[i], R.tmax[i], 0.0f, h, st, cnt);
         if (ANY) {
             occ[i] = r ? 1 : 0;
         } else {
@@ -344,8 +347,8 @@ __global__ void kCamera(WaveArgs A) {
     V3 ro, rd;
     float rayTime;
     const float w = shootRay(J.cam, fx, fy, lx, ly, lambda, time, ro, rd, rayTime);
-    (void)rayTime;  // static geometry: the path's time is not needed
     PathSoA& P = A.P;
+    P.time[slot] = rayTime;
     P.tpr[slot] = 1.0f; P.tpg[slot] = 1.0f; P.tpb[slot] = 1.0f;
     P.Lr[slot] = 0.0f; P.Lg[slot] = 0.0f; P.Lb[slot] = 0.0f;
     P.pdfPrev[slot] = 1.0f;
@@ -440,7 +443,7 @@ struct RayFeed {
 };
 
 // closest hit for the live path rays (Scene::rayIntersect at the top of Li)
-template <int LDSD, bool COUNT>
+template <int LDSD, bool COUNT, bool MOTION>
 GND_TRACE_KERNEL void kTracePaths(WaveArgs A) {
     extern __shared__ int sStack[];
     const uint32_t n = A.counts[0];
@@ -452,9 +455,9 @@ GND_TRACE_KERNEL void kTracePaths(WaveArgs A) {
     RayFeed feed;
     feed.init(A.counts + kFeedTrace, n, A.refill);
     int64_t cur = -1;  // queue index of this lane's ray
-    traceStream<false, COUNT, LDSD>(
+    traceStream<false, COUNT, LDSD, MOTION>(
         A.S, st, cnt,
-        [&](bool idle, V3& o, V3& d, float& tmin, float& tmax, bool& drained) {
+        [&](bool idle, V3& o, V3& d, float& tmin, float& tmax, float& time, bool& drained) {
             const int64_t i = feed.take(idle, waveMask, drained);
             if (i < 0) return false;
             cur = i;
@@ -462,6 +465,10 @@ GND_TRACE_KERNEL void kTracePaths(WaveArgs A) {
             if (COUNT && !finite3(d)) bad++;
             tmin = kEps;
             tmax = finf();
+            if (MOTION) {
+                const int q = A.qIn[i];
+                time = (A.P.flags[q] & F_TIME0) ? 0.0f : A.P.time[q];
+            }
             return true;
         },
         [&](bool found, const HitRec& h, uint32_t nodes) {
@@ -479,16 +486,20 @@ GND_TRACE_KERNEL void kTracePaths(WaveArgs A) {
 struct PathLocal {
     V3 o, d, tp, L, last;
     float pdfPrev;
+    float time;  // the camera sample's ray time
     Rng rng;
     uint32_t c0, c1, flags;
     int job;
 };
+// the time of the path's current ray
+GDEV_INLINE float rayTime(const PathLocal& s) { return (s.flags & F_TIME0) ? 0.0f : s.time; }
 
 GDEV_INLINE void loadPath(const PathSoA& P, int p, PathLocal& s) {
     s.tp = V3{P.tpr[p], P.tpg[p], P.tpb[p]};
     s.L = V3{P.Lr[p], P.Lg[p], P.Lb[p]};
     s.last = V3{P.lx[p], P.ly[p], P.lz[p]};
     s.pdfPrev = P.pdfPrev[p];
+    s.time = P.time[p];
     s.rng.s = P.rng[p];
     s.c0 = P.c0[p];
     s.c1 = P.c1[p];
@@ -647,14 +658,21 @@ GDEV_INLINE void aovSurface(const WaveArgs& A, const DJob& J, int p, PathLocal& 
     }
 }
 
+// A shadow ray's time: the reference interaction's, except a point light's,
+// whose shadow ray takes the time of the fresh light interaction, 0
+// (pointlight.cpp:29)
+GDEV_INLINE float shadowTime(const DScene& S, int e, float t) {
+    return (e < 0 && S.lights[-e - 1].type == GND_LIGHT_POINT) ? 0.0f : t;
+}
+
 // First half of a bounce: medium sampling (volpath), emission with MIS, null
 // BSDF pass-through and the NEE light sample with its shadow ray.
 // Returns: 0 = dead, 1 = continues straight (already a new ray), 2 = needs scatter.
 // pre: the medium was a density grid that kMedium already walked (throughput
 // and sampler state updated); preHit / preP = its collision, if any.
 GDEV_INLINE int neeStep(const WaveArgs& A, const DJob& J, int p, PathLocal& s, const HitRec& h, Event& ev,
-                        bool& shadow, V3& so, V3& sd, float& stmax, int& smedium, V3& pending, V3& lightP, bool pre,
-                        bool preHit, V3 preP) {
+                        bool& shadow, V3& so, V3& sd, float& stmax, int& smedium, V3& pending, V3& lightP,
+                        float& stime, bool pre, bool preHit, V3 preP) {
     const DScene& S = A.S;
     const bool vol = J.integrator == GND_INTEGRATOR_VOLPATH;
     const bool hit = h.inst >= 0;
@@ -679,7 +697,9 @@ GDEV_INLINE int neeStep(const WaveArgs& A, const DJob& J, int p, PathLocal& s, c
             const float ux = s.rng.next1D(), uy = s.rng.next1D();
             LightSample ls;
             bool delta;
-            emitterSample(S, ep, me.p, ux, uy, ls, delta);
+            // the light is sampled from a fresh SurfaceInteraction holding
+            // only the point (volPath.cpp:60-61): time 0
+            emitterSample(S, ep, me.p, ux, uy, ls, delta, 0.0f);
             const V3 Li = cdiv(ls.Li, lsp);
             if (ls.ok && ls.pdf > 1e-6f && !isBlack(Li)) {
                 const V3 wi = normalize(ls.p - me.p);
@@ -687,6 +707,7 @@ GDEV_INLINE int neeStep(const WaveArgs& A, const DJob& J, int p, PathLocal& s, c
                 pending = pp > 1e-6f ? s.tp * pp * Li * powerHeuristic(ls.pdf * lsp, delta ? 0.0f : pp) : sameV(0.0f);
                 shadow = true;
                 so = me.p; sd = ls.dir; stmax = ls.tmax; smedium = medium; lightP = ls.p;
+                stime = 0.0f;
             }
             ev.kind = EV_MEDIUM;
             ev.p = me.p;
@@ -716,7 +737,7 @@ GDEV_INLINE int neeStep(const WaveArgs& A, const DJob& J, int p, PathLocal& s, c
         return 0;
     }
     Isect it;
-    fillInteraction(S, s.o, s.d, h, it);
+    fillInteraction(S, s.o, s.d, h, it, rayTime(s));
     const DPrim& pr = S.prims[it.prim];
     Bsdf b;
     const bool hasBsdf = makeBsdf(S, it, b);
@@ -727,7 +748,8 @@ GDEV_INLINE int neeStep(const WaveArgs& A, const DJob& J, int p, PathLocal& s, c
             if (!isBlack(Le)) {
                 float w = 1.0f;
                 if (!(s.flags & F_SPECULAR)) {
-                    float lp = areaPdf(S, it.prim, s.last, it.p, it.n);
+                    // dummyRef holds only the point (path.cpp:76, volPath.cpp:161): time 0
+                    float lp = areaPdf(S, it.prim, s.last, it.p, it.n, 0.0f);
                     w = powerHeuristic(s.pdfPrev, lp * invNE);
                 }
                 s.L = s.L + s.tp * Le * w;
@@ -749,12 +771,13 @@ GDEV_INLINE int neeStep(const WaveArgs& A, const DJob& J, int p, PathLocal& s, c
     const float ux = s.rng.next1D(), uy = s.rng.next1D();
     LightSample ls;
     bool delta;
-    emitterSample(S, ep, it.p, ux, uy, ls, delta);
+    emitterSample(S, ep, it.p, ux, uy, ls, delta, rayTime(s));
     const V3 Li = cdiv(ls.Li, lsp);
     if (ls.ok && ls.pdf > 1e-6f && !isBlack(Li)) {
         const V3 wi = normalize(ls.p - it.p);
         const float bp = b.pdf(-s.d, wi);
         const V3 f = b.f(-s.d, wi);
+        stime = shadowTime(S, ep, rayTime(s));
         if (!isBlack(f) && bp > 1e-6f)
             pending = s.tp * f * Li * __builtin_fabsf(dot(it.n, wi)) * powerHeuristic(ls.pdf * lsp, delta ? 0.0f : bp);
         else pending = sameV(0.0f);
@@ -784,19 +807,19 @@ GDEV_INLINE int neeStep(const WaveArgs& A, const DJob& J, int p, PathLocal& s, c
 // here since each also carries a reflection or transmission bit).
 // Returns 0 = done, 1 = continues with the BSDF ray.
 GDEV_INLINE int directStep(const WaveArgs& A, PathLocal& s, const HitRec& h, bool& shadow, V3& so, V3& sd,
-                           float& stmax, V3& pending, V3& lightP) {
+                           float& stmax, V3& pending, V3& lightP, float& stime) {
     const DScene& S = A.S;
     shadow = false;
     if (h.inst < 0) return 0;  // no environment term in this integrator
     Isect it;
-    fillInteraction(S, s.o, s.d, h, it);
+    fillInteraction(S, s.o, s.d, h, it, rayTime(s));  // the BSDF ray has time 0 (direct.cpp:60)
     const DPrim& pr = S.prims[it.prim];
     const float lsp = 1.0f / (float)(uint32_t)S.nEmitters;
     if ((s.c0 & 0xFFFFu) > 0) {
         if (pr.emitter >= 0) {
             const V3 Lb = areaEval(pr, it.n, -s.d);
             if (!isBlack(Lb)) {
-                float lpg = areaPdf(S, it.prim, s.last, it.p, it.n);
+                float lpg = areaPdf(S, it.prim, s.last, it.p, it.n, s.time);  // pdf(isect, ...)
                 if (__builtin_isinf(lpg)) lpg = 0.0f;
                 const float pl = (s.flags & F_SPECULAR) ? 0.0f : lpg * lsp;
                 const float den = s.pdfPrev + pl;
@@ -812,9 +835,10 @@ GDEV_INLINE int directStep(const WaveArgs& A, PathLocal& s, const HitRec& h, boo
     const float ux = s.rng.next1D(), uy = s.rng.next1D();
     LightSample ls;
     bool delta;
-    emitterSample(S, ep, it.p, ux, uy, ls, delta);
+    emitterSample(S, ep, it.p, ux, uy, ls, delta, s.time);
     const V3 Li = cdiv(ls.Li, lsp);
     if (ls.ok && ls.pdf > 1e-6f && !isBlack(Li)) {
+        stime = shadowTime(S, ep, s.time);
         // the reference tests the shadow ray first and then weighs; the
         // weight draws no random numbers, so the ray is only cast when the
         // contribution is non-zero
@@ -847,6 +871,7 @@ GDEV_INLINE int directStep(const WaveArgs& A, PathLocal& s, const HitRec& h, boo
     s.o = it.p;
     s.d = wiW;
     s.c0 = 1u;
+    s.flags |= F_TIME0;
     return 1;
 }
 
@@ -856,14 +881,14 @@ GDEV_INLINE int directStep(const WaveArgs& A, PathLocal& s, const HitRec& h, boo
 // nothing is hit.  The occlusion ray goes through the shadow queue with a
 // contribution of 1.
 GDEV_INLINE void aoStep(const WaveArgs& A, const DJob& J, PathLocal& s, const HitRec& h, bool& shadow, V3& so,
-                        V3& sd, float& stmax, V3& pending, V3& lightP) {
+                        V3& sd, float& stmax, V3& pending, V3& lightP, float& stime) {
     shadow = false;
     if (h.inst < 0) {
         s.L = sameV(1.0f);
         return;
     }
     Isect it;
-    fillInteraction(A.S, s.o, s.d, h, it);
+    fillInteraction(A.S, s.o, s.d, h, it, rayTime(s));
     const float ux = s.rng.next1D(), uy = s.rng.next1D();
     const V3 ld = squareToCosineHemisphere(ux, uy);
     const V3 n = it.n;
@@ -873,6 +898,7 @@ GDEV_INLINE void aoStep(const WaveArgs& A, const DJob& J, PathLocal& s, const Hi
     sd = t1 * ld.x + t2 * ld.y + n * ld.z;
     so = it.p;
     stmax = J.aoLength;
+    stime = 0.0f;  // Ray(isect.p, dir, length) (ao.cpp:34)
     lightP = it.p;
     pending = sameV(1.0f);
     shadow = true;
@@ -965,7 +991,7 @@ __global__ __launch_bounds__(kShadeBlock, GND_SHADE_WAVES) void kShade(WaveArgs 
         int p = -1;
         PathLocal s;
         V3 so, sd, pending = sameV(0.0f), lightP;
-        float stmax = 0.0f;
+        float stmax = 0.0f, stime = 0.0f;
         int smedium = -1;
         if (active) {
             p = A.qIn[i];
@@ -989,12 +1015,13 @@ __global__ __launch_bounds__(kShadeBlock, GND_SHADE_WAVES) void kShade(WaveArgs 
             }
             int r;
             if (J.integrator == GND_INTEGRATOR_DIRECT) {
-                r = directStep(A, s, h, shadow, so, sd, stmax, pending, lightP);
+                r = directStep(A, s, h, shadow, so, sd, stmax, pending, lightP, stime);
             } else if (J.integrator == GND_INTEGRATOR_AO) {
-                aoStep(A, J, s, h, shadow, so, sd, stmax, pending, lightP);
+                aoStep(A, J, s, h, shadow, so, sd, stmax, pending, lightP, stime);
                 r = 0;
             } else {
-                r = neeStep(A, J, p, s, h, ev, shadow, so, sd, stmax, smedium, pending, lightP, pre, preHit, preP);
+                r = neeStep(A, J, p, s, h, ev, shadow, so, sd, stmax, smedium, pending, lightP, stime, pre, preHit,
+                            preP);
             }
             if (!A.pathOnlyFast && shadow) {
                 // the general shadow kernel always needs the ray; the any-hit
@@ -1039,6 +1066,7 @@ __global__ __launch_bounds__(kShadeBlock, GND_SHADE_WAVES) void kShade(WaveArgs 
             Sh.ox[ss] = so.x; Sh.oy[ss] = so.y; Sh.oz[ss] = so.z;
             Sh.dx[ss] = sd.x; Sh.dy[ss] = sd.y; Sh.dz[ss] = sd.z;
             Sh.tmax[ss] = stmax;
+            Sh.time[ss] = stime;
             Sh.pr[ss] = pending.x; Sh.pg[ss] = pending.y; Sh.pb[ss] = pending.z;
             Sh.lpx[ss] = lightP.x; Sh.lpy[ss] = lightP.y; Sh.lpz[ss] = lightP.z;
         }
@@ -1090,7 +1118,7 @@ __global__ __launch_bounds__(kShadeBlock) void kScatter(WaveArgs A) {
 
 // Shadow rays when no surface is a null BSDF and no segment can cross a
 // medium: one any-hit test each, persistent with refill like kTracePaths.
-template <int LDSD, bool COUNT>
+template <int LDSD, bool COUNT, bool MOTION>
 __global__ __launch_bounds__(256, GND_SHADOW_WAVES) void kShadowAny(WaveArgs A) {
     extern __shared__ int sStack[];
     const uint32_t n = A.counts[2];
@@ -1103,9 +1131,9 @@ __global__ __launch_bounds__(256, GND_SHADOW_WAVES) void kShadowAny(WaveArgs A) 
     RayFeed feed;
     feed.init(A.counts + kFeedShadow, n, A.srefill);
     int64_t cur = -1;
-    traceStream<true, COUNT, LDSD>(
+    traceStream<true, COUNT, LDSD, MOTION>(
         S, st, cnt,
-        [&](bool idle, V3& o, V3& d, float& tmin, float& tmax, bool& drained) {
+        [&](bool idle, V3& o, V3& d, float& tmin, float& tmax, float& time, bool& drained) {
             const int64_t i = feed.take(idle, waveMask, drained);
             if (i < 0) return false;
             cur = i;
@@ -1113,6 +1141,7 @@ __global__ __launch_bounds__(256, GND_SHADOW_WAVES) void kShadowAny(WaveArgs A) 
             d = V3{Sh.dx[i], Sh.dy[i], Sh.dz[i]};
             tmin = kEps;
             tmax = Sh.tmax[i];
+            if (MOTION) time = Sh.time[i];
             return true;
         },
         [&](bool occluded, const HitRec&, uint32_t) {
@@ -1133,7 +1162,7 @@ __global__ __launch_bounds__(256, GND_SHADOW_WAVES) void kShadowAny(WaveArgs A) 
 // time) and takes the next ray when it ends.  An iteration either traverses
 // or steps the walks, whichever more lanes wait for: a traversal costs many
 // walk steps, so doing both every iteration would leave the walks crawling.
-template <int LDSD, bool COUNT>
+template <int LDSD, bool COUNT, bool MOTION>
 __global__ __launch_bounds__(256, GND_SHADOWTR_WAVES) void kShadow(WaveArgs A) {
     extern __shared__ int sStack[];
     const uint32_t n = A.counts[2];
@@ -1151,6 +1180,7 @@ __global__ __launch_bounds__(256, GND_SHADOWTR_WAVES) void kShadow(WaveArgs A) {
     int64_t cur = -1;
     int phase = kIdle, medium = -1, iter = 0, p = -1;
     bool vol = false, opaque = false;  // opaque: direct / ao, any surface occludes
+    float stime = 0.0f;                // the shadow ray's time (constant along its segments)
     V3 o = sameV(0.0f), d = sameV(0.0f), Tr = sameV(1.0f);
     float tmax = 0.0f;
     Rng rng;
@@ -1169,6 +1199,7 @@ __global__ __launch_bounds__(256, GND_SHADOWTR_WAVES) void kShadow(WaveArgs A) {
                 o = V3{Sh.ox[i], Sh.oy[i], Sh.oz[i]};
                 d = V3{Sh.dx[i], Sh.dy[i], Sh.dz[i]};
                 tmax = Sh.tmax[i];
+                stime = S.hasMotion ? Sh.time[i] : 0.0f;
                 {
                     const int32_t integ = A.jobs[A.P.job[p]].integrator;
                     vol = integ == GND_INTEGRATOR_VOLPATH;
@@ -1192,12 +1223,12 @@ __global__ __launch_bounds__(256, GND_SHADOWTR_WAVES) void kShadow(WaveArgs A) {
         bool segDone = false;
         if (tracing && phase == kTrace) {
             HitRec h;
-            const bool hit = traceRay<false, COUNT, LDSD>(S, o, d, kEps, tmax, h, st, cnt);
+            const bool hit = traceRay<false, COUNT, LDSD, MOTION>(S, o, d, kEps, tmax, stime, h, st, cnt);
             if (!hit) {
                 next = kVisible;
             } else {
                 Isect it;
-                fillInteraction(S, o, d, h, it);
+                fillInteraction(S, o, d, h, it, stime);
                 const DMaterial& m = S.mats[S.prims[it.prim].material];
                 if (m.type != GND_MAT_INDEX || opaque) {
                     next = kOccluded;
@@ -1401,13 +1432,13 @@ struct gnd_hip_scene {
     DScene view{};
     std::vector<void*> allocations;
     size_t bytes = 0;
-    int hasIndex = 0, hasGrid = 0, nMedia = 0;
+    int hasIndex = 0, hasGrid = 0, nMedia = 0, hasMotion = 0;
     uint32_t nPrims = 0;
     size_t accelBytes = 0;  // TLAS, instances, BLAS and triangle packs
 };
 
 // Path state, hit and shadow queues and queue counters for `capacity` paths
-// in flight: 280 bytes per path slot in 69 SoA arrays, each 256-B aligned.
+// in flight: 288 bytes per path slot in 71 SoA arrays, each 256-B aligned.
 struct WaveBuffers {
     size_t capacity = 0;
     void* pathBuf = nullptr;
@@ -1646,7 +1677,7 @@ void ensureDev(T*& p, size_t& cap, size_t need) {
 
 size_t waveBufferBytes(size_t cap) {
     const size_t per = ((cap * 4 + 255) / 256) * 256;
-    return 68 * per + ((cap * 8 + 255) / 256) * 256;
+    return 70 * per + ((cap * 8 + 255) / 256) * 256;
 }
 
 void freeWaveBuffers(WaveBuffers& wb) {
@@ -1666,7 +1697,7 @@ void ensureWaveBuffers(WaveBuffers& wb, size_t cap) {
     wb.P.aov = nullptr;
     wb.P.aovStride = 0;
     wb.capacity = 0;
-    // 68 four-byte fields and one eight-byte field per slot
+    // 70 four-byte fields and one eight-byte field per slot
     const size_t bytes = waveBufferBytes(cap);
     GND_CHECK(hipMalloc(&wb.pathBuf, bytes));
     char* b = static_cast<char*>(wb.pathBuf);
@@ -1680,7 +1711,7 @@ void ensureWaveBuffers(WaveBuffers& wb, size_t cap) {
     P.rng = (uint64_t*)take(8);
     float** f[] = {&P.tpr, &P.tpg, &P.tpb, &P.Lr, &P.Lg, &P.Lb,
                    &P.pdfPrev, &P.lx, &P.ly, &P.lz, &P.weight, &P.pfx, &P.pfy, &P.evpx, &P.evpy, &P.evpz,
-                   &P.evnx, &P.evny, &P.evnz, &P.evtx, &P.evty, &P.evtz, &P.evu, &P.evv};
+                   &P.evnx, &P.evny, &P.evnz, &P.evtx, &P.evty, &P.evtz, &P.evu, &P.evv, &P.time};
     for (auto pp : f) *pp = (float*)take(4);
     P.c0 = (uint32_t*)take(4);
     P.c1 = (uint32_t*)take(4);
@@ -1698,7 +1729,8 @@ void ensureWaveBuffers(WaveBuffers& wb, size_t cap) {
     ShadowSoA& S = wb.Sh;
     S.path = (int32_t*)take(4);
     S.medium = (int32_t*)take(4);
-    float** g[] = {&S.ox, &S.oy, &S.oz, &S.dx, &S.dy, &S.dz, &S.tmax, &S.pr, &S.pg, &S.pb, &S.lpx, &S.lpy, &S.lpz};
+    float** g[] = {&S.ox, &S.oy, &S.oz, &S.dx, &S.dy, &S.dz, &S.tmax, &S.pr, &S.pg, &S.pb, &S.lpx, &S.lpy, &S.lpz,
+                   &S.time};
     for (auto pp : g) *pp = (float*)take(4);
     wb.q[0] = (int32_t*)take(4);
     wb.q[1] = (int32_t*)take(4);
@@ -1733,23 +1765,29 @@ TraceKernel pickTrace(int lds) {
 int normLds(int lds) { return lds <= 8 ? 8 : (lds <= 16 ? 16 : (lds <= 24 ? 24 : 32)); }
 
 using WaveKernel = void (*)(WaveArgs);
-template <bool COUNT>
+template <bool COUNT, bool MOTION>
 WaveKernel pickTracePaths(int lds) {
     switch (lds) {
-        case 8: return kTracePaths<8, COUNT>;
-        case 24: return kTracePaths<24, COUNT>;
-        case 32: return kTracePaths<32, COUNT>;
-        default: return kTracePaths<16, COUNT>;
+        case 8: return kTracePaths<8, COUNT, MOTION>;
+        case 24: return kTracePaths<24, COUNT, MOTION>;
+        case 32: return kTracePaths<32, COUNT, MOTION>;
+        default: return kTracePaths<16, COUNT, MOTION>;
     }
 }
-template <bool FAST, bool COUNT>
+template <bool FAST, bool COUNT, bool MOTION>
 WaveKernel pickShadow(int lds) {
     switch (lds) {
-        case 8: return FAST ? kShadowAny<8, COUNT> : kShadow<8, COUNT>;
-        case 24: return FAST ? kShadowAny<24, COUNT> : kShadow<24, COUNT>;
-        case 32: return FAST ? kShadowAny<32, COUNT> : kShadow<32, COUNT>;
-        default: return FAST ? kShadowAny<16, COUNT> : kShadow<16, COUNT>;
+        case 8: return FAST ? kShadowAny<8, COUNT, MOTION> : kShadow<8, COUNT, MOTION>;
+        case 24: return FAST ? kShadowAny<24, COUNT, MOTION> : kShadow<24, COUNT, MOTION>;
+        case 32: return FAST ? kShadowAny<32, COUNT, MOTION> : kShadow<32, COUNT, MOTION>;
+        default: return FAST ? kShadowAny<16, COUNT, MOTION> : kShadow<16, COUNT, MOTION>;
     }
+}
+template <bool MOTION>
+void pickBounceKernels(bool fast, bool cnt, int lds, WaveKernel& trace, WaveKernel& shadow) {
+    trace = cnt ? pickTracePaths<true, MOTION>(lds) : pickTracePaths<false, MOTION>(lds);
+    shadow = fast ? (cnt ? pickShadow<true, true, MOTION>(lds) : pickShadow<true, false, MOTION>(lds))
+                  : (cnt ? pickShadow<false, true, MOTION>(lds) : pickShadow<false, false, MOTION>(lds));
 }
 
 float elapsedMs(hipEvent_t a, hipEvent_t b) {
@@ -1916,13 +1954,28 @@ int gnd_hip_scene_upload(gnd_hip_ctx* ctx, const gnd_hip_scene_arrays* a, gnd_hi
         if (d.nprims == 0) throw std::runtime_error("scene has no primitives");
         std::vector<DInst> inst(d.nprims);
         std::vector<DPrim> prims(d.nprims);
+        std::vector<DPrimMotion> motions;
         std::vector<int32_t> emitters;
         for (uint32_t i = 0; i < d.nprims; ++i) {
             const gnd_primitive_desc& p = d.prims[i];
             DInst& I = inst[i];
             std::memcpy(I.worldToObject.m, p.to_world_inv, sizeof(I.worldToObject.m));
             I.radius = p.radius;
-            I.pad0 = I.pad1 = 0;
+            I.motion = -1;
+            I.pad1 = 0;
+            // a moving primitive (AnimatedTransform: animated when the matrices differ)
+            int motion = -1;
+            if (p.has_motion && std::memcmp(p.to_world, p.to_world_end, sizeof(p.to_world)) != 0) {
+                DPrimMotion m;
+                std::memcpy(m.start.m, p.to_world, sizeof(m.start.m));
+                std::memcpy(m.startInv.m, p.to_world_inv, sizeof(m.startInv.m));
+                std::memcpy(m.end.m, p.to_world_end, sizeof(m.end.m));
+                std::memcpy(m.endInv.m, p.to_world_end_inv, sizeof(m.endInv.m));
+                gndMotionFrom(p.to_world, p.to_world_end, m.mo);
+                motion = (int)motions.size();
+                motions.push_back(m);
+            }
+            I.motion = motion;
             DPrim& P = prims[i];
             std::memset(&P, 0, sizeof(P));
             std::memcpy(P.toWorld.m, p.to_world, sizeof(P.toWorld.m));
@@ -1934,6 +1987,7 @@ int gnd_hip_scene_upload(gnd_hip_ctx* ctx, const gnd_hip_scene_arrays* a, gnd_hi
             P.radius = p.radius;
             P.radiance[0] = p.radiance[0]; P.radiance[1] = p.radiance[1]; P.radiance[2] = p.radiance[2];
             P.emitter = p.has_emitter ? 0 : -1;  // slot assigned below
+            P.motion = motion;
             if (p.shape_type == GND_SHAPE_MESH) {
                 if (p.mesh < 0 || (uint32_t)p.mesh >= a->n_meshes) throw std::runtime_error("bad mesh index");
                 const gnd_mesh_info& mi = a->meshes[p.mesh];
@@ -2069,6 +2123,9 @@ int gnd_hip_scene_upload(gnd_hip_ctx* ctx, const gnd_hip_scene_arrays* a, gnd_hi
         v.media = devAlloc<DMedium>(s, media.size(), media.data());
         v.emitters = devAlloc<int32_t>(s, emitters.size(), emitters.data());
         v.lights = devAlloc<DLight>(s, lights.size(), lights.data());
+        v.motions = motions.empty() ? nullptr : devAlloc<DPrimMotion>(s, motions.size(), motions.data());
+        v.hasMotion = motions.empty() ? 0 : 1;
+        s->hasMotion = v.hasMotion;
         v.nLights = (int32_t)d.nlights;
         v.envLight = envLight;
         if (envLight >= 0) {
@@ -2263,9 +2320,9 @@ static BouncePlan makeBouncePlan(const gnd_hip_ctx* ctx, const gnd_hip_scene* sc
     const int lds = normLds(ctx->tuning.stack_lds);
     K.shm = (size_t)K.B * lds * sizeof(int);
     const bool cnt = ctx->counters;
-    K.trace = cnt ? pickTracePaths<true>(lds) : pickTracePaths<false>(lds);
-    K.shadow = K.fastShadow ? (cnt ? pickShadow<true, true>(lds) : pickShadow<true, false>(lds))
-                            : (cnt ? pickShadow<false, true>(lds) : pickShadow<false, false>(lds));
+    // moving primitives: the kernels that give each ray its time
+    if (scene->hasMotion) pickBounceKernels<true>(K.fastShadow, cnt, lds, K.trace, K.shadow);
+    else pickBounceKernels<false>(K.fastShadow, cnt, lds, K.trace, K.shadow);
 #ifndef GND_GRID_WAVES_PER_CU
 #define GND_GRID_WAVES_PER_CU 64
 #endif

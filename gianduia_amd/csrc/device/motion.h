@@ -12,6 +12,8 @@
 //     (sin((1 - a) t) x + sin(a t) y) / sin(t), t = acos(dot)
 //   translate(I, T) * mat4_cast(R) * scale(I, S), mat4 products with columns
 //     ((A0 b0 + A1 b1) + A2 b2) + A3 b3
+//   inverse (Transform(M) keeps glm::inverse(M); detail::compute_inverse):
+//     cofactors, columns scaled by the alternating signs, then * 1 / det
 // Matrices are column-major float[16] (m[c * 4 + r]).
 #pragma once
 
@@ -138,4 +140,34 @@ GND_HD void gndMotionMatrix(const GndMotion& mo, float dt, float* out) {
     Rm[10] = 1.0f - 2.0f * (qxx + qyy);
     gndMul44(Tm, Rm, TR);
     gndMul44(TR, Sm, out);
+}
+
+// glm::inverse of a 4x4 matrix
+GND_HD void gndInverse44(const float* M, float* out) {
+#define GM(c, r) M[(c) * 4 + (r)]
+    const float c00 = GM(2, 2) * GM(3, 3) - GM(3, 2) * GM(2, 3), c02 = GM(1, 2) * GM(3, 3) - GM(3, 2) * GM(1, 3);
+    const float c03 = GM(1, 2) * GM(2, 3) - GM(2, 2) * GM(1, 3), c04 = GM(2, 1) * GM(3, 3) - GM(3, 1) * GM(2, 3);
+    const float c06 = GM(1, 1) * GM(3, 3) - GM(3, 1) * GM(1, 3), c07 = GM(1, 1) * GM(2, 3) - GM(2, 1) * GM(1, 3);
+    const float c08 = GM(2, 1) * GM(3, 2) - GM(3, 1) * GM(2, 2), c10 = GM(1, 1) * GM(3, 2) - GM(3, 1) * GM(1, 2);
+    const float c11 = GM(1, 1) * GM(2, 2) - GM(2, 1) * GM(1, 2), c12 = GM(2, 0) * GM(3, 3) - GM(3, 0) * GM(2, 3);
+    const float c14 = GM(1, 0) * GM(3, 3) - GM(3, 0) * GM(1, 3), c15 = GM(1, 0) * GM(2, 3) - GM(2, 0) * GM(1, 3);
+    const float c16 = GM(2, 0) * GM(3, 2) - GM(3, 0) * GM(2, 2), c18 = GM(1, 0) * GM(3, 2) - GM(3, 0) * GM(1, 2);
+    const float c19 = GM(1, 0) * GM(2, 2) - GM(2, 0) * GM(1, 2), c20 = GM(2, 0) * GM(3, 1) - GM(3, 0) * GM(2, 1);
+    const float c22 = GM(1, 0) * GM(3, 1) - GM(3, 0) * GM(1, 1), c23 = GM(1, 0) * GM(2, 1) - GM(2, 0) * GM(1, 1);
+    const float f0[4] = {c00, c00, c02, c03}, f1[4] = {c04, c04, c06, c07}, f2[4] = {c08, c08, c10, c11};
+    const float f3[4] = {c12, c12, c14, c15}, f4[4] = {c16, c16, c18, c19}, f5[4] = {c20, c20, c22, c23};
+    const float v0[4] = {GM(1, 0), GM(0, 0), GM(0, 0), GM(0, 0)}, v1[4] = {GM(1, 1), GM(0, 1), GM(0, 1), GM(0, 1)};
+    const float v2[4] = {GM(1, 2), GM(0, 2), GM(0, 2), GM(0, 2)}, v3[4] = {GM(1, 3), GM(0, 3), GM(0, 3), GM(0, 3)};
+    float inv[16];
+    for (int k = 0; k < 4; ++k) {
+        const float sa = (k % 2 == 0) ? 1.0f : -1.0f;
+        inv[0 * 4 + k] = ((v1[k] * f0[k] - v2[k] * f1[k]) + v3[k] * f2[k]) * sa;
+        inv[1 * 4 + k] = ((v0[k] * f0[k] - v2[k] * f3[k]) + v3[k] * f4[k]) * -sa;
+        inv[2 * 4 + k] = ((v0[k] * f1[k] - v1[k] * f3[k]) + v3[k] * f5[k]) * sa;
+        inv[3 * 4 + k] = ((v0[k] * f2[k] - v1[k] * f4[k]) + v2[k] * f5[k]) * -sa;
+    }
+    const float d0 = GM(0, 0) * inv[0], d1 = GM(0, 1) * inv[4], d2 = GM(0, 2) * inv[8], d3 = GM(0, 3) * inv[12];
+    const float one = 1.0f / ((d0 + d1) + (d2 + d3));
+    for (int k = 0; k < 16; ++k) out[k] = inv[k] * one;
+#undef GM
 }

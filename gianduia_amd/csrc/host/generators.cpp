@@ -215,8 +215,13 @@ std::unique_ptr<HostScene> cornellBox(const Params& p) {
     int cube = s->addMesh(cubeMesh());
     Xform shortBox = xformScale(0.6f, 0.6f, 0.6f).then(xformRotate(15.0f, 0, 1, 0)).then(xformTranslate(0.33f, -0.7f, 0.35f));
     Xform tallBox = xformScale(0.6f, 1.2f, 0.6f).then(xformRotate(-18.0f, 0, 1, 0)).then(xformTranslate(-0.35f, -0.4f, -0.3f));
-    s->addPrim(cube, shortBox, white);
+    const int sb = s->addPrim(cube, shortBox, white);
     s->addPrim(cube, tallBox, white);
+    // movebox=d: the short box slides d along x and turns 20 degrees about y
+    // over the shutter interval (motion blur)
+    if (const double mv = num(p, "movebox", 0.0); mv != 0.0)
+        s->setMotion(sb, xformScale(0.6f, 0.6f, 0.6f).then(xformRotate(35.0f, 0, 1, 0))
+                             .then(xformTranslate(0.33f + (float)mv, -0.7f, 0.35f)));
     const float Le[3] = {17.0f, 12.0f, 4.0f};
     const float h = 0.25f, y = 0.999f;
     s->addPrim(s->addMesh(quadMesh(Vec3(-h, y, -h), Vec3(h, y, -h), Vec3(h, y, h), Vec3(-h, y, h), Vec3(0, -1, 0))), I,
@@ -267,7 +272,16 @@ std::unique_ptr<HostScene> scaledScene(const Params& p) {
         Xform x = xformScale(sc, sc, sc)
                       .then(xformRotate(ang, axis.x, axis.y, axis.z))
                       .then(xformTranslate(rad * std::cos(phi), yy, rad * std::sin(phi)));
-        s->addPrim(small, x, s->addMaterial(randomMaterial()));
+        const int id = s->addPrim(small, x, s->addMaterial(randomMaterial()));
+        // instmove=d: every instance moves d along a random direction (motion blur)
+        if (const double mv = num(p, "instmove", 0.0); mv != 0.0) {
+            std::uniform_real_distribution<float> Um(-1.0f, 1.0f);
+            std::mt19937 mrng(seed * 7919u + (uint32_t)i);
+            Vec3 dir(Um(mrng), Um(mrng), Um(mrng));
+            if (length(dir) < 1e-3f) dir = Vec3(1, 0, 0);
+            dir = normalize(dir) * (float)mv;
+            s->setMotion(id, x.then(xformTranslate(dir.x, dir.y, dir.z)));
+        }
     }
     int floorMat = s->addMaterial(materialMatte(0.5f, 0.5f, 0.5f));
     const float F = 6.0f, fy = -1.25f;

@@ -2,6 +2,8 @@
 // Mesh::activate and BVH::build of the reference, host half).
 #include "scene_host.h"
 
+#include "../device/motion.h"
+
 #include <atomic>
 #include <chrono>
 #include <cstring>
@@ -83,6 +85,16 @@ int HostScene::addMaterial(const gnd_material_desc& m) {
 static void setXform(gnd_primitive_desc& p, const Xform& x) {
     std::memcpy(p.to_world, x.m.m, sizeof(p.to_world));
     std::memcpy(p.to_world_inv, x.inv.m, sizeof(p.to_world_inv));
+    std::memcpy(p.to_world_end, x.m.m, sizeof(p.to_world_end));
+    std::memcpy(p.to_world_end_inv, x.inv.m, sizeof(p.to_world_end_inv));
+}
+
+void HostScene::setMotion(int prim, const Xform& end) {
+    gnd_primitive_desc& p = prims.at((size_t)prim);
+    p.has_motion = 1;
+    std::memcpy(p.to_world_end, end.m.m, sizeof(p.to_world_end));
+    std::memcpy(p.to_world_end_inv, end.inv.m, sizeof(p.to_world_end_inv));
+    built_ = false;
 }
 
 int HostScene::addPrim(int mesh, const Xform& toWorld, int material, const float* radiance) {
@@ -402,6 +414,17 @@ const gnd_hip_scene_arrays& HostScene::build(int threads, double* ms) {
             Vec3 c0 = xformPoint(M, sb.corner(0));
             wb.lo = wb.hi = c0;
             for (int c = 1; c < 8; ++c) wb.grow(xformPoint(M, sb.corner(c)));
+            if (p.has_motion && std::memcmp(p.to_world, p.to_world_end, sizeof(p.to_world)) != 0) {
+                // AnimatedTransform::motionBounds (animated.cpp:49-60): the
+                // box at time 0, at time 1 and at the interpolated time 0.5
+                GndMotion mo;
+                gndMotionFrom(p.to_world, p.to_world_end, mo);
+                Mat4 ends[2];
+                std::memcpy(ends[0].m, p.to_world_end, sizeof(ends[0].m));
+                gndMotionMatrix(mo, 0.5f, ends[1].m);
+                for (const Mat4& E : ends)
+                    for (int c = 0; c < 8; ++c) wb.grow(xformPoint(E, sb.corner(c)));
+            }
             refs[i] = {wb, wb.lo * 0.5f + wb.hi * 0.5f, (int)i};
         }
         BinaryBVH tl = buildBinary(refs, nullptr);

@@ -52,6 +52,8 @@ public:
     int addMesh(MeshData m);
     int addMaterial(const gnd_material_desc& m);
     int addPrim(int mesh, const Xform& toWorld, int material, const float* radiance = nullptr);
+    // the primitive moves to `end` at time 1 (toWorldEnd, primitive.cpp:6-11)
+    void setMotion(int prim, const Xform& end);
     int addSphere(float radius, const Xform& toWorld, int material, const float* radiance = nullptr);
     // appends a light after every emitter declared so far (document order)
     void addLight(LightData l);

@@ -356,7 +356,7 @@ struct Loader {
         std::vector<const XNode*> kids;
         Props p = readProps(n, kids);
         Xform x = p.t.count("toWorld") ? p.t.at("toWorld") : Xform();
-        if (p.t.count("toWorldEnd")) throw std::runtime_error("animated transforms (toWorldEnd) are not supported");
+
         int mesh = -1, material = -1;
         bool sphere = false, emitter = false;
         float radius = 1.0f, Le[3] = {0, 0, 0};
@@ -402,8 +402,9 @@ struct Loader {
             m.albedo_is_checker = 1;  // missing texture matte fallback (primitive.cpp:39-49)
             material = sc.addMaterial(m);
         }
-        if (sphere) sc.addSphere(radius, x, material, emitter ? Le : nullptr);
-        else sc.addPrim(mesh, x, material, emitter ? Le : nullptr);
+        const int id = sphere ? sc.addSphere(radius, x, material, emitter ? Le : nullptr)
+                              : sc.addPrim(mesh, x, material, emitter ? Le : nullptr);
+        if (p.t.count("toWorldEnd")) sc.setMotion(id, p.t.at("toWorldEnd"));  // primitive.cpp:6-11
     }
 
     // A scene-level emitter (Scene::addChild, scene.cpp:33-40): point,

@@ -229,7 +229,7 @@ typedef struct gnd_hip_server gnd_hip_server;
 
 typedef struct gnd_hip_server_config {
     size_t hbm_budget;     /* bytes for path slots + resident scenes; 0 = 90% of HBM */
-    uint32_t path_slots;   /* paths in flight (280 B each); a pass needs W*H of them */
+    uint32_t path_slots;   /* paths in flight (288 B each); a pass needs W*H of them */
     uint32_t max_jobs;     /* jobs admitted at once                               */
 } gnd_hip_server_config;
 

@@ -104,6 +104,12 @@ typedef struct gnd_primitive_desc {
     float radiance[3];
     float to_world[16];
     float to_world_inv[16];
+    /* motion blur (Primitive ctor, primitive.cpp:6-11): with has_motion the
+     * primitive moves from to_world at time 0 to to_world_end at time 1
+     * (AnimatedTransform; rays carry the camera sample's time)              */
+    int32_t has_motion;
+    float to_world_end[16];
+    float to_world_end_inv[16];
 } gnd_primitive_desc;
 
 typedef struct gnd_camera_desc {

@@ -294,16 +294,6 @@ struct Medium {
     std::vector<float> majorants;
     M4 toLocal;
 };
-struct Prim {
-    int shapeType = GND_SHAPE_MESH;
-    int mesh = -1;
-    float radius = 1.0f;
-    int material = 0;
-    int emitter = -1;  // index into Scene::emitters
-    C3 radiance;
-    M4 toWorld, toWorldInv;
-    Bounds worldBounds;
-};
 // AnimatedTransform (src/math/animated.cpp, include/gianduia/math/animated.h):
 // keyframes at times 0 and 1, decomposed into translation, rotation
 // quaternion (x, y, z, w) and scale when the matrices differ
@@ -314,6 +304,23 @@ struct Motion {
 };
 Motion motionFromMatrices(const float* start, const float* end);
 M4 motionInterpolate(const Motion& mo, float time);
+// glm::inverse (Transform(M) keeps it, transform.h:20)
+M4 motionInverse(const M4& m);
+
+struct Prim {
+    int shapeType = GND_SHAPE_MESH;
+    int mesh = -1;
+    float radius = 1.0f;
+    int material = 0;
+    int emitter = -1;  // index into Scene::emitters
+    C3 radiance;
+    M4 toWorld, toWorldInv;
+    Motion motion;  // Primitive ctor (primitive.cpp:6-11): toWorld -> toWorldEnd
+    M4 endInv;      // toWorldEnd's inverse as given
+    Bounds worldBounds;
+};
+// Primitive::getToWorld(time) and its inverse (primitive.cpp:48-51, 82-84)
+void primAt(const Prim& p, float time, M4& W, M4& Wi);
 
 struct Camera {
     gnd_camera_desc d;

@@ -474,6 +474,8 @@ void sceneBuild(Scene& s, const gnd_scene_desc& d) {
         p.material = pd.material;
         p.toWorld = toM4(pd.to_world);
         p.toWorldInv = toM4(pd.to_world_inv);
+        p.motion = motionFromMatrices(pd.to_world, pd.has_motion ? pd.to_world_end : pd.to_world);
+        p.endInv = toM4(pd.has_motion ? pd.to_world_end_inv : pd.to_world_inv);
         p.radiance = C3(pd.radiance[0], pd.radiance[1], pd.radiance[2]);
         if (pd.has_emitter) p.emitter = 0;  // slot assigned below
         Bounds sb;
@@ -483,6 +485,12 @@ void sceneBuild(Scene& s, const gnd_scene_desc& d) {
         Bounds wb;
         wb.pMin = wb.pMax = xfPoint(p.toWorld, sb.corner(0));
         for (int c = 1; c < 8; ++c) wb = Union(wb, xfPoint(p.toWorld, sb.corner(c)));
+        if (p.motion.animated) {
+            // AnimatedTransform::motionBounds (animated.cpp:49-60): also the
+            // box at time 1 and at the interpolated time 0.5
+            for (const M4& E : {p.motion.end, motionInterpolate(p.motion, 0.5f)})
+                for (int c = 0; c < 8; ++c) wb = Union(wb, xfPoint(E, sb.corner(c)));
+        }
         p.worldBounds = wb;
     }
     // emitter list in document order: lights at their emitter_index, area
@@ -679,9 +687,11 @@ static bool sphereIntersect(float radius, const Ray& ray, Hit& h, bool predicate
 
 // Transform::operator()(Ray) (transform.h:63-67): tMin resets to Epsilon
 static Ray toObject(const Prim& p, const Ray& rw) {
+    M4 W, Wi;
+    primAt(p, rw.time, W, Wi);  // interpolate(time).inverse() (primitive.cpp:48-50)
     Ray r;
-    r.o = xfPoint(p.toWorldInv, rw.o);
-    r.d = xfVector(p.toWorldInv, rw.d);
+    r.o = xfPoint(Wi, rw.o);
+    r.d = xfVector(Wi, rw.d);
     r.tMax = rw.tMax;
     r.tMin = kEpsilon;
     return r;

@@ -166,6 +166,54 @@ M4 motionInterpolate(const Motion& mo, float time) {
     return product(product(Tm, Rm), Sm);
 }
 
+// glm::inverse (detail::compute_inverse<4, 4>): the cofactor form, columns
+// scaled by alternating signs, then times 1 / determinant
+M4 motionInverse(const M4& mat) {
+    const float* m = mat.m;
+    auto e = [&](int c, int r) { return m[c * 4 + r]; };
+    const float c00 = e(2, 2) * e(3, 3) - e(3, 2) * e(2, 3), c02 = e(1, 2) * e(3, 3) - e(3, 2) * e(1, 3);
+    const float c03 = e(1, 2) * e(2, 3) - e(2, 2) * e(1, 3), c04 = e(2, 1) * e(3, 3) - e(3, 1) * e(2, 3);
+    const float c06 = e(1, 1) * e(3, 3) - e(3, 1) * e(1, 3), c07 = e(1, 1) * e(2, 3) - e(2, 1) * e(1, 3);
+    const float c08 = e(2, 1) * e(3, 2) - e(3, 1) * e(2, 2), c10 = e(1, 1) * e(3, 2) - e(3, 1) * e(1, 2);
+    const float c11 = e(1, 1) * e(2, 2) - e(2, 1) * e(1, 2), c12 = e(2, 0) * e(3, 3) - e(3, 0) * e(2, 3);
+    const float c14 = e(1, 0) * e(3, 3) - e(3, 0) * e(1, 3), c15 = e(1, 0) * e(2, 3) - e(2, 0) * e(1, 3);
+    const float c16 = e(2, 0) * e(3, 2) - e(3, 0) * e(2, 2), c18 = e(1, 0) * e(3, 2) - e(3, 0) * e(1, 2);
+    const float c19 = e(1, 0) * e(2, 2) - e(2, 0) * e(1, 2), c20 = e(2, 0) * e(3, 1) - e(3, 0) * e(2, 1);
+    const float c22 = e(1, 0) * e(3, 1) - e(3, 0) * e(1, 1), c23 = e(1, 0) * e(2, 1) - e(2, 0) * e(1, 1);
+    const float fac[6][4] = {{c00, c00, c02, c03}, {c04, c04, c06, c07}, {c08, c08, c10, c11},
+                             {c12, c12, c14, c15}, {c16, c16, c18, c19}, {c20, c20, c22, c23}};
+    float vec[4][4];
+    for (int r = 0; r < 4; ++r) {
+        vec[r][0] = e(1, r);
+        vec[r][1] = vec[r][2] = vec[r][3] = e(0, r);
+    }
+    M4 inv;
+    for (int k = 0; k < 4; ++k) {
+        const float sign = (k % 2 == 0) ? 1.0f : -1.0f;
+        inv.m[0 * 4 + k] = ((vec[1][k] * fac[0][k] - vec[2][k] * fac[1][k]) + vec[3][k] * fac[2][k]) * sign;
+        inv.m[1 * 4 + k] = ((vec[0][k] * fac[0][k] - vec[2][k] * fac[3][k]) + vec[3][k] * fac[4][k]) * -sign;
+        inv.m[2 * 4 + k] = ((vec[0][k] * fac[1][k] - vec[1][k] * fac[3][k]) + vec[3][k] * fac[5][k]) * sign;
+        inv.m[3 * 4 + k] = ((vec[0][k] * fac[2][k] - vec[1][k] * fac[4][k]) + vec[2][k] * fac[5][k]) * -sign;
+    }
+    const float det = (e(0, 0) * inv.m[0] + e(0, 1) * inv.m[4]) + (e(0, 2) * inv.m[8] + e(0, 3) * inv.m[12]);
+    const float one = 1.0f / det;
+    for (float& v : inv.m) v = v * one;
+    return inv;
+}
+
+void primAt(const Prim& p, float time, M4& W, M4& Wi) {
+    if (!p.motion.animated || time <= 0.0f) {
+        W = p.toWorld;
+        Wi = p.toWorldInv;
+    } else if (time >= 1.0f) {
+        W = p.motion.end;
+        Wi = p.endInv;
+    } else {
+        W = motionInterpolate(p.motion, time);
+        Wi = motionInverse(W);
+    }
+}
+
 }  // namespace gref
 
 using namespace gref;

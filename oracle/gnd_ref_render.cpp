@@ -13,9 +13,11 @@ static inline float clampf(float v, float lo, float hi) { return v < lo ? lo : (
 
 // ------------------------------------------------------------ interaction
 static Ray toObjectRay(const Prim& p, const Ray& rw) {
+    M4 W, Wi;
+    primAt(p, rw.time, W, Wi);
     Ray r;
-    r.o = xfPoint(p.toWorldInv, rw.o);
-    r.d = xfVector(p.toWorldInv, rw.d);
+    r.o = xfPoint(Wi, rw.o);
+    r.d = xfVector(Wi, rw.d);
     r.tMax = rw.tMax;
     return r;
 }
@@ -72,9 +74,11 @@ void fillInteraction(const Scene& s, const Ray& rw, const Hit& h, Isect& it) {
         dpdu = V3((float)((double)(P.z * 2.0f) * M_PI), 0.0f, (float)((double)(-P.x * 2.0f) * M_PI));
     }
     it.uv = uv;
-    it.p = xfPoint(pr.toWorld, P);
-    it.n = Normalize(xfNormal(pr.toWorldInv, N));
-    it.dpdu = Normalize(xfVector(pr.toWorld, dpdu));
+    M4 W, Wi;
+    primAt(pr, rw.time, W, Wi);  // interpolate(rWorld.time) (primitive.cpp:61-72)
+    it.p = xfPoint(W, P);
+    it.n = Normalize(xfNormal(Wi, N));
+    it.dpdu = Normalize(xfVector(W, dpdu));
     const gnd_material_desc& md = s.materials[pr.material].d;
     it.hasInterface = false;
     it.mediumInside = it.mediumOutside = -1;
@@ -544,7 +548,9 @@ static C3 areaEval(const Prim& pr, V3 n, V3 w) {  // area.cpp:45-49
     return C3(0.0f);
 }
 // area.cpp:51-88 with Mesh::sampleSurface (mesh.cpp:396-434) / Sphere::sampleSurface (sphere.cpp:64-81)
-static C3 areaSample(const Scene& s, const Prim& pr, V3 refP, P2 u, LightSample& ls, bool& pdfSet) {
+static C3 areaSample(const Scene& s, const Prim& pr, V3 refP, P2 u, LightSample& ls, bool& pdfSet, float time) {
+    M4 W, Wi;
+    primAt(pr, time, W, Wi);  // getToWorld(ref.time) (area.cpp:54)
     pdfSet = false;
     V3 lp, ln;
     float areaPdf;
@@ -571,16 +577,17 @@ static C3 areaSample(const Scene& s, const Prim& pr, V3 refP, P2 u, LightSample&
         ln = Normalize(lp - V3(0, 0, 0));
         areaPdf = kInv4Pi / (pr.radius * pr.radius);
     }
-    ls.p = xfPoint(pr.toWorld, lp);
-    ls.n = Normalize(xfNormal(pr.toWorldInv, ln));
+    ls.p = xfPoint(W, lp);
+    ls.n = Normalize(xfNormal(Wi, ln));
     Frame fr = Frame::fromNormal(ln);
-    V3 t1 = xfVector(pr.toWorld, fr.x), t2 = xfVector(pr.toWorld, fr.y);
+    V3 t1 = xfVector(W, fr.x), t2 = xfVector(W, fr.y);
     float areaScale = Length(Cross(t1, t2));
     if (areaScale > 0.0f) areaPdf /= areaScale;
     ls.shadow.o = refP;
     ls.shadow.d = Normalize(ls.p - refP);
     ls.shadow.tMin = kEpsilon;
     ls.shadow.tMax = Length(ls.p - refP) - kEpsilon;
+    ls.shadow.time = time;  // area.cpp:75
     float dist = LengthSq(refP - ls.p);
     V3 wi = ls.shadow.d;
     float cosT = Dot(ls.n, -wi);
@@ -591,13 +598,15 @@ static C3 areaSample(const Scene& s, const Prim& pr, V3 refP, P2 u, LightSample&
     return areaEval(pr, ls.n, Normalize(refP - ls.p)) / ls.pdf;
 }
 // area.cpp:90-119
-static float areaPdfAt(const Scene& s, const Prim& pr, V3 refP, V3 infoP, V3 infoN) {
-    V3 ln = xfNormal(pr.toWorld, infoN);  // toLocal(n): transpose(mat3(toWorld.m)) * n
+static float areaPdfAt(const Scene& s, const Prim& pr, V3 refP, V3 infoP, V3 infoN, float time) {
+    M4 W, Wi;
+    primAt(pr, time, W, Wi);  // getToWorld(ref.time) (area.cpp:92)
+    V3 ln = xfNormal(W, infoN);  // toLocal(n): transpose(mat3(toWorld.m)) * n
     float areaPdf = pr.shapeType == GND_SHAPE_MESH
                         ? (s.meshes[pr.mesh].totalArea > 0.0f ? 1.0f / s.meshes[pr.mesh].totalArea : 0.0f)
                         : kInv4Pi / (pr.radius * pr.radius);
     Frame fr = Frame::fromNormal(ln);
-    V3 t1 = xfVector(pr.toWorld, fr.x), t2 = xfVector(pr.toWorld, fr.y);
+    V3 t1 = xfVector(W, fr.x), t2 = xfVector(W, fr.y);
     float areaScale = Length(Cross(t1, t2));
     if (areaScale > 0.0f) areaPdf /= areaScale;
     float dist = LengthSq(refP - infoP);
@@ -668,7 +677,7 @@ static float envPdf(const Light& L, V3 refP, V3 infoP) {
 }
 // Emitter::sample of the non-area emitters (pointlight.cpp:18-33,
 // spotlight.cpp:28-60, directional.cpp:17-33, environment.cpp:42-73)
-static C3 lightSample(const Light& L, V3 refP, P2 u, LightSample& ls) {
+static C3 lightSample(const Light& L, V3 refP, P2 u, LightSample& ls, float time) {
     switch (L.d.type) {
         case GND_LIGHT_POINT: {
             C3 I = L.power * kInv4Pi;
@@ -679,6 +688,7 @@ static C3 lightSample(const Light& L, V3 refP, P2 u, LightSample& ls) {
             ls.shadow.d = Normalize(ls.p - refP);
             ls.shadow.tMin = kEpsilon;
             ls.shadow.tMax = Length(ls.p - refP) - kEpsilon;
+            ls.shadow.time = 0.0f;  // info.time of a fresh interaction (pointlight.cpp:29)
             return I / squaredDist;
         }
         case GND_LIGHT_SPOT: {
@@ -691,6 +701,7 @@ static C3 lightSample(const Light& L, V3 refP, P2 u, LightSample& ls) {
             ls.shadow.d = wi;
             ls.shadow.tMin = kEpsilon;
             ls.shadow.tMax = dist - kEpsilon;
+            ls.shadow.time = time;  // spotlight.cpp:42
             float cosTheta = Dot(-wi, L.direction);
             if (cosTheta <= L.cosTotalWidth) return C3(0.0f);
             float falloff = 1.0f;
@@ -709,6 +720,7 @@ static C3 lightSample(const Light& L, V3 refP, P2 u, LightSample& ls) {
             ls.shadow.d = wi;
             ls.shadow.tMin = kEpsilon;
             ls.shadow.tMax = std::numeric_limits<float>::max();
+            ls.shadow.time = time;  // directional.cpp:31
             return L.irradiance;
         }
         default: {  // environment
@@ -726,6 +738,7 @@ static C3 lightSample(const Light& L, V3 refP, P2 u, LightSample& ls) {
             ls.shadow = Ray();
             ls.shadow.o = refP;
             ls.shadow.d = wi;
+            ls.shadow.time = time;  // environment.cpp:62
             float sinTheta = std::sin(theta);
             if (sinTheta <= 0.0f) {
                 ls.pdf = 0.0f;
@@ -738,15 +751,16 @@ static C3 lightSample(const Light& L, V3 refP, P2 u, LightSample& ls) {
 }
 // one NEE light sample from emitter-list entry `e` (area or other light);
 // `delta` is Emitter::isDelta
-static C3 emitterSample(const Scene& s, int e, V3 refP, P2 u, LightSample& ls, bool& pdfSet, bool& delta) {
+static C3 emitterSample(const Scene& s, int e, V3 refP, P2 u, LightSample& ls, bool& pdfSet, bool& delta,
+                        float time) {
     if (e >= 0) {
         delta = false;
-        return areaSample(s, s.prims[e], refP, u, ls, pdfSet);
+        return areaSample(s, s.prims[e], refP, u, ls, pdfSet, time);
     }
     const Light& L = s.lights[-e - 1];
     delta = L.isDelta();
     pdfSet = true;
-    return lightSample(L, refP, u, ls);
+    return lightSample(L, refP, u, ls, time);
 }
 // the environment seen by a ray that left the scene, with the MIS weight
 // against NEE (path.cpp:36-55, volPath.cpp:117-139); pdf against the origin
@@ -1119,7 +1133,7 @@ static C3 LiPath(const Scene& s, Ray r, Sampler& smp, Counters* cnt, AOV* aov) {
                 if (!Le.isBlack()) {
                     float w = 1.0f;
                     if (!specular) {
-                        float lp = areaPdfAt(s, pr, lastScatter, it.p, it.n);
+                        float lp = areaPdfAt(s, pr, lastScatter, it.p, it.n, 0.0f);  // dummyRef: time 0
                         w = powerHeuristic(pdfPrev, lp * invNE);
                     }
                     L = L + tp * Le * w;
@@ -1129,6 +1143,7 @@ static C3 LiPath(const Scene& s, Ray r, Sampler& smp, Counters* cnt, AOV* aov) {
         if (!hasBsdf) {
             if (nullBounces++ > 100) break;
             Ray nr;
+            nr.time = r.time;  // the ray's time carries on (path.cpp:90, 139, 176)
             nr.o = it.p;
             nr.d = r.d;
             r = nr;
@@ -1141,7 +1156,7 @@ static C3 LiPath(const Scene& s, Ray r, Sampler& smp, Counters* cnt, AOV* aov) {
         LightSample ls;
         bool pdfSet;
         bool delta;
-        C3 Li = emitterSample(s, ep, it.p, smp.next2D(), ls, pdfSet, delta);
+        C3 Li = emitterSample(s, ep, it.p, smp.next2D(), ls, pdfSet, delta, r.time);
         Li = Li / lsp;
         if (pdfSet && ls.pdf > 1e-6f && !Li.isBlack()) {
             bool occluded = false;
@@ -1155,6 +1170,7 @@ static C3 LiPath(const Scene& s, Ray r, Sampler& smp, Counters* cnt, AOV* aov) {
                 if (makeBSDF(s, sit, sb)) { occluded = true; break; }
                 float nextT = tr.tMax - Length(sit.p - tr.o);
                 Ray nr;
+                nr.time = tr.time;  // the ray's time carries on (path.cpp:90, 139, 176)
                 nr.o = sit.p;
                 nr.d = tr.d;
                 nr.tMax = nextT;
@@ -1186,6 +1202,7 @@ static C3 LiPath(const Scene& s, Ray r, Sampler& smp, Counters* cnt, AOV* aov) {
         }
         lastScatter = it.p;
         Ray nr;
+        nr.time = r.time;  // the ray's time carries on (path.cpp:90, 139, 176)
         nr.o = it.p;
         nr.d = wi;
         r = nr;
@@ -1212,6 +1229,7 @@ static C3 evaluateTr(const Scene& s, Ray r, Sampler& smp, V3 lightP, Counters* c
         BSDF b;
         if (makeBSDF(s, it, b)) return C3(0.0f);
         Ray nr;
+        nr.time = r.time;  // the ray's time carries on (path.cpp:90, 139, 176)
         nr.o = it.p;
         nr.d = r.d;
         nr.medium = Dot(r.d, it.n) > 0.0f ? it.mediumOutside : it.mediumInside;  // getMedium
@@ -1246,7 +1264,7 @@ static C3 LiDirect(const Scene& s, Ray r, Sampler& smp, Counters* cnt) {
     int ep = pickEmitter(s, smp.next1D());
     LightSample ls;
     bool pdfSet, delta;
-    C3 Li = emitterSample(s, ep, it.p, smp.next2D(), ls, pdfSet, delta);
+    C3 Li = emitterSample(s, ep, it.p, smp.next2D(), ls, pdfSet, delta, r.time);
     Li = Li / lightSelectPdf;
     if (pdfSet && ls.pdf > 1e-6f && !Li.isBlack()) {
         Ray sr = ls.shadow;
@@ -1280,7 +1298,7 @@ static C3 LiDirect(const Scene& s, Ray r, Sampler& smp, Counters* cnt) {
             if (p2.emitter >= 0) {
                 C3 Lb = areaEval(p2, it2.n, -wi);
                 if (!Lb.isBlack()) {
-                    float lightPdfGeo = areaPdfAt(s, p2, it.p, it2.p, it2.n);
+                    float lightPdfGeo = areaPdfAt(s, p2, it.p, it2.p, it2.n, r.time);  // pdf(isect, ...)
                     if (std::isinf(lightPdfGeo)) lightPdfGeo = 0.0f;
                     float pLight = st == BSDF_SPECULAR ? 0.0f : lightPdfGeo * lightSelectPdf;
                     float misDenom = bsdfPdf + pLight;
@@ -1333,7 +1351,8 @@ static C3 LiVolPath(const Scene& s, Ray r, Sampler& smp, Counters* cnt, AOV* aov
             LightSample ls;
             bool pdfSet;
             bool delta;
-            C3 Li = emitterSample(s, ep, mi.p, smp.next2D(), ls, pdfSet, delta);
+            // mi_ref holds only the point (volPath.cpp:60-61): time 0
+            C3 Li = emitterSample(s, ep, mi.p, smp.next2D(), ls, pdfSet, delta, 0.0f);
             Li = Li / lsp;
             if (pdfSet && ls.pdf > 1e-6f && !Li.isBlack()) {
                 Ray sr = ls.shadow;
@@ -1354,6 +1373,7 @@ static C3 LiVolPath(const Scene& s, Ray r, Sampler& smp, Counters* cnt, AOV* aov
             int prevMedium = r.medium;
             lastScatter = mi.p;
             Ray nr;
+            nr.time = r.time;  // the ray's time carries on (path.cpp:90, 139, 176)
             nr.o = mi.p;
             nr.d = wi;
             nr.medium = prevMedium;
@@ -1387,7 +1407,7 @@ static C3 LiVolPath(const Scene& s, Ray r, Sampler& smp, Counters* cnt, AOV* aov
                 if (!Le.isBlack()) {
                     float w = 1.0f;
                     if (!specular) {
-                        float lp = areaPdfAt(s, pr, lastScatter, it.p, it.n);
+                        float lp = areaPdfAt(s, pr, lastScatter, it.p, it.n, 0.0f);  // dummyRef: time 0
                         w = powerHeuristic(pdfPrev, lp * invNE);
                     }
                     L = L + tp * Le * w;
@@ -1397,6 +1417,7 @@ static C3 LiVolPath(const Scene& s, Ray r, Sampler& smp, Counters* cnt, AOV* aov
         if (!hasBsdf) {
             if (nullB++ > 100) break;
             Ray nr;
+            nr.time = r.time;  // the ray's time carries on (path.cpp:90, 139, 176)
             nr.o = it.p;
             nr.d = r.d;
             nr.medium = Dot(r.d, it.n) > 0.0f ? it.mediumOutside : it.mediumInside;
@@ -1409,7 +1430,7 @@ static C3 LiVolPath(const Scene& s, Ray r, Sampler& smp, Counters* cnt, AOV* aov
         LightSample ls;
         bool pdfSet;
         bool delta;
-        C3 Li = emitterSample(s, ep, it.p, smp.next2D(), ls, pdfSet, delta);
+        C3 Li = emitterSample(s, ep, it.p, smp.next2D(), ls, pdfSet, delta, r.time);
         Li = Li / lsp;
         if (pdfSet && ls.pdf > 1e-6f && !Li.isBlack()) {
             Ray sr = ls.shadow;
@@ -1436,6 +1457,7 @@ static C3 LiVolPath(const Scene& s, Ray r, Sampler& smp, Counters* cnt, AOV* aov
         V3 inc = -r.d;
         lastScatter = it.p;
         Ray nr;
+        nr.time = r.time;  // the ray's time carries on (path.cpp:90, 139, 176)
         nr.o = it.p;
         nr.d = wi;
         nr.medium = nextMedium(inc, wi, curMedium, it);
@@ -1823,7 +1845,7 @@ float gnd_ref_grid_sample(void* h, int medium, double x, double y, double z) {
 void gnd_ref_light_sample(void* h, int k, const float* refP, const float* u, float* out) {
     Scene* s = static_cast<Scene*>(h);
     LightSample ls;
-    C3 Li = lightSample(s->lights[k], V3(refP[0], refP[1], refP[2]), P2{u[0], u[1]}, ls);
+    C3 Li = lightSample(s->lights[k], V3(refP[0], refP[1], refP[2]), P2{u[0], u[1]}, ls, 0.0f);
     const float v[11] = {ls.p.x, ls.p.y, ls.p.z, ls.pdf, Li.r, Li.g, Li.b,
                          ls.shadow.d.x, ls.shadow.d.y, ls.shadow.d.z, ls.shadow.tMax};
     for (int i = 0; i < 11; ++i) out[i] = v[i];

@@ -817,3 +817,28 @@ def test_camera_motion_and_rolling_shutter_rays():
         assert abs(rt - want) < 1e-6
         o, _, _ = ref.camera_ray((0.5, fy), time=t)
         assert np.allclose(o, (2 * rt, 0, 0), atol=1e-5)
+
+
+@pytest.mark.parametrize("rot", [False, True])
+def test_moving_sphere_coverage(rot):
+    """A primitive with toWorldEnd (primitive.cpp:6-11) is intersected at the
+    ray's time: an emitting sphere of radius 0.5 sweeping from x = -2 to
+    x = +2 (optionally spinning, which must not change the coverage) crosses
+    the centre pixel's ray a quarter of the shutter interval, so with the
+    direct integrator (emission only: a black sphere and no other light) the
+    pixel converges to 0.25.  The TLAS must hold the motion bounds; bounds
+    of the start pose alone would lose the later hits."""
+    b = SceneBuilder()
+    blk = b.matte((0.0, 0.0, 0.0))
+    M0, M0i = trs(trans=(-2.0, 0.0, -5.0))
+    M1, M1i = trs(rot_deg=170.0 if rot else 0.0, axis=(0, 1, 0), trans=(2.0, 0.0, -5.0))
+    i = b.prim(-1, M0, M0i, material=blk, radiance=(1, 1, 1), sphere_radius=0.5)
+    b.prims[i].has_motion = 1
+    b.prims[i].to_world_end[:] = col_major(M1)
+    b.prims[i].to_world_end_inv[:] = col_major(M1i)
+    d = b.desc(width=9, height=9)
+    d.camera.fov = 1.0                       # the centre pixel sees a thin cone around -z
+    d.integrator = A.GND_INTEGRATOR_DIRECT
+    img, _ = RefScene(d).render(threads=8, spp=4096, width=9, height=9)
+    v = img[4, 4, 0] / img[4, 4, 3]
+    assert abs(v - 0.25) < 0.02, v
