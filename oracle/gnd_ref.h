@@ -28,7 +28,9 @@
  * SplitMix64 outputs, brute-force traversal, warp chi^2 / KS tests, BSDF
  * energy and pdf integrals, HG normalisation, media closed forms and the
  * integrator furnace.  Parity unpinned (no reference test or closed form
- * fixes them): the glm scalar evaluation orders above, the Highway lane
+ * fixes them): the glm scalar evaluation orders above, glm::decompose /
+ * slerp / mat4_cast / inverse of moving transforms (restated in
+ * gnd_ref_motion.cpp, pinned only by closed forms), the Highway lane
  * semantics, OpenVDB BoxSampler / worldToIndex on the dense grid, thin-lens
  * distortion, chromatic aberration, vignetting and polygonal aperture, the
  * Gaussian filter's shape, and the exact area-light sample positions.
