@@ -1850,6 +1850,16 @@ void gnd_ref_light_sample(void* h, int k, const float* refP, const float* u, flo
                          ls.shadow.d.x, ls.shadow.d.y, ls.shadow.d.z, ls.shadow.tMax};
     for (int i = 0; i < 11; ++i) out[i] = v[i];
 }
+// AreaLight::sample of primitive `prim` from refP at time 0: out p[3], n[3], pdf
+int gnd_ref_area_sample(void* h, int prim, const float* refP, const float* u, float* out) {
+    Scene* s = static_cast<Scene*>(h);
+    LightSample ls;
+    bool pdfSet = false;
+    areaSample(*s, s->prims[prim], V3(refP[0], refP[1], refP[2]), P2{u[0], u[1]}, ls, pdfSet, 0.0f);
+    const float v[7] = {ls.p.x, ls.p.y, ls.p.z, ls.n.x, ls.n.y, ls.n.z, pdfSet ? ls.pdf : 0.0f};
+    for (int i = 0; i < 7; ++i) out[i] = v[i];
+    return pdfSet ? 1 : 0;
+}
 // EnvironmentMap::pdf towards direction dir from the origin, and eval along it
 float gnd_ref_env_pdf(void* h, const float* dir, float* Le) {
     Scene* s = static_cast<Scene*>(h);

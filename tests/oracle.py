@@ -39,6 +39,7 @@ _API = {
     "gnd_ref_splitmix64": (C.c_uint64, [C.c_uint64]),
     "gnd_ref_sampler_floats": (None, [C.c_uint64, C.c_int, f32p]),
     "gnd_ref_denoise": (None, [f32p, f32p, f32p, C.c_int, C.c_int, f32p]),
+    "gnd_ref_area_sample": (C.c_int, [C.c_void_p, C.c_int, f32p, f32p, f32p]),
     "gnd_ref_fr_dielectric": (C.c_float, [C.c_float, C.c_float, C.c_float]),
     "gnd_ref_refract": (C.c_int, [f32p, f32p, C.c_float, f32p]),
     "gnd_ref_fr_conductor": (None, [C.c_float, f32p, f32p, f32p, f32p]),
@@ -213,6 +214,12 @@ class RefScene:
         if with_time:
             return out[:3], out[3:6], float(out[6]), float(out[7])
         return out[:3], out[3:6], float(out[6])
+
+    def area_sample(self, prim, ref_p, u):
+        """Area light of primitive `prim` sampled from ref_p: (p, n, pdf), pdf 0 when unset."""
+        out = np.zeros(7, np.float32)
+        lib().gnd_ref_area_sample(self.h, prim, fp(vec(*ref_p)), fp(vec(*u)), fp(out))
+        return out[0:3], out[3:6], float(out[6])
 
     def light_sample(self, k, ref_p, u):
         """Light k's NEE sample from ref_p: dict(p, pdf, Li, dir, tmax)."""

@@ -842,3 +842,58 @@ def test_moving_sphere_coverage(rot):
     img, _ = RefScene(d).render(threads=8, spp=4096, width=9, height=9)
     v = img[4, 4, 0] / img[4, 4, 3]
     assert abs(v - 0.25) < 0.02, v
+
+
+# ---------------------------------------------------------------- film filter and area-light sampling
+def test_gaussian_filter_weights_closed_form():
+    """At 1 spp each pixel's film weight is the sum, over the samples whose
+    filter footprint reaches it, of (exp(-a dx^2) - exp(-a r^2))^+ times the
+    same in y (GaussianFilter, filter.h:33-54, Film::addSample, film.h:26-69),
+    with every sample's film position known from the sampler."""
+    b = SceneBuilder()
+    b.prim(b.mesh(quad(-5, 5, -5, 5, -3.0)), material=b.matte(), radiance=(1, 1, 1))
+    W, H = 7, 5
+    d = b.desc(width=W, height=H)
+    d.camera.filter = A.GND_FILTER_GAUSSIAN
+    d.camera.filter_radius[:] = (2.0, 2.0)
+    d.camera.filter_alpha = 2.0
+    img, _ = RefScene(d).render(threads=2, spp=1, width=W, height=H)
+    a, r = 2.0, 2.0
+    g = lambda t: max(0.0, math.exp(-a * t * t) - math.exp(-a * r * r))
+    want = np.zeros((H, W))
+    for y in range(H):
+        for x in range(W):
+            u = np.zeros(2, np.float32)
+            O.lib().gnd_ref_sampler_floats(y * W + x, 2, fp(u))
+            px, py = x + float(u[0]), y + float(u[1])
+            for qy in range(H):
+                for qx in range(W):
+                    if abs(px - (qx + 0.5)) <= r and abs(py - (qy + 0.5)) <= r:
+                        want[qy, qx] += g(px - (qx + 0.5)) * g(py - (qy + 0.5))
+    assert np.allclose(img[..., 3], want, rtol=1e-4, atol=1e-6)
+
+
+def test_area_light_sampling_uniform_over_area():
+    """AreaLight::sample on a mesh emitter (area.cpp:50-88, mesh.cpp:396-438):
+    a triangle picked in proportion to its area from the area CDF, a point
+    uniform inside it; the pdf is dist^2 / (A cos).  Two triangles of areas 1
+    and 3 under a chi-square over a 4 x 4 grid of the 2 x 2 square."""
+    b = SceneBuilder()
+    # triangle A: (0,0)-(1,0)-(0,2) area 1; triangle B: (1,0)-(2,0)-(2,2) + (1,0)-(2,2)-(0,2) area 3 (z = 0, facing +z)
+    tris = np.array([[0, 0, 0, 1, 0, 0, 0, 2, 0], [1, 0, 0, 2, 0, 0, 2, 2, 0], [1, 0, 0, 2, 2, 0, 0, 2, 0]], np.float32)
+    b.prim(b.mesh(tris), material=b.matte(), radiance=(1, 1, 1))
+    ref = RefScene(b.desc(width=4, height=4))
+    rng = np.random.default_rng(3)
+    n = 32000
+    counts = np.zeros((4, 4))
+    refp = (1.0, 1.0, 3.0)
+    for _ in range(n):
+        p, nn, pdf = ref.area_sample(0, refp, rng.random(2).astype(np.float32))
+        assert abs(p[2]) < 1e-6 and np.allclose(nn, (0, 0, 1), atol=1e-6)
+        dv = np.array(refp) - p
+        want = (dv @ dv) / (4.0 * (dv[2] / np.linalg.norm(dv)))
+        assert abs(pdf - want) <= 1e-4 * want
+        counts[min(int(p[1] * 2), 3), min(int(p[0] * 2), 3)] += 1
+    expect = np.full((4, 4), n / 16)
+    chi2 = float(((counts - expect) ** 2 / expect).sum())
+    assert chi2 < sst.chi2.ppf(0.999, 15), chi2
