@@ -33,7 +33,8 @@
  * gnd_ref_motion.cpp, pinned only by closed forms), the Highway lane
  * semantics, OpenVDB BoxSampler / worldToIndex on the dense grid, thin-lens
  * distortion, chromatic aberration, vignetting and polygonal aperture, the
- * Gaussian filter's shape, and the exact area-light sample positions.
+ * exact area-light sample positions (their distribution and the Gaussian
+ * filter's weights are pinned by closed forms).
  */
 #pragma once
 
