@@ -31,10 +31,10 @@
  * fixes them): the glm scalar evaluation orders above, glm::decompose /
  * slerp / mat4_cast / inverse of moving transforms (restated in
  * gnd_ref_motion.cpp, pinned only by closed forms), the Highway lane
- * semantics, OpenVDB BoxSampler / worldToIndex on the dense grid, thin-lens
- * distortion, chromatic aberration, vignetting and polygonal aperture, the
- * exact area-light sample positions (their distribution and the Gaussian
- * filter's weights are pinned by closed forms).
+ * semantics, OpenVDB BoxSampler / worldToIndex on the dense grid, and the
+ * exact positions of area-light and polygonal-aperture samples (their
+ * distributions, the thin-lens effects and the Gaussian filter's weights are
+ * pinned by closed forms).
  */
 #pragma once
 

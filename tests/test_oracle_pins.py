@@ -897,3 +897,58 @@ def test_area_light_sampling_uniform_over_area():
     expect = np.full((4, 4), n / 16)
     chi2 = float(((counts - expect) ** 2 / expect).sum())
     assert chi2 < sst.chi2.ppf(0.999, 15), chi2
+
+
+# ---------------------------------------------------------------- thin-lens camera effects
+def _lens_camera(**kw):
+    b = SceneBuilder()
+    b.prim(b.mesh(quad(-5, 5, -5, 5, -3.0)), material=b.matte(), radiance=(1, 1, 1))
+    d = b.desc(width=40, height=20)
+    d.camera.type = A.GND_CAMERA_THINLENS
+    d.camera.fov = 50.0
+    for k, v in kw.items():
+        setattr(d.camera, k, v)
+    return RefScene(d), math.tan(math.radians(25.0)), 2.0
+
+
+def test_thin_lens_distortion_ca_vignetting_closed_form():
+    """ThinLensCamera::shootRay (thinLens.cpp:34-95) with a pinhole lens: the
+    barrel distortion scales the NDC by 1 + k1 r^2, lateral CA scales the
+    image plane by 1 + ca_lateral * lambda, vignetting weighs by cos^4, and
+    the direction is the normalised (x, y, -1) (identity camera transform)."""
+    ref, sc, aspect = _lens_camera(k1=0.2, ca_lateral=0.05, vignetting=1)
+    for (fx, fy), lam in (((0.8, 0.3), 1.0), ((0.1, 0.9), -1.0), ((0.5, 0.5), 0.0)):
+        o, dr, w = ref.camera_ray((fx, fy), lam=lam)
+        nx, ny = 2 * fx - 1, 2 * fy - 1
+        dist = 1 + 0.2 * (nx * nx + ny * ny)
+        s = sc * (1 + 0.05 * lam)
+        v = np.array([nx * dist * aspect * s, ny * dist * s, -1.0])
+        v /= np.linalg.norm(v)
+        assert np.allclose(o, 0, atol=1e-7) and np.allclose(dr, v, atol=2e-6)
+        assert abs(w - v[2] ** 4) < 2e-6
+
+
+@pytest.mark.parametrize("blades", [0, 6])
+def test_thin_lens_focus_and_aperture(blades):
+    """Rays through different lens points meet at the focal point of their
+    film position, at focal_distance * (1 + ca_axial * lambda) / |dir.z|
+    along the pinhole direction; lens points lie in the disk (blades < 3) or
+    the regular polygon of `blades` sides inscribed in it."""
+    R = 0.3
+    ref, sc, aspect = _lens_camera(lens_radius=R, focal_distance=4.0, ca_axial=0.1, blades=blades)
+    rng = np.random.default_rng(2)
+    for lam in (0.0, 1.0):
+        fx, fy = 0.7, 0.35
+        nx, ny = 2 * fx - 1, 2 * fy - 1
+        pin = np.array([nx * aspect * sc, ny * sc, -1.0])
+        focus = pin * (4.0 * (1 + 0.1 * lam)) / abs(pin[2])
+        for _ in range(20):
+            o, dr, _ = ref.camera_ray((fx, fy), plens=tuple(rng.random(2)), lam=lam)
+            t = (focus[2] - o[2]) / dr[2]
+            assert np.allclose(o + dr * t, focus, atol=2e-5)
+            r = math.hypot(o[0], o[1])
+            assert r <= R * (1 + 1e-6)
+            if blades >= 3:  # inside the polygon (a vertex at the top, warp.cpp:99-102): apothem bound
+                ang = (math.atan2(o[1], o[0]) - math.pi / 2) % (2 * math.pi / blades)
+                apothem = R * math.cos(math.pi / blades)
+                assert r * math.cos(ang - math.pi / blades) <= apothem * (1 + 1e-5)
