@@ -30,9 +30,10 @@
 // input by a power of two scales the output by exactly the same factor (an
 // HDR filter, as hdr = true asks of OIDN).
 //
-// Memory: one thread per pixel, float4 records; the 5x5 gather of a pass
-// reads the neighbours' 48-byte records from L2 / the Infinity Cache (the
-// whole 1080p working set is 130 MB).
+// Memory: one thread per pixel, float4 records.  An a-trous pass stages each
+// 16 x 16 tile of a sub-lattice (pixels `step` apart) with its halo in LDS, so
+// the 25 taps of a pixel come from LDS (the whole 1080p working set, 130 MB,
+// stays in the Infinity Cache between passes).
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -116,14 +117,37 @@ __global__ __launch_bounds__(256) void kDnVariance(const float4* __restrict__ Ei
     Eout[(size_t)y * W + x] = float4{c.x, c.y, c.z, var};
 }
 
+// One a-trous pass over the sub-lattice of pixels congruent to (ox, oy)
+// modulo `step`: a block takes a 16 x 16 tile of that lattice (pixels step
+// apart), stages the tile and its 2-point halo (20 x 20 records of radiance
+// and both guides, 19 KB) in LDS once, and every pixel then reads its 25 taps
+// from LDS instead of from L2.  The taps, their order and the arithmetic are
+// those of the per-pixel gather.
+constexpr int kDnTile = 16, kDnSpan = kDnTile + 4;
+
 __global__ __launch_bounds__(256) void kDnAtrous(const float4* __restrict__ Ein, const float4* __restrict__ G1,
                                                  const float4* __restrict__ G2, int W, int H, int step,
                                                  int useAlbedo, float4* __restrict__ Eout) {
-    const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    __shared__ float4 sE[kDnSpan * kDnSpan], sG[kDnSpan * kDnSpan], sN[kDnSpan * kDnSpan];
+    const int ox = (int)(blockIdx.x % (unsigned)step), tx = (int)(blockIdx.x / (unsigned)step);
+    const int oy = (int)(blockIdx.y % (unsigned)step), ty = (int)(blockIdx.y / (unsigned)step);
+    for (int k = threadIdx.x; k < kDnSpan * kDnSpan; k += 256) {
+        const int px = ox + (tx * kDnTile - 2 + k % kDnSpan) * step;
+        const int py = oy + (ty * kDnTile - 2 + k / kDnSpan) * step;
+        if (px >= 0 && px < W && py >= 0 && py < H) {
+            const size_t q = (size_t)py * W + px;
+            sE[k] = Ein[q];
+            sG[k] = G1[q];
+            sN[k] = G2[q];
+        }
+    }
+    __syncthreads();
+    const int li = (int)(threadIdx.x & 15), lj = (int)(threadIdx.x >> 4);
+    const int x = ox + (tx * kDnTile + li) * step, y = oy + (ty * kDnTile + lj) * step;
     if (x >= W || y >= H) return;
     const float h[5] = {1.0f / 16.0f, 1.0f / 4.0f, 3.0f / 8.0f, 1.0f / 4.0f, 1.0f / 16.0f};
-    const size_t p = (size_t)y * W + x;
-    const float4 ep = Ein[p], gp = G1[p], np = G2[p];
+    const int c = (lj + 2) * kDnSpan + (li + 2);
+    const float4 ep = sE[c], gp = sG[c], np = sN[c];
     const float lp = lum3(ep.x, ep.y, ep.z);
     const float sigma = kSigmaL * __builtin_sqrtf(ep.w);
     float sr = 0.0f, sg = 0.0f, sb = 0.0f, sv = 0.0f, sw = 0.0f;
@@ -133,15 +157,15 @@ __global__ __launch_bounds__(256) void kDnAtrous(const float4* __restrict__ Ein,
         for (int i = 0; i < 5; ++i) {
             const int qx = x + (i - 2) * step;
             if (qx < 0 || qx >= W) continue;
-            const size_t q = (size_t)qy * W + qx;
-            const float4 eq = Ein[q], gq = G1[q], nq = G2[q];
+            const int k = (lj + j) * kDnSpan + (li + i);
+            const float4 eq = sE[k], gq = sG[k], nq = sN[k];
             float wn;
             if (gp.w > 0.0f && gq.w > 0.0f) {
-                float c = (np.x * nq.x + np.y * nq.y) + np.z * nq.z;
-                c = c > 0.0f ? c : 0.0f;
+                float cs = (np.x * nq.x + np.y * nq.y) + np.z * nq.z;
+                cs = cs > 0.0f ? cs : 0.0f;
 #pragma unroll
-                for (int k = 0; k < 7; ++k) c = c * c;  // ^128
-                wn = c;
+                for (int r = 0; r < 7; ++r) cs = cs * cs;  // ^128
+                wn = cs;
             } else {
                 wn = gp.w == gq.w ? 1.0f : 0.0f;
             }
@@ -161,7 +185,7 @@ __global__ __launch_bounds__(256) void kDnAtrous(const float4* __restrict__ Ein,
         }
     }
     const float inv = 1.0f / sw;  // the centre tap alone is > 0
-    Eout[p] = float4{sr * inv, sg * inv, sb * inv, (sv * inv) * inv};
+    Eout[(size_t)y * W + x] = float4{sr * inv, sg * inv, sb * inv, (sv * inv) * inv};
 }
 
 __global__ __launch_bounds__(256) void kDnFinish(const float4* __restrict__ E, const float* __restrict__ albedo,
@@ -197,7 +221,11 @@ hipError_t gndDenoiseLaunch(hipStream_t stream, const float* color, const float*
     float4* o = E0;
     for (int it = 0; it < kDenoisePasses; ++it) {
         if (it == 0 && passBegin) (void)hipEventRecord(passBegin, stream);
-        hipLaunchKernelGGL(kDnAtrous, grid, block, 0, stream, in, G1, G2, W, H, 1 << it, albedo ? 1 : 0, o);
+        const int step = 1 << it;
+        // step x step sub-lattices, each tiled 16 x 16 (blocks of a sub-lattice interleave)
+        const unsigned lw = (unsigned)((W + step - 1) / step), lh = (unsigned)((H + step - 1) / step);
+        const dim3 tgrid(((lw + kDnTile - 1) / kDnTile) * (unsigned)step, ((lh + kDnTile - 1) / kDnTile) * (unsigned)step);
+        hipLaunchKernelGGL(kDnAtrous, tgrid, block, 0, stream, in, G1, G2, W, H, step, albedo ? 1 : 0, o);
         if (it == 0 && passEnd) (void)hipEventRecord(passEnd, stream);
         float4* t = in;
         in = o;
