@@ -19,6 +19,7 @@ GND_FILTER_GAUSSIAN, GND_FILTER_BOX = 0, 1
 GND_INTEGRATOR_PATH, GND_INTEGRATOR_VOLPATH, GND_INTEGRATOR_DIRECT, GND_INTEGRATOR_AO = 0, 1, 2, 3
 
 GND_OK, GND_E_INVALID, GND_E_DEVICE, GND_E_NOMEM, GND_E_CANCELLED, GND_E_STACK = 0, -1, -2, -3, -4, -5
+GND_E_TIMEOUT, GND_E_BUSY = -6, -7
 
 
 class gnd_mesh_desc(C.Structure):
@@ -136,16 +137,19 @@ class gnd_hip_stats(C.Structure):
 
 class gnd_hip_tuning(C.Structure):
     _fields_ = [("block_size", C.c_int32), ("stack_lds", C.c_int32), ("queue_capacity", C.c_int64),
-                ("use_graph", C.c_int32), ("refill_lanes", C.c_int32), ("shadow_refill_lanes", C.c_int32)]
+                ("use_graph", C.c_int32), ("refill_lanes", C.c_int32), ("shadow_refill_lanes", C.c_int32),
+                ("stack_limit", C.c_int32)]
 
 
 class gnd_hip_server_config(C.Structure):
-    _fields_ = [("hbm_budget", C.c_size_t), ("path_slots", C.c_uint32), ("max_jobs", C.c_uint32)]
+    _fields_ = [("hbm_budget", C.c_size_t), ("path_slots", C.c_uint32), ("max_jobs", C.c_uint32),
+                ("lanes", C.c_uint32)]
 
 
 class gnd_hip_job_status(C.Structure):
     _fields_ = [("state", C.c_int32), ("passes_done", C.c_uint32), ("queue_ms", C.c_double),
-                ("run_ms", C.c_double), ("latency_ms", C.c_double)]
+                ("run_ms", C.c_double), ("latency_ms", C.c_double), ("t_submit_ms", C.c_double),
+                ("t_admit_ms", C.c_double), ("t_done_ms", C.c_double)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -186,6 +190,8 @@ HIP_API = {
     "gnd_hip_server_submit": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(gnd_hip_job), C.POINTER(C.c_uint64)]),
     "gnd_hip_server_poll": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(gnd_hip_job_status)]),
     "gnd_hip_server_wait": (C.c_int, [C.c_void_p, C.c_uint64, C.c_double, C.POINTER(gnd_hip_job_status)]),
+    "gnd_hip_server_cancel": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "gnd_hip_server_release": (C.c_int, [C.c_void_p, C.c_uint64]),
     "gnd_hip_server_stats": (C.c_int, [C.c_void_p, C.POINTER(gnd_hip_stats), C.POINTER(C.c_int32),
                                        C.POINTER(C.c_size_t)]),
 }

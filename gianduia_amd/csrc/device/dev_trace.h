@@ -32,24 +32,29 @@ constexpr int spillSize() { return kStackCap - LDSD > 0 ? kStackCap - LDSD : 1; 
 // Per-lane stack: `LDSD` entries in LDS (this lane's column, stride = block
 // size), deeper entries in a private array owned by the kernel.  Only
 // scalars and pointers live here, so the state stays in registers.
+// `cap` (1..kStackCap, wave-uniform) bounds the depth: a push beyond it sets
+// `overflow` and is dropped, so the walk skips that subtree and still ends
+// (the caller reports the overflow as an error, the result is not used).
 template <int LDSD>
 struct TravStack {
     LdsPtr lds;
     int* spill;
     int stride;
     int sp;
+    int cap;
     bool overflow;
     GDEV_INLINE void push(int v) {
+        if (sp >= cap) {
+            overflow = true;
+            return;
+        }
         if (sp < LDSD) lds[sp * stride] = v;
-        else if (sp < kStackCap) spill[sp - LDSD] = v;
-        else overflow = true;
+        else spill[sp - LDSD] = v;
         ++sp;
     }
     GDEV_INLINE int pop() {
         --sp;
-        if (sp < LDSD) return lds[sp * stride];
-        if (sp < kStackCap) return spill[sp - LDSD];
-        return 0;
+        return sp < LDSD ? lds[sp * stride] : spill[sp - LDSD];
     }
 };
 

@@ -17,6 +17,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -198,12 +199,13 @@ GDEV_INLINE uint32_t xcdBlock() {
 }
 
 template <int LDSD>
-GDEV_INLINE TravStack<LDSD> makeStack(int* lds, int* spill) {
+GDEV_INLINE TravStack<LDSD> makeStack(int* lds, int* spill, int cap) {
     TravStack<LDSD> st;
     st.lds = (LdsPtr)(lds + threadIdx.x);
     st.spill = spill;
     st.stride = (int)blockDim.x;
     st.sp = 0;
+    st.cap = cap;
     st.overflow = false;
     return st;
 }
@@ -248,12 +250,13 @@ GDEV_INLINE void flushCounters(Counters* c, const uint32_t* cnt, bool overflow, 
 #endif
 
 template <int LDSD, bool ANY, bool COUNT>
-GND_WAVE_KERNEL void kTraceBatch(DScene S, gnd_ray_soa R, int64_t n, gnd_hit_soa H, int32_t* occ, Counters* ctr) {
+GND_WAVE_KERNEL void kTraceBatch(DScene S, gnd_ray_soa R, int64_t n, gnd_hit_soa H, int32_t* occ, Counters* ctr,
+                                 int32_t stackCap) {
     extern __shared__ int sStack[];
     const int64_t i = (int64_t)xcdBlock() * blockDim.x + threadIdx.x;
     uint32_t cnt[4] = {0, 0, 0, 0};
     int spill[spillSize<LDSD>()];
-    TravStack<LDSD> st = makeStack<LDSD>(sStack, spill);
+    TravStack<LDSD> st = makeStack<LDSD>(sStack, spill, stackCap);
     if (i < n) {
         HitRec h;
         bool r = traceRay<ANY, COUNT, LDSD>(S, V3{R.ox[i], R.oy[i], R.oz[i]}, V3{R.dx[i], R.dy[i], R.dz[i]},
@@ -298,6 +301,7 @@ struct WaveArgs {
                           // delta tracking in kMedium ahead of them
     int32_t refill;       // idle lanes before a wave refills: closest-hit
     int32_t srefill;      // and any-hit traversal
+    int32_t stackCap;     // traversal stack entries per lane (1..kStackCap)
 };
 
 GDEV_INLINE void storeRay(const RayQ& R, uint32_t i, V3 o, V3 d) {
@@ -451,7 +455,7 @@ GND_TRACE_KERNEL void kTracePaths(WaveArgs A) {
     uint32_t cnt[4] = {0, 0, 0, 0};
     uint32_t maxRay = 0, bad = 0;
     int spill[spillSize<LDSD>()];
-    TravStack<LDSD> st = makeStack<LDSD>(sStack, spill);
+    TravStack<LDSD> st = makeStack<LDSD>(sStack, spill, A.stackCap);
     RayFeed feed;
     feed.init(A.counts + kFeedTrace, n, A.refill);
     int64_t cur = -1;  // queue index of this lane's ray
@@ -1124,7 +1128,7 @@ __global__ __launch_bounds__(256, GND_SHADOW_WAVES) void kShadowAny(WaveArgs A) 
     const uint32_t n = A.counts[2];
     uint32_t cnt[4] = {0, 0, 0, 0};
     int spill[spillSize<LDSD>()];
-    TravStack<LDSD> st = makeStack<LDSD>(sStack, spill);
+    TravStack<LDSD> st = makeStack<LDSD>(sStack, spill, A.stackCap);
     const DScene& S = A.S;
     const ShadowSoA& Sh = A.Sh;
     const uint64_t waveMask = __ballot(1);
@@ -1168,7 +1172,7 @@ __global__ __launch_bounds__(256, GND_SHADOWTR_WAVES) void kShadow(WaveArgs A) {
     const uint32_t n = A.counts[2];
     uint32_t cnt[4] = {0, 0, 0, 0};
     int spill[spillSize<LDSD>()];
-    TravStack<LDSD> st = makeStack<LDSD>(sStack, spill);
+    TravStack<LDSD> st = makeStack<LDSD>(sStack, spill, A.stackCap);
     const DScene& S = A.S;
     const ShadowSoA& Sh = A.Sh;
     const uint64_t waveMask = __ballot(1);
@@ -1437,19 +1441,31 @@ struct gnd_hip_scene {
     size_t accelBytes = 0;  // TLAS, instances, BLAS and triangle packs
 };
 
-// Path state, hit and shadow queues and queue counters for `capacity` paths
-// in flight: 288 bytes per path slot in 71 SoA arrays, each 256-B aligned.
-struct WaveBuffers {
+// Queues of one bounce loop for `capacity` paths in flight: hit records and
+// path indices by queue position, the rays of the two path queues, the
+// shadow queue, the two density-grid lists and the queue counters (148 bytes
+// per entry in 37 SoA arrays, each 256-B aligned).  A bounce loop reads and
+// writes only its own queues, so loops with their own queue sets can run
+// concurrently over one pool of path slots.
+struct QueueSet {
     size_t capacity = 0;
-    void* pathBuf = nullptr;
-    void* aovBuf = nullptr;  // per-slot AOV records, allocated on first use
-    PathSoA P{};
+    void* buf = nullptr;
     HitSoA Hq{};
     ShadowSoA Sh{};
     int32_t* q[2] = {nullptr, nullptr};
     RayQ rq[2] = {};
     int32_t* gl[2] = {nullptr, nullptr};  // density-grid lists of the two queues
     uint32_t* counts = nullptr;  // kNumCounts queue and ray-feed counters
+};
+
+// Path state per slot (140 bytes in 34 SoA arrays, each 256-B aligned) and
+// the queues of gnd_hip_render's bounce loop: 288 bytes per path slot.
+struct WaveBuffers {
+    size_t capacity = 0;
+    void* pathBuf = nullptr;
+    void* aovBuf = nullptr;  // per-slot AOV records, allocated on first use
+    PathSoA P{};
+    QueueSet Q;
 };
 
 struct gnd_hip_ctx {
@@ -1459,7 +1475,7 @@ struct gnd_hip_ctx {
     std::mutex mu;
     // 40M paths in flight (~10 GB of path state): a whole 1080p x 16 spp frame
     // per wave, so every bounce is one large launch
-    gnd_hip_tuning tuning{128, 16, 40000000, 0, 0, 0};
+    gnd_hip_tuning tuning{128, 16, 40000000, 0, 0, 0, 0};
     bool counters = false;
     int numCUs = 0;
     size_t hbm = 0;
@@ -1675,20 +1691,71 @@ void ensureDev(T*& p, size_t& cap, size_t need) {
     cap = n;
 }
 
-size_t waveBufferBytes(size_t cap) {
-    const size_t per = ((cap * 4 + 255) / 256) * 256;
-    return 70 * per + ((cap * 8 + 255) / 256) * 256;
+constexpr size_t kPathFields4 = 33, kQueueFields4 = 37;  // four-byte fields per slot / queue entry
+
+size_t alignedField(size_t cap, size_t elemBytes) { return ((cap * elemBytes + 255) / 256) * 256; }
+size_t pathStateBytes(size_t cap) { return kPathFields4 * alignedField(cap, 4) + alignedField(cap, 8); }
+size_t queueSetBytes(size_t cap) { return kQueueFields4 * alignedField(cap, 4); }
+size_t waveBufferBytes(size_t cap) { return pathStateBytes(cap) + queueSetBytes(cap); }
+
+// Hands out 256-B aligned SoA fields of `cap` entries from one allocation.
+struct FieldCarver {
+    char* base;
+    size_t cap, off = 0;
+    void* take(size_t elemBytes) {
+        void* p = base + off;
+        off += alignedField(cap, elemBytes);
+        return p;
+    }
+};
+
+void freeQueueSet(QueueSet& qs) {
+    if (qs.buf) (void)hipFree(qs.buf);
+    if (qs.counts) (void)hipFree(qs.counts);
+    qs = QueueSet{};
+}
+
+void ensureQueueSet(QueueSet& qs, size_t cap) {
+    if (!qs.counts) GND_CHECK(hipMalloc(&qs.counts, kNumCounts * sizeof(uint32_t)));
+    if (cap <= qs.capacity) return;
+    if (qs.buf) GND_CHECK(hipFree(qs.buf));
+    qs.buf = nullptr;
+    qs.capacity = 0;
+    const size_t bytes = queueSetBytes(cap);
+    GND_CHECK(hipMalloc(&qs.buf, bytes));
+    FieldCarver c{static_cast<char*>(qs.buf), cap};
+    qs.Hq.t = (float*)c.take(4);
+    qs.Hq.u = (float*)c.take(4);
+    qs.Hq.v = (float*)c.take(4);
+    qs.Hq.inst = (int32_t*)c.take(4);
+    qs.Hq.prim = (int32_t*)c.take(4);
+    ShadowSoA& S = qs.Sh;
+    S.path = (int32_t*)c.take(4);
+    S.medium = (int32_t*)c.take(4);
+    float** g[] = {&S.ox, &S.oy, &S.oz, &S.dx, &S.dy, &S.dz, &S.tmax, &S.pr, &S.pg, &S.pb, &S.lpx, &S.lpy, &S.lpz,
+                   &S.time};
+    for (auto pp : g) *pp = (float*)c.take(4);
+    qs.q[0] = (int32_t*)c.take(4);
+    qs.q[1] = (int32_t*)c.take(4);
+    qs.gl[0] = (int32_t*)c.take(4);
+    qs.gl[1] = (int32_t*)c.take(4);
+    for (RayQ& r : qs.rq)
+        for (float** pp : {&r.ox, &r.oy, &r.oz, &r.dx, &r.dy, &r.dz}) *pp = (float*)c.take(4);
+    if (c.off > bytes) throw std::runtime_error("queue layout overflow");
+    qs.capacity = cap;
 }
 
 void freeWaveBuffers(WaveBuffers& wb) {
     if (wb.pathBuf) (void)hipFree(wb.pathBuf);
     if (wb.aovBuf) (void)hipFree(wb.aovBuf);
-    if (wb.counts) (void)hipFree(wb.counts);
+    freeQueueSet(wb.Q);
     wb = WaveBuffers{};
 }
 
-void ensureWaveBuffers(WaveBuffers& wb, size_t cap) {
-    if (!wb.counts) GND_CHECK(hipMalloc(&wb.counts, kNumCounts * sizeof(uint32_t)));
+// Path state for `cap` slots; with `queues`, also the queue set of one bounce
+// loop over them.
+void ensureWaveBuffers(WaveBuffers& wb, size_t cap, bool queues = true) {
+    if (queues) ensureQueueSet(wb.Q, cap);
     if (cap <= wb.capacity) return;
     if (wb.pathBuf) GND_CHECK(hipFree(wb.pathBuf));
     wb.pathBuf = nullptr;
@@ -1697,48 +1764,25 @@ void ensureWaveBuffers(WaveBuffers& wb, size_t cap) {
     wb.P.aov = nullptr;
     wb.P.aovStride = 0;
     wb.capacity = 0;
-    // 70 four-byte fields and one eight-byte field per slot
-    const size_t bytes = waveBufferBytes(cap);
+    // 33 four-byte fields and one eight-byte field per slot
+    const size_t bytes = pathStateBytes(cap);
     GND_CHECK(hipMalloc(&wb.pathBuf, bytes));
-    char* b = static_cast<char*>(wb.pathBuf);
-    size_t off = 0;
-    auto take = [&](size_t elemBytes) {
-        void* p = b + off;
-        off += ((cap * elemBytes + 255) / 256) * 256;
-        return p;
-    };
+    FieldCarver c{static_cast<char*>(wb.pathBuf), cap};
     PathSoA& P = wb.P;
-    P.rng = (uint64_t*)take(8);
+    P.rng = (uint64_t*)c.take(8);
     float** f[] = {&P.tpr, &P.tpg, &P.tpb, &P.Lr, &P.Lg, &P.Lb,
                    &P.pdfPrev, &P.lx, &P.ly, &P.lz, &P.weight, &P.pfx, &P.pfy, &P.evpx, &P.evpy, &P.evpz,
                    &P.evnx, &P.evny, &P.evnz, &P.evtx, &P.evty, &P.evtz, &P.evu, &P.evv, &P.time};
-    for (auto pp : f) *pp = (float*)take(4);
-    P.c0 = (uint32_t*)take(4);
-    P.c1 = (uint32_t*)take(4);
-    P.flags = (uint32_t*)take(4);
-    P.job = (int32_t*)take(4);
-    P.chunk = (int32_t*)take(4);
-    P.evKind = (int32_t*)take(4);
-    P.evPrim = (int32_t*)take(4);
-    P.evMedium = (int32_t*)take(4);
-    wb.Hq.t = (float*)take(4);
-    wb.Hq.u = (float*)take(4);
-    wb.Hq.v = (float*)take(4);
-    wb.Hq.inst = (int32_t*)take(4);
-    wb.Hq.prim = (int32_t*)take(4);
-    ShadowSoA& S = wb.Sh;
-    S.path = (int32_t*)take(4);
-    S.medium = (int32_t*)take(4);
-    float** g[] = {&S.ox, &S.oy, &S.oz, &S.dx, &S.dy, &S.dz, &S.tmax, &S.pr, &S.pg, &S.pb, &S.lpx, &S.lpy, &S.lpz,
-                   &S.time};
-    for (auto pp : g) *pp = (float*)take(4);
-    wb.q[0] = (int32_t*)take(4);
-    wb.q[1] = (int32_t*)take(4);
-    wb.gl[0] = (int32_t*)take(4);
-    wb.gl[1] = (int32_t*)take(4);
-    for (RayQ& r : wb.rq)
-        for (float** pp : {&r.ox, &r.oy, &r.oz, &r.dx, &r.dy, &r.dz}) *pp = (float*)take(4);
-    if (off > bytes) throw std::runtime_error("wave buffer layout overflow");
+    for (auto pp : f) *pp = (float*)c.take(4);
+    P.c0 = (uint32_t*)c.take(4);
+    P.c1 = (uint32_t*)c.take(4);
+    P.flags = (uint32_t*)c.take(4);
+    P.job = (int32_t*)c.take(4);
+    P.chunk = (int32_t*)c.take(4);
+    P.evKind = (int32_t*)c.take(4);
+    P.evPrim = (int32_t*)c.take(4);
+    P.evMedium = (int32_t*)c.take(4);
+    if (c.off > bytes) throw std::runtime_error("path state layout overflow");
     wb.capacity = cap;
 }
 
@@ -1750,7 +1794,7 @@ void ensureAovBuffers(WaveBuffers& wb) {
     wb.P.aovStride = (uint32_t)wb.capacity;
 }
 
-using TraceKernel = void (*)(DScene, gnd_ray_soa, int64_t, gnd_hit_soa, int32_t*, Counters*);
+using TraceKernel = void (*)(DScene, gnd_ray_soa, int64_t, gnd_hit_soa, int32_t*, Counters*, int32_t);
 
 template <bool ANY, bool COUNT>
 TraceKernel pickTrace(int lds) {
@@ -1763,6 +1807,8 @@ TraceKernel pickTrace(int lds) {
 }
 // LDS stack depths compiled in: 8, 16, 24, 32 entries per lane
 int normLds(int lds) { return lds <= 8 ? 8 : (lds <= 16 ? 16 : (lds <= 24 ? 24 : 32)); }
+// traversal stack entries per lane: the tuning's limit, or the whole stack
+int stackCapOf(const gnd_hip_tuning& t) { return t.stack_limit > 0 ? std::min(t.stack_limit, kStackCap) : kStackCap; }
 
 using WaveKernel = void (*)(WaveArgs);
 template <bool COUNT, bool MOTION>
@@ -1803,6 +1849,7 @@ float elapsedMs(hipEvent_t a, hipEvent_t b) {
 // Defined with the host scene shell: frees the device copies that host scenes
 // made on `ctx` (called at shutdown while the context is still valid).
 void gndReleaseContextUploads(gnd_hip_ctx* ctx);
+static int sceneUpload(gnd_hip_ctx* ctx, const gnd_hip_scene_arrays* a, gnd_hip_scene** out);
 
 extern "C" {
 
@@ -1880,6 +1927,10 @@ int gnd_hip_set_tuning(gnd_hip_ctx* ctx, const gnd_hip_tuning* t) {
         ctx->err = "refill lanes must be 0 (per scene) or 1..64";
         return GND_E_INVALID;
     }
+    if (t->stack_limit < 0 || t->stack_limit > kStackCap) {
+        ctx->err = "stack_limit must be 0 (the full stack) or 1.." + std::to_string(kStackCap);
+        return GND_E_INVALID;
+    }
     std::lock_guard<std::mutex> g(ctx->mu);
     ctx->tuning = *t;
     ctx->tuning.stack_lds = normLds(t->stack_lds);
@@ -1932,6 +1983,13 @@ int gnd_hip_synchronize(gnd_hip_ctx* ctx) {
 int gnd_hip_scene_upload(gnd_hip_ctx* ctx, const gnd_hip_scene_arrays* a, gnd_hip_scene** out) {
     if (!ctx || !a || !out || !a->desc) return GND_E_INVALID;
     std::lock_guard<std::mutex> g(ctx->mu);
+    return sceneUpload(ctx, a, out);
+}
+
+}  // extern "C"
+
+// gnd_hip_scene_upload with the context lock held by the caller
+static int sceneUpload(gnd_hip_ctx* ctx, const gnd_hip_scene_arrays* a, gnd_hip_scene** out) {
     gnd_hip_scene* s = new gnd_hip_scene();
     int rc = guarded(ctx, [&] {
         GND_CHECK(hipSetDevice(ctx->device));
@@ -2160,6 +2218,8 @@ int gnd_hip_scene_upload(gnd_hip_ctx* ctx, const gnd_hip_scene_arrays* a, gnd_hi
     return GND_OK;
 }
 
+extern "C" {
+
 void gnd_hip_scene_free(gnd_hip_ctx* ctx, gnd_hip_scene* s) {
     if (!s) return;
     if (ctx) {
@@ -2190,7 +2250,7 @@ static int traceBatch(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_ray_soa*
                             : (ctx->counters ? pickTrace<false, true>(lds) : pickTrace<false, false>(lds));
         GND_CHECK(hipEventRecord(ctx->ev[0], ctx->stream));
         hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(B), (size_t)B * lds * sizeof(int), ctx->stream,
-                           scene->view, *rays, n, h, occ, ctx->dCounters);
+                           scene->view, *rays, n, h, occ, ctx->dCounters, stackCapOf(ctx->tuning));
         GND_CHECK(hipGetLastError());
         GND_CHECK(hipEventRecord(ctx->ev[1], ctx->stream));
         Counters c;
@@ -2289,6 +2349,7 @@ struct BouncePlan {
     bool split = false, fastShadow = false;
     int B = 128;
     int refill = 32, srefill = 40;
+    int stackCap = kStackCap;
     size_t shm = 0;
     unsigned gridCap = 8;
     // blocks for n queue entries: one entry per thread up to the grid cap
@@ -2334,6 +2395,7 @@ static BouncePlan makeBouncePlan(const gnd_hip_ctx* ctx, const gnd_hip_scene* sc
     const bool small = scene->accelBytes <= (size_t(4) << 20);
     K.refill = ctx->tuning.refill_lanes > 0 ? ctx->tuning.refill_lanes : (small ? 56 : 32);
     K.srefill = ctx->tuning.shadow_refill_lanes > 0 ? ctx->tuning.shadow_refill_lanes : (small ? 64 : 40);
+    K.stackCap = stackCapOf(ctx->tuning);
     return K;
 }
 
@@ -2369,45 +2431,45 @@ struct Timeline {
 };
 
 // in-queue `cur` of the two (with its rays and density-grid list), out-queue the other
-static void bindQueues(WaveArgs& A, const WaveBuffers& wb, int cur) {
-    A.qIn = wb.q[cur];
-    A.qOut = wb.q[1 - cur];
-    A.rIn = wb.rq[cur];
-    A.rOut = wb.rq[1 - cur];
-    A.gIn = wb.gl[cur];
-    A.gOut = wb.gl[1 - cur];
+static void bindQueues(WaveArgs& A, const QueueSet& qs, int cur) {
+    A.qIn = qs.q[cur];
+    A.qOut = qs.q[1 - cur];
+    A.rIn = qs.rq[cur];
+    A.rOut = qs.rq[1 - cur];
+    A.gIn = qs.gl[cur];
+    A.gOut = qs.gl[1 - cur];
 }
 
 // One bounce over the in-queue A.qIn (count in counts[0], at most `live`):
 // closest hit, shading and NEE, shadow rays, the split scatter, then the queue
 // rotation.  No host synchronisation, so the chain can be captured.
-static void launchBounce(gnd_hip_ctx* ctx, const WaveArgs& A, const BouncePlan& K, uint32_t live, Timeline* tl) {
+static void launchBounce(hipStream_t stream, const WaveArgs& A, const BouncePlan& K, uint32_t live, Timeline* tl) {
     const unsigned g = K.grid(live);
     size_t t = tl ? tl->mark() : 0;
-    hipLaunchKernelGGL(K.trace, dim3(g), dim3(K.B), K.shm, ctx->stream, A);
+    hipLaunchKernelGGL(K.trace, dim3(g), dim3(K.B), K.shm, stream, A);
     GND_CHECK(hipGetLastError());
     if (tl) tl->lap(t, 0);
     const unsigned gs = K.shadeGrid(live);
     if (K.split) {
         // density-grid delta tracking for the paths that start in a grid
-        hipLaunchKernelGGL(kMedium, dim3(K.gridFor(live, 256)), dim3(256), 0, ctx->stream, A);
+        hipLaunchKernelGGL(kMedium, dim3(K.gridFor(live, 256)), dim3(256), 0, stream, A);
         GND_CHECK(hipGetLastError());
     }
-    if (K.split) hipLaunchKernelGGL(kShade<true>, dim3(gs), dim3(kShadeBlock), 0, ctx->stream, A);
-    else hipLaunchKernelGGL(kShade<false>, dim3(gs), dim3(kShadeBlock), 0, ctx->stream, A);
+    if (K.split) hipLaunchKernelGGL(kShade<true>, dim3(gs), dim3(kShadeBlock), 0, stream, A);
+    else hipLaunchKernelGGL(kShade<false>, dim3(gs), dim3(kShadeBlock), 0, stream, A);
     GND_CHECK(hipGetLastError());
     if (tl) tl->lap(t, 2);
     // NEE emits at most one shadow ray per path
-    hipLaunchKernelGGL(K.shadow, dim3(g), dim3(K.B), K.shm, ctx->stream, A);
+    hipLaunchKernelGGL(K.shadow, dim3(g), dim3(K.B), K.shm, stream, A);
     GND_CHECK(hipGetLastError());
     if (tl) tl->lap(t, 1);
     if (K.split) {
         // the paths of this bounce with an event scatter after their shadow
         // rays consumed the transmittance samples
-        hipLaunchKernelGGL(kScatter, dim3(gs), dim3(kShadeBlock), 0, ctx->stream, A);
+        hipLaunchKernelGGL(kScatter, dim3(gs), dim3(kShadeBlock), 0, stream, A);
         GND_CHECK(hipGetLastError());
     }
-    hipLaunchKernelGGL(kRotate, dim3(1), dim3(64), 0, ctx->stream, A.counts, A.ctr);
+    hipLaunchKernelGGL(kRotate, dim3(1), dim3(64), 0, stream, A.counts, A.ctr);
     GND_CHECK(hipGetLastError());
     if (tl) tl->lap(t, 2);
 }
@@ -2434,16 +2496,15 @@ static DJob makeDJob(const gnd_hip_job& j) {
     return J;
 }
 
-static void clearFilm(gnd_hip_ctx* ctx, const DJob& J) {
+static void clearFilm(hipStream_t stream, const DJob& J) {
     const size_t np = (size_t)J.W * J.H;
-    hipLaunchKernelGGL(kFill, dim3((unsigned)((np * 4 + 255) / 256)), dim3(256), 0, ctx->stream, J.film, np * 4,
-                       0.0f);
+    hipLaunchKernelGGL(kFill, dim3((unsigned)((np * 4 + 255) / 256)), dim3(256), 0, stream, J.film, np * 4, 0.0f);
     if (J.filmSq)
-        hipLaunchKernelGGL(kFill, dim3((unsigned)((np * 3 + 255) / 256)), dim3(256), 0, ctx->stream, J.filmSq,
-                           np * 3, 0.0f);
+        hipLaunchKernelGGL(kFill, dim3((unsigned)((np * 3 + 255) / 256)), dim3(256), 0, stream, J.filmSq, np * 3,
+                           0.0f);
     if (J.filmAov)
-        hipLaunchKernelGGL(kFill, dim3((unsigned)((np * kAovFloats + 255) / 256)), dim3(256), 0, ctx->stream,
-                           J.filmAov, np * kAovFloats, 0.0f);
+        hipLaunchKernelGGL(kFill, dim3((unsigned)((np * kAovFloats + 255) / 256)), dim3(256), 0, stream, J.filmAov,
+                           np * kAovFloats, 0.0f);
 }
 
 static void fillStats(gnd_hip_stats* stats, const Counters& c, uint64_t samples) {
@@ -2492,7 +2553,7 @@ int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jo
         std::vector<DJob> dj(jobs.size());
         for (size_t j = 0; j < jobs.size(); ++j) {
             dj[j] = makeDJob(jobs[j]);
-            clearFilm(ctx, dj[j]);
+            clearFilm(ctx->stream, dj[j]);
         }
         ensureDev(ctx->dJobs, ctx->jobCap, dj.size());
         GND_CHECK(hipMemcpyAsync(ctx->dJobs, dj.data(), dj.size() * sizeof(DJob), hipMemcpyHostToDevice,
@@ -2512,7 +2573,7 @@ int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jo
                                      hipMemcpyHostToDevice, ctx->stream));
             GND_CHECK(hipMemcpyAsync(ctx->dTasks, w.tasks.data(), w.tasks.size() * sizeof(DSplatTask),
                                      hipMemcpyHostToDevice, ctx->stream));
-            hipLaunchKernelGGL(kFillU32, dim3(1), dim3(64), 0, ctx->stream, ctx->wb.counts, (size_t)kNumCounts, 0u);
+            hipLaunchKernelGGL(kFillU32, dim3(1), dim3(64), 0, ctx->stream, ctx->wb.Q.counts, (size_t)kNumCounts, 0u);
             WaveArgs A{};
             A.S = scene->view;
             A.jobs = ctx->dJobs;
@@ -2520,21 +2581,22 @@ int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jo
             A.nChunks = (int32_t)w.chunks.size();
             A.nSlots = w.slots;
             A.P = ctx->wb.P;
-            A.Hq = ctx->wb.Hq;
-            A.Sh = ctx->wb.Sh;
-            A.counts = ctx->wb.counts;
+            A.Hq = ctx->wb.Q.Hq;
+            A.Sh = ctx->wb.Q.Sh;
+            A.counts = ctx->wb.Q.counts;
             A.chunkLive = nullptr;
             A.ctr = ctx->dCounters;
             A.pathOnlyFast = K.fastShadow ? 1 : 0;
             A.split = K.split ? 1 : 0;
             A.refill = K.refill;
             A.srefill = K.srefill;
-            bindQueues(A, ctx->wb, 0);
+            A.stackCap = K.stackCap;
+            bindQueues(A, ctx->wb.Q, 0);
             size_t tc = tl.mark();
             A.camQueueBase = 0;
             hipLaunchKernelGGL(kCamera, dim3((w.slots + 255) / 256), dim3(256), 0, ctx->stream, A);
             GND_CHECK(hipGetLastError());
-            hipLaunchKernelGGL(kFillU32, dim3(1), dim3(64), 0, ctx->stream, ctx->wb.counts, (size_t)1, w.live);
+            hipLaunchKernelGGL(kFillU32, dim3(1), dim3(64), 0, ctx->stream, ctx->wb.Q.counts, (size_t)1, w.live);
             if (tlp) tl.lap(tc, 2);
             int cur = 0;
             uint32_t live = w.live;
@@ -2548,8 +2610,8 @@ int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jo
                 GND_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
                 try {
                     for (int d = 0; d < depth; ++d) {
-                        bindQueues(A, ctx->wb, cur);
-                        launchBounce(ctx, A, K, live, nullptr);
+                        bindQueues(A, ctx->wb.Q, cur);
+                        launchBounce(ctx->stream, A, K, live, nullptr);
                         cur = 1 - cur;
                     }
                 } catch (...) {
@@ -2559,16 +2621,16 @@ int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jo
                 GND_CHECK(hipStreamEndCapture(ctx->stream, &gr));
                 const hipError_t ie = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
                 if (ie == hipSuccess) GND_CHECK(hipGraphLaunch(ge, ctx->stream));
-                live = ie == hipSuccess ? readLive(ctx, ctx->wb.counts) : 0;
+                live = ie == hipSuccess ? readLive(ctx, ctx->wb.Q.counts) : 0;
                 if (ge) (void)hipGraphExecDestroy(ge);
                 (void)hipGraphDestroy(gr);
                 GND_CHECK(ie);
             }
             while (live > 0) {
-                bindQueues(A, ctx->wb, cur);
-                launchBounce(ctx, A, K, live, tlp);
+                bindQueues(A, ctx->wb.Q, cur);
+                launchBounce(ctx->stream, A, K, live, tlp);
                 cur = 1 - cur;
-                live = readLive(ctx, ctx->wb.counts);
+                live = readLive(ctx, ctx->wb.Q.counts);
                 if (ctx->cancel.load()) throw HipError(GND_E_CANCELLED, "render cancelled");
             }
             size_t ts = tl.mark();
@@ -2700,77 +2762,126 @@ int gnd_hip_autotune(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* 
 
 // ---------------------------------------------------------------- serving engine
 //
-// Continuous batching: jobs queue in submission order; between bounces the
-// engine admits passes of queued jobs into free path slots (a pass takes W*H
-// contiguous slots), runs one bounce over every live path of the active scene,
-// retires passes whose paths have all ended (per-chunk live counters), and
+// Continuous batching.  A server owns one pool of path slots and several
+// bounce loops ("lanes"), each with its own queues, counters and stream.  A
+// lane renders the jobs of one resident scene, so lanes of different scenes
+// run at the same time on the device.  Requests go into an inbox under a
+// short lock and never wait for device work.  The worker thread admits
+// passes of waiting jobs into free slots of their scene's lane whenever that
+// lane is between bounces, launches one bounce per lane on the lane's stream
+// without waiting for the others, and, as each lane's bounce completes,
+// retires passes whose paths have all ended (per-chunk live counters) and
 // gathers finished passes into their job's film in pass order, so a film is
 // bit-identical whatever else shares the GPU.  Scenes stay resident in HBM
-// until the budget needs their space (least recently used first).
+// until the budget needs their space (least recently used scene without a
+// lane first).
+
+using SrvClock = std::chrono::steady_clock;
 
 struct SrvScene {
     const gnd_hip_scene_arrays* arrays = nullptr;  // borrowed until the server is destroyed
-    gnd_hip_scene* dev = nullptr;
+    gnd_hip_scene* dev = nullptr;                  // worker-owned
     size_t bytes = 0;
     uint64_t lastUse = 0;
-    int running = 0;  // jobs of this scene admitted and not finished
+    int lane = -1;  // lane rendering it, -1 none
 };
 
 struct SrvJob {
+    uint64_t ticket = 0;
     int scene = -1;
     gnd_hip_job job{};
-    int slot = -1;            // entry in the device job table
-    uint32_t nextPass = 0;    // next pass to admit
-    uint32_t nextGather = 0;  // next pass to gather into the film
-    int32_t state = 0;        // 0 queued, 1 running, 2 done, < 0 failed (GND_E_*)
-    std::chrono::steady_clock::time_point tSubmit, tAdmit, tDone;
-    std::map<uint32_t, int> finished;  // pass -> chunk, all paths ended, awaiting the gather
+    // published under the server lock
+    int32_t state = 0;  // 0 queued, 1 running, 2 done, < 0 failed (GND_E_*)
+    uint32_t passesDone = 0;
+    SrvClock::time_point tSubmit, tAdmit, tDone;
+    bool cancelRequested = false;
+    // owned by the worker
+    int lane = -1;
+    int slot = -1;                    // entry in the device job table
+    uint32_t nextPass = 0;            // next pass to admit
+    uint32_t nextGather = 0;          // next pass to gather into the film
+    uint32_t chunksOut = 0;           // passes admitted and not yet gathered or dropped
+    int32_t failCode = 0;             // != 0: stop, drop what is in flight, then fail with it
+    SrvClock::time_point blockedSince{};  // first failed attempt to find slots for its next pass
+    std::map<uint32_t, int> finished; // pass -> chunk whose paths have all ended
 };
 
 struct SrvChunk {
-    uint64_t ticket = 0;
+    SrvJob* job = nullptr;
     uint32_t pass = 0, slotBase = 0, npix = 0;
-    bool running = false;
 };
 
-struct gnd_hip_server {
-    gnd_hip_ctx* ctx = nullptr;
-    gnd_hip_server_config cfg{};
-    size_t pathBytes = 0;
-    WaveBuffers wb;
-    std::mutex mu;
-    std::condition_variable cv, done;
-    std::thread worker;
-    bool stop = false;
-    std::string err;
-    std::vector<SrvScene> scenes;
-    std::deque<uint64_t> queue;
-    std::unordered_map<uint64_t, SrvJob> jobs;
-    uint64_t nextTicket = 1, useClock = 0;
-    int active = -1;                  // scene of the running batch
-    // device tables
-    DJob* dJobs = nullptr;
-    std::vector<int> freeJobSlots;
-    uint32_t* dLive = nullptr;        // live paths per chunk
-    std::vector<SrvChunk> chunks;
-    std::vector<int> freeChunks;
-    std::vector<uint32_t> hLive;
+// D2H words a lane reads back after each bounce (pinned host memory):
+// [0] live paths, [2..3] stack overflows (u64), [4 + c] live paths of chunk c
+constexpr int kLaneReadHead = 4;
+
+struct SrvLane {
+    int scene = -1;
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    QueueSet Q;
+    Counters* dCtr = nullptr;
+    BouncePlan K{};
+    uint32_t live = 0;  // paths in the in-queue
+    int cur = 0;
+    bool inFlight = false;
+    uint32_t readChunks = 0;  // chunk ids [0, readChunks) read back by the bounce in flight
+    unsigned long long overflowSeen = 0;
+    std::vector<int> running;        // chunks with paths in flight
+    std::vector<SrvJob*> jobs;       // admitted jobs not yet finished
+    std::vector<int> fresh;          // chunks admitted for the next launch
+    std::vector<SrvJob*> freshJobs;  // jobs admitted for the next launch
     DChunk* dCam = nullptr;
     size_t camCap = 0;
     DChunk* dGather = nullptr;
     size_t gatherCap = 0;
     DSplatTask* dTasks = nullptr;
     size_t taskCap = 0;
-    Counters* dCounters = nullptr;
+    char* hStage = nullptr;  // pinned H2D staging, reused once the stream is idle
+    size_t stageCap = 0;
+    uint32_t* hRead = nullptr;  // pinned D2H words (kLaneReadHead + kSrvMaxChunks)
+};
+
+struct gnd_hip_server {
+    gnd_hip_ctx* ctx = nullptr;
+    gnd_hip_server_config cfg{};
+    size_t pathBytes = 0;  // path state + the lanes' queues
+    SrvClock::time_point t0;
+    // ---- shared with callers, under mu
+    std::mutex mu;
+    std::condition_variable cv, done;
+    bool stop = false;
+    std::vector<SrvScene> scenes;  // appended by add_scene under mu; other fields worker-owned
+    std::unordered_map<uint64_t, std::unique_ptr<SrvJob>> jobs;
+    std::vector<SrvJob*> inbox;
+    std::vector<uint64_t> cancels;
+    uint64_t nextTicket = 1;
+    uint64_t samplesDone = 0;
+    int residentScenes = 0;
+    size_t residentBytes = 0;
+    // ---- owned by the worker
+    std::thread worker;
+    WaveBuffers wb;  // path state of the slot pool (queues live in the lanes)
+    std::vector<SrvLane> lanes;
+    std::deque<SrvJob*> pending;  // submitted, passes left to admit (submission order)
+    uint64_t useClock = 0;
+    DJob* dJobs = nullptr;
+    std::vector<int> freeJobSlots;
+    uint32_t* dLive = nullptr;  // live paths per chunk
+    std::vector<SrvChunk> chunks;
+    std::vector<int> freeChunks;
     std::map<uint32_t, uint32_t> freeSlots;  // begin -> end of free path-slot ranges
-    uint32_t live = 0;
-    int cur = 0;
-    std::vector<int> running;  // chunks with paths in flight
 };
 
 namespace {
 
 constexpr int kSrvMaxChunks = 1 << 16;
+constexpr int kSrvDefaultLanes = 4;
+constexpr int kSrvMaxLanes = 16;
+
+double msSince(const gnd_hip_server* s, SrvClock::time_point t) {
+    return std::chrono::duration<double, std::milli>(t - s->t0).count();
+}
 
 bool takeSlots(gnd_hip_server* s, uint32_t n, uint32_t& base) {
     for (auto it = s->freeSlots.begin(); it != s->freeSlots.end(); ++it) {
@@ -2812,26 +2923,35 @@ size_t sceneEstimate(const gnd_hip_scene_arrays* a) {
     return b;
 }
 
-size_t residentBytes(const gnd_hip_server* s) {
-    size_t b = s->pathBytes;
+size_t residentBytesOf(const gnd_hip_server* s) {
+    size_t b = s->pathBytes + (s->wb.aovBuf ? s->wb.capacity * kAovFloats * sizeof(float) : 0);
     for (const SrvScene& sc : s->scenes)
         if (sc.dev) b += sc.bytes;
     return b;
 }
 
-// Makes scene `i` resident, evicting idle scenes (least recently used first)
-// while the budget would be exceeded.
+void publishResidency(gnd_hip_server* s) {
+    int n = 0;
+    for (const SrvScene& sc : s->scenes) n += sc.dev ? 1 : 0;
+    const size_t b = residentBytesOf(s);
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->residentScenes = n;
+    s->residentBytes = b;
+}
+
+// Makes scene `i` resident, evicting scenes no lane renders (least recently
+// used first) while the budget would be exceeded.
 void makeResident(gnd_hip_server* s, int i) {
     SrvScene& sc = s->scenes[i];
     sc.lastUse = ++s->useClock;
     if (sc.dev) return;
     const size_t need = sceneEstimate(sc.arrays);
     for (;;) {
-        if (residentBytes(s) + need <= s->cfg.hbm_budget) break;
+        if (residentBytesOf(s) + need <= s->cfg.hbm_budget) break;
         int victim = -1;
         for (int k = 0; k < (int)s->scenes.size(); ++k) {
             const SrvScene& o = s->scenes[k];
-            if (k == i || !o.dev || o.running) continue;
+            if (k == i || !o.dev || o.lane >= 0) continue;
             if (victim < 0 || o.lastUse < s->scenes[victim].lastUse) victim = k;
         }
         if (victim < 0) throw HipError(GND_E_NOMEM, "scene does not fit the server's HBM budget");
@@ -2839,250 +2959,482 @@ void makeResident(gnd_hip_server* s, int i) {
         s->scenes[victim].dev = nullptr;
     }
     gnd_hip_scene* dev = nullptr;
-    s->ctx->mu.unlock();  // gnd_hip_scene_upload takes the context lock
-    const int rc = gnd_hip_scene_upload(s->ctx, sc.arrays, &dev);
-    s->ctx->mu.lock();
+    const int rc = sceneUpload(s->ctx, sc.arrays, &dev);  // the worker holds the context lock
     if (rc != GND_OK) throw HipError(rc, std::string("scene upload: ") + s->ctx->err);
     sc.dev = dev;
     sc.bytes = gnd_hip_scene_bytes(dev);
+    publishResidency(s);
 }
 
-void failJob(gnd_hip_server* s, uint64_t t, int code) {
-    SrvJob& j = s->jobs[t];
-    j.tDone = std::chrono::steady_clock::now();
-    if (j.state == 0) j.tAdmit = j.tDone;
-    j.state = code;
-    if (j.slot >= 0) s->freeJobSlots.push_back(j.slot);
-    j.slot = -1;
-}
+// A state change to publish under the server lock (kSrvProgress: passes
+// gathered so far, the state unchanged).
+struct SrvEvent {
+    SrvJob* job;
+    int32_t state;
+    SrvClock::time_point t;
+};
+constexpr int32_t kSrvProgress = 100;
 
-// Admission: passes of the queue head while it belongs to the active scene and
-// path slots, chunk ids and job-table entries are free.  Returns new chunks.
-std::vector<int> admit(gnd_hip_server* s) {
-    std::vector<int> fresh;
-    while (!s->queue.empty()) {
-        const uint64_t t = s->queue.front();
-        SrvJob& j = s->jobs[t];
-        if (j.scene != s->active) break;
-        if (j.state == 0) {
-            if (s->freeJobSlots.empty()) break;
-            j.slot = s->freeJobSlots.back();
-            s->freeJobSlots.pop_back();
-            const DJob dj = makeDJob(j.job);
-            GND_CHECK(hipMemcpyAsync(s->dJobs + j.slot, &dj, sizeof(DJob), hipMemcpyHostToDevice,
-                                     s->ctx->stream));
-            clearFilm(s->ctx, dj);
-            j.state = 1;
-            j.tAdmit = std::chrono::steady_clock::now();
-            s->scenes[j.scene].running++;
+void publish(gnd_hip_server* s, const std::vector<SrvEvent>& evs) {
+    if (evs.empty()) return;
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (const SrvEvent& e : evs) {
+        SrvJob* j = e.job;
+        if (e.state == kSrvProgress) {
+            j->passesDone = j->nextGather;
+            continue;
         }
-        const uint32_t npix = (uint32_t)j.job.camera.width * (uint32_t)j.job.camera.height;
-        while (j.nextPass < j.job.spp) {
+        if (e.state == 1) {
+            j->tAdmit = e.t;
+        } else {
+            if (j->state == 0) j->tAdmit = e.t;
+            j->tDone = e.t;
+            if (e.state == 2)
+                s->samplesDone += (uint64_t)j->job.camera.width * (uint64_t)j->job.camera.height * j->job.spp;
+        }
+        j->state = e.state;
+        j->passesDone = j->nextGather;
+    }
+    s->done.notify_all();
+}
+
+// Removes a job from its lane's lists and the job table, as it ends.
+void endJob(gnd_hip_server* s, SrvJob* j) {
+    if (j->lane >= 0) {
+        auto& v = s->lanes[j->lane].jobs;
+        v.erase(std::remove(v.begin(), v.end(), j), v.end());
+        auto& f = s->lanes[j->lane].freshJobs;
+        f.erase(std::remove(f.begin(), f.end(), j), f.end());
+    }
+    if (j->slot >= 0) s->freeJobSlots.push_back(j->slot);
+    j->slot = -1;
+}
+
+void releaseChunk(gnd_hip_server* s, int c) {
+    SrvChunk& ch = s->chunks[c];
+    giveSlots(s, ch.slotBase, ch.npix);
+    if (ch.job) ch.job->chunksOut--;
+    ch = SrvChunk{};
+    s->freeChunks.push_back(c);
+}
+
+bool laneIdle(const SrvLane& L) {
+    return !L.inFlight && L.live == 0 && L.running.empty() && L.jobs.empty() && L.fresh.empty();
+}
+
+// The lane of scene `sc`: the one rendering it, else an idle lane (the scene
+// is made resident first), else -1.
+int laneFor(gnd_hip_server* s, int sc) {
+    if (s->scenes[sc].lane >= 0) return s->scenes[sc].lane;
+    for (int l = 0; l < (int)s->lanes.size(); ++l) {
+        SrvLane& L = s->lanes[l];
+        if (!laneIdle(L)) continue;
+        if (L.scene >= 0) s->scenes[L.scene].lane = -1;
+        L.scene = -1;
+        makeResident(s, sc);
+        L.scene = sc;
+        s->scenes[sc].lane = l;
+        return l;
+    }
+    return -1;
+}
+
+// Admission between bounces.  Passes go to waiting jobs one per job per
+// round, in submission order, until no more fit, so a job that arrives while
+// a long one runs gets slots as soon as passes retire.  A job waits (without
+// holding back the others) while its scene has no lane, its lane is
+// mid-bounce or its pass does not fit; once the oldest waiting job has found
+// no room for longer than kSrvReserve, later jobs stop taking slots until it
+// fits (a large pass is not starved by small ones).
+constexpr auto kSrvReserve = std::chrono::milliseconds(20);
+
+void admit(gnd_hip_server* s, std::vector<SrvEvent>& evs) {
+    const auto now = SrvClock::now();
+    auto drop = [&](SrvJob* j) { s->pending.erase(std::remove(s->pending.begin(), s->pending.end(), j), s->pending.end()); };
+    std::vector<SrvJob*> gone;
+    for (SrvJob* j : s->pending) {
+        if (!j->failCode) continue;
+        // cancelled or failed: no more passes; a job with a lane ends there
+        // once its passes in flight are dropped
+        if (j->lane < 0) evs.push_back({j, j->failCode, now});
+        gone.push_back(j);
+    }
+    for (SrvJob* j : gone) drop(j);
+    bool reserve = false;
+    for (bool any = true; any && !reserve;) {
+        any = false;
+        gone.clear();
+        bool oldest = true;
+        for (SrvJob* j : s->pending) {
+            const bool first = oldest;
+            oldest = false;
+            int l = j->lane;
+            if (l < 0) {
+                try {
+                    l = laneFor(s, j->scene);
+                } catch (const HipError& e) {
+                    s->ctx->err = e.what();
+                    evs.push_back({j, e.code, now});
+                    gone.push_back(j);
+                    continue;
+                }
+                if (l < 0 || s->freeJobSlots.empty()) continue;
+            }
+            SrvLane& L = s->lanes[l];
+            if (L.inFlight) continue;
+            const uint32_t npix = (uint32_t)j->job.camera.width * (uint32_t)j->job.camera.height;
             uint32_t base = 0;
-            if (s->freeChunks.empty() || !takeSlots(s, npix, base)) return fresh;
+            if (s->freeChunks.empty() || !takeSlots(s, npix, base)) {
+                if (j->blockedSince == SrvClock::time_point{}) j->blockedSince = now;
+                else if (first && now - j->blockedSince > kSrvReserve) {
+                    reserve = true;
+                    break;
+                }
+                continue;
+            }
+            j->blockedSince = SrvClock::time_point{};
+            if (j->lane < 0) {
+                if (j->job.film_aov && !s->wb.aovBuf) {
+                    ensureAovBuffers(s->wb);
+                    publishResidency(s);
+                }
+                j->lane = l;
+                j->slot = s->freeJobSlots.back();
+                s->freeJobSlots.pop_back();
+                L.jobs.push_back(j);
+                L.freshJobs.push_back(j);
+                evs.push_back({j, 1, now});
+            }
             const int c = s->freeChunks.back();
             s->freeChunks.pop_back();
-            s->chunks[c] = SrvChunk{t, j.nextPass, base, npix, true};
-            s->hLive[c] = jobDepth(j.job) > 0 ? npix : 0u;
-            fresh.push_back(c);
-            j.nextPass++;
+            s->chunks[c] = SrvChunk{j, j->nextPass, base, npix};
+            j->chunksOut++;
+            L.fresh.push_back(c);
+            j->nextPass++;
+            any = true;
+            if (j->nextPass == j->job.spp) gone.push_back(j);
         }
-        s->queue.pop_front();
+        for (SrvJob* j : gone) drop(j);
     }
-    return fresh;
 }
 
-WaveArgs serverArgs(gnd_hip_server* s, const BouncePlan& K) {
+WaveArgs laneArgs(gnd_hip_server* s, const SrvLane& L) {
     WaveArgs A{};
-    A.S = s->scenes[s->active].dev->view;
+    A.S = s->scenes[L.scene].dev->view;
     A.jobs = s->dJobs;
     A.P = s->wb.P;
-    A.Hq = s->wb.Hq;
-    A.Sh = s->wb.Sh;
-    A.counts = s->wb.counts;
+    A.Hq = L.Q.Hq;
+    A.Sh = L.Q.Sh;
+    A.counts = L.Q.counts;
     A.chunkLive = s->dLive;
-    A.ctr = s->dCounters;
-    A.pathOnlyFast = K.fastShadow ? 1 : 0;
-    A.split = K.split ? 1 : 0;
-    A.refill = K.refill;
-    A.srefill = K.srefill;
+    A.ctr = L.dCtr;
+    A.pathOnlyFast = L.K.fastShadow ? 1 : 0;
+    A.split = L.K.split ? 1 : 0;
+    A.refill = L.K.refill;
+    A.srefill = L.K.srefill;
+    A.stackCap = L.K.stackCap;
     return A;
 }
 
-// One engine iteration; called with the server lock held.
-void serverStep(gnd_hip_server* s) {
+void ensureStage(SrvLane& L, size_t bytes) {
+    if (bytes <= L.stageCap) return;
+    if (L.hStage) GND_CHECK(hipHostFree(L.hStage));
+    L.hStage = nullptr;
+    L.stageCap = 0;
+    const size_t n = std::max(bytes, L.stageCap * 2);
+    GND_CHECK(hipHostMalloc((void**)&L.hStage, n, hipHostMallocDefault));
+    L.stageCap = n;
+}
+
+// Launches one step of an idle lane on its stream: records and film clears of
+// newly admitted jobs, camera rays of newly admitted passes, one bounce over
+// every live path, and the read-back of the counts; no host wait.
+void launchLane(gnd_hip_server* s, SrvLane& L) {
     gnd_hip_ctx* ctx = s->ctx;
-    std::lock_guard<std::mutex> g(ctx->mu);
-    GND_CHECK(hipSetDevice(ctx->device));
-    if (s->running.empty() && s->live == 0) {
-        if (s->queue.empty()) return;
-        s->active = s->jobs[s->queue.front()].scene;
-        makeResident(s, s->active);
-    }
-    s->scenes[s->active].lastUse = ++s->useClock;
-    // the active scene decides the bounce kernels (every job of the batch renders it)
     bool anyVol = false;
-    for (int c : s->running)
-        if (s->jobs[s->chunks[c].ticket].job.integrator == GND_INTEGRATOR_VOLPATH) anyVol = true;
-    for (uint64_t t : s->queue)
-        if (s->jobs[t].scene == s->active && s->jobs[t].job.integrator == GND_INTEGRATOR_VOLPATH) anyVol = true;
-    const BouncePlan K = makeBouncePlan(ctx, s->scenes[s->active].dev, anyVol);
-    WaveArgs A = serverArgs(s, K);
-    // 1. admission: camera rays of the new passes join the in-queue
-    const std::vector<int> fresh = admit(s);
-    if (!fresh.empty()) {
-        std::vector<DChunk> dc;
+    for (SrvJob* j : L.jobs) anyVol = anyVol || j->job.integrator == GND_INTEGRATOR_VOLPATH;
+    const gnd_hip_scene* scene = s->scenes[L.scene].dev;
+    L.K = makeBouncePlan(ctx, scene, anyVol);
+    WaveArgs A = laneArgs(s, L);
+    if (!L.fresh.empty()) {
+        const size_t nj = L.freshJobs.size(), nc = L.fresh.size();
+        ensureStage(L, nj * sizeof(DJob) + nc * (sizeof(DChunk) + sizeof(uint32_t)) + 64);
+        DJob* sj = reinterpret_cast<DJob*>(L.hStage);
+        uint32_t* sl = reinterpret_cast<uint32_t*>(sj + nj);
+        DChunk* sc = reinterpret_cast<DChunk*>(sl + nc + (nc & 1));
+        for (size_t k = 0; k < nj; ++k) {
+            SrvJob* j = L.freshJobs[k];
+            sj[k] = makeDJob(j->job);
+            GND_CHECK(hipMemcpyAsync(s->dJobs + j->slot, &sj[k], sizeof(DJob), hipMemcpyHostToDevice, L.stream));
+            clearFilm(L.stream, sj[k]);
+        }
         uint32_t n = 0, nLive = 0;
-        for (int c : fresh) {
+        for (size_t k = 0; k < nc; ++k) {
+            const int c = L.fresh[k];
             const SrvChunk& ch = s->chunks[c];
-            dc.push_back({ch.slotBase, ch.npix, s->jobs[ch.ticket].slot, (int32_t)ch.pass, n, c, nLive});
+            const bool lives = jobDepth(ch.job->job) > 0;
+            sc[k] = DChunk{ch.slotBase, ch.npix, ch.job->slot, (int32_t)ch.pass, n, c, nLive};
             n += ch.npix;
-            if (jobDepth(s->jobs[ch.ticket].job) > 0) nLive += ch.npix;
-            GND_CHECK(hipMemcpyAsync(s->dLive + c, &s->hLive[c], sizeof(uint32_t), hipMemcpyHostToDevice,
-                                     ctx->stream));
-            s->running.push_back(c);
+            if (lives) nLive += ch.npix;
+            sl[k] = lives ? ch.npix : 0u;
+            GND_CHECK(hipMemcpyAsync(s->dLive + c, &sl[k], sizeof(uint32_t), hipMemcpyHostToDevice, L.stream));
+            L.running.push_back(c);
         }
-        ensureDev(s->dCam, s->camCap, dc.size());
-        GND_CHECK(hipMemcpyAsync(s->dCam, dc.data(), dc.size() * sizeof(DChunk), hipMemcpyHostToDevice,
-                                 ctx->stream));
-        A.chunks = s->dCam;
-        A.nChunks = (int32_t)dc.size();
+        ensureDev(L.dCam, L.camCap, nc);
+        GND_CHECK(hipMemcpyAsync(L.dCam, sc, nc * sizeof(DChunk), hipMemcpyHostToDevice, L.stream));
+        A.chunks = L.dCam;
+        A.nChunks = (int32_t)nc;
         A.nSlots = n;
-        bindQueues(A, s->wb, s->cur);
-        A.camQueueBase = s->live;  // after the survivors of the last bounce
-        hipLaunchKernelGGL(kCamera, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, A);
+        bindQueues(A, L.Q, L.cur);
+        A.camQueueBase = L.live;  // after the survivors of the last bounce
+        hipLaunchKernelGGL(kCamera, dim3((n + 255) / 256), dim3(256), 0, L.stream, A);
         GND_CHECK(hipGetLastError());
-        s->live += nLive;
-        hipLaunchKernelGGL(kFillU32, dim3(1), dim3(64), 0, ctx->stream, s->wb.counts, (size_t)1, s->live);
+        L.live += nLive;
+        hipLaunchKernelGGL(kFillU32, dim3(1), dim3(64), 0, L.stream, L.Q.counts, (size_t)1, L.live);
+        L.fresh.clear();
+        L.freshJobs.clear();
     }
-    // 2. one bounce for every live path
-    if (s->live > 0) {
-        bindQueues(A, s->wb, s->cur);
-        launchBounce(ctx, A, K, s->live, nullptr);
-        s->cur = 1 - s->cur;
+    if (L.live > 0) {
+        bindQueues(A, L.Q, L.cur);
+        launchBounce(L.stream, A, L.K, L.live, nullptr);
+        L.cur = 1 - L.cur;
     }
-    // 3. retirement: passes whose paths have all ended
-    int used = 0;  // chunk ids are handed out lowest first
-    for (int c : s->running) used = std::max(used, c + 1);
-    GND_CHECK(hipMemcpyAsync(&s->live, s->wb.counts, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    uint32_t used = 0;  // chunk ids are handed out lowest first
+    for (int c : L.running) used = std::max(used, (uint32_t)c + 1u);
+    L.readChunks = used;
+    GND_CHECK(hipMemcpyAsync(L.hRead, L.Q.counts, sizeof(uint32_t), hipMemcpyDeviceToHost, L.stream));
+    GND_CHECK(hipMemcpyAsync(L.hRead + 2, &L.dCtr->stackOverflow, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                             L.stream));
     if (used)
-        GND_CHECK(hipMemcpyAsync(s->hLive.data(), s->dLive, (size_t)used * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                                 ctx->stream));
-    GND_CHECK(hipStreamSynchronize(ctx->stream));
-    std::vector<int> still;
-    for (int c : s->running) {
-        SrvChunk& ch = s->chunks[c];
-        if (s->hLive[c] != 0) {
-            still.push_back(c);
-            continue;
-        }
-        ch.running = false;
-        s->jobs[ch.ticket].finished[ch.pass] = c;
+        GND_CHECK(hipMemcpyAsync(L.hRead + kLaneReadHead, s->dLive, (size_t)used * sizeof(uint32_t),
+                                 hipMemcpyDeviceToHost, L.stream));
+    GND_CHECK(hipEventRecord(L.done, L.stream));
+    L.inFlight = true;
+}
+
+// After a lane's step completed: stack-overflow check, retirement, the film
+// gather of finished passes in pass order, and the end of finished jobs.
+void finishLane(gnd_hip_server* s, SrvLane& L, std::vector<SrvEvent>& evs) {
+    L.inFlight = false;
+    L.live = L.hRead[0];
+    unsigned long long overflow = 0;
+    std::memcpy(&overflow, L.hRead + 2, sizeof(overflow));
+    if (overflow > L.overflowSeen) {
+        // as gnd_hip_render: the images of the jobs this bounce served are not valid
+        L.overflowSeen = overflow;
+        for (SrvJob* j : L.jobs)
+            if (!j->failCode) j->failCode = GND_E_STACK;
+        s->ctx->err = "traversal stack overflow";
     }
-    s->running.swap(still);
-    // 4. gather finished passes into the films in pass order
+    std::vector<int> still;
+    for (int c : L.running) {
+        if ((uint32_t)c < L.readChunks && L.hRead[kLaneReadHead + c] == 0) {
+            SrvChunk& ch = s->chunks[c];
+            ch.job->finished[ch.pass] = c;
+        } else {
+            still.push_back(c);
+        }
+    }
+    L.running.swap(still);
+    // gather finished passes in pass order; a failing job's passes are dropped
     std::vector<DChunk> gc;
     std::vector<DSplatTask> tasks;
-    std::vector<uint64_t> completed;
     std::vector<int> released;
     uint32_t pix = 0;
-    for (auto& kv : s->jobs) {
-        SrvJob& j = kv.second;
-        if (j.state != 1) continue;
-        const int first = (int)gc.size();
-        for (auto it = j.finished.find(j.nextGather); it != j.finished.end(); it = j.finished.find(j.nextGather)) {
-            const SrvChunk& ch = s->chunks[it->second];
-            gc.push_back({ch.slotBase, ch.npix, j.slot, (int32_t)ch.pass, 0u, it->second, 0u});
-            released.push_back(it->second);
-            j.finished.erase(it);
-            j.nextGather++;
-        }
-        if ((int)gc.size() > first) {
-            tasks.push_back({pix, j.slot, first, (int32_t)gc.size()});
-            pix += (uint32_t)j.job.camera.width * (uint32_t)j.job.camera.height;
-        }
-        if (j.nextGather == j.job.spp) completed.push_back(kv.first);
-    }
-    if (!tasks.empty()) {
-        ensureDev(s->dGather, s->gatherCap, gc.size());
-        ensureDev(s->dTasks, s->taskCap, tasks.size());
-        GND_CHECK(hipMemcpyAsync(s->dGather, gc.data(), gc.size() * sizeof(DChunk), hipMemcpyHostToDevice,
-                                 ctx->stream));
-        GND_CHECK(hipMemcpyAsync(s->dTasks, tasks.data(), tasks.size() * sizeof(DSplatTask),
-                                 hipMemcpyHostToDevice, ctx->stream));
-        A.chunks = s->dGather;
-        A.nChunks = (int32_t)gc.size();
-        hipLaunchKernelGGL(kSplat, dim3((pix + 255) / 256), dim3(256), 0, ctx->stream, A,
-                           (const DSplatTask*)s->dTasks, (int)tasks.size(), pix);
-        GND_CHECK(hipGetLastError());
-        GND_CHECK(hipStreamSynchronize(ctx->stream));
-    }
-    for (int c : released) {
-        giveSlots(s, s->chunks[c].slotBase, s->chunks[c].npix);
-        s->chunks[c] = SrvChunk{};
-        s->freeChunks.push_back(c);
-    }
-    const auto now = std::chrono::steady_clock::now();
-    for (uint64_t t : completed) {
-        SrvJob& j = s->jobs[t];
-        j.state = 2;
-        j.tDone = now;
-        s->freeJobSlots.push_back(j.slot);
-        j.slot = -1;
-        s->scenes[j.scene].running--;
-    }
-    if (!completed.empty()) s->done.notify_all();
-}
-
-bool serverHasWork(const gnd_hip_server* s) { return !s->queue.empty() || !s->running.empty() || s->live > 0; }
-
-void serverLoop(gnd_hip_server* s) {
-    std::unique_lock<std::mutex> lk(s->mu);
-    while (!s->stop) {
-        if (!serverHasWork(s)) {
-            s->cv.wait(lk, [&] { return s->stop || serverHasWork(s); });
+    for (SrvJob* j : L.jobs) {
+        if (j->failCode) {
+            for (auto& kv : j->finished) released.push_back(kv.second);
+            j->finished.clear();
             continue;
         }
-        try {
-            serverStep(s);
-        } catch (const std::exception& e) {
-            // a device error fails every job in flight; the queue stays
-            s->err = e.what();
-            {
-                std::lock_guard<std::mutex> g(s->ctx->mu);
-                s->ctx->err = e.what();
-            }
-            const int code = dynamic_cast<const HipError*>(&e) ? static_cast<const HipError&>(e).code : GND_E_DEVICE;
-            for (auto& kv : s->jobs)
-                if (kv.second.state == 1) failJob(s, kv.first, code);
-            for (uint64_t t : s->queue)
-                if (s->jobs[t].scene == s->active) failJob(s, t, code);
-            std::deque<uint64_t> keep;
-            for (uint64_t t : s->queue)
-                if (s->jobs[t].state == 0) keep.push_back(t);
-            s->queue.swap(keep);
-            for (int c = 0; c < (int)s->chunks.size(); ++c)
-                if (s->chunks[c].ticket) {
-                    giveSlots(s, s->chunks[c].slotBase, s->chunks[c].npix);
-                    s->chunks[c] = SrvChunk{};
-                    s->freeChunks.push_back(c);
-                }
-            for (SrvScene& sc : s->scenes) sc.running = 0;
-            s->running.clear();
-            s->live = 0;
-            hipLaunchKernelGGL(kFillU32, dim3(1), dim3(64), 0, s->ctx->stream, s->wb.counts, (size_t)kNumCounts, 0u);
-            s->done.notify_all();
+        const int first = (int)gc.size();
+        for (auto it = j->finished.find(j->nextGather); it != j->finished.end();
+             it = j->finished.find(j->nextGather)) {
+            const SrvChunk& ch = s->chunks[it->second];
+            gc.push_back({ch.slotBase, ch.npix, j->slot, (int32_t)ch.pass, 0u, it->second, 0u});
+            released.push_back(it->second);
+            j->finished.erase(it);
+            j->nextGather++;
         }
+        if ((int)gc.size() > first) {
+            tasks.push_back({pix, j->slot, first, (int32_t)gc.size()});
+            pix += (uint32_t)j->job.camera.width * (uint32_t)j->job.camera.height;
+            if (j->nextGather < j->job.spp) evs.push_back({j, kSrvProgress, SrvClock::now()});
+        }
+    }
+    if (!tasks.empty()) {
+        const size_t bytes = gc.size() * sizeof(DChunk) + tasks.size() * sizeof(DSplatTask);
+        ensureStage(L, bytes);
+        std::memcpy(L.hStage, gc.data(), gc.size() * sizeof(DChunk));
+        std::memcpy(L.hStage + gc.size() * sizeof(DChunk), tasks.data(), tasks.size() * sizeof(DSplatTask));
+        ensureDev(L.dGather, L.gatherCap, gc.size());
+        ensureDev(L.dTasks, L.taskCap, tasks.size());
+        GND_CHECK(hipMemcpyAsync(L.dGather, L.hStage, gc.size() * sizeof(DChunk), hipMemcpyHostToDevice, L.stream));
+        GND_CHECK(hipMemcpyAsync(L.dTasks, L.hStage + gc.size() * sizeof(DChunk), tasks.size() * sizeof(DSplatTask),
+                                 hipMemcpyHostToDevice, L.stream));
+        WaveArgs A = laneArgs(s, L);
+        A.chunks = L.dGather;
+        A.nChunks = (int32_t)gc.size();
+        hipLaunchKernelGGL(kSplat, dim3((pix + 255) / 256), dim3(256), 0, L.stream, A, (const DSplatTask*)L.dTasks,
+                           (int)tasks.size(), pix);
+        GND_CHECK(hipGetLastError());
+        GND_CHECK(hipStreamSynchronize(L.stream));
+    }
+    for (int c : released) releaseChunk(s, c);
+    const auto now = SrvClock::now();
+    std::vector<SrvJob*> ended;
+    for (SrvJob* j : L.jobs) {
+        if (j->failCode ? j->chunksOut == 0 : j->nextGather == j->job.spp) ended.push_back(j);
+    }
+    for (SrvJob* j : ended) {
+        if (j->failCode) {
+            // passes not admitted yet are dropped with the job
+            s->pending.erase(std::remove(s->pending.begin(), s->pending.end(), j), s->pending.end());
+        }
+        endJob(s, j);
+        evs.push_back({j, j->failCode ? j->failCode : 2, now});
+    }
+    if (laneIdle(L) && L.scene >= 0) {
+        s->scenes[L.scene].lane = -1;
+        L.scene = -1;
     }
 }
 
-void statusOf(const SrvJob& j, gnd_hip_job_status* st) {
+// A device error: every job with passes in flight fails, the lanes are reset
+// and their slots returned; queued jobs stay queued.
+void failInFlight(gnd_hip_server* s, int code, std::vector<SrvEvent>& evs) {
+    const auto now = SrvClock::now();
+    for (SrvLane& L : s->lanes) {
+        (void)hipStreamSynchronize(L.stream);
+        std::vector<SrvJob*> js = L.jobs;
+        for (SrvJob* j : js) {
+            j->failCode = code;
+            s->pending.erase(std::remove(s->pending.begin(), s->pending.end(), j), s->pending.end());
+            endJob(s, j);
+            evs.push_back({j, code, now});
+        }
+        L.jobs.clear();
+        L.running.clear();
+        L.fresh.clear();
+        L.freshJobs.clear();
+        L.live = 0;
+        L.cur = 0;
+        L.inFlight = false;
+        hipLaunchKernelGGL(kFillU32, dim3(1), dim3(64), 0, L.stream, L.Q.counts, (size_t)kNumCounts, 0u);
+        (void)hipStreamSynchronize(L.stream);
+        if (L.scene >= 0) s->scenes[L.scene].lane = -1;
+        L.scene = -1;
+    }
+    for (int c = 0; c < (int)s->chunks.size(); ++c)
+        if (s->chunks[c].job) releaseChunk(s, c);
+}
+
+bool workerBusy(const gnd_hip_server* s) {
+    if (!s->pending.empty()) return true;
+    for (const SrvLane& L : s->lanes)
+        if (!laneIdle(L)) return true;
+    return false;
+}
+
+void serverLoop(gnd_hip_server* s) {
+    gnd_hip_ctx* ctx = s->ctx;
+    (void)hipSetDevice(ctx->device);
+    for (;;) {
+        std::vector<SrvEvent> evs;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            if (!s->stop && s->inbox.empty() && s->cancels.empty() && !workerBusy(s))
+                s->cv.wait(lk, [&] { return s->stop || !s->inbox.empty() || !s->cancels.empty(); });
+            if (s->stop) break;
+            for (SrvJob* j : s->inbox) s->pending.push_back(j);
+            s->inbox.clear();
+            for (uint64_t t : s->cancels) {
+                auto it = s->jobs.find(t);
+                if (it == s->jobs.end()) continue;
+                SrvJob* j = it->second.get();
+                if ((j->state == 0 || j->state == 1) && !j->failCode) j->failCode = GND_E_CANCELLED;
+            }
+            s->cancels.clear();
+        }
+        try {
+            {
+                std::lock_guard<std::mutex> g(ctx->mu);
+                admit(s, evs);
+                // cancelled jobs of a lane between bounces that hold no chunks end here
+                for (SrvLane& L : s->lanes) {
+                    if (L.inFlight) continue;
+                    std::vector<SrvJob*> ended;
+                    for (SrvJob* j : L.jobs)
+                        if (j->failCode && j->chunksOut == 0) ended.push_back(j);
+                    for (SrvJob* j : ended) {
+                        endJob(s, j);
+                        evs.push_back({j, j->failCode, SrvClock::now()});
+                    }
+                }
+                for (SrvLane& L : s->lanes) {
+                    if (L.scene >= 0 && !L.inFlight && (!L.fresh.empty() || L.live > 0)) launchLane(s, L);
+                    if (laneIdle(L) && L.scene >= 0) {
+                        s->scenes[L.scene].lane = -1;
+                        L.scene = -1;
+                    }
+                }
+            }
+            publish(s, evs);
+            evs.clear();
+            // wait for the first lane to finish its step
+            bool any = false, finished = false;
+            for (SrvLane& L : s->lanes) {
+                if (!L.inFlight) continue;
+                any = true;
+                const hipError_t q = hipEventQuery(L.done);
+                if (q == hipErrorNotReady) continue;
+                GND_CHECK(q);
+                std::lock_guard<std::mutex> g(ctx->mu);
+                finishLane(s, L, evs);
+                finished = true;
+            }
+            publish(s, evs);
+            if (any && !finished) std::this_thread::sleep_for(std::chrono::microseconds(20));
+        } catch (const std::exception& e) {
+            const int code = dynamic_cast<const HipError*>(&e) ? static_cast<const HipError&>(e).code : GND_E_DEVICE;
+            std::vector<SrvEvent> fail;
+            {
+                std::lock_guard<std::mutex> g(ctx->mu);
+                ctx->err = e.what();
+                failInFlight(s, code, fail);
+            }
+            publish(s, evs);
+            publish(s, fail);
+        }
+    }
+    for (SrvLane& L : s->lanes) (void)hipStreamSynchronize(L.stream);
+}
+
+void statusOf(const gnd_hip_server* s, const SrvJob& j, gnd_hip_job_status* st) {
     using ms = std::chrono::duration<double, std::milli>;
+    const bool admitted = j.state != 0, ended = j.state == 2 || j.state < 0;
     st->state = j.state;
-    st->queue_ms = j.state >= 1 || j.state < 0 ? ms(j.tAdmit - j.tSubmit).count() : 0.0;
+    st->passes_done = j.passesDone;
+    st->queue_ms = admitted ? ms(j.tAdmit - j.tSubmit).count() : 0.0;
     st->run_ms = j.state == 2 ? ms(j.tDone - j.tAdmit).count() : 0.0;
-    st->latency_ms = j.state == 2 || j.state < 0 ? ms(j.tDone - j.tSubmit).count() : 0.0;
-    st->passes_done = j.nextGather;
+    st->latency_ms = ended ? ms(j.tDone - j.tSubmit).count() : 0.0;
+    st->t_submit_ms = msSince(s, j.tSubmit);
+    st->t_admit_ms = admitted ? msSince(s, j.tAdmit) : -1.0;
+    st->t_done_ms = ended ? msSince(s, j.tDone) : -1.0;
+}
+
+void destroyLanes(gnd_hip_server* s) {
+    for (SrvLane& L : s->lanes) {
+        if (L.stream) (void)hipStreamSynchronize(L.stream);
+        freeQueueSet(L.Q);
+        for (void* p : {(void*)L.dCtr, (void*)L.dCam, (void*)L.dGather, (void*)L.dTasks})
+            if (p) (void)hipFree(p);
+        if (L.hStage) (void)hipHostFree(L.hStage);
+        if (L.hRead) (void)hipHostFree(L.hRead);
+        if (L.done) (void)hipEventDestroy(L.done);
+        if (L.stream) (void)hipStreamDestroy(L.stream);
+    }
+    s->lanes.clear();
 }
 
 }  // namespace
@@ -3090,37 +3442,49 @@ void statusOf(const SrvJob& j, gnd_hip_job_status* st) {
 extern "C" {
 
 int gnd_hip_server_create(gnd_hip_ctx* ctx, const gnd_hip_server_config* cfg, gnd_hip_server** out) {
-    if (!ctx || !cfg || !out || cfg->path_slots < 1024 || cfg->max_jobs < 1) return GND_E_INVALID;
+    if (!ctx || !cfg || !out || cfg->path_slots < 1024 || cfg->max_jobs < 1 || cfg->lanes > kSrvMaxLanes)
+        return GND_E_INVALID;
     gnd_hip_server* s = new gnd_hip_server();
     s->ctx = ctx;
     s->cfg = *cfg;
+    s->t0 = SrvClock::now();
     if (s->cfg.hbm_budget == 0) s->cfg.hbm_budget = ctx->hbm / 10 * 9;
-    s->pathBytes = waveBufferBytes(cfg->path_slots);
+    if (s->cfg.lanes == 0) s->cfg.lanes = kSrvDefaultLanes;
+    const size_t slots = cfg->path_slots;
+    s->pathBytes = pathStateBytes(slots) + s->cfg.lanes * queueSetBytes(slots);
     int rc = guarded(ctx, [&] {
         std::lock_guard<std::mutex> g(ctx->mu);
         GND_CHECK(hipSetDevice(ctx->device));
         if (s->pathBytes > s->cfg.hbm_budget) throw HipError(GND_E_NOMEM, "path slots exceed the HBM budget");
-        ensureWaveBuffers(s->wb, cfg->path_slots);
+        ensureWaveBuffers(s->wb, slots, false);
         GND_CHECK(hipMalloc(&s->dJobs, (size_t)cfg->max_jobs * sizeof(DJob)));
         GND_CHECK(hipMalloc(&s->dLive, (size_t)kSrvMaxChunks * sizeof(uint32_t)));
-        GND_CHECK(hipMalloc(&s->dCounters, sizeof(Counters)));
-        GND_CHECK(hipMemset(s->dCounters, 0, sizeof(Counters)));
-        hipLaunchKernelGGL(kFillU32, dim3(1), dim3(64), 0, ctx->stream, s->wb.counts, (size_t)kNumCounts, 0u);
-        GND_CHECK(hipStreamSynchronize(ctx->stream));
+        s->lanes.resize(s->cfg.lanes);
+        for (SrvLane& L : s->lanes) {
+            GND_CHECK(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
+            GND_CHECK(hipEventCreateWithFlags(&L.done, hipEventDisableTiming));
+            ensureQueueSet(L.Q, slots);
+            GND_CHECK(hipMalloc(&L.dCtr, sizeof(Counters)));
+            GND_CHECK(hipMemsetAsync(L.dCtr, 0, sizeof(Counters), L.stream));
+            GND_CHECK(hipHostMalloc((void**)&L.hRead, (kLaneReadHead + (size_t)kSrvMaxChunks) * sizeof(uint32_t),
+                                    hipHostMallocDefault));
+            hipLaunchKernelGGL(kFillU32, dim3(1), dim3(64), 0, L.stream, L.Q.counts, (size_t)kNumCounts, 0u);
+            GND_CHECK(hipStreamSynchronize(L.stream));
+        }
     });
     if (rc != GND_OK) {
+        destroyLanes(s);
         freeWaveBuffers(s->wb);
         if (s->dJobs) (void)hipFree(s->dJobs);
         if (s->dLive) (void)hipFree(s->dLive);
-        if (s->dCounters) (void)hipFree(s->dCounters);
         delete s;
         return rc;
     }
     for (int i = (int)cfg->max_jobs - 1; i >= 0; --i) s->freeJobSlots.push_back(i);
     s->chunks.resize(kSrvMaxChunks);
-    s->hLive.assign(kSrvMaxChunks, 0u);
     for (int c = kSrvMaxChunks - 1; c >= 0; --c) s->freeChunks.push_back(c);
     s->freeSlots[0] = cfg->path_slots;
+    s->residentBytes = residentBytesOf(s);
     s->worker = std::thread(serverLoop, s);
     *out = s;
     return GND_OK;
@@ -3136,18 +3500,20 @@ void gnd_hip_server_destroy(gnd_hip_server* s) {
     if (s->worker.joinable()) s->worker.join();
     gnd_hip_ctx* ctx = s->ctx;
     (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
+    destroyLanes(s);
     for (SrvScene& sc : s->scenes)
         if (sc.dev) gnd_hip_scene_free(ctx, sc.dev);
     freeWaveBuffers(s->wb);
-    for (void* p : {(void*)s->dJobs, (void*)s->dLive, (void*)s->dCam, (void*)s->dGather, (void*)s->dTasks,
-                    (void*)s->dCounters})
+    for (void* p : {(void*)s->dJobs, (void*)s->dLive})
         if (p) (void)hipFree(p);
     delete s;
 }
 
 int gnd_hip_server_add_scene(gnd_hip_server* s, const gnd_hip_scene_arrays* arrays, int32_t* scene_id) {
     if (!s || !arrays || !arrays->desc || !scene_id) return GND_E_INVALID;
+    // the worker reads the scene list without the lock: pause it while the
+    // list may reallocate (it holds the context lock across each step)
+    std::lock_guard<std::mutex> g(s->ctx->mu);
     std::lock_guard<std::mutex> lk(s->mu);
     SrvScene sc;
     sc.arrays = arrays;
@@ -3157,31 +3523,25 @@ int gnd_hip_server_add_scene(gnd_hip_server* s, const gnd_hip_scene_arrays* arra
 }
 
 int gnd_hip_server_submit(gnd_hip_server* s, int32_t scene_id, const gnd_hip_job* job, uint64_t* ticket) {
+    const auto arrival = SrvClock::now();  // before any lock: latency runs from here
     if (!s || !job || !ticket || job->spp == 0 || !job->film || job->camera.width <= 0 || job->camera.height <= 0 ||
         job->integrator < GND_INTEGRATOR_PATH || job->integrator > GND_INTEGRATOR_AO)
         return GND_E_INVALID;
-    std::lock_guard<std::mutex> lk(s->mu);
-    if (scene_id < 0 || scene_id >= (int32_t)s->scenes.size()) return GND_E_INVALID;
     if ((uint64_t)job->camera.width * (uint64_t)job->camera.height > s->cfg.path_slots) {
+        std::lock_guard<std::mutex> g(s->ctx->mu);
         s->ctx->err = "one pass of the job needs more path slots than the server has";
         return GND_E_NOMEM;
     }
-    if (job->film_aov && !s->wb.aovBuf) {
-        // AOV records for the slot pool (the worker is not inside a step while
-        // the server lock is held)
-        std::lock_guard<std::mutex> g(s->ctx->mu);
-        const int rc = guarded(s->ctx, [&] {
-            GND_CHECK(hipSetDevice(s->ctx->device));
-            ensureAovBuffers(s->wb);
-        });
-        if (rc != GND_OK) return rc;
-    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (scene_id < 0 || scene_id >= (int32_t)s->scenes.size()) return GND_E_INVALID;
     const uint64_t t = s->nextTicket++;
-    SrvJob& j = s->jobs[t];
-    j.scene = scene_id;
-    j.job = *job;
-    j.tSubmit = std::chrono::steady_clock::now();
-    s->queue.push_back(t);
+    auto j = std::make_unique<SrvJob>();
+    j->ticket = t;
+    j->scene = scene_id;
+    j->job = *job;
+    j->tSubmit = arrival;
+    s->inbox.push_back(j.get());
+    s->jobs.emplace(t, std::move(j));
     *ticket = t;
     s->cv.notify_all();
     return GND_OK;
@@ -3192,7 +3552,7 @@ int gnd_hip_server_poll(gnd_hip_server* s, uint64_t ticket, gnd_hip_job_status* 
     std::lock_guard<std::mutex> lk(s->mu);
     auto it = s->jobs.find(ticket);
     if (it == s->jobs.end()) return GND_E_INVALID;
-    statusOf(it->second, st);
+    statusOf(s, *it->second, st);
     return GND_OK;
 }
 
@@ -3201,32 +3561,76 @@ int gnd_hip_server_wait(gnd_hip_server* s, uint64_t ticket, double timeout_s, gn
     std::unique_lock<std::mutex> lk(s->mu);
     auto it = s->jobs.find(ticket);
     if (it == s->jobs.end()) return GND_E_INVALID;
-    const auto until = std::chrono::steady_clock::now() + std::chrono::duration<double>(timeout_s);
-    s->done.wait_until(lk, until, [&] { return it->second.state == 2 || it->second.state < 0; });
-    statusOf(it->second, st);
+    SrvJob* j = it->second.get();
+    const double secs = std::min(std::max(0.0, timeout_s), 1e7);  // NaN -> 0
+    const auto until = SrvClock::now() + std::chrono::duration_cast<SrvClock::duration>(
+                                             std::chrono::duration<double>(secs));
+    const bool ended = s->done.wait_until(lk, until, [&] { return j->state == 2 || j->state < 0; });
+    statusOf(s, *j, st);
+    return ended ? GND_OK : GND_E_TIMEOUT;
+}
+
+int gnd_hip_server_cancel(gnd_hip_server* s, uint64_t ticket) {
+    if (!s) return GND_E_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->jobs.find(ticket);
+    if (it == s->jobs.end()) return GND_E_INVALID;
+    SrvJob* j = it->second.get();
+    if (j->state == 0 || j->state == 1) {
+        j->cancelRequested = true;
+        s->cancels.push_back(ticket);
+        s->cv.notify_all();
+    }
+    return GND_OK;
+}
+
+int gnd_hip_server_release(gnd_hip_server* s, uint64_t ticket) {
+    if (!s) return GND_E_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->jobs.find(ticket);
+    if (it == s->jobs.end()) return GND_E_INVALID;
+    if (it->second->state != 2 && it->second->state >= 0) return GND_E_BUSY;
+    s->jobs.erase(it);
     return GND_OK;
 }
 
 int gnd_hip_server_stats(gnd_hip_server* s, gnd_hip_stats* stats, int32_t* resident_scenes, size_t* resident_bytes) {
     if (!s) return GND_E_INVALID;
-    std::lock_guard<std::mutex> lk(s->mu);
+    uint64_t samples = 0;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        samples = s->samplesDone;
+        if (resident_scenes) *resident_scenes = s->residentScenes;
+        if (resident_bytes) *resident_bytes = s->residentBytes;
+    }
+    if (!stats) return GND_OK;
+    std::lock_guard<std::mutex> g(s->ctx->mu);
     return guarded(s->ctx, [&] {
-        std::lock_guard<std::mutex> g(s->ctx->mu);
-        if (stats) {
+        GND_CHECK(hipSetDevice(s->ctx->device));
+        // the lanes' counters summed (kernels in flight may still add to them)
+        Counters sum{};
+        for (SrvLane& L : s->lanes) {
             Counters c;
-            GND_CHECK(hipMemcpyAsync(&c, s->dCounters, sizeof(c), hipMemcpyDeviceToHost, s->ctx->stream));
+            GND_CHECK(hipMemcpyAsync(&c, L.dCtr, sizeof(c), hipMemcpyDeviceToHost, s->ctx->stream));
             GND_CHECK(hipStreamSynchronize(s->ctx->stream));
-            std::memset(stats, 0, sizeof(*stats));
-            uint64_t samples = 0;
-            for (auto& kv : s->jobs)
-                if (kv.second.state == 2)
-                    samples += (uint64_t)kv.second.job.camera.width * kv.second.job.camera.height * kv.second.job.spp;
-            fillStats(stats, c, samples);
+            sum.nodes += c.nodes;
+            sum.packs += c.packs;
+            sum.insts += c.insts;
+            sum.snodes += c.snodes;
+            sum.spacks += c.spacks;
+            sum.sinsts += c.sinsts;
+            sum.nan += c.nan;
+            sum.stackOverflow += c.stackOverflow;
+            sum.maxRayNodes = std::max(sum.maxRayNodes, c.maxRayNodes);
+            sum.badRays += c.badRays;
+            sum.iters += c.iters;
+            sum.siters += c.siters;
+            sum.rays += c.rays;
+            sum.srays += c.srays;
+            sum.launches += c.launches;
         }
-        int n = 0;
-        for (const SrvScene& sc : s->scenes) n += sc.dev ? 1 : 0;
-        if (resident_scenes) *resident_scenes = n;
-        if (resident_bytes) *resident_bytes = residentBytes(s);
+        std::memset(stats, 0, sizeof(*stats));
+        fillStats(stats, sum, samples);
     });
 }
 

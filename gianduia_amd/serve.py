@@ -29,15 +29,21 @@ class Ticket:
 
 
 class Server:
-    """A continuous-batching server on one device."""
+    """A continuous-batching server on one device.
 
-    def __init__(self, dev: Device, path_slots: int = 8_000_000, max_jobs: int = 256, hbm_budget: int = 0):
+    The server keeps each submitted job's device buffers alive until the job
+    has ended (done, failed or cancelled), so a dropped Ticket or a wait that
+    timed out can never free a film the device still writes."""
+
+    def __init__(self, dev: Device, path_slots: int = 8_000_000, max_jobs: int = 256, hbm_budget: int = 0,
+                 lanes: int = 0):
         self.dev = dev
-        cfg = A.gnd_hip_server_config(hbm_budget, path_slots, max_jobs)
+        cfg = A.gnd_hip_server_config(hbm_budget, path_slots, max_jobs, lanes)
         h = C.c_void_p()
         dev._check(_lib().gnd_hip_server_create(dev.ctx, C.byref(cfg), C.byref(h)))
         self.handle = h
         self.scenes: list[HostScene] = []  # keeps the borrowed host arrays alive
+        self._owned: dict[int, tuple] = {}  # ticket -> buffers of a job that has not ended
 
     def add_scene(self, hs: HostScene) -> int:
         arrays = hs.build()
@@ -46,9 +52,14 @@ class Server:
         self.scenes.append(hs)
         return sid.value
 
-    def submit(self, scene_id: int, job: Job, want_aov: bool = False) -> Ticket:
+    def submit(self, scene_id: int, job: Job, want_aov: bool = False, film: DeviceBuffer | None = None) -> Ticket:
+        """Queues `job`; `film` (optional) is a caller's buffer of at least
+        W*H*16 bytes to render into instead of a new one."""
         w, h = job.camera.width, job.camera.height
-        film = self.dev.alloc(w * h * 16)
+        if film is None:
+            film = self.dev.alloc(w * h * 16)
+        elif film.nbytes < w * h * 16:
+            raise ValueError("film buffer too small")
         aov = self.dev.alloc(w * h * 36) if want_aov else None
         cj = A.gnd_hip_job()
         cj.camera = job.camera
@@ -62,20 +73,39 @@ class Server:
         cj.film_aov = aov.addr if aov is not None else None
         t = C.c_uint64()
         self.dev._check(_lib().gnd_hip_server_submit(self.handle, scene_id, C.byref(cj), C.byref(t)))
+        self._owned[t.value] = (film, aov)
         return Ticket(t.value, job, film, time.perf_counter(), aov)
+
+    def _ended(self, tid: int, d: dict) -> dict:
+        if d["state"] == 2 or d["state"] < 0:
+            self._owned.pop(tid, None)
+        return d
 
     def poll(self, t: Ticket) -> dict:
         st = A.gnd_hip_job_status()
         self.dev._check(_lib().gnd_hip_server_poll(self.handle, t.id, C.byref(st)))
-        return st.as_dict()
+        return self._ended(t.id, st.as_dict())
 
     def wait(self, t: Ticket, timeout_s: float = 600.0) -> dict:
+        """Waits for the job to end.  Raises TimeoutError while it is still
+        queued or running, GndError if it failed."""
         st = A.gnd_hip_job_status()
-        self.dev._check(_lib().gnd_hip_server_wait(self.handle, t.id, timeout_s, C.byref(st)))
-        d = st.as_dict()
+        rc = _lib().gnd_hip_server_wait(self.handle, t.id, timeout_s, C.byref(st))
+        if rc == A.GND_E_TIMEOUT:
+            raise TimeoutError(f"job {t.id} not finished after {timeout_s} s (state {st.state})")
+        self.dev._check(rc)
+        d = self._ended(t.id, st.as_dict())
         if d["state"] < 0:
             raise GndError(d["state"], f"job {t.id} failed: {self.dev.last_error()}")
         return d
+
+    def cancel(self, t: Ticket) -> None:
+        """Stops the job (gnd_hip_server_cancel); wait() then reports GND_E_CANCELLED."""
+        self.dev._check(_lib().gnd_hip_server_cancel(self.handle, t.id))
+
+    def release(self, t: Ticket) -> None:
+        """Forgets an ended job on the server side (its status can no longer be read)."""
+        self.dev._check(_lib().gnd_hip_server_release(self.handle, t.id))
 
     def film(self, t: Ticket) -> np.ndarray:
         w, h = t.job.camera.width, t.job.camera.height
@@ -97,9 +127,12 @@ class Server:
         return d
 
     def close(self) -> None:
+        """Stops the worker; buffers of jobs that had not ended are freed only
+        after that (no kernel runs any more)."""
         if self.handle:
             _lib().gnd_hip_server_destroy(self.handle)
             self.handle = None
+        self._owned.clear()
 
     def __del__(self):
         try:
@@ -137,8 +170,11 @@ def random_job(rng: np.random.Generator, hs: HostScene, res=(128, 1920), spp=(4,
 def poisson_load(server: Server, scene_ids: list[int], qps: float, n_jobs: int, seed: int = 0,
                  timeout_s: float = 600.0, **job_kw) -> dict:
     """Submits n_jobs with exponential inter-arrival times (rate qps) on
-    scenes drawn uniformly from the pool; returns latency percentiles (ms,
-    submission to film complete) and throughput."""
+    scenes drawn uniformly from the pool and returns latency percentiles (ms,
+    from each request's scheduled arrival to its film being complete) and
+    throughput.  Films come from a pool allocated before the clock starts, as
+    a serving front end would keep them; a submission that falls behind its
+    arrival time is charged the delay."""
     rng = np.random.default_rng(seed)
     hosts = {sid: server.scenes[sid] for sid in scene_ids}
     plan = []
@@ -147,24 +183,29 @@ def poisson_load(server: Server, scene_ids: list[int], qps: float, n_jobs: int, 
         t += float(rng.exponential(1.0 / qps))
         sid = int(rng.choice(scene_ids))
         plan.append((t, sid, random_job(rng, hosts[sid], **job_kw)))
+    films = [server.dev.alloc(job.camera.width * job.camera.height * 16) for _, _, job in plan]
     tickets = []
+    late = []
     t0 = time.perf_counter()
-    for at, sid, job in plan:
+    for (at, sid, job), film in zip(plan, films):
         delay = at - (time.perf_counter() - t0)
         if delay > 0:
             time.sleep(delay)
-        tickets.append(server.submit(sid, job))
+        late.append(max(0.0, (time.perf_counter() - t0) - at) * 1e3)
+        tickets.append(server.submit(sid, job, film=film))
     lat, queue, samples = [], [], 0
-    for tk in tickets:
+    for tk, lt in zip(tickets, late):
         st = server.wait(tk, timeout_s)
-        lat.append(st["latency_ms"])
+        lat.append(lt + st["latency_ms"])
         queue.append(st["queue_ms"])
         samples += tk.job.camera.width * tk.job.camera.height * tk.job.spp
+        server.release(tk)
     wall = time.perf_counter() - t0
-    for tk in tickets:  # hipFree waits for the device: only after the run
-        tk.film.free()
+    for f in films:  # hipFree waits for the device: only after the run
+        f.free()
     lat = np.array(lat)
     return {"qps_offered": qps, "jobs": n_jobs, "p50_ms": round(float(np.percentile(lat, 50)), 2),
             "p99_ms": round(float(np.percentile(lat, 99)), 2), "mean_queue_ms": round(float(np.mean(queue)), 2),
+            "max_submit_lag_ms": round(float(max(late)), 2),
             "msamples_per_s": round(samples / wall / 1e6, 2), "wall_s": round(wall, 2),
             "qps_achieved": round(n_jobs / wall, 2)}

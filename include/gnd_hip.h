@@ -33,7 +33,9 @@ enum {
     GND_E_DEVICE = -2,    /* HIP runtime error                                  */
     GND_E_NOMEM = -3,     /* device allocation failed / over the HBM budget     */
     GND_E_CANCELLED = -4, /* render stopped by gnd_hip_cancel at a pass boundary */
-    GND_E_STACK = -5      /* traversal stack overflow detected on the device   */
+    GND_E_STACK = -5,     /* traversal stack overflow detected on the device   */
+    GND_E_TIMEOUT = -6,   /* gnd_hip_server_wait: the job is still queued or running */
+    GND_E_BUSY = -7       /* gnd_hip_server_release: the job has not ended yet  */
 };
 
 typedef struct gnd_hip_ctx gnd_hip_ctx;
@@ -143,6 +145,10 @@ typedef struct gnd_hip_tuning {
                                  4 MiB, else 32)                                        */
     int32_t shadow_refill_lanes;  /* the same for any-hit shadow traversal; 0 = per
                                  scene (64, else 40)                                    */
+    int32_t stack_limit;      /* traversal stack entries per lane: 0 = all 128 (the
+                                 reference's int stack[64] per level, TLAS + BLAS);
+                                 1..128 caps it, so the GND_E_STACK path can be
+                                 exercised on small scenes                              */
 } gnd_hip_tuning;
 
 /* ---- context ------------------------------------------------------------ */
@@ -218,19 +224,26 @@ int gnd_hip_autotune(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* 
 /* ---- serving ---------------------------------------------------------------
  * No counterpart in the reference (it renders one scene per process,
  * include/gianduia/core/integrator.h:49-155).  A server owns a pool of path
- * slots and renders queued jobs by continuous batching: between bounces it
- * admits passes of queued jobs (in submission order) into free slots, and
- * gathers each job's finished passes into its film in pass order, so a film
- * is bit-identical to gnd_hip_render of the same job alone.  Scenes stay
- * resident in HBM until the budget needs their space (least recently used
- * scene without running jobs first).  A worker thread drives the engine;
- * every call is thread-safe. */
+ * slots and renders jobs by continuous batching.  A request goes into an
+ * inbox and never waits for device work.  The server runs up to `lanes`
+ * bounce loops at once, each on its own stream and each rendering the jobs of
+ * one resident scene, so jobs on different scenes run concurrently; between
+ * two bounces of a lane, passes of waiting jobs of its scene join it (path
+ * slots go to jobs in submission order).  Each job's finished passes are
+ * gathered into its film in pass order, so a film is bit-identical to
+ * gnd_hip_render of the same job alone.  Scenes stay resident in HBM until
+ * the budget needs their space (least recently used scene without a lane
+ * first).  A worker thread drives the engine; every call is thread-safe. */
 typedef struct gnd_hip_server gnd_hip_server;
 
 typedef struct gnd_hip_server_config {
-    size_t hbm_budget;     /* bytes for path slots + resident scenes; 0 = 90% of HBM */
-    uint32_t path_slots;   /* paths in flight (288 B each); a pass needs W*H of them */
+    size_t hbm_budget;     /* bytes for path slots + lane queues + resident scenes;
+                              0 = 90% of HBM                                      */
+    uint32_t path_slots;   /* paths in flight; a pass needs W*H of them. Each slot
+                              costs 140 B of path state plus 148 B of queues per lane */
     uint32_t max_jobs;     /* jobs admitted at once                               */
+    uint32_t lanes;        /* concurrent bounce loops (scenes rendered at once),
+                              1..16; 0 = 4                                        */
 } gnd_hip_server_config;
 
 typedef struct gnd_hip_job_status {
@@ -239,6 +252,10 @@ typedef struct gnd_hip_job_status {
     double queue_ms;       /* submission -> first admission                      */
     double run_ms;         /* first admission -> film complete                   */
     double latency_ms;     /* submission -> film complete (or failure)           */
+    double t_submit_ms;    /* arrival (entry of gnd_hip_server_submit), ms since the
+                              server was created                                  */
+    double t_admit_ms;     /* first admission, same clock; -1 while queued       */
+    double t_done_ms;      /* film complete or failure, same clock; -1 before    */
 } gnd_hip_job_status;
 
 int gnd_hip_server_create(gnd_hip_ctx* ctx, const gnd_hip_server_config* cfg, gnd_hip_server** out);
@@ -251,8 +268,17 @@ int gnd_hip_server_add_scene(gnd_hip_server* srv, const gnd_hip_scene_arrays* ar
 /* Queues a job (job->film: device memory the caller keeps until it is done). */
 int gnd_hip_server_submit(gnd_hip_server* srv, int32_t scene_id, const gnd_hip_job* job, uint64_t* ticket);
 int gnd_hip_server_poll(gnd_hip_server* srv, uint64_t ticket, gnd_hip_job_status* status);
-/* Waits until the job is done or failed, or timeout_s passes. */
+/* Waits until the job is done or failed (GND_OK, status filled), or until
+ * timeout_s passes (GND_E_TIMEOUT, status filled: the job still owns its film). */
 int gnd_hip_server_wait(gnd_hip_server* srv, uint64_t ticket, double timeout_s, gnd_hip_job_status* status);
+/* Asks the server to stop a queued or running job (Integrator::cancel for one
+ * request): it admits no more of its passes, drops those in flight and then
+ * fails it with GND_E_CANCELLED; once a wait reports that, no kernel touches
+ * its film any more.  No effect on a job that has ended. */
+int gnd_hip_server_cancel(gnd_hip_server* srv, uint64_t ticket);
+/* Forgets an ended job (its status can no longer be read); GND_E_BUSY while
+ * it is queued or running.  A long-running server releases what it has read. */
+int gnd_hip_server_release(gnd_hip_server* srv, uint64_t ticket);
 /* Totals since creation (samples of finished jobs, rays, counters). */
 int gnd_hip_server_stats(gnd_hip_server* srv, gnd_hip_stats* stats, int32_t* resident_scenes,
                          size_t* resident_bytes);

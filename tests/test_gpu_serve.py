@@ -1,7 +1,12 @@
 """Continuous-batching server (gnd_hip_server_*): jobs admitted between
 bounces into free path slots give films bit-identical to rendering each job
-alone with gnd_hip_render; scenes are evicted and re-uploaded under a small
-HBM budget; oversize jobs are refused."""
+alone with gnd_hip_render; requests that arrive while a long job renders are
+admitted and finish first, on its scene and on others; cancel, wait time-outs
+and release; a traversal stack overflow fails the served job as it fails the
+solo render; scenes are evicted and re-uploaded under a small HBM budget;
+oversize jobs are refused."""
+import time
+
 import numpy as np
 import pytest
 
@@ -95,8 +100,8 @@ def test_server_evicts_under_budget(device):
     one = probe.nbytes
     probe.free()
     slots = 16384  # a pass of the largest job (96 x 96) fits
-    # path state (288 B per slot) plus room for one scene but not two
-    srv = Server(device, path_slots=slots, max_jobs=4, hbm_budget=288 * slots + int(1.5 * one))
+    # path state and one lane's queues (288 B per slot) plus room for one scene but not two
+    srv = Server(device, path_slots=slots, max_jobs=4, hbm_budget=288 * slots + int(1.5 * one), lanes=1)
     try:
         ids = [srv.add_scene(h) for h in scenes]
         tickets = []
@@ -137,3 +142,105 @@ def test_poisson_load_completes(device):
         assert r["jobs"] == 20 and r["p99_ms"] >= r["p50_ms"] > 0
     finally:
         srv.close()
+
+
+def _wait_until(pred, timeout_s=60.0):
+    t0 = time.perf_counter()
+    while not pred():
+        assert time.perf_counter() - t0 < timeout_s, "condition not reached"
+        time.sleep(0.001)
+
+
+def test_server_overlaps_requests_with_a_running_job(device):
+    """Continuous batching: while a long job A renders (its passes fill the
+    slot pool), a job on another scene (B) and a job on A's scene (C) are
+    admitted between bounces and finish while A is still running.  Every
+    film equals its solo render bit for bit."""
+    a = HostScene.generate("cbox", "width=64;height=64")
+    b = HostScene.generate("scaled", "level=3;instances=6;width=48;height=40")
+    srv = Server(device, path_slots=64 * 64 * 8, max_jobs=8)
+    try:
+        ia, ib = srv.add_scene(a), srv.add_scene(b)
+        job_a = a.default_job(spp=4000)
+        ta = srv.submit(ia, job_a)
+        _wait_until(lambda: srv.poll(ta)["passes_done"] >= 8)
+        job_b = _jobs(b, 1, 21)[0]
+        job_c = a.default_job(spp=3)
+        job_c.seed_offset = 7
+        tb, tc = srv.submit(ib, job_b), srv.submit(ia, job_c)
+        sb, sc = srv.wait(tb, 120), srv.wait(tc, 120)
+        sa_mid = srv.poll(ta)
+        assert sa_mid["state"] == 1, "the long job finished before the later requests: no overlap shown"
+        assert sa_mid["passes_done"] < job_a.spp
+        sa = srv.wait(ta, 300)
+        for st in (sb, sc):
+            assert st["t_submit_ms"] > sa["t_admit_ms"]
+            assert st["t_admit_ms"] < sa["t_done_ms"] and st["t_done_ms"] < sa["t_done_ms"]
+        for hs, job, tk in ((a, job_a, ta), (b, job_b, tb), (a, job_c, tc)):
+            ds = DeviceScene(device, hs)
+            solo, _ = device.render(ds, [job])
+            assert np.array_equal(srv.film(tk), solo[0]), "served film differs from the solo render"
+            ds.free()
+    finally:
+        srv.close()
+
+
+def test_server_cancel_timeout_and_release(device):
+    """wait() raises TimeoutError while a job runs (its film stays owned by
+    the server); cancel() ends it with GND_E_CANCELLED; release() forgets an
+    ended job and refuses a running one; the server then renders the next
+    job correctly."""
+    hs = HostScene.generate("cbox", "width=64;height=64")
+    srv = Server(device, path_slots=64 * 64 * 8, max_jobs=4)
+    try:
+        sid = srv.add_scene(hs)
+        long = srv.submit(sid, hs.default_job(spp=100000))
+        with pytest.raises(TimeoutError):
+            srv.wait(long, 0.05)
+        with pytest.raises(GndError) as e:
+            srv.release(long)
+        assert e.value.code == A.GND_E_BUSY
+        srv.cancel(long)
+        with pytest.raises(GndError) as e:
+            srv.wait(long, 60)
+        assert e.value.code == A.GND_E_CANCELLED
+        srv.release(long)
+        with pytest.raises(GndError):
+            srv.poll(long)
+        j = hs.default_job(spp=5)
+        tk = srv.submit(sid, j)
+        assert srv.wait(tk, 60)["state"] == 2
+        ds = DeviceScene(device, hs)
+        assert np.array_equal(srv.film(tk), device.render(ds, [j])[0][0])
+        ds.free()
+    finally:
+        srv.close()
+
+
+def test_server_stack_overflow_fails_the_job(device):
+    """A traversal stack overflow (forced with a 2-entry stack limit) fails
+    the served job with GND_E_STACK, as it fails gnd_hip_render; with the
+    whole stack the same job renders."""
+    hs = HostScene.generate("scaled", "level=4;instances=6;width=32;height=32")  # BLAS deeper than 2 entries
+    ds = DeviceScene(device, hs)
+    srv = Server(device, path_slots=32 * 32 * 4, max_jobs=4)
+    old = device.tuning()
+    try:
+        sid = srv.add_scene(hs)
+        j = hs.default_job(spp=4)
+        device.set_tuning(stack_limit=2)
+        with pytest.raises(GndError) as e:
+            device.render(ds, [j])
+        assert e.value.code == A.GND_E_STACK
+        tk = srv.submit(sid, j)
+        with pytest.raises(GndError) as e:
+            srv.wait(tk, 60)
+        assert e.value.code == A.GND_E_STACK
+        device.set_tuning(stack_limit=0)
+        tk = srv.submit(sid, j)
+        assert srv.wait(tk, 60)["state"] == 2
+        assert np.array_equal(srv.film(tk), device.render(ds, [j])[0][0])
+    finally:
+        device.set_tuning(**old)
+        srv.close()
+        ds.free()
