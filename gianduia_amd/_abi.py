@@ -183,6 +183,8 @@ HIP_API = {
     "gnd_hip_copy_probe": (C.c_int, [C.c_void_p, C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
     "gnd_hip_denoise": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
                                   C.POINTER(C.c_double)]),
+    "gnd_hip_denoise_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
+                                          C.c_void_p, C.POINTER(C.c_double)]),
     "gnd_hip_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(gnd_hip_job), C.POINTER(gnd_hip_tuning)]),
     "gnd_hip_server_create": (C.c_int, [C.c_void_p, C.POINTER(gnd_hip_server_config), C.POINTER(C.c_void_p)]),
     "gnd_hip_server_destroy": (None, [C.c_void_p]),

@@ -136,10 +136,12 @@ class Device:
         self._check(_lib().gnd_hip_cancel(self.ctx))
 
     def denoise(self, color: np.ndarray, albedo: np.ndarray | None = None, normal: np.ndarray | None = None,
-                timed: bool = False):
+                timed: bool = False, noalloc: bool = False):
         """gnd_hip_denoise of (H, W, 3) float32 images (radiance; albedo and
         the AOV normal n*0.5+0.5 as optional guides).  Returns the denoised
-        image, and the device milliseconds with timed=True."""
+        image, and the device milliseconds with timed=True.  noalloc=True
+        calls gnd_hip_denoise_noalloc (static scratch, at most 2,097,152
+        pixels)."""
         color = np.ascontiguousarray(color, np.float32)
         H, W = color.shape[:2]
         bufs = []
@@ -155,7 +157,8 @@ class Device:
                 bufs.append(b)
             ms = C.c_double()
             ptr = [b.ptr if b is not None else None for b in bufs]
-            self._check(_lib().gnd_hip_denoise(self.ctx, ptr[0], ptr[1], ptr[2], W, H, ptr[0], C.byref(ms)))
+            fn = _lib().gnd_hip_denoise_noalloc if noalloc else _lib().gnd_hip_denoise
+            self._check(fn(self.ctx, ptr[0], ptr[1], ptr[2], W, H, ptr[0], C.byref(ms)))
             out = np.empty_like(color)
             self._check(_lib().gnd_hip_memcpy_d2h(self.ctx, out.ctypes.data_as(C.c_void_p), bufs[0].ptr, out.nbytes))
         finally:

@@ -1468,6 +1468,17 @@ struct WaveBuffers {
     QueueSet Q;
 };
 
+// Scratch of gnd_hip_denoise_noalloc: the denoiser's four float4 records per
+// pixel (two radiance buffers, two guide buffers) for up to kDnStaticPixels
+// pixels, 128 MiB on every device the library runs on, held until the
+// process ends.  Its name is a host-side stand-in: host code uses the device
+// address that gnd_hip_init looks up (gnd_hip_ctx::dnStatic).  Every call on
+// one device shares it, so calls that overlap on one device race (calls on
+// one context are serialised by its lock).  Each call writes every record it
+// reads, so nothing needs clearing between calls.
+constexpr size_t kDnStaticPixels = (size_t)1 << 21;  // 2,097,152 pixels: 1920 x 1080 fits
+static __device__ __align__(256) float4 scratch[4 * kDnStaticPixels];
+
 struct gnd_hip_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -1492,6 +1503,7 @@ struct gnd_hip_ctx {
     hipEvent_t ev[2] = {nullptr, nullptr};
     float4* dnScratch = nullptr;  // denoiser buffers (grown on demand)
     size_t dnCap = 0;             // pixels dnScratch holds
+    float4* dnStatic = nullptr;   // device address of `scratch` on this device
 };
 
 namespace {
@@ -1874,6 +1886,8 @@ int gnd_hip_init(int device, gnd_hip_ctx** out) {
         GND_CHECK(hipMemset(ctx->dCounters, 0, sizeof(Counters)));
         GND_CHECK(hipEventCreate(&ctx->ev[0]));
         GND_CHECK(hipEventCreate(&ctx->ev[1]));
+        // the no-allocation denoiser's scratch: this device's copy of the array
+        GND_CHECK(hipGetSymbolAddress((void**)&ctx->dnStatic, HIP_SYMBOL(scratch)));
     });
     if (rc != GND_OK) {
         g_lastError = ctx->err;
@@ -2670,6 +2684,31 @@ int gnd_hip_denoise(gnd_hip_ctx* ctx, const float* color, const float* albedo, c
         }
         GND_CHECK(hipEventRecord(ctx->ev[0], ctx->stream));
         GND_CHECK(gndDenoiseLaunch(ctx->stream, color, albedo, normal, width, height, output, ctx->dnScratch, nullptr,
+                                   nullptr));
+        GND_CHECK(hipEventRecord(ctx->ev[1], ctx->stream));
+        GND_CHECK(hipEventSynchronize(ctx->ev[1]));
+        if (ms) *ms = elapsedMs(ctx->ev[0], ctx->ev[1]);
+    });
+}
+
+int gnd_hip_denoise_noalloc(gnd_hip_ctx* ctx, const float* color, const float* albedo, const float* normal,
+                            int width, int height, float* output, double* ms) {
+    if (!ctx || !color || !output || width <= 0 || height <= 0) return GND_E_INVALID;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    const size_t np = (size_t)width * (size_t)height;
+    if (np > kDnStaticPixels) {
+        ctx->err = "gnd_hip_denoise_noalloc: " + std::to_string(width) + "x" + std::to_string(height) +
+                   " exceeds its static scratch of " + std::to_string(kDnStaticPixels) + " pixels";
+        return GND_E_INVALID;
+    }
+    if (!ctx->dnStatic) {
+        ctx->err = "gnd_hip_denoise_noalloc: the static scratch has no device address";
+        return GND_E_DEVICE;
+    }
+    return guarded(ctx, [&] {
+        GND_CHECK(hipSetDevice(ctx->device));
+        GND_CHECK(hipEventRecord(ctx->ev[0], ctx->stream));
+        GND_CHECK(gndDenoiseLaunch(ctx->stream, color, albedo, normal, width, height, output, ctx->dnStatic, nullptr,
                                    nullptr));
         GND_CHECK(hipEventRecord(ctx->ev[1], ctx->stream));
         GND_CHECK(hipEventSynchronize(ctx->ev[1]));

@@ -213,6 +213,16 @@ int gnd_hip_copy_probe(gnd_hip_ctx* ctx, size_t bytes, int repeats, double* ms_p
 int gnd_hip_denoise(gnd_hip_ctx* ctx, const float* color, const float* albedo, const float* normal, int width,
                     int height, float* output, double* ms);
 
+/* gnd_hip_denoise without any device allocation: no counterpart in the
+ * reference.  For callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * kernels and results as gnd_hip_denoise; its scratch is a static device array
+ * of 2,097,152 pixels (128 MiB per device, held until the process ends), so
+ * width*height above that is refused with GND_E_INVALID.  Calls that overlap
+ * on one device share that array and race. */
+int gnd_hip_denoise_noalloc(gnd_hip_ctx* ctx, const float* color, const float* albedo, const float* normal,
+                            int width, int height, float* output, double* ms);
+
 /* Picks the block size (64/128/256) and LDS stack depth (8/16/24/32), then
  * the closest-hit and the any-hit refill thresholds (24..64), by timing one
  * pass of `job` on `scene` (best of two) per candidate, keeping the fastest of
