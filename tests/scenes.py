@@ -97,6 +97,16 @@ class SceneBuilder:
         self.materials.append(m)
         return len(self.materials) - 1
 
+    def index(self, medium_inside: int, medium_outside: int = -1) -> int:
+        """IndexMaterial (src/materials/index.cpp:10-16): a null BSDF that only
+        bounds a medium."""
+        m = A.gnd_material_desc()
+        m.type = A.GND_MAT_INDEX
+        m.medium_inside = medium_inside
+        m.medium_outside = medium_outside
+        self.materials.append(m)
+        return len(self.materials) - 1
+
     def prim(self, mesh: int, M=None, Minv=None, material: int = 0, radiance=None, sphere_radius=None) -> int:
         p = A.gnd_primitive_desc()
         if sphere_radius is not None:
