@@ -133,29 +133,6 @@ def cpu_baseline(hs, target_s=12.0):
                       f"integrator, C++ restatement of the reference CPU integrator, {threads} threads)"}
 
 
-def batch_jobs(hs, n=64, seed=0):
-    """The batching workload: n jobs over the shared scene, each with a random
-    pose (thin lens looking at the scene centre from a random direction),
-    resolution log-uniform in 64^2..1024^2 and spp log-uniform in 1..256."""
-    import numpy as np
-    from gianduia_amd import Job, camera_lookat
-    rng = np.random.default_rng(seed)
-    base = hs.camera()
-    d = hs.desc().contents
-    jobs = []
-    for _ in range(n):
-        side = int(round(float(np.exp(rng.uniform(np.log(64), np.log(1024))))))
-        spp = int(round(float(np.exp(rng.uniform(np.log(1), np.log(256))))))
-        theta = rng.uniform(0.15, 0.45) * np.pi      # above the floor
-        phi = rng.uniform(0, 2 * np.pi)
-        r = rng.uniform(4.5, 6.5)
-        eye = (r * np.sin(theta) * np.cos(phi), r * np.cos(theta), r * np.sin(theta) * np.sin(phi))
-        cam = camera_lookat(base, eye, (0.0, 0.0, 0.0))
-        cam.width = cam.height = side
-        jobs.append(Job(cam, spp, d.max_depth, d.integrator, 0))
-    return jobs
-
-
 def batch_workload(dev, ds, hs):
     """64-job mix rendered four ways, films resident in HBM: `plain` = one job
     at a time with one pass per wave (the reference's pass loop,
@@ -164,6 +141,7 @@ def batch_workload(dev, ds, hs):
     `batched_graph` = the same with the bounce chain captured in a hipGraph.
     Films are identical in all four."""
     from gianduia_amd.api import RenderSession
+    from gianduia_amd.serve import batch_jobs
     jobs = batch_jobs(hs)
     samples = sum(j.camera.width * j.camera.height * j.spp for j in jobs)
     old = dev.tuning()
@@ -245,6 +223,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-counters", action="store_true", help="skip the untimed counting pass")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-job batching workload")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default tuning instead of running gnd_hip_autotune (untimed) on the workload")
     ap.add_argument("--serve", action="store_true",
                     help="only run the serving workload (Poisson arrivals over a scene pool) and print its JSON")
     ap.add_argument("--scene", default="scaled", choices=sorted(WORKLOADS),
@@ -297,6 +277,9 @@ def main():
     log(f"scene: generated {gen_s:.1f}s, BVH build {hs.build_ms:.0f} ms, {inst_tris} instanced triangles, "
         f"{ds.nbytes / 2**20:.0f} MiB in HBM")
     job = hs.default_job()
+    t_tune = time.perf_counter()
+    tuning = dev.tuning() if a.no_autotune else dev.autotune(ds, job)
+    log(f"tuning {'(default)' if a.no_autotune else '(autotuned)'}: {tuning} in {time.perf_counter() - t_tune:.1f} s")
     sess = RenderSession(dev, ds, [job], want_aov=a.aov)
     for i in range(a.warmup):
         st = sess.run()
@@ -373,6 +356,7 @@ def main():
         "trace_ms_per_step": round(sum(s["ms_trace"] for s in per_step) / a.steps, 3),
         "shadow_ms_per_step": round(sum(s["ms_shadow"] for s in per_step) / a.steps, 3),
         "shade_ms_per_step": round(sum(s["ms_shade"] for s in per_step) / a.steps, 3),
+        "tuning": tuning,
         "roofline": roof,
         "cpu_baseline": cpu,
         "batch": batch,

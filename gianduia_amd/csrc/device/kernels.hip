@@ -1504,7 +1504,20 @@ struct gnd_hip_ctx {
     float4* dnScratch = nullptr;  // denoiser buffers (grown on demand)
     size_t dnCap = 0;             // pixels dnScratch holds
     float4* dnStatic = nullptr;   // device address of `scratch` on this device
+    // instantiated bounce chains (use_graph), keyed by their exact launches
+    struct CachedGraph {
+        std::string key;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        uint64_t lastUse = 0;
+    };
+    std::vector<CachedGraph> graphs;
+    uint64_t graphClock = 0;
+    std::map<std::string, gnd_hip_tuning> tuned;  // gnd_hip_autotune results per scene and job shape
 };
+
+// gnd_hip_autotune's probe: at most this many samples (one default wave)
+constexpr uint64_t kAutotuneSamples = 40000000;
 
 namespace {
 
@@ -1912,6 +1925,10 @@ void gnd_hip_shutdown(gnd_hip_ctx* ctx) {
     if (ctx->dnScratch) (void)hipFree(ctx->dnScratch);
     for (auto e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto& g : ctx->graphs) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (g.graph) (void)hipGraphDestroy(g.graph);
+    }
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -2488,6 +2505,62 @@ static void launchBounce(hipStream_t stream, const WaveArgs& A, const BouncePlan
     if (tl) tl->lap(t, 2);
 }
 
+// The instantiated chain of `depth` bounces starting from the queues bound in
+// A.  A chain's nodes hold their kernel arguments and launch shapes by value,
+// so a cached chain whose arguments and shapes are byte-identical is the same
+// work: it is replayed instead of captured again (the key is exactly those
+// bytes, so buffers reallocated elsewhere never match a stale chain).  Up to
+// kGraphCache chains are kept, least recently used replaced first.
+constexpr size_t kGraphCache = 32;
+
+static hipGraphExec_t bounceGraph(gnd_hip_ctx* ctx, const WaveArgs& A0, const BouncePlan& K, uint32_t live,
+                                  int depth) {
+    std::string key;
+    auto put = [&](const void* p, size_t n) { key.append(static_cast<const char*>(p), n); };
+    const unsigned shape[4] = {K.grid(live), K.shadeGrid(live), K.gridFor(live, 256), (unsigned)K.B};
+    const void* fns[2] = {(const void*)K.trace, (const void*)K.shadow};
+    const int flags[3] = {K.split ? 1 : 0, (int)K.shm, depth};
+    put(&A0, sizeof(A0));
+    put(shape, sizeof(shape));
+    put(fns, sizeof(fns));
+    put(flags, sizeof(flags));
+    for (auto& c : ctx->graphs)
+        if (c.key == key) {
+            c.lastUse = ++ctx->graphClock;
+            return c.exec;
+        }
+    hipGraph_t gr = nullptr;
+    hipGraphExec_t ge = nullptr;
+    WaveArgs A = A0;
+    GND_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    try {
+        for (int d = 0; d < depth; ++d) {  // bounce d + 1 reads the queues bounce d wrote
+            launchBounce(ctx->stream, A, K, live, nullptr);
+            std::swap(A.qIn, A.qOut);
+            std::swap(A.rIn, A.rOut);
+            std::swap(A.gIn, A.gOut);
+        }
+    } catch (...) {
+        if (hipStreamEndCapture(ctx->stream, &gr) == hipSuccess && gr) (void)hipGraphDestroy(gr);
+        throw;
+    }
+    GND_CHECK(hipStreamEndCapture(ctx->stream, &gr));
+    const hipError_t ie = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+        (void)hipGraphDestroy(gr);
+        GND_CHECK(ie);
+    }
+    if (ctx->graphs.size() >= kGraphCache) {
+        auto victim = std::min_element(ctx->graphs.begin(), ctx->graphs.end(),
+                                       [](const auto& a, const auto& b) { return a.lastUse < b.lastUse; });
+        (void)hipGraphExecDestroy(victim->exec);
+        (void)hipGraphDestroy(victim->graph);
+        ctx->graphs.erase(victim);
+    }
+    ctx->graphs.push_back({key, gr, ge, ++ctx->graphClock});
+    return ge;
+}
+
 static uint32_t readLive(gnd_hip_ctx* ctx, const uint32_t* counts) {
     uint32_t live = 0;
     GND_CHECK(hipMemcpyAsync(&live, counts, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
@@ -2615,30 +2688,14 @@ int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jo
             int cur = 0;
             uint32_t live = w.live;
             if (graph) {
-                // a fixed chain of bounces captured once and launched as one
-                // graph (kernels exit at once on an empty queue); paths still
-                // alive after it continue below
+                // a fixed chain of bounces launched as one graph (kernels exit
+                // at once on an empty queue); paths still alive after it
+                // continue below
                 const int depth = std::max(1, w.maxDepth + 1);
-                hipGraph_t gr = nullptr;
-                hipGraphExec_t ge = nullptr;
-                GND_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-                try {
-                    for (int d = 0; d < depth; ++d) {
-                        bindQueues(A, ctx->wb.Q, cur);
-                        launchBounce(ctx->stream, A, K, live, nullptr);
-                        cur = 1 - cur;
-                    }
-                } catch (...) {
-                    if (hipStreamEndCapture(ctx->stream, &gr) == hipSuccess && gr) (void)hipGraphDestroy(gr);
-                    throw;
-                }
-                GND_CHECK(hipStreamEndCapture(ctx->stream, &gr));
-                const hipError_t ie = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
-                if (ie == hipSuccess) GND_CHECK(hipGraphLaunch(ge, ctx->stream));
-                live = ie == hipSuccess ? readLive(ctx, ctx->wb.Q.counts) : 0;
-                if (ge) (void)hipGraphExecDestroy(ge);
-                (void)hipGraphDestroy(gr);
-                GND_CHECK(ie);
+                bindQueues(A, ctx->wb.Q, cur);
+                GND_CHECK(hipGraphLaunch(bounceGraph(ctx, A, K, live, depth), ctx->stream));
+                if (depth % 2) cur = 1 - cur;
+                live = readLive(ctx, ctx->wb.Q.counts);
             }
             while (live > 0) {
                 bindQueues(A, ctx->wb.Q, cur);
@@ -2743,14 +2800,31 @@ int gnd_hip_copy_probe(gnd_hip_ctx* ctx, size_t bytes, int repeats, double* ms_p
 }
 
 int gnd_hip_autotune(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* job, gnd_hip_tuning* out) {
-    if (!ctx || !scene || !job || !out || job->spp == 0) return GND_E_INVALID;
-    // probe: one pass of the job (its film is overwritten), timed on the
-    // device (best of two) for every candidate; coordinate search over block
-    // size x LDS stack depth, then the closest-hit and the any-hit refill
-    // thresholds, keeping the fastest setting of each in turn
+    if (!ctx || !scene || !job || !out || job->spp == 0 || job->camera.width <= 0 || job->camera.height <= 0)
+        return GND_E_INVALID;
+    // probe: the job itself, cut to at most one default wave of samples (its
+    // film is overwritten), timed on the device (best of two) for every
+    // candidate.  Coordinate search over block size x LDS stack depth, then
+    // the closest-hit and the any-hit refill thresholds, then the wave
+    // capacity (whole passes per wave), keeping the fastest setting of each
+    // in turn.  The result is cached per scene and job shape.
+    const uint64_t npix = (uint64_t)job->camera.width * (uint64_t)job->camera.height;
     gnd_hip_job probe = *job;
-    probe.spp = 1;
     probe.film_sq = nullptr;
+    probe.spp = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(job->spp, kAutotuneSamples / npix));
+    char keyBuf[160];
+    std::snprintf(keyBuf, sizeof(keyBuf), "%p %d %d %u %d %d %d %d", (void*)scene, job->camera.width,
+                  job->camera.height, probe.spp, job->integrator, job->max_depth, job->camera.type,
+                  job->film_aov ? 1 : 0);
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        auto it = ctx->tuned.find(keyBuf);
+        if (it != ctx->tuned.end()) {
+            ctx->tuning = it->second;
+            *out = it->second;
+            return GND_OK;
+        }
+    }
     const gnd_hip_tuning start = ctx->tuning;
     gnd_hip_tuning best = start;
     double bestMs = 1e300;
@@ -2788,12 +2862,26 @@ int gnd_hip_autotune(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* 
         t.shadow_refill_lanes = r;
         tryTuning(t);
     }
+    // wave capacity: all of the probe's passes in one wave, or 1/2, 1/4, 1/8
+    // of them per wave (at least one pass)
+    const gnd_hip_tuning refills = best;
+    for (int div : {1, 2, 4, 8}) {
+        const uint64_t passes = std::max<uint64_t>(1, (probe.spp + div - 1) / div);
+        gnd_hip_tuning t = refills;
+        t.queue_capacity = (int64_t)std::max<uint64_t>(1024, passes * npix);
+        if (t.queue_capacity == refills.queue_capacity) continue;
+        tryTuning(t);
+    }
     if (rc != GND_OK) {
         (void)gnd_hip_set_tuning(ctx, &start);
         return rc;
     }
     rc = gnd_hip_set_tuning(ctx, &best);
-    *out = ctx->tuning;
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        ctx->tuned[keyBuf] = ctx->tuning;
+        *out = ctx->tuning;
+    }
     return rc;
 }
 

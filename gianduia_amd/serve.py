@@ -141,6 +141,28 @@ class Server:
             pass
 
 
+def batch_jobs(hs: HostScene, n: int = 64, seed: int = 0) -> list[Job]:
+    """The batching workload: n jobs over the shared scene `hs`, each with a
+    random pose (looking at the scene centre from a random direction above
+    the floor, the scene's own camera model), a square resolution
+    log-uniform in 64^2..1024^2 and spp log-uniform in 1..256 (seeded)."""
+    rng = np.random.default_rng(seed)
+    base = hs.camera()
+    d = hs.desc().contents
+    jobs = []
+    for _ in range(n):
+        side = int(round(float(np.exp(rng.uniform(np.log(64), np.log(1024))))))
+        spp = int(round(float(np.exp(rng.uniform(np.log(1), np.log(256))))))
+        theta = rng.uniform(0.15, 0.45) * np.pi
+        phi = rng.uniform(0, 2 * np.pi)
+        r = rng.uniform(4.5, 6.5)
+        eye = (r * np.sin(theta) * np.cos(phi), r * np.cos(theta), r * np.sin(theta) * np.sin(phi))
+        cam = camera_lookat(base, eye, (0.0, 0.0, 0.0))
+        cam.width = cam.height = side
+        jobs.append(Job(cam, spp, d.max_depth, d.integrator, 0))
+    return jobs
+
+
 def scene_pool(n: int = 8, lo: float = 1e4, hi: float = 5e6, seed: int = 0) -> list[HostScene]:
     """n procedural scenes with log-uniform triangle counts in [lo, hi]."""
     rng = np.random.default_rng(seed)

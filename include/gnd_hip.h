@@ -224,9 +224,12 @@ int gnd_hip_denoise_noalloc(gnd_hip_ctx* ctx, const float* color, const float* a
                             int width, int height, float* output, double* ms);
 
 /* Picks the block size (64/128/256) and LDS stack depth (8/16/24/32), then
- * the closest-hit and the any-hit refill thresholds (24..64), by timing one
- * pass of `job` on `scene` (best of two) per candidate, keeping the fastest of
- * each in turn; sets the result as the context's tuning and returns it.
+ * the closest-hit and the any-hit refill thresholds (24..64), then the wave
+ * capacity (all of the probe's passes in one wave, or a half, a quarter or an
+ * eighth of them per wave), by timing `job` cut to at most 40M samples on
+ * `scene` (best of two) per candidate, keeping the fastest of each in turn;
+ * sets the result as the context's tuning and returns it.  The result is
+ * remembered per scene and job shape, so a second call returns at once.
  * job->film is overwritten.  No counterpart in the reference (its CPU render
  * has no launch shape). */
 int gnd_hip_autotune(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* job, gnd_hip_tuning* out);

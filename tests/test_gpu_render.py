@@ -1,13 +1,17 @@
-"""GPU path and volumetric-path integrators vs the CPU oracle on the same seeds.
+"""GPU path and volumetric-path integrators vs the CPU oracle.
 
 The GPU follows the reference's per-path random-number order, so most
-samples reproduce the CPU samples; transcendental functions (ROCm device
-libm vs glibc) differ in the last bits, so a few paths diverge after a
-bounce.  Agreement is checked two ways:
-  * single-sample replay: at 1 spp with the box filter every film pixel is
-    one path's value, and the large majority must equal the oracle's;
-  * statistics: per-pixel Welch t-test over the per-pass squares, with the
-    fraction of failing pixel-channels inside the binomial 99.9% bound of 1%.
+samples reproduce the CPU samples at the same seeds; transcendental
+functions (ROCm device libm vs glibc) differ in the last bits, so a few
+paths diverge after a bounce.  Agreement is checked two ways:
+  * single-sample replay (same seeds): at 1 spp with the box filter every
+    film pixel is one path's value, and the large majority must equal the
+    oracle's;
+  * statistics (independent seeds: the GPU renders other sample indices than
+    the CPU, so the two images share no path): per-pixel Welch t-test over
+    the per-pass squares, with the fraction of failing pixel-channels inside
+    the binomial 99.9% bound of 1%, and relative MSE against a 16384-spp CPU
+    render (test_equal_image_error).
 """
 import numpy as np
 import pytest
@@ -20,12 +24,19 @@ from oracle import RefScene
 pytestmark = pytest.mark.gpu
 
 
+# sample-index offset of the GPU side of the statistical comparisons: the CPU
+# renders sample indices [0, spp*W*H), the GPU [GPU_STREAM, GPU_STREAM + spp*W*H)
+GPU_STREAM = 1 << 36
+
+
 def _render_pair(device, name, params, spp, squares=True):
+    """GPU and oracle renders of one generated scene; with squares (the
+    statistical comparisons) the GPU uses independent random streams."""
     hs = HostScene.generate(name, params)
     ds = DeviceScene(device, hs)
     ref = RefScene(hs.desc())
     cam = hs.camera()
-    job = hs.default_job(spp=spp)
+    job = hs.default_job(spp=spp, seed_offset=GPU_STREAM if squares else 0)
     out, st = device.render(ds, [job], want_sq=squares)
     if squares:
         g, gsq = out[0]
@@ -35,30 +46,54 @@ def _render_pair(device, name, params, spp, squares=True):
     return out[0], None, r, None, st
 
 
+def _pixel_moments(f, sq, n):
+    """Per pixel: the mean of the n per-pass values (the resolved value,
+    sum(w L) / sum(w); exact for the box filter, the ratio estimate for the
+    Gaussian one) and their sample variance from the per-pass squares (the
+    film's squares are of each pass's resolved value)."""
+    m = f[..., :3] / np.maximum(f[..., 3:4], 1e-30)
+    v = np.maximum(sq / n - m * m, 0) * n / (n - 1)
+    return m, v
+
+
 def _welch(g, gsq, r, rsq, n):
-    gm, rm = g[..., :3] / n, r[..., :3] / n
-    gv = np.maximum(gsq / n - gm * gm, 0) * n / (n - 1)
-    rv = np.maximum(rsq / n - rm * rm, 0) * n / (n - 1)
+    gm, gv = _pixel_moments(g, gsq, n)
+    rm, rv = _pixel_moments(r, rsq, n)
     with np.errstate(all="ignore"):
         _, p = sst.ttest_ind_from_stats(gm, np.sqrt(gv), n, rm, np.sqrt(rv), n, equal_var=False)
-    p = np.where(np.isnan(p), 1.0, p)
-    # equal means pass outright: where both sides hold the same paths the
-    # variance estimate sq/n - m^2 cancels to ~0 and t is ill-conditioned
-    p = np.where(np.isclose(gm, rm, rtol=1e-4, atol=1e-7), 1.0, p)
+    # no variance on either side (e.g. pixels that see only black): equal
+    # means pass, unequal ones fail
+    p = np.where(np.isnan(p), np.where(np.isclose(gm, rm, rtol=1e-5, atol=1e-8), 1.0, 0.0), p)
     frac = float((p < 0.01).mean())
-    bound = 0.01 + 3.1 * np.sqrt(0.01 * 0.99 / p.size)
+    # independent tests: every pixel-channel for the box filter; for the
+    # Gaussian filter (radius 2) a sample reaches a 4 x 4 footprint of pixels,
+    # so neighbouring pixels are correlated: one test per footprint
+    box = np.allclose(g[..., 3], n) and np.allclose(r[..., 3], n)
+    tests = p.size if box else p.size / 16
+    bound = 0.01 + 3.1 * np.sqrt(0.01 * 0.99 / tests)
     return frac, bound
 
 
 def _check_stats(g, gsq, r, rsq, n, label, mean_tol=0.03):
-    assert np.allclose(g[..., 3], r[..., 3], rtol=1e-5), f"{label}: filter weights differ"
+    # filter weights: exact for the box filter (n per pixel), statistical for
+    # the Gaussian one (its weights depend on the jitter of each sample)
+    assert abs(g[..., 3].mean() - r[..., 3].mean()) < 0.01 * r[..., 3].mean(), f"{label}: filter weights differ"
     frac, bound = _welch(g, gsq, r, rsq, n)
     print(f"{label}: welch fail fraction {frac:.4f} (bound {bound:.4f})", flush=True)
     assert frac <= bound, label
+    # image means: within mean_tol, or within 5 standard errors of their
+    # difference (from the per-pixel variances; x4 for the Gaussian filter,
+    # whose neighbouring pixels share samples)
     img_g, img_r = resolve(g), resolve(r)
-    rel = abs(img_g.mean() - img_r.mean()) / img_r.mean()
-    print(f"{label}: mean {img_g.mean():.5f} vs {img_r.mean():.5f} (rel {rel:.4f})", flush=True)
-    assert rel < mean_tol, label
+    _, gv = _pixel_moments(g, gsq, n)
+    _, rv = _pixel_moments(r, rsq, n)
+    shared = 1.0 if np.allclose(r[..., 3], n) else 4.0
+    se = shared * float(np.sqrt((gv.sum() + rv.sum()) / n)) / gv.size
+    diff = abs(float(img_g.mean()) - float(img_r.mean()))
+    rel = diff / float(img_r.mean())
+    print(f"{label}: mean {img_g.mean():.5f} vs {img_r.mean():.5f} (rel {rel:.4f}, {diff / max(se, 1e-30):.2f} s.e.)",
+          flush=True)
+    assert rel < mean_tol or diff < 5 * se, label
 
 
 def _replay_fraction(g, r):
@@ -71,9 +106,6 @@ def test_cbox_matches_oracle(device):
     print("render cbox 128x128 16spp", flush=True)
     g, gsq, r, rsq, _ = _render_pair(device, "cbox", "width=128;height=128", 16)
     _check_stats(g, gsq, r, rsq, 16, "cbox")
-    close = np.isclose(g[..., :3], r[..., :3], rtol=1e-3, atol=1e-5).mean()
-    print("close fraction", close)
-    assert close > 0.9
 
 
 @pytest.mark.parametrize("name,params,min_frac", [
@@ -84,6 +116,10 @@ def test_cbox_matches_oracle(device):
     ("volumetric", "grid=0;width=96;height=64", 0.995),
     # + the index-material box holding a 32^3 density grid (split kernels)
     ("volumetric", "grid=32;width=96;height=64", 0.99),
+    # the full 256^3 grid of the benchmark workload: 8^3 voxels per macro cell
+    ("volumetric", "grid=256;width=80;height=45", 0.99),
+    # the full scaled-geometry workload's scene (1.31M-tri icosphere + 64 instances)
+    ("scaled", "width=96;height=54", 0.99),
 ])
 def test_single_sample_replay(device, name, params, min_frac):
     """At 1 spp each pixel holds one path: most must equal the oracle's path."""
@@ -94,28 +130,34 @@ def test_single_sample_replay(device, name, params, min_frac):
     assert frac >= min_frac
 
 
-@pytest.mark.parametrize("name,params,n,ref_spp", [("cbox", "width=48;height=48", 16, 2048),
-                                                   ("volumetric", "grid=32;width=40;height=32", 16, 2048)])
-def test_equal_image_error(device, name, params, n, ref_spp):
-    """relMSE(GPU@n vs CPU@ref_spp) within 2% of relMSE(CPU@n vs CPU@ref_spp);
-    the high-spp CPU reference uses independent random streams (seed offset)."""
+@pytest.mark.parametrize("name,params", [("cbox", "width=48;height=48"),
+                                         ("volumetric", "grid=32;width=40;height=32")])
+def test_equal_image_error(device, name, params):
+    """Equal image error at equal spp: the relative MSE of 16-spp GPU renders
+    against a 16384-spp CPU render is within 2% of that of 16-spp CPU renders.
+    All three use disjoint sample indices (independent streams); each side
+    averages 32 independent renders so that the comparison is not noise."""
+    n, trials, ref_spp = 16, 32, 16384
     hs = HostScene.generate(name, params)
     ds = DeviceScene(device, hs)
     ref = RefScene(hs.desc())
     cam = hs.camera()
     W, H = cam.width, cam.height
+    span = n * W * H  # sample indices one render uses
     print(f"equal image error {name}: CPU reference {ref_spp} spp", flush=True)
     hi, _ = ref.render(threads=16, spp=ref_spp, seed_offset=1 << 40, width=W, height=H)
-    cpu, _ = ref.render(threads=16, spp=n, width=W, height=H)
-    gpu, _ = device.render(ds, [hs.default_job(spp=n)])
     r = resolve(hi)
 
     def relmse(x):
         img = resolve(x)
         return float(np.mean((img - r) ** 2 / (r * r + 1e-2)))
 
-    e_gpu, e_cpu = relmse(gpu[0]), relmse(cpu)
-    print(f"{name}: relMSE GPU {e_gpu:.6f} CPU {e_cpu:.6f}", flush=True)
+    jobs = [hs.default_job(spp=n, seed_offset=(1 << 38) + k * span) for k in range(trials)]
+    gpu, _ = device.render(ds, jobs)
+    e_gpu = float(np.mean([relmse(f) for f in gpu]))
+    e_cpu = float(np.mean([relmse(ref.render(threads=16, spp=n, seed_offset=(1 << 39) + k * span, width=W,
+                                             height=H)[0]) for k in range(trials)]))
+    print(f"{name}: mean relMSE over {trials} renders: GPU {e_gpu:.6f} CPU {e_cpu:.6f}", flush=True)
     assert e_gpu <= 1.02 * e_cpu
 
 
@@ -183,11 +225,11 @@ def test_scaled_materials_match_oracle(device):
 
 
 def test_gaussian_filter_matches_oracle(device):
-    print("render cbox gaussian filter 64x64 8spp", flush=True)
-    g, gsq, r, rsq, _ = _render_pair(device, "cbox", "width=64;height=64;filter=gaussian", 8)
+    print("render cbox gaussian filter 64x64 32spp", flush=True)
+    g, gsq, r, rsq, _ = _render_pair(device, "cbox", "width=64;height=64;filter=gaussian", 32)
     # Gaussian splat: every pixel gathers from up to 5x5 samples per pass
     assert np.all(g[..., 3] > 0)
-    _check_stats(g, gsq, r, rsq, 8, "gaussian")
+    _check_stats(g, gsq, r, rsq, 32, "gaussian")
 
 
 # ---------------------------------------------------------------- point / spot / directional / environment lights

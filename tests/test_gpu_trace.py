@@ -109,3 +109,43 @@ def test_trace_empty_batch(device):
     rays = {k: np.zeros(0, np.float32) for k in ("ox", "oy", "oz", "dx", "dy", "dz", "tmin", "tmax")}
     g, st = device.trace_closest(ds, rays)
     assert len(g["t"]) == 0
+
+
+def _scene_rays(rng, n, radius=6.0, target=2.0):
+    """Rays from a sphere of the given radius around the origin towards random
+    points of [-target, target]^3 (most of them cross the scene)."""
+    o = rng.normal(size=(n, 3))
+    o = (o / np.linalg.norm(o, axis=1, keepdims=True) * radius).astype(np.float32)
+    d = rng.uniform(-target, target, size=(n, 3)).astype(np.float32) - o
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    return {"ox": o[:, 0].copy(), "oy": o[:, 1].copy(), "oz": o[:, 2].copy(), "dx": d[:, 0].copy(),
+            "dy": d[:, 1].copy(), "dz": d[:, 2].copy(), "tmin": np.full(n, 1e-4, np.float32),
+            "tmax": np.full(n, np.inf, np.float32)}
+
+
+def test_trace_full_scaled_scene(device):
+    """The benchmark's own scene at full size (1,310,720-triangle icosphere at
+    subdivision level 8 + 64 instances of a 10k-triangle sphere): 65,536
+    seeded rays, bit-identical closest hits, reference-order visit counts and
+    any-hit occlusion against the oracle."""
+    hs = HostScene.generate("scaled", "")
+    ds = DeviceScene(device, hs)
+    ref = RefScene(hs.desc())
+    rays = _scene_rays(np.random.default_rng(31), 65536)
+    print("trace_closest on the full scaled scene", flush=True)
+    device.set_counters(True)
+    try:
+        g, st = device.trace_closest(ds, rays)
+    finally:
+        device.set_counters(False)
+    r, cnt = ref.trace(rays, counts=True)
+    _compare_closest(g, r, "scaled full")
+    assert (r["inst"] >= 0).sum() > 30000
+    assert st["node_visits"] == int(cnt[:, 0].sum())
+    assert st["pack_tests"] == int(cnt[:, 1].sum())
+    assert st["inst_tests"] == int(cnt[:, 2].sum())
+    short = dict(rays)
+    short["tmax"] = np.random.default_rng(32).uniform(0.5, 8.0, 65536).astype(np.float32)
+    occ, _ = device.trace_any(ds, short)
+    assert np.array_equal(occ, ref.trace(short, any_hit=True))
+    assert 0 < occ.sum() < len(occ)
