@@ -86,6 +86,34 @@ def measured_traffic():
     return None
 
 
+def host_cores():
+    """Threads for the CPU baseline: every core this process may run on
+    (sched_getaffinity, what nproc reports), with what lscpu and the cgroup CPU
+    quota say about the host beside it."""
+    import subprocess
+    n = len(os.sched_getaffinity(0))
+    host = {"affinity_cores": n, "os_cpu_count": os.cpu_count()}
+    try:
+        host["nproc"] = int(subprocess.run(["nproc"], capture_output=True, text=True, check=True).stdout)
+    except Exception:
+        pass
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, check=True).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("CPU(s)", "Model name", "Thread(s) per core", "Socket(s)"):
+                host["lscpu " + k.strip()] = v.strip()
+    except Exception:
+        pass
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()
+            host["cgroup_cpu_max"] = q if q == "max" else round(int(q) / int(period), 2)
+    except Exception:
+        pass
+    return n, host
+
+
 def cpu_baseline(hs, target_s=12.0):
     """Times the CPU restatement (oracle) of the reference integrator on the host
     cores: a centred band of rows of pass 0 grown until it takes target_s, or
@@ -94,7 +122,7 @@ def cpu_baseline(hs, target_s=12.0):
     import ctypes as C
     import numpy as np
     from oracle import RefScene, lib
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads, host = host_cores()
     t0 = time.time()
     ref = RefScene(hs.desc())
     log(f"cpu baseline: oracle scene built in {time.time() - t0:.1f}s, {threads} threads")
@@ -128,7 +156,7 @@ def cpu_baseline(hs, target_s=12.0):
         passes += more
         desc = f"passes 0-{passes - 1} (full frames)"
     integ = INTEGRATOR_NAMES[hs.desc().contents.integrator]
-    return {"value": samples / secs / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
+    return {"value": samples / secs / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port", "host": host,
             "sample": f"{desc} of the same {W}x{H} frame ({samples} camera samples, {secs:.1f} s, {integ} "
                       f"integrator, C++ restatement of the reference CPU integrator, {threads} threads)"}
 
@@ -150,6 +178,9 @@ def batch_workload(dev, ds, hs):
     res = {"jobs": len(jobs), "samples": samples}
     try:
         together.run()  # warm-up: wave buffers at full size
+        dev.set_tuning(use_graph=True)
+        together.run()  # warm-up: the bounce chains of each wave shape, captured once and reused
+        dev.set_tuning(**old)
         for mode in ("plain", "solo", "batched", "batched_graph"):
             t0 = time.perf_counter()
             if mode.startswith("batched"):
@@ -303,7 +334,10 @@ def main():
         dev.set_counters(True)
         cst = sess.run()  # identical rays (deterministic seeds); untimed
         dev.set_counters(False)
-        byts = algorithmic_bytes(cst["rays_closest"], cst["node_visits"], cst["pack_tests"], cst["inst_tests"])
+        # the traversal kernel's own queries: the tail kernel's closest hits
+        # are in rays_closest but neither in the visit counts nor in ms_trace
+        trays = cst["rays_closest"] - cst["rays_tail"]
+        byts = algorithmic_bytes(trays, cst["node_visits"], cst["pack_tests"], cst["inst_tests"])
         trace_ms = sum(s["ms_trace"] for s in per_step) / a.steps
         launches = per_step[0]["trace_launches"]
         achieved = byts / (trace_ms / 1000.0) / 1e9
@@ -318,9 +352,10 @@ def main():
                 # instance steps; an instance costs a start and a finish step)
                 "simd_utilization": round((cst["node_visits"] + cst["pack_tests"] + 2 * cst["inst_tests"])
                                           / max(1, cst["lane_iterations"]), 4),
-                "visits_per_ray": {"nodes": cst["node_visits"] / cst["rays_closest"],
-                                   "packs": cst["pack_tests"] / cst["rays_closest"],
-                                   "insts": cst["inst_tests"] / cst["rays_closest"]}}
+                "rays_per_launch": trays / launches,
+                "visits_per_ray": {"nodes": cst["node_visits"] / trays,
+                                   "packs": cst["pack_tests"] / trays,
+                                   "insts": cst["inst_tests"] / trays}}
         log(f"counting pass: {json.dumps(roof)}")
 
     den = None

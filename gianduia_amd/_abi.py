@@ -129,7 +129,8 @@ class gnd_hip_stats(C.Structure):
                 ("inst_tests", C.c_uint64), ("trace_launches", C.c_uint64), ("nan_samples", C.c_uint64),
                 ("shadow_node_visits", C.c_uint64), ("shadow_pack_tests", C.c_uint64),
                 ("shadow_inst_tests", C.c_uint64), ("max_ray_nodes", C.c_uint64), ("bad_rays", C.c_uint64),
-                ("lane_iterations", C.c_uint64), ("shadow_lane_iterations", C.c_uint64)]
+                ("lane_iterations", C.c_uint64), ("shadow_lane_iterations", C.c_uint64), ("ms_tail", C.c_double),
+                ("rays_tail", C.c_uint64)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -138,7 +139,7 @@ class gnd_hip_stats(C.Structure):
 class gnd_hip_tuning(C.Structure):
     _fields_ = [("block_size", C.c_int32), ("stack_lds", C.c_int32), ("queue_capacity", C.c_int64),
                 ("use_graph", C.c_int32), ("refill_lanes", C.c_int32), ("shadow_refill_lanes", C.c_int32),
-                ("stack_limit", C.c_int32)]
+                ("stack_limit", C.c_int32), ("tail_paths", C.c_int32)]
 
 
 class gnd_hip_server_config(C.Structure):

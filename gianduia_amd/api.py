@@ -93,19 +93,22 @@ class Device:
         self._check(_lib().gnd_hip_get_tuning(self.ctx, C.byref(t)))
         return {"block_size": t.block_size, "stack_lds": t.stack_lds, "queue_capacity": t.queue_capacity,
                 "use_graph": t.use_graph, "refill_lanes": t.refill_lanes,
-                "shadow_refill_lanes": t.shadow_refill_lanes, "stack_limit": t.stack_limit}
+                "shadow_refill_lanes": t.shadow_refill_lanes, "stack_limit": t.stack_limit,
+                "tail_paths": t.tail_paths}
 
     def set_tuning(self, block_size=None, stack_lds=None, queue_capacity=None, use_graph=None,
-                   refill_lanes=None, shadow_refill_lanes=None, stack_limit=None) -> None:
+                   refill_lanes=None, shadow_refill_lanes=None, stack_limit=None, tail_paths=None) -> None:
         """Changes the given tunables; refill_lanes / shadow_refill_lanes 0
-        means the per-scene default, stack_limit 0 the whole traversal stack."""
+        means the per-scene default, stack_limit 0 the whole traversal stack,
+        tail_paths 0 the per-device default and -1 no tail kernel."""
         cur = self.tuning()
         t = A.gnd_hip_tuning(block_size or cur["block_size"], stack_lds or cur["stack_lds"],
                              queue_capacity or cur["queue_capacity"],
                              cur["use_graph"] if use_graph is None else int(use_graph),
                              cur["refill_lanes"] if refill_lanes is None else int(refill_lanes),
                              cur["shadow_refill_lanes"] if shadow_refill_lanes is None else int(shadow_refill_lanes),
-                             cur["stack_limit"] if stack_limit is None else int(stack_limit))
+                             cur["stack_limit"] if stack_limit is None else int(stack_limit),
+                             cur["tail_paths"] if tail_paths is None else int(tail_paths))
         self._check(_lib().gnd_hip_set_tuning(self.ctx, C.byref(t)))
 
     def autotune(self, scene: "DeviceScene", job: "Job") -> dict:

@@ -113,6 +113,7 @@ struct Counters {
     unsigned long long maxRayNodes, badRays;     // worst closest-hit ray, rays with non-finite directions
     unsigned long long iters, siters;            // lane slots of traversal loop iterations (64 per wave step)
     unsigned long long rays, srays, launches;    // closest-hit and shadow queries, bounces with work
+    unsigned long long tailRays, tailSrays;      // the same queries made by the tail kernel
 };
 
 // Paths that end leave their engine chunk (continuous batching): one atomic
@@ -396,7 +397,7 @@ constexpr int kWalkRefill = 24;
 #endif
 constexpr int kParts = GND_FEED_PARTS;
 constexpr uint32_t kFeedTrace = 8, kFeedShadow = kFeedTrace + kParts, kFeedMedium = kFeedShadow + kParts,
-                   kNumCounts = 64;
+                   kFeedTail = kFeedMedium + kParts, kNumCounts = 64;
 
 // The persistent traversals' ray feed: waves take rays in queue order with
 // one atomic per refill.  The queue can be cut into kParts contiguous parts
@@ -1302,6 +1303,171 @@ __global__ __launch_bounds__(256, GND_SHADOWTR_WAVES) void kShadow(WaveArgs A) {
     else if (st.overflow) atomicAdd(&A.ctr->stackOverflow, 1ull);
 }
 
+// ============================================================================ tail kernel
+
+// The last bounces of a wave, once few paths are left: a launch per bounce
+// then lasts as long as its slowest ray, so most of the GPU idles.  kTail runs
+// every path of the in-queue to its end in one launch.  Each lane takes a
+// path (persistent, refilled from the queue as paths end) and repeats, per
+// bounce: closest hit, medium sampling + emission + NEE (neeStep / directStep
+// / aoStep), the shadow ray with its transmittance, then the scatter.  That is
+// the order of random-number draws of the bounce kernels in split mode
+// (kMedium, kShade<true>, kShadow, kScatter), and the radiance sums are formed
+// in the same order, so a path gives the same bits whichever way it ran.
+// Traversals are wave-collective (traceRay): lanes without a ray take part
+// idle.  FAST: every shadow ray is one any-hit query (no null-BSDF surface,
+// no medium on a segment).
+#ifndef GND_TAIL_WAVES
+#define GND_TAIL_WAVES 2
+#endif
+// a lane's work between refills is a whole bounce, so idle lanes refill early
+constexpr int kTailRefill = 8;
+// traversal stack entries in LDS per lane of the tail kernel
+constexpr int kTailLds = 16;
+template <int LDSD, bool MOTION, bool FAST>
+__global__ __launch_bounds__(256, GND_TAIL_WAVES) void kTail(WaveArgs A) {
+    extern __shared__ int sStack[];
+    const uint32_t n = A.counts[0];
+    const uint64_t waveMask = __ballot(1);
+    uint32_t cnt[4] = {0, 0, 0, 0};
+    int spill[spillSize<LDSD>()];
+    TravStack<LDSD> st = makeStack<LDSD>(sStack, spill, A.stackCap);
+    const DScene& S = A.S;
+    RayFeed feed;
+    feed.init(A.counts + kFeedTail, n, kTailRefill);
+    bool drained = false, busy = false;
+    int p = -1;
+    PathLocal s;
+    uint32_t nRays = 0, nShadow = 0;
+    for (;;) {
+        if (!drained) {
+            const int64_t i = feed.take(!busy, waveMask, drained);
+            if (i >= 0) {
+                p = A.qIn[i];
+                loadPath(A.P, p, s);
+                loadRay(A.rIn, (uint32_t)i, s.o, s.d);
+                busy = true;
+            }
+        }
+        if (__ballot(busy) == 0) {
+            if (drained) break;
+            continue;
+        }
+        // closest hit (Scene::rayIntersect at the top of Li)
+        HitRec h;
+        h.t = finf();
+        h.u = h.v = 0.0f;
+        h.inst = h.prim = -1;
+        const bool found = traceRay<false, false, LDSD, MOTION>(S, s.o, s.d, kEps, finf(), rayTime(s), h, st, cnt,
+                                                                busy);
+        if (busy) {
+            nRays++;
+            if (!found) h.inst = h.prim = -1;
+        }
+        // emission, NEE and the event of this bounce
+        bool shadow = false;
+        int r = 0, smedium = -1;
+        V3 so = sameV(0.0f), sd = sameV(0.0f), pending = sameV(0.0f), lightP = sameV(0.0f);
+        float stmax = 0.0f, stime = 0.0f;
+        Event ev;
+        ev.kind = EV_NONE;
+        const DJob* J = busy ? &A.jobs[s.job] : nullptr;
+        if (busy) {
+            if (J->integrator == GND_INTEGRATOR_DIRECT) {
+                r = directStep(A, s, h, shadow, so, sd, stmax, pending, lightP, stime);
+            } else if (J->integrator == GND_INTEGRATOR_AO) {
+                aoStep(A, *J, s, h, shadow, so, sd, stmax, pending, lightP, stime);
+                r = 0;
+            } else {
+                r = neeStep(A, *J, p, s, h, ev, shadow, so, sd, stmax, smedium, pending, lightP, stime, false, false,
+                            sameV(0.0f));
+            }
+            if (FAST && shadow && isBlack(pending)) shadow = false;
+            if (shadow) nShadow++;
+        }
+        // the shadow ray, segment by segment through null-BSDF surfaces
+        // (path.cpp:122-140, volPath.cpp:261-291)
+        {
+            const bool vol = busy && J->integrator == GND_INTEGRATOR_VOLPATH;
+            const bool opaque = busy && (J->integrator == GND_INTEGRATOR_DIRECT || J->integrator == GND_INTEGRATOR_AO);
+            bool open = shadow, visible = false;
+            V3 Tr = sameV(1.0f), o = so;
+            float tmax = stmax;
+            int medium = smedium, iter = 0;
+            while (__ballot(open)) {
+                HitRec sh;
+                const bool hit = traceRay<FAST, false, LDSD, MOTION>(S, o, sd, kEps, tmax, stime, sh, st, cnt, open);
+                if (!open) continue;
+                if (FAST) {
+                    visible = !hit;
+                    open = false;
+                    continue;
+                }
+                int next = 0, nmedium = -1;  // 0 visible, 1 occluded, 2 onward
+                V3 no = sameV(0.0f);
+                float ntmax = 0.0f;
+                if (hit) {
+                    Isect it;
+                    fillInteraction(S, o, sd, sh, it, stime);
+                    const DMaterial& m = S.mats[S.prims[it.prim].material];
+                    if (m.type != GND_MAT_INDEX || opaque) {
+                        next = 1;
+                    } else {
+                        next = 2;
+                        if (vol) {
+                            int inside, outside;
+                            mediumInterface(S, it.prim, inside, outside);
+                            nmedium = dot(sd, it.n) > 0.0f ? outside : inside;
+                            ntmax = length(it.p - lightP) - kEps;
+                        } else {
+                            nmedium = medium;
+                            ntmax = sh.t - length(it.p - o);
+                        }
+                        no = it.p;
+                    }
+                }
+                if (vol && medium >= 0) Tr = Tr * mediumTr(S.media[medium], o, sd, kEps, hit ? sh.t : tmax, s.rng);
+                if (vol && isBlack(Tr)) {
+                    open = false;
+                } else if (next == 0) {
+                    visible = true;
+                    open = false;
+                } else if (next == 2 && ++iter < 1024) {
+                    o = no;
+                    medium = nmedium;
+                    tmax = ntmax;
+                } else {
+                    open = false;
+                }
+            }
+            if (visible) s.L = s.L + (vol ? pending * Tr : pending);
+        }
+        // scatter, or the end of the path
+        bool alive = false;
+        if (busy) {
+            if (r == 1) alive = true;
+            else if (r == 2) alive = scatterStep(S, *J, s, ev);
+        }
+        const bool ending = busy && !alive;
+        if (ending) {
+            storePath(A.P, p, s);
+            busy = false;
+        }
+        retirePaths(A.chunkLive, ending, ending && A.chunkLive ? A.P.chunk[p] : -1);
+    }
+    // queries for the ray statistics (closest-hit and shadow), one atomic per wave
+    unsigned long long a = nRays, b = nShadow;
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_down(a, off);
+        b += __shfl_down(b, off);
+    }
+    if (__lane_id() == 0) {
+        if (a) atomicAdd(&A.ctr->tailRays, a);
+        if (b) atomicAdd(&A.ctr->tailSrays, b);
+    }
+    if (st.overflow) atomicAdd(&A.ctr->stackOverflow, 1ull);
+}
+
 // Film gather (Film::addSample, film.h:26-69): one thread per (job, pixel)
 // sums the filtered contributions of the neighbouring samples of each pass in
 // pass order, so results do not depend on scheduling.
@@ -1486,7 +1652,7 @@ struct gnd_hip_ctx {
     std::mutex mu;
     // 40M paths in flight (~10 GB of path state): a whole 1080p x 16 spp frame
     // per wave, so every bounce is one large launch
-    gnd_hip_tuning tuning{128, 16, 40000000, 0, 0, 0, 0};
+    gnd_hip_tuning tuning{128, 16, 40000000, 0, 0, 0, 0, 0};
     bool counters = false;
     int numCUs = 0;
     size_t hbm = 0;
@@ -1958,6 +2124,10 @@ int gnd_hip_set_tuning(gnd_hip_ctx* ctx, const gnd_hip_tuning* t) {
         ctx->err = "refill lanes must be 0 (per scene) or 1..64";
         return GND_E_INVALID;
     }
+    if (t->tail_paths < -1) {
+        ctx->err = "tail_paths must be -1 (no tail kernel), 0 (per device) or a path count";
+        return GND_E_INVALID;
+    }
     if (t->stack_limit < 0 || t->stack_limit > kStackCap) {
         ctx->err = "stack_limit must be 0 (the full stack) or 1.." + std::to_string(kStackCap);
         return GND_E_INVALID;
@@ -2381,6 +2551,9 @@ struct BouncePlan {
     int B = 128;
     int refill = 32, srefill = 40;
     int stackCap = kStackCap;
+    WaveKernel tail = nullptr;  // kTail for this scene
+    uint32_t tailPaths = 0;     // in-queue sizes at or below this run kTail
+    unsigned tailGridCap = 8;
     size_t shm = 0;
     unsigned gridCap = 8;
     // blocks for n queue entries: one entry per thread up to the grid cap
@@ -2427,11 +2600,20 @@ static BouncePlan makeBouncePlan(const gnd_hip_ctx* ctx, const gnd_hip_scene* sc
     K.refill = ctx->tuning.refill_lanes > 0 ? ctx->tuning.refill_lanes : (small ? 56 : 32);
     K.srefill = ctx->tuning.shadow_refill_lanes > 0 ? ctx->tuning.shadow_refill_lanes : (small ? 64 : 40);
     K.stackCap = stackCapOf(ctx->tuning);
+    if (scene->hasMotion) K.tail = K.fastShadow ? kTail<kTailLds, true, true> : kTail<kTailLds, true, false>;
+    else K.tail = K.fastShadow ? kTail<kTailLds, false, true> : kTail<kTailLds, false, false>;
+    // resident lanes of the tail kernel: GND_TAIL_WAVES waves on each of 4 SIMDs per CU
+    const uint32_t tailLanes = (uint32_t)ctx->numCUs * 4u * GND_TAIL_WAVES * 64u;
+    K.tailGridCap = std::max(8u, tailLanes / (unsigned)K.B);
+    const int32_t tp = ctx->tuning.tail_paths;
+    // default: eight paths per resident lane (1M on 256 CUs), measured near
+    // the best on the 1080p scaled, volumetric and Cornell-box frames
+    K.tailPaths = tp < 0 ? 0u : (tp == 0 ? 8u * tailLanes : (uint32_t)tp);
     return K;
 }
 
 // Device-timed spans of the render stream (kind 0 closest hit, 1 shadow,
-// 2 camera / shading / film).
+// 2 camera / shading / film, 3 tail kernel).
 struct Timeline {
     hipStream_t stream;
     std::vector<hipEvent_t> evs;
@@ -2453,10 +2635,10 @@ struct Timeline {
         spans.push_back({since, now, kind});
         since = now;
     }
-    void sum(double& trace, double& shadow, double& other) const {
+    void sum(double& trace, double& shadow, double& other, double& tail) const {
         for (const Span& sp : spans) {
             const double ms = elapsedMs(evs[sp.a], evs[sp.b]);
-            (sp.kind == 0 ? trace : sp.kind == 1 ? shadow : other) += ms;
+            (sp.kind == 0 ? trace : sp.kind == 1 ? shadow : sp.kind == 3 ? tail : other) += ms;
         }
     }
 };
@@ -2561,6 +2743,18 @@ static hipGraphExec_t bounceGraph(gnd_hip_ctx* ctx, const WaveArgs& A0, const Bo
     return ge;
 }
 
+// The rest of the in-queue's paths in one kTail launch, then the queue
+// counters restart at zero (no path is left).
+static void launchTail(hipStream_t stream, const WaveArgs& A, const BouncePlan& K, uint32_t live, Timeline* tl) {
+    size_t t = tl ? tl->mark() : 0;
+    const unsigned g = std::max(8u, std::min(K.tailGridCap, (live + (unsigned)K.B - 1) / (unsigned)K.B) / 8 * 8);
+    hipLaunchKernelGGL(K.tail, dim3(g), dim3(K.B), (size_t)K.B * kTailLds * sizeof(int), stream, A);
+    GND_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(kFillU32, dim3(1), dim3(64), 0, stream, A.counts, (size_t)kNumCounts, 0u);
+    GND_CHECK(hipGetLastError());
+    if (tl) tl->lap(t, 3);
+}
+
 static uint32_t readLive(gnd_hip_ctx* ctx, const uint32_t* counts) {
     uint32_t live = 0;
     GND_CHECK(hipMemcpyAsync(&live, counts, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
@@ -2596,8 +2790,9 @@ static void clearFilm(hipStream_t stream, const DJob& J) {
 
 static void fillStats(gnd_hip_stats* stats, const Counters& c, uint64_t samples) {
     stats->samples = samples;
-    stats->rays_closest = c.rays;
-    stats->rays_shadow = c.srays;
+    stats->rays_closest = c.rays + c.tailRays;
+    stats->rays_shadow = c.srays + c.tailSrays;
+    stats->rays_tail = c.tailRays;
     stats->trace_launches = c.launches;
     stats->node_visits = c.nodes;
     stats->pack_tests = c.packs;
@@ -2699,6 +2894,11 @@ int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jo
             }
             while (live > 0) {
                 bindQueues(A, ctx->wb.Q, cur);
+                if (live <= K.tailPaths) {
+                    launchTail(ctx->stream, A, K, live, tlp);
+                    live = 0;
+                    break;
+                }
                 launchBounce(ctx->stream, A, K, live, tlp);
                 cur = 1 - cur;
                 live = readLive(ctx, ctx->wb.Q.counts);
@@ -2717,7 +2917,7 @@ int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jo
         if (stats) {
             std::memset(stats, 0, sizeof(*stats));
             stats->ms_total = elapsedMs(tl.evs[t0], tl.evs[t1]);
-            tl.sum(stats->ms_trace, stats->ms_shadow, stats->ms_shade);
+            tl.sum(stats->ms_trace, stats->ms_shadow, stats->ms_shade, stats->ms_tail);
             fillStats(stats, c, samples);
         }
         if (c.stackOverflow) throw HipError(GND_E_STACK, "traversal stack overflow");
@@ -2860,6 +3060,13 @@ int gnd_hip_autotune(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* 
     for (int r : {24, 32, 40, 48, 56, 64}) {
         gnd_hip_tuning t = closest;
         t.shadow_refill_lanes = r;
+        tryTuning(t);
+    }
+    // the path count below which the tail kernel finishes a wave
+    const gnd_hip_tuning shadows = best;
+    for (int tp : {-1, 131072, 524288, 1048576, 2097152}) {
+        gnd_hip_tuning t = shadows;
+        t.tail_paths = tp;
         tryTuning(t);
     }
     // wave capacity: all of the probe's passes in one wave, or 1/2, 1/4, 1/8
@@ -3323,8 +3530,12 @@ void launchLane(gnd_hip_server* s, SrvLane& L) {
     }
     if (L.live > 0) {
         bindQueues(A, L.Q, L.cur);
-        launchBounce(L.stream, A, L.K, L.live, nullptr);
-        L.cur = 1 - L.cur;
+        if (L.live <= L.K.tailPaths) {
+            launchTail(L.stream, A, L.K, L.live, nullptr);
+        } else {
+            launchBounce(L.stream, A, L.K, L.live, nullptr);
+            L.cur = 1 - L.cur;
+        }
     }
     uint32_t used = 0;  // chunk ids are handed out lowest first
     for (int c : L.running) used = std::max(used, (uint32_t)c + 1u);
@@ -3754,6 +3965,8 @@ int gnd_hip_server_stats(gnd_hip_server* s, gnd_hip_stats* stats, int32_t* resid
             sum.siters += c.siters;
             sum.rays += c.rays;
             sum.srays += c.srays;
+            sum.tailRays += c.tailRays;
+            sum.tailSrays += c.tailSrays;
             sum.launches += c.launches;
         }
         std::memset(stats, 0, sizeof(*stats));

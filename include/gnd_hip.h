@@ -132,6 +132,8 @@ typedef struct gnd_hip_stats {
     uint64_t bad_rays;      /* rays with a non-finite direction (counters on)       */
     uint64_t lane_iterations;        /* traversal loop iterations x 64 lanes, closest hit (counters on) */
     uint64_t shadow_lane_iterations; /* the same for shadow rays (counters on)                        */
+    double ms_tail;         /* time in the tail kernel (the last bounces of each wave) */
+    uint64_t rays_tail;     /* closest-hit queries made by the tail kernel (in rays_closest) */
 } gnd_hip_stats;
 
 /* Tunables picked by gnd_hip_autotune or set by the caller. */
@@ -149,6 +151,11 @@ typedef struct gnd_hip_tuning {
                                  reference's int stack[64] per level, TLAS + BLAS);
                                  1..128 caps it, so the GND_E_STACK path can be
                                  exercised on small scenes                              */
+    int32_t tail_paths;       /* once a wave's in-queue holds at most this many paths,
+                                 one persistent tail kernel runs each of them to its
+                                 end (a launch per bounce would last as long as its
+                                 slowest ray); 0 = eight per resident lane of that
+                                 kernel (1M on 256 CUs), -1 = never                    */
 } gnd_hip_tuning;
 
 /* ---- context ------------------------------------------------------------ */

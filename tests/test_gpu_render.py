@@ -121,10 +121,18 @@ def test_cbox_matches_oracle(device):
     # the full scaled-geometry workload's scene (1.31M-tri icosphere + 64 instances)
     ("scaled", "width=96;height=54", 0.99),
 ])
-def test_single_sample_replay(device, name, params, min_frac):
-    """At 1 spp each pixel holds one path: most must equal the oracle's path."""
-    print(f"replay {name} {params}", flush=True)
-    g, _, r, _, _ = _render_pair(device, name, params, 1, squares=False)
+@pytest.mark.parametrize("tail", [0, -1])
+def test_single_sample_replay(device, name, params, min_frac, tail):
+    """At 1 spp each pixel holds one path: most must equal the oracle's path,
+    with the tail kernel finishing the wave (tail 0, the default) and with
+    the bounce kernels throughout (tail -1)."""
+    print(f"replay {name} {params} tail {tail}", flush=True)
+    old = device.tuning()
+    device.set_tuning(tail_paths=tail)
+    try:
+        g, _, r, _, _ = _render_pair(device, name, params, 1, squares=False)
+    finally:
+        device.set_tuning(**old)
     frac = _replay_fraction(g, r)
     print(f"{name} {params}: {frac:.4f} of paths reproduce the oracle", flush=True)
     assert frac >= min_frac
@@ -402,14 +410,51 @@ def test_tunings_give_identical_films(device, params):
     old = device.tuning()
     films = []
     try:
-        for b, l, q, r, sr in [(64, 8, 1 << 20, 0, 0), (128, 16, 64 * 40, 1, 64), (256, 24, 1 << 20, 64, 1),
-                               (128, 32, 3000, 40, 24)]:
-            device.set_tuning(block_size=b, stack_lds=l, queue_capacity=q, refill_lanes=r, shadow_refill_lanes=sr)
+        for b, l, q, r, sr, tp in [(64, 8, 1 << 20, 0, 0, -1), (128, 16, 64 * 40, 1, 64, 0),
+                                   (256, 24, 1 << 20, 64, 1, 500), (128, 32, 3000, 40, 24, -1)]:
+            device.set_tuning(block_size=b, stack_lds=l, queue_capacity=q, refill_lanes=r, shadow_refill_lanes=sr,
+                              tail_paths=tp)
             films.append(device.render(ds, [job])[0][0])
     finally:
         device.set_tuning(**old)
     for f in films[1:]:
         assert np.array_equal(films[0], f)
+
+
+@pytest.mark.parametrize("name,params,spp,aov", [
+    ("cbox", "width=64;height=48", 3, True),
+    ("volumetric", "grid=16;width=48;height=32", 3, True),              # split mode: grid media, Tr shadows
+    ("volumetric", "grid=0;width=48;height=32", 2, False),              # homogeneous medium, fused shading
+    ("scaled", "level=4;instances=12;width=64;height=40", 2, True),     # GGX, instances, thin lens
+    ("cbox", "width=48;height=40;integrator=direct;lights=point,spot,sun,env;envres=16", 3, False),
+    ("volumetric", "grid=16;width=40;height=32;integrator=ao", 2, False),
+    ("cbox", "width=48;height=40;cammove=0.3;camturn=10", 2, False),    # camera motion
+    ("scaled", "level=3;instances=6;width=48;height=32;instmove=0.3", 2, False),  # moving primitives
+])
+def test_tail_kernel_identical_films(device, name, params, spp, aov):
+    """The tail kernel (the last bounces of a wave in one persistent launch)
+    gives the bounce kernels' bits: no tail, a tail from a few hundred paths
+    left, and the tail for everything after the camera rays."""
+    hs = HostScene.generate(name, params)
+    ds = DeviceScene(device, hs)
+    job = hs.default_job(spp=spp)
+    old = device.tuning()
+    out = []
+    try:
+        for tp in (-1, 300, 1 << 30):
+            device.set_tuning(tail_paths=tp)
+            films, st = device.render(ds, [job], want_aov=aov)
+            out.append((films[0], st))
+    finally:
+        device.set_tuning(**old)
+    assert out[0][1]["ms_tail"] == 0 and out[2][1]["ms_tail"] > 0
+    assert out[0][1]["rays_closest"] == out[2][1]["rays_closest"]
+    assert out[0][1]["rays_shadow"] == out[2][1]["rays_shadow"]
+    for f, _ in out[1:]:
+        if aov:
+            assert np.array_equal(out[0][0][0], f[0]) and np.array_equal(out[0][0][1], f[1])
+        else:
+            assert np.array_equal(out[0][0], f)
 
 
 @pytest.mark.parametrize("name,params,spp", [("cbox", "width=64;height=48", 3),
