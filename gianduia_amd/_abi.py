@@ -119,7 +119,8 @@ class gnd_hip_scene_arrays(C.Structure):
 class gnd_hip_job(C.Structure):
     _fields_ = [("camera", gnd_camera_desc), ("spp", C.c_uint32), ("max_depth", C.c_int32),
                 ("integrator", C.c_int32), ("seed_offset", C.c_uint64), ("film", C.c_void_p),
-                ("film_sq", C.c_void_p), ("film_aov", C.c_void_p), ("ao_length", C.c_float)]
+                ("film_sq", C.c_void_p), ("film_aov", C.c_void_p), ("ao_length", C.c_float),
+                ("accumulate", C.c_int32)]
 
 
 class gnd_hip_stats(C.Structure):
@@ -218,6 +219,7 @@ HOST_API = {
     "gnd_host_integrator_film": (C.c_int, [C.c_void_p, f32p, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                            C.POINTER(C.c_int)]),
     "gnd_host_integrator_save": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "gnd_host_integrator_set_output": (C.c_int, [C.c_void_p, C.c_char_p]),
     "gnd_host_integrator_set_aovs": (C.c_int, [C.c_void_p, C.c_int]),
     "gnd_host_integrator_save_aov": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p]),
     "gnd_host_integrator_aovs": (C.c_int, [C.c_void_p, f32p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),

@@ -489,6 +489,13 @@ class Integrator:
         if rc != 0:
             raise GndError(rc, host_error())
 
+    def set_output(self, path_stem: str | None) -> None:
+        """Every render then ends by writing <path_stem>.exr and .png (the
+        reference's film->saveEXR(); film->savePNG()); None turns it off."""
+        rc = _lib().gnd_host_integrator_set_output(self.handle, path_stem.encode() if path_stem else None)
+        if rc != 0:
+            raise GndError(rc, host_error())
+
     def save(self, path: str) -> None:
         rc = _lib().gnd_host_integrator_save(self.handle, path.encode())
         if rc != 0:

@@ -2835,7 +2835,7 @@ int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jo
         std::vector<DJob> dj(jobs.size());
         for (size_t j = 0; j < jobs.size(); ++j) {
             dj[j] = makeDJob(jobs[j]);
-            clearFilm(ctx->stream, dj[j]);
+            if (!jobs[j].accumulate) clearFilm(ctx->stream, dj[j]);
         }
         ensureDev(ctx->dJobs, ctx->jobCap, dj.size());
         GND_CHECK(hipMemcpyAsync(ctx->dJobs, dj.data(), dj.size() * sizeof(DJob), hipMemcpyHostToDevice,
@@ -3500,7 +3500,7 @@ void launchLane(gnd_hip_server* s, SrvLane& L) {
             SrvJob* j = L.freshJobs[k];
             sj[k] = makeDJob(j->job);
             GND_CHECK(hipMemcpyAsync(s->dJobs + j->slot, &sj[k], sizeof(DJob), hipMemcpyHostToDevice, L.stream));
-            clearFilm(L.stream, sj[k]);
+            if (!j->job.accumulate) clearFilm(L.stream, sj[k]);
         }
         uint32_t n = 0, nLive = 0;
         for (size_t k = 0; k < nc; ++k) {

@@ -22,6 +22,7 @@ struct gnd_host_scene {
 struct gnd_host_integrator {
     std::unique_ptr<gnd::Integrator> impl;
     gnd_hip_ctx* ctx = nullptr;
+    std::string output;  // path stem saved as .exr and .png after each render, or empty
 };
 
 namespace {
@@ -219,8 +220,20 @@ int gnd_host_integrator_render(gnd_host_integrator* it, gnd_host_scene* s) {
         rs.device = deviceScene(it->ctx, s);
         rs.desc = &s->scene->desc();
         it->impl->render(rs);
+        // SamplerIntegrator::render ends with film->saveEXR(); film->savePNG()
+        // (integrator.h:152-153), also after a cancel
+        if (!it->output.empty()) {
+            it->impl->film().saveEXR(it->output + ".exr");
+            it->impl->film().savePNG(it->output + ".png");
+        }
         return GND_OK;
     });
+}
+
+int gnd_host_integrator_set_output(gnd_host_integrator* it, const char* path_stem) {
+    if (!it) return GND_E_INVALID;
+    it->output = path_stem ? path_stem : "";
+    return GND_OK;
 }
 
 int gnd_host_integrator_cancel(gnd_host_integrator* it) {

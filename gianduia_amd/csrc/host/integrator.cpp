@@ -72,21 +72,21 @@ void Film::accumulate(const float* rgbw) {
     for (size_t i = 0; i < acc_.size(); ++i) acc_[i] += rgbw[i];
 }
 
-void Film::accumulateAovs(const float* sums) {
-    for (size_t i = 0; i < aovAcc_.size(); ++i) aovAcc_[i] += sums[i];
-}
+void Film::setSums(const float* rgbw) { std::copy(rgbw, rgbw + acc_.size(), acc_.begin()); }
+
+void Film::setAovSums(const float* sums) { std::copy(sums, sums + aovAcc_.size(), aovAcc_.begin()); }
 
 // Film::resolve (src/core/film.cpp:50-94): value = sum * (1 / weight); where
 // the weight is 0, black and the AOV defaults (depth -1)
 void Film::resolve() {
     const size_t n = (size_t)w_ * h_;
     for (size_t i = 0; i < n; ++i) {
-        const double wsum = acc_[4 * i + 3];
-        const float inv = wsum > 0.0 ? 1.0f / (float)wsum : 0.0f;
-        for (int c = 0; c < 3; ++c) rgb_[3 * i + c] = (float)acc_[4 * i + c] * inv;
+        const float wsum = acc_[4 * i + 3];
+        const float inv = wsum > 0.0f ? 1.0f / wsum : 0.0f;
+        for (int c = 0; c < 3; ++c) rgb_[3 * i + c] = acc_[4 * i + c] * inv;
         if (!aovAcc_.empty()) {
-            for (int k = 0; k < kAovs; ++k) aov_[kAovs * i + k] = (float)aovAcc_[kAovs * i + k] * inv;
-            if (!(wsum > 0.0)) aov_[kAovs * i + 6] = -1.0f;
+            for (int k = 0; k < kAovs; ++k) aov_[kAovs * i + k] = aovAcc_[kAovs * i + k] * inv;
+            if (!(wsum > 0.0f)) aov_[kAovs * i + 6] = -1.0f;
         }
     }
 }
@@ -305,14 +305,18 @@ void HipIntegrator::render(const RenderScene& scene) {
             job.seed_offset = (uint64_t)s * (uint64_t)W * (uint64_t)H;
             job.film = dev;
             job.film_aov = devAov;
+            // the passes of every batch after the first add onto the device
+            // film in pass order, so the film holds the bits of one render of
+            // all passes so far whatever the batch size
+            job.accumulate = s > 0 ? 1 : 0;
             const int rc = gnd_hip_render(ctx_, scene.device, &job, 1, nullptr);
             if (rc == GND_E_CANCELLED) break;
             check(rc, ctx_);
             check(gnd_hip_memcpy_d2h(ctx_, acc.data(), dev, acc.size() * sizeof(float)), ctx_);
-            film_.accumulate(acc.data());
+            film_.setSums(acc.data());
             if (devAov) {
                 check(gnd_hip_memcpy_d2h(ctx_, aovSums.data(), devAov, aovSums.size() * sizeof(float)), ctx_);
-                film_.accumulateAovs(aovSums.data());
+                film_.setAovSums(aovSums.data());
             }
             film_.resolve();
             passes_ += (int)job.spp;

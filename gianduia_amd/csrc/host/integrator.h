@@ -11,7 +11,8 @@
 //                   a duplicate name overrides with a warning)
 //
 // The GPU integrators render whole passes on the device through the C ABI
-// (gnd_hip_render) and accumulate them into a host Film between callbacks.
+// (gnd_hip_render), accumulating into one device film, and copy its sums into
+// the host Film between callbacks.
 #pragma once
 
 #include <atomic>
@@ -67,8 +68,9 @@ public:
     void clear();
     // adds device-format accumulators (w*h*4 floats: r, g, b, weight)
     void accumulate(const float* rgbw);
-    // adds device-format AOV sums (w*h*9 floats)
-    void accumulateAovs(const float* sums);
+    // takes a device film's sums as they are (w*h*4 floats), and its AOV sums (w*h*9)
+    void setSums(const float* rgbw);
+    void setAovSums(const float* sums);
     void resolve();
     int width() const { return w_; }
     int height() const { return h_; }
@@ -88,9 +90,9 @@ public:
 
 private:
     int w_ = 0, h_ = 0;
-    std::vector<double> acc_;  // r, g, b, weight per pixel
+    std::vector<float> acc_;  // r, g, b, weight sums per pixel (the device film)
     std::vector<float> rgb_;
-    std::vector<double> aovAcc_;
+    std::vector<float> aovAcc_;
     std::vector<float> aov_;
 };
 
@@ -145,8 +147,9 @@ protected:
 
 // path / volpath on the GPU: passes are rendered in batches of
 // `passesPerCallback` (one gnd_hip_render each, seed offset = first pass *
-// W*H so pass s draws the reference's stream pixelIdx + s*W*H), accumulated
-// into the film, resolved, and reported to the callback.  cancel() stops at
+// W*H so pass s draws the reference's stream pixelIdx + s*W*H, every batch
+// after the first adding onto the device film), copied into the film,
+// resolved, and reported to the callback.  cancel() stops at
 // the next batch and also interrupts the batch in flight.
 class HipIntegrator : public Integrator {
 public:

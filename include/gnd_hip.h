@@ -110,6 +110,10 @@ typedef struct gnd_hip_job {
                                roughness, metallic (9 floats); divide by the film's
                                weight to resolve (film.cpp:50-95)                 */
     float ao_length;        /* GND_INTEGRATOR_AO: occlusion ray tMax (> 0)     */
+    int32_t accumulate;     /* 0: the films are cleared first (Film::clear); 1: the
+                               passes are added onto the sums already in them, so a
+                               render split into calls (seed_offset advancing by
+                               spp*W*H) gives the bits of one call               */
 } gnd_hip_job;
 
 typedef struct gnd_hip_stats {

@@ -87,6 +87,11 @@ int gnd_host_integrator_aovs(gnd_host_integrator* it, float* out, int* width, in
  * ".png" (8-bit sRGB), ".exr" (OpenEXR scanline, float R G B, uncompressed)
  * or ".pfm" (32-bit float RGB). */
 int gnd_host_integrator_save(gnd_host_integrator* it, const char* path);
+/* The end of SamplerIntegrator::render (integrator.h:152-153): with a path
+ * stem set (FileResolver's output path / output name, fileResolver.cpp:44-59),
+ * every render, also a cancelled one, ends by writing <stem>.exr and
+ * <stem>.png of the film; NULL or "" turns it off (the default). */
+int gnd_host_integrator_set_output(gnd_host_integrator* it, const char* path_stem);
 /* One resolved AOV image (Image2D::saveEXR of the film's AOV images,
  * image2d.h:62-97): aov = "albedo" | "normal" (R G B) or "depth" |
  * "roughness" | "metallic" (Y), path ".exr" or ".pfm". */

@@ -127,6 +127,8 @@ def denoise(color: np.ndarray, albedo: np.ndarray | None = None, normal: np.ndar
 
 class RefScene:
     def __init__(self, desc_ptr):
+        d = desc_ptr.contents if hasattr(desc_ptr, "contents") else desc_ptr
+        self.width, self.height = int(d.camera.width), int(d.camera.height)  # the film gnd_ref_render fills
         self.h = lib().gnd_ref_scene_create(desc_ptr)
         if not self.h:
             raise RuntimeError(lib().gnd_ref_last_error().decode())
@@ -193,7 +195,10 @@ class RefScene:
                width=None, height=None, aovs=False):
         """Returns (rgbw, stats), (rgbw, sq, stats) with squares, and the AOV
         sums (H, W, 9) appended with aovs."""
-        W, H = width, height
+        W = self.width if width is None else width
+        H = self.height if height is None else height
+        if (W, H) != (self.width, self.height):  # the oracle writes its camera's whole film
+            raise ValueError(f"film {W}x{H} != the scene camera's {self.width}x{self.height}")
         rgbw = np.zeros((H, W, 4), np.float32)
         sq = np.zeros((H, W, 3), np.float32) if squares else None
         aov = np.zeros((H, W, 9), np.float32) if aovs else None

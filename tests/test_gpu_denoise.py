@@ -27,11 +27,16 @@ def _film(device, params, spp, seed_offset=0):
 
 
 @pytest.mark.parametrize("guides", ["none", "albedo", "both"])
-def test_denoise_matches_oracle(device, guides):
-    c, a, n = _film(device, "width=64;height=48", 4)
+@pytest.mark.parametrize("size", [(64, 48), (37, 29), (33, 17), (5, 70), (1, 1)])
+def test_denoise_matches_oracle(device, guides, size):
+    """Also at sizes that are not multiples of the a-trous steps (1..16), so
+    sub-lattices of unequal width, lattice origins near the edge and the
+    LDS tiles' halo bounds are compared with the oracle."""
+    w, h = size
+    c, a, n = _film(device, f"width={w};height={h}", 4)
     ga = a if guides in ("albedo", "both") else None
     gn = n if guides == "both" else None
-    print(f"denoise 64x48 guides={guides}", flush=True)
+    print(f"denoise {w}x{h} guides={guides}", flush=True)
     got = device.denoise(c, ga, gn)
     want = O.denoise(c, ga, gn)
     # same float operations in the same order; exp differs in the last bits

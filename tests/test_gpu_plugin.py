@@ -31,6 +31,9 @@ def test_callbacks_per_batch_and_cancel(device):
     assert [s[0] for s in seen] == [1, 3, 4]
     assert all(s[1] == (32, 32, 3) for s in seen)
     assert it.film()[1] == 5
+    # batches of 2 + 2 + 1 passes accumulate on the device: the bits of one 5-pass render
+    ds = DeviceScene(device, hs)
+    assert np.array_equal(it.film()[0], resolve(device.render(ds, [hs.default_job(spp=5)])[0][0]))
     # cancel from inside the callback: the render stops after the first batch
     it2 = Integrator(device, "path", max_depth=8, passes_per_callback=2)
     it2.set_callback(lambda idx, rgb: it2.cancel())
@@ -135,3 +138,75 @@ def test_integrator_saves_exr_and_aov_images(device, tmp_path):
     with pytest.raises(GndError):
         it.save_aov("emission", str(tmp_path / "x.exr"))
 
+
+
+def test_cancel_keeps_the_first_passes(device):
+    """A render cancelled after k passes (from the callback, and from another
+    thread) holds exactly the film of the first k passes: the same bits as a
+    render of k passes."""
+    import threading
+    import time
+    hs = HostScene.generate("cbox", "width=48;height=40;spp=64")
+    ds = DeviceScene(device, hs)
+    it = Integrator(device, "path", max_depth=8, passes_per_callback=1)
+    it.set_callback(lambda idx, rgb: it.cancel() if idx == 2 else None)
+    it.render(hs)
+    img, k = it.film()
+    assert k == 3
+    films, _ = device.render(ds, [hs.default_job(spp=k)])
+    assert np.array_equal(img, resolve(films[0]))
+    # from another thread, at whatever pass the render has reached
+    big = HostScene.generate("cbox", "width=256;height=256;spp=100000")
+    it2 = Integrator(device, "path", max_depth=8, passes_per_callback=1)
+    th = threading.Thread(target=lambda: it2.render(big))
+    th.start()
+    time.sleep(0.5)
+    it2.cancel()
+    th.join(timeout=120)
+    assert not th.is_alive()
+    img2, k2 = it2.film()
+    assert 0 < k2 < 100000
+    dsb = DeviceScene(device, big)
+    films, _ = device.render(dsb, [big.default_job(spp=k2)])
+    assert np.array_equal(img2, resolve(films[0]))
+
+
+def test_render_saves_exr_and_png_at_the_end(device, tmp_path):
+    """With an output set, render() ends by writing <output>.exr and
+    <output>.png, as SamplerIntegrator::render does (integrator.h:152-153)."""
+    import imgio
+    hs = HostScene.generate("cbox", "width=32;height=24;spp=2")
+    it = Integrator(device, "path", max_depth=8)
+    it.set_output(str(tmp_path / "cbox"))
+    it.render(hs)
+    assert (tmp_path / "cbox.png").read_bytes()[:4] == b"\x89PNG"
+    _, ch = imgio.read_exr((tmp_path / "cbox.exr").read_bytes())
+    img, _ = it.film()
+    assert np.array_equal(np.stack([ch["R"], ch["G"], ch["B"]], -1), img)
+
+
+def test_nan_and_inf_samples_are_zeroed(device):
+    """SamplerIntegrator::render's guard (integrator.h:137-141): a sample whose
+    radiance has a NaN or an Inf counts once and adds nothing.  A small sphere
+    emitting +inf sits in view beside a finite quad light: the film stays
+    finite, the guard fires, and the GPU film equals the oracle's at the same
+    seeds."""
+    from oracle import RefScene
+    from scenes import SceneBuilder, quad, trs
+    b = SceneBuilder()
+    wall = b.matte((0.6, 0.6, 0.6))
+    b.prim(b.mesh(quad(-3, 3, -3, 3, -3.0, facing=+1)), material=wall)
+    b.prim(b.mesh(quad(-1, 1, -1, 1, 2.5, facing=-1)), material=wall, radiance=(4.0, 4.0, 4.0))
+    M, Mi = trs(trans=(0.6, 0.3, -2.0))
+    b.prim(-1, M, Mi, material=b.matte(), sphere_radius=0.25, radiance=(np.inf, np.inf, np.inf))
+    d = b.desc(width=48, height=48)
+    d.max_depth = 4
+    hs = HostScene.from_desc(d)
+    ds = DeviceScene(device, hs)
+    films, st = device.render(ds, [hs.default_job(spp=2)])
+    g = films[0]
+    assert np.all(np.isfinite(g)) and st["nan_samples"] > 0
+    r, _ = RefScene(hs.desc()).render(threads=8, spp=2)
+    assert np.all(np.isfinite(r))
+    same = np.all(np.isclose(g[..., :3], r[..., :3], rtol=1e-4, atol=1e-6), axis=-1).mean()
+    assert same >= 0.99, same

@@ -539,7 +539,7 @@ def test_render_stats(device):
     films, st = device.render(ds, [hs.default_job(spp=2)])
     assert st["samples"] == 64 * 64 * 2
     assert st["rays_closest"] >= st["samples"]
-    assert st["ms_trace"] > 0
+    assert st["ms_trace"] + st["ms_tail"] > 0  # a small frame runs in the tail kernel after the camera rays
 
 
 def test_render_rejects_bad_jobs(device):
