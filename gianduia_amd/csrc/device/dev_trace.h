@@ -179,42 +179,46 @@ GDEV_INLINE bool finite3(V3 v) {
     return __builtin_isfinite(v.x) && __builtin_isfinite(v.y) && __builtin_isfinite(v.z);
 }
 
-// Up to four index ranges consumed in order (the leaf children of one node).
-struct RangeQueue {
-    uint32_t b0, e0, b1, e1, b2, e2, b3, e3;
-    int n;
-    GDEV_INLINE void clear() { n = 0; }
-    GDEV_INLINE bool empty() const { return n == 0; }
-    // add() and take() are written as per-field selects: stores chosen by
-    // `n`, or a run of plain field copies, are turned into an indexed array or
-    // a memmove by the optimiser, which then keeps the queue in scratch.
-    GDEV_INLINE void add(uint32_t b, uint32_t c) {
-        if (c == 0) return;
-        const uint32_t e = b + c;
-        const bool s0 = n == 0, s1 = n == 1, s2 = n == 2, s3 = n >= 3;
-        b0 = s0 ? b : b0;
-        e0 = s0 ? e : e0;
-        b1 = s1 ? b : b1;
-        e1 = s1 ? e : e1;
-        b2 = s2 ? b : b2;
-        e2 = s2 ? e : e2;
-        b3 = s3 ? b : b3;
-        e3 = s3 ? e : e3;
-        ++n;
+// The leaf children of one node, consumed in child order: each hit leaf
+// child is a range of `count` consecutive entries (triangle packs of a BLAS
+// leaf, primitive slots of a TLAS leaf) starting at its child index.  The
+// node's child and meta words are copied whole and a mask marks the leaves
+// still to consume, so recording a node's leaves is one unconditional copy
+// (no per-child branches whose joins would copy the queue state).
+struct LeafQueue {
+    uint4 child, meta;  // the node's words (count in meta's low 30 bits)
+    uint32_t mask;      // leaf children not consumed yet
+    uint32_t next;      // position inside the lowest one's range
+    GDEV_INLINE void clear() { mask = 0u; }
+    GDEV_INLINE bool empty() const { return mask == 0u; }
+    GDEV_INLINE void set(const uint4& c, const uint4& m, uint32_t leaves) {
+        child = c;
+        meta = m;
+        mask = leaves;
+        next = 0u;
     }
     GDEV_INLINE uint32_t take() {
-        const uint32_t v = b0;
-        const bool next = v + 1u == e0;
-        b0 = next ? b1 : v + 1u;
-        e0 = next ? e1 : e0;
-        b1 = next ? b2 : b1;
-        e1 = next ? e2 : e1;
-        b2 = next ? b3 : b2;
-        e2 = next ? e3 : e2;
-        n -= next ? 1 : 0;
+        const int i = __builtin_ctz(mask);
+        const uint32_t v = compu(child, i) + next;
+        const uint32_t cnt = compu(meta, i) & 0x3FFFFFFFu;
+        const bool last = next + 1u >= cnt;
+        mask = last ? (mask & (mask - 1u)) : mask;
+        next = last ? 0u : next + 1u;
         return v;
     }
 };
+
+// Children of a node with a non-empty leaf range: bit i when child i is a
+// leaf (type 2) with a count above zero.
+GDEV_INLINE uint32_t leafBits(const uint4& meta) {
+    uint32_t m = 0u;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const uint32_t w = compu(meta, i);
+        m |= ((w >> 30) == 2u && (w & 0x3FFFFFFFu) != 0u) ? (1u << i) : 0u;
+    }
+    return m;
+}
 
 enum : int { ACT_NONE = 0, ACT_PACK = 1, ACT_NODE = 2, ACT_INST = 3 };
 
@@ -276,7 +280,7 @@ GDEV_INLINE void traceStream(const DScene& S, TravStack<LDSD>& st, uint32_t* cnt
     hit.t = finf();
     hit.u = hit.v = 0.0f;
     hit.inst = hit.prim = -1;
-    RangeQueue tq, bq;
+    LeafQueue tq, bq;
     tq.clear();
     bq.clear();
     for (;;) {
@@ -353,18 +357,15 @@ GDEV_INLINE void traceStream(const DScene& S, TravStack<LDSD>& st, uint32_t* cnt
             const float rmax = blas ? lt : tcur;
             const bool fast = __all(finite3(ri));
             const uint32_t mask = fast ? slab4Fast(n, ro, ri, rmin, rmax) : slab4(n, ro, ri, rmin, rmax);
+            // inner children are pushed in child order; the hit leaves are
+            // then consumed in child order before the next pop (bvhTraversal.h:78-92)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                if (!((mask >> i) & 1u)) continue;
-                const uint32_t meta = compu(n.meta, i);
-                const uint32_t c = compu(n.child, i);
-                const uint32_t type = meta >> 30;
-                if (type == 1u) st.push((int)c);
-                else if (type == 2u) {
-                    if (blas) bq.add(c, meta & 0x3FFFFFFFu);
-                    else tq.add(c, meta & 0x3FFFFFFFu);
-                }
+                if (((mask >> i) & 1u) && (compu(n.meta, i) >> 30) == 1u) st.push((int)compu(n.child, i));
             }
+            const uint32_t leaves = mask & leafBits(n.meta);
+            if (blas) bq.set(n.child, n.meta, leaves);
+            else tq.set(n.child, n.meta, leaves);
         } else if (level == 1) {
             // Primitive::rayIntersect returned: BVH::rayIntersect keeps the hit
             if (instHit && lt < best) {

@@ -137,13 +137,14 @@ GDEV_INLINE void retirePaths(uint32_t* chunkLive, bool ending, int chunk) {
 constexpr int kShadeBlock = 512;
 // waves per SIMD the shading kernel and the transmittance shadow kernel are
 // compiled for (register budget; on the volumetric scene the shading kernel
-// gains nothing from 3, the persistent transmittance kernel runs 2 waves in
-// 45 ms per frame, 4 waves (with spills) in 52)
+// gains nothing from 3; the persistent transmittance kernel takes 33.8 ms per
+// frame at 3 waves (168 VGPRs or fewer), 43.4 ms at 2 and 52 ms at 4, which
+// spills)
 #ifndef GND_SHADE_WAVES
 #define GND_SHADE_WAVES 2
 #endif
 #ifndef GND_SHADOWTR_WAVES
-#define GND_SHADOWTR_WAVES 2
+#define GND_SHADOWTR_WAVES 3
 #endif
 GDEV_INLINE void blockAppend3(uint32_t* c0, bool p0, uint32_t& pos0, uint32_t* c1, bool p1, uint32_t& pos1,
                               uint32_t* c2, bool p2, uint32_t& pos2) {

@@ -244,3 +244,47 @@ def test_server_stack_overflow_fails_the_job(device):
         device.set_tuning(**old)
         srv.close()
         ds.free()
+
+
+def test_serving_pool_at_full_scale(device):
+    """The serving workload's own pool (8 procedural scenes of 10k..5M
+    triangles, log-uniform) under a Poisson stream at 40 requests/s: every job
+    completes, requests on different scenes overlap on the device (a job is
+    admitted while another is still running), and served films equal solo
+    renders for a job of the smallest and of the largest scene."""
+    from gianduia_amd.serve import scene_pool
+    pool = scene_pool()
+    srv = Server(device, path_slots=8_000_000, max_jobs=64)
+    try:
+        ids = [srv.add_scene(h) for h in pool]
+        rng = np.random.default_rng(17)
+        work = []
+        t = 0.0
+        import time as _t
+        t0 = _t.perf_counter()
+        for k in range(24):
+            t += float(rng.exponential(1 / 40.0))
+            sid = ids[k % len(ids)]
+            job = random_job(rng, pool[sid], res=(128, 640), spp=(2, 8))
+            delay = t - (_t.perf_counter() - t0)
+            if delay > 0:
+                _t.sleep(delay)
+            work.append((sid, job, srv.submit(sid, job)))
+        sts = [srv.wait(tk, 300) for _, _, tk in work]
+        assert all(s["state"] == 2 for s in sts)
+        spans = sorted((s["t_admit_ms"], s["t_done_ms"]) for s in sts)
+        overlapped = sum(1 for (a0, d0), (a1, _) in zip(spans, spans[1:]) if a1 < d0)
+        print(f"serving pool: {overlapped} of {len(spans) - 1} consecutive admissions overlap a running job",
+              flush=True)
+        assert overlapped > 0
+        tris = []
+        for hs in pool:
+            d = hs.desc().contents
+            tris.append(sum(d.meshes[d.prims[i].mesh].ntris for i in range(d.nprims) if d.prims[i].shape_type == 0))
+        for target in (int(np.argmin(tris)), int(np.argmax(tris))):
+            sid, job, tk = next(w for w in work if w[0] == ids[target])
+            ds = DeviceScene(device, pool[target])
+            assert np.array_equal(srv.film(tk), device.render(ds, [job])[0][0]), f"scene {target}: film differs"
+            ds.free()
+    finally:
+        srv.close()
