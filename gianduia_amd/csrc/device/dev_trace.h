@@ -56,6 +56,27 @@ struct TravStack {
         --sp;
         return sp < LDSD ? lds[sp * stride] : spill[sp - LDSD];
     }
+    // Pushes c.x, c.y, c.z, c.w for the set bits 0..3 of `bits`, in that
+    // order.  When four entries fit in LDS (the common case) this is branch
+    // free: child i is written to the slot its rank among the set bits gives,
+    // in increasing i, so a child whose bit is clear is overwritten by the
+    // next pushed one or left above the new top (a dead slot).
+    GDEV_INLINE void pushBits(const uint4& c, uint32_t bits) {
+        const int np = __builtin_popcount(bits);
+        if (sp + 4 <= LDSD && sp + np <= cap) {
+            const int b0 = (int)(bits & 1u), b1 = (int)((bits >> 1) & 1u), b2 = (int)((bits >> 2) & 1u);
+            lds[sp * stride] = (int)c.x;
+            lds[(sp + b0) * stride] = (int)c.y;
+            lds[(sp + b0 + b1) * stride] = (int)c.z;
+            lds[(sp + b0 + b1 + b2) * stride] = (int)c.w;
+            sp += np;
+            return;
+        }
+        if (bits & 1u) push((int)c.x);
+        if (bits & 2u) push((int)c.y);
+        if (bits & 4u) push((int)c.z);
+        if (bits & 8u) push((int)c.w);
+    }
 };
 
 GDEV_INLINE float comp(const float4& v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w)); }
@@ -208,6 +229,14 @@ struct LeafQueue {
     }
 };
 
+// Inner children of a node: bit i when child i is a node (type 1).
+GDEV_INLINE uint32_t innerBits(const uint4& meta) {
+    uint32_t m = 0u;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) m |= ((compu(meta, i) >> 30) == 1u) ? (1u << i) : 0u;
+    return m;
+}
+
 // Children of a node with a non-empty leaf range: bit i when child i is a
 // leaf (type 2) with a count above zero.
 GDEV_INLINE uint32_t leafBits(const uint4& meta) {
@@ -359,10 +388,7 @@ GDEV_INLINE void traceStream(const DScene& S, TravStack<LDSD>& st, uint32_t* cnt
             const uint32_t mask = fast ? slab4Fast(n, ro, ri, rmin, rmax) : slab4(n, ro, ri, rmin, rmax);
             // inner children are pushed in child order; the hit leaves are
             // then consumed in child order before the next pop (bvhTraversal.h:78-92)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                if (((mask >> i) & 1u) && (compu(n.meta, i) >> 30) == 1u) st.push((int)compu(n.child, i));
-            }
+            st.pushBits(n.child, mask & innerBits(n.meta));
             const uint32_t leaves = mask & leafBits(n.meta);
             if (blas) bq.set(n.child, n.meta, leaves);
             else tq.set(n.child, n.meta, leaves);
