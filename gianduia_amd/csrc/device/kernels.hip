@@ -1651,9 +1651,12 @@ struct gnd_hip_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     std::mutex mu;
-    // 40M paths in flight (~10 GB of path state): a whole 1080p x 16 spp frame
-    // per wave, so every bounce is one large launch
-    gnd_hip_tuning tuning{128, 16, 40000000, 0, 0, 0, 0, 0};
+    // 40M paths in flight (~11.5 GB of path state and queues): a whole
+    // 1080p x 16 spp frame per wave, so every bounce is one large launch.
+    // 32 LDS stack entries per lane: as fast as 16 on the 1080p scene, and
+    // the deeper stack rarely spills to scratch (16 traversal waves per CU,
+    // 64 lanes x 32 entries x 4 B each, take 128 KiB of its 160 KiB of LDS)
+    gnd_hip_tuning tuning{128, 32, 40000000, 0, 0, 0, 0, 0};
     bool counters = false;
     int numCUs = 0;
     size_t hbm = 0;
@@ -1681,6 +1684,7 @@ struct gnd_hip_ctx {
     std::vector<CachedGraph> graphs;
     uint64_t graphClock = 0;
     std::map<std::string, gnd_hip_tuning> tuned;  // gnd_hip_autotune results per scene and job shape
+    std::map<std::string, int> chainDepth;         // bounces before the tail kernel, per wave shape
 };
 
 // gnd_hip_autotune's probe: at most this many samples (one default wave)
@@ -2688,6 +2692,18 @@ static void launchBounce(hipStream_t stream, const WaveArgs& A, const BouncePlan
     if (tl) tl->lap(t, 2);
 }
 
+// The rest of the in-queue's paths in one kTail launch, then the queue
+// counters restart at zero (no path is left).
+static void launchTail(hipStream_t stream, const WaveArgs& A, const BouncePlan& K, uint32_t live, Timeline* tl) {
+    size_t t = tl ? tl->mark() : 0;
+    const unsigned g = std::max(8u, std::min(K.tailGridCap, (live + (unsigned)K.B - 1) / (unsigned)K.B) / 8 * 8);
+    hipLaunchKernelGGL(K.tail, dim3(g), dim3(K.B), (size_t)K.B * kTailLds * sizeof(int), stream, A);
+    GND_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(kFillU32, dim3(1), dim3(64), 0, stream, A.counts, (size_t)kNumCounts, 0u);
+    GND_CHECK(hipGetLastError());
+    if (tl) tl->lap(t, 3);
+}
+
 // The instantiated chain of `depth` bounces starting from the queues bound in
 // A.  A chain's nodes hold their kernel arguments and launch shapes by value,
 // so a cached chain whose arguments and shapes are byte-identical is the same
@@ -2696,17 +2712,28 @@ static void launchBounce(hipStream_t stream, const WaveArgs& A, const BouncePlan
 // kGraphCache chains are kept, least recently used replaced first.
 constexpr size_t kGraphCache = 32;
 
-static hipGraphExec_t bounceGraph(gnd_hip_ctx* ctx, const WaveArgs& A0, const BouncePlan& K, uint32_t live,
-                                  int depth) {
+// The bytes a chain's launches are made of: the wave's kernel arguments, the
+// first bounce's launch shapes and kernels.
+static std::string chainKey(const WaveArgs& A0, const BouncePlan& K, uint32_t live) {
     std::string key;
     auto put = [&](const void* p, size_t n) { key.append(static_cast<const char*>(p), n); };
-    const unsigned shape[4] = {K.grid(live), K.shadeGrid(live), K.gridFor(live, 256), (unsigned)K.B};
-    const void* fns[2] = {(const void*)K.trace, (const void*)K.shadow};
-    const int flags[3] = {K.split ? 1 : 0, (int)K.shm, depth};
+    const unsigned shape[6] = {K.grid(live), K.shadeGrid(live), K.gridFor(live, 256), (unsigned)K.B,
+                               K.tailPaths, K.tailGridCap};
+    const void* fns[3] = {(const void*)K.trace, (const void*)K.shadow, (const void*)K.tail};
+    const int flags[2] = {K.split ? 1 : 0, (int)K.shm};
     put(&A0, sizeof(A0));
     put(shape, sizeof(shape));
     put(fns, sizeof(fns));
     put(flags, sizeof(flags));
+    return key;
+}
+
+// `depth` bounces, then (with `tail`) the tail kernel over what is left.
+static hipGraphExec_t bounceGraph(gnd_hip_ctx* ctx, const WaveArgs& A0, const BouncePlan& K, uint32_t live,
+                                  int depth, bool tail) {
+    std::string key = chainKey(A0, K, live);
+    const int tag[2] = {depth, tail ? 1 : 0};
+    key.append(reinterpret_cast<const char*>(tag), sizeof(tag));
     for (auto& c : ctx->graphs)
         if (c.key == key) {
             c.lastUse = ++ctx->graphClock;
@@ -2723,6 +2750,8 @@ static hipGraphExec_t bounceGraph(gnd_hip_ctx* ctx, const WaveArgs& A0, const Bo
             std::swap(A.rIn, A.rOut);
             std::swap(A.gIn, A.gOut);
         }
+        // the tail kernel is persistent: its grid bounds the workers, not the paths
+        if (tail) launchTail(ctx->stream, A, K, K.tailPaths, nullptr);
     } catch (...) {
         if (hipStreamEndCapture(ctx->stream, &gr) == hipSuccess && gr) (void)hipGraphDestroy(gr);
         throw;
@@ -2742,18 +2771,6 @@ static hipGraphExec_t bounceGraph(gnd_hip_ctx* ctx, const WaveArgs& A0, const Bo
     }
     ctx->graphs.push_back({key, gr, ge, ++ctx->graphClock});
     return ge;
-}
-
-// The rest of the in-queue's paths in one kTail launch, then the queue
-// counters restart at zero (no path is left).
-static void launchTail(hipStream_t stream, const WaveArgs& A, const BouncePlan& K, uint32_t live, Timeline* tl) {
-    size_t t = tl ? tl->mark() : 0;
-    const unsigned g = std::max(8u, std::min(K.tailGridCap, (live + (unsigned)K.B - 1) / (unsigned)K.B) / 8 * 8);
-    hipLaunchKernelGGL(K.tail, dim3(g), dim3(K.B), (size_t)K.B * kTailLds * sizeof(int), stream, A);
-    GND_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(kFillU32, dim3(1), dim3(64), 0, stream, A.counts, (size_t)kNumCounts, 0u);
-    GND_CHECK(hipGetLastError());
-    if (tl) tl->lap(t, 3);
 }
 
 static uint32_t readLive(gnd_hip_ctx* ctx, const uint32_t* counts) {
@@ -2883,16 +2900,32 @@ int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jo
             if (tlp) tl.lap(tc, 2);
             int cur = 0;
             uint32_t live = w.live;
-            if (graph) {
-                // a fixed chain of bounces launched as one graph (kernels exit
-                // at once on an empty queue); paths still alive after it
-                // continue below
+            // with the hipGraph chain: the first wave of a shape runs the host
+            // loop below and records how many bounces come before the tail
+            // kernel; later waves of that shape launch those bounces and the
+            // tail as one graph, with no host synchronisation.  Without a
+            // tail kernel the chain is a fixed max depth + 1 bounces (kernels
+            // exit at once on an empty queue) and paths still alive after it
+            // continue in the loop.
+            std::string learnKey;
+            if (graph && K.tailPaths > 0) {
+                bindQueues(A, ctx->wb.Q, cur);
+                std::string key = chainKey(A, K, live);
+                auto it = ctx->chainDepth.find(key);
+                if (it != ctx->chainDepth.end()) {
+                    GND_CHECK(hipGraphLaunch(bounceGraph(ctx, A, K, live, it->second, true), ctx->stream));
+                    live = 0;
+                } else {
+                    learnKey.swap(key);
+                }
+            } else if (graph) {
                 const int depth = std::max(1, w.maxDepth + 1);
                 bindQueues(A, ctx->wb.Q, cur);
-                GND_CHECK(hipGraphLaunch(bounceGraph(ctx, A, K, live, depth), ctx->stream));
+                GND_CHECK(hipGraphLaunch(bounceGraph(ctx, A, K, live, depth, false), ctx->stream));
                 if (depth % 2) cur = 1 - cur;
                 live = readLive(ctx, ctx->wb.Q.counts);
             }
+            int bounces = 0;
             while (live > 0) {
                 bindQueues(A, ctx->wb.Q, cur);
                 if (live <= K.tailPaths) {
@@ -2902,8 +2935,13 @@ int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jo
                 }
                 launchBounce(ctx->stream, A, K, live, tlp);
                 cur = 1 - cur;
+                bounces++;
                 live = readLive(ctx, ctx->wb.Q.counts);
                 if (ctx->cancel.load()) throw HipError(GND_E_CANCELLED, "render cancelled");
+            }
+            if (!learnKey.empty()) {
+                if (ctx->chainDepth.size() >= 4096) ctx->chainDepth.clear();
+                ctx->chainDepth[learnKey] = bounces;
             }
             size_t ts = tl.mark();
             hipLaunchKernelGGL(kSplat, dim3((w.pix + 255) / 256), dim3(256), 0, ctx->stream, A,

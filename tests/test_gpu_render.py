@@ -471,12 +471,16 @@ def test_graph_bounce_chain_identical(device, name, params, spp):
     ref, st0 = device.render(ds, jobs)
     old = device.tuning()
     try:
-        device.set_tuning(use_graph=True)
-        for _ in range(3):
-            got, st = device.render(ds, jobs)
-            for a, b in zip(ref, got):
-                assert np.array_equal(a, b)
-            assert st["rays_closest"] == st0["rays_closest"] and st["rays_shadow"] == st0["rays_shadow"]
+        # the default tail (everything after the camera rays), a tail after a
+        # few bounces, no tail: the first call of each learns the chain, the
+        # next two replay it
+        for tail in (0, 200, -1):
+            device.set_tuning(use_graph=True, tail_paths=tail)
+            for _ in range(3):
+                got, st = device.render(ds, jobs)
+                for a, b in zip(ref, got):
+                    assert np.array_equal(a, b)
+                assert st["rays_closest"] == st0["rays_closest"] and st["rays_shadow"] == st0["rays_shadow"]
     finally:
         device.set_tuning(**old)
 
