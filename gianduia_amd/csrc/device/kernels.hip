@@ -62,10 +62,29 @@ struct PathSoA {
 // xyz mapped to [0,1], depth along the camera forward axis, roughness, metallic
 constexpr int kAovFloats = 9;
 
+// Closest hits per queue entry, one 32-byte record each: (t, u, v, instance)
+// then (primitive, unused x3).  A ray's hit is written once, when its
+// traversal ends, by lanes that end at different times: one record keeps that
+// write inside one 32-byte memory sector instead of dirtying a sector in each
+// of five arrays.
 struct HitSoA {
-    float *t, *u, *v;
-    int32_t *inst, *prim;
+    uint4* rec;
 };
+GDEV_INLINE void storeHit(const HitSoA& H, int64_t i, bool found, const HitRec& h) {
+    H.rec[2 * i] = make_uint4(__float_as_uint(found ? h.t : finf()), __float_as_uint(h.u), __float_as_uint(h.v),
+                              (uint32_t)(found ? h.inst : -1));
+    reinterpret_cast<int32_t*>(H.rec + 2 * i + 1)[0] = found ? h.prim : -1;
+}
+GDEV_INLINE HitRec loadHit(const HitSoA& H, int64_t i) {
+    const uint4 a = H.rec[2 * i];
+    HitRec h;
+    h.t = __uint_as_float(a.x);
+    h.u = __uint_as_float(a.y);
+    h.v = __uint_as_float(a.z);
+    h.inst = (int32_t)a.w;
+    h.prim = reinterpret_cast<const int32_t*>(H.rec + 2 * i + 1)[0];
+    return h;
+}
 
 struct ShadowSoA {
     int32_t *path, *medium;
@@ -478,11 +497,7 @@ GND_TRACE_KERNEL void kTracePaths(WaveArgs A) {
             return true;
         },
         [&](bool found, const HitRec& h, uint32_t nodes) {
-            A.Hq.t[cur] = found ? h.t : finf();
-            A.Hq.inst[cur] = found ? h.inst : -1;
-            A.Hq.prim[cur] = found ? h.prim : -1;
-            A.Hq.u[cur] = h.u;
-            A.Hq.v[cur] = h.v;
+            storeHit(A.Hq, cur, found, h);
             if (COUNT) maxRay = max(maxRay, nodes);
         });
     if (COUNT) flushCounters(A.ctr, cnt, st.overflow, false, maxRay, bad);
@@ -944,7 +959,8 @@ __global__ __launch_bounds__(256) void kMedium(WaveArgs A) {
                 medium = flagMedium(A.P.flags[q]);
                 V3 o, d;
                 loadRay(A.rIn, (uint32_t)i, o, d);
-                const float tm = A.Hq.inst[i] >= 0 ? A.Hq.t[i] : finf();
+                const uint4 hr = A.Hq.rec[2 * (int64_t)i];
+                const float tm = (int32_t)hr.w >= 0 ? __uint_as_float(hr.x) : finf();
                 p = q;
                 rng.s = A.P.rng[q];
                 cur = i;
@@ -1004,12 +1020,7 @@ __global__ __launch_bounds__(kShadeBlock, GND_SHADE_WAVES) void kShade(WaveArgs 
             loadPath(A.P, p, s);
             loadRay(A.rIn, i, s.o, s.d);
             const DJob& J = A.jobs[s.job];
-            HitRec h;
-            h.t = A.Hq.t[i];
-            h.inst = A.Hq.inst[i];
-            h.prim = A.Hq.prim[i];
-            h.u = A.Hq.u[i];
-            h.v = A.Hq.v[i];
+            const HitRec h = loadHit(A.Hq, i);
             Event ev;
             ev.kind = EV_NONE;
             bool pre = false, preHit = false;
@@ -1610,8 +1621,9 @@ struct gnd_hip_scene {
 
 // Queues of one bounce loop for `capacity` paths in flight: hit records and
 // path indices by queue position, the rays of the two path queues, the
-// shadow queue, the two density-grid lists and the queue counters (148 bytes
-// per entry in 37 SoA arrays, each 256-B aligned).  A bounce loop reads and
+// shadow queue, the two density-grid lists and the queue counters (160 bytes
+// per entry: the 32-byte hit record and 32 four-byte SoA arrays, each 256-B
+// aligned).  A bounce loop reads and
 // writes only its own queues, so loops with their own queue sets can run
 // concurrently over one pool of path slots.
 struct QueueSet {
@@ -1887,11 +1899,12 @@ void ensureDev(T*& p, size_t& cap, size_t need) {
     cap = n;
 }
 
-constexpr size_t kPathFields4 = 33, kQueueFields4 = 37;  // four-byte fields per slot / queue entry
+// four-byte fields per slot / per queue entry beside the 32-byte hit record
+constexpr size_t kPathFields4 = 33, kQueueFields4 = 32;
 
 size_t alignedField(size_t cap, size_t elemBytes) { return ((cap * elemBytes + 255) / 256) * 256; }
 size_t pathStateBytes(size_t cap) { return kPathFields4 * alignedField(cap, 4) + alignedField(cap, 8); }
-size_t queueSetBytes(size_t cap) { return kQueueFields4 * alignedField(cap, 4); }
+size_t queueSetBytes(size_t cap) { return kQueueFields4 * alignedField(cap, 4) + alignedField(cap, 32); }
 size_t waveBufferBytes(size_t cap) { return pathStateBytes(cap) + queueSetBytes(cap); }
 
 // Hands out 256-B aligned SoA fields of `cap` entries from one allocation.
@@ -1920,11 +1933,7 @@ void ensureQueueSet(QueueSet& qs, size_t cap) {
     const size_t bytes = queueSetBytes(cap);
     GND_CHECK(hipMalloc(&qs.buf, bytes));
     FieldCarver c{static_cast<char*>(qs.buf), cap};
-    qs.Hq.t = (float*)c.take(4);
-    qs.Hq.u = (float*)c.take(4);
-    qs.Hq.v = (float*)c.take(4);
-    qs.Hq.inst = (int32_t*)c.take(4);
-    qs.Hq.prim = (int32_t*)c.take(4);
+    qs.Hq.rec = (uint4*)c.take(32);
     ShadowSoA& S = qs.Sh;
     S.path = (int32_t*)c.take(4);
     S.medium = (int32_t*)c.take(4);

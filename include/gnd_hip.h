@@ -264,7 +264,7 @@ typedef struct gnd_hip_server_config {
     size_t hbm_budget;     /* bytes for path slots + lane queues + resident scenes;
                               0 = 90% of HBM                                      */
     uint32_t path_slots;   /* paths in flight; a pass needs W*H of them. Each slot
-                              costs 140 B of path state plus 148 B of queues per lane */
+                              costs 140 B of path state plus 160 B of queues per lane */
     uint32_t max_jobs;     /* jobs admitted at once                               */
     uint32_t lanes;        /* concurrent bounce loops (scenes rendered at once),
                               1..16; 0 = 4                                        */

@@ -100,8 +100,8 @@ def test_server_evicts_under_budget(device):
     one = probe.nbytes
     probe.free()
     slots = 16384  # a pass of the largest job (96 x 96) fits
-    # path state and one lane's queues (288 B per slot) plus room for one scene but not two
-    srv = Server(device, path_slots=slots, max_jobs=4, hbm_budget=288 * slots + int(1.5 * one), lanes=1)
+    # path state and one lane's queues (300 B per slot) plus room for one scene but not two
+    srv = Server(device, path_slots=slots, max_jobs=4, hbm_budget=300 * slots + int(1.5 * one), lanes=1)
     try:
         ids = [srv.add_scene(h) for h in scenes]
         tickets = []
