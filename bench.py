@@ -256,6 +256,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-job batching workload")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default tuning instead of running gnd_hip_autotune (untimed) on the workload")
+    ap.add_argument("--tuning", default="",
+                    help="explicit tunables, key=value;... (gnd_hip_tuning field names); skips the autotuner")
     ap.add_argument("--serve", action="store_true",
                     help="only run the serving workload (Poisson arrivals over a scene pool) and print its JSON")
     ap.add_argument("--scene", default="scaled", choices=sorted(WORKLOADS),
@@ -309,8 +311,14 @@ def main():
         f"{ds.nbytes / 2**20:.0f} MiB in HBM")
     job = hs.default_job()
     t_tune = time.perf_counter()
-    tuning = dev.tuning() if a.no_autotune else dev.autotune(ds, job)
-    log(f"tuning {'(default)' if a.no_autotune else '(autotuned)'}: {tuning} in {time.perf_counter() - t_tune:.1f} s")
+    if a.tuning:
+        dev.set_tuning(**{k.strip(): int(v) for k, v in (kv.split("=") for kv in a.tuning.split(";") if kv.strip())})
+        tuning, how = dev.tuning(), "(given)"
+    elif a.no_autotune:
+        tuning, how = dev.tuning(), "(default)"
+    else:
+        tuning, how = dev.autotune(ds, job), "(autotuned)"
+    log(f"tuning {how}: {tuning} in {time.perf_counter() - t_tune:.1f} s")
     sess = RenderSession(dev, ds, [job], want_aov=a.aov)
     for i in range(a.warmup):
         st = sess.run()

@@ -112,9 +112,11 @@ class Device:
         self._check(_lib().gnd_hip_set_tuning(self.ctx, C.byref(t)))
 
     def autotune(self, scene: "DeviceScene", job: "Job") -> dict:
-        """gnd_hip_autotune: times one pass of `job` per block size x LDS stack
-        depth, then per closest-hit and any-hit refill threshold, keeping the
-        fastest of each; returns the chosen tuning."""
+        """gnd_hip_autotune: times a probe of `job` (whole passes, up to 40 M
+        paths) per block size x LDS stack depth, then per closest-hit and
+        any-hit refill threshold, tail-kernel threshold and wave capacity,
+        keeping the fastest of each (every choice renders the same bits);
+        caches the result per scene and job shape; returns the chosen tuning."""
         w, h = job.camera.width, job.camera.height
         film = self.alloc(w * h * 16)
         cj = A.gnd_hip_job()
