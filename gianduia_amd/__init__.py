@@ -6,10 +6,11 @@ include/gnd_host.h.  This package is the Python driver of that library.
 """
 from ._abi import (GND_CAMERA_PERSPECTIVE, GND_CAMERA_THINLENS, GND_FILTER_BOX, GND_FILTER_GAUSSIAN,
                    GND_INTEGRATOR_PATH, GND_INTEGRATOR_VOLPATH, load_library)
-from .api import (Device, DeviceScene, GndError, HostScene, Integrator, Job, camera_lookat, integrator_names,
-                  load_exr, resolve, resolve_aovs, sampler_floats)
+from .api import (Device, DeviceScene, GndError, HostScene, Integrator, Job, SceneObjects, camera_lookat,
+                  integrator_names, load_exr, resolve, resolve_aovs, sampler_floats)
 
-__all__ = ["Device", "DeviceScene", "GndError", "HostScene", "Integrator", "Job", "camera_lookat", "integrator_names",
+__all__ = ["Device", "DeviceScene", "GndError", "HostScene", "Integrator", "Job", "SceneObjects", "camera_lookat",
+           "integrator_names",
            "load_exr", "resolve", "resolve_aovs", "sampler_floats", "load_library",
            "GND_CAMERA_PERSPECTIVE", "GND_CAMERA_THINLENS", "GND_FILTER_BOX", "GND_FILTER_GAUSSIAN",
            "GND_INTEGRATOR_PATH", "GND_INTEGRATOR_VOLPATH"]

@@ -157,6 +157,15 @@ class gnd_hip_job_status(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+GND_PROP_BOOLEAN, GND_PROP_INTEGER, GND_PROP_FLOAT, GND_PROP_STRING, GND_PROP_POINT, GND_PROP_VECTOR2, \
+    GND_PROP_VECTOR, GND_PROP_COLOR, GND_PROP_TRANSFORM = range(9)
+
+
+class gnd_prop(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("kind", C.c_int32), ("ivalue", C.c_int32), ("v", C.c_float * 3),
+                ("svalue", C.c_char_p), ("matrix", f32p)]
+
+
 # exported symbols with their signatures: name -> (restype, argtypes)
 HIP_API = {
     "gnd_hip_init": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
@@ -206,6 +215,12 @@ HOST_API = {
     "gnd_host_scene_load_xml": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
     "gnd_host_scene_generate": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p)]),
     "gnd_host_scene_free": (None, [C.c_void_p]),
+    "gnd_host_builder_create": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    "gnd_host_builder_object": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(gnd_prop),
+                                          C.c_int, C.POINTER(C.c_int32)]),
+    "gnd_host_builder_add_child": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "gnd_host_builder_finish": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p)]),
+    "gnd_host_builder_free": (None, [C.c_void_p]),
     "gnd_host_scene_desc": (C.POINTER(gnd_scene_desc), [C.c_void_p]),
     "gnd_host_scene_build": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.POINTER(gnd_hip_scene_arrays)),
                                        C.POINTER(C.c_double)]),
@@ -244,7 +259,7 @@ def load_library(path: str | None = None) -> C.CDLL:
         return _lib
     p = path or LIB_PATH
     if not os.path.exists(p):
-        raise RuntimeError(f"{p} is not built: run `python -m gianduia_amd.build` (or __graft_entry__.build())")
+        raise RuntimeError(f"{p} is not built: run `python -m gianduia_amd.build`")
     lib = C.CDLL(p)
     for table in (HIP_API, HOST_API):
         for name, (res, args) in table.items():

@@ -317,6 +317,77 @@ class Job:
     ao_length: float = 5.0  # ao integrator: occlusion ray length (ao.cpp:9)
 
 
+_PROP_KINDS = {"boolean": A.GND_PROP_BOOLEAN, "integer": A.GND_PROP_INTEGER, "float": A.GND_PROP_FLOAT,
+               "string": A.GND_PROP_STRING, "point": A.GND_PROP_POINT, "vector2": A.GND_PROP_VECTOR2,
+               "vector": A.GND_PROP_VECTOR, "color": A.GND_PROP_COLOR, "transform": A.GND_PROP_TRANSFORM}
+
+
+class SceneObjects:
+    """The reference's object graph handed over object by object
+    (gnd_host_builder_*, the objects src/core/parser.cpp:41-235 creates):
+    add() each object with its class tag, type name, name and properties,
+    add_child() each parent -> child link in document order, finish() the
+    scene.  A property is (name, kind, value) with kind one of boolean,
+    integer, float, string, point, vector2, vector, color, transform; a
+    transform's value is (matrix, inverse), each 16 floats column-major."""
+
+    def __init__(self, base_dir: str = ""):
+        self.handle = C.c_void_p()
+        rc = _lib().gnd_host_builder_create(base_dir.encode(), C.byref(self.handle))
+        if rc != 0:
+            raise GndError(rc, host_error())
+
+    def add(self, tag: str, type: str | None = None, name: str | None = None, props=()) -> int:
+        props = list(props)
+        arr = (A.gnd_prop * max(1, len(props)))()
+        keep = []
+        for p, (nm, kind, val) in zip(arr, props):
+            p.name = nm.encode()
+            p.kind = _PROP_KINDS[kind]
+            if kind in ("boolean", "integer"):
+                p.ivalue = int(val)
+            elif kind == "float":
+                p.v[0] = float(val)
+            elif kind == "string":
+                p.svalue = str(val).encode()
+            elif kind == "transform":
+                m = (C.c_float * 32)(*[float(x) for x in list(val[0]) + list(val[1])])
+                keep.append(m)
+                p.matrix = C.cast(m, A.f32p)
+            else:
+                for i, x in enumerate(val):
+                    p.v[i] = float(x)
+        oid = C.c_int32()
+        rc = _lib().gnd_host_builder_object(self.handle, tag.encode(), (type or tag).encode(),
+                                            name.encode() if name else None, arr, len(props), C.byref(oid))
+        if rc != 0:
+            raise GndError(rc, host_error())
+        return oid.value
+
+    def add_child(self, parent: int, child: int) -> None:
+        rc = _lib().gnd_host_builder_add_child(self.handle, parent, child)
+        if rc != 0:
+            raise GndError(rc, host_error())
+
+    def finish(self, root: int) -> "HostScene":
+        h = C.c_void_p()
+        rc = _lib().gnd_host_builder_finish(self.handle, root, C.byref(h))
+        if rc != 0:
+            raise GndError(rc, host_error())
+        return HostScene(h)
+
+    def close(self) -> None:
+        if self.handle:
+            _lib().gnd_host_builder_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class HostScene:
     """Host scene shell (XML / procedural / caller-built) with its BVH build."""
 

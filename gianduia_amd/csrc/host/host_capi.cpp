@@ -19,6 +19,10 @@ struct gnd_host_scene {
     std::mutex mu;
 };
 
+struct gnd_host_builder {
+    std::unique_ptr<gnd::ObjectGraph> graph;
+};
+
 struct gnd_host_integrator {
     std::unique_ptr<gnd::Integrator> impl;
     gnd_hip_ctx* ctx = nullptr;
@@ -84,6 +88,45 @@ int gnd_host_scene_load_xml(const char* path, gnd_host_scene** out) {
         return GND_OK;
     });
 }
+
+int gnd_host_builder_create(const char* base_dir, gnd_host_builder** out) {
+    if (!out) return GND_E_INVALID;
+    return hostGuard([&] {
+        auto* b = new gnd_host_builder();
+        b->graph = std::make_unique<gnd::ObjectGraph>(base_dir ? base_dir : "");
+        *out = b;
+        return GND_OK;
+    });
+}
+
+int gnd_host_builder_object(gnd_host_builder* b, const char* tag, const char* type, const char* name,
+                            const gnd_prop* props, int nprops, int32_t* id) {
+    if (!b || !tag || !id) return GND_E_INVALID;
+    return hostGuard([&] {
+        *id = b->graph->add(tag, type ? type : tag, name ? name : "", props, nprops);
+        return GND_OK;
+    });
+}
+
+int gnd_host_builder_add_child(gnd_host_builder* b, int32_t parent, int32_t child) {
+    if (!b) return GND_E_INVALID;
+    return hostGuard([&] {
+        b->graph->addChild(parent, child);
+        return GND_OK;
+    });
+}
+
+int gnd_host_builder_finish(gnd_host_builder* b, int32_t root, gnd_host_scene** out) {
+    if (!b || !out) return GND_E_INVALID;
+    return hostGuard([&] {
+        auto h = std::make_unique<gnd_host_scene>();
+        h->scene = b->graph->build(root);
+        *out = h.release();
+        return GND_OK;
+    });
+}
+
+void gnd_host_builder_free(gnd_host_builder* b) { delete b; }
 
 int gnd_host_scene_generate(const char* name, const char* params, gnd_host_scene** out) {
     if (!name || !out) return GND_E_INVALID;

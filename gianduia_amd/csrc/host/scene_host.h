@@ -7,6 +7,8 @@
 #include <vector>
 
 #include "../../../include/gnd_hip.h"
+
+struct gnd_prop;
 #include "bvh_build.h"
 #include "hmath.h"
 
@@ -86,6 +88,25 @@ gnd_material_desc materialConductor(const float* R, const float* eta, const floa
 gnd_material_desc materialGlass(float eta, float roughness, int inside = -1, int outside = -1);
 gnd_material_desc materialPlastic(const float* Kd, const float* Ks, float eta, float roughness);
 gnd_material_desc materialIndex(int inside, int outside);
+
+// The objects the reference's parser creates (parser.cpp:41-235), handed
+// over one by one: class tag, registered type name, name attribute and
+// PropertyList, then the parent -> child links in document order; build()
+// reads them into a scene exactly as the XML loader reads the same file
+// (xml_loader.cpp).
+class ObjectGraph {
+public:
+    explicit ObjectGraph(const std::string& baseDir);
+    ~ObjectGraph();
+    int add(const std::string& tag, const std::string& type, const std::string& name, const gnd_prop* props,
+            int nprops);
+    void addChild(int parent, int child);
+    std::unique_ptr<HostScene> build(int root) const;
+
+private:
+    struct Impl;
+    std::unique_ptr<Impl> impl_;
+};
 
 // procedural scenes (generators.cpp)
 std::unique_ptr<HostScene> generateScene(const std::string& name, const std::string& params);

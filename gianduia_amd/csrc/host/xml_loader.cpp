@@ -11,12 +11,16 @@
 // (the environment image an OpenEXR file), <ref id>
 // sharing of shapes, and <transform> with translate/scale/rotate/lookat steps
 // composed as t = step * t (parser.cpp:112-165).
+#include <algorithm>
 #include <cctype>
+#include <cstring>
+#include <deque>
 #include <fstream>
 #include <map>
 #include <sstream>
 #include <stdexcept>
 
+#include "../../../include/gnd_host.h"
 #include "integrator.h"
 #include "scene_host.h"
 
@@ -225,6 +229,51 @@ Props readProps(const XNode& n, std::vector<const XNode*>& children) {
     return p;
 }
 
+// An object of the scene description as the reference's parser builds it
+// (parser.cpp:41-235): its class tag ("scene", "primitive", "shape", ...),
+// its registered type name, its name attribute, its properties and its child
+// objects in document order.  A child shared through <ref id> (or given to
+// two parents) is the same node under each parent.
+struct ONode {
+    std::string tag, type, name;
+    Props p;
+    std::vector<const ONode*> kids;
+};
+struct OTree {
+    std::deque<ONode> nodes;  // stable addresses
+    ONode& add() {
+        nodes.emplace_back();
+        return nodes.back();
+    }
+};
+
+// XML element -> object node, registering its id before its children are
+// read, as parser.cpp:188-211 does
+const ONode* fromXml(const XNode& n, OTree& t, std::map<std::string, const ONode*>& ids) {
+    ONode& o = t.add();
+    o.tag = n.tag;
+    o.type = n.attrOr("type");
+    o.name = n.attrOr("name");
+    std::vector<const XNode*> kids;
+    o.p = readProps(n, kids);
+    const std::string id = n.attrOr("id");
+    if (!id.empty()) {
+        if (ids.count(id)) throw std::runtime_error("Parsing error: id <" + id + "> already exists.");
+        ids[id] = &o;
+    }
+    for (const XNode* k : kids) {
+        if (k->tag == "ref") {
+            const std::string rid = k->attrOr("id");
+            auto it = ids.find(rid);
+            if (it == ids.end()) throw std::runtime_error("Parsing error: no object referencing " + rid + " found.");
+            o.kids.push_back(it->second);
+        } else {
+            o.kids.push_back(fromXml(*k, t, ids));
+        }
+    }
+    return &o;
+}
+
 // Minimal OBJ reader (v / vn / vt / f with polygon fans), de-indexed like
 // Mesh::loadOBJ (mesh.cpp:148-229)
 MeshData loadObj(const std::string& path) {
@@ -293,14 +342,13 @@ MeshData loadObj(const std::string& path) {
 struct Loader {
     HostScene& sc;
     std::string baseDir;
-    std::map<std::string, int> meshIds;  // <shape id=...> sharing
-    std::map<std::string, float> sphereRadius;
+    std::map<const ONode*, int> meshIds;  // shapes shared by several primitives
+    std::map<const ONode*, float> sphereRadius;
 
-    int readMedium(const XNode& n) {
-        std::vector<const XNode*> kids;
-        Props p = readProps(n, kids);
+    int readMedium(const ONode& n) {
+        const Props& p = n.p;
         MediumData md;
-        std::string type = n.attrOr("type");
+        const std::string& type = n.type;
         if (type != "homogeneous") throw std::runtime_error("unsupported medium type: " + type);
         md.desc.type = GND_MEDIUM_HOMOGENEOUS;
         Vec3 sa = p.getV("sigma_a", Vec3(0.1f, 0.1f, 0.1f)), ss = p.getV("sigma_s", Vec3(0.1f, 0.1f, 0.1f));
@@ -313,15 +361,14 @@ struct Loader {
         return (int)sc.media.size() - 1;
     }
 
-    int readMaterial(const XNode& n) {
-        std::vector<const XNode*> kids;
-        Props p = readProps(n, kids);
-        std::string type = n.attrOr("type");
+    int readMaterial(const ONode& n) {
+        const Props& p = n.p;
+        const std::string& type = n.type;
         int inside = -1, outside = -1;
-        for (const XNode* k : kids) {
+        for (const ONode* k : n.kids) {
             if (k->tag == "medium") {
                 int id = readMedium(*k);
-                if (k->attrOr("name") == "outside") outside = id;
+                if (k->name == "outside") outside = id;
                 else inside = id;
             }
         }
@@ -352,45 +399,46 @@ struct Loader {
         return sc.addMaterial(m);
     }
 
-    void readPrimitive(const XNode& n) {
-        std::vector<const XNode*> kids;
-        Props p = readProps(n, kids);
+    void readPrimitive(const ONode& n) {
+        const Props& p = n.p;
         Xform x = p.t.count("toWorld") ? p.t.at("toWorld") : Xform();
 
         int mesh = -1, material = -1;
         bool sphere = false, emitter = false;
         float radius = 1.0f, Le[3] = {0, 0, 0};
-        for (const XNode* k : kids) {
-            if (k->tag == "shape" || k->tag == "ref") {
-                std::string id = k->attrOr("id");
-                if (k->tag == "ref") {
-                    if (meshIds.count(id)) mesh = meshIds[id];
-                    else if (sphereRadius.count(id)) { sphere = true; radius = sphereRadius[id]; }
-                    else throw std::runtime_error("Parsing error: no object referencing " + id + " found.");
+        for (const ONode* k : n.kids) {
+            if (k->tag == "shape") {
+                auto mi = meshIds.find(k);
+                if (mi != meshIds.end()) {
+                    mesh = mi->second;
                     continue;
                 }
-                std::vector<const XNode*> sk;
-                Props sp = readProps(*k, sk);
-                std::string type = k->attrOr("type");
+                auto si = sphereRadius.find(k);
+                if (si != sphereRadius.end()) {
+                    sphere = true;
+                    radius = si->second;
+                    continue;
+                }
+                const Props& sp = k->p;
+                const std::string& type = k->type;
                 if (type == "mesh") {
                     std::string f = sp.getS("filename", "");
                     if (f.empty()) throw std::runtime_error("Mesh: filename missing!");
                     std::string path = (!f.empty() && f[0] == '/') ? f : baseDir + f;
                     mesh = sc.addMesh(loadObj(path));
-                    if (!id.empty()) meshIds[id] = mesh;
+                    meshIds[k] = mesh;
                 } else if (type == "sphere") {
                     sphere = true;
                     radius = sp.getF("radius", 1.0f);
-                    if (!id.empty()) sphereRadius[id] = radius;
+                    sphereRadius[k] = radius;
                 } else {
                     throw std::runtime_error("unsupported shape type: " + type);
                 }
             } else if (k->tag == "material") {
                 material = readMaterial(*k);
             } else if (k->tag == "emitter") {
-                std::vector<const XNode*> ek;
-                Props ep = readProps(*k, ek);
-                if (k->attrOr("type") != "area") throw std::runtime_error("only area emitters are supported");
+                const Props& ep = k->p;
+                if (k->type != "area") throw std::runtime_error("only area emitters are supported");
                 emitter = true;
                 Vec3 r = ep.getV("radiance", Vec3(0, 0, 0));
                 Le[0] = r.x; Le[1] = r.y; Le[2] = r.z;
@@ -410,10 +458,9 @@ struct Loader {
     // A scene-level emitter (Scene::addChild, scene.cpp:33-40): point,
     // spot, directional or an environment map read from an EXR file.  Its
     // position in the emitter list is its place in the document.
-    void readEmitter(const XNode& n) {
-        std::vector<const XNode*> kids;
-        Props p = readProps(n, kids);
-        const std::string type = n.attrOr("type");
+    void readEmitter(const ONode& n) {
+        const Props& p = n.p;
+        const std::string& type = n.type;
         int t;
         if (type == "point") t = GND_LIGHT_POINT;
         else if (type == "spot") t = GND_LIGHT_SPOT;
@@ -450,10 +497,9 @@ struct Loader {
         sc.addLight(std::move(l));
     }
 
-    void readCamera(const XNode& n) {
-        std::vector<const XNode*> kids;
-        Props p = readProps(n, kids);
-        std::string type = n.attrOr("type");
+    void readCamera(const ONode& n) {
+        const Props& p = n.p;
+        const std::string& type = n.type;
         Xform x = p.t.count("toWorld") ? p.t.at("toWorld") : Xform();
         sc.setCamera(type == "thinlens" ? GND_CAMERA_THINLENS : GND_CAMERA_PERSPECTIVE, p.getI("width", 1280),
                      p.getI("height", 720), p.getF("fov", 45.0f), x);
@@ -474,11 +520,10 @@ struct Loader {
             c.scan_duration = p.getF("scan_duration", 0.9f);
             c.exposure_duration = p.getF("exposure_duration", 0.1f);
         }
-        for (const XNode* k : kids) {
+        for (const ONode* k : n.kids) {
             if (k->tag == "filter") {
-                std::vector<const XNode*> fk;
-                Props fp = readProps(*k, fk);
-                if (k->attrOr("type") == "box") c.filter = GND_FILTER_BOX;
+                const Props& fp = k->p;
+                if (k->type == "box") c.filter = GND_FILTER_BOX;
                 else {
                     c.filter = GND_FILTER_GAUSSIAN;
                     Vec3 r = fp.getV("radius", Vec3(2.0f, 2.0f, 0.0f));
@@ -490,12 +535,10 @@ struct Loader {
         }
     }
 
-    void readScene(const XNode& root) {
+    void readScene(const ONode& root) {
         if (root.tag != "scene") throw std::runtime_error("root element must be <scene>");
         bool haveCam = false, haveInt = false, haveSampler = false;
-        std::vector<const XNode*> kids;
-        readProps(root, kids);
-        for (const XNode* k : kids) {
+        for (const ONode* k : root.kids) {
             if (k->tag == "camera") {
                 if (haveCam) throw std::runtime_error("Scene: cannot add multiple cameras!");
                 haveCam = true;
@@ -503,14 +546,12 @@ struct Loader {
             } else if (k->tag == "sampler") {
                 if (haveSampler) throw std::runtime_error("Scene: cannot add multiple samplers!");
                 haveSampler = true;
-                std::vector<const XNode*> sk;
-                sc.spp = (uint32_t)readProps(*k, sk).getI("spp", 1);
+                sc.spp = (uint32_t)k->p.getI("spp", 1);
             } else if (k->tag == "integrator") {
                 if (haveInt) throw std::runtime_error("Scene: cannot add multiple integrators!");
                 haveInt = true;
-                std::vector<const XNode*> ik;
-                Props ip = readProps(*k, ik);
-                std::string t = k->attrOr("type");
+                const Props& ip = k->p;
+                const std::string& t = k->type;
                 if (t == "path" || t == "path_hip") sc.integrator = GND_INTEGRATOR_PATH;
                 else if (t == "volpath" || t == "volpath_hip") sc.integrator = GND_INTEGRATOR_VOLPATH;
                 else if (t == "direct" || t == "direct_hip") sc.integrator = GND_INTEGRATOR_DIRECT;
@@ -545,11 +586,83 @@ std::unique_ptr<HostScene> loadXmlScene(const std::string& path) {
     std::string text = ss.str();
     XmlReader r(text);
     XNode root = r.parseDocument();
+    OTree tree;
+    std::map<std::string, const ONode*> ids;
+    const ONode* top = fromXml(root, tree, ids);
     auto sc = std::make_unique<HostScene>();
     Loader L{*sc};
     size_t slash = path.find_last_of('/');
     L.baseDir = slash == std::string::npos ? "" : path.substr(0, slash + 1);
-    L.readScene(root);
+    L.readScene(*top);
+    return sc;
+}
+
+struct ObjectGraph::Impl {
+    OTree tree;
+    std::vector<ONode*> nodes;
+    std::string baseDir;
+};
+
+ObjectGraph::ObjectGraph(const std::string& baseDir) : impl_(std::make_unique<Impl>()) {
+    impl_->baseDir = baseDir;
+    if (!baseDir.empty() && baseDir.back() != '/') impl_->baseDir += '/';
+}
+
+ObjectGraph::~ObjectGraph() = default;
+
+int ObjectGraph::add(const std::string& tag, const std::string& type, const std::string& name, const gnd_prop* props,
+                     int nprops) {
+    // the reference's class tags (GndObject::classTypeToString, object.h)
+    static const char* const kTags[] = {"scene",   "primitive", "shape",    "camera",  "integrator", "sampler",
+                                        "texture", "material",  "emitter", "filter", "medium"};
+    if (std::find_if(std::begin(kTags), std::end(kTags), [&](const char* t) { return tag == t; }) == std::end(kTags))
+        throw std::runtime_error("unknown object class: " + tag);
+    if (nprops < 0 || (nprops > 0 && !props)) throw std::runtime_error("invalid property list");
+    ONode& o = impl_->tree.add();
+    o.tag = tag;
+    o.type = type;
+    o.name = name;
+    for (int i = 0; i < nprops; ++i) {
+        const gnd_prop& q = props[i];
+        if (!q.name) throw std::runtime_error("a property of " + tag + " has no name");
+        const std::string nm = q.name;
+        switch (q.kind) {
+            case GND_PROP_BOOLEAN: o.p.b[nm] = q.ivalue != 0; break;
+            case GND_PROP_INTEGER: o.p.i[nm] = q.ivalue; break;
+            case GND_PROP_FLOAT: o.p.f[nm] = q.v[0]; break;
+            case GND_PROP_STRING: o.p.s[nm] = q.svalue ? q.svalue : ""; break;
+            case GND_PROP_POINT:
+            case GND_PROP_VECTOR:
+            case GND_PROP_COLOR: o.p.v[nm] = Vec3(q.v[0], q.v[1], q.v[2]); break;
+            case GND_PROP_VECTOR2: o.p.v[nm] = Vec3(q.v[0], q.v[1], 0.0f); break;
+            case GND_PROP_TRANSFORM: {
+                if (!q.matrix) throw std::runtime_error("transform " + nm + " has no matrix");
+                Xform x;
+                std::memcpy(x.m.m, q.matrix, sizeof(x.m.m));
+                std::memcpy(x.inv.m, q.matrix + 16, sizeof(x.inv.m));
+                o.p.t[nm] = x;
+                break;
+            }
+            default: throw std::runtime_error("property " + nm + " has an unknown kind");
+        }
+    }
+    impl_->nodes.push_back(&o);
+    return (int)impl_->nodes.size() - 1;
+}
+
+void ObjectGraph::addChild(int parent, int child) {
+    const int n = (int)impl_->nodes.size();
+    if (parent < 0 || parent >= n || child < 0 || child >= n || parent == child)
+        throw std::runtime_error("invalid object ids");
+    impl_->nodes[parent]->kids.push_back(impl_->nodes[child]);
+}
+
+std::unique_ptr<HostScene> ObjectGraph::build(int root) const {
+    if (root < 0 || root >= (int)impl_->nodes.size()) throw std::runtime_error("invalid root object");
+    auto sc = std::make_unique<HostScene>();
+    Loader L{*sc};
+    L.baseDir = impl_->baseDir;
+    L.readScene(*impl_->nodes[root]);
     return sc;
 }
 

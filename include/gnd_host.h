@@ -36,6 +36,55 @@ int gnd_host_scene_load_xml(const char* path, gnd_host_scene** out);
 int gnd_host_scene_generate(const char* name, const char* params, gnd_host_scene** out);
 void gnd_host_scene_free(gnd_host_scene* s);
 
+/* ---- the reference's object graph, object by object ----
+ * A scene made from the objects the reference's parser creates instead of
+ * from a file: each object with its class tag, registered type name, name
+ * attribute and PropertyList (GndFactory::createInstance(type, props),
+ * src/core/parser.cpp:171-193), then each parent -> child link in document
+ * order (GndObject::addChild, parser.cpp:200-211).  gnd_host_builder_finish
+ * reads the graph exactly as gnd_host_scene_load_xml reads a file holding
+ * the same objects, so a binding can hand over the Scene* the reference has
+ * built.  Relative file names (OBJ meshes, EXR maps) resolve against
+ * base_dir (FileResolver's base path). */
+enum {
+    GND_PROP_BOOLEAN = 0, /* PropertyList::setBoolean: ivalue 0 / 1           */
+    GND_PROP_INTEGER = 1, /* setInteger: ivalue                               */
+    GND_PROP_FLOAT = 2,   /* setFloat: v[0]                                   */
+    GND_PROP_STRING = 3,  /* setString: svalue                                */
+    GND_PROP_POINT = 4,   /* setPoint3: v[0..2]                               */
+    GND_PROP_VECTOR2 = 5, /* setVector2 / setPoint2: v[0..1]                  */
+    GND_PROP_VECTOR = 6,  /* setVector3: v[0..2]                              */
+    GND_PROP_COLOR = 7,   /* setColor: v[0..2]                                */
+    GND_PROP_TRANSFORM = 8 /* setTransform: matrix = 32 floats, the matrix and
+                              then its inverse (Transform::getMatrix /
+                              getInverse), each column-major                 */
+};
+typedef struct gnd_prop {
+    const char* name;
+    int32_t kind;        /* GND_PROP_* */
+    int32_t ivalue;
+    float v[3];
+    const char* svalue;
+    const float* matrix;
+} gnd_prop;
+typedef struct gnd_host_builder gnd_host_builder;
+int gnd_host_builder_create(const char* base_dir, gnd_host_builder** out);
+/* tag: "scene", "primitive", "shape", "camera", "integrator", "sampler",
+ * "texture", "material", "emitter", "filter" or "medium"
+ * (GndObject::classTypeToString, include/gianduia/core/object.h); type: the
+ * name the object was created by ("mesh", "matte", "thinlens", ...; the tag
+ * for scene and primitive); name: the object's name or NULL.  The
+ * properties are copied.  *id receives the object's id. */
+int gnd_host_builder_object(gnd_host_builder* b, const char* tag, const char* type, const char* name,
+                            const gnd_prop* props, int nprops, int32_t* id);
+/* parent->addChild(child); an object added to two parents is shared (a
+ * <ref id>). */
+int gnd_host_builder_add_child(gnd_host_builder* b, int32_t parent, int32_t child);
+/* The scene rooted at `root` (a "scene" object), with the XML loader's
+ * checks and errors. */
+int gnd_host_builder_finish(gnd_host_builder* b, int32_t root, gnd_host_scene** out);
+void gnd_host_builder_free(gnd_host_builder* b);
+
 /* Description of the scene (valid until the scene is freed). */
 const gnd_scene_desc* gnd_host_scene_desc(gnd_host_scene* s);
 /* Builds (once) and returns the flattened arrays (valid until freed).

@@ -1,5 +1,5 @@
-"""Pins for the oracle's heterogeneous medium and volumetric path integrator
-(SURVEY rows a16 and a26): closed forms that a dropped term, a wrong cell
+"""Pins for the oracle's heterogeneous medium and volumetric path integrator:
+closed forms that a dropped term, a wrong cell
 size, an unskipped empty cell or a biased estimator would fail.
 
 * Grid DDA with several voxels per macro cell: alternate slabs of macro cells
