@@ -23,11 +23,16 @@
  * Build flags: -O3 -ffp-contract=off -fno-fast-math (no FMA contraction).
  *
  * How the restatement is pinned (tests/test_oracle_known_answers.py,
- * tests/test_oracle_pins.py): the reference's own test values (Fresnel, GGX,
+ * tests/test_oracle_pins.py, tests/test_oracle_media_pins.py,
+ * tests/test_golden_rays.py): the reference's own test values (Fresnel, GGX,
  * BSDF lobes, camera, bounds, frame, colour, sphere), published PCG32 and
- * SplitMix64 outputs, brute-force traversal, warp chi^2 / KS tests, BSDF
- * energy and pdf integrals, HG normalisation, media closed forms and the
- * integrator furnace.  Parity unpinned (no reference test or closed form
+ * SplitMix64 outputs, brute-force traversal (also of the committed ray-batch
+ * fixtures), warp chi^2 / KS tests, BSDF energy and pdf integrals, HG
+ * normalisation, homogeneous-media closed forms, the grid DDA over
+ * multi-voxel macro cells (ratio tracking's mean and delta tracking's
+ * escape probability and collision CDF against the closed-form optical
+ * depth), and furnaces for the path, direct and volumetric path integrators
+ * (the latter with a homogeneous and a grid medium).  Parity unpinned (no reference test or closed form
  * fixes them): the glm scalar evaluation orders above, glm::decompose /
  * slerp / mat4_cast / inverse of moving transforms (restated in
  * gnd_ref_motion.cpp, pinned only by closed forms), the Highway lane
