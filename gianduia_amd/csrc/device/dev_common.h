@@ -162,6 +162,10 @@ struct alignas(16) DMedium {
     M4 toLocal;
     const float* density;
     const float* majorants;
+    // per voxel (i, j, k) of the grid: the 8 corners of its trilinear cell,
+    // (i + di, j + dj, k + dk) at 4 di + 2 dj + dk (0 beyond the grid), as 2
+    // float4 in one 32-byte sector; null for grids too large to expand
+    const float4* corners;
 };
 
 // Point / spot / directional / environment emitter

@@ -840,7 +840,9 @@ GDEV_INLINE float hgSample(float g, V3 wo, V3& wi, float ux, float uy) {  // pha
 }
 GDEV_INLINE float hgClamp(float g) { return clampf(g, -0.999f, 0.999f); }
 
-// OpenVDB BoxSampler::isSample on the dense grid: lerp z, then y, then x
+// OpenVDB BoxSampler::isSample on the dense grid: lerp z, then y, then x.
+// The 8 corner values come from the voxel's corner record (one sector)
+// inside the grid, else one by one.
 GDEV_INLINE float gridFetch(const DMedium& m, int i, int j, int k) {
     if (i < 0 || j < 0 || k < 0 || i >= m.nx || j >= m.ny || k >= m.nz) return 0.0f;
     return m.density[((size_t)k * m.ny + j) * m.nx + i];
@@ -848,10 +850,18 @@ GDEV_INLINE float gridFetch(const DMedium& m, int i, int j, int k) {
 GDEV_INLINE float gridSample(const DMedium& m, double x, double y, double z) {
     int ix = (int)floor(x), iy = (int)floor(y), iz = (int)floor(z);
     double ux = x - ix, uy = y - iy, uz = z - iz;
-    float d000 = gridFetch(m, ix, iy, iz), d001 = gridFetch(m, ix, iy, iz + 1);
-    float d010 = gridFetch(m, ix, iy + 1, iz), d011 = gridFetch(m, ix, iy + 1, iz + 1);
-    float d100 = gridFetch(m, ix + 1, iy, iz), d101 = gridFetch(m, ix + 1, iy, iz + 1);
-    float d110 = gridFetch(m, ix + 1, iy + 1, iz), d111 = gridFetch(m, ix + 1, iy + 1, iz + 1);
+    float d000, d001, d010, d011, d100, d101, d110, d111;
+    if (m.corners && (unsigned)ix < (unsigned)m.nx && (unsigned)iy < (unsigned)m.ny && (unsigned)iz < (unsigned)m.nz) {
+        const float4* r = m.corners + 2 * (((size_t)iz * m.ny + iy) * m.nx + ix);
+        const float4 c0 = r[0], c1 = r[1];
+        d000 = c0.x; d001 = c0.y; d010 = c0.z; d011 = c0.w;
+        d100 = c1.x; d101 = c1.y; d110 = c1.z; d111 = c1.w;
+    } else {
+        d000 = gridFetch(m, ix, iy, iz); d001 = gridFetch(m, ix, iy, iz + 1);
+        d010 = gridFetch(m, ix, iy + 1, iz); d011 = gridFetch(m, ix, iy + 1, iz + 1);
+        d100 = gridFetch(m, ix + 1, iy, iz); d101 = gridFetch(m, ix + 1, iy, iz + 1);
+        d110 = gridFetch(m, ix + 1, iy + 1, iz); d111 = gridFetch(m, ix + 1, iy + 1, iz + 1);
+    }
     float a = d000 + (float)((d001 - d000) * uz);
     float b = d010 + (float)((d011 - d010) * uz);
     float r1 = a + (float)((b - a) * uy);

@@ -1735,6 +1735,21 @@ int guarded(gnd_hip_ctx* ctx, F&& f) {
     }
 }
 
+// Corner records of a density grid (the layout DMedium::corners holds):
+// thread v writes voxel v's 8 trilinear corners, 0 beyond the grid.
+__global__ __launch_bounds__(256) void kGridCorners(const float* __restrict__ d, float4* __restrict__ out, int nx,
+                                                    int ny, int nz) {
+    const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t nxy = (size_t)nx * ny;
+    if (v >= nxy * (size_t)nz) return;
+    const int x = (int)(v % (size_t)nx), y = (int)((v / (size_t)nx) % (size_t)ny), z = (int)(v / nxy);
+    auto at = [&](int i, int j, int k) { return (i < nx && j < ny && k < nz) ? d[(size_t)k * nxy + (size_t)j * nx + i] : 0.0f; };
+    out[2 * v] = make_float4(at(x, y, z), at(x, y, z + 1), at(x, y + 1, z), at(x, y + 1, z + 1));
+    out[2 * v + 1] = make_float4(at(x + 1, y, z), at(x + 1, y, z + 1), at(x + 1, y + 1, z), at(x + 1, y + 1, z + 1));
+}
+// grids whose corner records would take more than this keep the one-by-one lookups
+constexpr size_t kGridCornerBytes = size_t(4) << 30;
+
 template <class T>
 T* devAlloc(gnd_hip_scene* s, size_t count, const T* src) {
     if (count == 0) count = 1;
@@ -2376,6 +2391,16 @@ static int sceneUpload(gnd_hip_ctx* ctx, const gnd_hip_scene_arrays* a, gnd_hip_
                         }
                 M.density = devAlloc<float>(s, (size_t)m.nx * m.ny * m.nz, m.density);
                 M.majorants = devAlloc<float>(s, maj.size(), maj.data());
+                // corner records of the voxels, expanded on the device
+                const size_t nv = (size_t)m.nx * m.ny * m.nz;
+                if (nv * 32 <= kGridCornerBytes) {
+                    float4* cr = devAlloc<float4>(s, 2 * nv, nullptr);
+                    hipLaunchKernelGGL(kGridCorners, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, nullptr,
+                                       M.density, cr, m.nx, m.ny, m.nz);
+                    GND_CHECK(hipGetLastError());
+                    GND_CHECK(hipStreamSynchronize(nullptr));
+                    M.corners = cr;
+                }
             }
         }
         for (uint32_t i = 0; i < d.nmaterials; ++i)
