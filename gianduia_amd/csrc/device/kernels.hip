@@ -1329,8 +1329,13 @@ __global__ __launch_bounds__(256, GND_SHADOWTR_WAVES) void kShadow(WaveArgs A) {
 // Traversals are wave-collective (traceRay): lanes without a ray take part
 // idle.  FAST: every shadow ray is one any-hit query (no null-BSDF surface,
 // no medium on a segment).
+// Waves per SIMD the tail kernel is compiled for: at 4 its registers are cut
+// from about 240 to 128 and some spill, yet more resident walks and
+// traversals hide more memory latency (best frame, over the tail threshold:
+// 1080p 67.5 ms at 2 waves, 66.4 at 3, 65.9 at 4, 69.1 at 5; volumetric
+// 97.6, 95.4, 91.7 at 4 with a 4 M threshold, 101.6 at 5).
 #ifndef GND_TAIL_WAVES
-#define GND_TAIL_WAVES 2
+#define GND_TAIL_WAVES 4
 #endif
 // a lane's work between refills is a whole bounce, so idle lanes refill early
 constexpr int kTailRefill = 8;
@@ -3137,7 +3142,7 @@ int gnd_hip_autotune(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* 
     }
     // the path count below which the tail kernel finishes a wave
     const gnd_hip_tuning shadows = best;
-    for (int tp : {-1, 131072, 524288, 1048576, 2097152}) {
+    for (int tp : {-1, 131072, 524288, 1048576, 2097152, 4194304, 8388608}) {
         gnd_hip_tuning t = shadows;
         t.tail_paths = tp;
         tryTuning(t);
@@ -3326,7 +3331,11 @@ size_t sceneEstimate(const gnd_hip_scene_arrays* a) {
     const gnd_scene_desc& d = *a->desc;
     b += (size_t)d.nprims * (sizeof(DInst) + sizeof(DPrim)) + (size_t)d.nmaterials * sizeof(DMaterial);
     for (uint32_t m = 0; m < d.nmedia; ++m)
-        if (d.media[m].type == GND_MEDIUM_GRID) b += (size_t)d.media[m].nx * d.media[m].ny * d.media[m].nz * 4 * 2;
+        if (d.media[m].type == GND_MEDIUM_GRID) {
+            // the dense grid with its majorants, and the corner records
+            const size_t nv = (size_t)d.media[m].nx * d.media[m].ny * d.media[m].nz;
+            b += nv * 4 * 2 + (nv * 32 <= kGridCornerBytes ? nv * 32 : 0);
+        }
     return b;
 }
 

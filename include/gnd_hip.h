@@ -159,7 +159,7 @@ typedef struct gnd_hip_tuning {
                                  one persistent tail kernel runs each of them to its
                                  end (a launch per bounce would last as long as its
                                  slowest ray); 0 = eight per resident lane of that
-                                 kernel (1M on 256 CUs), -1 = never                    */
+                                 kernel (2M on 256 CUs), -1 = never                    */
 } gnd_hip_tuning;
 
 /* ---- context ------------------------------------------------------------ */
@@ -235,9 +235,10 @@ int gnd_hip_denoise_noalloc(gnd_hip_ctx* ctx, const float* color, const float* a
                             int width, int height, float* output, double* ms);
 
 /* Picks the block size (64/128/256) and LDS stack depth (8/16/24/32), then
- * the closest-hit and the any-hit refill thresholds (24..64), then the wave
- * capacity (all of the probe's passes in one wave, or a half, a quarter or an
- * eighth of them per wave), by timing `job` cut to at most 40M samples on
+ * the closest-hit and the any-hit refill thresholds (24..64), then the
+ * tail-kernel threshold (off, 128K..8M paths), then the wave capacity (all of
+ * the probe's passes in one wave, or a half, a quarter or an eighth of them
+ * per wave), by timing `job` cut to at most 40M samples on
  * `scene` (best of two) per candidate, keeping the fastest of each in turn;
  * sets the result as the context's tuning and returns it.  The result is
  * remembered per scene and job shape, so a second call returns at once.
