@@ -204,12 +204,79 @@ def batch_workload(dev, ds, hs):
     return res
 
 
-def serve_workload(qps_levels=(5, 10, 20, 50, 100, 200), jobs_per_level=40):
+# The server's slot pool: a pass holds its W*H slots until its last path ends,
+# so most slots of a deep pass sit idle in its late bounces; a large pool keeps
+# the bounce launches full (saturation on the serving mix, 4 lanes: 16M slots
+# 320, 40M 372, 120M 430 Msamples/s).  120M slots and 4 lanes' queues are
+# 94 GB of the 288 GB.
+SERVE_SLOTS = 120_000_000
+SERVE_LANES = 4
+
+
+def serve_vs_serial(dev, srv, ids, pool, n_jobs=48, seed=1000):
+    """The same n_jobs random jobs over the scene pool rendered three ways,
+    films resident in HBM: `served` = all submitted to the continuous-batching
+    server at once (saturation), `plain` = one job at a time through
+    gnd_hip_render with one pass per wave (the reference's pass loop,
+    integrator.h:79-150), `solo` = one job at a time with whole-frame waves.
+    Samples per second of each, and the served latency percentiles."""
+    import numpy as np
+    from gianduia_amd import DeviceScene
+    from gianduia_amd.api import RenderSession
+    from gianduia_amd.serve import random_job
+    rng = np.random.default_rng(seed)
+    plan = []
+    for _ in range(n_jobs):
+        sid = int(rng.choice(ids))
+        plan.append((sid, random_job(rng, pool[sid])))
+    samples = sum(j.camera.width * j.camera.height * j.spp for _, j in plan)
+    res = {"jobs": n_jobs, "samples": samples}
+    films = [dev.alloc(j.camera.width * j.camera.height * 16) for _, j in plan]
+    t0 = time.perf_counter()
+    tickets = [srv.submit(sid, j, film=f) for (sid, j), f in zip(plan, films)]
+    lat = []
+    for tk in tickets:
+        lat.append(srv.wait(tk)["latency_ms"])
+        srv.release(tk)
+    served_s = time.perf_counter() - t0
+    for f in films:
+        f.free()
+    res["served_s"] = round(served_s, 3)
+    res["served_p50_ms"] = round(float(np.percentile(lat, 50)), 2)
+    res["served_p99_ms"] = round(float(np.percentile(lat, 99)), 2)
+    scenes = {sid: DeviceScene(dev, pool[sid]) for sid in sorted({sid for sid, _ in plan})}
+    sessions = [RenderSession(dev, scenes[sid], [j]) for sid, j in plan]
+    old = dev.tuning()
+    try:
+        for sess in sessions:  # warm-up: buffers and code objects
+            sess.run()
+        for mode in ("plain", "solo"):
+            t0 = time.perf_counter()
+            for (_, j), sess in zip(plan, sessions):
+                if mode == "plain":
+                    dev.set_tuning(queue_capacity=max(1024, j.camera.width * j.camera.height))
+                sess.run()
+            res[f"{mode}_s"] = round(time.perf_counter() - t0, 3)
+            dev.set_tuning(**old)
+    finally:
+        dev.set_tuning(**old)
+        del sessions
+        for sc in scenes.values():
+            sc.free()
+    for mode in ("served", "plain", "solo"):
+        res[f"{mode}_msamples_per_s"] = round(samples / res[f"{mode}_s"] / 1e6, 2)
+    res["served_vs_plain"] = round(res["plain_s"] / res["served_s"], 2)
+    res["served_vs_solo"] = round(res["solo_s"] / res["served_s"], 2)
+    return res
+
+
+def serve_workload(qps_levels=(5, 10, 20, 50, 100, 200), jobs_per_level=100):
     """Serving workload: a pool of 8 procedural scenes (1e4..5e6 triangles,
     log-uniform, seeded), Poisson arrivals at each rate, jobs with random pose,
     width log-uniform in 128..1920 (height 9/16..1 of it), spp log-uniform in
     4..64, rendered by the continuous-batching server.  Latency is submission
-    to film complete."""
+    to film complete.  Then the saturation throughput of the server against
+    the serial plain and solo kernels on one job mix (serve_vs_serial)."""
     from gianduia_amd import Device
     from gianduia_amd.serve import Server, poisson_load, scene_pool
     dev = Device(0)
@@ -220,7 +287,7 @@ def serve_workload(qps_levels=(5, 10, 20, 50, 100, 200), jobs_per_level=40):
         hs.build()
         build_ms.append(round(hs.build_ms, 1))
     log(f"serve: scene pool built in {time.time() - t0:.1f} s ({build_ms} ms BVH builds)")
-    srv = Server(dev, path_slots=16_000_000, max_jobs=256)
+    srv = Server(dev, path_slots=SERVE_SLOTS, max_jobs=256, lanes=SERVE_LANES)
     ids = [srv.add_scene(h) for h in pool]
     tris = []
     for hs in pool:
@@ -235,14 +302,18 @@ def serve_workload(qps_levels=(5, 10, 20, 50, 100, 200), jobs_per_level=40):
         r = poisson_load(srv, ids, qps=qps, n_jobs=jobs_per_level, seed=k)
         log(f"serve: {json.dumps(r)}")
         levels.append(r)
+    vs = serve_vs_serial(dev, srv, ids, pool)
+    log(f"serve vs serial: {json.dumps(vs)}")
     st = srv.stats()
     srv.close()
     dev.close()
     return {"metric": "p50/p99 job latency @QPS", "unit": "ms", "mode": "serve", "n_gpus": 1,
             "data": "synthetic (procedural scenes and jobs, seeded)",
-            "config": {"scenes": len(pool), "instanced_triangles": tris, "path_slots": 16_000_000,
+            "config": {"scenes": len(pool), "instanced_triangles": tris, "path_slots": SERVE_SLOTS,
+                       "lanes": SERVE_LANES,
                        "jobs_per_level": jobs_per_level, "width": [128, 1920], "spp": [4, 64]},
             "scene_build_ms": build_ms, "scene_upload_s": round(upload_s, 2), "levels": levels,
+            "vs_serial": vs,
             "rays_closest": st["rays_closest"], "rays_shadow": st["rays_shadow"]}
 
 

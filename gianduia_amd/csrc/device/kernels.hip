@@ -1668,12 +1668,15 @@ struct gnd_hip_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     std::mutex mu;
-    // 40M paths in flight (~11.5 GB of path state and queues): a whole
-    // 1080p x 16 spp frame per wave, so every bounce is one large launch.
+    // Up to 120M paths in flight per wave (300 B each with one queue set,
+    // 36 GB of the 288 GB; a render allocates only what its jobs' samples
+    // need): a 1080p x 16 spp frame is one wave, and a batch of many jobs
+    // takes few waves, so few bounces run below the GPU's width (64-job
+    // batch over the 1080p scene: 40M slots 347, 120M slots 371 Msamples/s).
     // 32 LDS stack entries per lane: as fast as 16 on the 1080p scene, and
     // the deeper stack rarely spills to scratch (16 traversal waves per CU,
     // 64 lanes x 32 entries x 4 B each, take 128 KiB of its 160 KiB of LDS)
-    gnd_hip_tuning tuning{128, 32, 40000000, 0, 0, 0, 0, 0};
+    gnd_hip_tuning tuning{128, 32, 120000000, 0, 0, 0, 0, 0};
     bool counters = false;
     int numCUs = 0;
     size_t hbm = 0;
@@ -2884,7 +2887,8 @@ int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jo
             anyVol = anyVol || j.integrator == GND_INTEGRATOR_VOLPATH;
             samples += (uint64_t)j.camera.width * j.camera.height * j.spp;
         }
-        const size_t cap = std::max<size_t>((size_t)ctx->tuning.queue_capacity, maxPix);
+        // slots for one wave: the capacity, or all the samples when fewer
+        const size_t cap = std::max<size_t>(std::min<uint64_t>((uint64_t)ctx->tuning.queue_capacity, samples), maxPix);
         if (cap > 0x7FFFFFFFull) throw std::runtime_error("wave capacity too large");
         ensureWaveBuffers(ctx->wb, cap);
         for (int j = 0; j < njobs; ++j)
