@@ -361,6 +361,22 @@ GDEV_INLINE void traceStream(const DScene& S, TravStack<LDSD>& st, uint32_t* cnt
         if (drained && __ballot(busy) == 0) break;
         const int pick = pickAction(act);
         if (pick == ACT_NONE || act != pick) continue;
+        // Primitive::rayIntersect returned: BVH::rayIntersect keeps the hit.
+        // Run by the BLAS step that leaves the instance with nothing left to
+        // visit, in the same iteration (no step of its own).
+        auto leaveInstance = [&]() {
+            if (instHit && lt < best) {
+                tcur = lt;
+                best = lt;
+                hit.t = lt;
+                hit.u = hu;
+                hit.v = hv;
+                hit.inst = curInst;
+                hit.prim = hp;
+                found = true;
+            }
+            level = 0;
+        };
         if (act == ACT_PACK) {
             if (COUNT) cnt[1]++;
             const uint32_t p = bq.take();
@@ -372,6 +388,7 @@ GDEV_INLINE void traceStream(const DScene& S, TravStack<LDSD>& st, uint32_t* cnt
                     instHit = true;
                 }
             }
+            if (bq.empty() && st.sp == blasBase) leaveInstance();
         } else if (act == ACT_NODE) {
             if (COUNT) {
                 cnt[0]++;
@@ -392,19 +409,9 @@ GDEV_INLINE void traceStream(const DScene& S, TravStack<LDSD>& st, uint32_t* cnt
             const uint32_t leaves = mask & leafBits(n.meta);
             if (blas) bq.set(n.child, n.meta, leaves);
             else tq.set(n.child, n.meta, leaves);
+            if (blas && leaves == 0u && st.sp == blasBase) leaveInstance();
         } else if (level == 1) {
-            // Primitive::rayIntersect returned: BVH::rayIntersect keeps the hit
-            if (instHit && lt < best) {
-                tcur = lt;
-                best = lt;
-                hit.t = lt;
-                hit.u = hu;
-                hit.v = hv;
-                hit.inst = curInst;
-                hit.prim = hp;
-                found = true;
-            }
-            level = 0;
+            leaveInstance();
         } else {
             const uint32_t slot = tq.take();
             const int pi = S.tlasPrims[slot];
