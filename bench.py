@@ -429,8 +429,7 @@ def main():
                 "max_nodes_one_ray": cst["max_ray_nodes"], "nonfinite_rays": cst["bad_rays"],
                 # lane steps / lane slots of the traversal loop (node, pack and
                 # instance steps; an instance costs a start and a finish step)
-                "simd_utilization": round((cst["node_visits"] + cst["pack_tests"] + 2 * cst["inst_tests"])
-                                          / max(1, cst["lane_iterations"]), 4),
+                "simd_utilization": round(cst["lane_steps"] / max(1, cst["lane_iterations"]), 4),
                 "rays_per_launch": trays / launches,
                 "visits_per_ray": {"nodes": cst["node_visits"] / trays,
                                    "packs": cst["pack_tests"] / trays,

@@ -131,7 +131,7 @@ class gnd_hip_stats(C.Structure):
                 ("shadow_node_visits", C.c_uint64), ("shadow_pack_tests", C.c_uint64),
                 ("shadow_inst_tests", C.c_uint64), ("max_ray_nodes", C.c_uint64), ("bad_rays", C.c_uint64),
                 ("lane_iterations", C.c_uint64), ("shadow_lane_iterations", C.c_uint64), ("ms_tail", C.c_double),
-                ("rays_tail", C.c_uint64)]
+                ("rays_tail", C.c_uint64), ("lane_steps", C.c_uint64), ("shadow_lane_steps", C.c_uint64)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -291,7 +291,7 @@ GDEV_INLINE int pickAction(int act) {
 // ray is done reports it through `finish(found, hit, nodePops)` and becomes
 // idle, so a wave does not run at the pace of its slowest ray.  With COUNT,
 // cnt[0..3] accumulate node pops, pack tests, instance tests and loop
-// iterations.
+// iterations, and cnt[4] the iterations in which the lane ran a step.
 // MOTION: the scene has moving primitives (instances take the ray's time)
 template <bool ANY, bool COUNT, int LDSD, bool MOTION = false, class Fetch, class Finish>
 GDEV_INLINE void traceStream(const DScene& S, TravStack<LDSD>& st, uint32_t* cnt, Fetch&& fetch,
@@ -361,6 +361,7 @@ GDEV_INLINE void traceStream(const DScene& S, TravStack<LDSD>& st, uint32_t* cnt
         if (drained && __ballot(busy) == 0) break;
         const int pick = pickAction(act);
         if (pick == ACT_NONE || act != pick) continue;
+        if (COUNT) cnt[4]++;
         // Primitive::rayIntersect returned: BVH::rayIntersect keeps the hit.
         // Run by the BLAS step that leaves the instance with nothing left to
         // visit, in the same iteration (no step of its own).
@@ -389,14 +390,63 @@ GDEV_INLINE void traceStream(const DScene& S, TravStack<LDSD>& st, uint32_t* cnt
                 }
             }
             if (bq.empty() && st.sp == blasBase) leaveInstance();
-        } else if (act == ACT_NODE) {
+        }
+        // the node this lane visits in this iteration: a popped one, or the
+        // BLAS root of the instance it enters (entering an instance and
+        // visiting its root are one step)
+        int visit = -1;
+        if (act == ACT_NODE) {
+            visit = st.pop();
+        } else if (act == ACT_INST) {
+            if (level == 1) {
+                leaveInstance();
+            } else {
+                const uint32_t slot = tq.take();
+                const int pi = S.tlasPrims[slot];
+                const DInst& I = S.inst[pi];
+                if (COUNT) cnt[2]++;
+                if (!MOTION || I.motion < 0) {
+                    lo = xfPoint(I.worldToObject, o);
+                    ld = xfVector(I.worldToObject, d);
+                } else {
+                    // Primitive::rayIntersect: interpolate(time).inverse() (primitive.cpp:48-50)
+                    M4 W, Wi;
+                    motionAt(S.motions[I.motion], time, W, Wi);
+                    lo = xfPoint(Wi, o);
+                    ld = xfVector(Wi, d);
+                }
+                lt = tcur;
+                curInst = pi;
+                instHit = false;
+                if (I.root >= 0) {
+                    linv = V3{1.0f / ld.x, 1.0f / ld.y, 1.0f / ld.z};
+                    level = 1;
+                    blasBase = st.sp;
+                    visit = I.root;
+                } else if (traceSphere(I.radius, lo, ld, lt)) {
+                    if (ANY) {
+                        found = true;
+                        done = true;
+                    } else if (lt < best) {
+                        tcur = lt;
+                        best = lt;
+                        hit.t = lt;
+                        hit.u = 0.0f;
+                        hit.v = 0.0f;
+                        hit.inst = pi;
+                        hit.prim = 0;
+                        found = true;
+                    }
+                }
+            }
+        }
+        if (visit >= 0) {
             if (COUNT) {
                 cnt[0]++;
                 nodes++;
             }
-            const int ni = st.pop();
             const bool blas = level == 1;
-            const DNode n = loadNode(blas ? S.blas : S.tlas, ni);
+            const DNode n = loadNode(blas ? S.blas : S.tlas, visit);
             const V3 ro = selV3(blas, lo, o);
             const V3 ri = selV3(blas, linv, winv);
             const float rmin = blas ? kEps : tmin;
@@ -410,46 +460,6 @@ GDEV_INLINE void traceStream(const DScene& S, TravStack<LDSD>& st, uint32_t* cnt
             if (blas) bq.set(n.child, n.meta, leaves);
             else tq.set(n.child, n.meta, leaves);
             if (blas && leaves == 0u && st.sp == blasBase) leaveInstance();
-        } else if (level == 1) {
-            leaveInstance();
-        } else {
-            const uint32_t slot = tq.take();
-            const int pi = S.tlasPrims[slot];
-            const DInst& I = S.inst[pi];
-            if (COUNT) cnt[2]++;
-            if (!MOTION || I.motion < 0) {
-                lo = xfPoint(I.worldToObject, o);
-                ld = xfVector(I.worldToObject, d);
-            } else {
-                // Primitive::rayIntersect: interpolate(time).inverse() (primitive.cpp:48-50)
-                M4 W, Wi;
-                motionAt(S.motions[I.motion], time, W, Wi);
-                lo = xfPoint(Wi, o);
-                ld = xfVector(Wi, d);
-            }
-            lt = tcur;
-            curInst = pi;
-            instHit = false;
-            if (I.root >= 0) {
-                linv = V3{1.0f / ld.x, 1.0f / ld.y, 1.0f / ld.z};
-                level = 1;
-                blasBase = st.sp;
-                st.push(I.root);
-            } else if (traceSphere(I.radius, lo, ld, lt)) {
-                if (ANY) {
-                    found = true;
-                    done = true;
-                } else if (lt < best) {
-                    tcur = lt;
-                    best = lt;
-                    hit.t = lt;
-                    hit.u = 0.0f;
-                    hit.v = 0.0f;
-                    hit.inst = pi;
-                    hit.prim = 0;
-                    found = true;
-                }
-            }
         }
     }
 }

@@ -131,6 +131,7 @@ struct Counters {
     unsigned long long nan, stackOverflow;
     unsigned long long maxRayNodes, badRays;     // worst closest-hit ray, rays with non-finite directions
     unsigned long long iters, siters;            // lane slots of traversal loop iterations (64 per wave step)
+    unsigned long long steps, ssteps;            // of those, the lane slots that ran a step
     unsigned long long rays, srays, launches;    // closest-hit and shadow queries, bounces with work
     unsigned long long tailRays, tailSrays;      // the same queries made by the tail kernel
 };
@@ -235,19 +236,21 @@ GDEV_INLINE void flushCounters(Counters* c, const uint32_t* cnt, bool overflow, 
                                uint32_t maxRay = 0, uint32_t bad = 0) {
     if (maxRay) atomicMax(&c->maxRayNodes, (unsigned long long)maxRay);
     if (bad) atomicAdd(&c->badRays, (unsigned long long)bad);
-    unsigned long long a = cnt[0], b = cnt[1], d = cnt[2], it = cnt[3];
+    unsigned long long a = cnt[0], b = cnt[1], d = cnt[2], it = cnt[3], w = cnt[4];
     // wave reduction then one atomic per wave
     for (int off = 32; off > 0; off >>= 1) {
         a += __shfl_down(a, off);
         b += __shfl_down(b, off);
         d += __shfl_down(d, off);
         it += __shfl_down(it, off);
+        w += __shfl_down(w, off);
     }
     if (__lane_id() == 0) {
         atomicAdd(shadow ? &c->snodes : &c->nodes, a);
         atomicAdd(shadow ? &c->spacks : &c->packs, b);
         atomicAdd(shadow ? &c->sinsts : &c->insts, d);
         atomicAdd(shadow ? &c->siters : &c->iters, it);
+        atomicAdd(shadow ? &c->ssteps : &c->steps, w);
     }
     if (overflow) atomicAdd(&c->stackOverflow, 1ull);
 }
@@ -275,7 +278,7 @@ GND_WAVE_KERNEL void kTraceBatch(DScene S, gnd_ray_soa R, int64_t n, gnd_hit_soa
                                  int32_t stackCap) {
     extern __shared__ int sStack[];
     const int64_t i = (int64_t)xcdBlock() * blockDim.x + threadIdx.x;
-    uint32_t cnt[4] = {0, 0, 0, 0};
+    uint32_t cnt[5] = {0, 0, 0, 0, 0};
     int spill[spillSize<LDSD>()];
     TravStack<LDSD> st = makeStack<LDSD>(sStack, spill, stackCap);
     if (i < n) {
@@ -473,7 +476,7 @@ GND_TRACE_KERNEL void kTracePaths(WaveArgs A) {
     extern __shared__ int sStack[];
     const uint32_t n = A.counts[0];
     const uint64_t waveMask = __ballot(1);
-    uint32_t cnt[4] = {0, 0, 0, 0};
+    uint32_t cnt[5] = {0, 0, 0, 0, 0};
     uint32_t maxRay = 0, bad = 0;
     int spill[spillSize<LDSD>()];
     TravStack<LDSD> st = makeStack<LDSD>(sStack, spill, A.stackCap);
@@ -1139,7 +1142,7 @@ template <int LDSD, bool COUNT, bool MOTION>
 __global__ __launch_bounds__(256, GND_SHADOW_WAVES) void kShadowAny(WaveArgs A) {
     extern __shared__ int sStack[];
     const uint32_t n = A.counts[2];
-    uint32_t cnt[4] = {0, 0, 0, 0};
+    uint32_t cnt[5] = {0, 0, 0, 0, 0};
     int spill[spillSize<LDSD>()];
     TravStack<LDSD> st = makeStack<LDSD>(sStack, spill, A.stackCap);
     const DScene& S = A.S;
@@ -1183,7 +1186,7 @@ template <int LDSD, bool COUNT, bool MOTION>
 __global__ __launch_bounds__(256, GND_SHADOWTR_WAVES) void kShadow(WaveArgs A) {
     extern __shared__ int sStack[];
     const uint32_t n = A.counts[2];
-    uint32_t cnt[4] = {0, 0, 0, 0};
+    uint32_t cnt[5] = {0, 0, 0, 0, 0};
     int spill[spillSize<LDSD>()];
     TravStack<LDSD> st = makeStack<LDSD>(sStack, spill, A.stackCap);
     const DScene& S = A.S;
@@ -1346,7 +1349,7 @@ __global__ __launch_bounds__(256, GND_TAIL_WAVES) void kTail(WaveArgs A) {
     extern __shared__ int sStack[];
     const uint32_t n = A.counts[0];
     const uint64_t waveMask = __ballot(1);
-    uint32_t cnt[4] = {0, 0, 0, 0};
+    uint32_t cnt[5] = {0, 0, 0, 0, 0};
     int spill[spillSize<LDSD>()];
     TravStack<LDSD> st = makeStack<LDSD>(sStack, spill, A.stackCap);
     const DScene& S = A.S;
@@ -2864,6 +2867,8 @@ static void fillStats(gnd_hip_stats* stats, const Counters& c, uint64_t samples)
     stats->bad_rays = c.badRays;
     stats->lane_iterations = c.iters;
     stats->shadow_lane_iterations = c.siters;
+    stats->lane_steps = c.steps;
+    stats->shadow_lane_steps = c.ssteps;
     stats->nan_samples = c.nan;
 }
 
@@ -4049,6 +4054,8 @@ int gnd_hip_server_stats(gnd_hip_server* s, gnd_hip_stats* stats, int32_t* resid
             sum.badRays += c.badRays;
             sum.iters += c.iters;
             sum.siters += c.siters;
+            sum.steps += c.steps;
+            sum.ssteps += c.ssteps;
             sum.rays += c.rays;
             sum.srays += c.srays;
             sum.tailRays += c.tailRays;

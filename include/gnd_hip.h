@@ -138,6 +138,10 @@ typedef struct gnd_hip_stats {
     uint64_t shadow_lane_iterations; /* the same for shadow rays (counters on)                        */
     double ms_tail;         /* time in the tail kernel (the last bounces of each wave) */
     uint64_t rays_tail;     /* closest-hit queries made by the tail kernel (in rays_closest) */
+    uint64_t lane_steps;    /* of lane_iterations, the lane slots that ran a traversal step
+                               (a node visit, a pack test, an instance entry with its root
+                               visit, or an instance exit) (counters on)              */
+    uint64_t shadow_lane_steps;  /* the same for shadow rays (counters on)             */
 } gnd_hip_stats;
 
 /* Tunables picked by gnd_hip_autotune or set by the caller. */
