@@ -154,7 +154,14 @@ GDEV_INLINE void retirePaths(uint32_t* chunkLive, bool ending, int chunk) {
 // (a counter shared by the whole launch serialises its atomics, so fewer is
 // faster).  Every thread of the block calls this; it returns this lane's
 // slots in the queues.  c1 and c2 may be null.
-constexpr int kShadeBlock = 512;
+// Threads per block of the shading kernels: at 2 waves per SIMD a CU holds
+// two 256-thread blocks, so it starts the next block while the slowest wave
+// of the other finishes (1080p shading 14.3 ms per frame with 512-thread
+// blocks, 13.8 with 256, 14.0 with 128).
+#ifndef GND_SHADE_BLOCK
+#define GND_SHADE_BLOCK 256
+#endif
+constexpr int kShadeBlock = GND_SHADE_BLOCK;
 // waves per SIMD the shading kernel and the transmittance shadow kernel are
 // compiled for (register budget; on the volumetric scene the shading kernel
 // gains nothing from 3; the persistent transmittance kernel takes 33.8 ms per
