@@ -110,14 +110,17 @@ struct alignas(16) DPack {
     uint4 prim;
 };
 
-// Traversal view of a primitive: world->object matrix + BLAS root
+// Traversal view of a primitive: world->object matrix + BLAS root.  Kept per
+// TLAS leaf slot (slot order, not primitive order), so an instance step reads
+// one 80-byte record instead of a slot -> primitive index and then the record.
 struct alignas(16) DInst {
     M4 worldToObject;
     int32_t root;   // absolute BLAS root node, -1 for an analytic sphere
     float radius;
     int32_t motion;  // index into DScene::motions, -1 when the primitive does not move
-    int32_t pad1;
+    int32_t prim;    // primitive index (BVH::rayIntersect's primitive)
 };
+static_assert(sizeof(DInst) == 80, "instance record is 5 x 16 bytes");
 
 // Shading view of a primitive
 struct alignas(16) DPrim {
@@ -180,8 +183,7 @@ struct DLight {
 
 struct DScene {
     const DNode* tlas;
-    const int32_t* tlasPrims;
-    const DInst* inst;
+    const DInst* tlasInst;    // per TLAS leaf slot: the instance record of the slot's primitive
     const DNode* blas;
     const DPack* packs;
     const DPrim* prims;

@@ -109,6 +109,25 @@ GDEV_INLINE DNode loadNode(const DNode* nodes, int idx) {
     return n;
 }
 
+// The whole 80-byte record in five 16-byte loads issued together (the BLAS
+// root and the motion index arrive with the matrix).
+GDEV_INLINE DInst loadInst(const DInst* inst, uint32_t slot) {
+    const float4* p = reinterpret_cast<const float4*>(inst + slot);
+    const float4 r0 = p[0], r1 = p[1], r2 = p[2], r3 = p[3];
+    const int4 r4 = reinterpret_cast<const int4*>(p)[4];
+    DInst I;
+    I.worldToObject.m[0] = r0.x; I.worldToObject.m[1] = r0.y; I.worldToObject.m[2] = r0.z; I.worldToObject.m[3] = r0.w;
+    I.worldToObject.m[4] = r1.x; I.worldToObject.m[5] = r1.y; I.worldToObject.m[6] = r1.z; I.worldToObject.m[7] = r1.w;
+    I.worldToObject.m[8] = r2.x; I.worldToObject.m[9] = r2.y; I.worldToObject.m[10] = r2.z; I.worldToObject.m[11] = r2.w;
+    I.worldToObject.m[12] = r3.x; I.worldToObject.m[13] = r3.y; I.worldToObject.m[14] = r3.z;
+    I.worldToObject.m[15] = r3.w;
+    I.root = r4.x;
+    I.radius = __int_as_float(r4.y);
+    I.motion = r4.z;
+    I.prim = r4.w;
+    return I;
+}
+
 // Mesh::rayIntersect's 4-lane Moller-Trumbore (mesh.cpp:262-319) on one pack:
 // lanes evaluated in order, committed with a strict t < lt.  Returns true on a
 // committed hit (for ANY: on the first valid lane).
@@ -401,9 +420,8 @@ GDEV_INLINE void traceStream(const DScene& S, TravStack<LDSD>& st, uint32_t* cnt
             if (level == 1) {
                 leaveInstance();
             } else {
-                const uint32_t slot = tq.take();
-                const int pi = S.tlasPrims[slot];
-                const DInst& I = S.inst[pi];
+                const DInst I = loadInst(S.tlasInst, tq.take());
+                const int pi = I.prim;
                 if (COUNT) cnt[2]++;
                 if (!MOTION || I.motion < 0) {
                     lo = xfPoint(I.worldToObject, o);

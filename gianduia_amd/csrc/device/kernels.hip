@@ -2268,7 +2268,7 @@ static int sceneUpload(gnd_hip_ctx* ctx, const gnd_hip_scene_arrays* a, gnd_hip_
             std::memcpy(I.worldToObject.m, p.to_world_inv, sizeof(I.worldToObject.m));
             I.radius = p.radius;
             I.motion = -1;
-            I.pad1 = 0;
+            I.prim = (int32_t)i;
             // a moving primitive (AnimatedTransform: animated when the matrices differ)
             int motion = -1;
             if (p.has_motion && std::memcmp(p.to_world, p.to_world_end, sizeof(p.to_world)) != 0) {
@@ -2425,12 +2425,17 @@ static int sceneUpload(gnd_hip_ctx* ctx, const gnd_hip_scene_arrays* a, gnd_hip_
             if (d.materials[i].type == GND_MAT_INDEX) s->hasIndex = 1;
         DScene& v = s->view;
         v.tlas = devAlloc<DNode>(s, tl.size(), tl.data());
-        v.tlasPrims = devAlloc<int32_t>(s, a->n_tlas_prims, a->tlas_prims);
-        v.inst = devAlloc<DInst>(s, inst.size(), inst.data());
+        // instance records in TLAS slot order
+        std::vector<DInst> slotInst(a->n_tlas_prims);
+        for (uint32_t k = 0; k < a->n_tlas_prims; ++k) {
+            const int32_t pi = a->tlas_prims[k];
+            if (pi < 0 || (uint32_t)pi >= d.nprims) throw std::runtime_error("TLAS primitive index out of range");
+            slotInst[k] = inst[pi];
+        }
+        v.tlasInst = devAlloc<DInst>(s, slotInst.size(), slotInst.data());
         v.blas = devAlloc<DNode>(s, bl.size(), bl.data());
         v.packs = reinterpret_cast<const DPack*>(devAlloc<gnd_tri_pack4>(s, a->n_packs, a->packs));
-        s->accelBytes = tl.size() * sizeof(DNode) + (size_t)a->n_tlas_prims * sizeof(int32_t) +
-                        inst.size() * sizeof(DInst) + bl.size() * sizeof(DNode) +
+        s->accelBytes = tl.size() * sizeof(DNode) + slotInst.size() * sizeof(DInst) + bl.size() * sizeof(DNode) +
                         (size_t)a->n_packs * sizeof(gnd_tri_pack4);
         v.prims = devAlloc<DPrim>(s, prims.size(), prims.data());
         v.shading = devAlloc<float>(s, (size_t)a->n_tris * 24, a->shading);
@@ -3343,9 +3348,9 @@ void giveSlots(gnd_hip_server* s, uint32_t b, uint32_t n) {
 size_t sceneEstimate(const gnd_hip_scene_arrays* a) {
     size_t b = (size_t)a->n_tlas_nodes * sizeof(DNode) + (size_t)a->n_blas_nodes * sizeof(DNode) +
                (size_t)a->n_packs * sizeof(DPack) + (size_t)a->n_tris * 24 * sizeof(float) +
-               (size_t)a->n_cdf * sizeof(float) + (size_t)a->n_tlas_prims * 4;
+               (size_t)a->n_cdf * sizeof(float) + (size_t)a->n_tlas_prims * sizeof(DInst);
     const gnd_scene_desc& d = *a->desc;
-    b += (size_t)d.nprims * (sizeof(DInst) + sizeof(DPrim)) + (size_t)d.nmaterials * sizeof(DMaterial);
+    b += (size_t)d.nprims * sizeof(DPrim) + (size_t)d.nmaterials * sizeof(DMaterial);
     for (uint32_t m = 0; m < d.nmedia; ++m)
         if (d.media[m].type == GND_MEDIUM_GRID) {
             // the dense grid with its majorants, and the corner records
