@@ -799,10 +799,12 @@ GDEV_INLINE void lightSample(const DScene& S, int k, V3 refP, float ux, float uy
     ls.Li = cdiv(envEval(S, wi), ls.pdf);
 }
 // the NEE sample of emitter-list entry e (area emitter primitive, or
-// -(light + 1)); delta = Emitter::isDelta
+// -(light + 1)); delta = Emitter::isDelta.  AREA_ONLY: the scene's emitter
+// list holds area emitters only (no light entries to branch to).
+template <bool AREA_ONLY = false>
 GDEV_INLINE void emitterSample(const DScene& S, int e, V3 refP, float ux, float uy, LightSample& ls, bool& delta,
                                float time) {
-    if (e >= 0) {
+    if (AREA_ONLY || e >= 0) {
         delta = false;
         areaSample(S, e, refP, ux, uy, ls, time);
         return;

@@ -566,11 +566,13 @@ GDEV_INLINE int nextMedium(V3 wo, V3 wi, int cur, V3 n, int inside, int outside)
 
 // Scatter half of a bounce: phase or BSDF sampling, Russian roulette, next ray
 // (path.cpp:159-177, volPath.cpp:88-111 and 222-248).  Returns true if alive.
+// LEAN: every job of the launch is a path-integrator job (no medium events).
+template <bool LEAN = false>
 GDEV_INLINE bool scatterStep(const DScene& S, const DJob& J, PathLocal& s, const Event& ev) {
-    const bool vol = J.integrator == GND_INTEGRATOR_VOLPATH;
+    const bool vol = !LEAN && J.integrator == GND_INTEGRATOR_VOLPATH;
     uint32_t bounces = s.c0 & 0xFFFFu, nullB = s.c0 >> 16;
     uint32_t surfB = s.c1 & 0xFFFFu, volB = s.c1 >> 16;
-    if (ev.kind == EV_MEDIUM) {
+    if (!LEAN && ev.kind == EV_MEDIUM) {
         const DMedium& m = S.media[ev.curMedium];
         V3 wo = -s.d, wi;
         float ux = s.rng.next1D(), uy = s.rng.next1D();
@@ -701,11 +703,16 @@ GDEV_INLINE float shadowTime(const DScene& S, int e, float t) {
 // Returns: 0 = dead, 1 = continues straight (already a new ray), 2 = needs scatter.
 // pre: the medium was a density grid that kMedium already walked (throughput
 // and sampler state updated); preHit / preP = its collision, if any.
+// LEAN: the launch holds path-integrator jobs without AOVs on a scene whose
+// emitters are all area emitters (no environment, no other lights), so the
+// medium, AOV, environment and light branches are left out of the code and
+// of its register budget.
+template <bool LEAN = false>
 GDEV_INLINE int neeStep(const WaveArgs& A, const DJob& J, int p, PathLocal& s, const HitRec& h, Event& ev,
                         bool& shadow, V3& so, V3& sd, float& stmax, int& smedium, V3& pending, V3& lightP,
                         float& stime, bool pre, bool preHit, V3 preP) {
     const DScene& S = A.S;
-    const bool vol = J.integrator == GND_INTEGRATOR_VOLPATH;
+    const bool vol = !LEAN && J.integrator == GND_INTEGRATOR_VOLPATH;
     const bool hit = h.inst >= 0;
     const float invNE = 1.0f / (float)(uint32_t)S.nEmitters;
     shadow = false;
@@ -746,14 +753,14 @@ GDEV_INLINE int neeStep(const WaveArgs& A, const DJob& J, int p, PathLocal& s, c
             return 2;
         }
     }
-    const bool aovOn = J.filmAov && A.P.aov;
+    const bool aovOn = !LEAN && J.filmAov && A.P.aov;
     if (!hit) {
         // the environment along the escaping ray, MIS-weighted against NEE
         // unless the last bounce was specular (path.cpp:36-55, volPath.cpp:
         // 117-139); its pdf is taken from the origin, the reference point of
         // the default SurfaceInteraction there
         V3 env = sameV(0.0f);
-        if (S.envLight >= 0) {
+        if (!LEAN && S.envLight >= 0) {
             env = envEval(S, s.d);
             if (!isBlack(env)) {
                 float w = 1.0f;
@@ -802,7 +809,7 @@ GDEV_INLINE int neeStep(const WaveArgs& A, const DJob& J, int p, PathLocal& s, c
     const float ux = s.rng.next1D(), uy = s.rng.next1D();
     LightSample ls;
     bool delta;
-    emitterSample(S, ep, it.p, ux, uy, ls, delta, rayTime(s));
+    emitterSample<LEAN>(S, ep, it.p, ux, uy, ls, delta, rayTime(s));
     const V3 Li = cdiv(ls.Li, lsp);
     if (ls.ok && ls.pdf > 1e-6f && !isBlack(Li)) {
         const V3 wi = normalize(ls.p - it.p);
@@ -1011,8 +1018,17 @@ __global__ __launch_bounds__(256) void kMedium(WaveArgs A) {
 // SPLIT = false: full bounce (NEE + scatter) in one kernel; shadow rays resolve
 // afterwards (valid when shadow rays consume no random numbers).
 // SPLIT = true: NEE only; the event is stored for kScatter.
-template <bool SPLIT>
-__global__ __launch_bounds__(kShadeBlock, GND_SHADE_WAVES) void kShade(WaveArgs A) {
+// LEAN (not with SPLIT): the neeStep<true> / scatterStep<true> form for
+// launches of path-integrator jobs without AOVs on scenes lit by area
+// emitters only.  Its smaller register budget fits GND_SHADE_LEAN_WAVES waves
+// per SIMD (1080p scene: 13.7 -> 12.7 ms of shading per frame at 3 waves,
+// 13.2 at 2).
+#ifndef GND_SHADE_LEAN_WAVES
+#define GND_SHADE_LEAN_WAVES 3
+#endif
+template <bool SPLIT, bool LEAN = false>
+__global__ __launch_bounds__(kShadeBlock, LEAN ? GND_SHADE_LEAN_WAVES : GND_SHADE_WAVES) void kShade(WaveArgs A) {
+    static_assert(!(SPLIT && LEAN), "the split kernels serve volumetric jobs");
     const uint32_t n = A.counts[0];
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t start = blockIdx.x * blockDim.x;
@@ -1041,14 +1057,14 @@ __global__ __launch_bounds__(kShadeBlock, GND_SHADE_WAVES) void kShade(WaveArgs 
                 if (preHit) preP = V3{A.P.evpx[p], A.P.evpy[p], A.P.evpz[p]};
             }
             int r;
-            if (J.integrator == GND_INTEGRATOR_DIRECT) {
+            if (!LEAN && J.integrator == GND_INTEGRATOR_DIRECT) {
                 r = directStep(A, s, h, shadow, so, sd, stmax, pending, lightP, stime);
-            } else if (J.integrator == GND_INTEGRATOR_AO) {
+            } else if (!LEAN && J.integrator == GND_INTEGRATOR_AO) {
                 aoStep(A, J, s, h, shadow, so, sd, stmax, pending, lightP, stime);
                 r = 0;
             } else {
-                r = neeStep(A, J, p, s, h, ev, shadow, so, sd, stmax, smedium, pending, lightP, stime, pre, preHit,
-                            preP);
+                r = neeStep<LEAN>(A, J, p, s, h, ev, shadow, so, sd, stmax, smedium, pending, lightP, stime, pre,
+                                  preHit, preP);
             }
             if (!A.pathOnlyFast && shadow) {
                 // the general shadow kernel always needs the ray; the any-hit
@@ -1068,7 +1084,7 @@ __global__ __launch_bounds__(kShadeBlock, GND_SHADE_WAVES) void kShade(WaveArgs 
                     A.P.evtx[p] = ev.dpdu.x; A.P.evty[p] = ev.dpdu.y; A.P.evtz[p] = ev.dpdu.z;
                     A.P.evu[p] = ev.u; A.P.evv[p] = ev.v;
                 } else {
-                    alive = scatterStep(A.S, J, s, ev);
+                    alive = scatterStep<LEAN>(A.S, J, s, ev);
                 }
             } else if (SPLIT) {
                 A.P.evKind[p] = EV_NONE;
@@ -1351,7 +1367,9 @@ __global__ __launch_bounds__(256, GND_SHADOWTR_WAVES) void kShadow(WaveArgs A) {
 constexpr int kTailRefill = 8;
 // traversal stack entries in LDS per lane of the tail kernel
 constexpr int kTailLds = 16;
-template <int LDSD, bool MOTION, bool FAST>
+// LEAN: the launch's jobs are path-integrator jobs without AOVs on a scene
+// lit by area emitters only (as for kShade<false, true>).
+template <int LDSD, bool MOTION, bool FAST, bool LEAN = false>
 __global__ __launch_bounds__(256, GND_TAIL_WAVES) void kTail(WaveArgs A) {
     extern __shared__ int sStack[];
     const uint32_t n = A.counts[0];
@@ -1400,14 +1418,14 @@ __global__ __launch_bounds__(256, GND_TAIL_WAVES) void kTail(WaveArgs A) {
         ev.kind = EV_NONE;
         const DJob* J = busy ? &A.jobs[s.job] : nullptr;
         if (busy) {
-            if (J->integrator == GND_INTEGRATOR_DIRECT) {
+            if (!LEAN && J->integrator == GND_INTEGRATOR_DIRECT) {
                 r = directStep(A, s, h, shadow, so, sd, stmax, pending, lightP, stime);
-            } else if (J->integrator == GND_INTEGRATOR_AO) {
+            } else if (!LEAN && J->integrator == GND_INTEGRATOR_AO) {
                 aoStep(A, *J, s, h, shadow, so, sd, stmax, pending, lightP, stime);
                 r = 0;
             } else {
-                r = neeStep(A, *J, p, s, h, ev, shadow, so, sd, stmax, smedium, pending, lightP, stime, false, false,
-                            sameV(0.0f));
+                r = neeStep<LEAN>(A, *J, p, s, h, ev, shadow, so, sd, stmax, smedium, pending, lightP, stime, false,
+                                  false, sameV(0.0f));
             }
             if (FAST && shadow && isBlack(pending)) shadow = false;
             if (shadow) nShadow++;
@@ -1415,8 +1433,9 @@ __global__ __launch_bounds__(256, GND_TAIL_WAVES) void kTail(WaveArgs A) {
         // the shadow ray, segment by segment through null-BSDF surfaces
         // (path.cpp:122-140, volPath.cpp:261-291)
         {
-            const bool vol = busy && J->integrator == GND_INTEGRATOR_VOLPATH;
-            const bool opaque = busy && (J->integrator == GND_INTEGRATOR_DIRECT || J->integrator == GND_INTEGRATOR_AO);
+            const bool vol = !LEAN && busy && J->integrator == GND_INTEGRATOR_VOLPATH;
+            const bool opaque =
+                !LEAN && busy && (J->integrator == GND_INTEGRATOR_DIRECT || J->integrator == GND_INTEGRATOR_AO);
             bool open = shadow, visible = false;
             V3 Tr = sameV(1.0f), o = so;
             float tmax = stmax;
@@ -1473,7 +1492,7 @@ __global__ __launch_bounds__(256, GND_TAIL_WAVES) void kTail(WaveArgs A) {
         bool alive = false;
         if (busy) {
             if (r == 1) alive = true;
-            else if (r == 2) alive = scatterStep(S, *J, s, ev);
+            else if (r == 2) alive = scatterStep<LEAN>(S, *J, s, ev);
         }
         const bool ending = busy && !alive;
         if (ending) {
@@ -2608,7 +2627,7 @@ static std::vector<WavePlan> planWaves(const std::vector<gnd_hip_job>& jobs, siz
 
 // Kernels and launch shape of a bounce for one scene and job mix.
 struct BouncePlan {
-    WaveKernel trace = nullptr, shadow = nullptr;
+    WaveKernel trace = nullptr, shadow = nullptr, shade = nullptr;
     bool split = false, fastShadow = false;
     int B = 128;
     int refill = 32, srefill = 40;
@@ -2635,11 +2654,28 @@ struct BouncePlan {
     unsigned shadeGrid(uint32_t n) const { return gridFor(n, (unsigned)kShadeBlock); }
 };
 
-static BouncePlan makeBouncePlan(const gnd_hip_ctx* ctx, const gnd_hip_scene* scene, bool anyVol) {
+// What the jobs of a launch ask of the shading kernels.
+struct JobMix {
+    bool vol = false;     // a volpath job
+    bool other = false;   // a direct or AO job
+    bool aov = false;     // a job with film AOVs
+    void add(const gnd_hip_job& j) {
+        vol = vol || j.integrator == GND_INTEGRATOR_VOLPATH;
+        other = other || j.integrator == GND_INTEGRATOR_DIRECT || j.integrator == GND_INTEGRATOR_AO;
+        aov = aov || j.film_aov != nullptr;
+    }
+};
+
+static BouncePlan makeBouncePlan(const gnd_hip_ctx* ctx, const gnd_hip_scene* scene, const JobMix& mix) {
     BouncePlan K;
+    const bool anyVol = mix.vol;
     // the split NEE / scatter kernels keep transmittance sampling in the
     // reference order when shadow rays cross a density grid
     K.split = anyVol && scene->hasGrid;
+    // path jobs without AOVs on a scene lit by area emitters only take the
+    // lean shading kernel
+    const bool lean = !mix.vol && !mix.other && !mix.aov && scene->view.nLights == 0 && scene->view.envLight < 0;
+    K.shade = K.split ? kShade<true> : (lean ? kShade<false, true> : kShade<false, false>);
     // any-hit shadows are exact when no surface is a null BSDF and no
     // shadow segment can cross a medium (the path integrator ignores media)
     K.fastShadow = !scene->hasIndex && (!anyVol || scene->nMedia == 0);
@@ -2663,6 +2699,7 @@ static BouncePlan makeBouncePlan(const gnd_hip_ctx* ctx, const gnd_hip_scene* sc
     K.srefill = ctx->tuning.shadow_refill_lanes > 0 ? ctx->tuning.shadow_refill_lanes : (small ? 64 : 40);
     K.stackCap = stackCapOf(ctx->tuning);
     if (scene->hasMotion) K.tail = K.fastShadow ? kTail<kTailLds, true, true> : kTail<kTailLds, true, false>;
+    else if (lean) K.tail = K.fastShadow ? kTail<kTailLds, false, true, true> : kTail<kTailLds, false, false, true>;
     else K.tail = K.fastShadow ? kTail<kTailLds, false, true> : kTail<kTailLds, false, false>;
     // resident lanes of the tail kernel: GND_TAIL_WAVES waves on each of 4 SIMDs per CU
     const uint32_t tailLanes = (uint32_t)ctx->numCUs * 4u * GND_TAIL_WAVES * 64u;
@@ -2730,8 +2767,7 @@ static void launchBounce(hipStream_t stream, const WaveArgs& A, const BouncePlan
         hipLaunchKernelGGL(kMedium, dim3(K.gridFor(live, 256)), dim3(256), 0, stream, A);
         GND_CHECK(hipGetLastError());
     }
-    if (K.split) hipLaunchKernelGGL(kShade<true>, dim3(gs), dim3(kShadeBlock), 0, stream, A);
-    else hipLaunchKernelGGL(kShade<false>, dim3(gs), dim3(kShadeBlock), 0, stream, A);
+    hipLaunchKernelGGL(K.shade, dim3(gs), dim3(kShadeBlock), 0, stream, A);
     GND_CHECK(hipGetLastError());
     if (tl) tl->lap(t, 2);
     // NEE emits at most one shadow ray per path
@@ -2776,7 +2812,7 @@ static std::string chainKey(const WaveArgs& A0, const BouncePlan& K, uint32_t li
     auto put = [&](const void* p, size_t n) { key.append(static_cast<const char*>(p), n); };
     const unsigned shape[6] = {K.grid(live), K.shadeGrid(live), K.gridFor(live, 256), (unsigned)K.B,
                                K.tailPaths, K.tailGridCap};
-    const void* fns[3] = {(const void*)K.trace, (const void*)K.shadow, (const void*)K.tail};
+    const void* fns[4] = {(const void*)K.trace, (const void*)K.shadow, (const void*)K.tail, (const void*)K.shade};
     const int flags[2] = {K.split ? 1 : 0, (int)K.shm};
     put(&A0, sizeof(A0));
     put(shape, sizeof(shape));
@@ -2893,7 +2929,7 @@ int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jo
         GND_CHECK(hipSetDevice(ctx->device));
         std::vector<gnd_hip_job> jobs(jobsIn, jobsIn + njobs);
         size_t maxPix = 0;
-        bool anyVol = false;
+        JobMix mix;
         uint64_t samples = 0;
         for (auto& j : jobs) {
             if (j.camera.width <= 0 || j.camera.height <= 0 || !j.film)
@@ -2901,7 +2937,7 @@ int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jo
             if (j.integrator < GND_INTEGRATOR_PATH || j.integrator > GND_INTEGRATOR_AO)
                 throw std::runtime_error("unknown integrator");
             maxPix = std::max(maxPix, (size_t)j.camera.width * j.camera.height);
-            anyVol = anyVol || j.integrator == GND_INTEGRATOR_VOLPATH;
+            mix.add(j);
             samples += (uint64_t)j.camera.width * j.camera.height * j.spp;
         }
         // slots for one wave: the capacity, or all the samples when fewer
@@ -2920,7 +2956,7 @@ int gnd_hip_render(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* jo
                                  ctx->stream));
         GND_CHECK(hipMemsetAsync(ctx->dCounters, 0, sizeof(Counters), ctx->stream));
         const std::vector<WavePlan> waves = planWaves(jobs, cap);
-        const BouncePlan K = makeBouncePlan(ctx, scene, anyVol);
+        const BouncePlan K = makeBouncePlan(ctx, scene, mix);
         const bool graph = ctx->tuning.use_graph != 0;
         Timeline tl(ctx->stream);
         Timeline* tlp = graph ? nullptr : &tl;
@@ -3588,10 +3624,10 @@ void ensureStage(SrvLane& L, size_t bytes) {
 // every live path, and the read-back of the counts; no host wait.
 void launchLane(gnd_hip_server* s, SrvLane& L) {
     gnd_hip_ctx* ctx = s->ctx;
-    bool anyVol = false;
-    for (SrvJob* j : L.jobs) anyVol = anyVol || j->job.integrator == GND_INTEGRATOR_VOLPATH;
+    JobMix mix;
+    for (SrvJob* j : L.jobs) mix.add(j->job);
     const gnd_hip_scene* scene = s->scenes[L.scene].dev;
-    L.K = makeBouncePlan(ctx, scene, anyVol);
+    L.K = makeBouncePlan(ctx, scene, mix);
     WaveArgs A = laneArgs(s, L);
     if (!L.fresh.empty()) {
         const size_t nj = L.freshJobs.size(), nc = L.fresh.size();
