@@ -566,13 +566,21 @@ GDEV_INLINE int nextMedium(V3 wo, V3 wi, int cur, V3 n, int inside, int outside)
 
 // Scatter half of a bounce: phase or BSDF sampling, Russian roulette, next ray
 // (path.cpp:159-177, volPath.cpp:88-111 and 222-248).  Returns true if alive.
-// LEAN: every job of the launch is a path-integrator job (no medium events).
-template <bool LEAN = false>
+// What a shading-kernel variant handles (its template argument FEAT):
+//  kFeatAll  every integrator, AOVs, every emitter kind;
+//  kFeatPath path-integrator jobs without AOVs on a scene whose emitters are
+//            all area emitters (no environment, no other lights);
+//  kFeatVol  the same with volpath jobs too (media).
+// The leaner forms leave the other branches out of the code and of its
+// register budget.
+constexpr int kFeatAll = 0, kFeatPath = 1, kFeatVol = 2;
+
+template <int FEAT = kFeatAll>
 GDEV_INLINE bool scatterStep(const DScene& S, const DJob& J, PathLocal& s, const Event& ev) {
-    const bool vol = !LEAN && J.integrator == GND_INTEGRATOR_VOLPATH;
+    const bool vol = FEAT != kFeatPath && J.integrator == GND_INTEGRATOR_VOLPATH;
     uint32_t bounces = s.c0 & 0xFFFFu, nullB = s.c0 >> 16;
     uint32_t surfB = s.c1 & 0xFFFFu, volB = s.c1 >> 16;
-    if (!LEAN && ev.kind == EV_MEDIUM) {
+    if (FEAT != kFeatPath && ev.kind == EV_MEDIUM) {
         const DMedium& m = S.media[ev.curMedium];
         V3 wo = -s.d, wi;
         float ux = s.rng.next1D(), uy = s.rng.next1D();
@@ -703,16 +711,12 @@ GDEV_INLINE float shadowTime(const DScene& S, int e, float t) {
 // Returns: 0 = dead, 1 = continues straight (already a new ray), 2 = needs scatter.
 // pre: the medium was a density grid that kMedium already walked (throughput
 // and sampler state updated); preHit / preP = its collision, if any.
-// LEAN: the launch holds path-integrator jobs without AOVs on a scene whose
-// emitters are all area emitters (no environment, no other lights), so the
-// medium, AOV, environment and light branches are left out of the code and
-// of its register budget.
-template <bool LEAN = false>
+template <int FEAT = kFeatAll>
 GDEV_INLINE int neeStep(const WaveArgs& A, const DJob& J, int p, PathLocal& s, const HitRec& h, Event& ev,
                         bool& shadow, V3& so, V3& sd, float& stmax, int& smedium, V3& pending, V3& lightP,
                         float& stime, bool pre, bool preHit, V3 preP) {
     const DScene& S = A.S;
-    const bool vol = !LEAN && J.integrator == GND_INTEGRATOR_VOLPATH;
+    const bool vol = FEAT != kFeatPath && J.integrator == GND_INTEGRATOR_VOLPATH;
     const bool hit = h.inst >= 0;
     const float invNE = 1.0f / (float)(uint32_t)S.nEmitters;
     shadow = false;
@@ -737,7 +741,7 @@ GDEV_INLINE int neeStep(const WaveArgs& A, const DJob& J, int p, PathLocal& s, c
             bool delta;
             // the light is sampled from a fresh SurfaceInteraction holding
             // only the point (volPath.cpp:60-61): time 0
-            emitterSample(S, ep, me.p, ux, uy, ls, delta, 0.0f);
+            emitterSample<FEAT != kFeatAll>(S, ep, me.p, ux, uy, ls, delta, 0.0f);
             const V3 Li = cdiv(ls.Li, lsp);
             if (ls.ok && ls.pdf > 1e-6f && !isBlack(Li)) {
                 const V3 wi = normalize(ls.p - me.p);
@@ -753,14 +757,14 @@ GDEV_INLINE int neeStep(const WaveArgs& A, const DJob& J, int p, PathLocal& s, c
             return 2;
         }
     }
-    const bool aovOn = !LEAN && J.filmAov && A.P.aov;
+    const bool aovOn = FEAT == kFeatAll && J.filmAov && A.P.aov;
     if (!hit) {
         // the environment along the escaping ray, MIS-weighted against NEE
         // unless the last bounce was specular (path.cpp:36-55, volPath.cpp:
         // 117-139); its pdf is taken from the origin, the reference point of
         // the default SurfaceInteraction there
         V3 env = sameV(0.0f);
-        if (!LEAN && S.envLight >= 0) {
+        if (FEAT == kFeatAll && S.envLight >= 0) {
             env = envEval(S, s.d);
             if (!isBlack(env)) {
                 float w = 1.0f;
@@ -809,7 +813,7 @@ GDEV_INLINE int neeStep(const WaveArgs& A, const DJob& J, int p, PathLocal& s, c
     const float ux = s.rng.next1D(), uy = s.rng.next1D();
     LightSample ls;
     bool delta;
-    emitterSample<LEAN>(S, ep, it.p, ux, uy, ls, delta, rayTime(s));
+    emitterSample<FEAT != kFeatAll>(S, ep, it.p, ux, uy, ls, delta, rayTime(s));
     const V3 Li = cdiv(ls.Li, lsp);
     if (ls.ok && ls.pdf > 1e-6f && !isBlack(Li)) {
         const V3 wi = normalize(ls.p - it.p);
@@ -1018,17 +1022,20 @@ __global__ __launch_bounds__(256) void kMedium(WaveArgs A) {
 // SPLIT = false: full bounce (NEE + scatter) in one kernel; shadow rays resolve
 // afterwards (valid when shadow rays consume no random numbers).
 // SPLIT = true: NEE only; the event is stored for kScatter.
-// LEAN (not with SPLIT): the neeStep<true> / scatterStep<true> form for
-// launches of path-integrator jobs without AOVs on scenes lit by area
-// emitters only.  Its smaller register budget fits GND_SHADE_LEAN_WAVES waves
-// per SIMD (1080p scene: 13.7 -> 12.7 ms of shading per frame at 3 waves,
-// 13.2 at 2).
+// FEAT (see kFeatAll): the kFeatPath form's smaller register budget fits
+// GND_SHADE_LEAN_WAVES waves per SIMD (1080p scene: 13.7 -> 12.7 ms of
+// shading per frame at 3 waves, 13.2 at 2).
 #ifndef GND_SHADE_LEAN_WAVES
 #define GND_SHADE_LEAN_WAVES 3
 #endif
-template <bool SPLIT, bool LEAN = false>
-__global__ __launch_bounds__(kShadeBlock, LEAN ? GND_SHADE_LEAN_WAVES : GND_SHADE_WAVES) void kShade(WaveArgs A) {
-    static_assert(!(SPLIT && LEAN), "the split kernels serve volumetric jobs");
+#ifndef GND_SHADE_VOL_WAVES
+#define GND_SHADE_VOL_WAVES 2
+#endif
+template <bool SPLIT, int FEAT = kFeatAll>
+__global__ __launch_bounds__(kShadeBlock, FEAT == kFeatPath  ? GND_SHADE_LEAN_WAVES
+                                          : FEAT == kFeatVol ? GND_SHADE_VOL_WAVES
+                                                             : GND_SHADE_WAVES) void kShade(WaveArgs A) {
+    static_assert(!(SPLIT && FEAT == kFeatPath), "the split kernels serve volumetric jobs");
     const uint32_t n = A.counts[0];
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t start = blockIdx.x * blockDim.x;
@@ -1057,13 +1064,13 @@ __global__ __launch_bounds__(kShadeBlock, LEAN ? GND_SHADE_LEAN_WAVES : GND_SHAD
                 if (preHit) preP = V3{A.P.evpx[p], A.P.evpy[p], A.P.evpz[p]};
             }
             int r;
-            if (!LEAN && J.integrator == GND_INTEGRATOR_DIRECT) {
+            if (FEAT == kFeatAll && J.integrator == GND_INTEGRATOR_DIRECT) {
                 r = directStep(A, s, h, shadow, so, sd, stmax, pending, lightP, stime);
-            } else if (!LEAN && J.integrator == GND_INTEGRATOR_AO) {
+            } else if (FEAT == kFeatAll && J.integrator == GND_INTEGRATOR_AO) {
                 aoStep(A, J, s, h, shadow, so, sd, stmax, pending, lightP, stime);
                 r = 0;
             } else {
-                r = neeStep<LEAN>(A, J, p, s, h, ev, shadow, so, sd, stmax, smedium, pending, lightP, stime, pre,
+                r = neeStep<FEAT>(A, J, p, s, h, ev, shadow, so, sd, stmax, smedium, pending, lightP, stime, pre,
                                   preHit, preP);
             }
             if (!A.pathOnlyFast && shadow) {
@@ -1084,7 +1091,7 @@ __global__ __launch_bounds__(kShadeBlock, LEAN ? GND_SHADE_LEAN_WAVES : GND_SHAD
                     A.P.evtx[p] = ev.dpdu.x; A.P.evty[p] = ev.dpdu.y; A.P.evtz[p] = ev.dpdu.z;
                     A.P.evu[p] = ev.u; A.P.evv[p] = ev.v;
                 } else {
-                    alive = scatterStep<LEAN>(A.S, J, s, ev);
+                    alive = scatterStep<FEAT>(A.S, J, s, ev);
                 }
             } else if (SPLIT) {
                 A.P.evKind[p] = EV_NONE;
@@ -1118,6 +1125,7 @@ __global__ __launch_bounds__(kShadeBlock, LEAN ? GND_SHADE_LEAN_WAVES : GND_SHAD
 
 // Split mode second half: the paths that recorded an event scatter now (after
 // their shadow rays consumed transmittance samples).
+template <int FEAT = kFeatAll>
 __global__ __launch_bounds__(kShadeBlock) void kScatter(WaveArgs A) {
     const uint32_t nEv = A.counts[0];
     const int32_t* qEv = A.qIn;
@@ -1142,7 +1150,7 @@ __global__ __launch_bounds__(kShadeBlock) void kScatter(WaveArgs A) {
                 ev.dpdu = V3{A.P.evtx[p], A.P.evty[p], A.P.evtz[p]};
                 ev.u = A.P.evu[p];
                 ev.v = A.P.evv[p];
-                alive = scatterStep(A.S, A.jobs[s.job], s, ev);
+                alive = scatterStep<FEAT>(A.S, A.jobs[s.job], s, ev);
                 toGrid = alive && gridWalked(A, A.jobs[s.job], s.flags);
                 storePath(A.P, p, s);
                 ending = !alive;
@@ -1367,9 +1375,8 @@ __global__ __launch_bounds__(256, GND_SHADOWTR_WAVES) void kShadow(WaveArgs A) {
 constexpr int kTailRefill = 8;
 // traversal stack entries in LDS per lane of the tail kernel
 constexpr int kTailLds = 16;
-// LEAN: the launch's jobs are path-integrator jobs without AOVs on a scene
-// lit by area emitters only (as for kShade<false, true>).
-template <int LDSD, bool MOTION, bool FAST, bool LEAN = false>
+// FEAT: the jobs and scene the launch serves (see kFeatAll).
+template <int LDSD, bool MOTION, bool FAST, int FEAT = kFeatAll>
 __global__ __launch_bounds__(256, GND_TAIL_WAVES) void kTail(WaveArgs A) {
     extern __shared__ int sStack[];
     const uint32_t n = A.counts[0];
@@ -1418,13 +1425,13 @@ __global__ __launch_bounds__(256, GND_TAIL_WAVES) void kTail(WaveArgs A) {
         ev.kind = EV_NONE;
         const DJob* J = busy ? &A.jobs[s.job] : nullptr;
         if (busy) {
-            if (!LEAN && J->integrator == GND_INTEGRATOR_DIRECT) {
+            if (FEAT == kFeatAll && J->integrator == GND_INTEGRATOR_DIRECT) {
                 r = directStep(A, s, h, shadow, so, sd, stmax, pending, lightP, stime);
-            } else if (!LEAN && J->integrator == GND_INTEGRATOR_AO) {
+            } else if (FEAT == kFeatAll && J->integrator == GND_INTEGRATOR_AO) {
                 aoStep(A, *J, s, h, shadow, so, sd, stmax, pending, lightP, stime);
                 r = 0;
             } else {
-                r = neeStep<LEAN>(A, *J, p, s, h, ev, shadow, so, sd, stmax, smedium, pending, lightP, stime, false,
+                r = neeStep<FEAT>(A, *J, p, s, h, ev, shadow, so, sd, stmax, smedium, pending, lightP, stime, false,
                                   false, sameV(0.0f));
             }
             if (FAST && shadow && isBlack(pending)) shadow = false;
@@ -1433,9 +1440,9 @@ __global__ __launch_bounds__(256, GND_TAIL_WAVES) void kTail(WaveArgs A) {
         // the shadow ray, segment by segment through null-BSDF surfaces
         // (path.cpp:122-140, volPath.cpp:261-291)
         {
-            const bool vol = !LEAN && busy && J->integrator == GND_INTEGRATOR_VOLPATH;
-            const bool opaque =
-                !LEAN && busy && (J->integrator == GND_INTEGRATOR_DIRECT || J->integrator == GND_INTEGRATOR_AO);
+            const bool vol = FEAT != kFeatPath && busy && J->integrator == GND_INTEGRATOR_VOLPATH;
+            const bool opaque = FEAT == kFeatAll && busy &&
+                                (J->integrator == GND_INTEGRATOR_DIRECT || J->integrator == GND_INTEGRATOR_AO);
             bool open = shadow, visible = false;
             V3 Tr = sameV(1.0f), o = so;
             float tmax = stmax;
@@ -1492,7 +1499,7 @@ __global__ __launch_bounds__(256, GND_TAIL_WAVES) void kTail(WaveArgs A) {
         bool alive = false;
         if (busy) {
             if (r == 1) alive = true;
-            else if (r == 2) alive = scatterStep<LEAN>(S, *J, s, ev);
+            else if (r == 2) alive = scatterStep<FEAT>(S, *J, s, ev);
         }
         const bool ending = busy && !alive;
         if (ending) {
@@ -2627,7 +2634,7 @@ static std::vector<WavePlan> planWaves(const std::vector<gnd_hip_job>& jobs, siz
 
 // Kernels and launch shape of a bounce for one scene and job mix.
 struct BouncePlan {
-    WaveKernel trace = nullptr, shadow = nullptr, shade = nullptr;
+    WaveKernel trace = nullptr, shadow = nullptr, shade = nullptr, scatter = nullptr;
     bool split = false, fastShadow = false;
     int B = 128;
     int refill = 32, srefill = 40;
@@ -2673,9 +2680,14 @@ static BouncePlan makeBouncePlan(const gnd_hip_ctx* ctx, const gnd_hip_scene* sc
     // reference order when shadow rays cross a density grid
     K.split = anyVol && scene->hasGrid;
     // path jobs without AOVs on a scene lit by area emitters only take the
-    // lean shading kernel
-    const bool lean = !mix.vol && !mix.other && !mix.aov && scene->view.nLights == 0 && scene->view.envLight < 0;
-    K.shade = K.split ? kShade<true> : (lean ? kShade<false, true> : kShade<false, false>);
+    // lean shading kernels: path / volpath jobs without AOVs on a scene lit
+    // by area emitters only
+    const bool areaOnly = scene->view.nLights == 0 && scene->view.envLight < 0;
+    const int feat = (!mix.other && !mix.aov && areaOnly) ? (mix.vol ? kFeatVol : kFeatPath) : kFeatAll;
+    if (K.split) K.shade = feat == kFeatVol ? kShade<true, kFeatVol> : kShade<true, kFeatAll>;
+    else if (feat == kFeatPath) K.shade = kShade<false, kFeatPath>;
+    else K.shade = feat == kFeatVol ? kShade<false, kFeatVol> : kShade<false, kFeatAll>;
+    K.scatter = feat == kFeatVol ? kScatter<kFeatVol> : kScatter<kFeatAll>;
     // any-hit shadows are exact when no surface is a null BSDF and no
     // shadow segment can cross a medium (the path integrator ignores media)
     K.fastShadow = !scene->hasIndex && (!anyVol || scene->nMedia == 0);
@@ -2699,7 +2711,10 @@ static BouncePlan makeBouncePlan(const gnd_hip_ctx* ctx, const gnd_hip_scene* sc
     K.srefill = ctx->tuning.shadow_refill_lanes > 0 ? ctx->tuning.shadow_refill_lanes : (small ? 64 : 40);
     K.stackCap = stackCapOf(ctx->tuning);
     if (scene->hasMotion) K.tail = K.fastShadow ? kTail<kTailLds, true, true> : kTail<kTailLds, true, false>;
-    else if (lean) K.tail = K.fastShadow ? kTail<kTailLds, false, true, true> : kTail<kTailLds, false, false, true>;
+    else if (feat == kFeatPath)
+        K.tail = K.fastShadow ? kTail<kTailLds, false, true, kFeatPath> : kTail<kTailLds, false, false, kFeatPath>;
+    else if (feat == kFeatVol)
+        K.tail = K.fastShadow ? kTail<kTailLds, false, true, kFeatVol> : kTail<kTailLds, false, false, kFeatVol>;
     else K.tail = K.fastShadow ? kTail<kTailLds, false, true> : kTail<kTailLds, false, false>;
     // resident lanes of the tail kernel: GND_TAIL_WAVES waves on each of 4 SIMDs per CU
     const uint32_t tailLanes = (uint32_t)ctx->numCUs * 4u * GND_TAIL_WAVES * 64u;
@@ -2777,7 +2792,7 @@ static void launchBounce(hipStream_t stream, const WaveArgs& A, const BouncePlan
     if (K.split) {
         // the paths of this bounce with an event scatter after their shadow
         // rays consumed the transmittance samples
-        hipLaunchKernelGGL(kScatter, dim3(gs), dim3(kShadeBlock), 0, stream, A);
+        hipLaunchKernelGGL(K.scatter, dim3(gs), dim3(kShadeBlock), 0, stream, A);
         GND_CHECK(hipGetLastError());
     }
     hipLaunchKernelGGL(kRotate, dim3(1), dim3(64), 0, stream, A.counts, A.ctr);
@@ -2812,7 +2827,8 @@ static std::string chainKey(const WaveArgs& A0, const BouncePlan& K, uint32_t li
     auto put = [&](const void* p, size_t n) { key.append(static_cast<const char*>(p), n); };
     const unsigned shape[6] = {K.grid(live), K.shadeGrid(live), K.gridFor(live, 256), (unsigned)K.B,
                                K.tailPaths, K.tailGridCap};
-    const void* fns[4] = {(const void*)K.trace, (const void*)K.shadow, (const void*)K.tail, (const void*)K.shade};
+    const void* fns[5] = {(const void*)K.trace, (const void*)K.shadow, (const void*)K.tail, (const void*)K.shade,
+                          (const void*)K.scatter};
     const int flags[2] = {K.split ? 1 : 0, (int)K.shm};
     put(&A0, sizeof(A0));
     put(shape, sizeof(shape));
