@@ -369,6 +369,30 @@ def test_batch_equals_solo(device):
         assert np.array_equal(alone[0], together[i]), f"job {i} differs when batched"
 
 
+@pytest.mark.parametrize("scene,params,spp", [
+    ("scaled", "level=4;instances=12;width=96;height=54", 4),
+    ("volumetric", "grid=32;width=64;height=40", 4),
+])
+def test_lean_and_general_shading_kernels_equal(device, scene, params, spp):
+    """A path / volpath job without AOVs on a scene lit by area emitters only
+    runs the lean shading, scatter and tail kernels; the same job with AOVs
+    runs the general ones.  The radiance films are the same bits, with the
+    tail kernel on (the default) and off."""
+    hs = HostScene.generate(scene, params)
+    ds = DeviceScene(device, hs)
+    job = hs.default_job(spp=spp)
+    old = device.tuning()
+    try:
+        for tail in (old["tail_paths"], -1):
+            device.set_tuning(tail_paths=tail)
+            print(f"lean vs general {scene} {params} tail {tail}", flush=True)
+            lean, _ = device.render(ds, [job])
+            general, _ = device.render(ds, [job], want_aov=True)
+            assert np.array_equal(lean[0], general[0][0]), f"{scene}: lean and general kernels differ (tail {tail})"
+    finally:
+        device.set_tuning(tail_paths=old["tail_paths"])
+
+
 def test_batch_wave_split_equals_one_wave(device):
     """A queue too small for the whole batch splits it into several waves
     (whole passes each); the films are unchanged."""
