@@ -419,7 +419,15 @@ __global__ void kCamera(WaveArgs A) {
 // next to a refill (fetching rays from HBM), so refills should gather more
 // lanes; a large BVH makes idle lanes the bigger loss.  The traversal
 // kernels take it from the tuning; the medium-walk kernels use kWalkRefill.
-constexpr int kWalkRefill = 24;
+// Volumetric 720p frame: 24 idle lanes 72.4 ms, 40 lanes 70.0 ms, 56 lanes
+// 70.1 ms (kMedium and kShadow together).
+constexpr int kWalkRefill = 40;
+// kShadow runs a traversal (all lanes waiting for one take part; a traversal
+// lasts as long as its longest ray) only when more than kShadowTraceRatio
+// times as many lanes wait for one as wait for a walk step (one step each):
+// volumetric 720p transmittance shadows 14.6 ms per frame at a ratio of 1,
+// 13.3 at 2, 12.6 at 4 and 8, 12.9 at 16, 16.8 at 1/2.
+constexpr int kShadowTraceRatio = 4;
 // Queue counters: [0] in-queue size, [1] out-queue, [2] shadow queue, then
 // the ray-feed part counters of the two persistent traversals.
 #ifndef GND_FEED_PARTS
@@ -1268,9 +1276,8 @@ __global__ __launch_bounds__(256, GND_SHADOWTR_WAVES) void kShadow(WaveArgs A) {
             if (drained) break;
             continue;
         }
-        // one traversal or one walk step per iteration, for whichever phase
-        // more lanes wait in
-        const bool tracing = nTrace > nWalk;
+        // one traversal or one walk step per iteration (kShadowTraceRatio)
+        const bool tracing = nTrace > nWalk * kShadowTraceRatio;
         bool segDone = false;
         if (tracing && phase == kTrace) {
             HitRec h;
@@ -1372,7 +1379,10 @@ __global__ __launch_bounds__(256, GND_SHADOWTR_WAVES) void kShadow(WaveArgs A) {
 #define GND_TAIL_WAVES 4
 #endif
 // a lane's work between refills is a whole bounce, so idle lanes refill early
-constexpr int kTailRefill = 8;
+#ifndef GND_TAIL_REFILL
+#define GND_TAIL_REFILL 8
+#endif
+constexpr int kTailRefill = GND_TAIL_REFILL;
 // traversal stack entries in LDS per lane of the tail kernel
 constexpr int kTailLds = 16;
 // FEAT: the jobs and scene the launch serves (see kFeatAll).
