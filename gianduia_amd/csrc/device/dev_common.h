@@ -192,7 +192,7 @@ struct DScene {
     const DMaterial* mats;
     const DMedium* media;
     const int32_t* emitters;  // emitter list: primitive index (area) or -(light + 1)
-    int32_t nTlas, nEmitters, nPrims, hasIndex, hasGrid, hasMotion;
+    int32_t nTlas, nEmitters, nPrims, hasIndex, hasGrid, hasMotion, nMedia;
     const DPrimMotion* motions;
     const DLight* lights;
     // environment map (envLight >= 0): RGB image, row-major, and its

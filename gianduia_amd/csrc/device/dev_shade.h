@@ -1038,6 +1038,40 @@ struct GridTrack {
     GDEV_INLINE V3 transmittance() const { return hit ? sameV(0.0f) : tr; }
 };
 
+// Whether the line o + t d, t in [0, tmax], may pass through density grid m:
+// the line in the grid's index space against the grid's box grown by a
+// margin, over a slightly longer t range (conservative: a ray that ratio
+// tracking would walk in the grid always passes, including the segments a
+// shadow ray continues along from null-surface crossings near the line).
+// Comparisons with NaN keep the answer "may".
+GDEV_INLINE bool lineMayCrossGrid(const DMedium& m, V3 o, V3 d, float tmax) {
+    const V3 lo = xfPoint(m.toLocal, o);
+    const V3 lt = xfPoint(m.toLocal, o + d);
+    const float dl[3] = {lt.x - lo.x, lt.y - lo.y, lt.z - lo.z};
+    const float ol[3] = {lo.x, lo.y, lo.z};
+    const float bmax[3] = {m.gridMaxX, m.gridMaxY, m.gridMaxZ};
+    const float mx = 1e-3f * stdMax(stdMax(m.gridMaxX, m.gridMaxY), m.gridMaxZ) + 1e-3f;
+    float t0 = -1e-3f, t1 = __builtin_isinf(tmax) ? finf() : tmax * 1.001f + 1e-3f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (__builtin_fabsf(dl[a]) < 1e-30f) {
+            if (ol[a] < -mx || ol[a] > bmax[a] + mx) return false;
+            continue;
+        }
+        const float inv = 1.0f / dl[a];
+        float tn = (-mx - ol[a]) * inv, tf = (bmax[a] + mx - ol[a]) * inv;
+        if (tn > tf) {
+            const float t = tn;
+            tn = tf;
+            tf = t;
+        }
+        if (tn > t0) t0 = tn;
+        if (tf < t1) t1 = tf;
+        if (t0 > t1) return false;
+    }
+    return true;
+}
+
 // Medium::Tr (homogeneous.cpp:22-30; heterogeneous.cpp:106-229 ratio tracking)
 GDEV_INLINE V3 mediumTr(const DMedium& m, V3 o, V3 d, float tmin, float tmax, Rng& rng) {
     if (m.type == GND_MEDIUM_HOMOGENEOUS) {

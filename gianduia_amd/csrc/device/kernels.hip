@@ -960,6 +960,19 @@ GDEV_INLINE bool gridWalked(const WaveArgs& A, const DJob& J, uint32_t flags) {
     return A.split && medium >= 0 && A.S.media[medium].type == GND_MEDIUM_GRID;
 }
 
+// Whether this bounce's shadow ray of a volpath job may walk a density grid
+// (ratio tracking in kShadow draws from the path's sampler between the NEE
+// sample and the scatter).  When it cannot, nothing draws in between and the
+// scatter runs in kShade, with the same draws in the same order.
+GDEV_INLINE bool shadowMayWalk(const WaveArgs& A, const DJob& J, bool shadow, V3 so, V3 sd, float stmax) {
+    if (!shadow || J.integrator != GND_INTEGRATOR_VOLPATH) return false;
+    for (int m = 0; m < A.S.nMedia; ++m) {
+        const DMedium& md = A.S.media[m];
+        if (md.type == GND_MEDIUM_GRID && lineMayCrossGrid(md, so, sd, stmax)) return true;
+    }
+    return false;
+}
+
 // Delta tracking (Medium::sample, heterogeneous.cpp:251-376) for the paths of
 // the in-queue whose ray starts in a density grid (the list gIn, built by the
 // shading kernels of the bounce before), ahead of kShade: the first sampler
@@ -1087,9 +1100,13 @@ __global__ __launch_bounds__(kShadeBlock, FEAT == kFeatPath  ? GND_SHADE_LEAN_WA
             } else if (shadow && isBlack(pending)) {
                 shadow = false;
             }
+            // split mode: the scatter waits for kScatter only when the shadow
+            // ray may draw from the sampler first
+            bool deferred = false;
             if (r == 1) alive = true;
             else if (r == 2) {
-                if (SPLIT) {
+                if (SPLIT && shadowMayWalk(A, J, shadow, so, sd, stmax)) {
+                    deferred = true;
                     alive = false;
                     A.P.evKind[p] = ev.kind;
                     A.P.evPrim[p] = ev.prim;
@@ -1101,13 +1118,11 @@ __global__ __launch_bounds__(kShadeBlock, FEAT == kFeatPath  ? GND_SHADE_LEAN_WA
                 } else {
                     alive = scatterStep<FEAT>(A.S, J, s, ev);
                 }
-            } else if (SPLIT) {
-                A.P.evKind[p] = EV_NONE;
             }
-            if (SPLIT && r == 1) A.P.evKind[p] = EV_NONE;
+            if (SPLIT && !deferred) A.P.evKind[p] = EV_NONE;
             toGrid = SPLIT && alive && gridWalked(A, J, s.flags);
             storePath(A.P, p, s);
-            ending = !alive && !(SPLIT && r == 2);
+            ending = !alive && !deferred;
         }
         retirePaths(A.chunkLive, ending, ending && A.chunkLive ? A.P.chunk[p] : -1);
         uint32_t qs, ss, gs;
@@ -2504,6 +2519,7 @@ static int sceneUpload(gnd_hip_ctx* ctx, const gnd_hip_scene_arrays* a, gnd_hip_
         v.nPrims = (int32_t)d.nprims;
         v.hasIndex = s->hasIndex;
         v.hasGrid = s->hasGrid;
+        v.nMedia = (int32_t)d.nmedia;
         s->nPrims = d.nprims;
         s->nMedia = (int)d.nmedia;
     });
