@@ -73,14 +73,16 @@ WORKLOADS = {
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "kTracePaths_scaled1080p_hbm_traffic.md")
 
 
-def measured_traffic():
+def measured_traffic(rays_per_launch):
     """HBM bytes per closest-hit launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE passes (scaled by the copy-kernel calibration), or None."""
+    WRITE_SIZE passes (scaled by the copy-kernel calibration): the profiled
+    bytes per ray times this run's rays per launch (the autotuner may cut the
+    frame into other launches than the profiled run), or None."""
     try:
         with open(TRAFFIC_PROFILE) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
-                    return float(line.split(":", 1)[1])
+                if line.startswith("traffic_bytes_per_ray:"):
+                    return float(line.split(":", 1)[1]) * rays_per_launch
     except OSError:
         pass
     return None
@@ -432,10 +434,11 @@ def main():
         trace_ms = sum(s["ms_trace"] for s in trace_steps) / len(trace_steps)
         launches = trace_steps[0]["trace_launches"]
         achieved = byts / (trace_ms / 1000.0) / 1e9
-        traffic = measured_traffic() if not a.params and a.scene == "scaled" else None
+        traffic = measured_traffic(trays / launches) if not a.params and a.scene == "scaled" else None
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT) if traffic else None,
+                "traffic_source": (os.path.relpath(TRAFFIC_PROFILE, ROOT) + ": measured bytes per ray x this run's "
+                                   "rays per launch") if traffic else None,
                 "kernel": "kTracePaths (closest-hit BVH4 traversal)",
                 "bytes_per_launch": byts / launches, "avg_launch_ms": trace_ms / launches,
                 "max_nodes_one_ray": cst["max_ray_nodes"], "nonfinite_rays": cst["bad_rays"],
