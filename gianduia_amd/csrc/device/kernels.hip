@@ -1729,15 +1729,16 @@ struct gnd_hip_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     std::mutex mu;
-    // Up to 120M paths in flight per wave (300 B each with one queue set,
-    // 36 GB of the 288 GB; a render allocates only what its jobs' samples
+    // Up to 256M paths in flight per wave (300 B each with one queue set,
+    // 77 GB of the 288 GB; a render allocates only what its jobs' samples
     // need): a 1080p x 16 spp frame is one wave, and a batch of many jobs
     // takes few waves, so few bounces run below the GPU's width (64-job
-    // batch over the 1080p scene: 40M slots 347, 120M slots 371 Msamples/s).
+    // batch over the 1080p scene, 527M samples: 40M slots 347, 120M 421,
+    // 264M 434, 540M 437 Msamples/s).
     // 32 LDS stack entries per lane: as fast as 16 on the 1080p scene, and
     // the deeper stack rarely spills to scratch (16 traversal waves per CU,
     // 64 lanes x 32 entries x 4 B each, take 128 KiB of its 160 KiB of LDS)
-    gnd_hip_tuning tuning{128, 32, 120000000, 0, 0, 0, 0, 0};
+    gnd_hip_tuning tuning{128, 32, 256000000, 0, 0, 0, 0, 0};
     bool counters = false;
     int numCUs = 0;
     size_t hbm = 0;

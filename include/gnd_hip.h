@@ -148,7 +148,7 @@ typedef struct gnd_hip_stats {
 typedef struct gnd_hip_tuning {
     int32_t block_size;       /* threads per block of the traversal kernels (64..256) */
     int32_t stack_lds;        /* per-lane traversal stack entries held in LDS (8/16/24/32) */
-    int64_t queue_capacity;   /* paths in flight per wave (default 120M; a render
+    int64_t queue_capacity;   /* paths in flight per wave (default 256M; a render
                                  allocates min(this, its samples) path slots)         */
     int32_t use_graph;        /* capture the bounce loop in a hipGraph                */
     int32_t refill_lanes;     /* idle lanes (1..64) before a closest-hit traversal wave
