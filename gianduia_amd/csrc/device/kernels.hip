@@ -445,26 +445,91 @@ constexpr uint32_t kFeedTrace = 8, kFeedShadow = kFeedTrace + kParts, kFeedMediu
 // 16: 76 ms of traversal per frame): with a single counter every wave works
 // on the same region of the queue, so the rays in flight are coherent and
 // share BVH nodes in L2.
+// With one counter (the default) a wave of the closest-hit traversal reserves
+// kFeedBlock rays per atomic and hands them out over its next refills (block
+// 0: one atomic per refill, for the k rays it hands out).  The counter is
+// shared by every wave of the launch and its atomics serialise: with an
+// atomic per refill, lowering the refill threshold from 40 to 16 lanes raised
+// lane use from 0.48 to 0.55 yet took the 1080p traversal from 24.3 to 39.6 ms
+// per frame.  Blocks of 256 rays: 1080p traversal 24.5 -> 23.2 ms, volumetric
+// (a small BVH, so refills dominate) 13.4 -> 9.0 ms; 128 and 512 are slower.
+// The any-hit, walk and tail kernels keep an atomic per refill: their waves'
+// work is uneven, and reserved rays left in a slow wave delayed the launch's
+// end (volumetric shadows 13.1 -> 14.5 ms with blocks of 64).
+#ifndef GND_FEED_BLOCK
+#define GND_FEED_BLOCK 256
+#endif
+constexpr uint32_t kFeedBlock = GND_FEED_BLOCK;
+#ifndef GND_SHADOW_FEED_BLOCK
+#define GND_SHADOW_FEED_BLOCK 0
+#endif
+#ifndef GND_WALK_FEED_BLOCK
+#define GND_WALK_FEED_BLOCK 0
+#endif
+constexpr uint32_t kShadowFeedBlock = GND_SHADOW_FEED_BLOCK, kWalkFeedBlock = GND_WALK_FEED_BLOCK;
 struct RayFeed {
     uint32_t* ctr;
     uint32_t n;
     int part, tried, refill;  // wave-uniform
-    GDEV_INLINE void init(uint32_t* counters, uint32_t count, int refillLanes) {
+    // one counter: the reserved rays not handed out yet, [resBase, resBase + resLeft)
+    uint32_t resBase, block;
+    int resLeft;
+    bool exhausted;
+    GDEV_INLINE void init(uint32_t* counters, uint32_t count, int refillLanes, uint32_t blockRays = kFeedBlock) {
         ctr = counters;
         n = count;
         refill = refillLanes;
+        block = blockRays;
         part = (int)(blockIdx.x % (unsigned)kParts);
         tried = 0;
+        resBase = 0;
+        resLeft = 0;
+        exhausted = false;
     }
     GDEV_INLINE uint32_t edge(int p) const { return (uint32_t)(((uint64_t)n * (uint64_t)p) / kParts); }
     // Idle lanes take the next rays (once `refill` lanes, or the whole wave,
     // are idle); returns this lane's queue index or -1.  Sets `drained`
-    // (wave-uniformly) once every part is used up.
+    // (wave-uniformly) once every ray is handed out.
     GDEV_INLINE int64_t take(bool idle, uint64_t waveMask, bool& drained) {
         const uint64_t m = __ballot(idle);
         const int k = __popcll(m);
         if (k == 0 || (k < refill && m != waveMask)) return -1;
         const int leader = __ffsll((unsigned long long)m) - 1;
+        if (block > 0 && kParts == 1) {
+            // a new block when the reservation cannot serve every idle lane;
+            // ranks below resLeft take the reservation, the next ones the block
+            uint32_t bBase = 0;
+            int bLeft = 0;
+            if (resLeft < k && !exhausted) {
+                uint32_t b = 0;
+                if ((int)__lane_id() == leader) b = atomicAdd(ctr, block);
+                bBase = __shfl(b, leader);
+                bLeft = bBase < n ? (int)min(block, n - bBase) : 0;
+                if (bBase >= n || n - bBase <= block) exhausted = true;
+            }
+            if (resLeft + bLeft == 0) {
+                drained = true;
+                return -1;
+            }
+            const int rank = __popcll(m & ((1ull << __lane_id()) - 1ull));
+            int64_t r = -1;
+            if (idle) {
+                if (rank < resLeft) r = (int64_t)resBase + rank;
+                else if (rank < resLeft + bLeft) r = (int64_t)bBase + (rank - resLeft);
+            }
+            const int given = min(k, resLeft + bLeft);
+            if (bLeft > 0) {
+                // the reservation is used up (a block is taken only when it
+                // could not serve all k lanes)
+                const int usedB = given - resLeft;
+                resBase = bBase + (uint32_t)usedB;
+                resLeft = bLeft - usedB;
+            } else {
+                resBase += (uint32_t)given;
+                resLeft -= given;
+            }
+            return r;
+        }
         while (tried < kParts) {
             const uint32_t lo = edge(part), len = edge(part + 1) - lo;
             uint32_t base = 0;
@@ -986,7 +1051,7 @@ __global__ __launch_bounds__(256) void kMedium(WaveArgs A) {
     const uint64_t waveMask = __ballot(1);
     const DScene& S = A.S;
     RayFeed feed;
-    feed.init(A.counts + kFeedMedium, n, kWalkRefill);
+    feed.init(A.counts + kFeedMedium, n, kWalkRefill, kWalkFeedBlock);
     bool drained = false;
     int64_t cur = -1;
     int p = -1, medium = 0;
@@ -1203,7 +1268,7 @@ __global__ __launch_bounds__(256, GND_SHADOW_WAVES) void kShadowAny(WaveArgs A) 
     const ShadowSoA& Sh = A.Sh;
     const uint64_t waveMask = __ballot(1);
     RayFeed feed;
-    feed.init(A.counts + kFeedShadow, n, A.srefill);
+    feed.init(A.counts + kFeedShadow, n, A.srefill, kShadowFeedBlock);
     int64_t cur = -1;
     traceStream<true, COUNT, LDSD, MOTION>(
         S, st, cnt,
@@ -1247,7 +1312,7 @@ __global__ __launch_bounds__(256, GND_SHADOWTR_WAVES) void kShadow(WaveArgs A) {
     const ShadowSoA& Sh = A.Sh;
     const uint64_t waveMask = __ballot(1);
     RayFeed feed;
-    feed.init(A.counts + kFeedShadow, n, kWalkRefill);
+    feed.init(A.counts + kFeedShadow, n, kWalkRefill, kWalkFeedBlock);
     bool drained = false;
     constexpr int kIdle = 0, kTrace = 1, kWalk = 2;
     constexpr int kVisible = 0, kOccluded = 1, kOnward = 2;
@@ -1398,6 +1463,12 @@ __global__ __launch_bounds__(256, GND_SHADOWTR_WAVES) void kShadow(WaveArgs A) {
 #define GND_TAIL_REFILL 8
 #endif
 constexpr int kTailRefill = GND_TAIL_REFILL;
+// paths reserved per atomic by a tail-kernel wave (0: one atomic per refill;
+// 1080p frame 57.2 ms, blocks of 32 57.8, 64 57.9, 128 58.4)
+#ifndef GND_TAIL_FEED_BLOCK
+#define GND_TAIL_FEED_BLOCK 0
+#endif
+constexpr uint32_t kTailFeedBlock = GND_TAIL_FEED_BLOCK;
 // traversal stack entries in LDS per lane of the tail kernel
 constexpr int kTailLds = 16;
 // FEAT: the jobs and scene the launch serves (see kFeatAll).
@@ -1411,7 +1482,7 @@ __global__ __launch_bounds__(256, GND_TAIL_WAVES) void kTail(WaveArgs A) {
     TravStack<LDSD> st = makeStack<LDSD>(sStack, spill, A.stackCap);
     const DScene& S = A.S;
     RayFeed feed;
-    feed.init(A.counts + kFeedTail, n, kTailRefill);
+    feed.init(A.counts + kFeedTail, n, kTailRefill, kTailFeedBlock);
     bool drained = false, busy = false;
     int p = -1;
     PathLocal s;
