@@ -2801,11 +2801,12 @@ static BouncePlan makeBouncePlan(const gnd_hip_ctx* ctx, const gnd_hip_scene* sc
 #endif
     // up to 64 waves per CU in flight: the traversal is latency bound
     K.gridCap = (unsigned)std::max(8, ctx->numCUs * GND_GRID_WAVES_PER_CU * 64 / K.B);
-    // refill thresholds: measured best on a 36-triangle Cornell box (56 / 64)
-    // and a 1M-triangle scene (32 / 40); a BVH within one XCD's 4 MiB L2 counts
-    // as small
+    // refill thresholds: measured best on a 36-triangle Cornell box (40 / 64;
+    // closest hit 823 Msamples/s at 40, 811 at 16, 799 at 56) and the 1.95M-
+    // triangle scene (16 / 40; 599 at 16, 592 at 24, 583 at 32), with the
+    // block-reserving ray feed; a BVH within one XCD's 4 MiB L2 counts as small
     const bool small = scene->accelBytes <= (size_t(4) << 20);
-    K.refill = ctx->tuning.refill_lanes > 0 ? ctx->tuning.refill_lanes : (small ? 56 : 32);
+    K.refill = ctx->tuning.refill_lanes > 0 ? ctx->tuning.refill_lanes : (small ? 40 : 16);
     K.srefill = ctx->tuning.shadow_refill_lanes > 0 ? ctx->tuning.shadow_refill_lanes : (small ? 64 : 40);
     K.stackCap = stackCapOf(ctx->tuning);
     if (scene->hasMotion) K.tail = K.fastShadow ? kTail<kTailLds, true, true> : kTail<kTailLds, true, false>;
@@ -3300,7 +3301,7 @@ int gnd_hip_autotune(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* 
             tryTuning(t);
         }
     const gnd_hip_tuning shape = best;
-    for (int r : {24, 32, 40, 48, 56, 64}) {
+    for (int r : {12, 16, 24, 32, 40, 48, 56}) {
         gnd_hip_tuning t = shape;
         t.refill_lanes = r;
         tryTuning(t);

@@ -152,8 +152,8 @@ typedef struct gnd_hip_tuning {
                                  allocates min(this, its samples) path slots)         */
     int32_t use_graph;        /* capture the bounce loop in a hipGraph                */
     int32_t refill_lanes;     /* idle lanes (1..64) before a closest-hit traversal wave
-                                 takes new rays; 0 = per scene (56 when the BVH fits in
-                                 4 MiB, else 32)                                        */
+                                 takes new rays; 0 = per scene (40 when the BVH fits in
+                                 4 MiB, else 16)                                        */
     int32_t shadow_refill_lanes;  /* the same for any-hit shadow traversal; 0 = per
                                  scene (64, else 40)                                    */
     int32_t stack_limit;      /* traversal stack entries per lane: 0 = all 128 (the
