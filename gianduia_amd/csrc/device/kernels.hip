@@ -453,15 +453,17 @@ constexpr uint32_t kFeedTrace = 8, kFeedShadow = kFeedTrace + kParts, kFeedMediu
 // lane use from 0.48 to 0.55 yet took the 1080p traversal from 24.3 to 39.6 ms
 // per frame.  Blocks of 256 rays: 1080p traversal 24.5 -> 23.2 ms, volumetric
 // (a small BVH, so refills dominate) 13.4 -> 9.0 ms; 128 and 512 are slower.
-// The any-hit, walk and tail kernels keep an atomic per refill: their waves'
-// work is uneven, and reserved rays left in a slow wave delayed the launch's
-// end (volumetric shadows 13.1 -> 14.5 ms with blocks of 64).
+// The any-hit traversal takes blocks of 256 too (1080p shadows 12.45 ms per
+// frame at a 40-lane threshold without them, 12.18 at 16 lanes with them).
+// The walk and tail kernels keep an atomic per refill: their waves' work is
+// uneven, and reserved rays left in a slow wave delayed the launch's end
+// (volumetric shadows 13.1 -> 14.5 ms with blocks of 64).
 #ifndef GND_FEED_BLOCK
 #define GND_FEED_BLOCK 256
 #endif
 constexpr uint32_t kFeedBlock = GND_FEED_BLOCK;
 #ifndef GND_SHADOW_FEED_BLOCK
-#define GND_SHADOW_FEED_BLOCK 0
+#define GND_SHADOW_FEED_BLOCK 256
 #endif
 #ifndef GND_WALK_FEED_BLOCK
 #define GND_WALK_FEED_BLOCK 0
@@ -2803,11 +2805,12 @@ static BouncePlan makeBouncePlan(const gnd_hip_ctx* ctx, const gnd_hip_scene* sc
     K.gridCap = (unsigned)std::max(8, ctx->numCUs * GND_GRID_WAVES_PER_CU * 64 / K.B);
     // refill thresholds: measured best on a 36-triangle Cornell box (40 / 64;
     // closest hit 823 Msamples/s at 40, 811 at 16, 799 at 56) and the 1.95M-
-    // triangle scene (16 / 40; 599 at 16, 592 at 24, 583 at 32), with the
-    // block-reserving ray feed; a BVH within one XCD's 4 MiB L2 counts as small
+    // triangle scene (16 / 16; 599 at 16, 592 at 24, 583 at 32; any hit 605 at
+    // 16, 602 at 24, 600 at 40), with the block-reserving ray feed; a BVH
+    // within one XCD's 4 MiB L2 counts as small
     const bool small = scene->accelBytes <= (size_t(4) << 20);
     K.refill = ctx->tuning.refill_lanes > 0 ? ctx->tuning.refill_lanes : (small ? 40 : 16);
-    K.srefill = ctx->tuning.shadow_refill_lanes > 0 ? ctx->tuning.shadow_refill_lanes : (small ? 64 : 40);
+    K.srefill = ctx->tuning.shadow_refill_lanes > 0 ? ctx->tuning.shadow_refill_lanes : (small ? 64 : 16);
     K.stackCap = stackCapOf(ctx->tuning);
     if (scene->hasMotion) K.tail = K.fastShadow ? kTail<kTailLds, true, true> : kTail<kTailLds, true, false>;
     else if (feat == kFeatPath)
@@ -3307,7 +3310,7 @@ int gnd_hip_autotune(gnd_hip_ctx* ctx, gnd_hip_scene* scene, const gnd_hip_job* 
         tryTuning(t);
     }
     const gnd_hip_tuning closest = best;
-    for (int r : {24, 32, 40, 48, 56, 64}) {
+    for (int r : {16, 24, 32, 40, 48, 56, 64}) {
         gnd_hip_tuning t = closest;
         t.shadow_refill_lanes = r;
         tryTuning(t);

@@ -155,7 +155,7 @@ typedef struct gnd_hip_tuning {
                                  takes new rays; 0 = per scene (40 when the BVH fits in
                                  4 MiB, else 16)                                        */
     int32_t shadow_refill_lanes;  /* the same for any-hit shadow traversal; 0 = per
-                                 scene (64, else 40)                                    */
+                                 scene (64, else 16)                                    */
     int32_t stack_limit;      /* traversal stack entries per lane: 0 = all 128 (the
                                  reference's int stack[64] per level, TLAS + BLAS);
                                  1..128 caps it, so the GND_E_STACK path can be
