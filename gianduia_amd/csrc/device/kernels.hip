@@ -205,17 +205,6 @@ GDEV_INLINE void blockAppend3(uint32_t* c0, bool p0, uint32_t& pos0, uint32_t* c
     __syncthreads();  // sN / sBase are reused by the next call
 }
 
-// wave-aggregated append: ballot, popcount, one atomic per wave
-GDEV_INLINE uint32_t waveAppend(uint32_t* counter, bool pred) {
-    const uint64_t mask = __ballot(pred);
-    const uint32_t lane = __lane_id();
-    const uint32_t prefix = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-    uint32_t base = 0;
-    if (mask != 0) {
-        const int leader = __ffsll((unsigned long long)mask) - 1;
-        if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-        base = __shfl(base, leader);
-    }
     return base + prefix;
 }
 
