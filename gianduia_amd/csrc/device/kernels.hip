@@ -205,9 +205,6 @@ GDEV_INLINE void blockAppend3(uint32_t* c0, bool p0, uint32_t& pos0, uint32_t* c
     __syncthreads();  // sN / sBase are reused by the next call
 }
 
-    return base + prefix;
-}
-
 // XCD-aware block order: the dispatcher deals workgroups round-robin over the
 // 8 XCDs; remap so each XCD walks a contiguous range (coherent rays share L2).
 GDEV_INLINE uint32_t xcdBlock() {
