@@ -2808,8 +2808,9 @@ static BouncePlan makeBouncePlan(const gnd_hip_ctx* ctx, const gnd_hip_scene* sc
     const uint32_t tailLanes = (uint32_t)ctx->numCUs * 4u * GND_TAIL_WAVES * 64u;
     K.tailGridCap = std::max(8u, tailLanes / (unsigned)K.B);
     const int32_t tp = ctx->tuning.tail_paths;
-    // default: eight paths per resident lane (1M on 256 CUs), measured near
-    // the best on the 1080p scaled, volumetric and Cornell-box frames
+    // default: eight paths per resident lane (2M on 256 CUs at 4 waves per
+    // SIMD), measured near the best on the 1080p scaled, volumetric and
+    // Cornell-box frames
     K.tailPaths = tp < 0 ? 0u : (tp == 0 ? 8u * tailLanes : (uint32_t)tp);
     return K;
 }
