@@ -446,6 +446,9 @@ def main():
                 # instance steps; an instance costs a start and a finish step)
                 "simd_utilization": round(cst["lane_steps"] / max(1, cst["lane_iterations"]), 4),
                 "rays_per_launch": trays / launches,
+                "bytes_note": "achieved counts every reference-layout node, pack and instance read of the "
+                              "reference-order traversal; most are served by L1, L2 and the Infinity Cache, so "
+                              "frac can pass 1; traffic is the measured HBM bytes per launch",
                 "visits_per_ray": {"nodes": cst["node_visits"] / trays,
                                    "packs": cst["pack_tests"] / trays,
                                    "insts": cst["inst_tests"] / trays},
