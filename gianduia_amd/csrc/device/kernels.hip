@@ -1625,7 +1625,12 @@ __global__ void kSplat(WaveArgs A, const DSplatTask* tasks, int nTasks, uint32_t
     const DCamera& c = J.cam;
     const float rx = c.filter == GND_FILTER_BOX ? 0.5f : c.radiusX;
     const float ry = c.filter == GND_FILTER_BOX ? 0.5f : c.radiusY;
-    const int reach = (int)ceilf(stdMax(rx, ry)) + 1;
+    // A sample of pixel column qx has x + jitter in [qx, qx + 1] (the float sum
+    // may round up to qx + 1), so its support [px - r - 0.5, px + r - 0.5]
+    // holds no pixel centre further than ceil(r) columns away: a neighbour
+    // beyond that fails the x0..x1 / y0..y1 test below, so skipping it leaves
+    // the sums and their order unchanged.
+    const int reach = max(0, (int)ceilf(stdMax(rx, ry)));
     float4 acc = reinterpret_cast<float4*>(J.film)[pix];
     float3 accSq = J.filmSq ? reinterpret_cast<float3*>(J.filmSq)[pix] : float3{0, 0, 0};
     const bool aov = J.filmAov && A.P.aov;

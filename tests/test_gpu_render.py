@@ -516,8 +516,9 @@ def test_autotune_picks_valid_tuning(device):
     try:
         t = device.autotune(ds, hs.default_job(spp=4))
         assert t["block_size"] in (64, 128, 256) and t["stack_lds"] in (8, 16, 24, 32)
-        assert t["refill_lanes"] in (0, 24, 32, 40, 48, 56, 64)
-        assert t["shadow_refill_lanes"] in (0, 24, 32, 40, 48, 56, 64)
+        # 0 keeps the built-in default; otherwise one of the searched thresholds
+        assert t["refill_lanes"] in (0, 12, 16, 24, 32, 40, 48, 56)
+        assert t["shadow_refill_lanes"] in (0, 16, 24, 32, 40, 48, 56, 64)
         a, _ = device.render(ds, [hs.default_job(spp=2)])
     finally:
         device.set_tuning(**old)
